@@ -1,0 +1,147 @@
+/* hpdmk.h -- C ABI of the MI355X DMK Monte Carlo library (libhpdmk_amd.so).
+ *
+ * Drop-in for the reference's hpdmk surface (ArrogantGao/PDMK4MC):
+ *   include/hpdmk.h:6-25      HPDMKParams, hpdmk_init, hpdmk_tree, hpdmk_tree_create
+ *   include/tree.hpp:98-132   HPDMKPtTree<Real> members used on the Monte Carlo
+ *                             path (form_outgoing_pw, form_incoming_pw,
+ *                             eval_energy, eval_shift_energy, update_shift)
+ * The reference's tree is a C++ class; here it is an opaque handle, every call
+ * returns an int status (0 = HPDMK_OK) and never throws, and the message of the
+ * last failure on the calling thread is hpdmk_last_error().
+ *
+ * Coordinates are N x 3 interleaved doubles in [0, L); charges are N doubles;
+ * particle indices are the caller's original indices (the reference maps them
+ * through indices_invmap, src/energy_shift.cpp:21).  The caller keeps
+ * ownership of all input arrays, which are copied to HBM at create.
+ *
+ * One handle = one thread at a time.  Calls are synchronous with respect to
+ * the host unless stated otherwise.
+ */
+#ifndef HPDMK_H
+#define HPDMK_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* include/hpdmk.h:6-9 */
+typedef enum { DIRECT = 1, PROXY = 2 } hpdmk_init;
+
+/* include/hpdmk.h:11-17.  C has no default member initialisers: start from
+ * hpdmk_default_params(), which fills the reference's defaults
+ * (n_per_leaf 200, digits 3, prolate_order 16, init PROXY). */
+typedef struct HPDMKParams {
+    int n_per_leaf;
+    int digits;
+    double L;
+    double prolate_order;
+    hpdmk_init init;
+} HPDMKParams;
+
+/* include/hpdmk.h:19 */
+typedef void* hpdmk_tree;
+
+/* arithmetic of the device state (no reference counterpart: the reference is
+ * templated on float/double, src/tree.cpp:599-600) */
+typedef enum {
+    HPDMK_FP64 = 0,  /* all-FP64 expansions and sums */
+    HPDMK_MIXED = 1, /* FP32 bulk, FP64 phase arguments and accumulation */
+} hpdmk_precision;
+
+/* status codes */
+#define HPDMK_OK 0
+#define HPDMK_EINVAL (-1)    /* bad argument (digits, coordinates, sizes, handle) */
+#define HPDMK_ERUNTIME (-2)  /* method error the reference throws (tree too shallow, ...) */
+#define HPDMK_EDEVICE (-3)   /* HIP runtime error */
+#define HPDMK_ECAPACITY (-4) /* an accepted move would overflow a leaf's slot capacity */
+#define HPDMK_ENOTIMPL (-5)  /* PROXY above 3 digits (src/upward_pass.cpp:9-14) */
+
+HPDMKParams hpdmk_default_params(void);
+const char* hpdmk_last_error(void);
+
+/* Builds the octree and kernel tables and uploads everything to HBM.
+ * Replaces include/hpdmk.h:25 (which returns void and discards the tree) and
+ * the HPDMKPtTree constructor (src/tree.cpp:268-444).  The vestigial MPI_Comm
+ * argument is dropped (the reference runs single-rank, src/tree.cpp:266).
+ * Precision is HPDMK_FP64. */
+int hpdmk_tree_create(HPDMKParams params, int n_src, const double* r_src, const double* charge,
+                      hpdmk_tree* tree);
+/* Same with an explicit precision (hpdmk_precision). */
+int hpdmk_tree_create_ex(HPDMKParams params, int precision, int n_src, const double* r_src,
+                         const double* charge, hpdmk_tree* tree);
+int hpdmk_tree_destroy(hpdmk_tree tree);
+
+/* info[10]: n_nodes, n_levels, n_window, n_diff, window elements, level
+ * elements, boxes carrying expansions, max near-list length, leaves, slots */
+int hpdmk_tree_info(hpdmk_tree tree, int* info);
+
+/* src/planewave.cpp:86-97 (root expansion exact, not NUFFT) */
+int hpdmk_form_outgoing_pw(hpdmk_tree tree);
+/* src/planewave.cpp:99-155 */
+int hpdmk_form_incoming_pw(hpdmk_tree tree);
+/* include/tree.hpp:105 eval_energy; parts (optional, 3 doubles): window,
+ * difference, residual (src/energy.cpp:21-173) */
+int hpdmk_eval_energy(hpdmk_tree tree, double* energy, double* parts);
+
+/* src/energy_shift.cpp:18-70: energy change of moving particle i by (dx,dy,dz)
+ * (target wrapped into the box).  parts (optional, 5 doubles): window,
+ * difference, residual at target, residual at origin, self correction. */
+int hpdmk_eval_shift_energy(hpdmk_tree tree, long i, double dx, double dy, double dz, double* dE,
+                            double* parts);
+/* src/tree.cpp:447-597: applies the move.  Unlike the reference it does not
+ * depend on a preceding eval_shift_energy call with the same arguments. */
+int hpdmk_update_shift(hpdmk_tree tree, long i, double dx, double dy, double dz);
+
+/* current coordinates in the caller's original order (N x 3) */
+int hpdmk_get_positions(hpdmk_tree tree, double* r_src);
+
+/* ---- extensions without a reference counterpart ----------------------- */
+
+/* n independent trial moves evaluated against the same state (no update). */
+int hpdmk_eval_shift_energy_batch(hpdmk_tree tree, int n, const long* idx, const double* d, double* dE);
+
+/* Metropolis chain on the device.  Move k proposes particle idx[k] displaced
+ * by d[3k..3k+2] and is accepted when u[k] < exp(-beta dE).
+ *   mode 0 (plain):   one trial kernel and one update kernel per move
+ *   mode 1 (batched): `batch` trials per step evaluated together, their mutual
+ *                     effects added exactly, decisions taken in order; the
+ *                     chain is identical to the plain one up to roundoff.
+ * hpdmk_mc_load copies the move list to HBM; hpdmk_mc_run runs it without
+ * host transfers and returns the GPU time (ms) between events around the
+ * run; hpdmk_mc_results copies dE and acceptance back. */
+int hpdmk_mc_load(hpdmk_tree tree, int n_moves, const long* idx, const double* d, const double* u);
+int hpdmk_mc_run(hpdmk_tree tree, double beta, int mode, int batch, double* gpu_ms);
+int hpdmk_mc_results(hpdmk_tree tree, double* dE, int* accept);
+
+/* Per-move counts of the last run (6 ints per move): difference levels read
+ * at the target and origin, near-field particles scanned at the target and
+ * origin, expansion arrays an acceptance rewrites for the target and origin
+ * paths.  They size the algorithmic bytes of each move. */
+int hpdmk_mc_stats(hpdmk_tree tree, int* stats);
+
+/* Per-kernel GPU time of the last hpdmk_mc_run with profiling on (events
+ * around every launch).  kinds: 0 trial, 1 update, 2 pair corrections,
+ * 3 decision scan.  ms[4], count[4]. */
+int hpdmk_set_profiling(hpdmk_tree tree, int on);
+int hpdmk_kernel_times(hpdmk_tree tree, double* ms, int* count);
+
+/* Host-only inspection (no device): the octree the library would build.
+ * Returns the node count through *n_nodes; when nodes is non-NULL it must
+ * hold 9 ints per node, written in Morton pre-order as: level, is_leaf,
+ * parent, cell x, cell y, cell z, particles, same-level neighbours, near-list
+ * length.  particle_leaf (optional, n_src ints) receives each particle's leaf. */
+int hpdmk_plan_tree(HPDMKParams params, int n_src, const double* r_src, int* n_nodes, int* nodes,
+                    int* particle_leaf);
+/* PSWF constants of a digits setting: c, lambda, C0, diff0, psi0(0),
+ * n_window, n_diff (out[7]). */
+int hpdmk_pswf_constants(int digits, double* out);
+
+/* Clears a sticky device error (HPDMK_ECAPACITY) after the caller rebuilt or
+ * gave up on the chain. */
+int hpdmk_clear_error(hpdmk_tree tree);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HPDMK_H */

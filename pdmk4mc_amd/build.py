@@ -1,0 +1,32 @@
+"""Compiles libhpdmk_amd.so for gfx950 with hipcc (no GPU needed)."""
+import os
+import pathlib
+import shutil
+import subprocess
+
+PKG = pathlib.Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+LIB = PKG / "libhpdmk_amd.so"
+SOURCES = ["dmk_kernels.hip", "hpdmk_capi.cpp", "dmk_host.cpp"]
+HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
+FLAGS = [
+    "-std=c++17", "-O3", "--offload-arch=gfx950", "-fPIC", "-shared",
+    "-munsafe-fp-atomics",
+    # no packed FP32 instructions (they are avoided on this machine pool)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def build(verbose=False):
+    cmd = [HIPCC, *FLAGS, "-o", str(LIB) + ".tmp", *[str(CSRC / s) for s in SOURCES]]
+    out = subprocess.run(cmd, capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + " ".join(cmd) + "\n" + out.stderr[-6000:])
+    os.replace(str(LIB) + ".tmp", LIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)

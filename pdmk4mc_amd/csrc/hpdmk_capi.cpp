@@ -1,0 +1,777 @@
+// C ABI of libhpdmk_amd.so (declared in include/hpdmk.h): owns the device
+// state of one DMK system and sequences the HIP kernels of dmk_kernels.hip on
+// one stream.  No exception crosses the boundary.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/hpdmk.h"
+#include "dmk_common.hpp"
+#include "dmk_host.hpp"
+#include "dmk_kernels.hpp"
+
+using namespace hpdmk_amd;
+
+namespace {
+
+thread_local std::string g_err;
+
+#define HIPX(call)                                                                                         \
+    do {                                                                                                   \
+        hipError_t e_ = (call);                                                                            \
+        if (e_ != hipSuccess)                                                                              \
+            throw HpdmkError(HPDMK_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_));            \
+    } while (0)
+
+template <typename V>
+V* dalloc(size_t n, std::vector<void*>& owned) {
+    void* p = nullptr;
+    HIPX(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(V)));
+    owned.push_back(p);
+    return static_cast<V*>(p);
+}
+
+template <typename V>
+V* dupload(const std::vector<V>& h, std::vector<void*>& owned) {
+    V* p = dalloc<V>(h.size(), owned);
+    if (!h.empty()) HIPX(hipMemcpy(p, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice));
+    return p;
+}
+
+struct MoveBuf {
+    int n = 0, cap = 0;
+    std::vector<long> h_idx;
+    int* idx = nullptr;
+    double* d = nullptr;
+    double* u = nullptr;
+    double* dE = nullptr;
+    double* dE_stale = nullptr;
+    double* parts = nullptr;
+    int* acc = nullptr;
+    int* stats = nullptr;
+};
+
+enum KernelKind { kKTrial = 0, kKUpdate = 1, kKCorr = 2, kKScan = 3, kKinds = 4 };
+
+struct Handle {
+    HPDMKParams params{};
+    int precision = HPDMK_FP64;
+    PswfConsts pw;
+    Octree tree;
+    int n_src = 0, n_leaves = 0, n_pw = 0;
+    double Q = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void*> owned;
+    DevTree<double> d64{};
+    DevTree<float> d32{};
+    PwNodes pn{};
+    int* leaves = nullptr;
+    double* acc_root = nullptr;
+    double* energy = nullptr;
+    int* err = nullptr;
+    // trial scratch (kMaxBatch trials)
+    double* t_acc = nullptr;
+    unsigned* t_done = nullptr;
+    TrialInfo* t_info = nullptr;
+    int* t_accept = nullptr;
+    double* corr = nullptr;
+    // move lists in HBM: `single` serves eval/update calls, `chain` the staged Metropolis run
+    MoveBuf single, chain;
+    // profiling
+    bool profiling = false;
+    double kms[kKinds] = {0, 0, 0, 0};
+    int kcount[kKinds] = {0, 0, 0, 0};
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
+
+    ~Handle() {
+        for (auto& e : ev_pool) (void)hipEventDestroy(e);
+        for (void* p : owned) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+template <typename F>
+int guard(F&& f) {
+    try {
+        f();
+        return HPDMK_OK;
+    } catch (const HpdmkError& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return HPDMK_ERUNTIME;
+    }
+}
+
+Handle* H(hpdmk_tree t) {
+    if (!t) throw HpdmkError(HPDMK_EINVAL, "null hpdmk_tree handle");
+    return static_cast<Handle*>(t);
+}
+
+template <typename T>
+DevTree<T>& dev(Handle* h);
+template <>
+DevTree<double>& dev<double>(Handle* h) {
+    return h->d64;
+}
+template <>
+DevTree<float>& dev<float>(Handle* h) {
+    return h->d32;
+}
+
+template <typename F>
+void dispatch(Handle* h, F&& f) {
+    if (h->precision == HPDMK_FP64)
+        f(h->d64);
+    else
+        f(h->d32);
+}
+
+void check_err(Handle* h) {
+    int e = 0;
+    HIPX(hipMemcpyAsync(&e, h->err, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPX(hipStreamSynchronize(h->stream));
+    if (e & kErrLeafOverflow)
+        throw HpdmkError(HPDMK_ECAPACITY,
+                         "an accepted move would overflow a leaf's slot capacity; rebuild the tree from "
+                         "hpdmk_get_positions and continue");
+}
+
+hipEvent_t take_event(Handle* h) {
+    if (h->ev_pool.empty()) {
+        hipEvent_t e;
+        HIPX(hipEventCreate(&e));
+        return e;
+    }
+    hipEvent_t e = h->ev_pool.back();
+    h->ev_pool.pop_back();
+    return e;
+}
+
+template <typename L>
+void timed(Handle* h, int kind, L&& launch) {
+    if (!h->profiling) {
+        launch();
+        return;
+    }
+    hipEvent_t a = take_event(h), b = take_event(h);
+    HIPX(hipEventRecord(a, h->stream));
+    launch();
+    HIPX(hipEventRecord(b, h->stream));
+    h->ev_used.push_back({kind, {a, b}});
+}
+
+void collect_events(Handle* h) {
+    if (h->ev_used.empty()) return;
+    HIPX(hipStreamSynchronize(h->stream));
+    for (auto& u : h->ev_used) {
+        float ms = 0;
+        HIPX(hipEventElapsedTime(&ms, u.second.first, u.second.second));
+        h->kms[u.first] += ms;
+        h->kcount[u.first] += 1;
+        h->ev_pool.push_back(u.second.first);
+        h->ev_pool.push_back(u.second.second);
+    }
+    h->ev_used.clear();
+}
+
+template <typename T>
+void fill_dev(Handle* h, DevTree<T>& d, const std::vector<std::vector<double>>& dtabs, const std::vector<double>& wtab,
+              const std::vector<int>& cap, const std::vector<int>& start, const std::vector<int>& count) {
+    auto& o = h->owned;
+    const Octree& tr = h->tree;
+    std::vector<T> W(wtab.begin(), wtab.end());
+    std::vector<T> D;
+    for (auto& v : dtabs) D.insert(D.end(), v.begin(), v.end());
+    d.Wmat = dupload(W, o);
+    d.Dmat = dupload(D, o);
+    d.root = dalloc<Cplx<T>>(d.Mw, o);
+    d.out = dalloc<Cplx<T>>((size_t)d.n_pw_nodes * d.Md, o);
+    d.inc = dalloc<Cplx<T>>((size_t)d.n_pw_nodes * d.Md, o);
+    HIPX(hipMemset(d.root, 0, sizeof(Cplx<T>) * d.Mw));
+    HIPX(hipMemset(d.out, 0, sizeof(Cplx<T>) * std::max<size_t>(1, (size_t)d.n_pw_nodes * d.Md)));
+    HIPX(hipMemset(d.inc, 0, sizeof(Cplx<T>) * std::max<size_t>(1, (size_t)d.n_pw_nodes * d.Md)));
+    (void)tr;
+    (void)cap;
+    (void)start;
+    (void)count;
+}
+
+}  // namespace
+
+extern "C" {
+
+HPDMKParams hpdmk_default_params(void) {
+    HPDMKParams p;
+    p.n_per_leaf = 200;  // include/hpdmk.h:12-16
+    p.digits = 3;
+    p.L = 0.0;
+    p.prolate_order = 16;
+    p.init = PROXY;
+    return p;
+}
+
+const char* hpdmk_last_error(void) { return g_err.c_str(); }
+
+int hpdmk_tree_create_ex(HPDMKParams params, int precision, int n_src, const double* r_src, const double* charge,
+                         hpdmk_tree* out) {
+    return guard([&] {
+        if (!out || !r_src || !charge) throw HpdmkError(HPDMK_EINVAL, "null argument");
+        *out = nullptr;
+        if (precision != HPDMK_FP64 && precision != HPDMK_MIXED)
+            throw HpdmkError(HPDMK_EINVAL, "precision must be HPDMK_FP64 or HPDMK_MIXED");
+        if (precision == HPDMK_MIXED && params.digits != 3 && params.digits != 6)
+            throw HpdmkError(HPDMK_EINVAL, "float only supports 3 and 6 digits accuracy");  // src/tree.cpp:19-23
+        if (params.n_per_leaf < 1) throw HpdmkError(HPDMK_EINVAL, "n_per_leaf must be positive");
+        auto h = std::make_unique<Handle>();
+        h->params = params;
+        h->precision = precision;
+        try {
+            h->pw = pswf_consts(params.digits);
+        } catch (const HpdmkError& e) {
+            throw HpdmkError(HPDMK_EINVAL, e.what());
+        }
+        h->tree = build_octree(n_src, r_src, params.L, params.n_per_leaf);
+        h->n_src = n_src;
+        const Octree& tr = h->tree;
+        const int D = tr.max_depth;
+        if (D > kMaxLevels - 1) throw HpdmkError(HPDMK_ERUNTIME, "tree deeper than supported");
+        HIPX(hipStreamCreate(&h->stream));
+        auto& o = h->owned;
+
+        // ---- slots: leaf B owns [start, start + cap)
+        const int nn = tr.n_nodes;
+        std::vector<int> cap(nn, 0), start(nn, 0), count(nn, 0);
+        int total = 0;
+        for (int i = 0; i < nn; ++i) {
+            if (!tr.is_leaf[i]) continue;
+            count[i] = tr.leaf_count[i];
+            cap[i] = ((std::max(2 * count[i], count[i] + 64) + 3) / 4) * 4;
+            start[i] = total;
+            total += cap[i];
+            ++h->n_leaves;
+        }
+        std::vector<double> px(total, 0), py(total, 0), pz(total, 0), pq(total, 0);
+        std::vector<int> porig(total, -1), slot_of(n_src), leaf_of(n_src);
+        {
+            int pos = 0;
+            for (int i = 0; i < nn; ++i) {
+                if (!tr.is_leaf[i]) continue;
+                for (int k = 0; k < count[i]; ++k) {
+                    const int id = tr.order[pos++];
+                    const int s = start[i] + k;
+                    px[s] = r_src[3 * id];
+                    py[s] = r_src[3 * id + 1];
+                    pz[s] = r_src[3 * id + 2];
+                    pq[s] = charge[id];
+                    porig[s] = id;
+                    slot_of[id] = s;
+                    leaf_of[id] = i;
+                }
+            }
+        }
+        for (int i = 0; i < n_src; ++i) h->Q += charge[i] * charge[i];  // src/tree.cpp:306-309
+
+        // ---- boxes carrying expansions: levels 2..D-2
+        std::vector<int> pw_slot(nn, -1), pn_node, pn_level, pn_lo, pn_hi;
+        for (int i = 0; i < nn; ++i)
+            if (tr.level[i] >= 2 && tr.level[i] <= D - 2) {
+                pw_slot[i] = (int)pn_node.size();
+                pn_node.push_back(i);
+                pn_level.push_back(tr.level[i]);
+                pn_lo.push_back(start[tr.first_leaf[i]]);
+                pn_hi.push_back(start[tr.last_leaf[i]] + cap[tr.last_leaf[i]]);
+            }
+        h->n_pw = (int)pn_node.size();
+        std::vector<int> leaves;
+        int max_near = 0;
+        for (int i = 0; i < nn; ++i)
+            if (tr.is_leaf[i]) {
+                leaves.push_back(i);
+                max_near = std::max(max_near, tr.near_off[i + 1] - tr.near_off[i]);
+            }
+
+        // ---- scalars shared by both precisions
+        auto set_common = [&](auto& d) {
+            d.n_src = n_src;
+            d.n_nodes = nn;
+            d.max_depth = D;
+            d.n_window = h->pw.n_window;
+            d.n_diff = h->pw.n_diff;
+            d.dw = 2 * d.n_window + 1;
+            d.dd = 2 * d.n_diff + 1;
+            d.Mw = d.dw * d.dw * (d.n_window + 1);
+            d.Md = d.dd * d.dd * (d.n_diff + 1);
+            d.n_pw_nodes = h->n_pw;
+            d.total_slots = total;
+            d.max_near = max_near;
+            d.L = params.L;
+            for (int l = 0; l <= kMaxLevels; ++l) d.boxsize[l] = params.L / std::ldexp(1.0, l);
+            d.delta_k[0] = d.delta_k[1] = 2 * M_PI / params.L;
+            for (int l = 2; l <= kMaxLevels; ++l) d.delta_k[l] = (2.0 * M_PI / 3.0) / d.boxsize[l];
+            const double tp3 = std::pow(2 * M_PI, 3);
+            d.cW = std::pow(d.delta_k[0], 3) / (2 * tp3);
+            for (int l = 0; l < kMaxLevels; ++l) {
+                d.cD[l] = std::pow(d.delta_k[l], 3) / (2 * tp3);
+                d.self_diff[l] = h->pw.psi0_0 * (1.0 / (2 * d.boxsize[l + 1] * h->pw.C0) -
+                                                 1.0 / (2 * d.boxsize[l] * h->pw.C0));
+            }
+            d.self_window = h->pw.psi0_0 / (2 * d.boxsize[2] * h->pw.C0);
+            d.n_poly = (int)h->pw.poly.size();
+            for (int k = 0; k < d.n_poly; ++k) d.poly[k] = h->pw.poly[k];
+        };
+        std::vector<int8_t> leaf8(tr.is_leaf.begin(), tr.is_leaf.end());
+        const int* child = dupload(tr.child, o);
+        const int* level = dupload(tr.level, o);
+        const int8_t* isl = dupload(leaf8, o);
+        const int* cellp = dupload(tr.cell, o);
+        const int* coll = dupload(tr.colleague, o);
+        const int* pws = dupload(pw_slot, o);
+        const int* noff = dupload(tr.near_off, o);
+        const int* nids = dupload(tr.near_ids, o);
+        const int8_t* nimg = dupload(tr.near_img, o);
+        const int* sstart = dupload(start, o);
+        const int* scap = dupload(cap, o);
+        int* scount = dupload(count, o);
+        double* dpx = dupload(px, o);
+        double* dpy = dupload(py, o);
+        double* dpz = dupload(pz, o);
+        double* dpq = dupload(pq, o);
+        int* dporig = dupload(porig, o);
+        int* dslot = dupload(slot_of, o);
+        int* dleaf = dupload(leaf_of, o);
+        h->err = dalloc<int>(1, o);
+        HIPX(hipMemset(h->err, 0, sizeof(int)));
+        auto set_ptrs = [&](auto& d) {
+            d.child = child;
+            d.level = level;
+            d.is_leaf = isl;
+            d.cell = cellp;
+            d.colleague = coll;
+            d.pw_slot = pws;
+            d.near_off = noff;
+            d.near_ids = nids;
+            d.near_img = nimg;
+            d.slot_start = sstart;
+            d.slot_cap = scap;
+            d.slot_count = scount;
+            d.px = dpx;
+            d.py = dpy;
+            d.pz = dpz;
+            d.pq = dpq;
+            d.porig = dporig;
+            d.slot_of = dslot;
+            d.leaf_of = dleaf;
+            d.err = h->err;
+        };
+        std::vector<double> wtab = window_table(h->pw, params.L);
+        std::vector<std::vector<double>> dtabs;
+        for (int l = 2; l <= D - 2; ++l) dtabs.push_back(difference_table(h->pw, params.L, l));
+        if (precision == HPDMK_FP64) {
+            set_common(h->d64);
+            set_ptrs(h->d64);
+            fill_dev(h.get(), h->d64, dtabs, wtab, cap, start, count);
+        } else {
+            set_common(h->d32);
+            set_ptrs(h->d32);
+            fill_dev(h.get(), h->d32, dtabs, wtab, cap, start, count);
+        }
+        h->pn.node = dupload(pn_node, o);
+        h->pn.level = dupload(pn_level, o);
+        h->pn.lo = dupload(pn_lo, o);
+        h->pn.hi = dupload(pn_hi, o);
+        h->leaves = dupload(leaves, o);
+        const int Mw = (2 * h->pw.n_window + 1) * (2 * h->pw.n_window + 1) * (h->pw.n_window + 1);
+        h->acc_root = dalloc<double>(2 * Mw, o);
+        h->energy = dalloc<double>(4, o);
+        h->t_acc = dalloc<double>(kAccPerTrial * kMaxBatch, o);
+        h->t_done = dalloc<unsigned>(kMaxBatch, o);
+        h->t_info = dalloc<TrialInfo>(kMaxBatch, o);
+        h->t_accept = dalloc<int>(kMaxBatch, o);
+        h->corr = dalloc<double>(kMaxBatch * kMaxBatch, o);
+        HIPX(hipMemset(h->t_acc, 0, sizeof(double) * kAccPerTrial * kMaxBatch));
+        HIPX(hipMemset(h->t_done, 0, sizeof(unsigned) * kMaxBatch));
+        HIPX(hipDeviceSynchronize());
+        *out = h.release();
+    });
+}
+
+int hpdmk_tree_create(HPDMKParams params, int n_src, const double* r_src, const double* charge, hpdmk_tree* out) {
+    return hpdmk_tree_create_ex(params, HPDMK_FP64, n_src, r_src, charge, out);
+}
+
+int hpdmk_tree_destroy(hpdmk_tree t) {
+    return guard([&] {
+        Handle* h = H(t);
+        (void)hipStreamSynchronize(h->stream);
+        delete h;
+    });
+}
+
+int hpdmk_tree_info(hpdmk_tree t, int* info) {
+    return guard([&] {
+        Handle* h = H(t);
+        dispatch(h, [&](auto& d) {
+            info[0] = d.n_nodes;
+            info[1] = d.max_depth;
+            info[2] = d.n_window;
+            info[3] = d.n_diff;
+            info[4] = d.Mw;
+            info[5] = d.Md;
+            info[6] = d.n_pw_nodes;
+            info[7] = d.max_near;
+            info[8] = h->n_leaves;
+            info[9] = d.total_slots;
+        });
+    });
+}
+
+int hpdmk_form_outgoing_pw(hpdmk_tree t) {
+    return guard([&] {
+        Handle* h = H(t);
+        if (h->params.init == PROXY && h->params.digits > 3)  // src/upward_pass.cpp:9-14
+            throw HpdmkError(HPDMK_ENOTIMPL, "Polynomial order for requested precision not implemented");
+        dispatch(h, [&](auto& d) { launch_outgoing(d, h->pn, h->acc_root, h->stream); });
+        HIPX(hipGetLastError());
+        HIPX(hipStreamSynchronize(h->stream));
+    });
+}
+
+int hpdmk_form_incoming_pw(hpdmk_tree t) {
+    return guard([&] {
+        Handle* h = H(t);
+        dispatch(h, [&](auto& d) { launch_incoming(d, h->pn, h->stream); });
+        HIPX(hipGetLastError());
+        HIPX(hipStreamSynchronize(h->stream));
+    });
+}
+
+int hpdmk_eval_energy(hpdmk_tree t, double* energy, double* parts) {
+    return guard([&] {
+        Handle* h = H(t);
+        double e[3];
+        dispatch(h, [&](auto& d) { launch_energy(d, h->pn, h->leaves, h->n_leaves, h->energy, h->stream); });
+        HIPX(hipGetLastError());
+        HIPX(hipMemcpyAsync(e, h->energy, sizeof(e), hipMemcpyDeviceToHost, h->stream));
+        HIPX(hipStreamSynchronize(h->stream));
+        const double self_w = h->Q * (h->precision == HPDMK_FP64 ? h->d64.self_window : h->d32.self_window);
+        e[0] -= self_w;  // src/energy.cpp:33-36
+        if (parts) std::memcpy(parts, e, sizeof(e));
+        if (energy) *energy = e[0] + e[1] + e[2];
+    });
+}
+
+namespace {
+void ensure_moves(Handle* h, MoveBuf& b, int n) {
+    if (n <= b.cap) return;
+    auto& o = h->owned;
+    const int c = std::max(n, 1024);
+    b.idx = dalloc<int>(c, o);
+    b.d = dalloc<double>(3 * (size_t)c, o);
+    b.u = dalloc<double>(c, o);
+    b.dE = dalloc<double>(c, o);
+    b.dE_stale = dalloc<double>(c, o);
+    b.parts = dalloc<double>(5 * (size_t)c, o);
+    b.acc = dalloc<int>(c, o);
+    b.stats = dalloc<int>(kStatsPerMove * (size_t)c, o);
+    b.cap = c;
+}
+
+void upload_moves(Handle* h, MoveBuf& b, int n, const long* idx, const double* d, const double* u) {
+    for (int k = 0; k < n; ++k)
+        if (idx[k] < 0 || idx[k] >= h->n_src) throw HpdmkError(HPDMK_EINVAL, "particle index out of range");
+    ensure_moves(h, b, n);
+    std::vector<int> i32(idx, idx + n);
+    b.h_idx.assign(idx, idx + n);
+    HIPX(hipMemcpyAsync(b.idx, i32.data(), sizeof(int) * n, hipMemcpyHostToDevice, h->stream));
+    HIPX(hipMemcpyAsync(b.d, d, sizeof(double) * 3 * n, hipMemcpyHostToDevice, h->stream));
+    if (u) HIPX(hipMemcpyAsync(b.u, u, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+    HIPX(hipStreamSynchronize(h->stream));
+    b.n = n;
+}
+
+TrialArgs trial_args(Handle* h, const MoveBuf& b, int move0, int mode, double beta, double* dE) {
+    TrialArgs a{};
+    a.idx = b.idx;
+    a.d = b.d;
+    a.u = b.u;
+    a.beta = beta;
+    a.move0 = move0;
+    a.mode = mode;
+    a.acc = h->t_acc;
+    a.done = h->t_done;
+    a.dE = dE;
+    a.parts = b.parts;
+    a.accept = h->t_accept;
+    a.accept_out = b.acc;
+    a.info = h->t_info;
+    a.stats = b.stats;
+    return a;
+}
+}  // namespace
+
+int hpdmk_eval_shift_energy(hpdmk_tree t, long i, double dx, double dy, double dz, double* dE, double* parts) {
+    return guard([&] {
+        Handle* h = H(t);
+        const double d[3] = {dx, dy, dz};
+        upload_moves(h, h->single, 1, &i, d, nullptr);
+        dispatch(h, [&](auto& dv) {
+            launch_trial(dv, trial_args(h, h->single, 0, kTrialOnly, 0.0, h->single.dE), 1, h->stream);
+        });
+        HIPX(hipGetLastError());
+        double p[5], e;
+        HIPX(hipMemcpyAsync(&e, h->single.dE, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPX(hipMemcpyAsync(p, h->single.parts, sizeof(p), hipMemcpyDeviceToHost, h->stream));
+        check_err(h);
+        if (dE) *dE = e;
+        if (parts) std::memcpy(parts, p, sizeof(p));
+    });
+}
+
+int hpdmk_update_shift(hpdmk_tree t, long i, double dx, double dy, double dz) {
+    return guard([&] {
+        Handle* h = H(t);
+        const double d[3] = {dx, dy, dz};
+        upload_moves(h, h->single, 1, &i, d, nullptr);
+        dispatch(h, [&](auto& dv) {
+            launch_trial(dv, trial_args(h, h->single, 0, kTrialForceAccept, 0.0, h->single.dE), 1, h->stream);
+            UpdateArgs ua{1, h->t_accept, h->t_info};
+            launch_update(dv, ua, h->stream);
+        });
+        HIPX(hipGetLastError());
+        check_err(h);
+    });
+}
+
+int hpdmk_get_positions(hpdmk_tree t, double* r) {
+    return guard([&] {
+        Handle* h = H(t);
+        dispatch(h, [&](auto& d) {
+            const int S = d.total_slots;
+            std::vector<double> x(S), y(S), z(S);
+            std::vector<int> orig(S);
+            HIPX(hipMemcpy(x.data(), d.px, sizeof(double) * S, hipMemcpyDeviceToHost));
+            HIPX(hipMemcpy(y.data(), d.py, sizeof(double) * S, hipMemcpyDeviceToHost));
+            HIPX(hipMemcpy(z.data(), d.pz, sizeof(double) * S, hipMemcpyDeviceToHost));
+            HIPX(hipMemcpy(orig.data(), d.porig, sizeof(int) * S, hipMemcpyDeviceToHost));
+            for (int s = 0; s < S; ++s)
+                if (orig[s] >= 0) {
+                    r[3 * orig[s]] = x[s];
+                    r[3 * orig[s] + 1] = y[s];
+                    r[3 * orig[s] + 2] = z[s];
+                }
+        });
+    });
+}
+
+int hpdmk_eval_shift_energy_batch(hpdmk_tree t, int n, const long* idx, const double* d, double* dE) {
+    return guard([&] {
+        Handle* h = H(t);
+        if (n < 0) throw HpdmkError(HPDMK_EINVAL, "n must be non-negative");
+        if (n == 0) return;
+        upload_moves(h, h->single, n, idx, d, nullptr);
+        for (int m0 = 0; m0 < n; m0 += kMaxBatch) {
+            const int k = std::min(kMaxBatch, n - m0);
+            dispatch(h, [&](auto& dv) {
+                launch_trial(dv, trial_args(h, h->single, m0, kTrialOnly, 0.0, h->single.dE), k, h->stream);
+            });
+        }
+        HIPX(hipGetLastError());
+        HIPX(hipMemcpyAsync(dE, h->single.dE, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+        check_err(h);
+    });
+}
+
+int hpdmk_mc_load(hpdmk_tree t, int n, const long* idx, const double* d, const double* u) {
+    return guard([&] {
+        Handle* h = H(t);
+        if (n <= 0 || !idx || !d || !u) throw HpdmkError(HPDMK_EINVAL, "empty or null move list");
+        upload_moves(h, h->chain, n, idx, d, u);
+    });
+}
+
+int hpdmk_mc_run(hpdmk_tree t, double beta, int mode, int batch, double* gpu_ms) {
+    return guard([&] {
+        Handle* h = H(t);
+        MoveBuf& mb = h->chain;
+        const int n = mb.n;
+        if (n <= 0) throw HpdmkError(HPDMK_EINVAL, "no moves loaded (hpdmk_mc_load)");
+        if (mode != 0 && mode != 1) throw HpdmkError(HPDMK_EINVAL, "mode must be 0 (plain) or 1 (batched)");
+        if (mode == 1 && (batch < 1 || batch > kMaxBatch)) throw HpdmkError(HPDMK_EINVAL, "batch must be in [1, 256]");
+        for (int k = 0; k < kKinds; ++k) {
+            h->kms[k] = 0;
+            h->kcount[k] = 0;
+        }
+        // batches never repeat a particle (the corrections assume distinct particles)
+        std::vector<std::pair<int, int>> batches;
+        if (mode == 1) {
+            std::vector<int> seen(h->n_src, -1);
+            int m0 = 0;
+            for (int m = 0; m < n; ++m) {
+                const int p = (int)mb.h_idx[m];
+                if (m - m0 == batch || seen[p] >= m0) {
+                    batches.push_back({m0, m - m0});
+                    m0 = m;
+                }
+                seen[p] = m;
+            }
+            batches.push_back({m0, n - m0});
+        }
+        hipEvent_t e0, e1;
+        HIPX(hipEventCreate(&e0));
+        HIPX(hipEventCreate(&e1));
+        HIPX(hipEventRecord(e0, h->stream));
+        dispatch(h, [&](auto& dv) {
+            if (mode == 0) {
+                for (int m = 0; m < n; ++m) {
+                    timed(h, kKTrial, [&] {
+                        launch_trial(dv, trial_args(h, mb, m, kTrialMetropolis, beta, mb.dE), 1, h->stream);
+                    });
+                    timed(h, kKUpdate, [&] {
+                        UpdateArgs ua{1, h->t_accept, h->t_info};
+                        launch_update(dv, ua, h->stream);
+                    });
+                }
+            } else {
+                for (auto& b : batches) {
+                    const int m0 = b.first, k = b.second;
+                    timed(h, kKTrial, [&] {
+                        launch_trial(dv, trial_args(h, mb, m0, kTrialOnly, beta, mb.dE_stale), k, h->stream);
+                    });
+                    if (k > 1)
+                        timed(h, kKCorr, [&] {
+                            CorrArgs ca{k, h->t_info, h->corr};
+                            launch_pair_corr(dv, ca, h->stream);
+                        });
+                    timed(h, kKScan, [&] {
+                        ScanArgs sa{k, m0, h->t_info, h->corr, mb.dE_stale, mb.u, beta, mb.dE, h->t_accept, mb.acc};
+                        launch_scan(dv, sa, h->stream);
+                    });
+                    timed(h, kKUpdate, [&] {
+                        UpdateArgs ua{k, h->t_accept, h->t_info};
+                        launch_update(dv, ua, h->stream);
+                    });
+                }
+            }
+        });
+        HIPX(hipGetLastError());
+        HIPX(hipEventRecord(e1, h->stream));
+        HIPX(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPX(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        collect_events(h);
+        if (gpu_ms) *gpu_ms = ms;
+        check_err(h);
+    });
+}
+
+int hpdmk_mc_results(hpdmk_tree t, double* dE, int* accept) {
+    return guard([&] {
+        Handle* h = H(t);
+        const MoveBuf& mb = h->chain;
+        if (dE) HIPX(hipMemcpyAsync(dE, mb.dE, sizeof(double) * mb.n, hipMemcpyDeviceToHost, h->stream));
+        if (accept) HIPX(hipMemcpyAsync(accept, mb.acc, sizeof(int) * mb.n, hipMemcpyDeviceToHost, h->stream));
+        HIPX(hipStreamSynchronize(h->stream));
+    });
+}
+
+int hpdmk_mc_stats(hpdmk_tree t, int* stats) {
+    return guard([&] {
+        Handle* h = H(t);
+        const MoveBuf& mb = h->chain;
+        HIPX(hipMemcpyAsync(stats, mb.stats, sizeof(int) * kStatsPerMove * mb.n, hipMemcpyDeviceToHost, h->stream));
+        HIPX(hipStreamSynchronize(h->stream));
+    });
+}
+
+int hpdmk_set_profiling(hpdmk_tree t, int on) {
+    return guard([&] { H(t)->profiling = on != 0; });
+}
+
+int hpdmk_kernel_times(hpdmk_tree t, double* ms, int* count) {
+    return guard([&] {
+        Handle* h = H(t);
+        for (int k = 0; k < kKinds; ++k) {
+            if (ms) ms[k] = h->kms[k];
+            if (count) count[k] = h->kcount[k];
+        }
+    });
+}
+
+int hpdmk_plan_tree(HPDMKParams params, int n_src, const double* r_src, int* n_nodes, int* nodes,
+                    int* particle_leaf) {
+    return guard([&] {
+        if (!r_src || !n_nodes) throw HpdmkError(HPDMK_EINVAL, "null argument");
+        if (params.n_per_leaf < 1) throw HpdmkError(HPDMK_EINVAL, "n_per_leaf must be positive");
+        const Octree tr = build_octree(n_src, r_src, params.L, params.n_per_leaf);
+        *n_nodes = tr.n_nodes;
+        if (nodes) {
+            std::vector<int> sub(tr.n_nodes, 0);
+            for (int i = tr.n_nodes - 1; i >= 0; --i) {
+                sub[i] = tr.is_leaf[i] ? tr.leaf_count[i] : 0;
+                if (!tr.is_leaf[i])
+                    for (int c = 0; c < 8; ++c) sub[i] += sub[tr.child[8 * i + c]];
+            }
+            for (int i = 0; i < tr.n_nodes; ++i) {
+                int* o = nodes + 9 * i;
+                o[0] = tr.level[i];
+                o[1] = tr.is_leaf[i];
+                o[2] = tr.parent[i];
+                o[3] = tr.cell[3 * i];
+                o[4] = tr.cell[3 * i + 1];
+                o[5] = tr.cell[3 * i + 2];
+                o[6] = sub[i];
+                int nc = 0;
+                if (tr.level[i] >= 2)
+                    for (int k = 0; k < 26; ++k) nc += tr.colleague[26 * i + k] >= 0;
+                o[7] = nc;
+                o[8] = tr.near_off[i + 1] - tr.near_off[i];
+            }
+        }
+        if (particle_leaf) {
+            int pos = 0;
+            for (int i = 0; i < tr.n_nodes; ++i)
+                if (tr.is_leaf[i])
+                    for (int k = 0; k < tr.leaf_count[i]; ++k) particle_leaf[tr.order[pos++]] = i;
+        }
+    });
+}
+
+int hpdmk_pswf_constants(int digits, double* out) {
+    return guard([&] {
+        PswfConsts p;
+        try {
+            p = pswf_consts(digits);
+        } catch (const HpdmkError& e) {
+            throw HpdmkError(HPDMK_EINVAL, e.what());
+        }
+        out[0] = p.c;
+        out[1] = p.lambda;
+        out[2] = p.C0;
+        out[3] = p.diff0;
+        out[4] = p.psi0_0;
+        out[5] = p.n_window;
+        out[6] = p.n_diff;
+    });
+}
+
+int hpdmk_clear_error(hpdmk_tree t) {
+    return guard([&] {
+        Handle* h = H(t);
+        HIPX(hipMemsetAsync(h->err, 0, sizeof(int), h->stream));
+        HIPX(hipMemsetAsync(h->t_acc, 0, sizeof(double) * kAccPerTrial * kMaxBatch, h->stream));
+        HIPX(hipMemsetAsync(h->t_done, 0, sizeof(unsigned) * kMaxBatch, h->stream));
+        HIPX(hipStreamSynchronize(h->stream));
+    });
+}
+
+}  // extern "C"

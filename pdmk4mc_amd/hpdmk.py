@@ -1,0 +1,235 @@
+"""ctypes mirror of the reference's HPDMKPtTree (include/tree.hpp:28-148) over
+the C ABI in include/hpdmk.h.  Fails loudly when libhpdmk_amd.so is missing:
+there is no CPU fallback."""
+import ctypes as C
+import pathlib
+
+import numpy as np
+
+DIRECT, PROXY = 1, 2
+HPDMK_FP64, HPDMK_MIXED = 0, 1
+_LIB = pathlib.Path(__file__).resolve().parent / "libhpdmk_amd.so"
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_lp = C.POINTER(C.c_long)
+
+
+class HPDMKParams(C.Structure):
+    """include/hpdmk.h:11-17 (defaults of the reference)."""
+    _fields_ = [("n_per_leaf", C.c_int), ("digits", C.c_int), ("L", C.c_double), ("prolate_order", C.c_double),
+                ("init", C.c_int)]
+
+    def __init__(self, n_per_leaf=200, digits=3, L=0.0, prolate_order=16.0, init=PROXY):
+        super().__init__(n_per_leaf, digits, L, prolate_order, init)
+
+
+class HPDMKError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+# every entry point of include/hpdmk.h with its ctypes signature
+SIGNATURES = {
+    "hpdmk_default_params": (HPDMKParams, []),
+    "hpdmk_last_error": (C.c_char_p, []),
+    "hpdmk_tree_create": (C.c_int, [HPDMKParams, C.c_int, _dp, _dp, C.POINTER(C.c_void_p)]),
+    "hpdmk_tree_create_ex": (C.c_int, [HPDMKParams, C.c_int, C.c_int, _dp, _dp, C.POINTER(C.c_void_p)]),
+    "hpdmk_tree_destroy": (C.c_int, [C.c_void_p]),
+    "hpdmk_tree_info": (C.c_int, [C.c_void_p, _ip]),
+    "hpdmk_form_outgoing_pw": (C.c_int, [C.c_void_p]),
+    "hpdmk_form_incoming_pw": (C.c_int, [C.c_void_p]),
+    "hpdmk_eval_energy": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "hpdmk_eval_shift_energy": (C.c_int, [C.c_void_p, C.c_long, C.c_double, C.c_double, C.c_double, _dp, _dp]),
+    "hpdmk_update_shift": (C.c_int, [C.c_void_p, C.c_long, C.c_double, C.c_double, C.c_double]),
+    "hpdmk_get_positions": (C.c_int, [C.c_void_p, _dp]),
+    "hpdmk_eval_shift_energy_batch": (C.c_int, [C.c_void_p, C.c_int, _lp, _dp, _dp]),
+    "hpdmk_mc_load": (C.c_int, [C.c_void_p, C.c_int, _lp, _dp, _dp]),
+    "hpdmk_mc_run": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_int, _dp]),
+    "hpdmk_mc_results": (C.c_int, [C.c_void_p, _dp, _ip]),
+    "hpdmk_mc_stats": (C.c_int, [C.c_void_p, _ip]),
+    "hpdmk_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "hpdmk_kernel_times": (C.c_int, [C.c_void_p, _dp, _ip]),
+    "hpdmk_clear_error": (C.c_int, [C.c_void_p]),
+    "hpdmk_plan_tree": (C.c_int, [HPDMKParams, C.c_int, _dp, _ip, _ip, _ip]),
+    "hpdmk_pswf_constants": (C.c_int, [C.c_int, _dp]),
+}
+
+
+def lib_path():
+    return _LIB
+
+
+def load_library():
+    global _lib
+    if _lib is None:
+        if not _LIB.exists():
+            raise ImportError(f"{_LIB} is missing: build it with pdmk4mc_amd.build.build() (hipcc, gfx950)")
+        lib = C.CDLL(str(_LIB))
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise HPDMKError(rc, load_library().hpdmk_last_error().decode())
+
+
+def _d(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+    return a, a.ctypes.data_as(_dp)
+
+
+def plan_tree(params, r_src):
+    """Host-only octree plan: (nodes[n,9], particle_leaf[n_src]); no device needed."""
+    L = load_library()
+    r, pr = _d(r_src)
+    n = r.size // 3
+    nn = C.c_int()
+    _check(L.hpdmk_plan_tree(params, n, pr, C.byref(nn), None, None))
+    nodes = np.zeros((nn.value, 9), dtype=np.int32)
+    leaf = np.zeros(n, dtype=np.int32)
+    _check(L.hpdmk_plan_tree(params, n, pr, C.byref(nn), nodes.ctypes.data_as(_ip), leaf.ctypes.data_as(_ip)))
+    return nodes, leaf
+
+
+def pswf_constants(digits):
+    out = np.zeros(7)
+    _check(load_library().hpdmk_pswf_constants(int(digits), out.ctypes.data_as(_dp)))
+    return dict(c=out[0], lam=out[1], C0=out[2], diff0=out[3], psi0_0=out[4], n_window=int(out[5]),
+                n_diff=int(out[6]))
+
+
+class HPDMKPtTree:
+    """Device-resident DMK system; method names follow include/tree.hpp:98-132."""
+
+    def __init__(self, params, r_src, charge, precision=HPDMK_FP64):
+        L = load_library()
+        r, pr = _d(r_src)
+        q, pq = _d(charge)
+        if r.size != 3 * q.size:
+            raise ValueError("r_src must hold 3 coordinates per charge")
+        self.params = params
+        self.n_src = q.size
+        self.precision = precision
+        h = C.c_void_p()
+        _check(L.hpdmk_tree_create_ex(params, precision, q.size, pr, pq, C.byref(h)))
+        self._h = h
+        info = np.zeros(10, dtype=np.int32)
+        _check(L.hpdmk_tree_info(h, info.ctypes.data_as(_ip)))
+        (self.n_nodes, self.max_depth, self.n_window, self.n_diff, self.dim_window, self.dim_diff,
+         self.n_pw_boxes, self.max_near, self.n_leaves, self.n_slots) = [int(v) for v in info]
+
+    # ---- lifecycle
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().hpdmk_tree_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- reference interface
+    def form_outgoing_pw(self):
+        _check(load_library().hpdmk_form_outgoing_pw(self._h))
+
+    def form_incoming_pw(self):
+        _check(load_library().hpdmk_form_incoming_pw(self._h))
+
+    def energy_parts(self):
+        e = C.c_double()
+        parts = np.zeros(3)
+        _check(load_library().hpdmk_eval_energy(self._h, C.byref(e), parts.ctypes.data_as(_dp)))
+        return parts
+
+    def eval_energy(self):
+        return float(self.energy_parts().sum())
+
+    def eval_energy_window(self):
+        return float(self.energy_parts()[0])
+
+    def eval_energy_diff(self):
+        return float(self.energy_parts()[1])
+
+    def eval_energy_res(self):
+        return float(self.energy_parts()[2])
+
+    def eval_shift_energy(self, i, dx, dy, dz, parts=False):
+        e = C.c_double()
+        p = np.zeros(5)
+        _check(load_library().hpdmk_eval_shift_energy(self._h, int(i), float(dx), float(dy), float(dz), C.byref(e),
+                                                      p.ctypes.data_as(_dp)))
+        return (e.value, p) if parts else e.value
+
+    def update_shift(self, i, dx, dy, dz):
+        _check(load_library().hpdmk_update_shift(self._h, int(i), float(dx), float(dy), float(dz)))
+
+    # ---- extensions
+    def positions(self):
+        r = np.zeros((self.n_src, 3))
+        _check(load_library().hpdmk_get_positions(self._h, r.ctypes.data_as(_dp)))
+        return r
+
+    def eval_shift_energy_batch(self, idx, d):
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        d, pd = _d(d)
+        out = np.zeros(idx.size)
+        _check(load_library().hpdmk_eval_shift_energy_batch(self._h, idx.size, idx.ctypes.data_as(_lp), pd,
+                                                            out.ctypes.data_as(_dp)))
+        return out
+
+    def mc_load(self, idx, d, u):
+        self._mc_idx = np.ascontiguousarray(idx, dtype=np.int64)
+        d, pd = _d(d)
+        u, pu = _d(u)
+        _check(load_library().hpdmk_mc_load(self._h, self._mc_idx.size, self._mc_idx.ctypes.data_as(_lp), pd, pu))
+
+    def mc_run(self, beta=1.0, mode=0, batch=64):
+        ms = C.c_double()
+        _check(load_library().hpdmk_mc_run(self._h, float(beta), int(mode), int(batch), C.byref(ms)))
+        return ms.value
+
+    def mc_results(self):
+        n = self._mc_idx.size
+        dE = np.zeros(n)
+        acc = np.zeros(n, dtype=np.int32)
+        _check(load_library().hpdmk_mc_results(self._h, dE.ctypes.data_as(_dp), acc.ctypes.data_as(_ip)))
+        return dE, acc
+
+    def mc_chain(self, idx, d, u, beta=1.0, mode=0, batch=64):
+        self.mc_load(idx, d, u)
+        ms = self.mc_run(beta, mode, batch)
+        dE, acc = self.mc_results()
+        return dE, acc, ms
+
+    def mc_stats(self):
+        st = np.zeros((self._mc_idx.size, 6), dtype=np.int32)
+        _check(load_library().hpdmk_mc_stats(self._h, st.ctypes.data_as(_ip)))
+        return st
+
+    def set_profiling(self, on=True):
+        _check(load_library().hpdmk_set_profiling(self._h, int(on)))
+
+    def kernel_times(self):
+        ms = np.zeros(4)
+        cnt = np.zeros(4, dtype=np.int32)
+        _check(load_library().hpdmk_kernel_times(self._h, ms.ctypes.data_as(_dp), cnt.ctypes.data_as(_ip)))
+        return dict(zip(["trial", "update", "pair_corr", "scan"], zip(ms.tolist(), cnt.tolist())))
+
+    def clear_error(self):
+        _check(load_library().hpdmk_clear_error(self._h))

@@ -1,0 +1,68 @@
+"""CPU checks of the product library: it loads, exports every entry point of
+include/hpdmk.h, and its host-side set-up (octree, PSWF constants) agrees with
+the oracle.  No compute call needs a GPU here."""
+import re
+import pathlib
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import pdmk4mc_amd as P
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+
+
+def test_library_exports_every_header_symbol():
+    import ctypes
+    lib = ctypes.CDLL(str(P.lib_path()))
+    header = (ROOT / "include" / "hpdmk.h").read_text()
+    names = set(re.findall(r"^\w[\w\s\*]*?\b(hpdmk_\w+)\s*\(", header, re.M))
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    assert names <= set(P.hpdmk.SIGNATURES), names - set(P.hpdmk.SIGNATURES)
+
+
+def test_default_params_match_reference():
+    p = P.load_library().hpdmk_default_params()
+    assert (p.n_per_leaf, p.digits, p.prolate_order, p.init) == (200, 3, 16.0, P.PROXY)
+
+
+@pytest.mark.parametrize("digits", [3, 6, 9, 12])
+def test_pswf_constants_match_oracle(digits):
+    a = P.pswf_constants(digits)
+    b = O.pswf_params(digits)
+    assert abs(a["lam"] - b["lam"]) < 1e-13
+    assert abs(a["diff0"] - b["diff0"]) < 1e-11 * abs(b["diff0"])
+    # psi0 normalisation is free; the method only uses psi0(0)/C0
+    assert abs(a["psi0_0"] / a["C0"] - b["psi0_0"] / b["C0"]) < 1e-13
+    assert (a["n_window"], a["n_diff"]) == (b["n_window"], b["n_diff"])
+
+
+def test_bad_digits_rejected():
+    with pytest.raises(P.HPDMKError):
+        P.pswf_constants(5)
+
+
+@pytest.mark.parametrize("n,L,leaf", [(1000, 20.0, 10), (1000, 20.0, 64), (1100, 20.0, 5), (20000, 58.5, 200)])
+def test_octree_identical_to_oracle(n, L, leaf):
+    r, q = O.reference_test_system(n, L)
+    t = O.OracleTree(r, q, L, n_per_leaf=leaf, digits=3)
+    on, ol = t.nodes(), t.particle_leaf()
+    pn, pl = P.plan_tree(P.HPDMKParams(leaf, 3, L), r)
+    assert on.shape == pn.shape
+    assert (on[:, :8] == pn[:, :8]).all()
+    assert (ol == pl).all()
+    # near list = self + leaves under colleagues + coarse-grain leaves
+    for i in np.where(pn[:, 1] == 1)[0][:50]:
+        assert pn[i, 8] >= 1 + on[i, 8]
+
+
+def test_plan_rejects_shallow_tree_and_bad_coordinates():
+    r, q = O.reference_test_system(50, 20.0)
+    with pytest.raises(P.HPDMKError, match="max depth"):
+        P.plan_tree(P.HPDMKParams(200, 3, 20.0), r)
+    r[0, 0] = 25.0
+    with pytest.raises(P.HPDMKError, match=r"\[0, L\)"):
+        P.plan_tree(P.HPDMKParams(5, 3, 20.0), r)
