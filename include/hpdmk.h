@@ -113,15 +113,15 @@ int hpdmk_mc_load(hpdmk_tree tree, int n_moves, const long* idx, const double* d
 int hpdmk_mc_run(hpdmk_tree tree, double beta, int mode, int batch, double* gpu_ms);
 int hpdmk_mc_results(hpdmk_tree tree, double* dE, int* accept);
 
-/* Per-move counts of the last run (6 ints per move): difference levels read
+/* Per-move counts of the last profiled run (6 ints per move): difference levels read
  * at the target and origin, near-field particles scanned at the target and
  * origin, expansion arrays an acceptance rewrites for the target and origin
  * paths.  They size the algorithmic bytes of each move. */
 int hpdmk_mc_stats(hpdmk_tree tree, int* stats);
 
 /* Per-kernel GPU time of the last hpdmk_mc_run with profiling on (events
- * around every launch).  kinds: 0 trial, 1 update, 2 pair corrections,
- * 3 decision scan.  ms[4], count[4]. */
+ * around every launch).  kinds: 0 trial, 1 update (apply), 2 pair
+ * corrections, 3 decision scan, 4 trial set-up.  ms[5], count[5]. */
 int hpdmk_set_profiling(hpdmk_tree tree, int on);
 int hpdmk_kernel_times(hpdmk_tree tree, double* ms, int* count);
 

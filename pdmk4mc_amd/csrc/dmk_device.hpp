@@ -1,0 +1,160 @@
+// Device helpers shared by the build and Monte Carlo kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <limits>
+
+#include "dmk_common.hpp"
+#include "dmk_kernels.hpp"
+
+namespace hpdmk_amd {
+
+constexpr int TB = 256;   // threads per block (4 waves of 64)
+constexpr int kRow = 136; // longest 1-D phase row set: 2(2n+1) + n + 1 for n <= 26
+
+// ------------------------------------------------------------------ helpers
+template <typename T>
+__device__ __forceinline__ Cplx<T> cmul(Cplx<T> a, Cplx<T> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// block-wide FP64 sum; result valid in thread 0
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += red[i];
+    __syncthreads();
+    return s;
+}
+
+// my_mod of include/utils.hpp:96-105 (repeated add/subtract, bounded)
+__device__ __forceinline__ double wrap(double x, double L) {
+    int guard = 0;
+    while (x < 0 && guard++ < 1024) x += L;
+    while (x >= L && guard++ < 2048) x -= L;
+    if (x < 0 || x >= L) {
+        x = fmod(x, L);
+        if (x < 0) x += L;
+        if (x >= L) x -= L;
+    }
+    return x;
+}
+
+__device__ __forceinline__ int cell_at(double x, double L, int l) {
+    const int n = 1 << l;
+    long c = (long)floor(x / L * (double)n);
+    c = c < 0 ? 0 : (c > n - 1 ? n - 1 : c);
+    return (int)c;
+}
+
+// descend from the root to the leaf containing (x,y,z); path[l] = node at level l
+template <typename T>
+__device__ int locate(const DevTree<T>& t, double x, double y, double z, int* path) {
+    int nd = 0, l = 0;
+    path[0] = 0;
+    while (!t.is_leaf[nd]) {
+        const int l1 = l + 1;
+        const int bits = (cell_at(x, t.L, l1) & 1) | ((cell_at(y, t.L, l1) & 1) << 1) | ((cell_at(z, t.L, l1) & 1) << 2);
+        nd = t.child[8 * nd + bits];
+        path[++l] = nd;
+    }
+    return l;  // leaf level
+}
+
+// residual kernel R_b(r) = P(2r/b - 1)/r for 0 < r < b (include/kernels.hpp:51-58)
+template <typename R>
+__device__ __forceinline__ R residual_poly_over_r(R r, R inv_b, const double* poly, int n_poly) {
+    const R u = (r * inv_b - R(0.5)) * R(2);
+    if (u >= R(1) || u <= R(-1)) return R(0);
+    R v = R(poly[n_poly - 1]);
+    for (int k = n_poly - 2; k >= 0; --k) v = v * u + R(poly[k]);
+    return v / r;
+}
+
+// 1-D phase rows e^{-i (m-n) dk c} (FP64 argument, stored as T); charge folded into X
+template <typename T>
+__device__ void phase_rows(Cplx<T>* X, Cplx<T>* Y, Cplx<T>* Z, int n, double dk, double x, double y, double z,
+                           double q, int tid, int nthr) {
+    const int d = 2 * n + 1;
+    for (int t = tid; t < 2 * d + n + 1; t += nthr) {
+        double c, m;
+        double scale = 1.0;
+        Cplx<T>* dst;
+        if (t < d) {
+            c = x;
+            m = t - n;
+            dst = X + t;
+            scale = q;
+        } else if (t < 2 * d) {
+            c = y;
+            m = t - d - n;
+            dst = Y + (t - d);
+        } else {
+            c = z;
+            m = t - 2 * d - n;
+            dst = Z + (t - 2 * d);
+        }
+        double s, co;
+        sincos(m * (dk * c), &s, &co);
+        dst->x = (T)(co * scale);
+        dst->y = (T)(-s * scale);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ Cplx<T> phase_at(const Cplx<T>* X, const Cplx<T>* Y, const Cplx<T>* Z, int d, int e) {
+    const int i = e % d, r = e / d, j = r % d, k = r / d;
+    return cmul(cmul(X[i], Y[j]), Z[k]);
+}
+
+// ------------------------------------------------------------------ trial set-up
+template <typename T>
+__device__ void trial_setup(const DevTree<T>& t, int i, const double* d, TrialInfo& ti) {
+    const int s = t.slot_of[i];
+    ti.i = i;
+    ti.q = t.pq[s];
+    ti.o[0] = t.px[s];
+    ti.o[1] = t.py[s];
+    ti.o[2] = t.pz[s];
+    for (int a = 0; a < 3; ++a) {
+        ti.dv[a] = d[a];
+        ti.t[a] = wrap(ti.o[a] + d[a], t.L);
+    }
+    ti.lam_o = locate(t, ti.o[0], ti.o[1], ti.o[2], ti.path_o);
+    ti.lam_t = locate(t, ti.t[0], ti.t[1], ti.t[2], ti.path_t);
+}
+
+// self-pair correction q^2 R_{b_2}(d_min) of src/energy_shift.cpp:33-41, :58
+template <typename T>
+__device__ double self_term(const DevTree<T>& t, const TrialInfo& ti) {
+    double dr = sqrt(ti.dv[0] * ti.dv[0] + ti.dv[1] * ti.dv[1] + ti.dv[2] * ti.dv[2]);
+    for (int mx = -1; mx <= 1; ++mx)
+        for (int my = -1; my <= 1; ++my)
+            for (int mz = -1; mz <= 1; ++mz) {
+                const double ax = ti.t[0] - (ti.o[0] + mx * t.L), ay = ti.t[1] - (ti.o[1] + my * t.L),
+                             az = ti.t[2] - (ti.o[2] + mz * t.L);
+                dr = fmin(dr, sqrt(ax * ax + ay * ay + az * az));
+            }
+    if (dr == 0) return 0.0;
+    const double b2 = t.boxsize[2];
+    const double u = (dr / b2 - 0.5) * 2.0;
+    if (fabs(u) >= 1.0) return 0.0;
+    double v = t.poly[t.n_poly - 1];
+    for (int k = t.n_poly - 2; k >= 0; --k) v = v * u + t.poly[k];
+    return v / dr * ti.q * ti.q;
+}
+
+}  // namespace hpdmk_amd

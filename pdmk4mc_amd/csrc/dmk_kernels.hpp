@@ -1,4 +1,4 @@
-// Kernel argument blocks and launchers of dmk_kernels.hip.
+// Kernel argument blocks and launchers of dmk_build.hip and dmk_mc.hip.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -7,7 +7,10 @@
 
 namespace hpdmk_amd {
 
-constexpr int kMaxBatch = 256;  // trials per batched step
+constexpr int kMaxBatch = 256;     // trials per batched step
+constexpr int kRowStride = 64;     // stride of one 1-D phase row (2n+1 <= 53)
+constexpr int kAccPerTrial = 6;    // window, difference, res_t, res_o, scanned_t, scanned_o
+constexpr int kStatsPerMove = 6;   // levels t/o, near particles t/o, rewritten arrays t/o
 
 // Everything later kernels need about one trial move: endpoints, charge and
 // the root-to-leaf paths of the origin o and the target t = wrap(o + d).
@@ -19,55 +22,47 @@ struct TrialInfo {
     int path_o[kMaxLevels], path_t[kMaxLevels];
 };
 
+// Phase rows of one trial (Cplx<T>, kRowStride apart):
+//   root:  [endpoint][axis]                     e^{-i (m-n_w) dk_0 x_a}
+//   level: [endpoint][level-2][axis][shift+1]    e^{-i (m-n_d) dk_l (x_a + shift L)}
+// endpoint 0 = target, 1 = origin; the x-row carries the charge.
+__host__ __device__ inline int table_stride(int max_depth) {
+    const int nL = max_depth >= 4 ? max_depth - 3 : 0;
+    return (6 + 2 * nL * 9) * kRowStride;
+}
+__host__ __device__ inline int root_row(int ep, int axis) { return (ep * 3 + axis) * kRowStride; }
+__host__ __device__ inline int level_row(int nL, int ep, int li, int axis, int shift) {
+    return (6 + ((ep * nL + li) * 3 + axis) * 3 + shift + 1) * kRowStride;
+}
+
 enum TrialMode : int {
     kTrialOnly = 0,         // dE only
-    kTrialMetropolis = 1,   // dE and accept = u < exp(-beta dE)
-    kTrialForceAccept = 2,  // dE and accept (update_shift after eval_shift_energy)
+    kTrialMetropolis = 1,   // dE, accept = u < exp(-beta dE), update plan
+    kTrialForceAccept = 2,  // dE, accept, update plan (update_shift)
 };
 
-struct TrialArgs {
-    const int* idx;     // caller's particle indices (move list)
+// device scratch of the Monte Carlo kernels (sized for kMaxBatch trials)
+struct McScratch {
+    TrialInfo* info;
+    void* tabs;            // Cplx<T>, table_stride(D) per trial
+    double* acc;           // kAccPerTrial per trial (self-resetting)
+    unsigned* done;        // block counter per trial (self-resetting)
+    int* accept;           // per trial of the current launch
+    double* corr;          // kMaxBatch^2
+    unsigned char* mark;   // 2 per expansion box (self-resetting)
+    int* dest;             // compact list of rewritten arrays: 2*slot + kind (0 out, 1 in)
+    int* plan;             // [0] arrays in dest, [1] any move accepted
+};
+
+struct MoveList {
+    const int* idx;     // caller's particle indices
     const double* d;    // displacements, 3 per move
-    const double* u;    // uniforms for the Metropolis test
-    double beta;
-    int move0;          // first move of this launch in the move list
-    int mode;
-    double* acc;        // kAccPerTrial FP64 accumulators per trial (self-resetting)
-    unsigned* done;     // block counter per trial (self-resetting)
-    double* dE;         // per move
+    const double* u;    // Metropolis uniforms
+    double* dE;         // final dE per move
+    double* dE_stale;   // batched: dE against the batch's starting state
     double* parts;      // optional: window, diff, res_t, res_o, self per move
-    int* accept;        // per trial of this launch (read by the update)
     int* accept_out;    // optional: per move
-    TrialInfo* info;    // per trial of this launch
-    int* stats;         // optional, per move: difference levels (t, o), near-field
-                        // particles scanned (t, o), arrays an accept rewrites (t, o)
-};
-
-constexpr int kAccPerTrial = 6;  // window, difference, res_t, res_o, scanned_t, scanned_o
-constexpr int kStatsPerMove = 6;
-
-struct UpdateArgs {
-    int n_moves;
-    const int* accept;
-    const TrialInfo* info;
-};
-
-struct CorrArgs {
-    int K;
-    const TrialInfo* info;
-    double* corr;  // K*K, corr[i*K + j] = effect of move i on trial j
-};
-
-struct ScanArgs {
-    int K, move0;
-    const TrialInfo* info;
-    const double* corr;
-    const double* dE_stale;  // per move
-    const double* u;
-    double beta;
-    double* dE;              // per move (final)
-    int* accept;             // per trial of the batch
-    int* accept_out;         // optional, per move
+    int* stats;         // optional: kStatsPerMove per move
 };
 
 // boxes of levels 2..D-2 in pw-slot order
@@ -78,14 +73,27 @@ struct PwNodes {
     const int* hi;
 };
 
+struct McArgs {
+    McScratch s;
+    MoveList ml;
+    PwNodes pn;
+    int move0;   // first move of this launch in the move list
+    int K;       // trials in this launch
+    int mode;    // TrialMode
+    double beta;
+};
+
 template <typename T>
-void launch_trial(const DevTree<T>& t, const TrialArgs& a, int n_trials, hipStream_t st);
+void launch_setup(const DevTree<T>& t, const McArgs& a, hipStream_t st);
 template <typename T>
-void launch_update(const DevTree<T>& t, const UpdateArgs& a, hipStream_t st);
+void launch_trial(const DevTree<T>& t, const McArgs& a, hipStream_t st);
 template <typename T>
-void launch_pair_corr(const DevTree<T>& t, const CorrArgs& a, hipStream_t st);
+void launch_pair_corr(const DevTree<T>& t, const McArgs& a, hipStream_t st);
 template <typename T>
-void launch_scan(const DevTree<T>& t, const ScanArgs& a, hipStream_t st);
+void launch_scan(const DevTree<T>& t, const McArgs& a, hipStream_t st);
+template <typename T>
+void launch_apply(const DevTree<T>& t, const McArgs& a, hipStream_t st);
+
 template <typename T>
 void launch_outgoing(const DevTree<T>& t, const PwNodes& pn, double* acc_root, hipStream_t st);
 template <typename T>

@@ -56,7 +56,7 @@ struct MoveBuf {
     int* stats = nullptr;
 };
 
-enum KernelKind { kKTrial = 0, kKUpdate = 1, kKCorr = 2, kKScan = 3, kKinds = 4 };
+enum KernelKind { kKTrial = 0, kKUpdate = 1, kKCorr = 2, kKScan = 3, kKSetup = 4, kKinds = 5 };
 
 struct Handle {
     HPDMKParams params{};
@@ -74,18 +74,16 @@ struct Handle {
     double* acc_root = nullptr;
     double* energy = nullptr;
     int* err = nullptr;
-    // trial scratch (kMaxBatch trials)
+    // Monte Carlo scratch (kMaxBatch trials)
+    McScratch sc{};
     double* t_acc = nullptr;
     unsigned* t_done = nullptr;
-    TrialInfo* t_info = nullptr;
-    int* t_accept = nullptr;
-    double* corr = nullptr;
     // move lists in HBM: `single` serves eval/update calls, `chain` the staged Metropolis run
     MoveBuf single, chain;
     // profiling
     bool profiling = false;
-    double kms[kKinds] = {0, 0, 0, 0};
-    int kcount[kKinds] = {0, 0, 0, 0};
+    double kms[kKinds] = {0, 0, 0, 0, 0};
+    int kcount[kKinds] = {0, 0, 0, 0, 0};
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
 
@@ -393,9 +391,18 @@ int hpdmk_tree_create_ex(HPDMKParams params, int precision, int n_src, const dou
         h->energy = dalloc<double>(4, o);
         h->t_acc = dalloc<double>(kAccPerTrial * kMaxBatch, o);
         h->t_done = dalloc<unsigned>(kMaxBatch, o);
-        h->t_info = dalloc<TrialInfo>(kMaxBatch, o);
-        h->t_accept = dalloc<int>(kMaxBatch, o);
-        h->corr = dalloc<double>(kMaxBatch * kMaxBatch, o);
+        h->sc.acc = h->t_acc;
+        h->sc.done = h->t_done;
+        h->sc.info = dalloc<TrialInfo>(kMaxBatch, o);
+        h->sc.accept = dalloc<int>(kMaxBatch, o);
+        h->sc.corr = dalloc<double>(kMaxBatch * kMaxBatch, o);
+        const size_t tsz = precision == HPDMK_FP64 ? sizeof(Cplx<double>) : sizeof(Cplx<float>);
+        h->sc.tabs = dalloc<char>(tsz * (size_t)table_stride(D) * kMaxBatch, o);
+        h->sc.mark = dalloc<unsigned char>(2 * (size_t)h->n_pw + 2, o);
+        h->sc.dest = dalloc<int>(2 * (size_t)h->n_pw + 2, o);
+        h->sc.plan = dalloc<int>(2, o);
+        HIPX(hipMemset(h->sc.mark, 0, 2 * (size_t)h->n_pw + 2));
+        HIPX(hipMemset(h->sc.plan, 0, 2 * sizeof(int)));
         HIPX(hipMemset(h->t_acc, 0, sizeof(double) * kAccPerTrial * kMaxBatch));
         HIPX(hipMemset(h->t_done, 0, sizeof(unsigned) * kMaxBatch));
         HIPX(hipDeviceSynchronize());
@@ -497,22 +504,22 @@ void upload_moves(Handle* h, MoveBuf& b, int n, const long* idx, const double* d
     b.n = n;
 }
 
-TrialArgs trial_args(Handle* h, const MoveBuf& b, int move0, int mode, double beta, double* dE) {
-    TrialArgs a{};
-    a.idx = b.idx;
-    a.d = b.d;
-    a.u = b.u;
-    a.beta = beta;
+McArgs mc_args(Handle* h, const MoveBuf& b, int move0, int K, int mode, double beta) {
+    McArgs a{};
+    a.s = h->sc;
+    a.ml.idx = b.idx;
+    a.ml.d = b.d;
+    a.ml.u = b.u;
+    a.ml.dE = b.dE;
+    a.ml.dE_stale = b.dE_stale;
+    a.ml.parts = b.parts;
+    a.ml.accept_out = b.acc;
+    a.ml.stats = h->profiling ? b.stats : nullptr;  // counting runs in the finalizer: profiled runs only
+    a.pn = h->pn;
     a.move0 = move0;
+    a.K = K;
     a.mode = mode;
-    a.acc = h->t_acc;
-    a.done = h->t_done;
-    a.dE = dE;
-    a.parts = b.parts;
-    a.accept = h->t_accept;
-    a.accept_out = b.acc;
-    a.info = h->t_info;
-    a.stats = b.stats;
+    a.beta = beta;
     return a;
 }
 }  // namespace
@@ -523,7 +530,9 @@ int hpdmk_eval_shift_energy(hpdmk_tree t, long i, double dx, double dy, double d
         const double d[3] = {dx, dy, dz};
         upload_moves(h, h->single, 1, &i, d, nullptr);
         dispatch(h, [&](auto& dv) {
-            launch_trial(dv, trial_args(h, h->single, 0, kTrialOnly, 0.0, h->single.dE), 1, h->stream);
+            const McArgs a = mc_args(h, h->single, 0, 1, kTrialOnly, 0.0);
+            launch_setup(dv, a, h->stream);
+            launch_trial(dv, a, h->stream);
         });
         HIPX(hipGetLastError());
         double p[5], e;
@@ -541,9 +550,10 @@ int hpdmk_update_shift(hpdmk_tree t, long i, double dx, double dy, double dz) {
         const double d[3] = {dx, dy, dz};
         upload_moves(h, h->single, 1, &i, d, nullptr);
         dispatch(h, [&](auto& dv) {
-            launch_trial(dv, trial_args(h, h->single, 0, kTrialForceAccept, 0.0, h->single.dE), 1, h->stream);
-            UpdateArgs ua{1, h->t_accept, h->t_info};
-            launch_update(dv, ua, h->stream);
+            const McArgs a = mc_args(h, h->single, 0, 1, kTrialForceAccept, 0.0);
+            launch_setup(dv, a, h->stream);
+            launch_trial(dv, a, h->stream);
+            launch_apply(dv, a, h->stream);
         });
         HIPX(hipGetLastError());
         check_err(h);
@@ -580,7 +590,9 @@ int hpdmk_eval_shift_energy_batch(hpdmk_tree t, int n, const long* idx, const do
         for (int m0 = 0; m0 < n; m0 += kMaxBatch) {
             const int k = std::min(kMaxBatch, n - m0);
             dispatch(h, [&](auto& dv) {
-                launch_trial(dv, trial_args(h, h->single, m0, kTrialOnly, 0.0, h->single.dE), k, h->stream);
+                const McArgs a = mc_args(h, h->single, m0, k, kTrialOnly, 0.0);
+                launch_setup(dv, a, h->stream);
+                launch_trial(dv, a, h->stream);
             });
         }
         HIPX(hipGetLastError());
@@ -631,33 +643,20 @@ int hpdmk_mc_run(hpdmk_tree t, double beta, int mode, int batch, double* gpu_ms)
         dispatch(h, [&](auto& dv) {
             if (mode == 0) {
                 for (int m = 0; m < n; ++m) {
-                    timed(h, kKTrial, [&] {
-                        launch_trial(dv, trial_args(h, mb, m, kTrialMetropolis, beta, mb.dE), 1, h->stream);
-                    });
-                    timed(h, kKUpdate, [&] {
-                        UpdateArgs ua{1, h->t_accept, h->t_info};
-                        launch_update(dv, ua, h->stream);
-                    });
+                    const McArgs a = mc_args(h, mb, m, 1, kTrialMetropolis, beta);
+                    timed(h, kKSetup, [&] { launch_setup(dv, a, h->stream); });
+                    timed(h, kKTrial, [&] { launch_trial(dv, a, h->stream); });
+                    timed(h, kKUpdate, [&] { launch_apply(dv, a, h->stream); });
                 }
             } else {
-                for (auto& b : batches) {
-                    const int m0 = b.first, k = b.second;
-                    timed(h, kKTrial, [&] {
-                        launch_trial(dv, trial_args(h, mb, m0, kTrialOnly, beta, mb.dE_stale), k, h->stream);
-                    });
-                    if (k > 1)
-                        timed(h, kKCorr, [&] {
-                            CorrArgs ca{k, h->t_info, h->corr};
-                            launch_pair_corr(dv, ca, h->stream);
-                        });
-                    timed(h, kKScan, [&] {
-                        ScanArgs sa{k, m0, h->t_info, h->corr, mb.dE_stale, mb.u, beta, mb.dE, h->t_accept, mb.acc};
-                        launch_scan(dv, sa, h->stream);
-                    });
-                    timed(h, kKUpdate, [&] {
-                        UpdateArgs ua{k, h->t_accept, h->t_info};
-                        launch_update(dv, ua, h->stream);
-                    });
+                for (auto& bt : batches) {
+                    const int m0 = bt.first, k = bt.second;
+                    const McArgs a = mc_args(h, mb, m0, k, kTrialOnly, beta);
+                    timed(h, kKSetup, [&] { launch_setup(dv, a, h->stream); });
+                    timed(h, kKTrial, [&] { launch_trial(dv, a, h->stream); });
+                    if (k > 1) timed(h, kKCorr, [&] { launch_pair_corr(dv, a, h->stream); });
+                    timed(h, kKScan, [&] { launch_scan(dv, a, h->stream); });
+                    timed(h, kKUpdate, [&] { launch_apply(dv, a, h->stream); });
                 }
             }
         });
