@@ -39,13 +39,24 @@ enum TrialMode : int {
     kTrialOnly = 0,         // dE only
     kTrialMetropolis = 1,   // dE, accept = u < exp(-beta dE), update plan
     kTrialForceAccept = 2,  // dE, accept, update plan (update_shift)
+    kTrialBatch = 3,        // partial sums only; the batch scan assembles dE
 };
+
+// blocks per trial in k_trial: window, difference (per endpoint and level),
+// near field (per endpoint); each block strides over its share of the work
+constexpr int kWinBlocks = 2;
+constexpr int kDiffBlocks = 8;
+constexpr int kNearBlocks = 8;
+__host__ __device__ inline int trial_blocks(int max_depth) {
+    const int nL = max_depth >= 4 ? max_depth - 3 : 0;
+    return kWinBlocks + 2 * nL * kDiffBlocks + 2 * kNearBlocks;
+}
 
 // device scratch of the Monte Carlo kernels (sized for kMaxBatch trials)
 struct McScratch {
     TrialInfo* info;
     void* tabs;            // Cplx<T>, table_stride(D) per trial
-    double* acc;           // kAccPerTrial per trial (self-resetting)
+    double* part;          // per trial and block: partial sum, particles scanned
     unsigned* done;        // block counter per trial (self-resetting)
     int* accept;           // per trial of the current launch
     double* corr;          // kMaxBatch^2

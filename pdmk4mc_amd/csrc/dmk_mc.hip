@@ -190,10 +190,72 @@ __device__ int capacity_ok(const DevTree<T>& t, const TrialInfo& ti, int delta) 
 }
 
 // ------------------------------------------------------------------ trial
-// grid (blocks per trial, K).  Block roles inside a trial:
-//   [0, nWb)                          window chunk
-//   [nWb, nWb + 2*nL*nDb)             difference chunk (endpoint, level, chunk)
-//   [.., + 2*max_near)                near-field entry (endpoint, entry)
+// Sums the per-block partials of trial tr into v: window, difference,
+// res_t, res_o, particles scanned at t and o.  Loads are device-coherent.
+template <typename T>
+__device__ void sum_trial_partials(const DevTree<T>& t, const McArgs& a, int tr, double* v) {
+    const int D = t.max_depth, nL = n_levels_pw(D), per = trial_blocks(D);
+    for (int k = 0; k < kAccPerTrial; ++k) v[k] = 0;
+    const double* p = a.s.part + 2 * (size_t)tr * per;
+    for (int b = 0; b < per; ++b) {
+        int slot;
+        if (b < kWinBlocks)
+            slot = 0;
+        else if (b < kWinBlocks + 2 * nL * kDiffBlocks)
+            slot = 1;
+        else
+            slot = 2 + (b - kWinBlocks - 2 * nL * kDiffBlocks) / kNearBlocks;
+        v[slot] += __hip_atomic_load(p + 2 * b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (slot >= 2) v[slot + 2] += __hip_atomic_load(p + 2 * b + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// dE of trial tr from its partials (src/energy_shift.cpp:67) plus the optional
+// per-move parts and counts
+template <typename T>
+__device__ double assemble_trial(const DevTree<T>& t, const McArgs& a, int tr, int m) {
+    const TrialInfo& ti = a.s.info[tr];
+    double v[kAccPerTrial];
+    sum_trial_partials(t, a, tr, v);
+    const double self = self_term(t, ti);
+    const double dE = v[0] + v[1] + v[2] - v[3] - self;
+    if (a.ml.parts) {
+        double* p = a.ml.parts + 5 * m;
+        p[0] = v[0];
+        p[1] = v[1];
+        p[2] = v[2];
+        p[3] = v[3];
+        p[4] = self;
+    }
+    if (a.ml.stats) {
+        int* st = a.ml.stats + kStatsPerMove * m;
+        st[0] = ti.lam_t - 2 > 0 ? ti.lam_t - 2 : 0;
+        st[1] = ti.lam_o - 2 > 0 ? ti.lam_o - 2 : 0;
+        st[2] = (int)v[4];
+        st[3] = (int)v[5];
+        for (int ep = 0; ep < 2; ++ep) {  // arrays an acceptance rewrites (src/tree.cpp:484-579)
+            const int lam = ep == 0 ? ti.lam_t : ti.lam_o;
+            int arrays = 0;
+            for (int l = 2; l <= lam && l <= t.max_depth - 2; ++l) {
+                const int node = ep == 0 ? ti.path_t[l] : ti.path_o[l];
+                arrays += t.is_leaf[node] ? 1 : 2;
+                for (int c = 0; c < 26; ++c) {
+                    const int C = t.colleague[26 * node + c];
+                    arrays += (C >= 0 && !t.is_leaf[C]);
+                }
+            }
+            st[4 + ep] = arrays;
+        }
+    }
+    return dE;
+}
+
+// grid (trial_blocks(D), K).  Block roles inside a trial (each strides over its share):
+//   [0, kWinBlocks)                        window
+//   [.., + 2*nL*kDiffBlocks)               difference (endpoint, level, share)
+//   [.., + 2*kNearBlocks)                  near field (endpoint, share of the near list)
+// Every block stores its partial sum; in single-move modes the last block of
+// the trial assembles dE, decides and plans the update.
 template <typename T>
 __global__ void __launch_bounds__(TB) k_trial(DevTree<T> t, McArgs a) {
     __shared__ double red[TB / 64];
@@ -201,62 +263,63 @@ __global__ void __launch_bounds__(TB) k_trial(DevTree<T> t, McArgs a) {
     if (*t.err) return;
     const int tr = blockIdx.y, b = blockIdx.x, m = a.move0 + tr;
     const TrialInfo& ti = a.s.info[tr];
-    const int D = t.max_depth, nL = n_levels_pw(D);
+    const int D = t.max_depth, nL = n_levels_pw(D), per = trial_blocks(D);
     const Cplx<T>* tb = tabs_of<T>(a.s, table_stride(D), tr);
-    const int nWb = (t.Mw + TB - 1) / TB;
-    const int nDb = (t.Md + TB - 1) / TB;
     double part = 0.0;
-    int slot_acc = 0, nscan = 0;
-    if (b < nWb) {
+    int nscan = 0;
+    if (b < kWinBlocks) {
         // window: 2 cW sum W Re[(tau - omega) conj(rho - omega)] (include/vecops.hpp:118-145)
-        const int e = b * TB + threadIdx.x;
-        if (e < t.Mw) {
-            const int dw = t.dw;
+        const int dw = t.dw;
+        for (int e = b * TB + threadIdx.x; e < t.Mw; e += kWinBlocks * TB) {
             const Cplx<T> ta = phase_at(tb + root_row(0, 0), tb + root_row(0, 1), tb + root_row(0, 2), dw, e);
             const Cplx<T> om = phase_at(tb + root_row(1, 0), tb + root_row(1, 1), tb + root_row(1, 2), dw, e);
             const Cplx<T> g = t.root[e];
             const T ar = ta.x - om.x, ai = ta.y - om.y, gr = g.x - om.x, gi = g.y - om.y;
-            part = (double)((ar * gr + ai * gi) * t.Wmat[e]) * (2.0 * t.cW);
+            part += (double)((ar * gr + ai * gi) * t.Wmat[e]);
         }
-    } else if (b < nWb + 2 * nL * nDb) {
+        part *= 2.0 * t.cW;
+    } else if (b < kWinBlocks + 2 * nL * kDiffBlocks) {
         // difference levels strictly above the endpoint's leaf (src/energy_shift.cpp:92-140)
-        const int r = b - nWb;
-        const int ep = r / (nL * nDb);  // 0 target, 1 origin
-        const int li = (r / nDb) % nL;
+        const int r = b - kWinBlocks;
+        const int ep = r / (nL * kDiffBlocks);  // 0 target, 1 origin
+        const int li = (r / kDiffBlocks) % nL;
         const int l = 2 + li;
-        const int e = (r % nDb) * TB + threadIdx.x;
         const int lam = ep == 0 ? ti.lam_t : ti.lam_o;
-        slot_acc = 1;
-        if (l <= lam - 1 && e < t.Md) {
+        if (l <= lam - 1) {
             const int node = ep == 0 ? ti.path_t[l] : ti.path_o[l];
-            const Cplx<T> v = t.inc[(size_t)t.pw_slot[node] * t.Md + e];
-            const Cplx<T> w = phase_at(tb + level_row(nL, ep, li, 0, 0), tb + level_row(nL, ep, li, 1, 0),
-                                       tb + level_row(nL, ep, li, 2, 0), t.dd, e);
-            T re = w.x * v.x - w.y * v.y;
-            if (ep == 1) re -= w.x * w.x + w.y * w.y;  // origin drops its own wave (vecops.hpp:161-163)
-            part = (double)(re * t.Dmat[(size_t)li * t.Md + e]) * (2.0 * t.cD[l] * (ep == 0 ? 1.0 : -1.0));
+            const Cplx<T>* in = t.inc + (size_t)t.pw_slot[node] * t.Md;
+            const T* Dl = t.Dmat + (size_t)li * t.Md;
+            const Cplx<T>* X = tb + level_row(nL, ep, li, 0, 0);
+            const Cplx<T>* Y = tb + level_row(nL, ep, li, 1, 0);
+            const Cplx<T>* Z = tb + level_row(nL, ep, li, 2, 0);
+            for (int e = (r % kDiffBlocks) * TB + threadIdx.x; e < t.Md; e += kDiffBlocks * TB) {
+                const Cplx<T> v = in[e];
+                const Cplx<T> w = phase_at(X, Y, Z, t.dd, e);
+                T re = w.x * v.x - w.y * v.y;
+                if (ep == 1) re -= w.x * w.x + w.y * w.y;  // origin drops its own wave (vecops.hpp:161-163)
+                part += (double)(re * Dl[e]);
+            }
+            part *= 2.0 * t.cD[l] * (ep == 0 ? 1.0 : -1.0);
         }
     } else {
         // near field of the endpoint's leaf (src/energy_shift.cpp:233-302, direct_eval.hpp:309-403)
-        const int r = b - nWb - 2 * nL * nDb;
-        const int ep = r / t.max_near;
-        const int k = r % t.max_near;
+        const int r = b - kWinBlocks - 2 * nL * kDiffBlocks;
+        const int ep = r / kNearBlocks;
         const int lam = ep == 0 ? ti.lam_t : ti.lam_o;
         const int leaf = ep == 0 ? ti.path_t[lam] : ti.path_o[lam];
-        slot_acc = 2 + ep;
         const int off = t.near_off[leaf], cnt = t.near_off[leaf + 1] - off;
-        if (k < cnt) {
+        const double cx = ep == 0 ? ti.t[0] : ti.o[0], cy = ep == 0 ? ti.t[1] : ti.o[1],
+                     cz = ep == 0 ? ti.t[2] : ti.o[2];
+        const double bb = t.boxsize[lam];
+        const T b2 = (T)(bb * bb), inv_b = (T)(1.0 / bb);
+        const T eps = std::numeric_limits<T>::epsilon();
+        double acc = 0;
+        for (int k = r % kNearBlocks; k < cnt; k += kNearBlocks) {
             const int nb = t.near_ids[off + k];
             const int code = t.near_img[off + k];
             const double sx = (code % 3 - 1) * t.L, sy = ((code / 3) % 3 - 1) * t.L, sz = (code / 9 - 1) * t.L;
-            const double cx = ep == 0 ? ti.t[0] : ti.o[0], cy = ep == 0 ? ti.t[1] : ti.o[1],
-                         cz = ep == 0 ? ti.t[2] : ti.o[2];
-            const double bb = t.boxsize[lam];
-            const T b2 = (T)(bb * bb), inv_b = (T)(1.0 / bb);
-            const T eps = std::numeric_limits<T>::epsilon();
             const int s0 = t.slot_start[nb], n = t.slot_count[nb];
-            nscan = n;
-            double acc = 0;
+            nscan += n;
             for (int p = threadIdx.x; p < n; p += blockDim.x) {
                 const int s = s0 + p;
                 const T dx = (T)(cx - (t.px[s] + sx)), dy = (T)(cy - (t.py[s] + sy)), dz = (T)(cz - (t.pz[s] + sz));
@@ -264,66 +327,35 @@ __global__ void __launch_bounds__(TB) k_trial(DevTree<T> t, McArgs a) {
                 if (r2 > eps && r2 < b2)
                     acc += (double)((T)t.pq[s] * residual_poly_over_r<T>(sqrt(r2), inv_b, t.poly, t.n_poly));
             }
-            part = acc * ti.q;
         }
+        part = acc * ti.q;
     }
     const double s = block_sum(part, red);
-    double* acc = a.s.acc + kAccPerTrial * tr;
     if (threadIdx.x == 0) {
-        if (s != 0.0) atomicAdd(acc + slot_acc, s);
-        if (nscan) atomicAdd(acc + 2 + slot_acc, (double)nscan);
+        double* p = a.s.part + 2 * ((size_t)tr * per + b);
+        __hip_atomic_store(p, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(p + 1, (double)nscan, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (a.mode == kTrialBatch) return;
+    if (threadIdx.x == 0) {
         __threadfence();
         const unsigned prev = atomicAdd(a.s.done + tr, 1u);
-        last = (prev == gridDim.x - 1);
+        last = (prev == (unsigned)per - 1);
     }
     __syncthreads();
     if (!last) return;
     if (threadIdx.x == 0) {
         __threadfence();
-        double v[kAccPerTrial];
-        for (int k = 0; k < kAccPerTrial; ++k)
-            v[k] = __hip_atomic_load(acc + k, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        const double self = self_term(t, ti);
-        const double dE = v[0] + v[1] + v[2] - v[3] - self;
-        double* dEout = a.mode == kTrialOnly ? a.ml.dE_stale : a.ml.dE;
-        dEout[m] = dE;
-        if (a.mode == kTrialOnly && a.ml.dE) a.ml.dE[m] = dE;
-        if (a.ml.parts) {
-            double* p = a.ml.parts + 5 * m;
-            p[0] = v[0];
-            p[1] = v[1];
-            p[2] = v[2];
-            p[3] = v[3];
-            p[4] = self;
-        }
-        if (a.ml.stats) {
-            int* st = a.ml.stats + kStatsPerMove * m;
-            st[0] = ti.lam_t - 2 > 0 ? ti.lam_t - 2 : 0;
-            st[1] = ti.lam_o - 2 > 0 ? ti.lam_o - 2 : 0;
-            st[2] = (int)v[4];
-            st[3] = (int)v[5];
-            for (int ep = 0; ep < 2; ++ep) {  // arrays an acceptance rewrites (src/tree.cpp:484-579)
-                const int lam = ep == 0 ? ti.lam_t : ti.lam_o;
-                int arrays = 0;
-                for (int l = 2; l <= lam && l <= D - 2; ++l) {
-                    const int node = ep == 0 ? ti.path_t[l] : ti.path_o[l];
-                    arrays += t.is_leaf[node] ? 1 : 2;
-                    for (int c = 0; c < 26; ++c) {
-                        const int C = t.colleague[26 * node + c];
-                        arrays += (C >= 0 && !t.is_leaf[C]);
-                    }
-                }
-                st[4 + ep] = arrays;
-            }
-        }
+        const double dE = assemble_trial(t, a, tr, m);
+        if (a.mode == kTrialOnly && a.ml.dE_stale) a.ml.dE_stale[m] = dE;
+        a.ml.dE[m] = dE;
         int ok = 0;
         if (a.mode == kTrialMetropolis) ok = a.ml.u[m] < exp(-a.beta * dE);
         if (a.mode == kTrialForceAccept) ok = 1;
         if (ok) ok = capacity_ok(t, ti, 0);
         a.s.accept[tr] = ok;
         if (a.ml.accept_out) a.ml.accept_out[m] = ok;
-        for (int k = 0; k < kAccPerTrial; ++k) acc[k] = 0.0;  // ready for the next launch
-        a.s.done[tr] = 0;
+        a.s.done[tr] = 0;  // ready for the next launch
     }
     if (a.mode == kTrialOnly) return;
     __syncthreads();
@@ -341,6 +373,7 @@ __global__ void __launch_bounds__(TB) k_trial(DevTree<T> t, McArgs a) {
 template <typename T>
 __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
     __shared__ double red[TB / 64];
+    __shared__ Cplx<T> rows[21 * kRowStride];
     const int K = a.K;
     int p = blockIdx.x, i = 0;
     while (p >= K - 1 - i) {
@@ -354,15 +387,20 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
     const int D = t.max_depth, nL = n_levels_pw(D), stride = table_stride(D);
     const Cplx<T>* ta = tabs_of<T>(a.s, stride, i);
     const Cplx<T>* tj = tabs_of<T>(a.s, stride, j);
+    // rows 0-5: j's root rows (t, o); 6-11: i's root rows
+    for (int q = threadIdx.x; q < 12 * kRowStride; q += blockDim.x)
+        rows[q] = q < 6 * kRowStride ? tj[q] : ta[q - 6 * kRowStride];
+    __syncthreads();
     double total = 0;
     {
         const int dw = t.dw;
+        const Cplx<T>* R = rows;
         double part = 0;
         for (int e = threadIdx.x; e < t.Mw; e += blockDim.x) {
-            const Cplx<T> x1 = phase_at(tj + root_row(0, 0), tj + root_row(0, 1), tj + root_row(0, 2), dw, e);
-            const Cplx<T> x2 = phase_at(tj + root_row(1, 0), tj + root_row(1, 1), tj + root_row(1, 2), dw, e);
-            const Cplx<T> y1 = phase_at(ta + root_row(0, 0), ta + root_row(0, 1), ta + root_row(0, 2), dw, e);
-            const Cplx<T> y2 = phase_at(ta + root_row(1, 0), ta + root_row(1, 1), ta + root_row(1, 2), dw, e);
+            const Cplx<T> x1 = phase_at(R, R + kRowStride, R + 2 * kRowStride, dw, e);
+            const Cplx<T> x2 = phase_at(R + 3 * kRowStride, R + 4 * kRowStride, R + 5 * kRowStride, dw, e);
+            const Cplx<T> y1 = phase_at(R + 6 * kRowStride, R + 7 * kRowStride, R + 8 * kRowStride, dw, e);
+            const Cplx<T> y2 = phase_at(R + 9 * kRowStride, R + 10 * kRowStride, R + 11 * kRowStride, dw, e);
             const T ar = x1.x - x2.x, ai = x1.y - x2.y, br = y1.x - y2.x, bi = y1.y - y2.y;
             part += (double)((ar * br + ai * bi) * t.Wmat[e]);
         }
@@ -373,7 +411,7 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
         for (int l = 2; l <= lam - 1; ++l) {
             const int li = l - 2, nl = 1 << l;
             const int node = ep == 0 ? Bj.path_t[l] : Bj.path_o[l];
-            int sh[2][3];
+            int sh[2][3] = {{0, 0, 0}, {0, 0, 0}};
             bool on[2];
             for (int w = 0; w < 2; ++w) {  // w = 0: i's target (+), 1: i's origin (-)
                 const int lami = w == 0 ? A.lam_t : A.lam_o;
@@ -386,22 +424,29 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
                 }
             }
             if (!on[0] && !on[1]) continue;  // uniform across the block
+            // rows 12-14: j's rows at this level; 15-17 / 18-20: i's shifted target / origin rows
+            for (int q = threadIdx.x; q < 9 * kRowStride; q += blockDim.x) {
+                const int rr = q / kRowStride, mm = q % kRowStride, ax = rr % 3;
+                Cplx<T> v = {T(0), T(0)};
+                if (rr < 3)
+                    v = tj[level_row(nL, ep, li, ax, 0) + mm];
+                else if (rr < 6 && on[0])
+                    v = ta[level_row(nL, 0, li, ax, sh[0][ax]) + mm];
+                else if (rr >= 6 && on[1])
+                    v = ta[level_row(nL, 1, li, ax, sh[1][ax]) + mm];
+                rows[12 * kRowStride + q] = v;
+            }
+            __syncthreads();
             const T* Dl = t.Dmat + (size_t)li * t.Md;
             const int dd = t.dd;
+            const Cplx<T>* R = rows + 12 * kRowStride;
             double part = 0;
             for (int e = threadIdx.x; e < t.Md; e += blockDim.x) {
-                const Cplx<T> w = phase_at(tj + level_row(nL, ep, li, 0, 0), tj + level_row(nL, ep, li, 1, 0),
-                                           tj + level_row(nL, ep, li, 2, 0), dd, e);
-                T dr = 0, di = 0;  // delta in = conj(tau_i s) - conj(omega_i s')
-                for (int wv = 0; wv < 2; ++wv) {
-                    if (!on[wv]) continue;
-                    const Cplx<T> v = phase_at(ta + level_row(nL, wv, li, 0, sh[wv][0]),
-                                               ta + level_row(nL, wv, li, 1, sh[wv][1]),
-                                               ta + level_row(nL, wv, li, 2, sh[wv][2]), dd, e);
-                    const T sg = wv == 0 ? T(1) : T(-1);
-                    dr += sg * v.x;
-                    di -= sg * v.y;
-                }
+                const Cplx<T> w = phase_at(R, R + kRowStride, R + 2 * kRowStride, dd, e);
+                // delta in = conj(tau_i s) - conj(omega_i s'); absent endpoints have zero rows
+                const Cplx<T> v0 = phase_at(R + 3 * kRowStride, R + 4 * kRowStride, R + 5 * kRowStride, dd, e);
+                const Cplx<T> v1 = phase_at(R + 6 * kRowStride, R + 7 * kRowStride, R + 8 * kRowStride, dd, e);
+                const T dr = v0.x - v1.x, di = -(v0.y - v1.y);
                 part += (double)((w.x * dr - w.y * di) * Dl[e]);
             }
             total += block_sum(part, red) * 2.0 * t.cD[l] * (ep == 0 ? 1.0 : -1.0);
@@ -432,40 +477,63 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
     }
 }
 
-// Sequential Metropolis decisions of a batch: dE_j = stale_j + sum over the
-// accepted i < j of corr[i][j]; then the update plan of the whole batch.
+// Batch decisions: dE_j = stale_j + sum over the accepted i < j of corr[i][j],
+// taken in order by one wave (lane l owns trials l, l+64, ...); the leaf
+// capacity check follows the counts the earlier acceptances leave.  Then the
+// update plan of the whole batch.
 template <typename T>
 __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
-    __shared__ double red[TB / 64];
-    __shared__ int delta;
+    __shared__ double dEs[kMaxBatch];
+    __shared__ int leafT[kMaxBatch], leafO[kMaxBatch], accs[kMaxBatch];
+    __shared__ double C[64 * 64];
     if (*t.err) return;
     const int K = a.K;
-    for (int j = 0; j < K; ++j) {
-        double part = 0;
-        int dcount = 0;
+    for (int j = threadIdx.x; j < K; j += blockDim.x) {
+        const int m = a.move0 + j;
+        const double dE = assemble_trial(t, a, j, m);
+        dEs[j] = dE;
+        a.ml.dE_stale[m] = dE;
         const TrialInfo& tj = a.s.info[j];
-        const int lt = tj.path_t[tj.lam_t];
-        for (int i = threadIdx.x; i < j; i += blockDim.x)
-            if (a.s.accept[i]) {
-                part += a.s.corr[i * K + j];
-                const TrialInfo& ti = a.s.info[i];
-                dcount += (ti.path_t[ti.lam_t] == lt) - (ti.path_o[ti.lam_o] == lt);
-            }
-        if (threadIdx.x == 0) delta = 0;
-        __syncthreads();
-        if (dcount) atomicAdd(&delta, dcount);
-        const double s = block_sum(part, red);
-        if (threadIdx.x == 0) {
-            const int m = a.move0 + j;
-            const double dE = a.ml.dE_stale[m] + s;
-            a.ml.dE[m] = dE;
-            int ok = a.ml.u[m] < exp(-a.beta * dE);
-            if (ok) ok = capacity_ok(t, tj, delta);
-            a.s.accept[j] = ok;
-            if (a.ml.accept_out) a.ml.accept_out[m] = ok;
-        }
-        __syncthreads();
+        leafT[j] = tj.path_t[tj.lam_t];
+        leafO[j] = tj.path_o[tj.lam_o];
     }
+    const bool in_lds = K <= 64;
+    if (in_lds)
+        for (int q = threadIdx.x; q < K * K; q += blockDim.x) C[q] = a.s.corr[q];
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        unsigned mine = 0;  // bit k: trial lane + 64 k accepted
+        for (int j = 0; j < K; ++j) {
+            double s = 0;
+            int dc = 0;
+            for (int i = lane, k = 0; i < j; i += 64, ++k)
+                if ((mine >> k) & 1u) {
+                    s += in_lds ? C[i * K + j] : a.s.corr[i * K + j];
+                    dc += (leafT[i] == leafT[j]) - (leafO[i] == leafT[j]);
+                }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                s += __shfl_xor(s, off, 64);
+                dc += __shfl_xor(dc, off, 64);
+            }
+            int ok = 0;
+            if (lane == 0) {
+                const int m = a.move0 + j;
+                const double dE = dEs[j] + s;
+                a.ml.dE[m] = dE;
+                ok = a.ml.u[m] < exp(-a.beta * dE);
+                if (ok) ok = capacity_ok(t, a.s.info[j], dc);
+                if (a.ml.accept_out) a.ml.accept_out[m] = ok;
+            }
+            ok = __shfl(ok, 0, 64);
+            if (lane == (j & 63) && ok) mine |= 1u << (j >> 6);
+        }
+        for (int k = 0; lane + 64 * k < K; ++k) accs[lane + 64 * k] = (mine >> k) & 1u;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < K; j += blockDim.x) a.s.accept[j] = accs[j];
+    __syncthreads();
     plan_update(t, a, K);
 }
 
@@ -585,10 +653,7 @@ void launch_setup(const DevTree<T>& t, const McArgs& a, hipStream_t st) {
 
 template <typename T>
 void launch_trial(const DevTree<T>& t, const McArgs& a, hipStream_t st) {
-    const int nWb = (t.Mw + TB - 1) / TB, nDb = (t.Md + TB - 1) / TB;
-    const int nL = t.max_depth >= 4 ? t.max_depth - 3 : 0;
-    const int per = nWb + 2 * nL * nDb + 2 * t.max_near;
-    hipLaunchKernelGGL(k_trial<T>, dim3(per, a.K), dim3(TB), 0, st, t, a);
+    hipLaunchKernelGGL(k_trial<T>, dim3(trial_blocks(t.max_depth), a.K), dim3(TB), 0, st, t, a);
 }
 
 template <typename T>

@@ -76,7 +76,6 @@ struct Handle {
     int* err = nullptr;
     // Monte Carlo scratch (kMaxBatch trials)
     McScratch sc{};
-    double* t_acc = nullptr;
     unsigned* t_done = nullptr;
     // move lists in HBM: `single` serves eval/update calls, `chain` the staged Metropolis run
     MoveBuf single, chain;
@@ -389,9 +388,8 @@ int hpdmk_tree_create_ex(HPDMKParams params, int precision, int n_src, const dou
         const int Mw = (2 * h->pw.n_window + 1) * (2 * h->pw.n_window + 1) * (h->pw.n_window + 1);
         h->acc_root = dalloc<double>(2 * Mw, o);
         h->energy = dalloc<double>(4, o);
-        h->t_acc = dalloc<double>(kAccPerTrial * kMaxBatch, o);
         h->t_done = dalloc<unsigned>(kMaxBatch, o);
-        h->sc.acc = h->t_acc;
+        h->sc.part = dalloc<double>(2 * (size_t)kMaxBatch * trial_blocks(D), o);
         h->sc.done = h->t_done;
         h->sc.info = dalloc<TrialInfo>(kMaxBatch, o);
         h->sc.accept = dalloc<int>(kMaxBatch, o);
@@ -403,7 +401,6 @@ int hpdmk_tree_create_ex(HPDMKParams params, int precision, int n_src, const dou
         h->sc.plan = dalloc<int>(2, o);
         HIPX(hipMemset(h->sc.mark, 0, 2 * (size_t)h->n_pw + 2));
         HIPX(hipMemset(h->sc.plan, 0, 2 * sizeof(int)));
-        HIPX(hipMemset(h->t_acc, 0, sizeof(double) * kAccPerTrial * kMaxBatch));
         HIPX(hipMemset(h->t_done, 0, sizeof(unsigned) * kMaxBatch));
         HIPX(hipDeviceSynchronize());
         *out = h.release();
@@ -651,7 +648,7 @@ int hpdmk_mc_run(hpdmk_tree t, double beta, int mode, int batch, double* gpu_ms)
             } else {
                 for (auto& bt : batches) {
                     const int m0 = bt.first, k = bt.second;
-                    const McArgs a = mc_args(h, mb, m0, k, kTrialOnly, beta);
+                    const McArgs a = mc_args(h, mb, m0, k, kTrialBatch, beta);
                     timed(h, kKSetup, [&] { launch_setup(dv, a, h->stream); });
                     timed(h, kKTrial, [&] { launch_trial(dv, a, h->stream); });
                     if (k > 1) timed(h, kKCorr, [&] { launch_pair_corr(dv, a, h->stream); });
@@ -767,7 +764,6 @@ int hpdmk_clear_error(hpdmk_tree t) {
     return guard([&] {
         Handle* h = H(t);
         HIPX(hipMemsetAsync(h->err, 0, sizeof(int), h->stream));
-        HIPX(hipMemsetAsync(h->t_acc, 0, sizeof(double) * kAccPerTrial * kMaxBatch, h->stream));
         HIPX(hipMemsetAsync(h->t_done, 0, sizeof(unsigned) * kMaxBatch, h->stream));
         HIPX(hipStreamSynchronize(h->stream));
     });
