@@ -50,6 +50,7 @@ struct DevTree {
     const int8_t* is_leaf;
     const int* cell;
     const int* colleague;
+    const int* col_in_slot;  // 26 per node: expansion slot of a non-leaf colleague, else -1
     const int* pw_slot;
     const int* near_off;
     const int* near_ids;

@@ -88,23 +88,34 @@ template <typename T>
 __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K) {
     __shared__ int wtot[TB / 64];
     __shared__ int base_sh;
-    const int D = t.max_depth;
-    for (int w = threadIdx.x; w < 2 * K; w += blockDim.x) {
-        const int m = w >> 1, ep = w & 1;
+    const int D = t.max_depth, nL = n_levels_pw(D);
+    // work item (move, endpoint, level, c): c < 26 colleague in, 26 own out, 27 own in
+    const int items = 2 * K * nL * 28;
+    for (int w = threadIdx.x; w < items; w += blockDim.x) {
+        const int c = w % 28, li = (w / 28) % nL, me = w / (28 * nL), m = me >> 1, ep = me & 1;
         if (!a.s.accept[m]) continue;
         const TrialInfo& ti = a.s.info[m];
-        const int lam = ep == 0 ? ti.lam_t : ti.lam_o;
-        const int* path = ep == 0 ? ti.path_t : ti.path_o;
-        for (int l = 2; l <= lam && l <= D - 2; ++l) {
-            const int node = path[l];
-            const int sl = t.pw_slot[node];
-            a.s.mark[2 * sl] = 1;
-            if (!t.is_leaf[node]) a.s.mark[2 * sl + 1] = 1;
-            for (int c = 0; c < 26; ++c) {
-                const int C = t.colleague[26 * node + c];
-                if (C >= 0 && !t.is_leaf[C]) a.s.mark[2 * t.pw_slot[C] + 1] = 1;
-            }
+        const int l = 2 + li;
+        if (l > (ep == 0 ? ti.lam_t : ti.lam_o)) continue;
+        const int node = ep == 0 ? ti.path_t[l] : ti.path_o[l];
+        if (c < 26) {
+            const int sl = t.col_in_slot[26 * node + c];
+            if (sl >= 0) a.s.mark[2 * sl + 1] = 1;
+        } else if (c == 26) {
+            a.s.mark[2 * t.pw_slot[node]] = 1;
+        } else if (!t.is_leaf[node]) {
+            a.s.mark[2 * t.pw_slot[node] + 1] = 1;
         }
+    }
+    // particles that stay in their leaf move in place, all at once
+    for (int m = threadIdx.x; m < K; m += blockDim.x) {
+        if (!a.s.accept[m]) continue;
+        const TrialInfo& ti = a.s.info[m];
+        if (ti.path_t[ti.lam_t] != ti.path_o[ti.lam_o]) continue;
+        const int s = t.slot_of[ti.i];
+        t.px[s] = ti.t[0];
+        t.py[s] = ti.t[1];
+        t.pz[s] = ti.t[2];
     }
     __syncthreads();
     const int n2 = 2 * t.n_pw_nodes;
@@ -135,37 +146,33 @@ __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K) {
     for (int m = 0; m < K; ++m) any |= a.s.accept[m];
     a.s.plan[0] = base_sh;
     a.s.plan[1] = any;
-    // particle relocation (src/tree.cpp:454-463, 583-596), exact membership
+    // leaf changes, in move order (src/tree.cpp:454-463, 583-596): append to the
+    // target leaf, swap the origin leaf's last particle into the freed slot
     for (int m = 0; m < K; ++m) {
         if (!a.s.accept[m]) continue;
         const TrialInfo& ti = a.s.info[m];
+        const int lo = ti.path_o[ti.lam_o], lt = ti.path_t[ti.lam_t];
+        if (lo == lt) continue;
         const int i = ti.i;
         const int s = t.slot_of[i];
-        const int lo = t.leaf_of[i];
-        const int lt = ti.path_t[ti.lam_t];
-        if (lo == lt) {
-            t.px[s] = ti.t[0];
-            t.py[s] = ti.t[1];
-            t.pz[s] = ti.t[2];
-            continue;
-        }
         const int c = t.slot_count[lt];
         const int ns = t.slot_start[lt] + c;
+        const int last = t.slot_start[lo] + t.slot_count[lo] - 1;
+        const double q = t.pq[s];
         t.px[ns] = ti.t[0];
         t.py[ns] = ti.t[1];
         t.pz[ns] = ti.t[2];
-        t.pq[ns] = t.pq[s];
+        t.pq[ns] = q;
         t.porig[ns] = i;
         t.slot_count[lt] = c + 1;
         t.slot_of[i] = ns;
         t.leaf_of[i] = lt;
-        const int last = t.slot_start[lo] + t.slot_count[lo] - 1;
         if (last != s) {
+            const int j = t.porig[last];
             t.px[s] = t.px[last];
             t.py[s] = t.py[last];
             t.pz[s] = t.pz[last];
             t.pq[s] = t.pq[last];
-            const int j = t.porig[last];
             t.porig[s] = j;
             t.slot_of[j] = s;
         }
@@ -239,10 +246,7 @@ __device__ double assemble_trial(const DevTree<T>& t, const McArgs& a, int tr, i
             for (int l = 2; l <= lam && l <= t.max_depth - 2; ++l) {
                 const int node = ep == 0 ? ti.path_t[l] : ti.path_o[l];
                 arrays += t.is_leaf[node] ? 1 : 2;
-                for (int c = 0; c < 26; ++c) {
-                    const int C = t.colleague[26 * node + c];
-                    arrays += (C >= 0 && !t.is_leaf[C]);
-                }
+                for (int c = 0; c < 26; ++c) arrays += t.col_in_slot[26 * node + c] >= 0;
             }
             st[4 + ep] = arrays;
         }
@@ -484,7 +488,7 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
 template <typename T>
 __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
     __shared__ double dEs[kMaxBatch];
-    __shared__ int leafT[kMaxBatch], leafO[kMaxBatch], accs[kMaxBatch];
+    __shared__ int leafT[kMaxBatch], leafO[kMaxBatch], accs[kMaxBatch], room[kMaxBatch];
     __shared__ double C[64 * 64];
     if (*t.err) return;
     const int K = a.K;
@@ -496,6 +500,7 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
         const TrialInfo& tj = a.s.info[j];
         leafT[j] = tj.path_t[tj.lam_t];
         leafO[j] = tj.path_o[tj.lam_o];
+        room[j] = leafT[j] == leafO[j] ? (1 << 30) : t.slot_cap[leafT[j]] - t.slot_count[leafT[j]];
     }
     const bool in_lds = K <= 64;
     if (in_lds)
@@ -523,7 +528,10 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
                 const double dE = dEs[j] + s;
                 a.ml.dE[m] = dE;
                 ok = a.ml.u[m] < exp(-a.beta * dE);
-                if (ok) ok = capacity_ok(t, a.s.info[j], dc);
+                if (ok && dc >= room[j]) {  // the target leaf is full
+                    atomicOr(t.err, (int)kErrLeafOverflow);
+                    ok = 0;
+                }
                 if (a.ml.accept_out) a.ml.accept_out[m] = ok;
             }
             ok = __shfl(ok, 0, 64);
@@ -538,14 +546,18 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
 }
 
 // ------------------------------------------------------------------ apply
-// Work items: root chunks, then (rewritten array, element chunk).  For an
-// array the block first lists the (move, endpoint, image) contributions in move
-// order, then every thread adds them to its element and writes it back once.
+// Work items: root chunks, then (rewritten array, element chunk).  For its
+// array a block lists the (move, endpoint, image) contributions in move order,
+// stages their phase rows in LDS a tile at a time, and every thread adds them
+// to its element, which it reads and writes once.
+constexpr int kApplyTile = 16;
+
 template <typename T>
 __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
     __shared__ int mlist[2 * kMaxBatch];
     __shared__ int wtot[TB / 64];
     __shared__ int mcount;
+    __shared__ Cplx<T> rows[kApplyTile * 3 * kRowStride];
     if (*t.err) return;
     const int nD = a.s.plan[0];
     if (!a.s.plan[1]) return;
@@ -554,30 +566,22 @@ __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
     const int total = nWb + nD * nDb;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
-        if (item < nWb) {  // root += tau - omega (src/tree.cpp:479-480)
-            const int e = item * TB + threadIdx.x;
-            if (e >= t.Mw) continue;
-            T ar = 0, ai = 0;
-            for (int m = 0; m < K; ++m) {
-                if (!a.s.accept[m]) continue;
-                const Cplx<T>* tb = tabs_of<T>(a.s, stride, m);
-                const Cplx<T> ta = phase_at(tb + root_row(0, 0), tb + root_row(0, 1), tb + root_row(0, 2), t.dw, e);
-                const Cplx<T> om = phase_at(tb + root_row(1, 0), tb + root_row(1, 1), tb + root_row(1, 2), t.dw, e);
-                ar += ta.x - om.x;
-                ai += ta.y - om.y;
-            }
-            Cplx<T> g = t.root[e];
-            g.x += ar;
-            g.y += ai;
-            t.root[e] = g;
-            continue;
+        const bool is_root = item < nWb;
+        int slot = 0, kind = 2, node = 0, l = 0, li = 0, chunk;
+        if (is_root) {
+            chunk = item;
+        } else {
+            const int r = item - nWb;
+            const int key = a.s.dest[r / nDb];
+            slot = key >> 1;
+            kind = key & 1;
+            node = a.pn.node[slot];
+            l = a.pn.level[slot];
+            li = l - 2;
+            chunk = r % nDb;
         }
-        const int r = item - nWb;
-        const int key = a.s.dest[r / nDb];
-        const int slot = key >> 1, kind = key & 1;
-        const int node = a.pn.node[slot];
-        const int l = a.pn.level[slot], li = l - 2, nl = 1 << l;
-        __syncthreads();  // mlist of the previous item fully consumed
+        const int nl = 1 << l;
+        __syncthreads();  // previous item's lists and rows fully consumed
         if (threadIdx.x == 0) mcount = 0;
         __syncthreads();
         for (int c0 = 0; c0 < 2 * K; c0 += blockDim.x) {
@@ -586,17 +590,21 @@ __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
             if (w < 2 * K) {
                 const int m = w >> 1, ep = w & 1;
                 if (a.s.accept[m]) {
-                    const TrialInfo& ti = a.s.info[m];
-                    const int lam = ep == 0 ? ti.lam_t : ti.lam_o;
-                    if (l <= lam) {
-                        const int B = ep == 0 ? ti.path_t[l] : ti.path_o[l];
-                        if (B == node) {
-                            code = 13;
-                        } else if (kind == 1) {
-                            int px = image_shift(t.cell[3 * B], t.cell[3 * node], nl);
-                            int py = image_shift(t.cell[3 * B + 1], t.cell[3 * node + 1], nl);
-                            int pz = image_shift(t.cell[3 * B + 2], t.cell[3 * node + 2], nl);
-                            if (px != 2 && py != 2 && pz != 2) code = (px + 1) + 3 * (py + 1) + 9 * (pz + 1);
+                    if (is_root) {
+                        code = 13;
+                    } else {
+                        const TrialInfo& ti = a.s.info[m];
+                        const int lam = ep == 0 ? ti.lam_t : ti.lam_o;
+                        if (l <= lam) {
+                            const int B = ep == 0 ? ti.path_t[l] : ti.path_o[l];
+                            if (B == node) {
+                                code = 13;
+                            } else if (kind == 1) {
+                                const int px = image_shift(t.cell[3 * B], t.cell[3 * node], nl);
+                                const int py = image_shift(t.cell[3 * B + 1], t.cell[3 * node + 1], nl);
+                                const int pz = image_shift(t.cell[3 * B + 2], t.cell[3 * node + 2], nl);
+                                if (px != 2 && py != 2 && pz != 2) code = (px + 1) + 3 * (py + 1) + 9 * (pz + 1);
+                            }
                         }
                     }
                 }
@@ -615,28 +623,46 @@ __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
                 for (int k = 0; k < (int)(blockDim.x >> 6); ++k) mcount += wtot[k];
             __syncthreads();
         }
-        const int e = (r % nDb) * TB + threadIdx.x;
-        if (e >= t.Md) continue;
-        T ar = 0, ai = 0;
         const int nm = mcount;
-        for (int k = 0; k < nm; ++k) {
-            const int c = mlist[k];
-            const int m = c >> 8, ep = (c >> 7) & 1, code = c & 127;
-            const int px = code % 3 - 1, py = (code / 3) % 3 - 1, pz = code / 9 - 1;
-            const Cplx<T>* tb = tabs_of<T>(a.s, stride, m);
-            const Cplx<T> v = phase_at(tb + level_row(nL, ep, li, 0, px), tb + level_row(nL, ep, li, 1, py),
-                                       tb + level_row(nL, ep, li, 2, pz), t.dd, e);
-            const T sg = ep == 0 ? T(1) : T(-1);
-            ar += sg * v.x;
-            ai += sg * v.y;
+        const int d = is_root ? t.dw : t.dd;
+        const int e = chunk * TB + threadIdx.x;
+        const int ei = e % d, er = e / d, ej = er % d, ek = er / d;
+        T ar = 0, ai = 0;
+        for (int k0 = 0; k0 < nm; k0 += kApplyTile) {
+            const int nt = nm - k0 < kApplyTile ? nm - k0 : kApplyTile;
+            __syncthreads();
+            for (int q = threadIdx.x; q < nt * 3 * kRowStride; q += blockDim.x) {
+                const int k = q / (3 * kRowStride), ax = (q / kRowStride) % 3, mm = q % kRowStride;
+                const int c = mlist[k0 + k];
+                const int m = c >> 8, ep = (c >> 7) & 1, code = c & 127;
+                const int sh = ax == 0 ? code % 3 - 1 : (ax == 1 ? (code / 3) % 3 - 1 : code / 9 - 1);
+                const Cplx<T>* tb = tabs_of<T>(a.s, stride, m);
+                rows[q] = tb[(is_root ? root_row(ep, ax) : level_row(nL, ep, li, ax, sh)) + mm];
+            }
+            __syncthreads();
+            if (e < (is_root ? t.Mw : t.Md))
+                for (int k = 0; k < nt; ++k) {
+                    const Cplx<T>* R = rows + k * 3 * kRowStride;
+                    const Cplx<T> v = cmul(cmul(R[ei], R[kRowStride + ej]), R[2 * kRowStride + ek]);
+                    const T sg = ((mlist[k0 + k] >> 7) & 1) ? T(-1) : T(1);
+                    ar += sg * v.x;
+                    ai += sg * v.y;
+                }
         }
-        const size_t at = (size_t)slot * t.Md + e;
-        if (kind == 0) {  // out[B] += tau
+        if (e >= (is_root ? t.Mw : t.Md)) continue;
+        if (is_root) {  // root += tau - omega (src/tree.cpp:479-480)
+            Cplx<T> g = t.root[e];
+            g.x += ar;
+            g.y += ai;
+            t.root[e] = g;
+        } else if (kind == 0) {  // out[B] += tau
+            const size_t at = (size_t)slot * t.Md + e;
             Cplx<T> o = t.out[at];
             o.x += ar;
             o.y += ai;
             t.out[at] = o;
         } else {  // in[C] += conj(tau s)
+            const size_t at = (size_t)slot * t.Md + e;
             Cplx<T> o = t.inc[at];
             o.x += ar;
             o.y -= ai;

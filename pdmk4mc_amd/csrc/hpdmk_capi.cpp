@@ -324,12 +324,19 @@ int hpdmk_tree_create_ex(HPDMKParams params, int precision, int n_src, const dou
             d.n_poly = (int)h->pw.poly.size();
             for (int k = 0; k < d.n_poly; ++k) d.poly[k] = h->pw.poly[k];
         };
+        std::vector<int> col_in(26 * (size_t)nn, -1);
+        for (int i = 0; i < nn; ++i)
+            for (int c = 0; c < 26; ++c) {
+                const int C = tr.colleague[26 * i + c];
+                if (C >= 0 && !tr.is_leaf[C] && pw_slot[C] >= 0) col_in[26 * i + c] = pw_slot[C];
+            }
         std::vector<int8_t> leaf8(tr.is_leaf.begin(), tr.is_leaf.end());
         const int* child = dupload(tr.child, o);
         const int* level = dupload(tr.level, o);
         const int8_t* isl = dupload(leaf8, o);
         const int* cellp = dupload(tr.cell, o);
         const int* coll = dupload(tr.colleague, o);
+        const int* colin = dupload(col_in, o);
         const int* pws = dupload(pw_slot, o);
         const int* noff = dupload(tr.near_off, o);
         const int* nids = dupload(tr.near_ids, o);
@@ -352,6 +359,7 @@ int hpdmk_tree_create_ex(HPDMKParams params, int precision, int n_src, const dou
             d.is_leaf = isl;
             d.cell = cellp;
             d.colleague = coll;
+            d.col_in_slot = colin;
             d.pw_slot = pws;
             d.near_off = noff;
             d.near_ids = nids;
