@@ -79,6 +79,42 @@ __global__ void __launch_bounds__(TB) k_setup(DevTree<T> t, McArgs a) {
 }
 
 // ------------------------------------------------------------------ update plan
+// moves an accepted particle whose leaf changes into its target leaf's slots
+template <typename T>
+__device__ void relocate(const DevTree<T>& t, const TrialInfo& ti) {
+    const int lo = ti.path_o[ti.lam_o], lt = ti.path_t[ti.lam_t];
+    if (lo == lt) return;
+    const int i = ti.i;
+    const int s = t.slot_of[i];
+    const int c = t.slot_count[lt];
+    const int ns = t.slot_start[lt] + c;
+    const int last = t.slot_start[lo] + t.slot_count[lo] - 1;
+    const double q = t.pq[s];
+    t.px[ns] = ti.t[0];
+    t.py[ns] = ti.t[1];
+    t.pz[ns] = ti.t[2];
+    t.pq[ns] = q;
+    t.porig[ns] = i;
+    t.slot_count[lt] = c + 1;
+    t.slot_of[i] = ns;
+    t.leaf_of[i] = lt;
+    if (last != s) {
+        const int j = t.porig[last];
+        t.px[s] = t.px[last];
+        t.py[s] = t.py[last];
+        t.pz[s] = t.pz[last];
+        t.pq[s] = t.pq[last];
+        t.porig[s] = j;
+        t.slot_of[j] = s;
+    }
+    t.px[last] = 0;
+    t.py[last] = 0;
+    t.pz[last] = 0;
+    t.pq[last] = 0;
+    t.porig[last] = -1;
+    t.slot_count[lo] -= 1;
+}
+
 // Marks every expansion array the accepted moves rewrite (out of each path box
 // of levels 2..min(leaf, D-2), in of the non-leaf ones and of their non-leaf
 // colleagues), compacts the marks into s.dest in slot order, and relocates
@@ -88,30 +124,37 @@ template <typename T>
 __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K) {
     __shared__ int wtot[TB / 64];
     __shared__ int base_sh;
+    __shared__ int lo_s[kMaxBatch], lt_s[kMaxBatch], ac_s[kMaxBatch], conflict[kMaxBatch];
     const int D = t.max_depth, nL = n_levels_pw(D);
-    // work item (move, endpoint, level, c): c < 26 colleague in, 26 own out, 27 own in
-    const int items = 2 * K * nL * 28;
-    for (int w = threadIdx.x; w < items; w += blockDim.x) {
-        const int c = w % 28, li = (w / 28) % nL, me = w / (28 * nL), m = me >> 1, ep = me & 1;
-        if (!a.s.accept[m]) continue;
+    for (int m = threadIdx.x; m < K; m += blockDim.x) {
+        const TrialInfo& ti = a.s.info[m];
+        ac_s[m] = a.s.accept[m];
+        lo_s[m] = ti.path_o[ti.lam_o];
+        lt_s[m] = ti.path_t[ti.lam_t];
+    }
+    __syncthreads();
+    // one thread per (move, endpoint, level): own out, own in, colleagues' in
+    for (int w = threadIdx.x; w < 2 * K * nL; w += blockDim.x) {
+        const int li = w % nL, me = w / nL, m = me >> 1, ep = me & 1;
+        if (!ac_s[m]) continue;
         const TrialInfo& ti = a.s.info[m];
         const int l = 2 + li;
         if (l > (ep == 0 ? ti.lam_t : ti.lam_o)) continue;
         const int node = ep == 0 ? ti.path_t[l] : ti.path_o[l];
-        if (c < 26) {
-            const int sl = t.col_in_slot[26 * node + c];
-            if (sl >= 0) a.s.mark[2 * sl + 1] = 1;
-        } else if (c == 26) {
-            a.s.mark[2 * t.pw_slot[node]] = 1;
-        } else if (!t.is_leaf[node]) {
-            a.s.mark[2 * t.pw_slot[node] + 1] = 1;
-        }
+        const int own = t.pw_slot[node];
+        a.s.mark[2 * own] = 1;
+        if (!t.is_leaf[node]) a.s.mark[2 * own + 1] = 1;
+        int sl[26];
+#pragma unroll
+        for (int c = 0; c < 26; ++c) sl[c] = t.col_in_slot[26 * node + c];
+#pragma unroll
+        for (int c = 0; c < 26; ++c)
+            if (sl[c] >= 0) a.s.mark[2 * sl[c] + 1] = 1;
     }
     // particles that stay in their leaf move in place, all at once
     for (int m = threadIdx.x; m < K; m += blockDim.x) {
-        if (!a.s.accept[m]) continue;
+        if (!ac_s[m] || lt_s[m] != lo_s[m]) continue;
         const TrialInfo& ti = a.s.info[m];
-        if (ti.path_t[ti.lam_t] != ti.path_o[ti.lam_o]) continue;
         const int s = t.slot_of[ti.i];
         t.px[s] = ti.t[0];
         t.py[s] = ti.t[1];
@@ -141,48 +184,31 @@ __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K) {
             for (int k = 0; k < nw; ++k) base_sh += wtot[k];
         __syncthreads();
     }
+    // leaf changes (src/tree.cpp:454-463, 583-596): append to the target leaf
+    // and swap the origin leaf's last particle into the freed slot.  Changes
+    // that share no leaf with another change of the batch run in parallel; the
+    // others run after them, in move order.
+    for (int m = threadIdx.x; m < K; m += blockDim.x) {
+        int cf = 0;
+        const int lo = lo_s[m], lt = lt_s[m];
+        if (ac_s[m] && lo != lt)
+            for (int k = 0; k < K && !cf; ++k) {
+                if (k == m || !ac_s[k] || lo_s[k] == lt_s[k]) continue;
+                cf = (lo_s[k] == lo || lo_s[k] == lt || lt_s[k] == lo || lt_s[k] == lt);
+            }
+        conflict[m] = cf;
+    }
+    __syncthreads();
+    for (int m = threadIdx.x; m < K; m += blockDim.x)
+        if (ac_s[m] && !conflict[m]) relocate(t, a.s.info[m]);
+    __syncthreads();
     if (threadIdx.x != 0) return;
     int any = 0;
-    for (int m = 0; m < K; ++m) any |= a.s.accept[m];
+    for (int m = 0; m < K; ++m) any |= ac_s[m];
     a.s.plan[0] = base_sh;
     a.s.plan[1] = any;
-    // leaf changes, in move order (src/tree.cpp:454-463, 583-596): append to the
-    // target leaf, swap the origin leaf's last particle into the freed slot
-    for (int m = 0; m < K; ++m) {
-        if (!a.s.accept[m]) continue;
-        const TrialInfo& ti = a.s.info[m];
-        const int lo = ti.path_o[ti.lam_o], lt = ti.path_t[ti.lam_t];
-        if (lo == lt) continue;
-        const int i = ti.i;
-        const int s = t.slot_of[i];
-        const int c = t.slot_count[lt];
-        const int ns = t.slot_start[lt] + c;
-        const int last = t.slot_start[lo] + t.slot_count[lo] - 1;
-        const double q = t.pq[s];
-        t.px[ns] = ti.t[0];
-        t.py[ns] = ti.t[1];
-        t.pz[ns] = ti.t[2];
-        t.pq[ns] = q;
-        t.porig[ns] = i;
-        t.slot_count[lt] = c + 1;
-        t.slot_of[i] = ns;
-        t.leaf_of[i] = lt;
-        if (last != s) {
-            const int j = t.porig[last];
-            t.px[s] = t.px[last];
-            t.py[s] = t.py[last];
-            t.pz[s] = t.pz[last];
-            t.pq[s] = t.pq[last];
-            t.porig[s] = j;
-            t.slot_of[j] = s;
-        }
-        t.px[last] = 0;
-        t.py[last] = 0;
-        t.pz[last] = 0;
-        t.pq[last] = 0;
-        t.porig[last] = -1;
-        t.slot_count[lo] -= 1;
-    }
+    for (int m = 0; m < K; ++m)
+        if (ac_s[m] && conflict[m]) relocate(t, a.s.info[m]);
 }
 
 // capacity check of an accepted move given the count change `delta` that
@@ -489,6 +515,7 @@ template <typename T>
 __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
     __shared__ double dEs[kMaxBatch];
     __shared__ int leafT[kMaxBatch], leafO[kMaxBatch], accs[kMaxBatch], room[kMaxBatch];
+    __shared__ double us[kMaxBatch];
     __shared__ double C[64 * 64];
     if (*t.err) return;
     const int K = a.K;
@@ -501,6 +528,7 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
         leafT[j] = tj.path_t[tj.lam_t];
         leafO[j] = tj.path_o[tj.lam_o];
         room[j] = leafT[j] == leafO[j] ? (1 << 30) : t.slot_cap[leafT[j]] - t.slot_count[leafT[j]];
+        us[j] = a.ml.u[m];
     }
     const bool in_lds = K <= 64;
     if (in_lds)
@@ -527,7 +555,7 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
                 const int m = a.move0 + j;
                 const double dE = dEs[j] + s;
                 a.ml.dE[m] = dE;
-                ok = a.ml.u[m] < exp(-a.beta * dE);
+                ok = us[j] < exp(-a.beta * dE);
                 if (ok && dc >= room[j]) {  // the target leaf is full
                     atomicOr(t.err, (int)kErrLeafOverflow);
                     ok = 0;
@@ -551,6 +579,7 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
 // stages their phase rows in LDS a tile at a time, and every thread adds them
 // to its element, which it reads and writes once.
 constexpr int kApplyTile = 16;
+constexpr int kApplyPer = 4;  // elements per thread and work item
 
 template <typename T>
 __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
@@ -558,11 +587,14 @@ __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
     __shared__ int wtot[TB / 64];
     __shared__ int mcount;
     __shared__ Cplx<T> rows[kApplyTile * 3 * kRowStride];
+    __shared__ int ac_s[kMaxBatch];
     if (*t.err) return;
     const int nD = a.s.plan[0];
     if (!a.s.plan[1]) return;
     const int K = a.K, D = t.max_depth, nL = n_levels_pw(D), stride = table_stride(D);
-    const int nWb = (t.Mw + TB - 1) / TB, nDb = (t.Md + TB - 1) / TB;
+    for (int m = threadIdx.x; m < K; m += blockDim.x) ac_s[m] = a.s.accept[m];
+    constexpr int CH = TB * kApplyPer;
+    const int nWb = (t.Mw + CH - 1) / CH, nDb = (t.Md + CH - 1) / CH;
     const int total = nWb + nD * nDb;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
@@ -589,7 +621,7 @@ __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
             int code = -1;
             if (w < 2 * K) {
                 const int m = w >> 1, ep = w & 1;
-                if (a.s.accept[m]) {
+                if (ac_s[m]) {
                     if (is_root) {
                         code = 13;
                     } else {
@@ -625,9 +657,19 @@ __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
         }
         const int nm = mcount;
         const int d = is_root ? t.dw : t.dd;
-        const int e = chunk * TB + threadIdx.x;
-        const int ei = e % d, er = e / d, ej = er % d, ek = er / d;
-        T ar = 0, ai = 0;
+        const int M = is_root ? t.Mw : t.Md;
+        int ei[kApplyPer], ej[kApplyPer], ek[kApplyPer];
+        T ar[kApplyPer], ai[kApplyPer];
+#pragma unroll
+        for (int q = 0; q < kApplyPer; ++q) {
+            const int e = chunk * CH + q * TB + threadIdx.x;
+            const int er = e / d;
+            ei[q] = e % d;
+            ej[q] = er % d;
+            ek[q] = er / d;
+            ar[q] = 0;
+            ai[q] = 0;
+        }
         for (int k0 = 0; k0 < nm; k0 += kApplyTile) {
             const int nt = nm - k0 < kApplyTile ? nm - k0 : kApplyTile;
             __syncthreads();
@@ -640,33 +682,40 @@ __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
                 rows[q] = tb[(is_root ? root_row(ep, ax) : level_row(nL, ep, li, ax, sh)) + mm];
             }
             __syncthreads();
-            if (e < (is_root ? t.Mw : t.Md))
-                for (int k = 0; k < nt; ++k) {
-                    const Cplx<T>* R = rows + k * 3 * kRowStride;
-                    const Cplx<T> v = cmul(cmul(R[ei], R[kRowStride + ej]), R[2 * kRowStride + ek]);
-                    const T sg = ((mlist[k0 + k] >> 7) & 1) ? T(-1) : T(1);
-                    ar += sg * v.x;
-                    ai += sg * v.y;
+            for (int k = 0; k < nt; ++k) {
+                const Cplx<T>* R = rows + k * 3 * kRowStride;
+                const T sg = ((mlist[k0 + k] >> 7) & 1) ? T(-1) : T(1);
+#pragma unroll
+                for (int q = 0; q < kApplyPer; ++q) {
+                    if (ek[q] >= (is_root ? t.n_window : t.n_diff) + 1) continue;  // beyond the half grid
+                    const Cplx<T> v = cmul(cmul(R[ei[q]], R[kRowStride + ej[q]]), R[2 * kRowStride + ek[q]]);
+                    ar[q] += sg * v.x;
+                    ai[q] += sg * v.y;
                 }
+            }
         }
-        if (e >= (is_root ? t.Mw : t.Md)) continue;
-        if (is_root) {  // root += tau - omega (src/tree.cpp:479-480)
-            Cplx<T> g = t.root[e];
-            g.x += ar;
-            g.y += ai;
-            t.root[e] = g;
-        } else if (kind == 0) {  // out[B] += tau
-            const size_t at = (size_t)slot * t.Md + e;
-            Cplx<T> o = t.out[at];
-            o.x += ar;
-            o.y += ai;
-            t.out[at] = o;
-        } else {  // in[C] += conj(tau s)
-            const size_t at = (size_t)slot * t.Md + e;
-            Cplx<T> o = t.inc[at];
-            o.x += ar;
-            o.y -= ai;
-            t.inc[at] = o;
+#pragma unroll
+        for (int q = 0; q < kApplyPer; ++q) {
+            const int e = chunk * CH + q * TB + threadIdx.x;
+            if (e >= M) continue;
+            if (is_root) {  // root += tau - omega (src/tree.cpp:479-480)
+                Cplx<T> g = t.root[e];
+                g.x += ar[q];
+                g.y += ai[q];
+                t.root[e] = g;
+            } else if (kind == 0) {  // out[B] += tau
+                const size_t at = (size_t)slot * t.Md + e;
+                Cplx<T> o = t.out[at];
+                o.x += ar[q];
+                o.y += ai[q];
+                t.out[at] = o;
+            } else {  // in[C] += conj(tau s)
+                const size_t at = (size_t)slot * t.Md + e;
+                Cplx<T> o = t.inc[at];
+                o.x += ar[q];
+                o.y -= ai[q];
+                t.inc[at] = o;
+            }
         }
     }
 }
