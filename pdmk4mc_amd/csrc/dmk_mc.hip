@@ -224,23 +224,31 @@ __device__ int capacity_ok(const DevTree<T>& t, const TrialInfo& ti, int delta) 
 
 // ------------------------------------------------------------------ trial
 // Sums the per-block partials of trial tr into v: window, difference,
-// res_t, res_o, particles scanned at t and o.  Loads are device-coherent.
+// res_t, res_o, particles scanned at t and o.  Callers make the partials
+// visible first (kernel boundary, or an agent-scope fence after the block
+// counter); plain loads then see them.
 template <typename T>
 __device__ void sum_trial_partials(const DevTree<T>& t, const McArgs& a, int tr, double* v) {
     const int D = t.max_depth, nL = n_levels_pw(D), per = trial_blocks(D);
-    for (int k = 0; k < kAccPerTrial; ++k) v[k] = 0;
     const double* p = a.s.part + 2 * (size_t)tr * per;
-    for (int b = 0; b < per; ++b) {
-        int slot;
-        if (b < kWinBlocks)
-            slot = 0;
-        else if (b < kWinBlocks + 2 * nL * kDiffBlocks)
-            slot = 1;
-        else
-            slot = 2 + (b - kWinBlocks - 2 * nL * kDiffBlocks) / kNearBlocks;
-        v[slot] += __hip_atomic_load(p + 2 * b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (slot >= 2) v[slot + 2] += __hip_atomic_load(p + 2 * b + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int d0 = kWinBlocks, d1 = d0 + 2 * nL * kDiffBlocks;
+    double w = 0, df = 0, rt = 0, ro = 0, nt = 0, no = 0;
+#pragma unroll
+    for (int b = 0; b < kWinBlocks; ++b) w += p[2 * b];
+    for (int b = d0; b < d1; ++b) df += p[2 * b];
+#pragma unroll
+    for (int b = 0; b < kNearBlocks; ++b) {
+        rt += p[2 * (d1 + b)];
+        nt += p[2 * (d1 + b) + 1];
+        ro += p[2 * (d1 + kNearBlocks + b)];
+        no += p[2 * (d1 + kNearBlocks + b) + 1];
     }
+    v[0] = w;
+    v[1] = df;
+    v[2] = rt;
+    v[3] = ro;
+    v[4] = nt;
+    v[5] = no;
 }
 
 // dE of trial tr from its partials (src/energy_shift.cpp:67) plus the optional
@@ -579,7 +587,7 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
 // stages their phase rows in LDS a tile at a time, and every thread adds them
 // to its element, which it reads and writes once.
 constexpr int kApplyTile = 16;
-constexpr int kApplyPer = 4;  // elements per thread and work item
+constexpr int kApplyPer = 32;  // elements per thread: a work item covers a whole level array (digits <= 6)
 
 template <typename T>
 __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
@@ -658,15 +666,13 @@ __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
         const int nm = mcount;
         const int d = is_root ? t.dw : t.dd;
         const int M = is_root ? t.Mw : t.Md;
-        int ei[kApplyPer], ej[kApplyPer], ek[kApplyPer];
+        int idx[kApplyPer];  // packed i | j << 8 | k << 16 of each element, -1 beyond the array
         T ar[kApplyPer], ai[kApplyPer];
 #pragma unroll
         for (int q = 0; q < kApplyPer; ++q) {
             const int e = chunk * CH + q * TB + threadIdx.x;
             const int er = e / d;
-            ei[q] = e % d;
-            ej[q] = er % d;
-            ek[q] = er / d;
+            idx[q] = e < M ? ((e % d) | ((er % d) << 8) | ((er / d) << 16)) : -1;
             ar[q] = 0;
             ai[q] = 0;
         }
@@ -687,8 +693,9 @@ __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
                 const T sg = ((mlist[k0 + k] >> 7) & 1) ? T(-1) : T(1);
 #pragma unroll
                 for (int q = 0; q < kApplyPer; ++q) {
-                    if (ek[q] >= (is_root ? t.n_window : t.n_diff) + 1) continue;  // beyond the half grid
-                    const Cplx<T> v = cmul(cmul(R[ei[q]], R[kRowStride + ej[q]]), R[2 * kRowStride + ek[q]]);
+                    if (idx[q] < 0) continue;
+                    const int ii = idx[q] & 255, jj = (idx[q] >> 8) & 255, kk = idx[q] >> 16;
+                    const Cplx<T> v = cmul(cmul(R[ii], R[kRowStride + jj]), R[2 * kRowStride + kk]);
                     ar[q] += sg * v.x;
                     ai[q] += sg * v.y;
                 }
