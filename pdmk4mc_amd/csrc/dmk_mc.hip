@@ -543,37 +543,46 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
         for (int q = threadIdx.x; q < K * K; q += blockDim.x) C[q] = a.s.corr[q];
     __syncthreads();
     if (threadIdx.x < 64) {
+        // lane l owns trials l, l+64, l+128, l+192 and keeps their running
+        // corrected dE and leaf-count change; decisions go in trial order
         const int lane = threadIdx.x;
-        unsigned mine = 0;  // bit k: trial lane + 64 k accepted
-        for (int j = 0; j < K; ++j) {
-            double s = 0;
-            int dc = 0;
-            for (int i = lane, k = 0; i < j; i += 64, ++k)
-                if ((mine >> k) & 1u) {
-                    s += in_lds ? C[i * K + j] : a.s.corr[i * K + j];
-                    dc += (leafT[i] == leafT[j]) - (leafO[i] == leafT[j]);
-                }
+        double run[4];
+        int dcr[4];
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                s += __shfl_xor(s, off, 64);
-                dc += __shfl_xor(dc, off, 64);
-            }
+        for (int k = 0; k < 4; ++k) {
+            const int j = lane + 64 * k;
+            run[k] = j < K ? dEs[j] : 0.0;
+            dcr[k] = 0;
+        }
+        for (int i = 0; i < K; ++i) {
+            const int owner = i & 63, sl = i >> 6;
             int ok = 0;
-            if (lane == 0) {
-                const int m = a.move0 + j;
-                const double dE = dEs[j] + s;
+            if (lane == owner) {
+                const double dE = sl == 0 ? run[0] : sl == 1 ? run[1] : sl == 2 ? run[2] : run[3];
+                const int dc = sl == 0 ? dcr[0] : sl == 1 ? dcr[1] : sl == 2 ? dcr[2] : dcr[3];
+                const int m = a.move0 + i;
                 a.ml.dE[m] = dE;
-                ok = us[j] < exp(-a.beta * dE);
-                if (ok && dc >= room[j]) {  // the target leaf is full
+                ok = us[i] < exp(-a.beta * dE);
+                if (ok && dc >= room[i]) {  // the target leaf is full
                     atomicOr(t.err, (int)kErrLeafOverflow);
                     ok = 0;
                 }
                 if (a.ml.accept_out) a.ml.accept_out[m] = ok;
+                accs[i] = ok;
             }
-            ok = __shfl(ok, 0, 64);
-            if (lane == (j & 63) && ok) mine |= 1u << (j >> 6);
+            ok = __builtin_amdgcn_readlane(ok, owner);
+            if (ok) {
+                const int lti = leafT[i], loi = leafO[i];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int j = lane + 64 * k;
+                    if (j > i && j < K) {
+                        run[k] += in_lds ? C[i * K + j] : a.s.corr[i * K + j];
+                        dcr[k] += (lti == leafT[j]) - (loi == leafT[j]);
+                    }
+                }
+            }
         }
-        for (int k = 0; lane + 64 * k < K; ++k) accs[lane + 64 * k] = (mine >> k) & 1u;
     }
     __syncthreads();
     for (int j = threadIdx.x; j < K; j += blockDim.x) a.s.accept[j] = accs[j];
@@ -587,7 +596,8 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
 // stages their phase rows in LDS a tile at a time, and every thread adds them
 // to its element, which it reads and writes once.
 constexpr int kApplyTile = 16;
-constexpr int kApplyPer = 32;  // elements per thread: a work item covers a whole level array (digits <= 6)
+constexpr int kApplyPer = 8;  // elements per thread and work item
+constexpr int kApplyLv = 4;   // levels of the per-block path table (deeper levels read TrialInfo)
 
 template <typename T>
 __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
@@ -596,11 +606,19 @@ __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
     __shared__ int mcount;
     __shared__ Cplx<T> rows[kApplyTile * 3 * kRowStride];
     __shared__ int ac_s[kMaxBatch];
+    __shared__ int pnode[2 * kMaxBatch * kApplyLv];  // (move, endpoint, level-2) -> box, -1 if none
     if (*t.err) return;
     const int nD = a.s.plan[0];
     if (!a.s.plan[1]) return;
     const int K = a.K, D = t.max_depth, nL = n_levels_pw(D), stride = table_stride(D);
     for (int m = threadIdx.x; m < K; m += blockDim.x) ac_s[m] = a.s.accept[m];
+    for (int w = threadIdx.x; w < 2 * K * kApplyLv; w += blockDim.x) {
+        const int li = w % kApplyLv, me = w / kApplyLv, m = me >> 1, ep = me & 1;
+        const TrialInfo& ti = a.s.info[m];
+        const int lam = ep == 0 ? ti.lam_t : ti.lam_o;
+        const int l = 2 + li;
+        pnode[w] = (l <= lam && l <= D - 2) ? (ep == 0 ? ti.path_t[l] : ti.path_o[l]) : -1;
+    }
     constexpr int CH = TB * kApplyPer;
     const int nWb = (t.Mw + CH - 1) / CH, nDb = (t.Md + CH - 1) / CH;
     const int total = nWb + nD * nDb;
@@ -633,10 +651,14 @@ __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
                     if (is_root) {
                         code = 13;
                     } else {
-                        const TrialInfo& ti = a.s.info[m];
-                        const int lam = ep == 0 ? ti.lam_t : ti.lam_o;
-                        if (l <= lam) {
-                            const int B = ep == 0 ? ti.path_t[l] : ti.path_o[l];
+                        int B;
+                        if (li < kApplyLv) {
+                            B = pnode[(2 * m + ep) * kApplyLv + li];
+                        } else {
+                            const TrialInfo& ti = a.s.info[m];
+                            B = l <= (ep == 0 ? ti.lam_t : ti.lam_o) ? (ep == 0 ? ti.path_t[l] : ti.path_o[l]) : -1;
+                        }
+                        if (B >= 0) {
                             if (B == node) {
                                 code = 13;
                             } else if (kind == 1) {
