@@ -115,6 +115,7 @@ def main():
     gp.mc_load(idx[psl], d[psl], u[psl])
     gp.mc_run(1.0, 1, args.batch)
     kt = gp.kernel_times()
+    phases = gp.phase_times()
     _, acc_p = gp.mc_results()
     st_p = gp.mc_stats()
     Mw, Md = g.dim_window, g.dim_diff
@@ -128,7 +129,7 @@ def main():
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                 "bytes_per_launch": int(per_launch), "avg_launch_us": round(avg_ms * 1e3, 3),
-                "kernel_ms": {k: [round(v[0], 3), v[1]] for k, v in kt.items()}}
+                "kernel_ms": {k: [round(v[0], 3), v[1]] for k, v in kt.items()}, "phase_us": phases}
     gp.close()
 
     # plain chain (one trial kernel + one update kernel per move) on the same state
@@ -144,6 +145,15 @@ def main():
     plain_wall = time.perf_counter() - t0
     plain = args.plain_moves / plain_wall
     gpl.close()
+    gpp = P.HPDMKPtTree(params, r, q, prec)
+    gpp.form_outgoing_pw()
+    gpp.form_incoming_pw()
+    gpp.set_profiling(True)
+    gpp.mc_load(idx[psl][:500], d[psl][:500], u[psl][:500])
+    gpp.mc_run(1.0, 0)
+    plain_prof = {"kernel_ms": {k: [round(v[0], 3), v[1]] for k, v in gpp.kernel_times().items()},
+                  "phase_us": gpp.phase_times()}
+    gpp.close()
     print(f"# plain: {plain:.0f} moves/s; batched/plain = {value / plain:.2f}", file=sys.stderr, flush=True)
 
     # accuracy of the arithmetic mode: trial dE of this precision vs the FP64 path on the same state
@@ -191,7 +201,8 @@ def main():
                                "1e4 shifts/step dx~U[-1,1]^3, beta=1",
                    "n": args.n, "L": args.L, "digits": args.digits, "leaf": args.leaf, "batch": args.batch,
                    "moves_per_step": M, "precision": args.precision},
-        "plain_gpu": {"value": round(plain, 2), "unit": "trial moves/s", "moves": args.plain_moves},
+        "plain_gpu": {"value": round(plain, 2), "unit": "trial moves/s", "moves": args.plain_moves,
+                      "profile": plain_prof},
         "batched_over_plain": round(value / plain, 3),
         "acceptance": round(float(acc_b.mean()), 4),
         "roofline": roofline,

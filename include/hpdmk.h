@@ -125,6 +125,11 @@ int hpdmk_mc_stats(hpdmk_tree tree, int* stats);
 int hpdmk_set_profiling(hpdmk_tree tree, int on);
 int hpdmk_kernel_times(hpdmk_tree tree, double* ms, int* count);
 
+/* Average duration (us) of device-side phases of the last profiled run:
+ * 0 batch scan assembly, 1 batch decisions, 2 update marking, 3 update
+ * compaction, 4 particle relocation, 5 single-move finalisation.  us[6], count[6]. */
+int hpdmk_phase_times(hpdmk_tree tree, double* us, int* count);
+
 /* Host-only inspection (no device): the octree the library would build.
  * Returns the node count through *n_nodes; when nodes is non-NULL it must
  * hold 9 ints per node, written in Morton pre-order as: level, is_leaf,

@@ -63,6 +63,13 @@ struct McScratch {
     unsigned char* mark;   // 2 per expansion box (self-resetting)
     int* dest;             // compact list of rewritten arrays: 2*slot + kind (0 out, 1 in)
     int* plan;             // [0] arrays in dest, [1] any move accepted
+    unsigned long long* probe;  // optional phase timers (profiling): kProbes sums + counts
+};
+
+// phase timers of the decision/plan path (wall_clock64 ticks, summed over launches)
+enum Probe : int {
+    kPrScanAssemble = 0, kPrScanDecide, kPrPlanMark, kPrPlanCompact, kPrPlanRelocate, kPrTrialFinal,
+    kProbes
 };
 
 struct MoveList {

@@ -22,6 +22,14 @@ __device__ __forceinline__ Cplx<T>* tabs_of(const McScratch& s, int stride, int 
 
 __device__ __forceinline__ int n_levels_pw(int D) { return D >= 4 ? D - 3 : 0; }
 
+__device__ __forceinline__ void probe_add(const McScratch& s, int k, unsigned long long& t0) {
+    if (!s.probe) return;
+    const unsigned long long t1 = wall_clock64();
+    atomicAdd(s.probe + k, t1 - t0);
+    atomicAdd(s.probe + kProbes + k, 1ull);
+    t0 = t1;
+}
+
 // periodic shift (-1, 0, 1) of a particle in box a so that it sits next to box
 // b (same level, cells ca, cb); returns 2 when the boxes are not neighbours
 __device__ __forceinline__ int image_shift(int ca, int cb, int nl) {
@@ -126,6 +134,7 @@ __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K) {
     __shared__ int base_sh;
     __shared__ int lo_s[kMaxBatch], lt_s[kMaxBatch], ac_s[kMaxBatch], conflict[kMaxBatch];
     const int D = t.max_depth, nL = n_levels_pw(D);
+    unsigned long long tp = wall_clock64();
     for (int m = threadIdx.x; m < K; m += blockDim.x) {
         const TrialInfo& ti = a.s.info[m];
         ac_s[m] = a.s.accept[m];
@@ -161,6 +170,7 @@ __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K) {
         t.pz[s] = ti.t[2];
     }
     __syncthreads();
+    if (threadIdx.x == 0) probe_add(a.s, kPrPlanMark, tp);
     const int n2 = 2 * t.n_pw_nodes;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
     if (threadIdx.x == 0) base_sh = 0;
@@ -184,6 +194,7 @@ __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K) {
             for (int k = 0; k < nw; ++k) base_sh += wtot[k];
         __syncthreads();
     }
+    if (threadIdx.x == 0) probe_add(a.s, kPrPlanCompact, tp);
     // leaf changes (src/tree.cpp:454-463, 583-596): append to the target leaf
     // and swap the origin leaf's last particle into the freed slot.  Changes
     // that share no leaf with another change of the batch run in parallel; the
@@ -209,6 +220,7 @@ __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K) {
     a.s.plan[1] = any;
     for (int m = 0; m < K; ++m)
         if (ac_s[m] && conflict[m]) relocate(t, a.s.info[m]);
+    probe_add(a.s, kPrPlanRelocate, tp);
 }
 
 // capacity check of an accepted move given the count change `delta` that
@@ -383,6 +395,7 @@ __global__ void __launch_bounds__(TB) k_trial(DevTree<T> t, McArgs a) {
     __syncthreads();
     if (!last) return;
     if (threadIdx.x == 0) {
+        unsigned long long tp = wall_clock64();
         __threadfence();
         const double dE = assemble_trial(t, a, tr, m);
         if (a.mode == kTrialOnly && a.ml.dE_stale) a.ml.dE_stale[m] = dE;
@@ -394,6 +407,7 @@ __global__ void __launch_bounds__(TB) k_trial(DevTree<T> t, McArgs a) {
         a.s.accept[tr] = ok;
         if (a.ml.accept_out) a.ml.accept_out[m] = ok;
         a.s.done[tr] = 0;  // ready for the next launch
+        probe_add(a.s, kPrTrialFinal, tp);
     }
     if (a.mode == kTrialOnly) return;
     __syncthreads();
@@ -527,6 +541,7 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
     __shared__ double C[64 * 64];
     if (*t.err) return;
     const int K = a.K;
+    unsigned long long tp = wall_clock64();
     for (int j = threadIdx.x; j < K; j += blockDim.x) {
         const int m = a.move0 + j;
         const double dE = assemble_trial(t, a, j, m);
@@ -542,6 +557,7 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
     if (in_lds)
         for (int q = threadIdx.x; q < K * K; q += blockDim.x) C[q] = a.s.corr[q];
     __syncthreads();
+    if (threadIdx.x == 0) probe_add(a.s, kPrScanAssemble, tp);
     if (threadIdx.x < 64) {
         // lane l owns trials l, l+64, l+128, l+192 and keeps their running
         // corrected dE and leaf-count change; decisions go in trial order
@@ -585,6 +601,7 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
         }
     }
     __syncthreads();
+    if (threadIdx.x == 0) probe_add(a.s, kPrScanDecide, tp);
     for (int j = threadIdx.x; j < K; j += blockDim.x) a.s.accept[j] = accs[j];
     __syncthreads();
     plan_update(t, a, K);

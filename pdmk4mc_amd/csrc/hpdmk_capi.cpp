@@ -76,6 +76,7 @@ struct Handle {
     int* err = nullptr;
     // Monte Carlo scratch (kMaxBatch trials)
     McScratch sc{};
+    unsigned long long* probe = nullptr;
     unsigned* t_done = nullptr;
     // move lists in HBM: `single` serves eval/update calls, `chain` the staged Metropolis run
     MoveBuf single, chain;
@@ -407,6 +408,8 @@ int hpdmk_tree_create_ex(HPDMKParams params, int precision, int n_src, const dou
         h->sc.mark = dalloc<unsigned char>(2 * (size_t)h->n_pw + 2, o);
         h->sc.dest = dalloc<int>(2 * (size_t)h->n_pw + 2, o);
         h->sc.plan = dalloc<int>(2, o);
+        h->probe = dalloc<unsigned long long>(2 * kProbes, o);
+        HIPX(hipMemset(h->probe, 0, sizeof(unsigned long long) * 2 * kProbes));
         HIPX(hipMemset(h->sc.mark, 0, 2 * (size_t)h->n_pw + 2));
         HIPX(hipMemset(h->sc.plan, 0, 2 * sizeof(int)));
         HIPX(hipMemset(h->t_done, 0, sizeof(unsigned) * kMaxBatch));
@@ -512,6 +515,7 @@ void upload_moves(Handle* h, MoveBuf& b, int n, const long* idx, const double* d
 McArgs mc_args(Handle* h, const MoveBuf& b, int move0, int K, int mode, double beta) {
     McArgs a{};
     a.s = h->sc;
+    a.s.probe = h->profiling ? h->probe : nullptr;
     a.ml.idx = b.idx;
     a.ml.d = b.d;
     a.ml.u = b.u;
@@ -626,6 +630,7 @@ int hpdmk_mc_run(hpdmk_tree t, double beta, int mode, int batch, double* gpu_ms)
             h->kms[k] = 0;
             h->kcount[k] = 0;
         }
+        if (h->profiling) HIPX(hipMemsetAsync(h->probe, 0, sizeof(unsigned long long) * 2 * kProbes, h->stream));
         // batches never repeat a particle (the corrections assume distinct particles)
         std::vector<std::pair<int, int>> batches;
         if (mode == 1) {
@@ -765,6 +770,20 @@ int hpdmk_pswf_constants(int digits, double* out) {
         out[4] = p.psi0_0;
         out[5] = p.n_window;
         out[6] = p.n_diff;
+    });
+}
+
+int hpdmk_phase_times(hpdmk_tree t, double* us, int* count) {
+    return guard([&] {
+        Handle* h = H(t);
+        unsigned long long p[2 * kProbes];
+        HIPX(hipMemcpy(p, h->probe, sizeof(p), hipMemcpyDeviceToHost));
+        int freq_khz = 100000;
+        (void)hipDeviceGetAttribute(&freq_khz, hipDeviceAttributeWallClockRate, 0);
+        for (int k = 0; k < kProbes; ++k) {
+            if (us) us[k] = p[kProbes + k] ? (double)p[k] / p[kProbes + k] / (freq_khz * 1e-3) : 0.0;
+            if (count) count[k] = (int)p[kProbes + k];
+        }
     });
 }
 

@@ -53,6 +53,7 @@ SIGNATURES = {
     "hpdmk_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "hpdmk_kernel_times": (C.c_int, [C.c_void_p, _dp, _ip]),
     "hpdmk_clear_error": (C.c_int, [C.c_void_p]),
+    "hpdmk_phase_times": (C.c_int, [C.c_void_p, _dp, _ip]),
     "hpdmk_plan_tree": (C.c_int, [HPDMKParams, C.c_int, _dp, _ip, _ip, _ip]),
     "hpdmk_pswf_constants": (C.c_int, [C.c_int, _dp]),
 }
@@ -230,6 +231,13 @@ class HPDMKPtTree:
         cnt = np.zeros(5, dtype=np.int32)
         _check(load_library().hpdmk_kernel_times(self._h, ms.ctypes.data_as(_dp), cnt.ctypes.data_as(_ip)))
         return dict(zip(["trial", "update", "pair_corr", "scan", "setup"], zip(ms.tolist(), cnt.tolist())))
+
+    def phase_times(self):
+        us = np.zeros(6)
+        cnt = np.zeros(6, dtype=np.int32)
+        _check(load_library().hpdmk_phase_times(self._h, us.ctypes.data_as(_dp), cnt.ctypes.data_as(_ip)))
+        names = ["scan_assemble", "scan_decide", "plan_mark", "plan_compact", "plan_relocate", "trial_final"]
+        return {n: (round(float(u), 2), int(c)) for n, u, c in zip(names, us, cnt)}
 
     def clear_error(self):
         _check(load_library().hpdmk_clear_error(self._h))
