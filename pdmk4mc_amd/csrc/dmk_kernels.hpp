@@ -111,6 +111,8 @@ template <typename T>
 void launch_scan(const DevTree<T>& t, const McArgs& a, hipStream_t st);
 template <typename T>
 void launch_apply(const DevTree<T>& t, const McArgs& a, hipStream_t st);
+template <typename T>
+void launch_compact(const DevTree<T>& t, const int* leaves, int n_leaves, hipStream_t st);
 
 template <typename T>
 void launch_outgoing(const DevTree<T>& t, const PwNodes& pn, double* acc_root, hipStream_t st);

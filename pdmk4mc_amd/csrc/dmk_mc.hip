@@ -22,6 +22,13 @@ __device__ __forceinline__ Cplx<T>* tabs_of(const McScratch& s, int stride, int 
 
 __device__ __forceinline__ int n_levels_pw(int D) { return D >= 4 ? D - 3 : 0; }
 
+// Metropolis test u < exp(-beta dE) written as dE < -ln(u) / beta, one
+// comparison shared by the single-move and the batched decisions
+__device__ __forceinline__ double mh_threshold(double u, double beta) {
+    if (beta <= 0) return INFINITY;
+    return u > 0 ? -log(u) / beta : INFINITY;
+}
+
 __device__ __forceinline__ void probe_add(const McScratch& s, int k, unsigned long long& t0) {
     if (!s.probe) return;
     const unsigned long long t1 = wall_clock64();
@@ -87,40 +94,28 @@ __global__ void __launch_bounds__(TB) k_setup(DevTree<T> t, McArgs a) {
 }
 
 // ------------------------------------------------------------------ update plan
-// moves an accepted particle whose leaf changes into its target leaf's slots
+// Moves an accepted particle whose leaf changes: it is appended to the
+// target leaf at slot_count + rank (rank = earlier movers of the batch into the
+// same leaf) and its old slot becomes a hole (q = 0, porig = -1) until the
+// next compaction (k_compact).
 template <typename T>
-__device__ void relocate(const DevTree<T>& t, const TrialInfo& ti) {
-    const int lo = ti.path_o[ti.lam_o], lt = ti.path_t[ti.lam_t];
-    if (lo == lt) return;
+__device__ void relocate(const DevTree<T>& t, const TrialInfo& ti, int rank) {
+    const int lt = ti.path_t[ti.lam_t];
     const int i = ti.i;
     const int s = t.slot_of[i];
-    const int c = t.slot_count[lt];
-    const int ns = t.slot_start[lt] + c;
-    const int last = t.slot_start[lo] + t.slot_count[lo] - 1;
-    const double q = t.pq[s];
+    const int ns = t.slot_start[lt] + t.slot_count[lt] + rank;
     t.px[ns] = ti.t[0];
     t.py[ns] = ti.t[1];
     t.pz[ns] = ti.t[2];
-    t.pq[ns] = q;
+    t.pq[ns] = t.pq[s];
     t.porig[ns] = i;
-    t.slot_count[lt] = c + 1;
     t.slot_of[i] = ns;
     t.leaf_of[i] = lt;
-    if (last != s) {
-        const int j = t.porig[last];
-        t.px[s] = t.px[last];
-        t.py[s] = t.py[last];
-        t.pz[s] = t.pz[last];
-        t.pq[s] = t.pq[last];
-        t.porig[s] = j;
-        t.slot_of[j] = s;
-    }
-    t.px[last] = 0;
-    t.py[last] = 0;
-    t.pz[last] = 0;
-    t.pq[last] = 0;
-    t.porig[last] = -1;
-    t.slot_count[lo] -= 1;
+    t.px[s] = 0;
+    t.py[s] = 0;
+    t.pz[s] = 0;
+    t.pq[s] = 0;
+    t.porig[s] = -1;
 }
 
 // Marks every expansion array the accepted moves rewrite (out of each path box
@@ -195,31 +190,28 @@ __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K) {
         __syncthreads();
     }
     if (threadIdx.x == 0) probe_add(a.s, kPrPlanCompact, tp);
-    // leaf changes (src/tree.cpp:454-463, 583-596): append to the target leaf
-    // and swap the origin leaf's last particle into the freed slot.  Changes
-    // that share no leaf with another change of the batch run in parallel; the
-    // others run after them, in move order.
+    // leaf changes (src/tree.cpp:454-463, 583-596), all in parallel: each mover
+    // gets the next free slots of its target leaf in move order
     for (int m = threadIdx.x; m < K; m += blockDim.x) {
-        int cf = 0;
-        const int lo = lo_s[m], lt = lt_s[m];
-        if (ac_s[m] && lo != lt)
-            for (int k = 0; k < K && !cf; ++k) {
-                if (k == m || !ac_s[k] || lo_s[k] == lt_s[k]) continue;
-                cf = (lo_s[k] == lo || lo_s[k] == lt || lt_s[k] == lo || lt_s[k] == lt);
-            }
-        conflict[m] = cf;
+        int rank = 0, total = 0;
+        const bool mover = ac_s[m] && lo_s[m] != lt_s[m];
+        if (mover)
+            for (int k = 0; k < K; ++k)
+                if (ac_s[k] && lo_s[k] != lt_s[k] && lt_s[k] == lt_s[m]) {
+                    rank += k < m;
+                    ++total;
+                }
+        conflict[m] = mover ? (rank == 0 ? total : 0) : -1;  // first mover into a leaf carries the count
+        if (mover) relocate(t, a.s.info[m], rank);
     }
     __syncthreads();
     for (int m = threadIdx.x; m < K; m += blockDim.x)
-        if (ac_s[m] && !conflict[m]) relocate(t, a.s.info[m]);
-    __syncthreads();
+        if (conflict[m] > 0) t.slot_count[lt_s[m]] += conflict[m];
     if (threadIdx.x != 0) return;
     int any = 0;
     for (int m = 0; m < K; ++m) any |= ac_s[m];
     a.s.plan[0] = base_sh;
     a.s.plan[1] = any;
-    for (int m = 0; m < K; ++m)
-        if (ac_s[m] && conflict[m]) relocate(t, a.s.info[m]);
     probe_add(a.s, kPrPlanRelocate, tp);
 }
 
@@ -229,7 +221,7 @@ template <typename T>
 __device__ int capacity_ok(const DevTree<T>& t, const TrialInfo& ti, int delta) {
     const int lt = ti.path_t[ti.lam_t], lo = ti.path_o[ti.lam_o];
     if (lt == lo) return 1;
-    if (t.slot_count[lt] + delta < t.slot_cap[lt]) return 1;
+    if (t.slot_count[lt] + delta < t.slot_cap[lt]) return 1;  // holes keep their slots until compaction
     atomicOr(t.err, (int)kErrLeafOverflow);
     return 0;
 }
@@ -401,7 +393,7 @@ __global__ void __launch_bounds__(TB) k_trial(DevTree<T> t, McArgs a) {
         if (a.mode == kTrialOnly && a.ml.dE_stale) a.ml.dE_stale[m] = dE;
         a.ml.dE[m] = dE;
         int ok = 0;
-        if (a.mode == kTrialMetropolis) ok = a.ml.u[m] < exp(-a.beta * dE);
+        if (a.mode == kTrialMetropolis) ok = dE < mh_threshold(a.ml.u[m], a.beta);
         if (a.mode == kTrialForceAccept) ok = 1;
         if (ok) ok = capacity_ok(t, ti, 0);
         a.s.accept[tr] = ok;
@@ -530,14 +522,13 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
 }
 
 // Batch decisions: dE_j = stale_j + sum over the accepted i < j of corr[i][j],
-// taken in order by one wave (lane l owns trials l, l+64, ...); the leaf
-// capacity check follows the counts the earlier acceptances leave.  Then the
+// taken in order by one wave (lane l owns trials l, l+64, ...: it keeps their
+// running corrected dE and the appends into their target leaves); then the
 // update plan of the whole batch.
 template <typename T>
 __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
-    __shared__ double dEs[kMaxBatch];
+    __shared__ double dEs[kMaxBatch], thr[kMaxBatch], dEf[kMaxBatch];
     __shared__ int leafT[kMaxBatch], leafO[kMaxBatch], accs[kMaxBatch], room[kMaxBatch];
-    __shared__ double us[kMaxBatch];
     __shared__ double C[64 * 64];
     if (*t.err) return;
     const int K = a.K;
@@ -551,7 +542,7 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
         leafT[j] = tj.path_t[tj.lam_t];
         leafO[j] = tj.path_o[tj.lam_o];
         room[j] = leafT[j] == leafO[j] ? (1 << 30) : t.slot_cap[leafT[j]] - t.slot_count[leafT[j]];
-        us[j] = a.ml.u[m];
+        thr[j] = mh_threshold(a.ml.u[m], a.beta);
     }
     const bool in_lds = K <= 64;
     if (in_lds)
@@ -559,8 +550,6 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
     __syncthreads();
     if (threadIdx.x == 0) probe_add(a.s, kPrScanAssemble, tp);
     if (threadIdx.x < 64) {
-        // lane l owns trials l, l+64, l+128, l+192 and keeps their running
-        // corrected dE and leaf-count change; decisions go in trial order
         const int lane = threadIdx.x;
         double run[4];
         int dcr[4];
@@ -576,25 +565,19 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
             if (lane == owner) {
                 const double dE = sl == 0 ? run[0] : sl == 1 ? run[1] : sl == 2 ? run[2] : run[3];
                 const int dc = sl == 0 ? dcr[0] : sl == 1 ? dcr[1] : sl == 2 ? dcr[2] : dcr[3];
-                const int m = a.move0 + i;
-                a.ml.dE[m] = dE;
-                ok = us[i] < exp(-a.beta * dE);
-                if (ok && dc >= room[i]) {  // the target leaf is full
-                    atomicOr(t.err, (int)kErrLeafOverflow);
-                    ok = 0;
-                }
-                if (a.ml.accept_out) a.ml.accept_out[m] = ok;
+                ok = dE < thr[i] ? (dc < room[i] ? 1 : 2) : 0;  // 2: the target leaf is full
+                dEf[i] = dE;
                 accs[i] = ok;
             }
-            ok = __builtin_amdgcn_readlane(ok, owner);
+            ok = __builtin_amdgcn_readlane(ok, owner) == 1;
             if (ok) {
-                const int lti = leafT[i], loi = leafO[i];
+                const int lti = leafT[i], mv = leafO[i] != lti;
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const int j = lane + 64 * k;
                     if (j > i && j < K) {
                         run[k] += in_lds ? C[i * K + j] : a.s.corr[i * K + j];
-                        dcr[k] += (lti == leafT[j]) - (loi == leafT[j]);
+                        dcr[k] += mv && (lti == leafT[j]);
                     }
                 }
             }
@@ -602,9 +585,74 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
     }
     __syncthreads();
     if (threadIdx.x == 0) probe_add(a.s, kPrScanDecide, tp);
-    for (int j = threadIdx.x; j < K; j += blockDim.x) a.s.accept[j] = accs[j];
+    for (int j = threadIdx.x; j < K; j += blockDim.x) {
+        const int m = a.move0 + j;
+        int ok = accs[j];
+        if (ok == 2) {
+            atomicOr(t.err, (int)kErrLeafOverflow);
+            ok = 0;
+        }
+        a.ml.dE[m] = dEf[j];
+        a.s.accept[j] = ok;
+        if (a.ml.accept_out) a.ml.accept_out[m] = ok;
+    }
     __syncthreads();
     plan_update(t, a, K);
+}
+
+// Packs the live particles of every leaf (porig >= 0) to the front of its
+// slots, in slot order, after relocations left holes behind.  One block per leaf.
+template <typename T>
+__global__ void __launch_bounds__(TB) k_compact(DevTree<T> t, const int* leaves) {
+    __shared__ int wtot[TB / 64], base;
+    if (*t.err) return;
+    const int leaf = leaves[blockIdx.x];
+    const int s0 = t.slot_start[leaf], n = t.slot_count[leaf];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (threadIdx.x == 0) base = 0;
+    __syncthreads();
+    // chunks are processed in order; every chunk is read completely before any write
+    for (int c0 = 0; c0 < n; c0 += blockDim.x) {
+        const int s = s0 + c0 + threadIdx.x;
+        const bool in = c0 + threadIdx.x < n;
+        const int o = in ? t.porig[s] : -1;
+        double x = 0, y = 0, z = 0, q = 0;
+        if (o >= 0) {
+            x = t.px[s];
+            y = t.py[s];
+            z = t.pz[s];
+            q = t.pq[s];
+        }
+        const bool live = o >= 0;
+        const unsigned long long bal = __ballot(live);
+        const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        if (lane == 0) wtot[wv] = __popcll(bal);
+        __syncthreads();
+        int off = base;
+        for (int k = 0; k < wv; ++k) off += wtot[k];
+        const int dst = s0 + off + __popcll(bal & lt);
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int k = 0; k < (int)(blockDim.x >> 6); ++k) base += wtot[k];
+        if (live) {
+            t.px[dst] = x;
+            t.py[dst] = y;
+            t.pz[dst] = z;
+            t.pq[dst] = q;
+            t.porig[dst] = o;
+            t.slot_of[o] = dst;
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    for (int s = s0 + base + threadIdx.x; s < s0 + n; s += blockDim.x) {
+        t.px[s] = 0;
+        t.py[s] = 0;
+        t.pz[s] = 0;
+        t.pq[s] = 0;
+        t.porig[s] = -1;
+    }
+    if (threadIdx.x == 0) t.slot_count[leaf] = base;
 }
 
 // ------------------------------------------------------------------ apply
@@ -793,7 +841,13 @@ void launch_apply(const DevTree<T>& t, const McArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_apply<T>, dim3(1024), dim3(TB), 0, st, t, a);
 }
 
+template <typename T>
+void launch_compact(const DevTree<T>& t, const int* leaves, int n_leaves, hipStream_t st) {
+    if (n_leaves > 0) hipLaunchKernelGGL(k_compact<T>, dim3(n_leaves), dim3(TB), 0, st, t, leaves);
+}
+
 #define INSTANTIATE(T)                                                                \
+    template void launch_compact<T>(const DevTree<T>&, const int*, int, hipStream_t); \
     template void launch_setup<T>(const DevTree<T>&, const McArgs&, hipStream_t);     \
     template void launch_trial<T>(const DevTree<T>&, const McArgs&, hipStream_t);     \
     template void launch_pair_corr<T>(const DevTree<T>&, const McArgs&, hipStream_t); \

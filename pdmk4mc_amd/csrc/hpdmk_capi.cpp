@@ -56,6 +56,12 @@ struct MoveBuf {
     int* stats = nullptr;
 };
 
+// Moved particles leave holes in their old leaf slots; the slots are packed
+// again after at most this many moves.  Leaf capacity is twice the initial
+// count (at least count + 64), so a leaf would need 64 net arrivals in between
+// to run out.
+constexpr int kCompactEvery = 2048;
+
 enum KernelKind { kKTrial = 0, kKUpdate = 1, kKCorr = 2, kKScan = 3, kKSetup = 4, kKinds = 5 };
 
 struct Handle {
@@ -80,6 +86,7 @@ struct Handle {
     unsigned* t_done = nullptr;
     // move lists in HBM: `single` serves eval/update calls, `chain` the staged Metropolis run
     MoveBuf single, chain;
+    int since_compact = 0;  // leaf changes (upper bound) since the last slot compaction
     // profiling
     bool profiling = false;
     double kms[kKinds] = {0, 0, 0, 0, 0};
@@ -563,6 +570,10 @@ int hpdmk_update_shift(hpdmk_tree t, long i, double dx, double dy, double dz) {
             launch_setup(dv, a, h->stream);
             launch_trial(dv, a, h->stream);
             launch_apply(dv, a, h->stream);
+            if (++h->since_compact >= kCompactEvery) {
+                launch_compact(dv, h->leaves, h->n_leaves, h->stream);
+                h->since_compact = 0;
+            }
         });
         HIPX(hipGetLastError());
         check_err(h);
@@ -651,12 +662,20 @@ int hpdmk_mc_run(hpdmk_tree t, double beta, int mode, int batch, double* gpu_ms)
         HIPX(hipEventCreate(&e1));
         HIPX(hipEventRecord(e0, h->stream));
         dispatch(h, [&](auto& dv) {
+            auto compact_if_due = [&](int moved) {
+                h->since_compact += moved;
+                if (h->since_compact >= kCompactEvery) {
+                    launch_compact(dv, h->leaves, h->n_leaves, h->stream);
+                    h->since_compact = 0;
+                }
+            };
             if (mode == 0) {
                 for (int m = 0; m < n; ++m) {
                     const McArgs a = mc_args(h, mb, m, 1, kTrialMetropolis, beta);
                     timed(h, kKSetup, [&] { launch_setup(dv, a, h->stream); });
                     timed(h, kKTrial, [&] { launch_trial(dv, a, h->stream); });
                     timed(h, kKUpdate, [&] { launch_apply(dv, a, h->stream); });
+                    compact_if_due(1);
                 }
             } else {
                 for (auto& bt : batches) {
@@ -667,6 +686,7 @@ int hpdmk_mc_run(hpdmk_tree t, double beta, int mode, int batch, double* gpu_ms)
                     if (k > 1) timed(h, kKCorr, [&] { launch_pair_corr(dv, a, h->stream); });
                     timed(h, kKScan, [&] { launch_scan(dv, a, h->stream); });
                     timed(h, kKUpdate, [&] { launch_apply(dv, a, h->stream); });
+                    compact_if_due(k);
                 }
             }
         });
