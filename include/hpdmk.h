@@ -127,7 +127,9 @@ int hpdmk_kernel_times(hpdmk_tree tree, double* ms, int* count);
 
 /* Average duration (us) of device-side phases of the last profiled run:
  * 0 batch scan assembly, 1 batch decisions, 2 update marking, 3 update
- * compaction, 4 particle relocation, 5 single-move finalisation.  us[6], count[6]. */
+ * compaction, 4 particle relocation, 5 single-move finalisation, 6 update
+ * contribution lists, 7 update accumulation, 8 update write-back (per work
+ * item).  us[9], count[9]. */
 int hpdmk_phase_times(hpdmk_tree tree, double* us, int* count);
 
 /* Host-only inspection (no device): the octree the library would build.

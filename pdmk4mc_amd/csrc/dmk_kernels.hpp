@@ -69,6 +69,7 @@ struct McScratch {
 // phase timers of the decision/plan path (wall_clock64 ticks, summed over launches)
 enum Probe : int {
     kPrScanAssemble = 0, kPrScanDecide, kPrPlanMark, kPrPlanCompact, kPrPlanRelocate, kPrTrialFinal,
+    kPrApplyMatch, kPrApplyTiles, kPrApplyRmw,
     kProbes
 };
 

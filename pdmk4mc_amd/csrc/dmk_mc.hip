@@ -312,6 +312,7 @@ __global__ void __launch_bounds__(TB) k_trial(DevTree<T> t, McArgs a) {
     if (b < kWinBlocks) {
         // window: 2 cW sum W Re[(tau - omega) conj(rho - omega)] (include/vecops.hpp:118-145)
         const int dw = t.dw;
+#pragma unroll 4
         for (int e = b * TB + threadIdx.x; e < t.Mw; e += kWinBlocks * TB) {
             const Cplx<T> ta = phase_at(tb + root_row(0, 0), tb + root_row(0, 1), tb + root_row(0, 2), dw, e);
             const Cplx<T> om = phase_at(tb + root_row(1, 0), tb + root_row(1, 1), tb + root_row(1, 2), dw, e);
@@ -334,6 +335,7 @@ __global__ void __launch_bounds__(TB) k_trial(DevTree<T> t, McArgs a) {
             const Cplx<T>* X = tb + level_row(nL, ep, li, 0, 0);
             const Cplx<T>* Y = tb + level_row(nL, ep, li, 1, 0);
             const Cplx<T>* Z = tb + level_row(nL, ep, li, 2, 0);
+#pragma unroll 4
             for (int e = (r % kDiffBlocks) * TB + threadIdx.x; e < t.Md; e += kDiffBlocks * TB) {
                 const Cplx<T> v = in[e];
                 const Cplx<T> w = phase_at(X, Y, Z, t.dd, e);
@@ -440,6 +442,7 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
         const int dw = t.dw;
         const Cplx<T>* R = rows;
         double part = 0;
+#pragma unroll 4
         for (int e = threadIdx.x; e < t.Mw; e += blockDim.x) {
             const Cplx<T> x1 = phase_at(R, R + kRowStride, R + 2 * kRowStride, dw, e);
             const Cplx<T> x2 = phase_at(R + 3 * kRowStride, R + 4 * kRowStride, R + 5 * kRowStride, dw, e);
@@ -485,6 +488,7 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
             const int dd = t.dd;
             const Cplx<T>* R = rows + 12 * kRowStride;
             double part = 0;
+#pragma unroll 4
             for (int e = threadIdx.x; e < t.Md; e += blockDim.x) {
                 const Cplx<T> w = phase_at(R, R + kRowStride, R + 2 * kRowStride, dd, e);
                 // delta in = conj(tau_i s) - conj(omega_i s'); absent endpoints have zero rows
@@ -704,6 +708,7 @@ __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
             chunk = r % nDb;
         }
         const int nl = 1 << l;
+        unsigned long long tp = wall_clock64();
         __syncthreads();  // previous item's lists and rows fully consumed
         if (threadIdx.x == 0) mcount = 0;
         __syncthreads();
@@ -751,6 +756,7 @@ __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
             __syncthreads();
         }
         const int nm = mcount;
+        if (threadIdx.x == 0) probe_add(a.s, kPrApplyMatch, tp);
         const int d = is_root ? t.dw : t.dd;
         const int M = is_root ? t.Mw : t.Md;
         int idx[kApplyPer];  // packed i | j << 8 | k << 16 of each element, -1 beyond the array
@@ -788,6 +794,7 @@ __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
                 }
             }
         }
+        if (threadIdx.x == 0) probe_add(a.s, kPrApplyTiles, tp);
 #pragma unroll
         for (int q = 0; q < kApplyPer; ++q) {
             const int e = chunk * CH + q * TB + threadIdx.x;
@@ -811,6 +818,7 @@ __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
                 t.inc[at] = o;
             }
         }
+        if (threadIdx.x == 0) probe_add(a.s, kPrApplyRmw, tp);
     }
 }
 

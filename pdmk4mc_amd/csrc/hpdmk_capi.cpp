@@ -793,7 +793,7 @@ int hpdmk_pswf_constants(int digits, double* out) {
     });
 }
 
-int hpdmk_phase_times(hpdmk_tree t, double* us, int* count) {
+int hpdmk_phase_times(hpdmk_tree t, double* us, int* count) {  // us[kProbes], count[kProbes]
     return guard([&] {
         Handle* h = H(t);
         unsigned long long p[2 * kProbes];
