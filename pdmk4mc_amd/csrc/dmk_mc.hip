@@ -434,8 +434,20 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
     const Cplx<T>* ta = tabs_of<T>(a.s, stride, i);
     const Cplx<T>* tj = tabs_of<T>(a.s, stride, j);
     // rows 0-5: j's root rows (t, o); 6-11: i's root rows
-    for (int q = threadIdx.x; q < 12 * kRowStride; q += blockDim.x)
-        rows[q] = q < 6 * kRowStride ? tj[q] : ta[q - 6 * kRowStride];
+    {
+        constexpr int R = (12 * kRowStride + TB - 1) / TB;
+        Cplx<T> v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int q = threadIdx.x + r * TB;
+            if (q < 12 * kRowStride) v[r] = q < 6 * kRowStride ? tj[q] : ta[q - 6 * kRowStride];
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int q = threadIdx.x + r * TB;
+            if (q < 12 * kRowStride) rows[q] = v[r];
+        }
+    }
     __syncthreads();
     double total = 0;
     {
@@ -472,16 +484,29 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
             }
             if (!on[0] && !on[1]) continue;  // uniform across the block
             // rows 12-14: j's rows at this level; 15-17 / 18-20: i's shifted target / origin rows
-            for (int q = threadIdx.x; q < 9 * kRowStride; q += blockDim.x) {
-                const int rr = q / kRowStride, mm = q % kRowStride, ax = rr % 3;
-                Cplx<T> v = {T(0), T(0)};
-                if (rr < 3)
-                    v = tj[level_row(nL, ep, li, ax, 0) + mm];
-                else if (rr < 6 && on[0])
-                    v = ta[level_row(nL, 0, li, ax, sh[0][ax]) + mm];
-                else if (rr >= 6 && on[1])
-                    v = ta[level_row(nL, 1, li, ax, sh[1][ax]) + mm];
-                rows[12 * kRowStride + q] = v;
+            {
+                constexpr int R = (9 * kRowStride + TB - 1) / TB;
+                Cplx<T> vv[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int q = threadIdx.x + r * TB;
+                    const int rr = q / kRowStride, mm = q % kRowStride, ax = rr % 3;
+                    Cplx<T> v = {T(0), T(0)};
+                    if (q < 9 * kRowStride) {
+                        if (rr < 3)
+                            v = tj[level_row(nL, ep, li, ax, 0) + mm];
+                        else if (rr < 6 && on[0])
+                            v = ta[level_row(nL, 0, li, ax, sh[0][ax]) + mm];
+                        else if (rr >= 6 && on[1])
+                            v = ta[level_row(nL, 1, li, ax, sh[1][ax]) + mm];
+                    }
+                    vv[r] = v;
+                }
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int q = threadIdx.x + r * TB;
+                    if (q < 9 * kRowStride) rows[12 * kRowStride + q] = vv[r];
+                }
             }
             __syncthreads();
             const T* Dl = t.Dmat + (size_t)li * t.Md;
@@ -549,8 +574,19 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
         thr[j] = mh_threshold(a.ml.u[m], a.beta);
     }
     const bool in_lds = K <= 64;
-    if (in_lds)
-        for (int q = threadIdx.x; q < K * K; q += blockDim.x) C[q] = a.s.corr[q];
+    if (in_lds) {
+        double v[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int q = threadIdx.x + r * TB;
+            if (q < K * K) v[r] = a.s.corr[q];
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int q = threadIdx.x + r * TB;
+            if (q < K * K) C[q] = v[r];
+        }
+    }
     __syncthreads();
     if (threadIdx.x == 0) probe_add(a.s, kPrScanAssemble, tp);
     if (threadIdx.x < 64) {
@@ -772,13 +808,27 @@ __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
         for (int k0 = 0; k0 < nm; k0 += kApplyTile) {
             const int nt = nm - k0 < kApplyTile ? nm - k0 : kApplyTile;
             __syncthreads();
-            for (int q = threadIdx.x; q < nt * 3 * kRowStride; q += blockDim.x) {
-                const int k = q / (3 * kRowStride), ax = (q / kRowStride) % 3, mm = q % kRowStride;
-                const int c = mlist[k0 + k];
-                const int m = c >> 8, ep = (c >> 7) & 1, code = c & 127;
-                const int sh = ax == 0 ? code % 3 - 1 : (ax == 1 ? (code / 3) % 3 - 1 : code / 9 - 1);
-                const Cplx<T>* tb = tabs_of<T>(a.s, stride, m);
-                rows[q] = tb[(is_root ? root_row(ep, ax) : level_row(nL, ep, li, ax, sh)) + mm];
+            {
+                // all loads of the tile are issued before the LDS stores
+                constexpr int R = (kApplyTile * 3 * kRowStride + TB - 1) / TB;
+                Cplx<T> v[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int q = threadIdx.x + r * TB;
+                    if (q < nt * 3 * kRowStride) {
+                        const int k = q / (3 * kRowStride), ax = (q / kRowStride) % 3, mm = q % kRowStride;
+                        const int c = mlist[k0 + k];
+                        const int m = c >> 8, ep = (c >> 7) & 1, code = c & 127;
+                        const int sh = ax == 0 ? code % 3 - 1 : (ax == 1 ? (code / 3) % 3 - 1 : code / 9 - 1);
+                        const Cplx<T>* tb = tabs_of<T>(a.s, stride, m);
+                        v[r] = tb[(is_root ? root_row(ep, ax) : level_row(nL, ep, li, ax, sh)) + mm];
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int q = threadIdx.x + r * TB;
+                    if (q < nt * 3 * kRowStride) rows[q] = v[r];
+                }
             }
             __syncthreads();
             for (int k = 0; k < nt; ++k) {
@@ -795,28 +845,19 @@ __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
             }
         }
         if (threadIdx.x == 0) probe_add(a.s, kPrApplyTiles, tp);
+        // read-modify-write: every load of the item first, then every store
+        Cplx<T>* dst = is_root ? t.root : (kind == 0 ? t.out : t.inc) + (size_t)slot * t.Md;
+        const T sy = (is_root || kind == 0) ? T(1) : T(-1);  // in[C] += conj(tau s)
+        Cplx<T> cur[kApplyPer];
 #pragma unroll
         for (int q = 0; q < kApplyPer; ++q) {
             const int e = chunk * CH + q * TB + threadIdx.x;
-            if (e >= M) continue;
-            if (is_root) {  // root += tau - omega (src/tree.cpp:479-480)
-                Cplx<T> g = t.root[e];
-                g.x += ar[q];
-                g.y += ai[q];
-                t.root[e] = g;
-            } else if (kind == 0) {  // out[B] += tau
-                const size_t at = (size_t)slot * t.Md + e;
-                Cplx<T> o = t.out[at];
-                o.x += ar[q];
-                o.y += ai[q];
-                t.out[at] = o;
-            } else {  // in[C] += conj(tau s)
-                const size_t at = (size_t)slot * t.Md + e;
-                Cplx<T> o = t.inc[at];
-                o.x += ar[q];
-                o.y -= ai[q];
-                t.inc[at] = o;
-            }
+            if (e < M) cur[q] = dst[e];
+        }
+#pragma unroll
+        for (int q = 0; q < kApplyPer; ++q) {
+            const int e = chunk * CH + q * TB + threadIdx.x;
+            if (e < M) dst[e] = {cur[q].x + ar[q], cur[q].y + sy * ai[q]};
         }
         if (threadIdx.x == 0) probe_add(a.s, kPrApplyRmw, tp);
     }
