@@ -27,6 +27,30 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def measured_traffic(path, kernel):
+    """HBM bytes per launch of `kernel` from a rocprofv3 --pmc summary (JSON
+    written from FETCH_SIZE and WRITE_SIZE runs of this workload), or None."""
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if rec.get("kernel") != kernel:
+        return None
+    return rec.get("bytes_per_launch")
+
+
 def workload(n, L, n_moves, seed):
     rng = np.random.default_rng(seed)
     r = rng.uniform(0, L, (n, 3))
@@ -62,10 +86,18 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--plain-moves", type=int, default=3000)
-    ap.add_argument("--cpu-moves", type=int, default=4000)
+    ap.add_argument("--cpu-moves", type=int, default=12000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--legs", default="batched,plain,accuracy,cpu,copy",
+                    help="comma list of batched, plain, accuracy, cpu, copy (copy = a 1 GiB streaming "
+                         "copy: measured HBM ceiling and byte-counter calibration)")
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_update_batched_n1e5.json"),
+                    help="measured HBM bytes per launch of the dominant kernel (from rocprofv3 --pmc)")
     args = ap.parse_args()
+    legs = set(args.legs.split(","))
+    if args.no_cpu:
+        legs.discard("cpu")
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); independent replicas shard "
               "trivially across GPUs and no multi-GPU figure is produced", file=sys.stderr)
@@ -77,7 +109,8 @@ def main():
     bc, br, bn = (8, 4, 16) if prec == P.HPDMK_MIXED else (16, 8, 32)
     M = args.moves_per_step
     n_total = (args.warmup + args.steps) * M
-    r, q, idx, d, u = workload(args.n, args.L, n_total + args.plain_moves, args.seed)
+    n_extra = max(args.plain_moves, args.cpu_moves)
+    r, q, idx, d, u = workload(args.n, args.L, n_total + n_extra, args.seed)
     params = P.HPDMKParams(args.leaf, args.digits, args.L, init=P.DIRECT)
 
     t0 = time.time()
@@ -105,19 +138,28 @@ def main():
     print(f"# batched: {moves} moves in {wall:.3f}s (gpu {gpu_ms / 1e3:.3f}s) = {value:.0f} moves/s, "
           f"acceptance {acc_b.mean():.3f}", file=sys.stderr, flush=True)
 
-    # per-kernel times on a profiled segment (events around every launch)
+    # per-kernel times on a profiled segment: start/end timestamps of every
+    # dispatch (level 1: no device-side timers in the kernels)
     prof_moves = min(M, 4000)
     psl = slice(n_total - prof_moves, n_total)
-    gp = P.HPDMKPtTree(params, r, q, prec)
-    gp.form_outgoing_pw()
-    gp.form_incoming_pw()
-    gp.set_profiling(True)
-    gp.mc_load(idx[psl], d[psl], u[psl])
-    gp.mc_run(1.0, 1, args.batch)
+
+    def profiled(level):
+        gp = P.HPDMKPtTree(params, r, q, prec)
+        gp.form_outgoing_pw()
+        gp.form_incoming_pw()
+        gp.set_profiling(level)
+        gp.mc_load(idx[psl], d[psl], u[psl])
+        gp.mc_run(1.0, 1, args.batch)
+        return gp
+
+    gp = profiled(1)
     kt = gp.kernel_times()
-    phases = gp.phase_times()
     _, acc_p = gp.mc_results()
     st_p = gp.mc_stats()
+    gp.close()
+    gp = profiled(2)  # device phase timers (they slow the kernels that carry them)
+    phases = gp.phase_times()
+    gp.close()
     Mw, Md = g.dim_window, g.dim_diff
     kb = {"trial": trial_bytes(st_p, Mw, Md, bc, br, bn).sum(),
           "update": accept_bytes(st_p, acc_p, Mw, Md, bc).sum()}
@@ -126,39 +168,55 @@ def main():
     avg_ms = ms_tot / max(cnt, 1)
     per_launch = kb.get(dom, 0.0) / max(cnt, 1)
     achieved = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+    traffic = measured_traffic(args.traffic_file, dom)
+    symbol = {"trial": "k_trial", "update": "k_apply", "pair_corr": "k_pair_corr", "scan": "k_scan",
+              "setup": "k_setup"}
+    roofline = {"bound": "hbm", "kernel": dom, "kernel_symbol": symbol.get(dom, dom), "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "bytes_per_launch": int(per_launch), "avg_launch_us": round(avg_ms * 1e3, 3),
                 "kernel_ms": {k: [round(v[0], 3), v[1]] for k, v in kt.items()}, "phase_us": phases}
-    gp.close()
 
-    # plain chain (one trial kernel + one update kernel per move) on the same state
-    gpl = P.HPDMKPtTree(params, r, q, prec)
-    gpl.form_outgoing_pw()
-    gpl.form_incoming_pw()
+    # plain chain (one trial kernel + one update kernel per move).  The timed
+    # run follows a 200-move warmup; the profiled run starts from the fresh
+    # state on the same moves as the CPU baseline, whose decisions it checks.
+    plain = None
+    plain_info = None
+    acc_plain_ref = None
     psl = slice(n_total, n_total + args.plain_moves)
-    gpl.mc_load(idx[psl][:200], d[psl][:200], u[psl][:200])
-    gpl.mc_run(1.0, 0)
-    gpl.mc_load(idx[psl], d[psl], u[psl])
-    t0 = time.perf_counter()
-    gpl.mc_run(1.0, 0)
-    plain_wall = time.perf_counter() - t0
-    plain = args.plain_moves / plain_wall
-    gpl.close()
-    gpp = P.HPDMKPtTree(params, r, q, prec)
-    gpp.form_outgoing_pw()
-    gpp.form_incoming_pw()
-    gpp.set_profiling(True)
-    gpp.mc_load(idx[psl][:500], d[psl][:500], u[psl][:500])
-    gpp.mc_run(1.0, 0)
-    plain_prof = {"kernel_ms": {k: [round(v[0], 3), v[1]] for k, v in gpp.kernel_times().items()},
-                  "phase_us": gpp.phase_times()}
-    gpp.close()
-    print(f"# plain: {plain:.0f} moves/s; batched/plain = {value / plain:.2f}", file=sys.stderr, flush=True)
+    if "plain" in legs:
+        gpl = P.HPDMKPtTree(params, r, q, prec)
+        gpl.form_outgoing_pw()
+        gpl.form_incoming_pw()
+        wsl = slice(n_total + args.plain_moves, n_total + args.plain_moves + 200)
+        gpl.mc_load(idx[wsl], d[wsl], u[wsl])
+        gpl.mc_run(1.0, 0)
+        gpl.mc_load(idx[psl], d[psl], u[psl])
+        t0 = time.perf_counter()
+        gpl.mc_run(1.0, 0)
+        plain_wall = time.perf_counter() - t0
+        plain = args.plain_moves / plain_wall
+        gpl.close()
+        gpp = P.HPDMKPtTree(params, r, q, prec)
+        gpp.form_outgoing_pw()
+        gpp.form_incoming_pw()
+        gpp.set_profiling(1)
+        n_pp = min(500, args.plain_moves)
+        gpp.mc_load(idx[psl][:n_pp], d[psl][:n_pp], u[psl][:n_pp])
+        gpp.mc_run(1.0, 0)
+        _, acc_plain_ref = gpp.mc_results()
+        plain_kt = gpp.kernel_times()
+        gpp.set_profiling(2)
+        gpp.mc_load(idx[psl][n_pp:2 * n_pp], d[psl][n_pp:2 * n_pp], u[psl][n_pp:2 * n_pp])
+        gpp.mc_run(1.0, 0)
+        plain_info = {"value": round(plain, 2), "unit": "trial moves/s", "moves": args.plain_moves,
+                      "profile": {"kernel_ms": {k: [round(v[0], 3), v[1]] for k, v in plain_kt.items()},
+                                  "phase_us": gpp.phase_times()}}
+        gpp.close()
+        print(f"# plain: {plain:.0f} moves/s; batched/plain = {value / plain:.2f}", file=sys.stderr, flush=True)
 
     # accuracy of the arithmetic mode: trial dE of this precision vs the FP64 path on the same state
     acc_info = None
-    if prec == P.HPDMK_MIXED:
+    if prec == P.HPDMK_MIXED and "accuracy" in legs:
         g64 = P.HPDMKPtTree(params, r, q, P.HPDMK_FP64)
         g64.form_outgoing_pw()
         g64.form_incoming_pw()
@@ -176,20 +234,31 @@ def main():
         gm.close()
 
     cpu = None
-    if not args.no_cpu:
+    if "cpu" in legs:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as O
 
         S = args.cpu_moves
         t = O.OracleTree(r, q, args.L, n_per_leaf=args.leaf, digits=args.digits, precision=64, fast=True)
         t.form_pw()
-        sl = slice(args.warmup * M, args.warmup * M + S)
-        dE_c, acc_c, t2 = t.mc_chain(idx[sl], d[sl], u[sl], 1.0)
+        csl = slice(n_total, n_total + S)
+        dE_c, acc_c, t2 = t.mc_chain(idx[csl], d[csl], u[csl], 1.0)
         cpu = {"value": round(S / t2.sum(), 2), "unit": "trial moves/s", "cores": 1, "kind": "port",
-               "sample": f"first {S} moves of the timed chain, FP64, oracle built -O3 -ffast-math, one thread",
-               "eval_us": round(t2[0] / S * 1e6, 2), "update_us_per_accept": round(t2[1] / max(1, acc_c.sum()) * 1e6, 2),
-               "agree_with_gpu_decisions": float((acc_c == acc_b[:S]).mean())}
+               "sample": f"{S} moves of a chain from the initial state (same system and move distribution), "
+                         "FP64, oracle built -O3 -ffast-math, one thread",
+               "cpu_model": cpu_model(),
+               "eval_us": round(t2[0] / S * 1e6, 2), "update_us_per_accept": round(t2[1] / max(1, acc_c.sum()) * 1e6, 2)}
+        if acc_plain_ref is not None:
+            n_cmp = len(acc_plain_ref)
+            cpu["decisions_equal_to_plain_gpu"] = f"{int((acc_c[:n_cmp] == acc_plain_ref).sum())}/{n_cmp}"
         print(f"# cpu: {cpu}", file=sys.stderr, flush=True)
+
+    copy_gbs = None
+    if "copy" in legs:
+        # streaming copy of 1 GiB: the measured HBM ceiling, and the
+        # calibration kernel for the FETCH_SIZE / WRITE_SIZE byte counters
+        copy_gbs = P.copy_probe(1 << 30, 5)
+        print(f"# copy probe: {copy_gbs:.0f} GB/s read+write", file=sys.stderr, flush=True)
 
     out = {
         "metric": "dE trial moves/sec at 1e-6 rel err, N=1e5; batched+learned vs plain GPU",
@@ -201,11 +270,10 @@ def main():
                                "1e4 shifts/step dx~U[-1,1]^3, beta=1",
                    "n": args.n, "L": args.L, "digits": args.digits, "leaf": args.leaf, "batch": args.batch,
                    "moves_per_step": M, "precision": args.precision},
-        "plain_gpu": {"value": round(plain, 2), "unit": "trial moves/s", "moves": args.plain_moves,
-                      "profile": plain_prof},
-        "batched_over_plain": round(value / plain, 3),
+        "plain_gpu": plain_info,
+        "batched_over_plain": round(value / plain, 3) if plain else None,
         "acceptance": round(float(acc_b.mean()), 4),
-        "roofline": roofline,
+        "roofline": dict(roofline, measured_copy_gbs=round(copy_gbs, 1) if copy_gbs else None),
         "cpu_baseline": cpu,
         "accuracy": acc_info,
     }

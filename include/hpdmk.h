@@ -119,10 +119,13 @@ int hpdmk_mc_results(hpdmk_tree tree, double* dE, int* accept);
  * paths.  They size the algorithmic bytes of each move. */
 int hpdmk_mc_stats(hpdmk_tree tree, int* stats);
 
-/* Per-kernel GPU time of the last hpdmk_mc_run with profiling on (events
- * around every launch).  kinds: 0 trial, 1 update (apply), 2 pair
- * corrections, 3 decision scan, 4 trial set-up.  ms[5], count[5]. */
-int hpdmk_set_profiling(hpdmk_tree tree, int on);
+/* Profiling level of later hpdmk_mc_run calls: 0 off; 1 kernel times
+ * (start/end timestamps of every dispatch) and per-move counts; 2 also the
+ * device-side phase timers, which slow the kernels that carry them.
+ * hpdmk_kernel_times: per-kernel GPU time of the last profiled run, kinds
+ * 0 trial, 1 update (apply), 2 pair corrections, 3 decision scan, 4 trial
+ * set-up.  ms[5], count[5]. */
+int hpdmk_set_profiling(hpdmk_tree tree, int level);
 int hpdmk_kernel_times(hpdmk_tree tree, double* ms, int* count);
 
 /* Average duration (us) of device-side phases of the last profiled run:
@@ -142,6 +145,12 @@ int hpdmk_plan_tree(HPDMKParams params, int n_src, const double* r_src, int* n_n
 /* PSWF constants of a digits setting: c, lambda, C0, diff0, psi0(0),
  * n_window, n_diff (out[7]). */
 int hpdmk_pswf_constants(int digits, double* out);
+
+/* HBM streaming-copy probe (no counterpart in the reference; a measurement
+ * aid): copies `bytes` between two fresh device buffers `reps` times and
+ * returns read+write GB/s.  Each launch reads and writes `bytes` once, which
+ * calibrates hardware byte counters (FETCH_SIZE, WRITE_SIZE). */
+int hpdmk_copy_probe(long bytes, int reps, double* gbs);
 
 /* Clears a sticky device error (HPDMK_ECAPACITY) after the caller rebuilt or
  * gave up on the chain. */

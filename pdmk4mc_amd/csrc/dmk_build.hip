@@ -199,6 +199,17 @@ __global__ void k_zero(double* p, int n) {
     if (i < n) p[i] = 0.0;
 }
 
+// streaming copy of n 16-byte words (HBM bandwidth probe and byte-counter
+// calibration): each word is read once and written once
+__global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+void launch_copy16(const void* src, void* dst, size_t n16, hipStream_t st) {
+    hipLaunchKernelGGL(k_copy16, dim3(4096), dim3(TB), 0, st, (const uint4*)src, (uint4*)dst, n16);
+}
+
 template <typename T>
 void launch_outgoing(const DevTree<T>& t, const PwNodes& pn, double* acc_root, hipStream_t st) {
     hipLaunchKernelGGL(k_zero, dim3((2 * t.Mw + TB - 1) / TB), dim3(TB), 0, st, acc_root, 2 * t.Mw);

@@ -103,17 +103,24 @@ struct McArgs {
 };
 
 template <typename T>
-void launch_setup(const DevTree<T>& t, const McArgs& a, hipStream_t st);
+void launch_setup(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0 = nullptr,
+                  hipEvent_t e1 = nullptr);
 template <typename T>
-void launch_trial(const DevTree<T>& t, const McArgs& a, hipStream_t st);
+void launch_trial(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0 = nullptr,
+                  hipEvent_t e1 = nullptr);
 template <typename T>
-void launch_pair_corr(const DevTree<T>& t, const McArgs& a, hipStream_t st);
+void launch_pair_corr(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0 = nullptr,
+                  hipEvent_t e1 = nullptr);
 template <typename T>
-void launch_scan(const DevTree<T>& t, const McArgs& a, hipStream_t st);
+void launch_scan(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0 = nullptr,
+                  hipEvent_t e1 = nullptr);
 template <typename T>
-void launch_apply(const DevTree<T>& t, const McArgs& a, hipStream_t st);
+void launch_apply(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0 = nullptr,
+                  hipEvent_t e1 = nullptr);
 template <typename T>
 void launch_compact(const DevTree<T>& t, const int* leaves, int n_leaves, hipStream_t st);
+
+void launch_copy16(const void* src, void* dst, size_t n16, hipStream_t st);
 
 template <typename T>
 void launch_outgoing(const DevTree<T>& t, const PwNodes& pn, double* acc_root, hipStream_t st);

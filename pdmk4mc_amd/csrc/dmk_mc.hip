@@ -10,6 +10,8 @@
 //               rewritten arrays; every array element is read and written once
 // T is the storage/arithmetic type of the bulk (double or float); phase
 // arguments are formed in FP64 and every reduction accumulates in FP64.
+#include <hip/hip_ext.h>
+
 #include "dmk_device.hpp"
 #include "dmk_kernels.hpp"
 
@@ -864,30 +866,31 @@ __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
 }
 
 // ------------------------------------------------------------------ launchers
+// e0/e1 (may be null) take the kernel's own start and end timestamps
 template <typename T>
-void launch_setup(const DevTree<T>& t, const McArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_setup<T>, dim3(a.K), dim3(TB), 0, st, t, a);
+void launch_setup(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+    hipExtLaunchKernelGGL(k_setup<T>, dim3(a.K), dim3(TB), 0, st, e0, e1, 0, t, a);
 }
 
 template <typename T>
-void launch_trial(const DevTree<T>& t, const McArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_trial<T>, dim3(trial_blocks(t.max_depth), a.K), dim3(TB), 0, st, t, a);
+void launch_trial(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+    hipExtLaunchKernelGGL(k_trial<T>, dim3(trial_blocks(t.max_depth), a.K), dim3(TB), 0, st, e0, e1, 0, t, a);
 }
 
 template <typename T>
-void launch_pair_corr(const DevTree<T>& t, const McArgs& a, hipStream_t st) {
+void launch_pair_corr(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     const int np = a.K * (a.K - 1) / 2;
-    if (np > 0) hipLaunchKernelGGL(k_pair_corr<T>, dim3(np), dim3(TB), 0, st, t, a);
+    if (np > 0) hipExtLaunchKernelGGL(k_pair_corr<T>, dim3(np), dim3(TB), 0, st, e0, e1, 0, t, a);
 }
 
 template <typename T>
-void launch_scan(const DevTree<T>& t, const McArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_scan<T>, dim3(1), dim3(TB), 0, st, t, a);
+void launch_scan(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+    hipExtLaunchKernelGGL(k_scan<T>, dim3(1), dim3(TB), 0, st, e0, e1, 0, t, a);
 }
 
 template <typename T>
-void launch_apply(const DevTree<T>& t, const McArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_apply<T>, dim3(1024), dim3(TB), 0, st, t, a);
+void launch_apply(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+    hipExtLaunchKernelGGL(k_apply<T>, dim3(1024), dim3(TB), 0, st, e0, e1, 0, t, a);
 }
 
 template <typename T>
@@ -897,11 +900,11 @@ void launch_compact(const DevTree<T>& t, const int* leaves, int n_leaves, hipStr
 
 #define INSTANTIATE(T)                                                                \
     template void launch_compact<T>(const DevTree<T>&, const int*, int, hipStream_t); \
-    template void launch_setup<T>(const DevTree<T>&, const McArgs&, hipStream_t);     \
-    template void launch_trial<T>(const DevTree<T>&, const McArgs&, hipStream_t);     \
-    template void launch_pair_corr<T>(const DevTree<T>&, const McArgs&, hipStream_t); \
-    template void launch_scan<T>(const DevTree<T>&, const McArgs&, hipStream_t);      \
-    template void launch_apply<T>(const DevTree<T>&, const McArgs&, hipStream_t);
+    template void launch_setup<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);     \
+    template void launch_trial<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);     \
+    template void launch_pair_corr<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t); \
+    template void launch_scan<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);      \
+    template void launch_apply<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);
 INSTANTIATE(double)
 INSTANTIATE(float)
 

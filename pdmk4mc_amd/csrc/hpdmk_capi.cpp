@@ -88,7 +88,7 @@ struct Handle {
     MoveBuf single, chain;
     int since_compact = 0;  // leaf changes (upper bound) since the last slot compaction
     // profiling
-    bool profiling = false;
+    int profiling = 0;  // 0 off, 1 kernel events + per-move counts, 2 + phase timers
     double kms[kKinds] = {0, 0, 0, 0, 0};
     int kcount[kKinds] = {0, 0, 0, 0, 0};
     std::vector<hipEvent_t> ev_pool;
@@ -163,13 +163,13 @@ hipEvent_t take_event(Handle* h) {
 template <typename L>
 void timed(Handle* h, int kind, L&& launch) {
     if (!h->profiling) {
-        launch();
+        launch(nullptr, nullptr);
         return;
     }
+    // the events are attached to the dispatch itself, so they bracket the
+    // kernel's execution only (not the queueing around it)
     hipEvent_t a = take_event(h), b = take_event(h);
-    HIPX(hipEventRecord(a, h->stream));
-    launch();
-    HIPX(hipEventRecord(b, h->stream));
+    launch(a, b);
     h->ev_used.push_back({kind, {a, b}});
 }
 
@@ -522,7 +522,7 @@ void upload_moves(Handle* h, MoveBuf& b, int n, const long* idx, const double* d
 McArgs mc_args(Handle* h, const MoveBuf& b, int move0, int K, int mode, double beta) {
     McArgs a{};
     a.s = h->sc;
-    a.s.probe = h->profiling ? h->probe : nullptr;
+    a.s.probe = h->profiling >= 2 ? h->probe : nullptr;
     a.ml.idx = b.idx;
     a.ml.d = b.d;
     a.ml.u = b.u;
@@ -641,7 +641,7 @@ int hpdmk_mc_run(hpdmk_tree t, double beta, int mode, int batch, double* gpu_ms)
             h->kms[k] = 0;
             h->kcount[k] = 0;
         }
-        if (h->profiling) HIPX(hipMemsetAsync(h->probe, 0, sizeof(unsigned long long) * 2 * kProbes, h->stream));
+        if (h->profiling >= 2) HIPX(hipMemsetAsync(h->probe, 0, sizeof(unsigned long long) * 2 * kProbes, h->stream));
         // batches never repeat a particle (the corrections assume distinct particles)
         std::vector<std::pair<int, int>> batches;
         if (mode == 1) {
@@ -672,20 +672,22 @@ int hpdmk_mc_run(hpdmk_tree t, double beta, int mode, int batch, double* gpu_ms)
             if (mode == 0) {
                 for (int m = 0; m < n; ++m) {
                     const McArgs a = mc_args(h, mb, m, 1, kTrialMetropolis, beta);
-                    timed(h, kKSetup, [&] { launch_setup(dv, a, h->stream); });
-                    timed(h, kKTrial, [&] { launch_trial(dv, a, h->stream); });
-                    timed(h, kKUpdate, [&] { launch_apply(dv, a, h->stream); });
+                    timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
+                    timed(h, kKTrial, [&](hipEvent_t x, hipEvent_t y) { launch_trial(dv, a, h->stream, x, y); });
+                    timed(h, kKUpdate, [&](hipEvent_t x, hipEvent_t y) { launch_apply(dv, a, h->stream, x, y); });
                     compact_if_due(1);
                 }
             } else {
                 for (auto& bt : batches) {
                     const int m0 = bt.first, k = bt.second;
                     const McArgs a = mc_args(h, mb, m0, k, kTrialBatch, beta);
-                    timed(h, kKSetup, [&] { launch_setup(dv, a, h->stream); });
-                    timed(h, kKTrial, [&] { launch_trial(dv, a, h->stream); });
-                    if (k > 1) timed(h, kKCorr, [&] { launch_pair_corr(dv, a, h->stream); });
-                    timed(h, kKScan, [&] { launch_scan(dv, a, h->stream); });
-                    timed(h, kKUpdate, [&] { launch_apply(dv, a, h->stream); });
+                    timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
+                    timed(h, kKTrial, [&](hipEvent_t x, hipEvent_t y) { launch_trial(dv, a, h->stream, x, y); });
+                    if (k > 1)
+                        timed(h, kKCorr,
+                              [&](hipEvent_t x, hipEvent_t y) { launch_pair_corr(dv, a, h->stream, x, y); });
+                    timed(h, kKScan, [&](hipEvent_t x, hipEvent_t y) { launch_scan(dv, a, h->stream, x, y); });
+                    timed(h, kKUpdate, [&](hipEvent_t x, hipEvent_t y) { launch_apply(dv, a, h->stream, x, y); });
                     compact_if_due(k);
                 }
             }
@@ -722,8 +724,11 @@ int hpdmk_mc_stats(hpdmk_tree t, int* stats) {
     });
 }
 
-int hpdmk_set_profiling(hpdmk_tree t, int on) {
-    return guard([&] { H(t)->profiling = on != 0; });
+int hpdmk_set_profiling(hpdmk_tree t, int level) {
+    return guard([&] {
+        if (level < 0 || level > 2) throw HpdmkError(HPDMK_EINVAL, "profiling level must be 0, 1 or 2");
+        H(t)->profiling = level;
+    });
 }
 
 int hpdmk_kernel_times(hpdmk_tree t, double* ms, int* count) {
@@ -804,6 +809,39 @@ int hpdmk_phase_times(hpdmk_tree t, double* us, int* count) {  // us[kProbes], c
             if (us) us[k] = p[kProbes + k] ? (double)p[k] / p[kProbes + k] / (freq_khz * 1e-3) : 0.0;
             if (count) count[k] = (int)p[kProbes + k];
         }
+    });
+}
+
+int hpdmk_copy_probe(long bytes, int reps, double* gbs) {
+    return guard([&] {
+        if (bytes < 16 || reps < 1 || !gbs) throw HpdmkError(HPDMK_EINVAL, "copy probe needs bytes >= 16, reps >= 1");
+        const size_t n16 = (size_t)bytes / 16;
+        void *a = nullptr, *b = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        auto release = [&] {
+            if (a) (void)hipFree(a);
+            if (b) (void)hipFree(b);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+        };
+        try {
+            HIPX(hipMalloc(&a, n16 * 16));
+            HIPX(hipMalloc(&b, n16 * 16));
+            HIPX(hipEventCreate(&e0));
+            HIPX(hipEventCreate(&e1));
+            launch_copy16(a, b, n16, nullptr);  // warm-up (page mapping)
+            HIPX(hipEventRecord(e0, nullptr));
+            for (int r = 0; r < reps; ++r) launch_copy16(a, b, n16, nullptr);
+            HIPX(hipEventRecord(e1, nullptr));
+            HIPX(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPX(hipEventElapsedTime(&ms, e0, e1));
+            *gbs = 2.0 * 16.0 * (double)n16 * reps / (ms * 1e-3) / 1e9;
+        } catch (...) {
+            release();
+            throw;
+        }
+        release();
     });
 }
 

@@ -53,6 +53,7 @@ SIGNATURES = {
     "hpdmk_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "hpdmk_kernel_times": (C.c_int, [C.c_void_p, _dp, _ip]),
     "hpdmk_clear_error": (C.c_int, [C.c_void_p]),
+    "hpdmk_copy_probe": (C.c_int, [C.c_long, C.c_int, _dp]),
     "hpdmk_phase_times": (C.c_int, [C.c_void_p, _dp, _ip]),
     "hpdmk_plan_tree": (C.c_int, [HPDMKParams, C.c_int, _dp, _ip, _ip, _ip]),
     "hpdmk_pswf_constants": (C.c_int, [C.c_int, _dp]),
@@ -105,6 +106,13 @@ def pswf_constants(digits):
     _check(load_library().hpdmk_pswf_constants(int(digits), out.ctypes.data_as(_dp)))
     return dict(c=out[0], lam=out[1], C0=out[2], diff0=out[3], psi0_0=out[4], n_window=int(out[5]),
                 n_diff=int(out[6]))
+
+
+def copy_probe(n_bytes=256 << 20, reps=5):
+    """Read+write GB/s of a device streaming copy of n_bytes (HBM probe)."""
+    out = C.c_double(0.0)
+    _check(load_library().hpdmk_copy_probe(int(n_bytes), int(reps), C.byref(out)))
+    return out.value
 
 
 class HPDMKPtTree:
@@ -223,8 +231,9 @@ class HPDMKPtTree:
         _check(load_library().hpdmk_mc_stats(self._h, st.ctypes.data_as(_ip)))
         return st
 
-    def set_profiling(self, on=True):
-        _check(load_library().hpdmk_set_profiling(self._h, int(on)))
+    def set_profiling(self, level=1):
+        """0 off; 1 kernel times and per-move counts; 2 also device phase timers."""
+        _check(load_library().hpdmk_set_profiling(self._h, int(level)))
 
     def kernel_times(self):
         ms = np.zeros(5)
