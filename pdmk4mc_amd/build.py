@@ -17,6 +17,26 @@ FLAGS = [
 ]
 
 
+ROOT = PKG.parent
+CPP_CHECK_SRC = ROOT / "tests" / "cpp" / "tree_check.cpp"
+CPP_CHECK = ROOT / "tests" / "cpp" / "_build" / "tree_check"
+
+
+def build_cpp_check(verbose=False):
+    """Links the C++ caller of include/hpdmk_tree.hpp against the library
+    (host compiler only; the binary runs wherever the library does)."""
+    CPP_CHECK.parent.mkdir(parents=True, exist_ok=True)
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", str(ROOT / "include"), str(CPP_CHECK_SRC),
+           "-o", str(CPP_CHECK), "-L", str(PKG), "-lhpdmk_amd", "-Wl,-rpath,$ORIGIN/../../../pdmk4mc_amd",
+           "-Wl,-rpath-link,/opt/rocm/lib"]
+    out = subprocess.run(cmd, capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("g++ failed:\n" + " ".join(cmd) + "\n" + out.stderr[-4000:])
+    if verbose:
+        print("built", CPP_CHECK)
+    return CPP_CHECK
+
+
 def build(verbose=False):
     cmd = [HIPCC, *FLAGS, "-o", str(LIB) + ".tmp", *[str(CSRC / s) for s in SOURCES]]
     out = subprocess.run(cmd, capture_output=True, text=True)
@@ -25,6 +45,7 @@ def build(verbose=False):
     os.replace(str(LIB) + ".tmp", LIB)
     if verbose:
         print("built", LIB)
+    build_cpp_check(verbose)
     return LIB
 
 

@@ -66,3 +66,18 @@ def test_plan_rejects_shallow_tree_and_bad_coordinates():
     r[0, 0] = 25.0
     with pytest.raises(P.HPDMKError, match=r"\[0, L\)"):
         P.plan_tree(P.HPDMKParams(5, 3, 20.0), r)
+
+
+def test_cpp_class_header_compiles_and_caller_is_built():
+    import subprocess
+    from pdmk4mc_amd.build import CPP_CHECK
+    out = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-I", str(ROOT / "include"),
+                          "-x", "c++", "-"], input='#include "hpdmk_tree.hpp"\n'
+                         "template class hpdmk_amd::HPDMKPtTree<float>;\n"
+                         "template class hpdmk_amd::HPDMKPtTree<double>;\n",
+                         capture_output=True, text=True)
+    assert out.returncode == 0, out.stderr
+    assert CPP_CHECK.exists()
+    # argument errors are reported before any device call
+    res = subprocess.run([str(CPP_CHECK)], capture_output=True, text=True, timeout=60)
+    assert res.returncode == 2 and "usage" in res.stderr
