@@ -169,12 +169,19 @@ def main():
     per_launch = kb.get(dom, 0.0) / max(cnt, 1)
     achieved = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     traffic = measured_traffic(args.traffic_file, dom)
+    per_kernel = {}
+    for k, b in kb.items():
+        ms_k, n_k = kt[k]
+        if n_k and ms_k > 0:
+            per_kernel[k] = {"avg_launch_us": round(ms_k / n_k * 1e3, 3), "alg_bytes_per_launch": int(b / n_k),
+                             "achieved_gbs": round(b / n_k / (ms_k / n_k * 1e-3) / 1e9, 1)}
     symbol = {"trial": "k_trial", "update": "k_apply", "pair_corr": "k_pair_corr", "scan": "k_scan",
               "setup": "k_setup"}
     roofline = {"bound": "hbm", "kernel": dom, "kernel_symbol": symbol.get(dom, dom), "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "bytes_per_launch": int(per_launch), "avg_launch_us": round(avg_ms * 1e3, 3),
-                "kernel_ms": {k: [round(v[0], 3), v[1]] for k, v in kt.items()}, "phase_us": phases}
+                "kernel_ms": {k: [round(v[0], 3), v[1]] for k, v in kt.items()}, "per_kernel": per_kernel,
+                "phase_us": phases}
 
     # plain chain (one trial kernel + one update kernel per move).  The timed
     # run follows a 200-move warmup; the profiled run starts from the fresh

@@ -30,6 +30,7 @@ int run(std::ifstream& in) {
         r[3 * i + 2] = (Real)z;
         q[i] = (Real)c;
     }
+    if (!in) throw std::runtime_error("malformed input (header or particle lines)");
     HPDMKParams params = hpdmk_default_params();
     params.n_per_leaf = leaf;
     params.digits = digits;
@@ -46,6 +47,7 @@ int run(std::ifstream& in) {
         long i;
         double dx, dy, dz;
         in >> i >> dx >> dy >> dz;
+        if (!in) throw std::runtime_error("malformed move line");
         const Real dE = tree.eval_shift_energy(i, (Real)dx, (Real)dy, (Real)dz);
         tree.update_shift(i, (Real)dx, (Real)dy, (Real)dz);
         std::printf("dE %.17g\n", (double)dE);

@@ -25,12 +25,12 @@ L, N, LEAF = 20.0, 1000, 10
 def run_cpp(tmp_path, r, q, digits, idx, d, real):
     f = tmp_path / "in.txt"
     with open(f, "w") as fh:
-        fh.write(f"{len(q)} {L!r} {digits} {LEAF}\n")
+        fh.write(f"{len(q)} {L:.17g} {digits} {LEAF}\n")
         for k in range(len(q)):
-            fh.write(f"{r[k, 0]!r} {r[k, 1]!r} {r[k, 2]!r} {q[k]!r}\n")
+            fh.write(f"{r[k, 0]:.17g} {r[k, 1]:.17g} {r[k, 2]:.17g} {q[k]:.17g}\n")
         fh.write(f"{len(idx)}\n")
         for k in range(len(idx)):
-            fh.write(f"{int(idx[k])} {d[k, 0]!r} {d[k, 1]!r} {d[k, 2]!r}\n")
+            fh.write(f"{int(idx[k])} {d[k, 0]:.17g} {d[k, 1]:.17g} {d[k, 2]:.17g}\n")
     print("tree_check", real, "digits", digits, flush=True)
     sys.stdout.flush()
     out = subprocess.run([str(CPP_CHECK), str(f), real], capture_output=True, text=True, timeout=120)
