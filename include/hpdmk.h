@@ -110,6 +110,14 @@ int hpdmk_eval_shift_energy_batch(hpdmk_tree tree, int n, const long* idx, const
  * host transfers and returns the GPU time (ms) between events around the
  * run; hpdmk_mc_results copies dE and acceptance back. */
 int hpdmk_mc_load(hpdmk_tree tree, int n_moves, const long* idx, const double* d, const double* u);
+/* Model of later hpdmk_mc_run calls (no reference counterpart).  Energies
+ * enter the decision multiplied by the Bjerrum length `bjerrum` (the
+ * reference's Ewald uses eps = 1 / l_B for the same scaling, src/ewald.cpp:187),
+ * and a trial whose target lies closer than `sigma` to another particle
+ * (minimum image) is rejected as a hard-sphere overlap; its reported dE is
+ * +infinity.  Reported chain dE values are bjerrum x dE.  Defaults 1 and 0
+ * (no hard core); sigma may not exceed the finest box size. */
+int hpdmk_mc_set_model(hpdmk_tree tree, double bjerrum, double sigma);
 int hpdmk_mc_run(hpdmk_tree tree, double beta, int mode, int batch, double* gpu_ms);
 int hpdmk_mc_results(hpdmk_tree tree, double* dE, int* accept);
 

@@ -281,6 +281,56 @@ int oracle_mc_chain(void* h, int nmoves, const long* idx, const double* d, const
     });
 }
 
+// Metropolis chain of a charged hard-sphere system: energies scaled by the
+// Bjerrum length lB, a target closer than sigma (minimum image, brute force
+// over all other particles) to any other particle is an overlap: dE = +inf,
+// rejected.  Otherwise accepted when u < exp(-beta lB dE).  This model has no
+// reference counterpart; the scaling is the reference Ewald's eps = 1/lB
+// (src/ewald.cpp:187).
+int oracle_mc_chain_model(void* h, int nmoves, const long* idx, const double* d, const double* u, double beta,
+                          double lB, double sigma, double* dE, int* acc) {
+    return guard([&] {
+        with_tree(h, [&](auto& t) {
+            using R = typename std::decay_t<decltype(t)>::cplx::value_type;
+            const int n = t.n_src;
+            const double L = (double)t.L;
+            std::vector<double> r(3 * (size_t)n);
+            for (int i = 0; i < n; ++i)
+                for (int k = 0; k < 3; ++k) r[3 * i + k] = (double)t.r_sorted[3 * t.invmap[i] + k];
+            auto mod = [&](double x) {
+                while (x < 0) x += L;
+                while (x >= L) x -= L;
+                return x;
+            };
+            for (int m = 0; m < nmoves; ++m) {
+                const long p = idx[m];
+                double tg[3];
+                for (int k = 0; k < 3; ++k) tg[k] = mod(r[3 * p + k] + d[3 * m + k]);
+                bool overlap = false;
+                for (int j = 0; j < n && sigma > 0 && !overlap; ++j) {
+                    if (j == p) continue;
+                    double s2 = 0;
+                    for (int k = 0; k < 3; ++k) {
+                        double x = tg[k] - r[3 * j + k];
+                        x -= L * std::rint(x / L);
+                        s2 += x * x;
+                    }
+                    overlap = s2 < sigma * sigma;
+                }
+                const double e = (double)t.eval_shift_energy(p, (R)d[3 * m], (R)d[3 * m + 1], (R)d[3 * m + 2]);
+                dE[m] = overlap ? INFINITY : lB * e;
+                const bool ok = !overlap && u[m] < std::exp(-beta * lB * e);
+                acc[m] = ok;
+                if (ok) {
+                    t.update_shift(p, (R)d[3 * m], (R)d[3 * m + 1], (R)d[3 * m + 2]);
+                    for (int k = 0; k < 3; ++k) r[3 * p + k] = tg[k];
+                }
+            }
+            return 0;
+        });
+    });
+}
+
 // ---- Ewald ---------------------------------------------------------------
 double oracle_ewald_energy(double L, double s, double alpha, double eps, int n, const double* q, const double* r) {
     Ewald e(L, s, alpha, eps, n, q, r);

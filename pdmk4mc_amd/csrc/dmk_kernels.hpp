@@ -9,7 +9,8 @@ namespace hpdmk_amd {
 
 constexpr int kMaxBatch = 256;     // trials per batched step
 constexpr int kRowStride = 64;     // stride of one 1-D phase row (2n+1 <= 53)
-constexpr int kAccPerTrial = 6;    // window, difference, res_t, res_o, scanned_t, scanned_o
+constexpr int kAccPerTrial = 7;    // window, difference, res_t, res_o, scanned_t, scanned_o, overlaps_t
+constexpr int kPartStride = 3;     // per trial block: partial sum, particles scanned, hard-sphere overlaps
 constexpr int kStatsPerMove = 6;   // levels t/o, near particles t/o, rewritten arrays t/o
 
 // Everything later kernels need about one trial move: endpoints, charge and
@@ -56,10 +57,11 @@ __host__ __device__ inline int trial_blocks(int max_depth) {
 struct McScratch {
     TrialInfo* info;
     void* tabs;            // Cplx<T>, table_stride(D) per trial
-    double* part;          // per trial and block: partial sum, particles scanned
+    double* part;          // per trial and block: kPartStride values
     unsigned* done;        // block counter per trial (self-resetting)
     int* accept;           // per trial of the current launch
     double* corr;          // kMaxBatch^2
+    unsigned char* ovl;    // kMaxBatch^2 hard-sphere flags of pair (i, j): 1 t_j near o_i, 2 t_j near t_i
     unsigned char* mark;   // 2 per expansion box (self-resetting)
     int* dest;             // compact list of rewritten arrays: 2*slot + kind (0 out, 1 in)
     int* plan;             // [0] arrays in dest, [1] any move accepted
@@ -100,6 +102,11 @@ struct McArgs {
     int K;       // trials in this launch
     int mode;    // TrialMode
     double beta;
+    // Monte Carlo model (Metropolis and batch modes only): the decision uses
+    // lB * dE (Bjerrum length in units of the box length unit), and a target
+    // closer than sigma to another particle is rejected (sigma2 = sigma^2, 0 = off)
+    double lB;
+    double sigma2;
 };
 
 template <typename T>

@@ -236,18 +236,19 @@ __device__ int capacity_ok(const DevTree<T>& t, const TrialInfo& ti, int delta) 
 template <typename T>
 __device__ void sum_trial_partials(const DevTree<T>& t, const McArgs& a, int tr, double* v) {
     const int D = t.max_depth, nL = n_levels_pw(D), per = trial_blocks(D);
-    const double* p = a.s.part + 2 * (size_t)tr * per;
+    const double* p = a.s.part + kPartStride * (size_t)tr * per;
     const int d0 = kWinBlocks, d1 = d0 + 2 * nL * kDiffBlocks;
-    double w = 0, df = 0, rt = 0, ro = 0, nt = 0, no = 0;
+    double w = 0, df = 0, rt = 0, ro = 0, nt = 0, no = 0, ov = 0;
 #pragma unroll
-    for (int b = 0; b < kWinBlocks; ++b) w += p[2 * b];
-    for (int b = d0; b < d1; ++b) df += p[2 * b];
+    for (int b = 0; b < kWinBlocks; ++b) w += p[kPartStride * b];
+    for (int b = d0; b < d1; ++b) df += p[kPartStride * b];
 #pragma unroll
     for (int b = 0; b < kNearBlocks; ++b) {
-        rt += p[2 * (d1 + b)];
-        nt += p[2 * (d1 + b) + 1];
-        ro += p[2 * (d1 + kNearBlocks + b)];
-        no += p[2 * (d1 + kNearBlocks + b) + 1];
+        rt += p[kPartStride * (d1 + b)];
+        nt += p[kPartStride * (d1 + b) + 1];
+        ov += p[kPartStride * (d1 + b) + 2];
+        ro += p[kPartStride * (d1 + kNearBlocks + b)];
+        no += p[kPartStride * (d1 + kNearBlocks + b) + 1];
     }
     v[0] = w;
     v[1] = df;
@@ -255,15 +256,17 @@ __device__ void sum_trial_partials(const DevTree<T>& t, const McArgs& a, int tr,
     v[3] = ro;
     v[4] = nt;
     v[5] = no;
+    v[6] = ov;
 }
 
 // dE of trial tr from its partials (src/energy_shift.cpp:67) plus the optional
-// per-move parts and counts
+// per-move parts and counts; *overlaps = particles within sigma of the target
 template <typename T>
-__device__ double assemble_trial(const DevTree<T>& t, const McArgs& a, int tr, int m) {
+__device__ double assemble_trial(const DevTree<T>& t, const McArgs& a, int tr, int m, int* overlaps) {
     const TrialInfo& ti = a.s.info[tr];
     double v[kAccPerTrial];
     sum_trial_partials(t, a, tr, v);
+    *overlaps = (int)v[6];
     const double self = self_term(t, ti);
     const double dE = v[0] + v[1] + v[2] - v[3] - self;
     if (a.ml.parts) {
@@ -310,7 +313,7 @@ __global__ void __launch_bounds__(TB) k_trial(DevTree<T> t, McArgs a) {
     const int D = t.max_depth, nL = n_levels_pw(D), per = trial_blocks(D);
     const Cplx<T>* tb = tabs_of<T>(a.s, table_stride(D), tr);
     double part = 0.0;
-    int nscan = 0;
+    int nscan = 0, novl = 0;
     if (b < kWinBlocks) {
         // window: 2 cW sum W Re[(tau - omega) conj(rho - omega)] (include/vecops.hpp:118-145)
         const int dw = t.dw;
@@ -359,6 +362,11 @@ __global__ void __launch_bounds__(TB) k_trial(DevTree<T> t, McArgs a) {
         const double bb = t.boxsize[lam];
         const T b2 = (T)(bb * bb), inv_b = (T)(1.0 / bb);
         const T eps = std::numeric_limits<T>::epsilon();
+        // hard spheres: particles other than the mover (its old slot) within
+        // sigma of the target; sigma never exceeds the finest box size, so the
+        // near list covers them
+        const bool hs = ep == 0 && a.sigma2 > 0.0;
+        const int own = hs ? t.slot_of[ti.i] : -1;
         double acc = 0;
         for (int k = r % kNearBlocks; k < cnt; k += kNearBlocks) {
             const int nb = t.near_ids[off + k];
@@ -368,19 +376,23 @@ __global__ void __launch_bounds__(TB) k_trial(DevTree<T> t, McArgs a) {
             nscan += n;
             for (int p = threadIdx.x; p < n; p += blockDim.x) {
                 const int s = s0 + p;
-                const T dx = (T)(cx - (t.px[s] + sx)), dy = (T)(cy - (t.py[s] + sy)), dz = (T)(cz - (t.pz[s] + sz));
+                const double ex = cx - (t.px[s] + sx), ey = cy - (t.py[s] + sy), ez = cz - (t.pz[s] + sz);
+                const T dx = (T)ex, dy = (T)ey, dz = (T)ez;
                 const T r2 = dx * dx + dy * dy + dz * dz;
                 if (r2 > eps && r2 < b2)
                     acc += (double)((T)t.pq[s] * residual_poly_over_r<T>(sqrt(r2), inv_b, t.poly, t.n_poly));
+                if (hs && s != own && t.porig[s] >= 0 && ex * ex + ey * ey + ez * ez < a.sigma2) ++novl;
             }
         }
         part = acc * ti.q;
     }
     const double s = block_sum(part, red);
+    const double ovs = a.sigma2 > 0.0 ? block_sum((double)novl, red) : 0.0;  // uniform across the grid
     if (threadIdx.x == 0) {
-        double* p = a.s.part + 2 * ((size_t)tr * per + b);
+        double* p = a.s.part + kPartStride * ((size_t)tr * per + b);
         __hip_atomic_store(p, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(p + 1, (double)nscan, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(p + 2, ovs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     if (a.mode == kTrialBatch) return;
     if (threadIdx.x == 0) {
@@ -393,11 +405,15 @@ __global__ void __launch_bounds__(TB) k_trial(DevTree<T> t, McArgs a) {
     if (threadIdx.x == 0) {
         unsigned long long tp = wall_clock64();
         __threadfence();
-        const double dE = assemble_trial(t, a, tr, m);
+        int ovl = 0;
+        double dE = assemble_trial(t, a, tr, m, &ovl);
         if (a.mode == kTrialOnly && a.ml.dE_stale) a.ml.dE_stale[m] = dE;
-        a.ml.dE[m] = dE;
         int ok = 0;
-        if (a.mode == kTrialMetropolis) ok = dE < mh_threshold(a.ml.u[m], a.beta);
+        if (a.mode == kTrialMetropolis) {
+            dE = ovl > 0 ? (double)INFINITY : a.lB * dE;  // a hard-sphere overlap is an infinite energy
+            ok = ovl == 0 && dE < mh_threshold(a.ml.u[m], a.beta);
+        }
+        a.ml.dE[m] = dE;
         if (a.mode == kTrialForceAccept) ok = 1;
         if (ok) ok = capacity_ok(t, ti, 0);
         a.s.accept[tr] = ok;
@@ -529,6 +545,7 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
     }
     if (threadIdx.x == 0) {
         const T eps = std::numeric_limits<T>::epsilon();
+        int flags = 0;  // hard spheres: 1 t_j within sigma of o_i (counted, now gone), 2 of t_i (new)
         for (int ep = 0; ep < 2; ++ep) {
             const int lam = ep == 0 ? Bj.lam_t : Bj.lam_o;
             const double* xj = ep == 0 ? Bj.t : Bj.o;
@@ -545,10 +562,12 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
                 const T r2 = fx * fx + fy * fy + fz * fz;
                 if (r2 > eps && r2 < b2)
                     s += (w == 0 ? 1.0 : -1.0) * (double)residual_poly_over_r<T>(sqrt(r2), inv_b, t.poly, t.n_poly);
+                if (ep == 0 && dx * dx + dy * dy + dz * dz < a.sigma2) flags |= w == 0 ? 2 : 1;
             }
             total += (ep == 0 ? 1.0 : -1.0) * s * A.q * Bj.q;
         }
         a.s.corr[i * K + j] = total;
+        if (a.sigma2 > 0.0) a.s.ovl[i * K + j] = (unsigned char)flags;
     }
 }
 
@@ -559,15 +578,19 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
 template <typename T>
 __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
     __shared__ double dEs[kMaxBatch], thr[kMaxBatch], dEf[kMaxBatch];
-    __shared__ int leafT[kMaxBatch], leafO[kMaxBatch], accs[kMaxBatch], room[kMaxBatch];
+    __shared__ int leafT[kMaxBatch], leafO[kMaxBatch], accs[kMaxBatch], room[kMaxBatch], ov0[kMaxBatch];
     __shared__ double C[64 * 64];
+    __shared__ unsigned char F[64 * 64];
     if (*t.err) return;
     const int K = a.K;
+    const bool hs = a.sigma2 > 0.0;
     unsigned long long tp = wall_clock64();
     for (int j = threadIdx.x; j < K; j += blockDim.x) {
         const int m = a.move0 + j;
-        const double dE = assemble_trial(t, a, j, m);
+        int ov = 0;
+        const double dE = assemble_trial(t, a, j, m, &ov);
         dEs[j] = dE;
+        ov0[j] = ov;
         a.ml.dE_stale[m] = dE;
         const TrialInfo& tj = a.s.info[j];
         leafT[j] = tj.path_t[tj.lam_t];
@@ -588,27 +611,32 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
             const int q = threadIdx.x + r * TB;
             if (q < K * K) C[q] = v[r];
         }
+        if (hs)
+            for (int q = threadIdx.x; q < K * K; q += blockDim.x) F[q] = a.s.ovl[q];
     }
     __syncthreads();
     if (threadIdx.x == 0) probe_add(a.s, kPrScanAssemble, tp);
     if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
         double run[4];
-        int dcr[4];
+        int dcr[4], ovr[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int j = lane + 64 * k;
             run[k] = j < K ? dEs[j] : 0.0;
+            ovr[k] = j < K ? ov0[j] : 0;
             dcr[k] = 0;
         }
+        const double lB = a.lB;
         for (int i = 0; i < K; ++i) {
             const int owner = i & 63, sl = i >> 6;
             int ok = 0;
             if (lane == owner) {
-                const double dE = sl == 0 ? run[0] : sl == 1 ? run[1] : sl == 2 ? run[2] : run[3];
+                const double dE = lB * (sl == 0 ? run[0] : sl == 1 ? run[1] : sl == 2 ? run[2] : run[3]);
                 const int dc = sl == 0 ? dcr[0] : sl == 1 ? dcr[1] : sl == 2 ? dcr[2] : dcr[3];
-                ok = dE < thr[i] ? (dc < room[i] ? 1 : 2) : 0;  // 2: the target leaf is full
-                dEf[i] = dE;
+                const int ov = sl == 0 ? ovr[0] : sl == 1 ? ovr[1] : sl == 2 ? ovr[2] : ovr[3];
+                ok = (ov == 0 && dE < thr[i]) ? (dc < room[i] ? 1 : 2) : 0;  // 2: the target leaf is full
+                dEf[i] = ov > 0 ? (double)INFINITY : dE;  // a hard-sphere overlap is an infinite energy
                 accs[i] = ok;
             }
             ok = __builtin_amdgcn_readlane(ok, owner) == 1;
@@ -620,6 +648,10 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
                     if (j > i && j < K) {
                         run[k] += in_lds ? C[i * K + j] : a.s.corr[i * K + j];
                         dcr[k] += mv && (lti == leafT[j]);
+                        if (hs) {
+                            const int f = in_lds ? F[i * K + j] : a.s.ovl[i * K + j];
+                            ovr[k] += (f >> 1) - (f & 1);
+                        }
                     }
                 }
             }

@@ -87,6 +87,9 @@ struct Handle {
     // move lists in HBM: `single` serves eval/update calls, `chain` the staged Metropolis run
     MoveBuf single, chain;
     int since_compact = 0;  // leaf changes (upper bound) since the last slot compaction
+    // Monte Carlo model of hpdmk_mc_run (hpdmk_mc_set_model)
+    double bjerrum = 1.0;
+    double sigma = 0.0;
     // profiling
     int profiling = 0;  // 0 off, 1 kernel events + per-move counts, 2 + phase timers
     double kms[kKinds] = {0, 0, 0, 0, 0};
@@ -405,11 +408,12 @@ int hpdmk_tree_create_ex(HPDMKParams params, int precision, int n_src, const dou
         h->acc_root = dalloc<double>(2 * Mw, o);
         h->energy = dalloc<double>(4, o);
         h->t_done = dalloc<unsigned>(kMaxBatch, o);
-        h->sc.part = dalloc<double>(2 * (size_t)kMaxBatch * trial_blocks(D), o);
+        h->sc.part = dalloc<double>(kPartStride * (size_t)kMaxBatch * trial_blocks(D), o);
         h->sc.done = h->t_done;
         h->sc.info = dalloc<TrialInfo>(kMaxBatch, o);
         h->sc.accept = dalloc<int>(kMaxBatch, o);
         h->sc.corr = dalloc<double>(kMaxBatch * kMaxBatch, o);
+        h->sc.ovl = dalloc<unsigned char>(kMaxBatch * kMaxBatch, o);
         const size_t tsz = precision == HPDMK_FP64 ? sizeof(Cplx<double>) : sizeof(Cplx<float>);
         h->sc.tabs = dalloc<char>(tsz * (size_t)table_stride(D) * kMaxBatch, o);
         h->sc.mark = dalloc<unsigned char>(2 * (size_t)h->n_pw + 2, o);
@@ -536,6 +540,9 @@ McArgs mc_args(Handle* h, const MoveBuf& b, int move0, int K, int mode, double b
     a.K = K;
     a.mode = mode;
     a.beta = beta;
+    const bool chain = mode == kTrialMetropolis || mode == kTrialBatch;
+    a.lB = chain ? h->bjerrum : 1.0;
+    a.sigma2 = chain ? h->sigma * h->sigma : 0.0;
     return a;
 }
 }  // namespace
@@ -809,6 +816,21 @@ int hpdmk_phase_times(hpdmk_tree t, double* us, int* count) {  // us[kProbes], c
             if (us) us[k] = p[kProbes + k] ? (double)p[k] / p[kProbes + k] / (freq_khz * 1e-3) : 0.0;
             if (count) count[k] = (int)p[kProbes + k];
         }
+    });
+}
+
+int hpdmk_mc_set_model(hpdmk_tree t, double bjerrum, double sigma) {
+    return guard([&] {
+        Handle* h = H(t);
+        if (!(bjerrum > 0.0) || !std::isfinite(bjerrum)) throw HpdmkError(HPDMK_EINVAL, "Bjerrum length must be > 0");
+        if (!(sigma >= 0.0) || !std::isfinite(sigma)) throw HpdmkError(HPDMK_EINVAL, "sigma must be >= 0");
+        // the overlap search reads the target leaf's near list, which covers at
+        // least one box of the leaf's level in every direction
+        const double finest = h->params.L / (double)(1 << (h->tree.max_depth - 1));
+        if (sigma > finest)
+            throw HpdmkError(HPDMK_EINVAL, "sigma exceeds the finest box size " + std::to_string(finest));
+        h->bjerrum = bjerrum;
+        h->sigma = sigma;
     });
 }
 

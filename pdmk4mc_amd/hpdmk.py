@@ -54,6 +54,7 @@ SIGNATURES = {
     "hpdmk_kernel_times": (C.c_int, [C.c_void_p, _dp, _ip]),
     "hpdmk_clear_error": (C.c_int, [C.c_void_p]),
     "hpdmk_copy_probe": (C.c_int, [C.c_long, C.c_int, _dp]),
+    "hpdmk_mc_set_model": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
     "hpdmk_phase_times": (C.c_int, [C.c_void_p, _dp, _ip]),
     "hpdmk_plan_tree": (C.c_int, [HPDMKParams, C.c_int, _dp, _ip, _ip, _ip]),
     "hpdmk_pswf_constants": (C.c_int, [C.c_int, _dp]),
@@ -207,6 +208,10 @@ class HPDMKPtTree:
         d, pd = _d(d)
         u, pu = _d(u)
         _check(load_library().hpdmk_mc_load(self._h, self._mc_idx.size, self._mc_idx.ctypes.data_as(_lp), pd, pu))
+
+    def mc_set_model(self, bjerrum=1.0, sigma=0.0):
+        """Bjerrum scaling of the chain's energies and hard-sphere diameter (0 = none)."""
+        _check(load_library().hpdmk_mc_set_model(self._h, float(bjerrum), float(sigma)))
 
     def mc_run(self, beta=1.0, mode=0, batch=64):
         ms = C.c_double()

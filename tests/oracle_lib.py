@@ -60,6 +60,8 @@ def _load(fast=False):
                                                _dp]),
         "oracle_update_shift": (C.c_int, [C.c_void_p, C.c_long, C.c_double, C.c_double, C.c_double]),
         "oracle_mc_chain": (C.c_int, [C.c_void_p, C.c_int, _lp, _dp, _dp, C.c_double, _dp, _ip, _dp]),
+        "oracle_mc_chain_model": (C.c_int, [C.c_void_p, C.c_int, _lp, _dp, _dp, C.c_double, C.c_double, C.c_double,
+                                            _dp, _ip]),
         "oracle_ewald_energy": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp]),
         "oracle_ewald_potential": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _dp,
                                                 _dp, C.c_double, C.c_double, C.c_double]),
@@ -251,6 +253,19 @@ class OracleTree:
         _check(self._L.oracle_mc_chain(self.h, n, idx.ctypes.data_as(_lp), pd, pu, beta, dE.ctypes.data_as(_dp),
                                        acc.ctypes.data_as(_ip), t2.ctypes.data_as(_dp)), self._L)
         return dE, acc, t2
+
+
+    def mc_chain_model(self, idx, d, u, beta, bjerrum, sigma):
+        """Chain with Bjerrum-scaled energies and a hard core of diameter sigma."""
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        d, pd = _d(d)
+        u, pu = _d(u)
+        n = len(idx)
+        dE = np.zeros(n)
+        acc = np.zeros(n, dtype=np.int32)
+        _check(self._L.oracle_mc_chain_model(self.h, n, idx.ctypes.data_as(_lp), pd, pu, beta, bjerrum, sigma,
+                                             dE.ctypes.data_as(_dp), acc.ctypes.data_as(_ip)), self._L)
+        return dE, acc
 
 
 # ---------------------------------------------------------------- Ewald
