@@ -154,6 +154,33 @@ int hpdmk_plan_tree(HPDMKParams params, int n_src, const double* r_src, int* n_n
  * n_window, n_diff (out[7]). */
 int hpdmk_pswf_constants(int digits, double* out);
 
+/* ---- replica sets (no reference counterpart) --------------------------
+ * Many independent systems of different sizes (ragged), each a full
+ * hpdmk_tree, advanced together: one Metropolis move of every replica per
+ * step, with one launch of each kernel over all replicas per step, and runs
+ * of steps captured into a HIP graph and replayed.  The chains are exactly
+ * the plain chains (hpdmk_mc_run mode 0) of the replicas. */
+typedef void* hpdmk_replicas;
+/* params[r], n_src[r]: replica r; r_src and charge concatenate the replicas
+ * (3 n_src[r] and n_src[r] doubles each).  One precision for the set. */
+int hpdmk_replicas_create(int n_rep, const HPDMKParams* params, int precision, const int* n_src,
+                          const double* r_src, const double* charge, hpdmk_replicas* set);
+int hpdmk_replicas_destroy(hpdmk_replicas set);
+int hpdmk_replicas_count(hpdmk_replicas set, int* n_rep);
+/* Replica r as an hpdmk_tree (energies, positions, single moves,
+ * hpdmk_mc_set_model); it belongs to the set and is destroyed with it. */
+int hpdmk_replicas_tree(hpdmk_replicas set, int r, hpdmk_tree* tree);
+/* outgoing and incoming expansions of every replica */
+int hpdmk_replicas_form_pw(hpdmk_replicas set);
+/* n_moves per replica, replica-major: idx[r * n_moves + k] (replica-local
+ * index), d[3 (r * n_moves + k) + a], u[r * n_moves + k]. */
+int hpdmk_replicas_mc_load(hpdmk_replicas set, int n_moves, const long* idx, const double* d, const double* u);
+/* Runs every replica's chain; graph_steps > 1 captures that many steps into
+ * one HIP graph and replays it. */
+int hpdmk_replicas_mc_run(hpdmk_replicas set, double beta, int graph_steps, double* gpu_ms);
+/* dE and decisions, replica-major like the move list */
+int hpdmk_replicas_mc_results(hpdmk_replicas set, double* dE, int* accept);
+
 /* HBM streaming-copy probe (no counterpart in the reference; a measurement
  * aid): copies `bytes` between two fresh device buffers `reps` times and
  * returns read+write GB/s.  Each launch reads and writes `bytes` once, which

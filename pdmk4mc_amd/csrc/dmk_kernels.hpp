@@ -99,6 +99,7 @@ struct McArgs {
     MoveList ml;
     PwNodes pn;
     int move0;   // first move of this launch in the move list
+    const int* move_ctr;  // when set, the first move is *move_ctr (replica steps replayed from a graph)
     int K;       // trials in this launch
     int mode;    // TrialMode
     double beta;
@@ -108,6 +109,23 @@ struct McArgs {
     double lB;
     double sigma2;
 };
+
+// One Metropolis step of every replica of a set (one move each): per-replica
+// device state and arguments live in device arrays indexed by blockIdx.z.
+template <typename T>
+struct ReplicaLaunch {
+    const DevTree<T>* trees;  // n
+    const McArgs* args;       // n (move_ctr set)
+    int* move_ctr;            // n step counters
+    const int* const* leaves; // n leaf lists
+    const int* n_leaves;      // n
+    int n, max_depth, max_leaves, apply_blocks;
+};
+
+template <typename T>
+void launch_replica_step(const ReplicaLaunch<T>& r, hipStream_t st);
+template <typename T>
+void launch_replica_compact(const ReplicaLaunch<T>& r, hipStream_t st);
 
 template <typename T>
 void launch_setup(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0 = nullptr,

@@ -49,12 +49,15 @@ __device__ __forceinline__ int image_shift(int ca, int cb, int nl) {
     return 2;
 }
 
+// first move of this launch: the replica's step counter when one is attached
+__device__ __forceinline__ int move_base(const McArgs& a) { return a.move_ctr ? *a.move_ctr : a.move0; }
+
 // ------------------------------------------------------------------ setup
 template <typename T>
-__global__ void __launch_bounds__(TB) k_setup(DevTree<T> t, McArgs a) {
+__device__ __forceinline__ void setup_body(const DevTree<T>& t, const McArgs& a) {
     __shared__ TrialInfo ti;
     if (*t.err) return;
-    const int tr = blockIdx.x, m = a.move0 + tr;
+    const int tr = blockIdx.x, m = move_base(a) + tr;
     if (threadIdx.x == 0) {
         trial_setup(t, a.ml.idx[m], a.ml.d + 3 * m, ti);
         a.s.info[tr] = ti;
@@ -93,6 +96,19 @@ __global__ void __launch_bounds__(TB) k_setup(DevTree<T> t, McArgs a) {
         const double scale = axis == 0 ? ti.q : 1.0;
         tb[r * kRowStride + mm] = {(T)(cs * scale), (T)(-sn * scale)};
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TB) k_setup(DevTree<T> t, McArgs a) {
+    setup_body(t, a);
+}
+
+// one trial per replica: replica blockIdx.z
+template <typename T>
+__global__ void __launch_bounds__(TB) k_setup_rep(const DevTree<T>* ts, const McArgs* as) {
+    const DevTree<T> t = ts[blockIdx.z];
+    const McArgs a = as[blockIdx.z];
+    setup_body(t, a);
 }
 
 // ------------------------------------------------------------------ update plan
@@ -304,11 +320,11 @@ __device__ double assemble_trial(const DevTree<T>& t, const McArgs& a, int tr, i
 // Every block stores its partial sum; in single-move modes the last block of
 // the trial assembles dE, decides and plans the update.
 template <typename T>
-__global__ void __launch_bounds__(TB) k_trial(DevTree<T> t, McArgs a) {
+__device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a) {
     __shared__ double red[TB / 64];
     __shared__ int last;
     if (*t.err) return;
-    const int tr = blockIdx.y, b = blockIdx.x, m = a.move0 + tr;
+    const int tr = blockIdx.y, b = blockIdx.x, m = move_base(a) + tr;
     const TrialInfo& ti = a.s.info[tr];
     const int D = t.max_depth, nL = n_levels_pw(D), per = trial_blocks(D);
     const Cplx<T>* tb = tabs_of<T>(a.s, table_stride(D), tr);
@@ -424,6 +440,21 @@ __global__ void __launch_bounds__(TB) k_trial(DevTree<T> t, McArgs a) {
     if (a.mode == kTrialOnly) return;
     __syncthreads();
     plan_update(t, a, 1);  // single-move step: this block plans the update
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TB) k_trial(DevTree<T> t, McArgs a) {
+    trial_body(t, a);
+}
+
+// replica blockIdx.z; the grid is sized for the deepest replica and the
+// blocks a shallower replica does not use return at once
+template <typename T>
+__global__ void __launch_bounds__(TB) k_trial_rep(const DevTree<T>* ts, const McArgs* as) {
+    const DevTree<T> t = ts[blockIdx.z];
+    const McArgs a = as[blockIdx.z];
+    if ((int)blockIdx.x >= trial_blocks(t.max_depth)) return;
+    trial_body(t, a);
 }
 
 // ------------------------------------------------------------------ batched trials
@@ -586,7 +617,7 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
     const bool hs = a.sigma2 > 0.0;
     unsigned long long tp = wall_clock64();
     for (int j = threadIdx.x; j < K; j += blockDim.x) {
-        const int m = a.move0 + j;
+        const int m = move_base(a) + j;
         int ov = 0;
         const double dE = assemble_trial(t, a, j, m, &ov);
         dEs[j] = dE;
@@ -660,7 +691,7 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
     __syncthreads();
     if (threadIdx.x == 0) probe_add(a.s, kPrScanDecide, tp);
     for (int j = threadIdx.x; j < K; j += blockDim.x) {
-        const int m = a.move0 + j;
+        const int m = move_base(a) + j;
         int ok = accs[j];
         if (ok == 2) {
             atomicOr(t.err, (int)kErrLeafOverflow);
@@ -677,7 +708,7 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
 // Packs the live particles of every leaf (porig >= 0) to the front of its
 // slots, in slot order, after relocations left holes behind.  One block per leaf.
 template <typename T>
-__global__ void __launch_bounds__(TB) k_compact(DevTree<T> t, const int* leaves) {
+__device__ __forceinline__ void compact_body(const DevTree<T>& t, const int* leaves) {
     __shared__ int wtot[TB / 64], base;
     if (*t.err) return;
     const int leaf = leaves[blockIdx.x];
@@ -729,6 +760,24 @@ __global__ void __launch_bounds__(TB) k_compact(DevTree<T> t, const int* leaves)
     if (threadIdx.x == 0) t.slot_count[leaf] = base;
 }
 
+template <typename T>
+__global__ void __launch_bounds__(TB) k_compact(DevTree<T> t, const int* leaves) {
+    compact_body(t, leaves);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TB) k_compact_rep(const DevTree<T>* ts, const int* const* leaves, const int* n_leaves) {
+    if ((int)blockIdx.x >= n_leaves[blockIdx.z]) return;
+    const DevTree<T> t = ts[blockIdx.z];
+    compact_body(t, leaves[blockIdx.z]);
+}
+
+// next move of every replica (graph-replayable step counter)
+__global__ void k_advance(int* move_ctr, int n) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) move_ctr[r] += 1;
+}
+
 // ------------------------------------------------------------------ apply
 // Work items: root chunks, then (rewritten array, element chunk).  For its
 // array a block lists the (move, endpoint, image) contributions in move order,
@@ -739,7 +788,7 @@ constexpr int kApplyPer = 8;  // elements per thread and work item
 constexpr int kApplyLv = 4;   // levels of the per-block path table (deeper levels read TrialInfo)
 
 template <typename T>
-__global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
+__device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a) {
     __shared__ int mlist[2 * kMaxBatch];
     __shared__ int wtot[TB / 64];
     __shared__ int mcount;
@@ -897,6 +946,18 @@ __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
     }
 }
 
+template <typename T>
+__global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
+    apply_body(t, a);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TB) k_apply_rep(const DevTree<T>* ts, const McArgs* as) {
+    const DevTree<T> t = ts[blockIdx.z];
+    const McArgs a = as[blockIdx.z];
+    apply_body(t, a);
+}
+
 // ------------------------------------------------------------------ launchers
 // e0/e1 (may be null) take the kernel's own start and end timestamps
 template <typename T>
@@ -930,13 +991,30 @@ void launch_compact(const DevTree<T>& t, const int* leaves, int n_leaves, hipStr
     if (n_leaves > 0) hipLaunchKernelGGL(k_compact<T>, dim3(n_leaves), dim3(TB), 0, st, t, leaves);
 }
 
+template <typename T>
+void launch_replica_step(const ReplicaLaunch<T>& r, hipStream_t st) {
+    const dim3 rep(1, 1, r.n);
+    hipLaunchKernelGGL(k_setup_rep<T>, rep, dim3(TB), 0, st, r.trees, r.args);
+    hipLaunchKernelGGL(k_trial_rep<T>, dim3(trial_blocks(r.max_depth), 1, r.n), dim3(TB), 0, st, r.trees, r.args);
+    hipLaunchKernelGGL(k_apply_rep<T>, dim3(r.apply_blocks, 1, r.n), dim3(TB), 0, st, r.trees, r.args);
+    hipLaunchKernelGGL(k_advance, dim3((r.n + TB - 1) / TB), dim3(TB), 0, st, r.move_ctr, r.n);
+}
+
+template <typename T>
+void launch_replica_compact(const ReplicaLaunch<T>& r, hipStream_t st) {
+    hipLaunchKernelGGL(k_compact_rep<T>, dim3(r.max_leaves, 1, r.n), dim3(TB), 0, st, r.trees, r.leaves,
+                       r.n_leaves);
+}
+
 #define INSTANTIATE(T)                                                                \
     template void launch_compact<T>(const DevTree<T>&, const int*, int, hipStream_t); \
     template void launch_setup<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);     \
     template void launch_trial<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);     \
     template void launch_pair_corr<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t); \
     template void launch_scan<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);      \
-    template void launch_apply<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);
+    template void launch_apply<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t); \
+    template void launch_replica_step<T>(const ReplicaLaunch<T>&, hipStream_t);                            \
+    template void launch_replica_compact<T>(const ReplicaLaunch<T>&, hipStream_t);
 INSTANTIATE(double)
 INSTANTIATE(float)
 

@@ -1,6 +1,7 @@
-// C ABI of libhpdmk_amd.so (declared in include/hpdmk.h): owns the device
-// state of one DMK system and sequences the HIP kernels of dmk_kernels.hip on
-// one stream.  No exception crosses the boundary.
+// C ABI of libhpdmk_amd.so for one DMK system (declared in include/hpdmk.h):
+// owns the device state of the system and sequences the HIP kernels of
+// dmk_mc.hip and dmk_build.hip on one stream.  No exception crosses the
+// boundary.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -10,137 +11,14 @@
 #include <string>
 #include <vector>
 
-#include "../../include/hpdmk.h"
-#include "dmk_common.hpp"
-#include "dmk_host.hpp"
-#include "dmk_kernels.hpp"
+#include "hpdmk_handle.hpp"
 
 using namespace hpdmk_amd;
 
-namespace {
+namespace hpdmk_amd {
+namespace capi {
 
 thread_local std::string g_err;
-
-#define HIPX(call)                                                                                         \
-    do {                                                                                                   \
-        hipError_t e_ = (call);                                                                            \
-        if (e_ != hipSuccess)                                                                              \
-            throw HpdmkError(HPDMK_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_));            \
-    } while (0)
-
-template <typename V>
-V* dalloc(size_t n, std::vector<void*>& owned) {
-    void* p = nullptr;
-    HIPX(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(V)));
-    owned.push_back(p);
-    return static_cast<V*>(p);
-}
-
-template <typename V>
-V* dupload(const std::vector<V>& h, std::vector<void*>& owned) {
-    V* p = dalloc<V>(h.size(), owned);
-    if (!h.empty()) HIPX(hipMemcpy(p, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice));
-    return p;
-}
-
-struct MoveBuf {
-    int n = 0, cap = 0;
-    std::vector<long> h_idx;
-    int* idx = nullptr;
-    double* d = nullptr;
-    double* u = nullptr;
-    double* dE = nullptr;
-    double* dE_stale = nullptr;
-    double* parts = nullptr;
-    int* acc = nullptr;
-    int* stats = nullptr;
-};
-
-// Moved particles leave holes in their old leaf slots; the slots are packed
-// again after at most this many moves.  Leaf capacity is twice the initial
-// count (at least count + 64), so a leaf would need 64 net arrivals in between
-// to run out.
-constexpr int kCompactEvery = 2048;
-
-enum KernelKind { kKTrial = 0, kKUpdate = 1, kKCorr = 2, kKScan = 3, kKSetup = 4, kKinds = 5 };
-
-struct Handle {
-    HPDMKParams params{};
-    int precision = HPDMK_FP64;
-    PswfConsts pw;
-    Octree tree;
-    int n_src = 0, n_leaves = 0, n_pw = 0;
-    double Q = 0;
-    hipStream_t stream = nullptr;
-    std::vector<void*> owned;
-    DevTree<double> d64{};
-    DevTree<float> d32{};
-    PwNodes pn{};
-    int* leaves = nullptr;
-    double* acc_root = nullptr;
-    double* energy = nullptr;
-    int* err = nullptr;
-    // Monte Carlo scratch (kMaxBatch trials)
-    McScratch sc{};
-    unsigned long long* probe = nullptr;
-    unsigned* t_done = nullptr;
-    // move lists in HBM: `single` serves eval/update calls, `chain` the staged Metropolis run
-    MoveBuf single, chain;
-    int since_compact = 0;  // leaf changes (upper bound) since the last slot compaction
-    // Monte Carlo model of hpdmk_mc_run (hpdmk_mc_set_model)
-    double bjerrum = 1.0;
-    double sigma = 0.0;
-    // profiling
-    int profiling = 0;  // 0 off, 1 kernel events + per-move counts, 2 + phase timers
-    double kms[kKinds] = {0, 0, 0, 0, 0};
-    int kcount[kKinds] = {0, 0, 0, 0, 0};
-    std::vector<hipEvent_t> ev_pool;
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
-
-    ~Handle() {
-        for (auto& e : ev_pool) (void)hipEventDestroy(e);
-        for (void* p : owned) (void)hipFree(p);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
-
-template <typename F>
-int guard(F&& f) {
-    try {
-        f();
-        return HPDMK_OK;
-    } catch (const HpdmkError& e) {
-        g_err = e.what();
-        return e.code;
-    } catch (const std::exception& e) {
-        g_err = e.what();
-        return HPDMK_ERUNTIME;
-    }
-}
-
-Handle* H(hpdmk_tree t) {
-    if (!t) throw HpdmkError(HPDMK_EINVAL, "null hpdmk_tree handle");
-    return static_cast<Handle*>(t);
-}
-
-template <typename T>
-DevTree<T>& dev(Handle* h);
-template <>
-DevTree<double>& dev<double>(Handle* h) {
-    return h->d64;
-}
-template <>
-DevTree<float>& dev<float>(Handle* h) {
-    return h->d32;
-}
-
-template <typename F>
-void dispatch(Handle* h, F&& f) {
-    if (h->precision == HPDMK_FP64)
-        f(h->d64);
-    else
-        f(h->d32);
-}
 
 void check_err(Handle* h) {
     int e = 0;
@@ -152,6 +30,7 @@ void check_err(Handle* h) {
                          "hpdmk_get_positions and continue");
 }
 
+
 hipEvent_t take_event(Handle* h) {
     if (h->ev_pool.empty()) {
         hipEvent_t e;
@@ -161,19 +40,6 @@ hipEvent_t take_event(Handle* h) {
     hipEvent_t e = h->ev_pool.back();
     h->ev_pool.pop_back();
     return e;
-}
-
-template <typename L>
-void timed(Handle* h, int kind, L&& launch) {
-    if (!h->profiling) {
-        launch(nullptr, nullptr);
-        return;
-    }
-    // the events are attached to the dispatch itself, so they bracket the
-    // kernel's execution only (not the queueing around it)
-    hipEvent_t a = take_event(h), b = take_event(h);
-    launch(a, b);
-    h->ev_used.push_back({kind, {a, b}});
 }
 
 void collect_events(Handle* h) {
@@ -190,11 +56,11 @@ void collect_events(Handle* h) {
     h->ev_used.clear();
 }
 
+namespace {
 template <typename T>
-void fill_dev(Handle* h, DevTree<T>& d, const std::vector<std::vector<double>>& dtabs, const std::vector<double>& wtab,
-              const std::vector<int>& cap, const std::vector<int>& start, const std::vector<int>& count) {
+void fill_dev(Handle* h, DevTree<T>& d, const std::vector<std::vector<double>>& dtabs,
+              const std::vector<double>& wtab) {
     auto& o = h->owned;
-    const Octree& tr = h->tree;
     std::vector<T> W(wtab.begin(), wtab.end());
     std::vector<T> D;
     for (auto& v : dtabs) D.insert(D.end(), v.begin(), v.end());
@@ -206,13 +72,267 @@ void fill_dev(Handle* h, DevTree<T>& d, const std::vector<std::vector<double>>& 
     HIPX(hipMemset(d.root, 0, sizeof(Cplx<T>) * d.Mw));
     HIPX(hipMemset(d.out, 0, sizeof(Cplx<T>) * std::max<size_t>(1, (size_t)d.n_pw_nodes * d.Md)));
     HIPX(hipMemset(d.inc, 0, sizeof(Cplx<T>) * std::max<size_t>(1, (size_t)d.n_pw_nodes * d.Md)));
-    (void)tr;
-    (void)cap;
-    (void)start;
-    (void)count;
+}
+}  // namespace
+
+Handle* create_handle(HPDMKParams params, int precision, int n_src, const double* r_src, const double* charge,
+                      hipStream_t stream) {
+    if (!r_src || !charge) throw HpdmkError(HPDMK_EINVAL, "null argument");
+    if (precision != HPDMK_FP64 && precision != HPDMK_MIXED)
+        throw HpdmkError(HPDMK_EINVAL, "precision must be HPDMK_FP64 or HPDMK_MIXED");
+    if (precision == HPDMK_MIXED && params.digits != 3 && params.digits != 6)
+        throw HpdmkError(HPDMK_EINVAL, "float only supports 3 and 6 digits accuracy");  // src/tree.cpp:19-23
+    if (params.n_per_leaf < 1) throw HpdmkError(HPDMK_EINVAL, "n_per_leaf must be positive");
+    auto h = std::make_unique<Handle>();
+    h->params = params;
+    h->precision = precision;
+    try {
+        h->pw = pswf_consts(params.digits);
+    } catch (const HpdmkError& e) {
+        throw HpdmkError(HPDMK_EINVAL, e.what());
+    }
+    h->tree = build_octree(n_src, r_src, params.L, params.n_per_leaf);
+    h->n_src = n_src;
+    const Octree& tr = h->tree;
+    const int D = tr.max_depth;
+    if (D > kMaxLevels - 1) throw HpdmkError(HPDMK_ERUNTIME, "tree deeper than supported");
+    if (stream) {
+        h->stream = stream;
+        h->own_stream = false;
+    } else {
+        HIPX(hipStreamCreate(&h->stream));
+    }
+    auto& o = h->owned;
+
+    // ---- slots: leaf B owns [start, start + cap)
+    const int nn = tr.n_nodes;
+    std::vector<int> cap(nn, 0), start(nn, 0), count(nn, 0);
+    int total = 0;
+    for (int i = 0; i < nn; ++i) {
+        if (!tr.is_leaf[i]) continue;
+        count[i] = tr.leaf_count[i];
+        cap[i] = ((std::max(2 * count[i], count[i] + 64) + 3) / 4) * 4;
+        start[i] = total;
+        total += cap[i];
+        ++h->n_leaves;
+    }
+    std::vector<double> px(total, 0), py(total, 0), pz(total, 0), pq(total, 0);
+    std::vector<int> porig(total, -1), slot_of(n_src), leaf_of(n_src);
+    {
+        int pos = 0;
+        for (int i = 0; i < nn; ++i) {
+            if (!tr.is_leaf[i]) continue;
+            for (int k = 0; k < count[i]; ++k) {
+                const int id = tr.order[pos++];
+                const int s = start[i] + k;
+                px[s] = r_src[3 * id];
+                py[s] = r_src[3 * id + 1];
+                pz[s] = r_src[3 * id + 2];
+                pq[s] = charge[id];
+                porig[s] = id;
+                slot_of[id] = s;
+                leaf_of[id] = i;
+            }
+        }
+    }
+    for (int i = 0; i < n_src; ++i) h->Q += charge[i] * charge[i];  // src/tree.cpp:306-309
+
+    // ---- boxes carrying expansions: levels 2..D-2
+    std::vector<int> pw_slot(nn, -1), pn_node, pn_level, pn_lo, pn_hi;
+    for (int i = 0; i < nn; ++i)
+        if (tr.level[i] >= 2 && tr.level[i] <= D - 2) {
+            pw_slot[i] = (int)pn_node.size();
+            pn_node.push_back(i);
+            pn_level.push_back(tr.level[i]);
+            pn_lo.push_back(start[tr.first_leaf[i]]);
+            pn_hi.push_back(start[tr.last_leaf[i]] + cap[tr.last_leaf[i]]);
+        }
+    h->n_pw = (int)pn_node.size();
+    std::vector<int> leaves;
+    int max_near = 0;
+    for (int i = 0; i < nn; ++i)
+        if (tr.is_leaf[i]) {
+            leaves.push_back(i);
+            max_near = std::max(max_near, tr.near_off[i + 1] - tr.near_off[i]);
+        }
+
+    // ---- scalars shared by both precisions
+    auto set_common = [&](auto& d) {
+        d.n_src = n_src;
+        d.n_nodes = nn;
+        d.max_depth = D;
+        d.n_window = h->pw.n_window;
+        d.n_diff = h->pw.n_diff;
+        d.dw = 2 * d.n_window + 1;
+        d.dd = 2 * d.n_diff + 1;
+        d.Mw = d.dw * d.dw * (d.n_window + 1);
+        d.Md = d.dd * d.dd * (d.n_diff + 1);
+        d.n_pw_nodes = h->n_pw;
+        d.total_slots = total;
+        d.max_near = max_near;
+        d.L = params.L;
+        for (int l = 0; l <= kMaxLevels; ++l) d.boxsize[l] = params.L / std::ldexp(1.0, l);
+        d.delta_k[0] = d.delta_k[1] = 2 * M_PI / params.L;
+        for (int l = 2; l <= kMaxLevels; ++l) d.delta_k[l] = (2.0 * M_PI / 3.0) / d.boxsize[l];
+        const double tp3 = std::pow(2 * M_PI, 3);
+        d.cW = std::pow(d.delta_k[0], 3) / (2 * tp3);
+        for (int l = 0; l < kMaxLevels; ++l) {
+            d.cD[l] = std::pow(d.delta_k[l], 3) / (2 * tp3);
+            d.self_diff[l] = h->pw.psi0_0 * (1.0 / (2 * d.boxsize[l + 1] * h->pw.C0) -
+                                             1.0 / (2 * d.boxsize[l] * h->pw.C0));
+        }
+        d.self_window = h->pw.psi0_0 / (2 * d.boxsize[2] * h->pw.C0);
+        d.n_poly = (int)h->pw.poly.size();
+        for (int k = 0; k < d.n_poly; ++k) d.poly[k] = h->pw.poly[k];
+    };
+    std::vector<int> col_in(26 * (size_t)nn, -1);
+    for (int i = 0; i < nn; ++i)
+        for (int c = 0; c < 26; ++c) {
+            const int C = tr.colleague[26 * i + c];
+            if (C >= 0 && !tr.is_leaf[C] && pw_slot[C] >= 0) col_in[26 * i + c] = pw_slot[C];
+        }
+    std::vector<int8_t> leaf8(tr.is_leaf.begin(), tr.is_leaf.end());
+    const int* child = dupload(tr.child, o);
+    const int* level = dupload(tr.level, o);
+    const int8_t* isl = dupload(leaf8, o);
+    const int* cellp = dupload(tr.cell, o);
+    const int* coll = dupload(tr.colleague, o);
+    const int* colin = dupload(col_in, o);
+    const int* pws = dupload(pw_slot, o);
+    const int* noff = dupload(tr.near_off, o);
+    const int* nids = dupload(tr.near_ids, o);
+    const int8_t* nimg = dupload(tr.near_img, o);
+    const int* sstart = dupload(start, o);
+    const int* scap = dupload(cap, o);
+    int* scount = dupload(count, o);
+    double* dpx = dupload(px, o);
+    double* dpy = dupload(py, o);
+    double* dpz = dupload(pz, o);
+    double* dpq = dupload(pq, o);
+    int* dporig = dupload(porig, o);
+    int* dslot = dupload(slot_of, o);
+    int* dleaf = dupload(leaf_of, o);
+    h->err = dalloc<int>(1, o);
+    HIPX(hipMemset(h->err, 0, sizeof(int)));
+    auto set_ptrs = [&](auto& d) {
+        d.child = child;
+        d.level = level;
+        d.is_leaf = isl;
+        d.cell = cellp;
+        d.colleague = coll;
+        d.col_in_slot = colin;
+        d.pw_slot = pws;
+        d.near_off = noff;
+        d.near_ids = nids;
+        d.near_img = nimg;
+        d.slot_start = sstart;
+        d.slot_cap = scap;
+        d.slot_count = scount;
+        d.px = dpx;
+        d.py = dpy;
+        d.pz = dpz;
+        d.pq = dpq;
+        d.porig = dporig;
+        d.slot_of = dslot;
+        d.leaf_of = dleaf;
+        d.err = h->err;
+    };
+    std::vector<double> wtab = window_table(h->pw, params.L);
+    std::vector<std::vector<double>> dtabs;
+    for (int l = 2; l <= D - 2; ++l) dtabs.push_back(difference_table(h->pw, params.L, l));
+    if (precision == HPDMK_FP64) {
+        set_common(h->d64);
+        set_ptrs(h->d64);
+        fill_dev(h.get(), h->d64, dtabs, wtab);
+    } else {
+        set_common(h->d32);
+        set_ptrs(h->d32);
+        fill_dev(h.get(), h->d32, dtabs, wtab);
+    }
+    h->pn.node = dupload(pn_node, o);
+    h->pn.level = dupload(pn_level, o);
+    h->pn.lo = dupload(pn_lo, o);
+    h->pn.hi = dupload(pn_hi, o);
+    h->leaves = dupload(leaves, o);
+    const int Mw = (2 * h->pw.n_window + 1) * (2 * h->pw.n_window + 1) * (h->pw.n_window + 1);
+    h->acc_root = dalloc<double>(2 * Mw, o);
+    h->energy = dalloc<double>(4, o);
+    h->t_done = dalloc<unsigned>(kMaxBatch, o);
+    h->sc.part = dalloc<double>(kPartStride * (size_t)kMaxBatch * trial_blocks(D), o);
+    h->sc.done = h->t_done;
+    h->sc.info = dalloc<TrialInfo>(kMaxBatch, o);
+    h->sc.accept = dalloc<int>(kMaxBatch, o);
+    h->sc.corr = dalloc<double>(kMaxBatch * kMaxBatch, o);
+    h->sc.ovl = dalloc<unsigned char>(kMaxBatch * kMaxBatch, o);
+    const size_t tsz = precision == HPDMK_FP64 ? sizeof(Cplx<double>) : sizeof(Cplx<float>);
+    h->sc.tabs = dalloc<char>(tsz * (size_t)table_stride(D) * kMaxBatch, o);
+    h->sc.mark = dalloc<unsigned char>(2 * (size_t)h->n_pw + 2, o);
+    h->sc.dest = dalloc<int>(2 * (size_t)h->n_pw + 2, o);
+    h->sc.plan = dalloc<int>(2, o);
+    h->probe = dalloc<unsigned long long>(2 * kProbes, o);
+    HIPX(hipMemset(h->probe, 0, sizeof(unsigned long long) * 2 * kProbes));
+    HIPX(hipMemset(h->sc.mark, 0, 2 * (size_t)h->n_pw + 2));
+    HIPX(hipMemset(h->sc.plan, 0, 2 * sizeof(int)));
+    HIPX(hipMemset(h->t_done, 0, sizeof(unsigned) * kMaxBatch));
+    HIPX(hipDeviceSynchronize());
+    return h.release();
 }
 
-}  // namespace
+void ensure_moves(Handle* h, MoveBuf& b, int n) {
+    if (n <= b.cap) return;
+    auto& o = h->owned;
+    const int c = std::max(n, 1024);
+    b.idx = dalloc<int>(c, o);
+    b.d = dalloc<double>(3 * (size_t)c, o);
+    b.u = dalloc<double>(c, o);
+    b.dE = dalloc<double>(c, o);
+    b.dE_stale = dalloc<double>(c, o);
+    b.parts = dalloc<double>(5 * (size_t)c, o);
+    b.acc = dalloc<int>(c, o);
+    b.stats = dalloc<int>(kStatsPerMove * (size_t)c, o);
+    b.cap = c;
+}
+
+void upload_moves(Handle* h, MoveBuf& b, int n, const long* idx, const double* d, const double* u) {
+    for (int k = 0; k < n; ++k)
+        if (idx[k] < 0 || idx[k] >= h->n_src) throw HpdmkError(HPDMK_EINVAL, "particle index out of range");
+    ensure_moves(h, b, n);
+    std::vector<int> i32(idx, idx + n);
+    b.h_idx.assign(idx, idx + n);
+    HIPX(hipMemcpyAsync(b.idx, i32.data(), sizeof(int) * n, hipMemcpyHostToDevice, h->stream));
+    HIPX(hipMemcpyAsync(b.d, d, sizeof(double) * 3 * n, hipMemcpyHostToDevice, h->stream));
+    if (u) HIPX(hipMemcpyAsync(b.u, u, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+    HIPX(hipStreamSynchronize(h->stream));
+    b.n = n;
+}
+
+McArgs mc_args(Handle* h, const MoveBuf& b, int move0, int K, int mode, double beta) {
+    McArgs a{};
+    a.s = h->sc;
+    a.s.probe = h->profiling >= 2 ? h->probe : nullptr;
+    a.ml.idx = b.idx;
+    a.ml.d = b.d;
+    a.ml.u = b.u;
+    a.ml.dE = b.dE;
+    a.ml.dE_stale = b.dE_stale;
+    a.ml.parts = b.parts;
+    a.ml.accept_out = b.acc;
+    a.ml.stats = h->profiling ? b.stats : nullptr;  // counting runs in the finalizer: profiled runs only
+    a.pn = h->pn;
+    a.move0 = move0;
+    a.K = K;
+    a.mode = mode;
+    a.beta = beta;
+    const bool chain = mode == kTrialMetropolis || mode == kTrialBatch;
+    a.lB = chain ? h->bjerrum : 1.0;
+    a.sigma2 = chain ? h->sigma * h->sigma : 0.0;
+    return a;
+}
+
+}  // namespace capi
+}  // namespace hpdmk_amd
+
+using namespace hpdmk_amd::capi;
 
 extern "C" {
 
@@ -231,201 +351,9 @@ const char* hpdmk_last_error(void) { return g_err.c_str(); }
 int hpdmk_tree_create_ex(HPDMKParams params, int precision, int n_src, const double* r_src, const double* charge,
                          hpdmk_tree* out) {
     return guard([&] {
-        if (!out || !r_src || !charge) throw HpdmkError(HPDMK_EINVAL, "null argument");
+        if (!out) throw HpdmkError(HPDMK_EINVAL, "null argument");
         *out = nullptr;
-        if (precision != HPDMK_FP64 && precision != HPDMK_MIXED)
-            throw HpdmkError(HPDMK_EINVAL, "precision must be HPDMK_FP64 or HPDMK_MIXED");
-        if (precision == HPDMK_MIXED && params.digits != 3 && params.digits != 6)
-            throw HpdmkError(HPDMK_EINVAL, "float only supports 3 and 6 digits accuracy");  // src/tree.cpp:19-23
-        if (params.n_per_leaf < 1) throw HpdmkError(HPDMK_EINVAL, "n_per_leaf must be positive");
-        auto h = std::make_unique<Handle>();
-        h->params = params;
-        h->precision = precision;
-        try {
-            h->pw = pswf_consts(params.digits);
-        } catch (const HpdmkError& e) {
-            throw HpdmkError(HPDMK_EINVAL, e.what());
-        }
-        h->tree = build_octree(n_src, r_src, params.L, params.n_per_leaf);
-        h->n_src = n_src;
-        const Octree& tr = h->tree;
-        const int D = tr.max_depth;
-        if (D > kMaxLevels - 1) throw HpdmkError(HPDMK_ERUNTIME, "tree deeper than supported");
-        HIPX(hipStreamCreate(&h->stream));
-        auto& o = h->owned;
-
-        // ---- slots: leaf B owns [start, start + cap)
-        const int nn = tr.n_nodes;
-        std::vector<int> cap(nn, 0), start(nn, 0), count(nn, 0);
-        int total = 0;
-        for (int i = 0; i < nn; ++i) {
-            if (!tr.is_leaf[i]) continue;
-            count[i] = tr.leaf_count[i];
-            cap[i] = ((std::max(2 * count[i], count[i] + 64) + 3) / 4) * 4;
-            start[i] = total;
-            total += cap[i];
-            ++h->n_leaves;
-        }
-        std::vector<double> px(total, 0), py(total, 0), pz(total, 0), pq(total, 0);
-        std::vector<int> porig(total, -1), slot_of(n_src), leaf_of(n_src);
-        {
-            int pos = 0;
-            for (int i = 0; i < nn; ++i) {
-                if (!tr.is_leaf[i]) continue;
-                for (int k = 0; k < count[i]; ++k) {
-                    const int id = tr.order[pos++];
-                    const int s = start[i] + k;
-                    px[s] = r_src[3 * id];
-                    py[s] = r_src[3 * id + 1];
-                    pz[s] = r_src[3 * id + 2];
-                    pq[s] = charge[id];
-                    porig[s] = id;
-                    slot_of[id] = s;
-                    leaf_of[id] = i;
-                }
-            }
-        }
-        for (int i = 0; i < n_src; ++i) h->Q += charge[i] * charge[i];  // src/tree.cpp:306-309
-
-        // ---- boxes carrying expansions: levels 2..D-2
-        std::vector<int> pw_slot(nn, -1), pn_node, pn_level, pn_lo, pn_hi;
-        for (int i = 0; i < nn; ++i)
-            if (tr.level[i] >= 2 && tr.level[i] <= D - 2) {
-                pw_slot[i] = (int)pn_node.size();
-                pn_node.push_back(i);
-                pn_level.push_back(tr.level[i]);
-                pn_lo.push_back(start[tr.first_leaf[i]]);
-                pn_hi.push_back(start[tr.last_leaf[i]] + cap[tr.last_leaf[i]]);
-            }
-        h->n_pw = (int)pn_node.size();
-        std::vector<int> leaves;
-        int max_near = 0;
-        for (int i = 0; i < nn; ++i)
-            if (tr.is_leaf[i]) {
-                leaves.push_back(i);
-                max_near = std::max(max_near, tr.near_off[i + 1] - tr.near_off[i]);
-            }
-
-        // ---- scalars shared by both precisions
-        auto set_common = [&](auto& d) {
-            d.n_src = n_src;
-            d.n_nodes = nn;
-            d.max_depth = D;
-            d.n_window = h->pw.n_window;
-            d.n_diff = h->pw.n_diff;
-            d.dw = 2 * d.n_window + 1;
-            d.dd = 2 * d.n_diff + 1;
-            d.Mw = d.dw * d.dw * (d.n_window + 1);
-            d.Md = d.dd * d.dd * (d.n_diff + 1);
-            d.n_pw_nodes = h->n_pw;
-            d.total_slots = total;
-            d.max_near = max_near;
-            d.L = params.L;
-            for (int l = 0; l <= kMaxLevels; ++l) d.boxsize[l] = params.L / std::ldexp(1.0, l);
-            d.delta_k[0] = d.delta_k[1] = 2 * M_PI / params.L;
-            for (int l = 2; l <= kMaxLevels; ++l) d.delta_k[l] = (2.0 * M_PI / 3.0) / d.boxsize[l];
-            const double tp3 = std::pow(2 * M_PI, 3);
-            d.cW = std::pow(d.delta_k[0], 3) / (2 * tp3);
-            for (int l = 0; l < kMaxLevels; ++l) {
-                d.cD[l] = std::pow(d.delta_k[l], 3) / (2 * tp3);
-                d.self_diff[l] = h->pw.psi0_0 * (1.0 / (2 * d.boxsize[l + 1] * h->pw.C0) -
-                                                 1.0 / (2 * d.boxsize[l] * h->pw.C0));
-            }
-            d.self_window = h->pw.psi0_0 / (2 * d.boxsize[2] * h->pw.C0);
-            d.n_poly = (int)h->pw.poly.size();
-            for (int k = 0; k < d.n_poly; ++k) d.poly[k] = h->pw.poly[k];
-        };
-        std::vector<int> col_in(26 * (size_t)nn, -1);
-        for (int i = 0; i < nn; ++i)
-            for (int c = 0; c < 26; ++c) {
-                const int C = tr.colleague[26 * i + c];
-                if (C >= 0 && !tr.is_leaf[C] && pw_slot[C] >= 0) col_in[26 * i + c] = pw_slot[C];
-            }
-        std::vector<int8_t> leaf8(tr.is_leaf.begin(), tr.is_leaf.end());
-        const int* child = dupload(tr.child, o);
-        const int* level = dupload(tr.level, o);
-        const int8_t* isl = dupload(leaf8, o);
-        const int* cellp = dupload(tr.cell, o);
-        const int* coll = dupload(tr.colleague, o);
-        const int* colin = dupload(col_in, o);
-        const int* pws = dupload(pw_slot, o);
-        const int* noff = dupload(tr.near_off, o);
-        const int* nids = dupload(tr.near_ids, o);
-        const int8_t* nimg = dupload(tr.near_img, o);
-        const int* sstart = dupload(start, o);
-        const int* scap = dupload(cap, o);
-        int* scount = dupload(count, o);
-        double* dpx = dupload(px, o);
-        double* dpy = dupload(py, o);
-        double* dpz = dupload(pz, o);
-        double* dpq = dupload(pq, o);
-        int* dporig = dupload(porig, o);
-        int* dslot = dupload(slot_of, o);
-        int* dleaf = dupload(leaf_of, o);
-        h->err = dalloc<int>(1, o);
-        HIPX(hipMemset(h->err, 0, sizeof(int)));
-        auto set_ptrs = [&](auto& d) {
-            d.child = child;
-            d.level = level;
-            d.is_leaf = isl;
-            d.cell = cellp;
-            d.colleague = coll;
-            d.col_in_slot = colin;
-            d.pw_slot = pws;
-            d.near_off = noff;
-            d.near_ids = nids;
-            d.near_img = nimg;
-            d.slot_start = sstart;
-            d.slot_cap = scap;
-            d.slot_count = scount;
-            d.px = dpx;
-            d.py = dpy;
-            d.pz = dpz;
-            d.pq = dpq;
-            d.porig = dporig;
-            d.slot_of = dslot;
-            d.leaf_of = dleaf;
-            d.err = h->err;
-        };
-        std::vector<double> wtab = window_table(h->pw, params.L);
-        std::vector<std::vector<double>> dtabs;
-        for (int l = 2; l <= D - 2; ++l) dtabs.push_back(difference_table(h->pw, params.L, l));
-        if (precision == HPDMK_FP64) {
-            set_common(h->d64);
-            set_ptrs(h->d64);
-            fill_dev(h.get(), h->d64, dtabs, wtab, cap, start, count);
-        } else {
-            set_common(h->d32);
-            set_ptrs(h->d32);
-            fill_dev(h.get(), h->d32, dtabs, wtab, cap, start, count);
-        }
-        h->pn.node = dupload(pn_node, o);
-        h->pn.level = dupload(pn_level, o);
-        h->pn.lo = dupload(pn_lo, o);
-        h->pn.hi = dupload(pn_hi, o);
-        h->leaves = dupload(leaves, o);
-        const int Mw = (2 * h->pw.n_window + 1) * (2 * h->pw.n_window + 1) * (h->pw.n_window + 1);
-        h->acc_root = dalloc<double>(2 * Mw, o);
-        h->energy = dalloc<double>(4, o);
-        h->t_done = dalloc<unsigned>(kMaxBatch, o);
-        h->sc.part = dalloc<double>(kPartStride * (size_t)kMaxBatch * trial_blocks(D), o);
-        h->sc.done = h->t_done;
-        h->sc.info = dalloc<TrialInfo>(kMaxBatch, o);
-        h->sc.accept = dalloc<int>(kMaxBatch, o);
-        h->sc.corr = dalloc<double>(kMaxBatch * kMaxBatch, o);
-        h->sc.ovl = dalloc<unsigned char>(kMaxBatch * kMaxBatch, o);
-        const size_t tsz = precision == HPDMK_FP64 ? sizeof(Cplx<double>) : sizeof(Cplx<float>);
-        h->sc.tabs = dalloc<char>(tsz * (size_t)table_stride(D) * kMaxBatch, o);
-        h->sc.mark = dalloc<unsigned char>(2 * (size_t)h->n_pw + 2, o);
-        h->sc.dest = dalloc<int>(2 * (size_t)h->n_pw + 2, o);
-        h->sc.plan = dalloc<int>(2, o);
-        h->probe = dalloc<unsigned long long>(2 * kProbes, o);
-        HIPX(hipMemset(h->probe, 0, sizeof(unsigned long long) * 2 * kProbes));
-        HIPX(hipMemset(h->sc.mark, 0, 2 * (size_t)h->n_pw + 2));
-        HIPX(hipMemset(h->sc.plan, 0, 2 * sizeof(int)));
-        HIPX(hipMemset(h->t_done, 0, sizeof(unsigned) * kMaxBatch));
-        HIPX(hipDeviceSynchronize());
-        *out = h.release();
+        *out = create_handle(params, precision, n_src, r_src, charge, nullptr);
     });
 }
 
@@ -436,6 +364,7 @@ int hpdmk_tree_create(HPDMKParams params, int n_src, const double* r_src, const 
 int hpdmk_tree_destroy(hpdmk_tree t) {
     return guard([&] {
         Handle* h = H(t);
+        if (h->borrowed) throw HpdmkError(HPDMK_EINVAL, "this tree belongs to a replica set (hpdmk_replicas_destroy)");
         (void)hipStreamSynchronize(h->stream);
         delete h;
     });
@@ -494,58 +423,6 @@ int hpdmk_eval_energy(hpdmk_tree t, double* energy, double* parts) {
     });
 }
 
-namespace {
-void ensure_moves(Handle* h, MoveBuf& b, int n) {
-    if (n <= b.cap) return;
-    auto& o = h->owned;
-    const int c = std::max(n, 1024);
-    b.idx = dalloc<int>(c, o);
-    b.d = dalloc<double>(3 * (size_t)c, o);
-    b.u = dalloc<double>(c, o);
-    b.dE = dalloc<double>(c, o);
-    b.dE_stale = dalloc<double>(c, o);
-    b.parts = dalloc<double>(5 * (size_t)c, o);
-    b.acc = dalloc<int>(c, o);
-    b.stats = dalloc<int>(kStatsPerMove * (size_t)c, o);
-    b.cap = c;
-}
-
-void upload_moves(Handle* h, MoveBuf& b, int n, const long* idx, const double* d, const double* u) {
-    for (int k = 0; k < n; ++k)
-        if (idx[k] < 0 || idx[k] >= h->n_src) throw HpdmkError(HPDMK_EINVAL, "particle index out of range");
-    ensure_moves(h, b, n);
-    std::vector<int> i32(idx, idx + n);
-    b.h_idx.assign(idx, idx + n);
-    HIPX(hipMemcpyAsync(b.idx, i32.data(), sizeof(int) * n, hipMemcpyHostToDevice, h->stream));
-    HIPX(hipMemcpyAsync(b.d, d, sizeof(double) * 3 * n, hipMemcpyHostToDevice, h->stream));
-    if (u) HIPX(hipMemcpyAsync(b.u, u, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
-    HIPX(hipStreamSynchronize(h->stream));
-    b.n = n;
-}
-
-McArgs mc_args(Handle* h, const MoveBuf& b, int move0, int K, int mode, double beta) {
-    McArgs a{};
-    a.s = h->sc;
-    a.s.probe = h->profiling >= 2 ? h->probe : nullptr;
-    a.ml.idx = b.idx;
-    a.ml.d = b.d;
-    a.ml.u = b.u;
-    a.ml.dE = b.dE;
-    a.ml.dE_stale = b.dE_stale;
-    a.ml.parts = b.parts;
-    a.ml.accept_out = b.acc;
-    a.ml.stats = h->profiling ? b.stats : nullptr;  // counting runs in the finalizer: profiled runs only
-    a.pn = h->pn;
-    a.move0 = move0;
-    a.K = K;
-    a.mode = mode;
-    a.beta = beta;
-    const bool chain = mode == kTrialMetropolis || mode == kTrialBatch;
-    a.lB = chain ? h->bjerrum : 1.0;
-    a.sigma2 = chain ? h->sigma * h->sigma : 0.0;
-    return a;
-}
-}  // namespace
 
 int hpdmk_eval_shift_energy(hpdmk_tree t, long i, double dx, double dy, double dz, double* dE, double* parts) {
     return guard([&] {

@@ -55,6 +55,14 @@ SIGNATURES = {
     "hpdmk_clear_error": (C.c_int, [C.c_void_p]),
     "hpdmk_copy_probe": (C.c_int, [C.c_long, C.c_int, _dp]),
     "hpdmk_mc_set_model": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    "hpdmk_replicas_create": (C.c_int, [C.c_int, C.c_void_p, C.c_int, _ip, _dp, _dp, C.POINTER(C.c_void_p)]),
+    "hpdmk_replicas_destroy": (C.c_int, [C.c_void_p]),
+    "hpdmk_replicas_count": (C.c_int, [C.c_void_p, _ip]),
+    "hpdmk_replicas_tree": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "hpdmk_replicas_form_pw": (C.c_int, [C.c_void_p]),
+    "hpdmk_replicas_mc_load": (C.c_int, [C.c_void_p, C.c_int, _lp, _dp, _dp]),
+    "hpdmk_replicas_mc_run": (C.c_int, [C.c_void_p, C.c_double, C.c_int, _dp]),
+    "hpdmk_replicas_mc_results": (C.c_int, [C.c_void_p, _dp, _ip]),
     "hpdmk_phase_times": (C.c_int, [C.c_void_p, _dp, _ip]),
     "hpdmk_plan_tree": (C.c_int, [HPDMKParams, C.c_int, _dp, _ip, _ip, _ip]),
     "hpdmk_pswf_constants": (C.c_int, [C.c_int, _dp]),
@@ -119,7 +127,7 @@ def copy_probe(n_bytes=256 << 20, reps=5):
 class HPDMKPtTree:
     """Device-resident DMK system; method names follow include/tree.hpp:98-132."""
 
-    def __init__(self, params, r_src, charge, precision=HPDMK_FP64):
+    def __init__(self, params, r_src, charge, precision=HPDMK_FP64, _borrowed=None):
         L = load_library()
         r, pr = _d(r_src)
         q, pq = _d(charge)
@@ -128,8 +136,12 @@ class HPDMKPtTree:
         self.params = params
         self.n_src = q.size
         self.precision = precision
-        h = C.c_void_p()
-        _check(L.hpdmk_tree_create_ex(params, precision, q.size, pr, pq, C.byref(h)))
+        self._owned = _borrowed is None
+        if _borrowed is None:
+            h = C.c_void_p()
+            _check(L.hpdmk_tree_create_ex(params, precision, q.size, pr, pq, C.byref(h)))
+        else:
+            h = _borrowed
         self._h = h
         info = np.zeros(10, dtype=np.int32)
         _check(L.hpdmk_tree_info(h, info.ctypes.data_as(_ip)))
@@ -139,7 +151,8 @@ class HPDMKPtTree:
     # ---- lifecycle
     def close(self):
         if getattr(self, "_h", None):
-            load_library().hpdmk_tree_destroy(self._h)
+            if self._owned:
+                load_library().hpdmk_tree_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -256,3 +269,72 @@ class HPDMKPtTree:
 
     def clear_error(self):
         _check(load_library().hpdmk_clear_error(self._h))
+
+
+class HPDMKReplicas:
+    """A set of independent systems of different sizes advanced together
+    (one Metropolis move of every replica per step, steps replayed from a HIP
+    graph).  systems: list of (r_src, charge); params: one HPDMKParams per
+    replica.  tree(r) gives replica r as an HPDMKPtTree owned by the set."""
+
+    def __init__(self, params, systems, precision=HPDMK_FP64):
+        L = load_library()
+        if len(params) != len(systems):
+            raise ValueError("one HPDMKParams per replica")
+        self.n = len(systems)
+        self.n_src = np.array([len(q) for _, q in systems], dtype=np.int32)
+        r_all, pr = _d(np.concatenate([np.asarray(r, dtype=np.float64).reshape(-1) for r, _ in systems]))
+        q_all, pq = _d(np.concatenate([np.asarray(q, dtype=np.float64) for _, q in systems]))
+        arr = (HPDMKParams * self.n)(*params)
+        h = C.c_void_p()
+        _check(L.hpdmk_replicas_create(self.n, C.cast(arr, C.c_void_p), precision, self.n_src.ctypes.data_as(_ip),
+                                       pr, pq, C.byref(h)))
+        self._h = h
+        self._trees = []
+        for k, (r, q) in enumerate(systems):
+            th = C.c_void_p()
+            _check(L.hpdmk_replicas_tree(h, k, C.byref(th)))
+            self._trees.append(HPDMKPtTree(params[k], r, q, precision, _borrowed=th))
+        self._n_moves = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for t in self._trees:
+                t.close()
+            load_library().hpdmk_replicas_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def tree(self, r):
+        return self._trees[r]
+
+    def form_pw(self):
+        _check(load_library().hpdmk_replicas_form_pw(self._h))
+
+    def mc_load(self, idx, d, u):
+        """idx (n_rep, M) replica-local indices, d (n_rep, M, 3), u (n_rep, M)."""
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        M = idx.shape[1]
+        d, pd = _d(np.asarray(d).reshape(-1))
+        u, pu = _d(np.asarray(u).reshape(-1))
+        if idx.shape[0] != self.n or d.size != 3 * self.n * M or u.size != self.n * M:
+            raise ValueError("move arrays must be (n_rep, M), (n_rep, M, 3), (n_rep, M)")
+        self._keep = (idx, d, u)
+        _check(load_library().hpdmk_replicas_mc_load(self._h, M, idx.ctypes.data_as(_lp), pd, pu))
+        self._n_moves = M
+
+    def mc_run(self, beta=1.0, graph_steps=64):
+        ms = C.c_double()
+        _check(load_library().hpdmk_replicas_mc_run(self._h, float(beta), int(graph_steps), C.byref(ms)))
+        return ms.value
+
+    def mc_results(self):
+        dE = np.zeros((self.n, self._n_moves))
+        acc = np.zeros((self.n, self._n_moves), dtype=np.int32)
+        _check(load_library().hpdmk_replicas_mc_results(self._h, dE.ctypes.data_as(_dp), acc.ctypes.data_as(_ip)))
+        return dE, acc
