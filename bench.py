@@ -89,9 +89,13 @@ def main():
     ap.add_argument("--cpu-moves", type=int, default=12000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2026)
-    ap.add_argument("--legs", default="batched,plain,accuracy,cpu,copy",
-                    help="comma list of batched, plain, accuracy, cpu, copy (copy = a 1 GiB streaming "
-                         "copy: measured HBM ceiling and byte-counter calibration)")
+    ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--replica-moves", type=int, default=1000)
+    ap.add_argument("--graph-steps", type=int, default=64)
+    ap.add_argument("--legs", default="batched,plain,accuracy,cpu,copy,replicas",
+                    help="comma list of batched, plain, accuracy, cpu, copy, replicas (copy = a 1 GiB "
+                         "streaming copy: measured HBM ceiling and byte-counter calibration; replicas = "
+                         "256 ragged independent systems stepped together)")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_update_batched_n1e5.json"),
                     help="measured HBM bytes per launch of the dominant kernel (from rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -260,6 +264,42 @@ def main():
             cpu["decisions_equal_to_plain_gpu"] = f"{int((acc_c[:n_cmp] == acc_plain_ref).sum())}/{n_cmp}"
         print(f"# cpu: {cpu}", file=sys.stderr, flush=True)
 
+    rep_info = None
+    if "replicas" in legs:
+        # 256 independent systems, N_r ~ U{2000..20000}, density 0.1, one move of
+        # every replica per step, steps replayed from a HIP graph
+        rng = np.random.default_rng(args.seed + 7)
+        systems, rparams = [], []
+        for _ in range(args.replicas):
+            n_r = int(rng.integers(2000, 20001))
+            L_r = (n_r / 0.1) ** (1.0 / 3.0)
+            systems.append((rng.uniform(0, L_r, (n_r, 3)), np.where(np.arange(n_r) % 2 == 0, 1.0, -1.0)))
+            rparams.append(P.HPDMKParams(args.leaf, args.digits, L_r, init=P.DIRECT))
+        t0 = time.perf_counter()
+        rs = P.HPDMKReplicas(rparams, systems, prec)
+        rs.form_pw()
+        t_rb = time.perf_counter() - t0
+        Mr = args.replica_moves
+        ridx = np.stack([rng.integers(0, len(q), Mr + 100) for _, q in systems])
+        rd = rng.uniform(-1.0, 1.0, (len(systems), Mr + 100, 3))
+        ru = rng.uniform(0, 1, (len(systems), Mr + 100))
+        rs.mc_load(ridx[:, :100], rd[:, :100], ru[:, :100])
+        rs.mc_run(1.0, args.graph_steps)  # warm-up (graph instantiation, first touches)
+        rs.mc_load(ridx[:, 100:], rd[:, 100:], ru[:, 100:])
+        t0 = time.perf_counter()
+        r_ms = rs.mc_run(1.0, args.graph_steps)
+        r_wall = time.perf_counter() - t0
+        _, racc = rs.mc_results()
+        depths = [rs.tree(k).max_depth for k in range(rs.n)]
+        rs.close()
+        rep_info = {"value": round(len(systems) * Mr / r_wall, 1), "unit": "trial moves/s",
+                    "workload": f"{len(systems)} replicas, N_r ~ U{{2000..20000}}, L_r = (N_r/0.1)^(1/3), +-1, "
+                                f"digits {args.digits}, leaf {args.leaf}, {Mr} moves each",
+                    "n_total": int(sum(len(q) for _, q in systems)), "levels": [min(depths), max(depths)],
+                    "graph_steps": args.graph_steps, "gpu_ms": round(r_ms, 2), "wall_s": round(r_wall, 3),
+                    "build_s": round(t_rb, 2), "acceptance": round(float(racc.mean()), 4)}
+        print(f"# replicas: {rep_info}", file=sys.stderr, flush=True)
+
     copy_gbs = None
     if "copy" in legs:
         # streaming copy of 1 GiB: the measured HBM ceiling, and the
@@ -282,6 +322,7 @@ def main():
         "acceptance": round(float(acc_b.mean()), 4),
         "roofline": dict(roofline, measured_copy_gbs=round(copy_gbs, 1) if copy_gbs else None),
         "cpu_baseline": cpu,
+        "replicas": rep_info,
         "accuracy": acc_info,
     }
     print(json.dumps(out))

@@ -105,9 +105,9 @@ __global__ void __launch_bounds__(TB) k_setup(DevTree<T> t, McArgs a) {
 
 // one trial per replica: replica blockIdx.z
 template <typename T>
-__global__ void __launch_bounds__(TB) k_setup_rep(const DevTree<T>* ts, const McArgs* as) {
-    const DevTree<T> t = ts[blockIdx.z];
-    const McArgs a = as[blockIdx.z];
+__global__ void __launch_bounds__(TB) k_setup_rep(const DevTree<T>* __restrict__ ts, const McArgs* __restrict__ as) {
+    const DevTree<T>& t = ts[blockIdx.z];
+    const McArgs& a = as[blockIdx.z];
     setup_body(t, a);
 }
 
@@ -450,9 +450,9 @@ __global__ void __launch_bounds__(TB) k_trial(DevTree<T> t, McArgs a) {
 // replica blockIdx.z; the grid is sized for the deepest replica and the
 // blocks a shallower replica does not use return at once
 template <typename T>
-__global__ void __launch_bounds__(TB) k_trial_rep(const DevTree<T>* ts, const McArgs* as) {
-    const DevTree<T> t = ts[blockIdx.z];
-    const McArgs a = as[blockIdx.z];
+__global__ void __launch_bounds__(TB) k_trial_rep(const DevTree<T>* __restrict__ ts, const McArgs* __restrict__ as) {
+    const DevTree<T>& t = ts[blockIdx.z];
+    const McArgs& a = as[blockIdx.z];
     if ((int)blockIdx.x >= trial_blocks(t.max_depth)) return;
     trial_body(t, a);
 }
@@ -766,9 +766,9 @@ __global__ void __launch_bounds__(TB) k_compact(DevTree<T> t, const int* leaves)
 }
 
 template <typename T>
-__global__ void __launch_bounds__(TB) k_compact_rep(const DevTree<T>* ts, const int* const* leaves, const int* n_leaves) {
+__global__ void __launch_bounds__(TB) k_compact_rep(const DevTree<T>* __restrict__ ts, const int* const* leaves, const int* n_leaves) {
     if ((int)blockIdx.x >= n_leaves[blockIdx.z]) return;
-    const DevTree<T> t = ts[blockIdx.z];
+    const DevTree<T>& t = ts[blockIdx.z];
     compact_body(t, leaves[blockIdx.z]);
 }
 
@@ -799,6 +799,10 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
     const int nD = a.s.plan[0];
     if (!a.s.plan[1]) return;
     const int K = a.K, D = t.max_depth, nL = n_levels_pw(D), stride = table_stride(D);
+    constexpr int CH = TB * kApplyPer;
+    const int nWb = (t.Mw + CH - 1) / CH, nDb = (t.Md + CH - 1) / CH;
+    const int total = nWb + nD * nDb;
+    if ((int)blockIdx.x >= total) return;  // no work item for this block
     for (int m = threadIdx.x; m < K; m += blockDim.x) ac_s[m] = a.s.accept[m];
     for (int w = threadIdx.x; w < 2 * K * kApplyLv; w += blockDim.x) {
         const int li = w % kApplyLv, me = w / kApplyLv, m = me >> 1, ep = me & 1;
@@ -807,9 +811,6 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
         const int l = 2 + li;
         pnode[w] = (l <= lam && l <= D - 2) ? (ep == 0 ? ti.path_t[l] : ti.path_o[l]) : -1;
     }
-    constexpr int CH = TB * kApplyPer;
-    const int nWb = (t.Mw + CH - 1) / CH, nDb = (t.Md + CH - 1) / CH;
-    const int total = nWb + nD * nDb;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
         const bool is_root = item < nWb;
@@ -952,9 +953,9 @@ __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(TB) k_apply_rep(const DevTree<T>* ts, const McArgs* as) {
-    const DevTree<T> t = ts[blockIdx.z];
-    const McArgs a = as[blockIdx.z];
+__global__ void __launch_bounds__(TB) k_apply_rep(const DevTree<T>* __restrict__ ts, const McArgs* __restrict__ as) {
+    const DevTree<T>& t = ts[blockIdx.z];
+    const McArgs& a = as[blockIdx.z];
     apply_body(t, a);
 }
 
