@@ -238,9 +238,24 @@ def main():
         e64 = g64.eval_shift_energy_batch(sidx, sd)
         em = gm.eval_shift_energy_batch(sidx, sd)
         E64 = g64.eval_energy()
+        Em = gm.eval_energy()
         acc_info = {"dE_vs_fp64_max_abs": float(np.abs(em - e64).max()),
                     "dE_vs_fp64_max_rel_to_E": float(np.abs(em - e64).max() / abs(E64)),
-                    "E_mixed_vs_fp64_rel": float(abs(gm.eval_energy() - E64) / abs(E64)), "sample": 256}
+                    "E_mixed_vs_fp64_rel": float(abs(Em - E64) / abs(E64)), "sample": 256}
+        # against FP64 Ewald (s = 5, alpha = 0.5: r_c = 10, 161^3 modes), the
+        # reference's error measure (dE_ewald - dE) / E_ewald (test/hpdmk_test.cpp:465)
+        t0 = time.perf_counter()
+        ew = P.Ewald(args.L, 5.0, 0.5, 1.0, q, r)
+        E_ew = ew.compute_energy()
+        dE_ew = ew.shift_chain(sidx, sd)
+        ew.close()
+        acc_info.update({"ewald": {"s": 5.0, "alpha": 0.5, "E": E_ew, "seconds": round(time.perf_counter() - t0, 3)},
+                         "E_mixed_vs_ewald_rel": float(abs(Em - E_ew) / abs(E_ew)),
+                         "E_fp64_vs_ewald_rel": float(abs(E64 - E_ew) / abs(E_ew)),
+                         "dE_mixed_vs_ewald_max_rel_to_E": float(np.abs(em - dE_ew).max() / abs(E_ew)),
+                         "dE_mixed_vs_ewald_mean_rel_to_E": float(np.abs(em - dE_ew).mean() / abs(E_ew)),
+                         "dE_fp64_vs_ewald_max_rel_to_E": float(np.abs(e64 - dE_ew).max() / abs(E_ew))})
+        print(f"# accuracy: {acc_info}", file=sys.stderr, flush=True)
         g64.close()
         gm.close()
 

@@ -181,6 +181,26 @@ int hpdmk_replicas_mc_run(hpdmk_replicas set, double beta, int graph_steps, doub
 /* dE and decisions, replica-major like the move list */
 int hpdmk_replicas_mc_results(hpdmk_replicas set, double* dE, int* accept);
 
+/* ---- FP64 Ewald reporter -------------------------------------------------
+ * The reference's Ewald oracle (include/ewald.hpp:14-57, src/ewald.cpp:17-308)
+ * on the GPU, all-FP64: the ground truth for DMK energies at sizes where the
+ * CPU sum is too slow.  r_c = s/alpha must be below L/2 (minimum image) and
+ * 2 s alpha L / (2 pi) at most 127. */
+typedef void* hpdmk_ewald;
+/* Ewald(L, s, alpha, eps, q, r, n) and its structure factor */
+int hpdmk_ewald_create(double L, double s, double alpha, double eps, int n, const double* q, const double* r,
+                       hpdmk_ewald* ewald);
+int hpdmk_ewald_destroy(hpdmk_ewald ewald);
+/* compute_energy (src/ewald.cpp:178-191); parts (optional, 3): short, long, self */
+int hpdmk_ewald_compute_energy(hpdmk_ewald ewald, double* energy, double* parts);
+/* compute_potential (src/ewald.cpp:281-308) at n points (3 doubles each) */
+int hpdmk_ewald_compute_potential(hpdmk_ewald ewald, int n, const double* points, double* phi);
+/* Energy change of each move of a chain (particle idx[k] displaced by
+ * d[3k..3k+2], wrapped); moves with apply[k] != 0 are applied before the
+ * next move (none when apply is NULL). */
+int hpdmk_ewald_shift_chain(hpdmk_ewald ewald, int n_moves, const long* idx, const double* d, const int* apply,
+                            double* dE);
+
 /* HBM streaming-copy probe (no counterpart in the reference; a measurement
  * aid): copies `bytes` between two fresh device buffers `reps` times and
  * returns read+write GB/s.  Each launch reads and writes `bytes` once, which

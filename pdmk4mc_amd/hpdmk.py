@@ -55,6 +55,12 @@ SIGNATURES = {
     "hpdmk_clear_error": (C.c_int, [C.c_void_p]),
     "hpdmk_copy_probe": (C.c_int, [C.c_long, C.c_int, _dp]),
     "hpdmk_mc_set_model": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    "hpdmk_ewald_create": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp,
+                                     C.POINTER(C.c_void_p)]),
+    "hpdmk_ewald_destroy": (C.c_int, [C.c_void_p]),
+    "hpdmk_ewald_compute_energy": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "hpdmk_ewald_compute_potential": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
+    "hpdmk_ewald_shift_chain": (C.c_int, [C.c_void_p, C.c_int, _lp, _dp, _ip, _dp]),
     "hpdmk_replicas_create": (C.c_int, [C.c_int, C.c_void_p, C.c_int, _ip, _dp, _dp, C.POINTER(C.c_void_p)]),
     "hpdmk_replicas_destroy": (C.c_int, [C.c_void_p]),
     "hpdmk_replicas_count": (C.c_int, [C.c_void_p, _ip]),
@@ -338,3 +344,53 @@ class HPDMKReplicas:
         acc = np.zeros((self.n, self._n_moves), dtype=np.int32)
         _check(load_library().hpdmk_replicas_mc_results(self._h, dE.ctypes.data_as(_dp), acc.ctypes.data_as(_ip)))
         return dE, acc
+
+
+class Ewald:
+    """FP64 Ewald summation on the GPU; names follow include/ewald.hpp:14-57."""
+
+    def __init__(self, L, s, alpha, eps, q, r):
+        q, pq = _d(q)
+        r, pr = _d(np.asarray(r).reshape(-1))
+        if r.size != 3 * q.size:
+            raise ValueError("r must hold 3 coordinates per charge")
+        self.n = q.size
+        h = C.c_void_p()
+        _check(load_library().hpdmk_ewald_create(float(L), float(s), float(alpha), float(eps), q.size, pq, pr,
+                                                 C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().hpdmk_ewald_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def compute_energy(self, parts=False):
+        e = C.c_double()
+        p = np.zeros(3)
+        _check(load_library().hpdmk_ewald_compute_energy(self._h, C.byref(e), p.ctypes.data_as(_dp)))
+        return (e.value, p) if parts else e.value
+
+    def compute_potential(self, points):
+        pts, pp = _d(np.asarray(points).reshape(-1))
+        phi = np.zeros(pts.size // 3)
+        _check(load_library().hpdmk_ewald_compute_potential(self._h, phi.size, pp, phi.ctypes.data_as(_dp)))
+        return phi
+
+    def shift_chain(self, idx, d, apply=None):
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        d, pd = _d(d)
+        out = np.zeros(idx.size)
+        ap = None
+        if apply is not None:
+            apply = np.ascontiguousarray(apply, dtype=np.int32)
+            ap = apply.ctypes.data_as(_ip)
+        _check(load_library().hpdmk_ewald_shift_chain(self._h, idx.size, idx.ctypes.data_as(_lp), pd, ap,
+                                                      out.ctypes.data_as(_dp)))
+        return out
