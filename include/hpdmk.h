@@ -118,7 +118,7 @@ int hpdmk_mc_load(hpdmk_tree tree, int n_moves, const long* idx, const double* d
  * +infinity.  Reported chain dE values are bjerrum x dE.  Defaults 1 and 0
  * (no hard core); sigma may not exceed the finest box size. */
 int hpdmk_mc_set_model(hpdmk_tree tree, double bjerrum, double sigma);
-int hpdmk_mc_run(hpdmk_tree tree, double beta, int mode, int batch, double* gpu_ms);
+int hpdmk_mc_run(hpdmk_tree tree, double beta, int mode, int batch, double* gpu_ms);  /* batch 0: tuned */
 int hpdmk_mc_results(hpdmk_tree tree, double* dE, int* accept);
 
 /* Per-move counts of the last profiled run (6 ints per move): difference levels read
@@ -153,6 +153,21 @@ int hpdmk_plan_tree(HPDMKParams params, int n_src, const double* r_src, int* n_n
 /* PSWF constants of a digits setting: c, lambda, C0, diff0, psi0(0),
  * n_window, n_diff (out[7]). */
 int hpdmk_pswf_constants(int digits, double* out);
+
+/* ---- launch shape and tile autotuner (no reference counterpart) --------
+ * tiles[4]: blocks per trial for the window sum, per (endpoint, difference
+ * level), per endpoint's near field (each 1..16), and blocks of the update
+ * kernel; batch: trials per batched step used when hpdmk_mc_run gets batch 0.
+ * The chain's results do not depend on them (up to FP summation order). */
+int hpdmk_set_tiles(hpdmk_tree tree, const int* tiles, int batch);
+/* out[5]: the four tiles and the batch size */
+int hpdmk_get_tiles(hpdmk_tree tree, int* out);
+/* Times the batched chain on a twin of the current state (this system is
+ * not changed) over the given sample moves, searching batch size then each
+ * tile in turn, keeps the fastest shape (hpdmk_get_tiles) and returns it in
+ * out[5] and its GPU time in *best_ms. */
+int hpdmk_autotune(hpdmk_tree tree, int n_moves, const long* idx, const double* d, const double* u, double beta,
+                   int* out, double* best_ms);
 
 /* ---- replica sets (no reference counterpart) --------------------------
  * Many independent systems of different sizes (ragged), each a full

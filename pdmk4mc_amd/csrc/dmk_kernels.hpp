@@ -43,14 +43,21 @@ enum TrialMode : int {
     kTrialBatch = 3,        // partial sums only; the batch scan assembles dE
 };
 
-// blocks per trial in k_trial: window, difference (per endpoint and level),
-// near field (per endpoint); each block strides over its share of the work
-constexpr int kWinBlocks = 2;
-constexpr int kDiffBlocks = 8;
-constexpr int kNearBlocks = 8;
-__host__ __device__ inline int trial_blocks(int max_depth) {
+// Launch shape of the trial and update kernels (tile autotuner's search
+// space).  k_trial gives each trial `win` window blocks, `diff` blocks per
+// (endpoint, difference level) and `near` blocks per endpoint's near field;
+// each block strides over its share.  k_apply runs `apply` blocks.
+struct Tiles {
+    int win = 2, diff = 8, near = 8, apply = 1024;
+};
+constexpr int kMaxRoleBlocks = 16;  // upper bound of win, diff, near (scratch sizing)
+__host__ __device__ inline int trial_blocks(int max_depth, const Tiles& g) {
     const int nL = max_depth >= 4 ? max_depth - 3 : 0;
-    return kWinBlocks + 2 * nL * kDiffBlocks + 2 * kNearBlocks;
+    return g.win + 2 * nL * g.diff + 2 * g.near;
+}
+__host__ __device__ inline int max_trial_blocks(int max_depth) {
+    const int nL = max_depth >= 4 ? max_depth - 3 : 0;
+    return kMaxRoleBlocks * (1 + 2 * nL + 2);
 }
 
 // device scratch of the Monte Carlo kernels (sized for kMaxBatch trials)
@@ -108,6 +115,7 @@ struct McArgs {
     // closer than sigma to another particle is rejected (sigma2 = sigma^2, 0 = off)
     double lB;
     double sigma2;
+    Tiles tiles;
 };
 
 // One Metropolis step of every replica of a set (one move each): per-replica
@@ -120,6 +128,7 @@ struct ReplicaLaunch {
     const int* const* leaves; // n leaf lists
     const int* n_leaves;      // n
     int n, max_depth, max_leaves, apply_blocks;
+    Tiles tiles;  // trial shape (the same for every replica)
 };
 
 template <typename T>

@@ -251,20 +251,19 @@ __device__ int capacity_ok(const DevTree<T>& t, const TrialInfo& ti, int delta) 
 // counter); plain loads then see them.
 template <typename T>
 __device__ void sum_trial_partials(const DevTree<T>& t, const McArgs& a, int tr, double* v) {
-    const int D = t.max_depth, nL = n_levels_pw(D), per = trial_blocks(D);
+    const Tiles& g = a.tiles;
+    const int D = t.max_depth, nL = n_levels_pw(D), per = trial_blocks(D, g);
     const double* p = a.s.part + kPartStride * (size_t)tr * per;
-    const int d0 = kWinBlocks, d1 = d0 + 2 * nL * kDiffBlocks;
+    const int d0 = g.win, d1 = d0 + 2 * nL * g.diff;
     double w = 0, df = 0, rt = 0, ro = 0, nt = 0, no = 0, ov = 0;
-#pragma unroll
-    for (int b = 0; b < kWinBlocks; ++b) w += p[kPartStride * b];
+    for (int b = 0; b < g.win; ++b) w += p[kPartStride * b];
     for (int b = d0; b < d1; ++b) df += p[kPartStride * b];
-#pragma unroll
-    for (int b = 0; b < kNearBlocks; ++b) {
+    for (int b = 0; b < g.near; ++b) {
         rt += p[kPartStride * (d1 + b)];
         nt += p[kPartStride * (d1 + b) + 1];
         ov += p[kPartStride * (d1 + b) + 2];
-        ro += p[kPartStride * (d1 + kNearBlocks + b)];
-        no += p[kPartStride * (d1 + kNearBlocks + b) + 1];
+        ro += p[kPartStride * (d1 + g.near + b)];
+        no += p[kPartStride * (d1 + g.near + b) + 1];
     }
     v[0] = w;
     v[1] = df;
@@ -313,10 +312,10 @@ __device__ double assemble_trial(const DevTree<T>& t, const McArgs& a, int tr, i
     return dE;
 }
 
-// grid (trial_blocks(D), K).  Block roles inside a trial (each strides over its share):
-//   [0, kWinBlocks)                        window
-//   [.., + 2*nL*kDiffBlocks)               difference (endpoint, level, share)
-//   [.., + 2*kNearBlocks)                  near field (endpoint, share of the near list)
+// grid (trial_blocks(D, tiles), K).  Block roles inside a trial (each strides over its share):
+//   [0, win)                               window
+//   [.., + 2*nL*diff)                      difference (endpoint, level, share)
+//   [.., + 2*near)                         near field (endpoint, share of the near list)
 // Every block stores its partial sum; in single-move modes the last block of
 // the trial assembles dE, decides and plans the update.
 template <typename T>
@@ -326,15 +325,16 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
     if (*t.err) return;
     const int tr = blockIdx.y, b = blockIdx.x, m = move_base(a) + tr;
     const TrialInfo& ti = a.s.info[tr];
-    const int D = t.max_depth, nL = n_levels_pw(D), per = trial_blocks(D);
+    const int D = t.max_depth, nL = n_levels_pw(D), per = trial_blocks(D, a.tiles);
     const Cplx<T>* tb = tabs_of<T>(a.s, table_stride(D), tr);
     double part = 0.0;
     int nscan = 0, novl = 0;
-    if (b < kWinBlocks) {
+    const int WB = a.tiles.win, DB = a.tiles.diff, NB = a.tiles.near;
+    if (b < WB) {
         // window: 2 cW sum W Re[(tau - omega) conj(rho - omega)] (include/vecops.hpp:118-145)
         const int dw = t.dw;
 #pragma unroll 4
-        for (int e = b * TB + threadIdx.x; e < t.Mw; e += kWinBlocks * TB) {
+        for (int e = b * TB + threadIdx.x; e < t.Mw; e += WB * TB) {
             const Cplx<T> ta = phase_at(tb + root_row(0, 0), tb + root_row(0, 1), tb + root_row(0, 2), dw, e);
             const Cplx<T> om = phase_at(tb + root_row(1, 0), tb + root_row(1, 1), tb + root_row(1, 2), dw, e);
             const Cplx<T> g = t.root[e];
@@ -342,11 +342,11 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
             part += (double)((ar * gr + ai * gi) * t.Wmat[e]);
         }
         part *= 2.0 * t.cW;
-    } else if (b < kWinBlocks + 2 * nL * kDiffBlocks) {
+    } else if (b < WB + 2 * nL * DB) {
         // difference levels strictly above the endpoint's leaf (src/energy_shift.cpp:92-140)
-        const int r = b - kWinBlocks;
-        const int ep = r / (nL * kDiffBlocks);  // 0 target, 1 origin
-        const int li = (r / kDiffBlocks) % nL;
+        const int r = b - WB;
+        const int ep = r / (nL * DB);  // 0 target, 1 origin
+        const int li = (r / DB) % nL;
         const int l = 2 + li;
         const int lam = ep == 0 ? ti.lam_t : ti.lam_o;
         if (l <= lam - 1) {
@@ -357,7 +357,7 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
             const Cplx<T>* Y = tb + level_row(nL, ep, li, 1, 0);
             const Cplx<T>* Z = tb + level_row(nL, ep, li, 2, 0);
 #pragma unroll 4
-            for (int e = (r % kDiffBlocks) * TB + threadIdx.x; e < t.Md; e += kDiffBlocks * TB) {
+            for (int e = (r % DB) * TB + threadIdx.x; e < t.Md; e += DB * TB) {
                 const Cplx<T> v = in[e];
                 const Cplx<T> w = phase_at(X, Y, Z, t.dd, e);
                 T re = w.x * v.x - w.y * v.y;
@@ -368,8 +368,8 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
         }
     } else {
         // near field of the endpoint's leaf (src/energy_shift.cpp:233-302, direct_eval.hpp:309-403)
-        const int r = b - kWinBlocks - 2 * nL * kDiffBlocks;
-        const int ep = r / kNearBlocks;
+        const int r = b - WB - 2 * nL * DB;
+        const int ep = r / NB;
         const int lam = ep == 0 ? ti.lam_t : ti.lam_o;
         const int leaf = ep == 0 ? ti.path_t[lam] : ti.path_o[lam];
         const int off = t.near_off[leaf], cnt = t.near_off[leaf + 1] - off;
@@ -384,7 +384,7 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
         const bool hs = ep == 0 && a.sigma2 > 0.0;
         const int own = hs ? t.slot_of[ti.i] : -1;
         double acc = 0;
-        for (int k = r % kNearBlocks; k < cnt; k += kNearBlocks) {
+        for (int k = r % NB; k < cnt; k += NB) {
             const int nb = t.near_ids[off + k];
             const int code = t.near_img[off + k];
             const double sx = (code % 3 - 1) * t.L, sy = ((code / 3) % 3 - 1) * t.L, sz = (code / 9 - 1) * t.L;
@@ -453,7 +453,7 @@ template <typename T>
 __global__ void __launch_bounds__(TB) k_trial_rep(const DevTree<T>* __restrict__ ts, const McArgs* __restrict__ as) {
     const DevTree<T>& t = ts[blockIdx.z];
     const McArgs& a = as[blockIdx.z];
-    if ((int)blockIdx.x >= trial_blocks(t.max_depth)) return;
+    if ((int)blockIdx.x >= trial_blocks(t.max_depth, a.tiles)) return;
     trial_body(t, a);
 }
 
@@ -968,7 +968,7 @@ void launch_setup(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent
 
 template <typename T>
 void launch_trial(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
-    hipExtLaunchKernelGGL(k_trial<T>, dim3(trial_blocks(t.max_depth), a.K), dim3(TB), 0, st, e0, e1, 0, t, a);
+    hipExtLaunchKernelGGL(k_trial<T>, dim3(trial_blocks(t.max_depth, a.tiles), a.K), dim3(TB), 0, st, e0, e1, 0, t, a);
 }
 
 template <typename T>
@@ -984,7 +984,7 @@ void launch_scan(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_
 
 template <typename T>
 void launch_apply(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
-    hipExtLaunchKernelGGL(k_apply<T>, dim3(1024), dim3(TB), 0, st, e0, e1, 0, t, a);
+    hipExtLaunchKernelGGL(k_apply<T>, dim3(a.tiles.apply), dim3(TB), 0, st, e0, e1, 0, t, a);
 }
 
 template <typename T>
@@ -996,7 +996,8 @@ template <typename T>
 void launch_replica_step(const ReplicaLaunch<T>& r, hipStream_t st) {
     const dim3 rep(1, 1, r.n);
     hipLaunchKernelGGL(k_setup_rep<T>, rep, dim3(TB), 0, st, r.trees, r.args);
-    hipLaunchKernelGGL(k_trial_rep<T>, dim3(trial_blocks(r.max_depth), 1, r.n), dim3(TB), 0, st, r.trees, r.args);
+    hipLaunchKernelGGL(k_trial_rep<T>, dim3(trial_blocks(r.max_depth, r.tiles), 1, r.n), dim3(TB), 0, st, r.trees,
+                       r.args);
     hipLaunchKernelGGL(k_apply_rep<T>, dim3(r.apply_blocks, 1, r.n), dim3(TB), 0, st, r.trees, r.args);
     hipLaunchKernelGGL(k_advance, dim3((r.n + TB - 1) / TB), dim3(TB), 0, st, r.move_ctr, r.n);
 }

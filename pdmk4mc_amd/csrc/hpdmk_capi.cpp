@@ -258,7 +258,7 @@ Handle* create_handle(HPDMKParams params, int precision, int n_src, const double
     h->acc_root = dalloc<double>(2 * Mw, o);
     h->energy = dalloc<double>(4, o);
     h->t_done = dalloc<unsigned>(kMaxBatch, o);
-    h->sc.part = dalloc<double>(kPartStride * (size_t)kMaxBatch * trial_blocks(D), o);
+    h->sc.part = dalloc<double>(kPartStride * (size_t)kMaxBatch * max_trial_blocks(D), o);
     h->sc.done = h->t_done;
     h->sc.info = dalloc<TrialInfo>(kMaxBatch, o);
     h->sc.accept = dalloc<int>(kMaxBatch, o);
@@ -323,10 +323,90 @@ McArgs mc_args(Handle* h, const MoveBuf& b, int move0, int K, int mode, double b
     a.K = K;
     a.mode = mode;
     a.beta = beta;
+    a.tiles = h->tiles;
     const bool chain = mode == kTrialMetropolis || mode == kTrialBatch;
     a.lB = chain ? h->bjerrum : 1.0;
     a.sigma2 = chain ? h->sigma * h->sigma : 0.0;
     return a;
+}
+
+// Runs the loaded move list as a Metropolis chain (mode 0 plain, 1 batched;
+// batch 0 = the tuned batch size); returns the GPU time in ms.
+double run_chain(Handle* h, double beta, int mode, int batch) {
+    MoveBuf& mb = h->chain;
+    const int n = mb.n;
+    if (n <= 0) throw HpdmkError(HPDMK_EINVAL, "no moves loaded (hpdmk_mc_load)");
+    if (mode != 0 && mode != 1) throw HpdmkError(HPDMK_EINVAL, "mode must be 0 (plain) or 1 (batched)");
+    if (mode == 1 && batch == 0) batch = h->tuned_batch;
+    if (mode == 1 && (batch < 1 || batch > kMaxBatch)) throw HpdmkError(HPDMK_EINVAL, "batch must be in [0, 256]");
+    for (int k = 0; k < kKinds; ++k) {
+        h->kms[k] = 0;
+        h->kcount[k] = 0;
+    }
+    if (h->profiling >= 2) HIPX(hipMemsetAsync(h->probe, 0, sizeof(unsigned long long) * 2 * kProbes, h->stream));
+    // batches never repeat a particle (the corrections assume distinct particles)
+    std::vector<std::pair<int, int>> batches;
+    if (mode == 1) {
+        std::vector<int> seen(h->n_src, -1);
+        int m0 = 0;
+        for (int m = 0; m < n; ++m) {
+            const int p = (int)mb.h_idx[m];
+            if (m - m0 == batch || seen[p] >= m0) {
+                batches.push_back({m0, m - m0});
+                m0 = m;
+            }
+            seen[p] = m;
+        }
+        batches.push_back({m0, n - m0});
+    }
+    struct Ev {
+        hipEvent_t e = nullptr;
+        ~Ev() {
+            if (e) (void)hipEventDestroy(e);
+        }
+    } e0, e1;
+    HIPX(hipEventCreate(&e0.e));
+    HIPX(hipEventCreate(&e1.e));
+    HIPX(hipEventRecord(e0.e, h->stream));
+    dispatch(h, [&](auto& dv) {
+        auto compact_if_due = [&](int moved) {
+            h->since_compact += moved;
+            if (h->since_compact >= kCompactEvery) {
+                launch_compact(dv, h->leaves, h->n_leaves, h->stream);
+                h->since_compact = 0;
+            }
+        };
+        if (mode == 0) {
+            for (int m = 0; m < n; ++m) {
+                const McArgs a = mc_args(h, mb, m, 1, kTrialMetropolis, beta);
+                timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
+                timed(h, kKTrial, [&](hipEvent_t x, hipEvent_t y) { launch_trial(dv, a, h->stream, x, y); });
+                timed(h, kKUpdate, [&](hipEvent_t x, hipEvent_t y) { launch_apply(dv, a, h->stream, x, y); });
+                compact_if_due(1);
+            }
+        } else {
+            for (auto& bt : batches) {
+                const int m0 = bt.first, k = bt.second;
+                const McArgs a = mc_args(h, mb, m0, k, kTrialBatch, beta);
+                timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
+                timed(h, kKTrial, [&](hipEvent_t x, hipEvent_t y) { launch_trial(dv, a, h->stream, x, y); });
+                if (k > 1)
+                    timed(h, kKCorr,
+                          [&](hipEvent_t x, hipEvent_t y) { launch_pair_corr(dv, a, h->stream, x, y); });
+                timed(h, kKScan, [&](hipEvent_t x, hipEvent_t y) { launch_scan(dv, a, h->stream, x, y); });
+                timed(h, kKUpdate, [&](hipEvent_t x, hipEvent_t y) { launch_apply(dv, a, h->stream, x, y); });
+                compact_if_due(k);
+            }
+        }
+    });
+    HIPX(hipGetLastError());
+    HIPX(hipEventRecord(e1.e, h->stream));
+    HIPX(hipEventSynchronize(e1.e));
+    float ms = 0;
+    HIPX(hipEventElapsedTime(&ms, e0.e, e1.e));
+    collect_events(h);
+    check_err(h);
+    return ms;
 }
 
 }  // namespace capi
@@ -515,77 +595,8 @@ int hpdmk_mc_load(hpdmk_tree t, int n, const long* idx, const double* d, const d
 
 int hpdmk_mc_run(hpdmk_tree t, double beta, int mode, int batch, double* gpu_ms) {
     return guard([&] {
-        Handle* h = H(t);
-        MoveBuf& mb = h->chain;
-        const int n = mb.n;
-        if (n <= 0) throw HpdmkError(HPDMK_EINVAL, "no moves loaded (hpdmk_mc_load)");
-        if (mode != 0 && mode != 1) throw HpdmkError(HPDMK_EINVAL, "mode must be 0 (plain) or 1 (batched)");
-        if (mode == 1 && (batch < 1 || batch > kMaxBatch)) throw HpdmkError(HPDMK_EINVAL, "batch must be in [1, 256]");
-        for (int k = 0; k < kKinds; ++k) {
-            h->kms[k] = 0;
-            h->kcount[k] = 0;
-        }
-        if (h->profiling >= 2) HIPX(hipMemsetAsync(h->probe, 0, sizeof(unsigned long long) * 2 * kProbes, h->stream));
-        // batches never repeat a particle (the corrections assume distinct particles)
-        std::vector<std::pair<int, int>> batches;
-        if (mode == 1) {
-            std::vector<int> seen(h->n_src, -1);
-            int m0 = 0;
-            for (int m = 0; m < n; ++m) {
-                const int p = (int)mb.h_idx[m];
-                if (m - m0 == batch || seen[p] >= m0) {
-                    batches.push_back({m0, m - m0});
-                    m0 = m;
-                }
-                seen[p] = m;
-            }
-            batches.push_back({m0, n - m0});
-        }
-        hipEvent_t e0, e1;
-        HIPX(hipEventCreate(&e0));
-        HIPX(hipEventCreate(&e1));
-        HIPX(hipEventRecord(e0, h->stream));
-        dispatch(h, [&](auto& dv) {
-            auto compact_if_due = [&](int moved) {
-                h->since_compact += moved;
-                if (h->since_compact >= kCompactEvery) {
-                    launch_compact(dv, h->leaves, h->n_leaves, h->stream);
-                    h->since_compact = 0;
-                }
-            };
-            if (mode == 0) {
-                for (int m = 0; m < n; ++m) {
-                    const McArgs a = mc_args(h, mb, m, 1, kTrialMetropolis, beta);
-                    timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
-                    timed(h, kKTrial, [&](hipEvent_t x, hipEvent_t y) { launch_trial(dv, a, h->stream, x, y); });
-                    timed(h, kKUpdate, [&](hipEvent_t x, hipEvent_t y) { launch_apply(dv, a, h->stream, x, y); });
-                    compact_if_due(1);
-                }
-            } else {
-                for (auto& bt : batches) {
-                    const int m0 = bt.first, k = bt.second;
-                    const McArgs a = mc_args(h, mb, m0, k, kTrialBatch, beta);
-                    timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
-                    timed(h, kKTrial, [&](hipEvent_t x, hipEvent_t y) { launch_trial(dv, a, h->stream, x, y); });
-                    if (k > 1)
-                        timed(h, kKCorr,
-                              [&](hipEvent_t x, hipEvent_t y) { launch_pair_corr(dv, a, h->stream, x, y); });
-                    timed(h, kKScan, [&](hipEvent_t x, hipEvent_t y) { launch_scan(dv, a, h->stream, x, y); });
-                    timed(h, kKUpdate, [&](hipEvent_t x, hipEvent_t y) { launch_apply(dv, a, h->stream, x, y); });
-                    compact_if_due(k);
-                }
-            }
-        });
-        HIPX(hipGetLastError());
-        HIPX(hipEventRecord(e1, h->stream));
-        HIPX(hipEventSynchronize(e1));
-        float ms = 0;
-        HIPX(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        collect_events(h);
+        const double ms = run_chain(H(t), beta, mode, batch);
         if (gpu_ms) *gpu_ms = ms;
-        check_err(h);
     });
 }
 
@@ -708,6 +719,118 @@ int hpdmk_mc_set_model(hpdmk_tree t, double bjerrum, double sigma) {
             throw HpdmkError(HPDMK_EINVAL, "sigma exceeds the finest box size " + std::to_string(finest));
         h->bjerrum = bjerrum;
         h->sigma = sigma;
+    });
+}
+
+int hpdmk_set_tiles(hpdmk_tree t, const int* tiles, int batch) {
+    return guard([&] {
+        Handle* h = H(t);
+        if (!tiles) throw HpdmkError(HPDMK_EINVAL, "null tiles");
+        for (int k = 0; k < 3; ++k)
+            if (tiles[k] < 1 || tiles[k] > kMaxRoleBlocks)
+                throw HpdmkError(HPDMK_EINVAL, "blocks per trial role must be in [1, 16]");
+        if (tiles[3] < 1 || tiles[3] > 65535) throw HpdmkError(HPDMK_EINVAL, "update blocks must be in [1, 65535]");
+        if (batch < 1 || batch > kMaxBatch) throw HpdmkError(HPDMK_EINVAL, "batch must be in [1, 256]");
+        h->tiles.win = tiles[0];
+        h->tiles.diff = tiles[1];
+        h->tiles.near = tiles[2];
+        h->tiles.apply = tiles[3];
+        h->tuned_batch = batch;
+    });
+}
+
+int hpdmk_get_tiles(hpdmk_tree t, int* out) {
+    return guard([&] {
+        Handle* h = H(t);
+        out[0] = h->tiles.win;
+        out[1] = h->tiles.diff;
+        out[2] = h->tiles.near;
+        out[3] = h->tiles.apply;
+        out[4] = h->tuned_batch;
+    });
+}
+
+int hpdmk_autotune(hpdmk_tree t, int n_moves, const long* idx, const double* d, const double* u, double beta,
+                   int* out, double* best_ms) {
+    return guard([&] {
+        Handle* h = H(t);
+        if (n_moves < 1 || !idx || !d || !u) throw HpdmkError(HPDMK_EINVAL, "empty or null move list");
+        // a twin of the current state takes the timing runs; this system is untouched
+        const int n = h->n_src;
+        std::vector<double> r(3 * (size_t)n), q(n);
+        dispatch(h, [&](auto& dv) {
+            const int S = dv.total_slots;
+            std::vector<double> x(S), y(S), z(S), c(S);
+            std::vector<int> orig(S);
+            HIPX(hipMemcpy(x.data(), dv.px, sizeof(double) * S, hipMemcpyDeviceToHost));
+            HIPX(hipMemcpy(y.data(), dv.py, sizeof(double) * S, hipMemcpyDeviceToHost));
+            HIPX(hipMemcpy(z.data(), dv.pz, sizeof(double) * S, hipMemcpyDeviceToHost));
+            HIPX(hipMemcpy(c.data(), dv.pq, sizeof(double) * S, hipMemcpyDeviceToHost));
+            HIPX(hipMemcpy(orig.data(), dv.porig, sizeof(int) * S, hipMemcpyDeviceToHost));
+            for (int s = 0; s < S; ++s)
+                if (orig[s] >= 0) {
+                    r[3 * orig[s]] = x[s];
+                    r[3 * orig[s] + 1] = y[s];
+                    r[3 * orig[s] + 2] = z[s];
+                    q[orig[s]] = c[s];
+                }
+        });
+        std::unique_ptr<Handle> tw(create_handle(h->params, h->precision, n, r.data(), q.data(), nullptr));
+        tw->bjerrum = h->bjerrum;
+        tw->sigma = h->sigma;
+        dispatch(tw.get(), [&](auto& dv) {
+            launch_outgoing(dv, tw->pn, tw->acc_root, tw->stream);
+            launch_incoming(dv, tw->pn, tw->stream);
+        });
+        upload_moves(tw.get(), tw->chain, n_moves, idx, d, u);
+        // coordinate search from the current shape: batch, then each role in turn
+        Tiles best = h->tiles;
+        int best_batch = h->tuned_batch;
+        auto timeit = [&](const Tiles& g, int batch) {
+            tw->tiles = g;
+            run_chain(tw.get(), beta, 1, batch);  // first run: warm caches and code objects
+            return run_chain(tw.get(), beta, 1, batch);
+        };
+        double best_t = timeit(best, best_batch);
+        auto consider = [&](const Tiles& g, int batch) {
+            const double tm = timeit(g, batch);
+            if (tm < best_t) {
+                best_t = tm;
+                best = g;
+                best_batch = batch;
+            }
+        };
+        for (int b : {16, 32, 48, 64, 96, 128, 192, 256}) consider(best, b);
+        for (int v : {4, 8, 12, 16}) {
+            Tiles g = best;
+            g.near = v;
+            consider(g, best_batch);
+        }
+        for (int v : {2, 4, 8, 16}) {
+            Tiles g = best;
+            g.diff = v;
+            consider(g, best_batch);
+        }
+        for (int v : {1, 2, 4}) {
+            Tiles g = best;
+            g.win = v;
+            consider(g, best_batch);
+        }
+        for (int v : {256, 512, 1024, 2048}) {
+            Tiles g = best;
+            g.apply = v;
+            consider(g, best_batch);
+        }
+        h->tiles = best;
+        h->tuned_batch = best_batch;
+        if (out) {
+            out[0] = best.win;
+            out[1] = best.diff;
+            out[2] = best.near;
+            out[3] = best.apply;
+            out[4] = best_batch;
+        }
+        if (best_ms) *best_ms = best_t;
     });
 }
 

@@ -89,6 +89,10 @@ struct Handle {
     // move lists in HBM: `single` serves eval/update calls, `chain` the staged Metropolis run
     MoveBuf single, chain;
     int since_compact = 0;  // leaf changes (upper bound) since the last slot compaction
+    // launch shape of the trial and update kernels, and the batch size used
+    // when hpdmk_mc_run is given batch 0 (hpdmk_set_tiles, hpdmk_autotune)
+    Tiles tiles{};
+    int tuned_batch = 64;
     // Monte Carlo model of hpdmk_mc_run (hpdmk_mc_set_model)
     double bjerrum = 1.0;
     double sigma = 0.0;
@@ -169,6 +173,7 @@ Handle* create_handle(HPDMKParams params, int precision, int n_src, const double
 void ensure_moves(Handle* h, MoveBuf& b, int n);
 void upload_moves(Handle* h, MoveBuf& b, int n, const long* idx, const double* d, const double* u);
 McArgs mc_args(Handle* h, const MoveBuf& b, int move0, int K, int mode, double beta);
+double run_chain(Handle* h, double beta, int mode, int batch);
 
 }  // namespace capi
 }  // namespace hpdmk_amd

@@ -62,6 +62,7 @@ ReplicaLaunch<T> launch_args(ReplicaSet* s) {
     r.max_depth = s->max_depth;
     r.max_leaves = s->max_leaves;
     r.apply_blocks = s->apply_blocks;
+    r.tiles = Tiles{};
     return r;
 }
 
@@ -181,7 +182,10 @@ int hpdmk_replicas_mc_run(hpdmk_replicas s, double beta, int graph_steps, double
             a.ml.stats = nullptr;
             args.push_back(a);
         }
-        for (int r = 0; r < set->n(); ++r) args[r].move_ctr = set->move_ctr + r;
+        for (int r = 0; r < set->n(); ++r) {
+            args[r].move_ctr = set->move_ctr + r;
+            args[r].tiles = Tiles{};  // the grid of a replica step uses the default shape
+        }
         HIPX(hipMemcpyAsync(set->args, args.data(), sizeof(McArgs) * args.size(), hipMemcpyHostToDevice, set->stream));
         HIPX(hipMemsetAsync(set->move_ctr, 0, sizeof(int) * set->n(), set->stream));
         hipEvent_t e0 = nullptr, e1 = nullptr;

@@ -61,6 +61,9 @@ SIGNATURES = {
     "hpdmk_ewald_compute_energy": (C.c_int, [C.c_void_p, _dp, _dp]),
     "hpdmk_ewald_compute_potential": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
     "hpdmk_ewald_shift_chain": (C.c_int, [C.c_void_p, C.c_int, _lp, _dp, _ip, _dp]),
+    "hpdmk_set_tiles": (C.c_int, [C.c_void_p, _ip, C.c_int]),
+    "hpdmk_get_tiles": (C.c_int, [C.c_void_p, _ip]),
+    "hpdmk_autotune": (C.c_int, [C.c_void_p, C.c_int, _lp, _dp, _dp, C.c_double, _ip, _dp]),
     "hpdmk_replicas_create": (C.c_int, [C.c_int, C.c_void_p, C.c_int, _ip, _dp, _dp, C.POINTER(C.c_void_p)]),
     "hpdmk_replicas_destroy": (C.c_int, [C.c_void_p]),
     "hpdmk_replicas_count": (C.c_int, [C.c_void_p, _ip]),
@@ -231,6 +234,27 @@ class HPDMKPtTree:
     def mc_set_model(self, bjerrum=1.0, sigma=0.0):
         """Bjerrum scaling of the chain's energies and hard-sphere diameter (0 = none)."""
         _check(load_library().hpdmk_mc_set_model(self._h, float(bjerrum), float(sigma)))
+
+    def set_tiles(self, win=2, diff=8, near=8, apply=1024, batch=64):
+        t = np.array([win, diff, near, apply], dtype=np.int32)
+        _check(load_library().hpdmk_set_tiles(self._h, t.ctypes.data_as(_ip), int(batch)))
+
+    def tiles(self):
+        out = np.zeros(5, dtype=np.int32)
+        _check(load_library().hpdmk_get_tiles(self._h, out.ctypes.data_as(_ip)))
+        return dict(zip(["win", "diff", "near", "apply", "batch"], out.tolist()))
+
+    def autotune(self, idx, d, u, beta=1.0):
+        """Fastest launch shape for the batched chain, timed on a twin of the
+        current state over the sample moves; returns (tiles, best ms)."""
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        d, pd = _d(d)
+        u, pu = _d(u)
+        out = np.zeros(5, dtype=np.int32)
+        ms = C.c_double()
+        _check(load_library().hpdmk_autotune(self._h, idx.size, idx.ctypes.data_as(_lp), pd, pu, float(beta),
+                                             out.ctypes.data_as(_ip), C.byref(ms)))
+        return dict(zip(["win", "diff", "near", "apply", "batch"], out.tolist())), ms.value
 
     def mc_run(self, beta=1.0, mode=0, batch=64):
         ms = C.c_double()
