@@ -84,7 +84,9 @@ def main():
     ap.add_argument("--digits", type=int, default=6)
     ap.add_argument("--leaf", type=int, default=200)
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
-    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--batch", type=int, default=64, help="batch size when not tuned")
+    ap.add_argument("--tune", type=int, default=1, help="1: tile autotuner picks batch size and launch shape")
+    ap.add_argument("--tune-moves", type=int, default=4000)
     ap.add_argument("--plain-moves", type=int, default=3000)
     ap.add_argument("--cpu-moves", type=int, default=12000)
     ap.add_argument("--no-cpu", action="store_true")
@@ -126,14 +128,29 @@ def main():
     print(f"# tree: {g.n_nodes} nodes, {g.max_depth} levels, {g.n_leaves} leaves, max near {g.max_near}; "
           f"build+energy {t_build:.2f}s E={E0:.6f}", file=sys.stderr, flush=True)
 
+    # launch shape: the tile autotuner times the batched chain on a twin of
+    # this state over a separate sample of moves (outside the timed region)
+    tiles = None
+    batch = args.batch
+    if args.tune:
+        rng_t = np.random.default_rng(args.seed + 11)
+        ti = rng_t.integers(0, args.n, args.tune_moves)
+        td = rng_t.uniform(-1.0, 1.0, (args.tune_moves, 3))
+        tu = rng_t.uniform(0.0, 1.0, args.tune_moves)
+        t0 = time.time()
+        tiles, tune_ms = g.autotune(ti, td, tu)
+        batch = 0
+        print(f"# tuned shape {tiles} ({tune_ms:.2f} ms for {args.tune_moves} moves; "
+              f"{time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+
     # warmup then timed steps of the batched chain (moves already in HBM)
     if args.warmup:
         g.mc_load(idx[:args.warmup * M], d[:args.warmup * M], u[:args.warmup * M])
-        g.mc_run(1.0, 1, args.batch)
+        g.mc_run(1.0, 1, batch)
     sl = slice(args.warmup * M, n_total)
     g.mc_load(idx[sl], d[sl], u[sl])
     t0 = time.perf_counter()
-    gpu_ms = g.mc_run(1.0, 1, args.batch)
+    gpu_ms = g.mc_run(1.0, 1, batch)
     wall = time.perf_counter() - t0
     dE_b, acc_b = g.mc_results()
     stats_b = g.mc_stats()
@@ -151,9 +168,11 @@ def main():
         gp = P.HPDMKPtTree(params, r, q, prec)
         gp.form_outgoing_pw()
         gp.form_incoming_pw()
+        if tiles:
+            gp.set_tiles(**tiles)
         gp.set_profiling(level)
         gp.mc_load(idx[psl], d[psl], u[psl])
-        gp.mc_run(1.0, 1, args.batch)
+        gp.mc_run(1.0, 1, batch)
         return gp
 
     gp = profiled(1)
@@ -330,7 +349,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "N=1e5 periodic cube L=100, +-1 alternating charges, digits 6, leaf 200, "
                                "1e4 shifts/step dx~U[-1,1]^3, beta=1",
-                   "n": args.n, "L": args.L, "digits": args.digits, "leaf": args.leaf, "batch": args.batch,
+                   "n": args.n, "L": args.L, "digits": args.digits, "leaf": args.leaf,
+                   "batch": tiles["batch"] if tiles else args.batch, "tiles": tiles,
                    "moves_per_step": M, "precision": args.precision},
         "plain_gpu": plain_info,
         "batched_over_plain": round(value / plain, 3) if plain else None,
