@@ -165,7 +165,9 @@ int hpdmk_get_tiles(hpdmk_tree tree, int* out);
 /* Times the batched chain on a twin of the current state (this system is
  * not changed) over the given sample moves, searching batch size then each
  * tile in turn, keeps the fastest shape (hpdmk_get_tiles) and returns it in
- * out[5] and its GPU time in *best_ms. */
+ * out[5] and its GPU time in *best_ms.  Each timing run applies the sample's
+ * displacements and uniforms to other particles (indices shifted), so that no
+ * run finds the previous run's data in cache. */
 int hpdmk_autotune(hpdmk_tree tree, int n_moves, const long* idx, const double* d, const double* u, double beta,
                    int* out, double* best_ms);
 

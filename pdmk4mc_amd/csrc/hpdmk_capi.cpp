@@ -782,13 +782,25 @@ int hpdmk_autotune(hpdmk_tree t, int n_moves, const long* idx, const double* d, 
             launch_outgoing(dv, tw->pn, tw->acc_root, tw->stream);
             launch_incoming(dv, tw->pn, tw->stream);
         });
-        upload_moves(tw.get(), tw->chain, n_moves, idx, d, u);
+        // Every timing run moves other particles (the sample's indices shifted by
+        // a multiple of a large odd step, displacements and uniforms kept):
+        // rerunning one move list would find its data in cache.
+        std::vector<long> shifted(n_moves);
+        long round = 0;
+        auto load_fresh = [&] {
+            const long step = 1000003L % n;
+            for (int k = 0; k < n_moves; ++k) shifted[k] = (idx[k] + (round * step) % n) % n;
+            ++round;
+            upload_moves(tw.get(), tw->chain, n_moves, shifted.data(), d, u);
+        };
         // coordinate search from the current shape: batch, then each role in turn
         Tiles best = h->tiles;
         int best_batch = h->tuned_batch;
         auto timeit = [&](const Tiles& g, int batch) {
             tw->tiles = g;
-            run_chain(tw.get(), beta, 1, batch);  // first run: warm caches and code objects
+            load_fresh();
+            run_chain(tw.get(), beta, 1, batch);  // first run: code objects, queues
+            load_fresh();
             return run_chain(tw.get(), beta, 1, batch);
         };
         double best_t = timeit(best, best_batch);
