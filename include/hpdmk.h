@@ -154,6 +154,39 @@ int hpdmk_plan_tree(HPDMKParams params, int n_src, const double* r_src, int* n_n
  * n_window, n_diff (out[7]). */
 int hpdmk_pswf_constants(int digits, double* out);
 
+/* ---- proposals (no reference counterpart) ------------------------------
+ * learned = 0: hpdmk_mc_load's d holds the displacements (uniform or any
+ * symmetric proposal).  learned = 1: d holds standard normal triples z and the
+ * displacement is delta = mu + sigma z, with mu = max_mean tanh(o[0:3]),
+ * sigma = sigma_min + (sigma_max - sigma_min) sigmoid(o[3:6]) and o the output
+ * of an MLP 8 -> 32 -> 32 -> 6 (tanh; float weights W1[8][32] b1 W2[32][32] b2
+ * W3[32][6] b3, 1542 values) applied to features of the particle's
+ * neighbourhood within radius rf (the particle itself removed).  The decision
+ * adds the exact proposal ratio: accept iff bjerrum dE - H / beta < -ln(u) /
+ * beta, H = log q(-delta | features at the target) - log q(delta | features
+ * at the origin).  rf > 0 with learned = 0 records the origin features of
+ * every move (training data).  The learned proposal runs in the plain chain. */
+int hpdmk_mc_set_proposal(hpdmk_tree tree, int learned, const float* weights, double rf, double max_mean,
+                          double sigma_min, double sigma_max);
+/* Of the last hpdmk_mc_run (each optional): delta[3n] displacements used,
+ * feat[8n] features at the origins (rf > 0), logratio[n] H (learned). */
+int hpdmk_mc_proposal_results(hpdmk_tree tree, double* delta, double* feat, double* logratio);
+
+/* Proposal MLP on the GPU (no reference counterpart).  Samples: features
+ * feat[n][8] (as recorded by hpdmk_mc_proposal_results), displacements
+ * delta[n][3] and weights weight[n] (e.g. 1 for accepted moves, 0 else).
+ * hpdmk_mlp_eval: mu_sigma[n][6] (optional), grad[1542] (optional) = the sum
+ * over samples of weight x d NLL / d w with NLL = sum_a log sigma_a +
+ * (delta_a - mu_a)^2 / (2 sigma_a^2), and *loss (optional) = the weighted NLL
+ * sum.  cfg3: max_mean, sigma_min, sigma_max.
+ * hpdmk_mlp_train: Adam on minibatches of the mean weighted NLL; cfg[7]:
+ * max_mean, sigma_min, sigma_max, learning rate, epochs, minibatch, seed;
+ * weights in/out; epoch_loss[epochs] (optional) = mean NLL per epoch. */
+int hpdmk_mlp_eval(int n, const float* weights, const double* feat, const double* delta, const double* weight,
+                   const double* cfg3, double* mu_sigma, float* grad, double* loss);
+int hpdmk_mlp_train(int n, const double* feat, const double* delta, const double* weight, const double* cfg,
+                    float* weights, double* epoch_loss);
+
 /* ---- launch shape and tile autotuner (no reference counterpart) --------
  * tiles[4]: blocks per trial for the window sum, per (endpoint, difference
  * level), per endpoint's near field (each 1..16), and blocks of the update

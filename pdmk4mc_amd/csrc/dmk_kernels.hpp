@@ -4,13 +4,16 @@
 #include <hip/hip_runtime.h>
 
 #include "dmk_common.hpp"
+#include "dmk_proposal.hpp"
 
 namespace hpdmk_amd {
 
 constexpr int kMaxBatch = 256;     // trials per batched step
 constexpr int kRowStride = 64;     // stride of one 1-D phase row (2n+1 <= 53)
 constexpr int kAccPerTrial = 7;    // window, difference, res_t, res_o, scanned_t, scanned_o, overlaps_t
-constexpr int kPartStride = 3;     // per trial block: partial sum, particles scanned, hard-sphere overlaps
+// per trial block: partial sum, particles scanned, hard-sphere overlaps, and
+// the learned proposal's feature sums at the target (kFeatSums)
+constexpr int kPartStride = 3 + kFeatSums;
 constexpr int kStatsPerMove = 6;   // levels t/o, near particles t/o, rewritten arrays t/o
 
 // Everything later kernels need about one trial move: endpoints, charge and
@@ -116,6 +119,7 @@ struct McArgs {
     double lB;
     double sigma2;
     Tiles tiles;
+    Proposal prop;  // learned proposal (Metropolis and batch modes)
 };
 
 // One Metropolis step of every replica of a set (one move each): per-replica
@@ -136,6 +140,9 @@ void launch_replica_step(const ReplicaLaunch<T>& r, hipStream_t st);
 template <typename T>
 void launch_replica_compact(const ReplicaLaunch<T>& r, hipStream_t st);
 
+// features at the origins and, for the learned proposal, the displacements
+template <typename T>
+void launch_propose(const DevTree<T>& t, const McArgs& a, hipStream_t st);
 template <typename T>
 void launch_setup(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0 = nullptr,
                   hipEvent_t e1 = nullptr);

@@ -111,6 +111,62 @@ __global__ void __launch_bounds__(TB) k_setup_rep(const DevTree<T>* __restrict__
     setup_body(t, a);
 }
 
+// ------------------------------------------------------------------ propose
+// Features of each trial's particle at its origin (particle removed) and, for
+// the learned proposal, its displacement delta = mu + sigma z and log q.
+// One block per trial.
+template <typename T>
+__device__ __forceinline__ void propose_body(const DevTree<T>& t, const McArgs& a) {
+    __shared__ double red[TB / 64];
+    __shared__ double sums[kFeatSums], phi[kFeat], hbuf[2 * kHid], o[kOut], mu[3], sig[3];
+    if (*t.err) return;
+    const Proposal& P = a.prop;
+    const int tr = blockIdx.x, m = move_base(a) + tr;
+    const int i = a.ml.idx[m];
+    const int own = t.slot_of[i], leaf = t.leaf_of[i];
+    const double ox = t.px[own], oy = t.py[own], oz = t.pz[own], q = t.pq[own];
+    const double rf2 = P.rf * P.rf;
+    double fs[kFeatSums] = {0, 0, 0, 0, 0, 0, 0};
+    const int off = t.near_off[leaf], cnt = t.near_off[leaf + 1] - off;
+    for (int k = 0; k < cnt; ++k) {
+        const int nb = t.near_ids[off + k];
+        const int code = t.near_img[off + k];
+        const double sx = (code % 3 - 1) * t.L, sy = ((code / 3) % 3 - 1) * t.L, sz = (code / 9 - 1) * t.L;
+        const int s0 = t.slot_start[nb], n = t.slot_count[nb];
+        for (int p = threadIdx.x; p < n; p += blockDim.x) {
+            const int s = s0 + p;
+            if (s == own || t.porig[s] < 0) continue;
+            const double ex = ox - (t.px[s] + sx), ey = oy - (t.py[s] + sy), ez = oz - (t.pz[s] + sz);
+            const double r2 = ex * ex + ey * ey + ez * ez;
+            if (r2 > 0.0 && r2 < rf2) feature_pair(ex, ey, ez, r2, t.pq[s], P.rf, fs);
+        }
+    }
+    for (int c = 0; c < kFeatSums; ++c) {
+        const double v = block_sum(fs[c], red);
+        if (threadIdx.x == 0) sums[c] = v;
+    }
+    if (threadIdx.x == 0) {
+        features_from_sums(q, sums, phi);
+        if (P.feat)
+            for (int f = 0; f < kFeat; ++f) P.feat[(size_t)kFeat * m + f] = phi[f];
+    }
+    if (!P.learned) return;  // uniform across the grid
+    mlp_forward_block(P.w, P, phi, hbuf, o, mu, sig);
+    if (threadIdx.x == 0) {
+        double dl[3];
+        for (int c = 0; c < 3; ++c) {
+            dl[c] = mu[c] + sig[c] * P.z[3 * m + c];
+            P.dprop[3 * m + c] = dl[c];
+        }
+        P.logq[m] = gauss_logq(dl, mu, sig);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TB) k_propose(DevTree<T> t, McArgs a) {
+    propose_body(t, a);
+}
+
 // ------------------------------------------------------------------ update plan
 // Moves an accepted particle whose leaf changes: it is appended to the
 // target leaf at slot_count + rank (rank = earlier movers of the batch into the
@@ -274,6 +330,18 @@ __device__ void sum_trial_partials(const DevTree<T>& t, const McArgs& a, int tr,
     v[6] = ov;
 }
 
+// feature sums of the target's near field (learned proposal) from the partials
+template <typename T>
+__device__ void sum_target_features(const DevTree<T>& t, const McArgs& a, int tr, double* fs) {
+    const Tiles& g = a.tiles;
+    const int nL = n_levels_pw(t.max_depth), per = trial_blocks(t.max_depth, g);
+    const double* p = a.s.part + kPartStride * (size_t)tr * per;
+    const int d1 = g.win + 2 * nL * g.diff;
+    for (int c = 0; c < kFeatSums; ++c) fs[c] = 0.0;
+    for (int b = 0; b < g.near; ++b)
+        for (int c = 0; c < kFeatSums; ++c) fs[c] += p[kPartStride * (d1 + b) + 3 + c];
+}
+
 // dE of trial tr from its partials (src/energy_shift.cpp:67) plus the optional
 // per-move parts and counts; *overlaps = particles within sigma of the target
 template <typename T>
@@ -329,6 +397,7 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
     const Cplx<T>* tb = tabs_of<T>(a.s, table_stride(D), tr);
     double part = 0.0;
     int nscan = 0, novl = 0;
+    double fsum[kFeatSums] = {0, 0, 0, 0, 0, 0, 0};
     const int WB = a.tiles.win, DB = a.tiles.diff, NB = a.tiles.near;
     if (b < WB) {
         // window: 2 cW sum W Re[(tau - omega) conj(rho - omega)] (include/vecops.hpp:118-145)
@@ -382,7 +451,9 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
         // sigma of the target; sigma never exceeds the finest box size, so the
         // near list covers them
         const bool hs = ep == 0 && a.sigma2 > 0.0;
-        const int own = hs ? t.slot_of[ti.i] : -1;
+        const bool ft = ep == 0 && a.prop.learned;  // reverse-move features at the target
+        const int own = (hs || ft) ? t.slot_of[ti.i] : -1;
+        const double rf2 = a.prop.rf * a.prop.rf;
         double acc = 0;
         for (int k = r % NB; k < cnt; k += NB) {
             const int nb = t.near_ids[off + k];
@@ -398,17 +469,26 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
                 if (r2 > eps && r2 < b2)
                     acc += (double)((T)t.pq[s] * residual_poly_over_r<T>(sqrt(r2), inv_b, t.poly, t.n_poly));
                 if (hs && s != own && t.porig[s] >= 0 && ex * ex + ey * ey + ez * ez < a.sigma2) ++novl;
+                if (ft && s != own && t.porig[s] >= 0) {
+                    const double r2d = ex * ex + ey * ey + ez * ez;
+                    if (r2d > 0.0 && r2d < rf2) feature_pair(ex, ey, ez, r2d, t.pq[s], a.prop.rf, fsum);
+                }
             }
         }
         part = acc * ti.q;
     }
     const double s = block_sum(part, red);
     const double ovs = a.sigma2 > 0.0 ? block_sum((double)novl, red) : 0.0;  // uniform across the grid
+    double fts[kFeatSums] = {0, 0, 0, 0, 0, 0, 0};
+    if (a.prop.learned)  // uniform across the grid
+        for (int c = 0; c < kFeatSums; ++c) fts[c] = block_sum(fsum[c], red);
     if (threadIdx.x == 0) {
         double* p = a.s.part + kPartStride * ((size_t)tr * per + b);
         __hip_atomic_store(p, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(p + 1, (double)nscan, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(p + 2, ovs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int c = 0; c < kFeatSums; ++c)
+            __hip_atomic_store(p + 3 + c, fts[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     if (a.mode == kTrialBatch) return;
     if (threadIdx.x == 0) {
@@ -418,16 +498,38 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
     }
     __syncthreads();
     if (!last) return;
+    __shared__ double sh_dE, phi[kFeat], hbuf[2 * kHid], o[kOut], mu[3], sig[3];
+    __shared__ int sh_ovl;
+    unsigned long long tp = wall_clock64();
+    const bool learned = a.prop.learned && a.mode == kTrialMetropolis;
     if (threadIdx.x == 0) {
-        unsigned long long tp = wall_clock64();
         __threadfence();
         int ovl = 0;
-        double dE = assemble_trial(t, a, tr, m, &ovl);
+        sh_dE = assemble_trial(t, a, tr, m, &ovl);
+        sh_ovl = ovl;
+        if (learned) {
+            double fs[kFeatSums];
+            sum_target_features(t, a, tr, fs);
+            features_from_sums(ti.q, fs, phi);
+        }
+    }
+    double H = 0.0;
+    if (learned) {  // proposal log-ratio: reverse move from the target
+        mlp_forward_block(a.prop.w, a.prop, phi, hbuf, o, mu, sig);
+        if (threadIdx.x == 0) {
+            const double back[3] = {-a.prop.dprop[3 * m], -a.prop.dprop[3 * m + 1], -a.prop.dprop[3 * m + 2]};
+            H = gauss_logq(back, mu, sig) - a.prop.logq[m];
+            a.prop.H[m] = H;
+        }
+    }
+    if (threadIdx.x == 0) {
+        double dE = sh_dE;
+        const int ovl = sh_ovl;
         if (a.mode == kTrialOnly && a.ml.dE_stale) a.ml.dE_stale[m] = dE;
         int ok = 0;
         if (a.mode == kTrialMetropolis) {
             dE = ovl > 0 ? (double)INFINITY : a.lB * dE;  // a hard-sphere overlap is an infinite energy
-            ok = ovl == 0 && dE < mh_threshold(a.ml.u[m], a.beta);
+            ok = ovl == 0 && dE - H / a.beta < mh_threshold(a.ml.u[m], a.beta);
         }
         a.ml.dE[m] = dE;
         if (a.mode == kTrialForceAccept) ok = 1;
@@ -962,6 +1064,11 @@ __global__ void __launch_bounds__(TB) k_apply_rep(const DevTree<T>* __restrict__
 // ------------------------------------------------------------------ launchers
 // e0/e1 (may be null) take the kernel's own start and end timestamps
 template <typename T>
+void launch_propose(const DevTree<T>& t, const McArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_propose<T>, dim3(a.K), dim3(TB), 0, st, t, a);
+}
+
+template <typename T>
 void launch_setup(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     hipExtLaunchKernelGGL(k_setup<T>, dim3(a.K), dim3(TB), 0, st, e0, e1, 0, t, a);
 }
@@ -1010,6 +1117,7 @@ void launch_replica_compact(const ReplicaLaunch<T>& r, hipStream_t st) {
 
 #define INSTANTIATE(T)                                                                \
     template void launch_compact<T>(const DevTree<T>&, const int*, int, hipStream_t); \
+    template void launch_propose<T>(const DevTree<T>&, const McArgs&, hipStream_t);                      \
     template void launch_setup<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);     \
     template void launch_trial<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);     \
     template void launch_pair_corr<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t); \

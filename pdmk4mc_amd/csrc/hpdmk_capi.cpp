@@ -290,6 +290,10 @@ void ensure_moves(Handle* h, MoveBuf& b, int n) {
     b.parts = dalloc<double>(5 * (size_t)c, o);
     b.acc = dalloc<int>(c, o);
     b.stats = dalloc<int>(kStatsPerMove * (size_t)c, o);
+    b.dprop = dalloc<double>(3 * (size_t)c, o);
+    b.logq = dalloc<double>(c, o);
+    b.feat = dalloc<double>(kFeat * (size_t)c, o);
+    b.H = dalloc<double>(c, o);
     b.cap = c;
 }
 
@@ -325,6 +329,23 @@ McArgs mc_args(Handle* h, const MoveBuf& b, int move0, int K, int mode, double b
     a.beta = beta;
     a.tiles = h->tiles;
     const bool chain = mode == kTrialMetropolis || mode == kTrialBatch;
+    a.prop = Proposal{};
+    if (chain && h->prop_rf > 0.0) {
+        a.prop.rf = h->prop_rf;
+        a.prop.feat = b.feat;
+        if (h->prop_learned) {
+            a.prop.learned = 1;
+            a.prop.w = h->mlp_w;
+            a.prop.z = b.d;
+            a.prop.max_mean = h->prop_max_mean;
+            a.prop.smin = h->prop_smin;
+            a.prop.smax = h->prop_smax;
+            a.prop.dprop = b.dprop;
+            a.prop.logq = b.logq;
+            a.prop.H = b.H;
+            a.ml.d = b.dprop;  // set-up and trial read the drawn displacements
+        }
+    }
     a.lB = chain ? h->bjerrum : 1.0;
     a.sigma2 = chain ? h->sigma * h->sigma : 0.0;
     return a;
@@ -338,6 +359,8 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
     if (n <= 0) throw HpdmkError(HPDMK_EINVAL, "no moves loaded (hpdmk_mc_load)");
     if (mode != 0 && mode != 1) throw HpdmkError(HPDMK_EINVAL, "mode must be 0 (plain) or 1 (batched)");
     if (mode == 1 && batch == 0) batch = h->tuned_batch;
+    if (mode == 1 && h->prop_learned)
+        throw HpdmkError(HPDMK_EINVAL, "the learned proposal runs in the plain chain (mode 0)");
     if (mode == 1 && (batch < 1 || batch > kMaxBatch)) throw HpdmkError(HPDMK_EINVAL, "batch must be in [0, 256]");
     for (int k = 0; k < kKinds; ++k) {
         h->kms[k] = 0;
@@ -379,6 +402,7 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
         if (mode == 0) {
             for (int m = 0; m < n; ++m) {
                 const McArgs a = mc_args(h, mb, m, 1, kTrialMetropolis, beta);
+                if (a.prop.rf > 0.0) launch_propose(dv, a, h->stream);
                 timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
                 timed(h, kKTrial, [&](hipEvent_t x, hipEvent_t y) { launch_trial(dv, a, h->stream, x, y); });
                 timed(h, kKUpdate, [&](hipEvent_t x, hipEvent_t y) { launch_apply(dv, a, h->stream, x, y); });
@@ -388,6 +412,7 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
             for (auto& bt : batches) {
                 const int m0 = bt.first, k = bt.second;
                 const McArgs a = mc_args(h, mb, m0, k, kTrialBatch, beta);
+                if (a.prop.rf > 0.0) launch_propose(dv, a, h->stream);  // features only (uniform proposals)
                 timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
                 timed(h, kKTrial, [&](hipEvent_t x, hipEvent_t y) { launch_trial(dv, a, h->stream, x, y); });
                 if (k > 1)
@@ -719,6 +744,50 @@ int hpdmk_mc_set_model(hpdmk_tree t, double bjerrum, double sigma) {
             throw HpdmkError(HPDMK_EINVAL, "sigma exceeds the finest box size " + std::to_string(finest));
         h->bjerrum = bjerrum;
         h->sigma = sigma;
+    });
+}
+
+int hpdmk_mc_set_proposal(hpdmk_tree t, int learned, const float* weights, double rf, double max_mean,
+                          double sigma_min, double sigma_max) {
+    return guard([&] {
+        Handle* h = H(t);
+        if (learned != 0 && learned != 1) throw HpdmkError(HPDMK_EINVAL, "learned must be 0 or 1");
+        if (!(rf >= 0.0) || !std::isfinite(rf)) throw HpdmkError(HPDMK_EINVAL, "feature radius must be >= 0");
+        const double finest = h->params.L / (double)(1 << (h->tree.max_depth - 1));
+        if (rf > finest)
+            throw HpdmkError(HPDMK_EINVAL, "feature radius exceeds the finest box size " + std::to_string(finest));
+        if (learned) {
+            if (!weights) throw HpdmkError(HPDMK_EINVAL, "learned proposal needs MLP weights");
+            if (!(rf > 0.0)) throw HpdmkError(HPDMK_EINVAL, "learned proposal needs a feature radius > 0");
+            if (!(sigma_min > 0.0) || !(sigma_max > sigma_min) || !(max_mean >= 0.0))
+                throw HpdmkError(HPDMK_EINVAL, "need 0 < sigma_min < sigma_max and max_mean >= 0");
+            if (sigma_max + max_mean > 0.25 * h->params.L)
+                throw HpdmkError(HPDMK_EINVAL, "proposal scale too large for the box (sigma_max + max_mean > L/4)");
+            if (!h->mlp_w) h->mlp_w = dalloc<float>(kMlpParams, h->owned);
+            HIPX(hipMemcpy(h->mlp_w, weights, sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
+            h->prop_max_mean = max_mean;
+            h->prop_smin = sigma_min;
+            h->prop_smax = sigma_max;
+        }
+        h->prop_learned = learned;
+        h->prop_rf = rf;
+    });
+}
+
+int hpdmk_mc_proposal_results(hpdmk_tree t, double* delta, double* feat, double* logratio) {
+    return guard([&] {
+        Handle* h = H(t);
+        const MoveBuf& mb = h->chain;
+        const size_t n = mb.n;
+        if (delta) {
+            if (h->prop_learned)
+                HIPX(hipMemcpyAsync(delta, mb.dprop, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, h->stream));
+            else
+                HIPX(hipMemcpyAsync(delta, mb.d, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, h->stream));
+        }
+        if (feat) HIPX(hipMemcpyAsync(feat, mb.feat, sizeof(double) * kFeat * n, hipMemcpyDeviceToHost, h->stream));
+        if (logratio) HIPX(hipMemcpyAsync(logratio, mb.H, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+        HIPX(hipStreamSynchronize(h->stream));
     });
 }
 

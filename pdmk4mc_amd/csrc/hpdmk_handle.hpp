@@ -54,6 +54,11 @@ struct MoveBuf {
     double* parts = nullptr;
     int* acc = nullptr;
     int* stats = nullptr;
+    // proposal: displacement used, log q, features at the origin, log-ratio
+    double* dprop = nullptr;
+    double* logq = nullptr;
+    double* feat = nullptr;
+    double* H = nullptr;
 };
 
 // Moved particles leave holes in their old leaf slots; the slots are packed
@@ -93,6 +98,10 @@ struct Handle {
     // when hpdmk_mc_run is given batch 0 (hpdmk_set_tiles, hpdmk_autotune)
     Tiles tiles{};
     int tuned_batch = 64;
+    // proposal of hpdmk_mc_run (hpdmk_mc_set_proposal)
+    int prop_learned = 0;
+    float* mlp_w = nullptr;  // kMlpParams on the device
+    double prop_rf = 0.0, prop_max_mean = 0.5, prop_smin = 0.05, prop_smax = 1.0;
     // Monte Carlo model of hpdmk_mc_run (hpdmk_mc_set_model)
     double bjerrum = 1.0;
     double sigma = 0.0;

@@ -176,6 +176,9 @@ int hpdmk_replicas_mc_run(hpdmk_replicas s, double beta, int graph_steps, double
         if (n <= 0) throw HpdmkError(HPDMK_EINVAL, "no moves loaded (hpdmk_replicas_mc_load)");
         if (graph_steps < 0) throw HpdmkError(HPDMK_EINVAL, "graph_steps must be >= 0");
         std::vector<McArgs> args;
+        for (auto& h : set->reps)
+            if (h->prop_rf > 0.0)
+                throw HpdmkError(HPDMK_EINVAL, "replica sets run uniform proposals without features (rf = 0)");
         for (auto& h : set->reps) {
             McArgs a = mc_args(h.get(), h->chain, 0, 1, kTrialMetropolis, beta);
             a.s.probe = nullptr;

@@ -61,6 +61,11 @@ SIGNATURES = {
     "hpdmk_ewald_compute_energy": (C.c_int, [C.c_void_p, _dp, _dp]),
     "hpdmk_ewald_compute_potential": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
     "hpdmk_ewald_shift_chain": (C.c_int, [C.c_void_p, C.c_int, _lp, _dp, _ip, _dp]),
+    "hpdmk_mc_set_proposal": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_double, C.c_double,
+                                        C.c_double, C.c_double]),
+    "hpdmk_mc_proposal_results": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "hpdmk_mlp_eval": (C.c_int, [C.c_int, C.POINTER(C.c_float), _dp, _dp, _dp, _dp, _dp, C.POINTER(C.c_float), _dp]),
+    "hpdmk_mlp_train": (C.c_int, [C.c_int, _dp, _dp, _dp, _dp, C.POINTER(C.c_float), _dp]),
     "hpdmk_set_tiles": (C.c_int, [C.c_void_p, _ip, C.c_int]),
     "hpdmk_get_tiles": (C.c_int, [C.c_void_p, _ip]),
     "hpdmk_autotune": (C.c_int, [C.c_void_p, C.c_int, _lp, _dp, _dp, C.c_double, _ip, _dp]),
@@ -124,6 +129,51 @@ def pswf_constants(digits):
     _check(load_library().hpdmk_pswf_constants(int(digits), out.ctypes.data_as(_dp)))
     return dict(c=out[0], lam=out[1], C0=out[2], diff0=out[3], psi0_0=out[4], n_window=int(out[5]),
                 n_diff=int(out[6]))
+
+
+MLP_FEATURES = 8
+MLP_PARAMS = 8 * 32 + 32 + 32 * 32 + 32 + 32 * 6 + 6
+
+
+def mlp_init(seed=0, scale=1.0):
+    """Initial proposal-MLP weights (float32): scaled Gaussian layers, zero biases."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for fan_in, fan_out in ((8, 32), (32, 32), (32, 6)):
+        parts.append(rng.normal(0, scale / np.sqrt(fan_in), (fan_in, fan_out)).reshape(-1))
+        parts.append(np.zeros(fan_out))
+    return np.concatenate(parts).astype(np.float32)
+
+
+def mlp_eval(weights, feat, delta, weight, max_mean=0.5, sigma_min=0.05, sigma_max=1.0):
+    """GPU FP32 forward and gradient: (mu_sigma[n,6], grad[1542], weighted NLL sum)."""
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    feat, pf = _d(np.asarray(feat).reshape(-1))
+    delta, pdl = _d(np.asarray(delta).reshape(-1))
+    weight, pw = _d(weight)
+    cfg, pc = _d([max_mean, sigma_min, sigma_max])
+    n = weight.size
+    ms = np.zeros((n, 6))
+    g = np.zeros(MLP_PARAMS, dtype=np.float32)
+    loss = C.c_double()
+    _check(load_library().hpdmk_mlp_eval(n, w.ctypes.data_as(C.POINTER(C.c_float)), pf, pdl, pw, pc,
+                                         ms.ctypes.data_as(_dp), g.ctypes.data_as(C.POINTER(C.c_float)),
+                                         C.byref(loss)))
+    return ms, g, loss.value
+
+
+def mlp_train(feat, delta, weight, weights0, max_mean=0.5, sigma_min=0.05, sigma_max=1.0, lr=3e-3, epochs=20,
+              batch=1024, seed=0):
+    """Adam on the mean weighted NLL of the displacements; returns (weights, loss per epoch)."""
+    w = np.array(weights0, dtype=np.float32, copy=True)
+    feat, pf = _d(np.asarray(feat).reshape(-1))
+    delta, pdl = _d(np.asarray(delta).reshape(-1))
+    weight, pw = _d(weight)
+    cfg, pc = _d([max_mean, sigma_min, sigma_max, lr, epochs, batch, seed])
+    losses = np.zeros(int(epochs))
+    _check(load_library().hpdmk_mlp_train(weight.size, pf, pdl, pw, pc, w.ctypes.data_as(C.POINTER(C.c_float)),
+                                          losses.ctypes.data_as(_dp)))
+    return w, losses
 
 
 def copy_probe(n_bytes=256 << 20, reps=5):
@@ -234,6 +284,28 @@ class HPDMKPtTree:
     def mc_set_model(self, bjerrum=1.0, sigma=0.0):
         """Bjerrum scaling of the chain's energies and hard-sphere diameter (0 = none)."""
         _check(load_library().hpdmk_mc_set_model(self._h, float(bjerrum), float(sigma)))
+
+    def mc_set_proposal(self, learned=False, weights=None, rf=0.0, max_mean=0.5, sigma_min=0.05, sigma_max=1.0):
+        """learned: displacements from the MLP (mc_load's d then holds N(0,1) triples);
+        rf > 0 records the origin features of every move."""
+        wp = None
+        if weights is not None:
+            self._mlp_w = np.ascontiguousarray(weights, dtype=np.float32)
+            if self._mlp_w.size != MLP_PARAMS:
+                raise ValueError(f"MLP weights must hold {MLP_PARAMS} values")
+            wp = self._mlp_w.ctypes.data_as(C.POINTER(C.c_float))
+        _check(load_library().hpdmk_mc_set_proposal(self._h, int(bool(learned)), wp, float(rf), float(max_mean),
+                                                    float(sigma_min), float(sigma_max)))
+
+    def mc_proposal_results(self):
+        """(displacements used, origin features, proposal log-ratios) of the last run."""
+        n = self._mc_idx.size
+        dl = np.zeros((n, 3))
+        ft = np.zeros((n, MLP_FEATURES))
+        H = np.zeros(n)
+        _check(load_library().hpdmk_mc_proposal_results(self._h, dl.ctypes.data_as(_dp), ft.ctypes.data_as(_dp),
+                                                        H.ctypes.data_as(_dp)))
+        return dl, ft, H
 
     def set_tiles(self, win=2, diff=8, near=8, apply=1024, batch=64):
         t = np.array([win, diff, near, apply], dtype=np.int32)

@@ -1,0 +1,111 @@
+"""Learned proposal on the GPU against the oracle (oracle/mlp_oracle.py and the
+DMK oracle).
+
+Bars: recorded origin features equal the oracle's to 1e-12; the FP32 MLP
+forward pass and weighted-NLL gradient equal the FP64 oracle's to 1e-5
+(outputs) and 1e-4 relative (gradient); training lowers the NLL of data with
+a learnable mean; the FP64 learned chain takes the oracle chain's decisions,
+with displacements to 1e-12 and dE and the proposal log-ratio H to 1e-9.
+"""
+import pathlib
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import pdmk4mc_amd as P
+from hs_system import hard_sphere_system
+
+sys.path.insert(0, str(pathlib.Path(__file__).resolve().parents[1] / "oracle"))
+import mlp_oracle as M  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+CFG = dict(max_mean=0.4, sigma_min=0.05, sigma_max=0.8)
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def gpu_tree(r, q, L, leaf):
+    g = P.HPDMKPtTree(P.HPDMKParams(leaf, 6, L, init=P.DIRECT), r, q, P.HPDMK_FP64)
+    g.form_outgoing_pw()
+    g.form_incoming_pw()
+    return g
+
+
+@pytest.mark.parametrize("leaf,rf", [(64, 2.5), (16, 2.0)])
+def test_recorded_features_match_oracle(leaf, rf):
+    log("features", leaf, rf)
+    r, q, L = hard_sphere_system(2000, 0.2, 1.0, seed=4)
+    g = gpu_tree(r, q, L, leaf)
+    g.mc_set_proposal(learned=False, rf=rf)
+    rng = np.random.default_rng(5)
+    idx, d, u = rng.integers(0, 2000, 120), rng.uniform(-1, 1, (120, 3)), rng.uniform(0, 1, 120)
+    _, acc, _ = g.mc_chain(idx, d, u, mode=0)
+    _, ft, _ = g.mc_proposal_results()
+    pos = r.copy()
+    for k in range(120):
+        i = idx[k]
+        assert np.abs(ft[k] - M.features(pos, q, L, i, pos[i], rf)).max() < 1e-12, k
+        if acc[k]:
+            pos[i] = np.mod(pos[i] + d[k], L)
+    assert np.abs(g.positions() - pos).max() < 1e-12
+
+
+def test_mlp_forward_and_gradient_match_oracle():
+    rng = np.random.default_rng(6)
+    w = P.mlp_init(3, scale=1.5)
+    phi = np.concatenate([np.sign(rng.normal(size=(1000, 1))), np.tanh(rng.normal(0, 1.5, (1000, 7)))], 1)
+    delta = rng.normal(0, 0.4, (1000, 3))
+    wt = rng.integers(0, 2, 1000).astype(float)
+    ms, g, loss = P.mlp_eval(w, phi, delta, wt, **CFG)
+    mu, sig, _ = M.forward(w.astype(np.float64), phi, CFG["max_mean"], CFG["sigma_min"], CFG["sigma_max"])
+    assert np.abs(ms[:, :3] - mu).max() < 1e-5 and np.abs(ms[:, 3:] - sig).max() < 1e-5
+    lo, go = M.nll_and_grad(w.astype(np.float64), phi, delta, wt, CFG["max_mean"], CFG["sigma_min"], CFG["sigma_max"])
+    assert abs(loss - lo) < 1e-5 * abs(lo)
+    assert np.abs(g - go).max() < 1e-4 * np.abs(go).max()
+
+
+def test_training_lowers_nll():
+    rng = np.random.default_rng(7)
+    n = 20000
+    phi = np.concatenate([np.sign(rng.normal(size=(n, 1))), np.tanh(rng.normal(0, 1.0, (n, 7)))], 1)
+    delta = 0.3 * phi[:, 1:4] + 0.1 * rng.normal(size=(n, 3))
+    wt = np.ones(n)
+    w0 = P.mlp_init(1)
+    _, _, l0 = P.mlp_eval(w0, phi, delta, wt, **CFG)
+    w, losses = P.mlp_train(phi, delta, wt, w0, lr=3e-3, epochs=15, batch=1024, **CFG)
+    _, _, l1 = P.mlp_eval(w, phi, delta, wt, **CFG)
+    log("nll per sample", l0 / n, "->", l1 / n, losses[[0, -1]])
+    assert l1 < l0 - 0.5 * n and losses[-1] < losses[0]
+    ms, _, _ = P.mlp_eval(w, phi, delta, wt, **CFG)
+    assert np.corrcoef(ms[:, 0], 0.3 * phi[:, 1])[0, 1] > 0.95
+
+
+def test_learned_chain_matches_oracle():
+    log("learned chain")
+    r, q, L = hard_sphere_system(2000, 0.2, 1.0, seed=9)
+    leaf, rf = 16, 2.0
+    w = P.mlp_init(5, scale=1.0)
+    g = gpu_tree(r, q, L, leaf)
+    g.mc_set_model(2.0, 1.0)
+    g.mc_set_proposal(learned=True, weights=w, rf=rf, **CFG)
+    rng = np.random.default_rng(10)
+    idx, z, u = rng.integers(0, 2000, 150), rng.normal(size=(150, 3)), rng.uniform(0, 1, 150)
+    dEg, accg, _ = g.mc_chain(idx, z, u, beta=1.0, mode=0)
+    dlg, _, Hg = g.mc_proposal_results()
+    t = O.OracleTree(r, q, L, n_per_leaf=leaf, digits=6)
+    t.form_pw()
+    dEo, acco, dlo, Ho = M.learned_chain(t, r, q, L, idx, z, u, 1.0, w.astype(np.float64), rf, CFG["max_mean"],
+                                         CFG["sigma_min"], CFG["sigma_max"], bjerrum=2.0, sigma=1.0)
+    assert (accg == acco).all()
+    assert np.abs(dlg - dlo).max() < 1e-12
+    assert np.abs(Hg - Ho).max() < 1e-9
+    fin = np.isfinite(dEo)
+    assert (np.isfinite(dEg) == fin).all() and np.abs(dEg[fin] - dEo[fin]).max() < 1e-9
+    assert 5 < accg.sum() < 145
+    with pytest.raises(P.HPDMKError):
+        g.mc_chain(idx, z, u, mode=1)  # the learned proposal runs in the plain chain
