@@ -165,7 +165,10 @@ int hpdmk_pswf_constants(int digits, double* out);
  * adds the exact proposal ratio: accept iff bjerrum dE - H / beta < -ln(u) /
  * beta, H = log q(-delta | features at the target) - log q(delta | features
  * at the origin).  rf > 0 with learned = 0 records the origin features of
- * every move (training data).  The learned proposal runs in the plain chain. */
+ * every move (training data).  In the batched chain a step decides its
+ * trials up to the first whose proposal an earlier accepted move of the step
+ * made stale (same particle, or an endpoint within rf of its endpoints); the
+ * rest are proposed again from the new state, so the chain is the plain one. */
 int hpdmk_mc_set_proposal(hpdmk_tree tree, int learned, const float* weights, double rf, double max_mean,
                           double sigma_min, double sigma_max);
 /* Of the last hpdmk_mc_run (each optional): delta[3n] displacements used,

@@ -109,8 +109,12 @@ struct McArgs {
     MoveList ml;
     PwNodes pn;
     int move0;   // first move of this launch in the move list
-    const int* move_ctr;  // when set, the first move is *move_ctr (replica steps replayed from a graph)
-    int K;       // trials in this launch
+    // when set, the first move is *move_ctr (replica steps replayed from a
+    // graph; batched learned steps, whose scan advances it) and a launch takes
+    // at most n_moves - *move_ctr trials
+    int* move_ctr;
+    int n_moves;
+    int K;       // trials in this launch (at most)
     int mode;    // TrialMode
     double beta;
     // Monte Carlo model (Metropolis and batch modes only): the decision uses

@@ -49,8 +49,12 @@ __device__ __forceinline__ int image_shift(int ca, int cb, int nl) {
     return 2;
 }
 
-// first move of this launch: the replica's step counter when one is attached
+// first move of this launch: the step counter when one is attached
 __device__ __forceinline__ int move_base(const McArgs& a) { return a.move_ctr ? *a.move_ctr : a.move0; }
+// trials of this launch: with a step counter, the moves left (at most K)
+__device__ __forceinline__ int keff(const McArgs& a) {
+    return a.move_ctr ? min(a.K, a.n_moves - *a.move_ctr) : a.K;
+}
 
 // ------------------------------------------------------------------ setup
 template <typename T>
@@ -58,6 +62,7 @@ __device__ __forceinline__ void setup_body(const DevTree<T>& t, const McArgs& a)
     __shared__ TrialInfo ti;
     if (*t.err) return;
     const int tr = blockIdx.x, m = move_base(a) + tr;
+    if (tr >= keff(a)) return;
     if (threadIdx.x == 0) {
         trial_setup(t, a.ml.idx[m], a.ml.d + 3 * m, ti);
         a.s.info[tr] = ti;
@@ -122,6 +127,7 @@ __device__ __forceinline__ void propose_body(const DevTree<T>& t, const McArgs& 
     if (*t.err) return;
     const Proposal& P = a.prop;
     const int tr = blockIdx.x, m = move_base(a) + tr;
+    if (tr >= keff(a)) return;
     const int i = a.ml.idx[m];
     const int own = t.slot_of[i], leaf = t.leaf_of[i];
     const double ox = t.px[own], oy = t.py[own], oz = t.pz[own], q = t.pq[own];
@@ -392,6 +398,7 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
     __shared__ int last;
     if (*t.err) return;
     const int tr = blockIdx.y, b = blockIdx.x, m = move_base(a) + tr;
+    if (tr >= keff(a)) return;
     const TrialInfo& ti = a.s.info[tr];
     const int D = t.max_depth, nL = n_levels_pw(D), per = trial_blocks(D, a.tiles);
     const Cplx<T>* tb = tabs_of<T>(a.s, table_stride(D), tr);
@@ -490,7 +497,7 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
         for (int c = 0; c < kFeatSums; ++c)
             __hip_atomic_store(p + 3 + c, fts[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    if (a.mode == kTrialBatch) return;
+    if (a.mode == kTrialBatch && !a.prop.learned) return;
     if (threadIdx.x == 0) {
         __threadfence();
         const unsigned prev = atomicAdd(a.s.done + tr, 1u);
@@ -500,6 +507,21 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
     if (!last) return;
     __shared__ double sh_dE, phi[kFeat], hbuf[2 * kHid], o[kOut], mu[3], sig[3];
     __shared__ int sh_ovl;
+    if (a.mode == kTrialBatch) {  // learned proposal: this trial's log-ratio H for the batch scan
+        if (threadIdx.x == 0) {
+            __threadfence();
+            double fs[kFeatSums];
+            sum_target_features(t, a, tr, fs);
+            features_from_sums(ti.q, fs, phi);
+        }
+        mlp_forward_block(a.prop.w, a.prop, phi, hbuf, o, mu, sig);
+        if (threadIdx.x == 0) {
+            const double back[3] = {-a.prop.dprop[3 * m], -a.prop.dprop[3 * m + 1], -a.prop.dprop[3 * m + 2]};
+            a.prop.H[m] = gauss_logq(back, mu, sig) - a.prop.logq[m];
+            a.s.done[tr] = 0;
+        }
+        return;
+    }
     unsigned long long tp = wall_clock64();
     const bool learned = a.prop.learned && a.mode == kTrialMetropolis;
     if (threadIdx.x == 0) {
@@ -578,7 +600,7 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
         ++i;
     }
     const int j = i + 1 + p;
-    if (*t.err) return;
+    if (*t.err || j >= keff(a)) return;
     const TrialInfo& A = a.s.info[i];
     const TrialInfo& Bj = a.s.info[j];
     const int D = t.max_depth, nL = n_levels_pw(D), stride = table_stride(D);
@@ -708,18 +730,40 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
 // taken in order by one wave (lane l owns trials l, l+64, ...: it keeps their
 // running corrected dE and the appends into their target leaves); then the
 // update plan of the whole batch.
+//
+// With a device move counter (learned proposal) the step decides the trials
+// up to the first one whose proposal an earlier accepted move of the batch
+// made stale: the same particle, or an origin or target within the feature
+// radius of its origin or target (its features, hence its proposal density,
+// changed).  That trial starts the next step; the counter advances by the
+// number decided, so the chain is the plain chain.
+__device__ __forceinline__ bool within(const double* x, const double* y, double L, double r2max) {
+    double s = 0;
+    for (int c = 0; c < 3; ++c) {
+        double v = x[c] - y[c];
+        v -= L * rint(v / L);
+        s += v * v;
+    }
+    return s < r2max;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
     __shared__ double dEs[kMaxBatch], thr[kMaxBatch], dEf[kMaxBatch];
     __shared__ int leafT[kMaxBatch], leafO[kMaxBatch], accs[kMaxBatch], room[kMaxBatch], ov0[kMaxBatch];
     __shared__ double C[64 * 64];
     __shared__ unsigned char F[64 * 64];
+    __shared__ int cons;
     if (*t.err) return;
-    const int K = a.K;
+    const int K = a.K;             // stride of the pair tables
+    const int base = move_base(a);
+    const int Ke = keff(a);        // trials of this step
     const bool hs = a.sigma2 > 0.0;
+    const bool dyn = a.move_ctr != nullptr;
+    const double rf2 = a.prop.rf * a.prop.rf;
     unsigned long long tp = wall_clock64();
-    for (int j = threadIdx.x; j < K; j += blockDim.x) {
-        const int m = move_base(a) + j;
+    for (int j = threadIdx.x; j < Ke; j += blockDim.x) {
+        const int m = base + j;
         int ov = 0;
         const double dE = assemble_trial(t, a, j, m, &ov);
         dEs[j] = dE;
@@ -729,7 +773,8 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
         leafT[j] = tj.path_t[tj.lam_t];
         leafO[j] = tj.path_o[tj.lam_o];
         room[j] = leafT[j] == leafO[j] ? (1 << 30) : t.slot_cap[leafT[j]] - t.slot_count[leafT[j]];
-        thr[j] = mh_threshold(a.ml.u[m], a.beta);
+        // learned proposal: lB dE - H / beta < -ln(u) / beta
+        thr[j] = mh_threshold(a.ml.u[m], a.beta) + (a.prop.learned ? a.prop.H[m] / a.beta : 0.0);
     }
     const bool in_lds = K <= 64;
     if (in_lds) {
@@ -747,22 +792,31 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
         if (hs)
             for (int q = threadIdx.x; q < K * K; q += blockDim.x) F[q] = a.s.ovl[q];
     }
+    if (threadIdx.x == 0) cons = Ke > 0 ? Ke : 0;
     __syncthreads();
     if (threadIdx.x == 0) probe_add(a.s, kPrScanAssemble, tp);
     if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
         double run[4];
-        int dcr[4], ovr[4];
+        int dcr[4], ovr[4], stl[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int j = lane + 64 * k;
-            run[k] = j < K ? dEs[j] : 0.0;
-            ovr[k] = j < K ? ov0[j] : 0;
+            run[k] = j < Ke ? dEs[j] : 0.0;
+            ovr[k] = j < Ke ? ov0[j] : 0;
             dcr[k] = 0;
+            stl[k] = 0;
         }
         const double lB = a.lB;
-        for (int i = 0; i < K; ++i) {
+        for (int i = 0; i < Ke; ++i) {
             const int owner = i & 63, sl = i >> 6;
+            if (dyn) {  // a stale proposal ends the step
+                const int st = lane == owner ? (sl == 0 ? stl[0] : sl == 1 ? stl[1] : sl == 2 ? stl[2] : stl[3]) : 0;
+                if (__builtin_amdgcn_readlane(st, owner)) {
+                    if (lane == 0) cons = i;
+                    break;
+                }
+            }
             int ok = 0;
             if (lane == owner) {
                 const double dE = lB * (sl == 0 ? run[0] : sl == 1 ? run[1] : sl == 2 ? run[2] : run[3]);
@@ -775,15 +829,22 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
             ok = __builtin_amdgcn_readlane(ok, owner) == 1;
             if (ok) {
                 const int lti = leafT[i], mv = leafO[i] != lti;
+                const TrialInfo& A = a.s.info[i];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const int j = lane + 64 * k;
-                    if (j > i && j < K) {
+                    if (j > i && j < Ke) {
                         run[k] += in_lds ? C[i * K + j] : a.s.corr[i * K + j];
                         dcr[k] += mv && (lti == leafT[j]);
                         if (hs) {
                             const int f = in_lds ? F[i * K + j] : a.s.ovl[i * K + j];
                             ovr[k] += (f >> 1) - (f & 1);
+                        }
+                        if (dyn) {
+                            const TrialInfo& B = a.s.info[j];
+                            if (B.i == A.i || within(A.o, B.o, t.L, rf2) || within(A.t, B.o, t.L, rf2) ||
+                                within(A.o, B.t, t.L, rf2) || within(A.t, B.t, t.L, rf2))
+                                stl[k] = 1;
                         }
                     }
                 }
@@ -792,19 +853,24 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
     }
     __syncthreads();
     if (threadIdx.x == 0) probe_add(a.s, kPrScanDecide, tp);
+    const int nc = cons;
     for (int j = threadIdx.x; j < K; j += blockDim.x) {
-        const int m = move_base(a) + j;
-        int ok = accs[j];
-        if (ok == 2) {
-            atomicOr(t.err, (int)kErrLeafOverflow);
-            ok = 0;
+        int ok = 0;
+        if (j < nc) {
+            const int m = base + j;
+            ok = accs[j];
+            if (ok == 2) {
+                atomicOr(t.err, (int)kErrLeafOverflow);
+                ok = 0;
+            }
+            a.ml.dE[m] = dEf[j];
+            if (a.ml.accept_out) a.ml.accept_out[m] = ok;
         }
-        a.ml.dE[m] = dEf[j];
-        a.s.accept[j] = ok;
-        if (a.ml.accept_out) a.ml.accept_out[m] = ok;
+        a.s.accept[j] = ok;  // undecided and unused trials apply nothing
     }
     __syncthreads();
     plan_update(t, a, K);
+    if (dyn && threadIdx.x == 0) *a.move_ctr = base + nc;
 }
 
 // Packs the live particles of every leaf (porig >= 0) to the front of its

@@ -359,8 +359,6 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
     if (n <= 0) throw HpdmkError(HPDMK_EINVAL, "no moves loaded (hpdmk_mc_load)");
     if (mode != 0 && mode != 1) throw HpdmkError(HPDMK_EINVAL, "mode must be 0 (plain) or 1 (batched)");
     if (mode == 1 && batch == 0) batch = h->tuned_batch;
-    if (mode == 1 && h->prop_learned)
-        throw HpdmkError(HPDMK_EINVAL, "the learned proposal runs in the plain chain (mode 0)");
     if (mode == 1 && (batch < 1 || batch > kMaxBatch)) throw HpdmkError(HPDMK_EINVAL, "batch must be in [0, 256]");
     for (int k = 0; k < kKinds; ++k) {
         h->kms[k] = 0;
@@ -369,7 +367,7 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
     if (h->profiling >= 2) HIPX(hipMemsetAsync(h->probe, 0, sizeof(unsigned long long) * 2 * kProbes, h->stream));
     // batches never repeat a particle (the corrections assume distinct particles)
     std::vector<std::pair<int, int>> batches;
-    if (mode == 1) {
+    if (mode == 1 && !h->prop_learned) {
         std::vector<int> seen(h->n_src, -1);
         int m0 = 0;
         for (int m = 0; m < n; ++m) {
@@ -407,6 +405,35 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
                 timed(h, kKTrial, [&](hipEvent_t x, hipEvent_t y) { launch_trial(dv, a, h->stream, x, y); });
                 timed(h, kKUpdate, [&](hipEvent_t x, hipEvent_t y) { launch_apply(dv, a, h->stream, x, y); });
                 compact_if_due(1);
+            }
+        } else if (h->prop_learned) {
+            // steps decide a variable number of moves (up to the first stale
+            // proposal); the device counter tracks the chain, read back after
+            // each round of launches
+            if (!h->step_ctr) h->step_ctr = dalloc<int>(1, h->owned);
+            HIPX(hipMemsetAsync(h->step_ctr, 0, sizeof(int), h->stream));
+            int done = 0;
+            while (done < n) {
+                const int steps = (n - done + batch - 1) / batch;
+                for (int k = 0; k < steps; ++k) {
+                    McArgs a = mc_args(h, mb, 0, batch, kTrialBatch, beta);
+                    a.move_ctr = h->step_ctr;
+                    a.n_moves = n;
+                    launch_propose(dv, a, h->stream);
+                    timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
+                    timed(h, kKTrial, [&](hipEvent_t x, hipEvent_t y) { launch_trial(dv, a, h->stream, x, y); });
+                    if (batch > 1)
+                        timed(h, kKCorr,
+                              [&](hipEvent_t x, hipEvent_t y) { launch_pair_corr(dv, a, h->stream, x, y); });
+                    timed(h, kKScan, [&](hipEvent_t x, hipEvent_t y) { launch_scan(dv, a, h->stream, x, y); });
+                    timed(h, kKUpdate, [&](hipEvent_t x, hipEvent_t y) { launch_apply(dv, a, h->stream, x, y); });
+                    compact_if_due(batch);
+                }
+                HIPX(hipMemcpyAsync(&done, h->step_ctr, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+                HIPX(hipStreamSynchronize(h->stream));
+                int err = 0;
+                HIPX(hipMemcpy(&err, h->err, sizeof(int), hipMemcpyDeviceToHost));
+                if (err) break;  // reported by check_err below
             }
         } else {
             for (auto& bt : batches) {

@@ -187,6 +187,7 @@ int hpdmk_replicas_mc_run(hpdmk_replicas s, double beta, int graph_steps, double
         }
         for (int r = 0; r < set->n(); ++r) {
             args[r].move_ctr = set->move_ctr + r;
+            args[r].n_moves = n;
             args[r].tiles = Tiles{};  // the grid of a replica step uses the default shape
         }
         HIPX(hipMemcpyAsync(set->args, args.data(), sizeof(McArgs) * args.size(), hipMemcpyHostToDevice, set->stream));

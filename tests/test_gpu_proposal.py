@@ -107,5 +107,33 @@ def test_learned_chain_matches_oracle():
     fin = np.isfinite(dEo)
     assert (np.isfinite(dEg) == fin).all() and np.abs(dEg[fin] - dEo[fin]).max() < 1e-9
     assert 5 < accg.sum() < 145
-    with pytest.raises(P.HPDMKError):
-        g.mc_chain(idx, z, u, mode=1)  # the learned proposal runs in the plain chain
+
+
+@pytest.mark.parametrize("precision", [P.HPDMK_FP64, P.HPDMK_MIXED])
+@pytest.mark.parametrize("batch", [8, 64])
+def test_batched_learned_chain_equals_plain(precision, batch):
+    log("batched learned", precision, batch)
+    r, q, L = hard_sphere_system(2000, 0.2, 1.0, seed=11)
+    w = P.mlp_init(7, scale=1.0)
+    runs = []
+    for mode in (0, 1):
+        g = P.HPDMKPtTree(P.HPDMKParams(16, 6, L, init=P.DIRECT), r, q, precision)
+        g.form_outgoing_pw()
+        g.form_incoming_pw()
+        g.mc_set_model(2.0, 1.0)
+        g.mc_set_proposal(learned=True, weights=w, rf=2.0, **CFG)
+        rng = np.random.default_rng(12)
+        idx, z, u = rng.integers(0, 2000, 600), rng.normal(size=(600, 3)), rng.uniform(0, 1, 600)
+        dE, acc, _ = g.mc_chain(idx, z, u, beta=1.0, mode=mode, batch=batch)
+        dl, _, H = g.mc_proposal_results()
+        runs.append((dE, acc, dl, H, g.positions()))
+        g.close()
+    (dE1, a1, d1, H1, x1), (dE2, a2, d2, H2, x2) = runs
+    if precision == P.HPDMK_FP64:
+        assert (a1 == a2).all()
+        assert np.abs(d1 - d2).max() < 1e-12 and np.abs(H1 - H2).max() < 1e-9
+        fin = np.isfinite(dE1)
+        assert (np.isfinite(dE2) == fin).all() and np.abs(dE1[fin] - dE2[fin]).max() < 1e-9
+        assert np.abs(x1 - x2).max() < 1e-12
+    else:
+        assert (a1 == a2).mean() > 0.99
