@@ -3,15 +3,19 @@
 
 Workload (N = 1e5 system): N = 100000 point charges uniform in a periodic
 cube L = 100, charges alternating +1/-1, PSWF tolerance 1e-6 (digits 6), leaf
-cap 200; trial moves pick a particle uniformly and displace it by
-U[-1,1]^3; Metropolis at beta = 1.  A step is 1e4 trial moves of one chain
-(energy change, decision, accept/update).  Moves, displacements and uniforms
-are in HBM before the timed region starts.
+cap 200; trial moves pick a particle uniformly; Metropolis at beta = 1.  A
+step is 1e4 trial moves of one chain (energy change, decision,
+accept/update).  Moves and random numbers are in HBM before the timed region
+starts.
 
-Prints one JSON line: trial moves/s of the batched chain (the headline), the
-plain one-trial-at-a-time chain beside it, the roofline of the dominant
-kernel and the CPU baseline (the oracle, i.e. the CPU restatement of the
-reference path, one thread, on a bounded sample of the same chain).
+Headline: trial moves/s of the batched chain with the learned proposal
+(displacements drawn from an MLP trained on the GPU from a uniform chain,
+exact Metropolis-Hastings ratio).  Beside it: the batched chain with uniform
+displacements U[-1,1]^3, the plain one-trial-at-a-time GPU chain (uniform),
+their acceptance and mean squared jump per move, the roofline of the
+dominant kernel, the accuracy against FP64 Ewald, 256 ragged replicas, and
+the CPU baseline (the oracle, i.e. the CPU restatement of the reference path,
+one thread, on a bounded sample of the same chain).
 """
 import argparse
 import json
@@ -91,11 +95,18 @@ def main():
     ap.add_argument("--cpu-moves", type=int, default=12000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--rf", type=float, default=2.5, help="feature radius of the learned proposal")
+    ap.add_argument("--max-mean", type=float, default=0.5)
+    ap.add_argument("--sigma-min", type=float, default=0.05)
+    ap.add_argument("--sigma-max", type=float, default=1.5)
+    ap.add_argument("--train-moves", type=int, default=20000)
+    ap.add_argument("--train-step", type=float, default=2.0, help="uniform step of the training-data chain")
+    ap.add_argument("--train-epochs", type=int, default=20)
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--replica-moves", type=int, default=1000)
     ap.add_argument("--graph-steps", type=int, default=64)
-    ap.add_argument("--legs", default="batched,plain,accuracy,cpu,copy,replicas",
-                    help="comma list of batched, plain, accuracy, cpu, copy, replicas (copy = a 1 GiB "
+    ap.add_argument("--legs", default="batched,learned,plain,accuracy,cpu,copy,replicas",
+                    help="comma list of batched, learned, plain, accuracy, cpu, copy, replicas (copy = a 1 GiB "
                          "streaming copy: measured HBM ceiling and byte-counter calibration; replicas = "
                          "256 ragged independent systems stepped together)")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_update_batched_n1e5.json"),
@@ -158,6 +169,62 @@ def main():
     value = moves / wall
     print(f"# batched: {moves} moves in {wall:.3f}s (gpu {gpu_ms / 1e3:.3f}s) = {value:.0f} moves/s, "
           f"acceptance {acc_b.mean():.3f}", file=sys.stderr, flush=True)
+
+    # batched chain with the learned proposal: training data from a broad
+    # uniform chain (origin features recorded), MLP trained on the GPU, then
+    # the same timed protocol with displacements drawn from the MLP
+    learned_info = None
+    value_uniform = value
+    if "learned" in legs:
+        cfgp = dict(max_mean=args.max_mean, sigma_min=args.sigma_min, sigma_max=args.sigma_max)
+        rng_l = np.random.default_rng(args.seed + 21)
+        Nc = args.train_moves
+        ci = rng_l.integers(0, args.n, Nc)
+        cd = rng_l.uniform(-args.train_step, args.train_step, (Nc, 3))
+        cu = rng_l.uniform(0.0, 1.0, Nc)
+        t0 = time.time()
+        gc = P.HPDMKPtTree(params, r, q, prec)
+        gc.form_outgoing_pw()
+        gc.form_incoming_pw()
+        gc.mc_set_proposal(False, rf=args.rf)
+        gc.mc_load(ci, cd, cu)
+        gc.mc_run(1.0, 1, args.batch)
+        _, cacc = gc.mc_results()
+        cdl, cft, _ = gc.mc_proposal_results()
+        gc.close()
+        w, losses = P.mlp_train(cft, cdl, cacc.astype(np.float64), P.mlp_init(args.seed), lr=3e-3,
+                                epochs=args.train_epochs, batch=1024, seed=args.seed, **cfgp)
+        t_train = time.time() - t0
+        gl = P.HPDMKPtTree(params, r, q, prec)
+        gl.form_outgoing_pw()
+        gl.form_incoming_pw()
+        if tiles:
+            gl.set_tiles(**tiles)
+        gl.mc_set_proposal(True, weights=w, rf=args.rf, **cfgp)
+        z = np.random.default_rng(args.seed + 22).normal(size=(n_total, 3))
+        if args.warmup:
+            gl.mc_load(idx[:args.warmup * M], z[:args.warmup * M], u[:args.warmup * M])
+            gl.mc_run(1.0, 1, batch)
+        gl.mc_load(idx[sl], z[sl], u[sl])
+        t0 = time.perf_counter()
+        gl_ms = gl.mc_run(1.0, 1, batch)
+        wall_l = time.perf_counter() - t0
+        _, acc_l = gl.mc_results()
+        dl_l, _, H_l = gl.mc_proposal_results()
+        gl.close()
+        value = moves / wall_l
+        esjd_l = float((acc_l * (dl_l ** 2).sum(1)).mean())
+        learned_info = {
+            "value": round(value, 2), "unit": "trial moves/s", "gpu_ms": round(gl_ms, 2), "acceptance": round(float(acc_l.mean()), 4),
+            "esjd_per_move": round(esjd_l, 5), "esjd_per_s": round(esjd_l * value, 1),
+            "mean_abs_logratio": round(float(np.abs(H_l).mean()), 4),
+            "proposal": dict(rf=args.rf, **cfgp),
+            "training": {"moves": Nc, "uniform_step": args.train_step, "acceptance": round(float(cacc.mean()), 4),
+                         "epochs": args.train_epochs, "nll_first": round(float(losses[0]), 4),
+                         "nll_last": round(float(losses[-1]), 4), "seconds": round(t_train, 2)}}
+        print(f"# batched+learned: {moves} moves in {wall_l:.3f}s = {value:.0f} moves/s, acceptance "
+              f"{acc_l.mean():.3f}, ESJD/move {esjd_l:.4f}; training {learned_info['training']}",
+              file=sys.stderr, flush=True)
 
     # per-kernel times on a profiled segment: start/end timestamps of every
     # dispatch (level 1: no device-side timers in the kernels)
@@ -225,6 +292,8 @@ def main():
         gpl.mc_run(1.0, 0)
         plain_wall = time.perf_counter() - t0
         plain = args.plain_moves / plain_wall
+        _, acc_pl = gpl.mc_results()
+        esjd_pl = float((acc_pl * (d[psl] ** 2).sum(1)).mean())
         gpl.close()
         gpp = P.HPDMKPtTree(params, r, q, prec)
         gpp.form_outgoing_pw()
@@ -239,10 +308,13 @@ def main():
         gpp.mc_load(idx[psl][n_pp:2 * n_pp], d[psl][n_pp:2 * n_pp], u[psl][n_pp:2 * n_pp])
         gpp.mc_run(1.0, 0)
         plain_info = {"value": round(plain, 2), "unit": "trial moves/s", "moves": args.plain_moves,
+                      "acceptance": round(float(acc_pl.mean()), 4), "esjd_per_move": round(esjd_pl, 5),
+                      "esjd_per_s": round(esjd_pl * plain, 1),
                       "profile": {"kernel_ms": {k: [round(v[0], 3), v[1]] for k, v in plain_kt.items()},
                                   "phase_us": gpp.phase_times()}}
         gpp.close()
-        print(f"# plain: {plain:.0f} moves/s; batched/plain = {value / plain:.2f}", file=sys.stderr, flush=True)
+        print(f"# plain: {plain:.0f} moves/s, acceptance {acc_pl.mean():.3f}, ESJD/move {esjd_pl:.4f}; "
+              f"headline/plain = {value / plain:.2f}", file=sys.stderr, flush=True)
 
     # accuracy of the arithmetic mode: trial dE of this precision vs the FP64 path on the same state
     acc_info = None
@@ -354,6 +426,11 @@ def main():
                    "moves_per_step": M, "precision": args.precision},
         "plain_gpu": plain_info,
         "batched_over_plain": round(value / plain, 3) if plain else None,
+        "esjd_per_s_over_plain": (round(learned_info["esjd_per_s"] / plain_info["esjd_per_s"], 3)
+                                  if learned_info and plain_info else None),
+        "batched_uniform": {"value": round(value_uniform, 2), "unit": "trial moves/s",
+                            "acceptance": round(float(acc_b.mean()), 4)},
+        "learned": learned_info,
         "acceptance": round(float(acc_b.mean()), 4),
         "roofline": dict(roofline, measured_copy_gbs=round(copy_gbs, 1) if copy_gbs else None),
         "cpu_baseline": cpu,
