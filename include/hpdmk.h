@@ -132,7 +132,7 @@ int hpdmk_mc_stats(hpdmk_tree tree, int* stats);
  * device-side phase timers, which slow the kernels that carry them.
  * hpdmk_kernel_times: per-kernel GPU time of the last profiled run, kinds
  * 0 trial, 1 update (apply), 2 pair corrections, 3 decision scan, 4 trial
- * set-up.  ms[5], count[5]. */
+ * set-up, 5 proposal (features, learned displacements).  ms[6], count[6]. */
 int hpdmk_set_profiling(hpdmk_tree tree, int level);
 int hpdmk_kernel_times(hpdmk_tree tree, double* ms, int* count);
 

@@ -146,7 +146,8 @@ void launch_replica_compact(const ReplicaLaunch<T>& r, hipStream_t st);
 
 // features at the origins and, for the learned proposal, the displacements
 template <typename T>
-void launch_propose(const DevTree<T>& t, const McArgs& a, hipStream_t st);
+void launch_propose(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0 = nullptr,
+                    hipEvent_t e1 = nullptr);
 template <typename T>
 void launch_setup(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0 = nullptr,
                   hipEvent_t e1 = nullptr);

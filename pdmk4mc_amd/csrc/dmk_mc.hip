@@ -138,6 +138,17 @@ __device__ __forceinline__ void propose_body(const DevTree<T>& t, const McArgs& 
         const int nb = t.near_ids[off + k];
         const int code = t.near_img[off + k];
         const double sx = (code % 3 - 1) * t.L, sy = ((code / 3) % 3 - 1) * t.L, sz = (code / 9 - 1) * t.L;
+        {  // the leaf's box (at its image) must reach into the ball of radius rf
+            const double bs = t.boxsize[t.level[nb]];
+            const double lo[3] = {t.cell[3 * nb] * bs + sx, t.cell[3 * nb + 1] * bs + sy, t.cell[3 * nb + 2] * bs + sz};
+            const double x[3] = {ox, oy, oz};
+            double g2 = 0;
+            for (int c = 0; c < 3; ++c) {
+                const double e = fmax(fmax(lo[c] - x[c], x[c] - (lo[c] + bs)), 0.0);
+                g2 += e * e;
+            }
+            if (g2 >= rf2) continue;  // uniform across the block
+        }
         const int s0 = t.slot_start[nb], n = t.slot_count[nb];
         for (int p = threadIdx.x; p < n; p += blockDim.x) {
             const int s = s0 + p;
@@ -487,7 +498,8 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
     const double s = block_sum(part, red);
     const double ovs = a.sigma2 > 0.0 ? block_sum((double)novl, red) : 0.0;  // uniform across the grid
     double fts[kFeatSums] = {0, 0, 0, 0, 0, 0, 0};
-    if (a.prop.learned)  // uniform across the grid
+    const bool target_near = b >= WB + 2 * nL * DB && b < WB + 2 * nL * DB + NB;
+    if (a.prop.learned && target_near)  // uniform across the block
         for (int c = 0; c < kFeatSums; ++c) fts[c] = block_sum(fsum[c], red);
     if (threadIdx.x == 0) {
         double* p = a.s.part + kPartStride * ((size_t)tr * per + b);
@@ -579,6 +591,17 @@ __global__ void __launch_bounds__(TB) k_trial_rep(const DevTree<T>* __restrict__
     const McArgs& a = as[blockIdx.z];
     if ((int)blockIdx.x >= trial_blocks(t.max_depth, a.tiles)) return;
     trial_body(t, a);
+}
+
+// |x - y| below sqrt(r2max) under the minimum image
+__device__ __forceinline__ bool within(const double* x, const double* y, double L, double r2max) {
+    double s = 0;
+    for (int c = 0; c < 3; ++c) {
+        double v = x[c] - y[c];
+        v -= L * rint(v / L);
+        s += v * v;
+    }
+    return s < r2max;
 }
 
 // ------------------------------------------------------------------ batched trials
@@ -721,8 +744,14 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
             }
             total += (ep == 0 ? 1.0 : -1.0) * s * A.q * Bj.q;
         }
+        if (a.prop.learned) {  // j's proposal is stale once i is accepted: bit 4
+            const double rf2 = a.prop.rf * a.prop.rf;
+            if (A.i == Bj.i || within(A.o, Bj.o, t.L, rf2) || within(A.t, Bj.o, t.L, rf2) ||
+                within(A.o, Bj.t, t.L, rf2) || within(A.t, Bj.t, t.L, rf2))
+                flags |= 4;
+        }
         a.s.corr[i * K + j] = total;
-        if (a.sigma2 > 0.0) a.s.ovl[i * K + j] = (unsigned char)flags;
+        if (a.sigma2 > 0.0 || a.prop.learned) a.s.ovl[i * K + j] = (unsigned char)flags;
     }
 }
 
@@ -737,16 +766,6 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
 // radius of its origin or target (its features, hence its proposal density,
 // changed).  That trial starts the next step; the counter advances by the
 // number decided, so the chain is the plain chain.
-__device__ __forceinline__ bool within(const double* x, const double* y, double L, double r2max) {
-    double s = 0;
-    for (int c = 0; c < 3; ++c) {
-        double v = x[c] - y[c];
-        v -= L * rint(v / L);
-        s += v * v;
-    }
-    return s < r2max;
-}
-
 template <typename T>
 __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
     __shared__ double dEs[kMaxBatch], thr[kMaxBatch], dEf[kMaxBatch];
@@ -760,7 +779,7 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
     const int Ke = keff(a);        // trials of this step
     const bool hs = a.sigma2 > 0.0;
     const bool dyn = a.move_ctr != nullptr;
-    const double rf2 = a.prop.rf * a.prop.rf;
+    const bool flg = hs || a.prop.learned;  // pair flags present
     unsigned long long tp = wall_clock64();
     for (int j = threadIdx.x; j < Ke; j += blockDim.x) {
         const int m = base + j;
@@ -789,7 +808,7 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
             const int q = threadIdx.x + r * TB;
             if (q < K * K) C[q] = v[r];
         }
-        if (hs)
+        if (flg)
             for (int q = threadIdx.x; q < K * K; q += blockDim.x) F[q] = a.s.ovl[q];
     }
     if (threadIdx.x == 0) cons = Ke > 0 ? Ke : 0;
@@ -829,23 +848,18 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
             ok = __builtin_amdgcn_readlane(ok, owner) == 1;
             if (ok) {
                 const int lti = leafT[i], mv = leafO[i] != lti;
-                const TrialInfo& A = a.s.info[i];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const int j = lane + 64 * k;
                     if (j > i && j < Ke) {
                         run[k] += in_lds ? C[i * K + j] : a.s.corr[i * K + j];
                         dcr[k] += mv && (lti == leafT[j]);
-                        if (hs) {
+                        if (flg) {
                             const int f = in_lds ? F[i * K + j] : a.s.ovl[i * K + j];
-                            ovr[k] += (f >> 1) - (f & 1);
+                            if (hs) ovr[k] += ((f >> 1) & 1) - (f & 1);
+                            stl[k] |= (f >> 2) & 1;
                         }
-                        if (dyn) {
-                            const TrialInfo& B = a.s.info[j];
-                            if (B.i == A.i || within(A.o, B.o, t.L, rf2) || within(A.t, B.o, t.L, rf2) ||
-                                within(A.o, B.t, t.L, rf2) || within(A.t, B.t, t.L, rf2))
-                                stl[k] = 1;
-                        }
+                        if (dyn && a.s.info[j].i == a.s.info[i].i) stl[k] = 1;  // the same particle again
                     }
                 }
             }
@@ -1130,8 +1144,8 @@ __global__ void __launch_bounds__(TB) k_apply_rep(const DevTree<T>* __restrict__
 // ------------------------------------------------------------------ launchers
 // e0/e1 (may be null) take the kernel's own start and end timestamps
 template <typename T>
-void launch_propose(const DevTree<T>& t, const McArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_propose<T>, dim3(a.K), dim3(TB), 0, st, t, a);
+void launch_propose(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+    hipExtLaunchKernelGGL(k_propose<T>, dim3(a.K), dim3(TB), 0, st, e0, e1, 0, t, a);
 }
 
 template <typename T>
@@ -1183,7 +1197,7 @@ void launch_replica_compact(const ReplicaLaunch<T>& r, hipStream_t st) {
 
 #define INSTANTIATE(T)                                                                \
     template void launch_compact<T>(const DevTree<T>&, const int*, int, hipStream_t); \
-    template void launch_propose<T>(const DevTree<T>&, const McArgs&, hipStream_t);                      \
+    template void launch_propose<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);   \
     template void launch_setup<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);     \
     template void launch_trial<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);     \
     template void launch_pair_corr<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t); \

@@ -400,7 +400,8 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
         if (mode == 0) {
             for (int m = 0; m < n; ++m) {
                 const McArgs a = mc_args(h, mb, m, 1, kTrialMetropolis, beta);
-                if (a.prop.rf > 0.0) launch_propose(dv, a, h->stream);
+                if (a.prop.rf > 0.0)
+                    timed(h, kKPropose, [&](hipEvent_t x, hipEvent_t y) { launch_propose(dv, a, h->stream, x, y); });
                 timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
                 timed(h, kKTrial, [&](hipEvent_t x, hipEvent_t y) { launch_trial(dv, a, h->stream, x, y); });
                 timed(h, kKUpdate, [&](hipEvent_t x, hipEvent_t y) { launch_apply(dv, a, h->stream, x, y); });
@@ -419,7 +420,7 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
                     McArgs a = mc_args(h, mb, 0, batch, kTrialBatch, beta);
                     a.move_ctr = h->step_ctr;
                     a.n_moves = n;
-                    launch_propose(dv, a, h->stream);
+                    timed(h, kKPropose, [&](hipEvent_t x, hipEvent_t y) { launch_propose(dv, a, h->stream, x, y); });
                     timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
                     timed(h, kKTrial, [&](hipEvent_t x, hipEvent_t y) { launch_trial(dv, a, h->stream, x, y); });
                     if (batch > 1)
@@ -439,7 +440,8 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
             for (auto& bt : batches) {
                 const int m0 = bt.first, k = bt.second;
                 const McArgs a = mc_args(h, mb, m0, k, kTrialBatch, beta);
-                if (a.prop.rf > 0.0) launch_propose(dv, a, h->stream);  // features only (uniform proposals)
+                if (a.prop.rf > 0.0)  // features only (uniform proposals)
+                    timed(h, kKPropose, [&](hipEvent_t x, hipEvent_t y) { launch_propose(dv, a, h->stream, x, y); });
                 timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
                 timed(h, kKTrial, [&](hipEvent_t x, hipEvent_t y) { launch_trial(dv, a, h->stream, x, y); });
                 if (k > 1)

@@ -67,7 +67,7 @@ struct MoveBuf {
 // to run out.
 constexpr int kCompactEvery = 2048;
 
-enum KernelKind { kKTrial = 0, kKUpdate = 1, kKCorr = 2, kKScan = 3, kKSetup = 4, kKinds = 5 };
+enum KernelKind { kKTrial = 0, kKUpdate = 1, kKCorr = 2, kKScan = 3, kKSetup = 4, kKPropose = 5, kKinds = 6 };
 
 struct Handle {
     HPDMKParams params{};
@@ -108,8 +108,8 @@ struct Handle {
     double sigma = 0.0;
     // profiling
     int profiling = 0;  // 0 off, 1 kernel events + per-move counts, 2 + phase timers
-    double kms[kKinds] = {0, 0, 0, 0, 0};
-    int kcount[kKinds] = {0, 0, 0, 0, 0};
+    double kms[kKinds] = {0, 0, 0, 0, 0, 0};
+    int kcount[kKinds] = {0, 0, 0, 0, 0, 0};
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
 

@@ -356,10 +356,10 @@ class HPDMKPtTree:
         _check(load_library().hpdmk_set_profiling(self._h, int(level)))
 
     def kernel_times(self):
-        ms = np.zeros(5)
-        cnt = np.zeros(5, dtype=np.int32)
+        ms = np.zeros(6)
+        cnt = np.zeros(6, dtype=np.int32)
         _check(load_library().hpdmk_kernel_times(self._h, ms.ctypes.data_as(_dp), cnt.ctypes.data_as(_ip)))
-        return dict(zip(["trial", "update", "pair_corr", "scan", "setup"], zip(ms.tolist(), cnt.tolist())))
+        return dict(zip(["trial", "update", "pair_corr", "scan", "setup", "propose"], zip(ms.tolist(), cnt.tolist())))
 
     def phase_times(self):
         us = np.zeros(9)
