@@ -40,6 +40,26 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
     return s;
 }
 
+// N block-wide sums at once (one barrier): v[] holds this thread's values;
+// on return thread 0's v[] holds the block totals.  red: (blockDim / 64) * N.
+template <int N>
+__device__ __forceinline__ void block_sum_n(double* v, double* red) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+        v[c] = wave_sum(v[c]);
+        if (lane == 0) red[w * N + c] = v[c];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int c = 0; c < N; ++c) {
+            double s = 0;
+            for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += red[k * N + c];
+            v[c] = s;
+        }
+    __syncthreads();
+}
+
 // my_mod of include/utils.hpp:96-105 (repeated add/subtract, bounded)
 __device__ __forceinline__ double wrap(double x, double L) {
     int guard = 0;

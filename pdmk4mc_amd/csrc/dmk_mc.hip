@@ -122,8 +122,9 @@ __global__ void __launch_bounds__(TB) k_setup_rep(const DevTree<T>* __restrict__
 // One block per trial.
 template <typename T>
 __device__ __forceinline__ void propose_body(const DevTree<T>& t, const McArgs& a) {
-    __shared__ double red[TB / 64];
-    __shared__ double sums[kFeatSums], phi[kFeat], hbuf[2 * kHid], o[kOut], mu[3], sig[3];
+    __shared__ double redn[(TB / 64) * kFeatSums];
+    __shared__ double phi[kFeat], hbuf[2 * kHid], o[kOut], mu[3], sig[3];
+    __shared__ float wsh[kMlpParams];
     if (*t.err) return;
     const Proposal& P = a.prop;
     const int tr = blockIdx.x, m = move_base(a) + tr;
@@ -158,17 +159,14 @@ __device__ __forceinline__ void propose_body(const DevTree<T>& t, const McArgs& 
             if (r2 > 0.0 && r2 < rf2) feature_pair(ex, ey, ez, r2, t.pq[s], P.rf, fs);
         }
     }
-    for (int c = 0; c < kFeatSums; ++c) {
-        const double v = block_sum(fs[c], red);
-        if (threadIdx.x == 0) sums[c] = v;
-    }
+    block_sum_n<kFeatSums>(fs, redn);
     if (threadIdx.x == 0) {
-        features_from_sums(q, sums, phi);
+        features_from_sums(q, fs, phi);
         if (P.feat)
             for (int f = 0; f < kFeat; ++f) P.feat[(size_t)kFeat * m + f] = phi[f];
     }
     if (!P.learned) return;  // uniform across the grid
-    mlp_forward_block(P.w, P, phi, hbuf, o, mu, sig);
+    mlp_forward_block(P.w, P, phi, hbuf, o, mu, sig, wsh);
     if (threadIdx.x == 0) {
         double dl[3];
         for (int c = 0; c < 3; ++c) {
@@ -497,17 +495,18 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
     }
     const double s = block_sum(part, red);
     const double ovs = a.sigma2 > 0.0 ? block_sum((double)novl, red) : 0.0;  // uniform across the grid
-    double fts[kFeatSums] = {0, 0, 0, 0, 0, 0, 0};
     const bool target_near = b >= WB + 2 * nL * DB && b < WB + 2 * nL * DB + NB;
-    if (a.prop.learned && target_near)  // uniform across the block
-        for (int c = 0; c < kFeatSums; ++c) fts[c] = block_sum(fsum[c], red);
+    if (a.prop.learned && target_near) {  // uniform across the block
+        __shared__ double redn[(TB / 64) * kFeatSums];
+        block_sum_n<kFeatSums>(fsum, redn);
+    }
     if (threadIdx.x == 0) {
         double* p = a.s.part + kPartStride * ((size_t)tr * per + b);
         __hip_atomic_store(p, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(p + 1, (double)nscan, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(p + 2, ovs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         for (int c = 0; c < kFeatSums; ++c)
-            __hip_atomic_store(p + 3 + c, fts[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(p + 3 + c, target_near ? fsum[c] : 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     if (a.mode == kTrialBatch && !a.prop.learned) return;
     if (threadIdx.x == 0) {
@@ -518,6 +517,7 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
     __syncthreads();
     if (!last) return;
     __shared__ double sh_dE, phi[kFeat], hbuf[2 * kHid], o[kOut], mu[3], sig[3];
+    __shared__ float wsh[kMlpParams];
     __shared__ int sh_ovl;
     if (a.mode == kTrialBatch) {  // learned proposal: this trial's log-ratio H for the batch scan
         if (threadIdx.x == 0) {
@@ -526,7 +526,7 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
             sum_target_features(t, a, tr, fs);
             features_from_sums(ti.q, fs, phi);
         }
-        mlp_forward_block(a.prop.w, a.prop, phi, hbuf, o, mu, sig);
+        mlp_forward_block(a.prop.w, a.prop, phi, hbuf, o, mu, sig, wsh);
         if (threadIdx.x == 0) {
             const double back[3] = {-a.prop.dprop[3 * m], -a.prop.dprop[3 * m + 1], -a.prop.dprop[3 * m + 2]};
             a.prop.H[m] = gauss_logq(back, mu, sig) - a.prop.logq[m];
@@ -549,7 +549,7 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
     }
     double H = 0.0;
     if (learned) {  // proposal log-ratio: reverse move from the target
-        mlp_forward_block(a.prop.w, a.prop, phi, hbuf, o, mu, sig);
+        mlp_forward_block(a.prop.w, a.prop, phi, hbuf, o, mu, sig, wsh);
         if (threadIdx.x == 0) {
             const double back[3] = {-a.prop.dprop[3 * m], -a.prop.dprop[3 * m + 1], -a.prop.dprop[3 * m + 2]};
             H = gauss_logq(back, mu, sig) - a.prop.logq[m];

@@ -79,34 +79,46 @@ __host__ __device__ inline double gauss_logq(const double* delta, const double* 
     return v;
 }
 
-// FP64 forward pass, all threads of a block: phi (8), hbuf (2 kHid) and o (kOut)
-// in shared memory; on return mu/sig (3 each, shared) are set.
+// FP64 forward pass by all threads of a block (at least 256): the weights are
+// staged in shared memory (wsh, kMlpParams floats), every unit is computed by
+// 8 lanes (partial dot products joined with shuffles).  phi (8), hbuf
+// (2 kHid) and o (kOut) are shared; on return mu/sig (3 each, shared) are set.
 __device__ inline void mlp_forward_block(const float* w, const Proposal& p, const double* phi, double* hbuf,
-                                         double* o, double* mu, double* sig) {
-    const float* W1 = w;
+                                         double* o, double* mu, double* sig, float* wsh) {
+    const int t = threadIdx.x;
+    for (int k = t; k < kMlpParams; k += blockDim.x) wsh[k] = w[k];
+    const float* W1 = wsh;
     const float* b1 = W1 + kFeat * kHid;
     const float* W2 = b1 + kHid;
     const float* b2 = W2 + kHid * kHid;
     const float* W3 = b2 + kHid;
     const float* b3 = W3 + kHid * kOut;
-    const int t = threadIdx.x;
+    const int unit = t >> 3, part = t & 7;  // 8 lanes per unit
     __syncthreads();
-    if (t < kHid) {
-        double s = b1[t];
-        for (int f = 0; f < kFeat; ++f) s += (double)W1[f * kHid + t] * phi[f];
-        hbuf[t] = tanh(s);
+    if (unit < kHid) {
+        double s = (double)W1[part * kHid + unit] * phi[part];  // kFeat = 8: one input per lane
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        s += __shfl_xor(s, 4);
+        if (part == 0) hbuf[unit] = tanh(s + (double)b1[unit]);
     }
     __syncthreads();
-    if (t < kHid) {
-        double s = b2[t];
-        for (int h = 0; h < kHid; ++h) s += (double)W2[h * kHid + t] * hbuf[h];
-        hbuf[kHid + t] = tanh(s);
+    if (unit < kHid) {
+        double s = 0.0;
+        for (int h = part; h < kHid; h += 8) s += (double)W2[h * kHid + unit] * hbuf[h];
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        s += __shfl_xor(s, 4);
+        if (part == 0) hbuf[kHid + unit] = tanh(s + (double)b2[unit]);
     }
     __syncthreads();
-    if (t < kOut) {
-        double s = b3[t];
-        for (int h = 0; h < kHid; ++h) s += (double)W3[h * kOut + t] * hbuf[kHid + h];
-        o[t] = s;
+    if (unit < kOut) {
+        double s = 0.0;
+        for (int h = part; h < kHid; h += 8) s += (double)W3[h * kOut + unit] * hbuf[kHid + h];
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        s += __shfl_xor(s, 4);
+        if (part == 0) o[unit] = s + (double)b3[unit];
     }
     __syncthreads();
     if (t < 3) {
