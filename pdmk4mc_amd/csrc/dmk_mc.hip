@@ -508,7 +508,7 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
         for (int c = 0; c < kFeatSums; ++c)
             __hip_atomic_store(p + 3 + c, target_near ? fsum[c] : 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    if (a.mode == kTrialBatch && !a.prop.learned) return;
+    if (a.mode == kTrialBatch) return;
     if (threadIdx.x == 0) {
         __threadfence();
         const unsigned prev = atomicAdd(a.s.done + tr, 1u);
@@ -519,21 +519,6 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
     __shared__ double sh_dE, phi[kFeat], hbuf[2 * kHid], o[kOut], mu[3], sig[3];
     __shared__ float wsh[kMlpParams];
     __shared__ int sh_ovl;
-    if (a.mode == kTrialBatch) {  // learned proposal: this trial's log-ratio H for the batch scan
-        if (threadIdx.x == 0) {
-            __threadfence();
-            double fs[kFeatSums];
-            sum_target_features(t, a, tr, fs);
-            features_from_sums(ti.q, fs, phi);
-        }
-        mlp_forward_block(a.prop.w, a.prop, phi, hbuf, o, mu, sig, wsh);
-        if (threadIdx.x == 0) {
-            const double back[3] = {-a.prop.dprop[3 * m], -a.prop.dprop[3 * m + 1], -a.prop.dprop[3 * m + 2]};
-            a.prop.H[m] = gauss_logq(back, mu, sig) - a.prop.logq[m];
-            a.s.done[tr] = 0;
-        }
-        return;
-    }
     unsigned long long tp = wall_clock64();
     const bool learned = a.prop.learned && a.mode == kTrialMetropolis;
     if (threadIdx.x == 0) {
@@ -602,6 +587,29 @@ __device__ __forceinline__ bool within(const double* x, const double* y, double 
         s += v * v;
     }
     return s < r2max;
+}
+
+// Learned proposal in a batch: each trial's proposal log-ratio
+// H = log q(-delta | phi(t)) - log q(delta | phi(o)) from the target feature
+// sums of its trial blocks (kernel boundary: no cross-block fences).  One
+// block per trial.
+template <typename T>
+__global__ void __launch_bounds__(TB) k_reverse(DevTree<T> t, McArgs a) {
+    __shared__ double phi[kFeat], hbuf[2 * kHid], o[kOut], mu[3], sig[3];
+    __shared__ float wsh[kMlpParams];
+    if (*t.err) return;
+    const int tr = blockIdx.x, m = move_base(a) + tr;
+    if (tr >= keff(a)) return;
+    if (threadIdx.x == 0) {
+        double fs[kFeatSums];
+        sum_target_features(t, a, tr, fs);
+        features_from_sums(a.s.info[tr].q, fs, phi);
+    }
+    mlp_forward_block(a.prop.w, a.prop, phi, hbuf, o, mu, sig, wsh);
+    if (threadIdx.x == 0) {
+        const double back[3] = {-a.prop.dprop[3 * m], -a.prop.dprop[3 * m + 1], -a.prop.dprop[3 * m + 2]};
+        a.prop.H[m] = gauss_logq(back, mu, sig) - a.prop.logq[m];
+    }
 }
 
 // ------------------------------------------------------------------ batched trials
@@ -1159,6 +1167,11 @@ void launch_trial(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent
 }
 
 template <typename T>
+void launch_reverse(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+    hipExtLaunchKernelGGL(k_reverse<T>, dim3(a.K), dim3(TB), 0, st, e0, e1, 0, t, a);
+}
+
+template <typename T>
 void launch_pair_corr(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     const int np = a.K * (a.K - 1) / 2;
     if (np > 0) hipExtLaunchKernelGGL(k_pair_corr<T>, dim3(np), dim3(TB), 0, st, e0, e1, 0, t, a);
@@ -1198,6 +1211,7 @@ void launch_replica_compact(const ReplicaLaunch<T>& r, hipStream_t st) {
 #define INSTANTIATE(T)                                                                \
     template void launch_compact<T>(const DevTree<T>&, const int*, int, hipStream_t); \
     template void launch_propose<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);   \
+    template void launch_reverse<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);   \
     template void launch_setup<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);     \
     template void launch_trial<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);     \
     template void launch_pair_corr<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t); \

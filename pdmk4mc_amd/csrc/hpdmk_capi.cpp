@@ -423,6 +423,7 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
                     timed(h, kKPropose, [&](hipEvent_t x, hipEvent_t y) { launch_propose(dv, a, h->stream, x, y); });
                     timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
                     timed(h, kKTrial, [&](hipEvent_t x, hipEvent_t y) { launch_trial(dv, a, h->stream, x, y); });
+                    timed(h, kKPropose, [&](hipEvent_t x, hipEvent_t y) { launch_reverse(dv, a, h->stream, x, y); });
                     if (batch > 1)
                         timed(h, kKCorr,
                               [&](hipEvent_t x, hipEvent_t y) { launch_pair_corr(dv, a, h->stream, x, y); });
