@@ -148,6 +148,10 @@ void launch_replica_compact(const ReplicaLaunch<T>& r, hipStream_t st);
 template <typename T>
 void launch_propose(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0 = nullptr,
                     hipEvent_t e1 = nullptr);
+// single moves and trial-only batches: dE, decision, update plan
+template <typename T>
+void launch_decide(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0 = nullptr,
+                   hipEvent_t e1 = nullptr);
 // learned proposal in a batch: the reverse-move log-ratio of every trial
 template <typename T>
 void launch_reverse(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0 = nullptr,

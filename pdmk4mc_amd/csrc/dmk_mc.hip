@@ -399,12 +399,11 @@ __device__ double assemble_trial(const DevTree<T>& t, const McArgs& a, int tr, i
 //   [0, win)                               window
 //   [.., + 2*nL*diff)                      difference (endpoint, level, share)
 //   [.., + 2*near)                         near field (endpoint, share of the near list)
-// Every block stores its partial sum; in single-move modes the last block of
-// the trial assembles dE, decides and plans the update.
+// Every block stores its partial sums; k_decide (single moves) or k_scan
+// (batches) assembles them after the kernel ends.
 template <typename T>
 __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a) {
     __shared__ double red[TB / 64];
-    __shared__ int last;
     if (*t.err) return;
     const int tr = blockIdx.y, b = blockIdx.x, m = move_base(a) + tr;
     if (tr >= keff(a)) return;
@@ -508,21 +507,24 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
         for (int c = 0; c < kFeatSums; ++c)
             __hip_atomic_store(p + 3 + c, target_near ? fsum[c] : 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    if (a.mode == kTrialBatch) return;
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned prev = atomicAdd(a.s.done + tr, 1u);
-        last = (prev == (unsigned)per - 1);
-    }
-    __syncthreads();
-    if (!last) return;
+}
+
+// Decision of one trial (single-move modes; trial-only batches): assembles dE
+// from the partials of its trial blocks (kernel boundary: no cross-block
+// fences), adds the learned proposal's log-ratio, takes the Metropolis
+// decision and, for a single move, plans the update.  One block per trial.
+template <typename T>
+__device__ __forceinline__ void decide_body(const DevTree<T>& t, const McArgs& a) {
     __shared__ double sh_dE, phi[kFeat], hbuf[2 * kHid], o[kOut], mu[3], sig[3];
     __shared__ float wsh[kMlpParams];
     __shared__ int sh_ovl;
+    if (*t.err) return;
+    const int tr = blockIdx.x, m = move_base(a) + tr;
+    if (tr >= keff(a)) return;
+    const TrialInfo& ti = a.s.info[tr];
     unsigned long long tp = wall_clock64();
     const bool learned = a.prop.learned && a.mode == kTrialMetropolis;
     if (threadIdx.x == 0) {
-        __threadfence();
         int ovl = 0;
         sh_dE = assemble_trial(t, a, tr, m, &ovl);
         sh_ovl = ovl;
@@ -555,12 +557,22 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
         if (ok) ok = capacity_ok(t, ti, 0);
         a.s.accept[tr] = ok;
         if (a.ml.accept_out) a.ml.accept_out[m] = ok;
-        a.s.done[tr] = 0;  // ready for the next launch
         probe_add(a.s, kPrTrialFinal, tp);
     }
     if (a.mode == kTrialOnly) return;
     __syncthreads();
     plan_update(t, a, 1);  // single-move step: this block plans the update
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TB) k_decide(DevTree<T> t, McArgs a) {
+    decide_body(t, a);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TB) k_decide_rep(const DevTree<T>* __restrict__ ts,
+                                                   const McArgs* __restrict__ as) {
+    decide_body(ts[blockIdx.z], as[blockIdx.z]);
 }
 
 template <typename T>
@@ -1167,6 +1179,11 @@ void launch_trial(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent
 }
 
 template <typename T>
+void launch_decide(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+    hipExtLaunchKernelGGL(k_decide<T>, dim3(a.K), dim3(TB), 0, st, e0, e1, 0, t, a);
+}
+
+template <typename T>
 void launch_reverse(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     hipExtLaunchKernelGGL(k_reverse<T>, dim3(a.K), dim3(TB), 0, st, e0, e1, 0, t, a);
 }
@@ -1198,6 +1215,7 @@ void launch_replica_step(const ReplicaLaunch<T>& r, hipStream_t st) {
     hipLaunchKernelGGL(k_setup_rep<T>, rep, dim3(TB), 0, st, r.trees, r.args);
     hipLaunchKernelGGL(k_trial_rep<T>, dim3(trial_blocks(r.max_depth, r.tiles), 1, r.n), dim3(TB), 0, st, r.trees,
                        r.args);
+    hipLaunchKernelGGL(k_decide_rep<T>, rep, dim3(TB), 0, st, r.trees, r.args);
     hipLaunchKernelGGL(k_apply_rep<T>, dim3(r.apply_blocks, 1, r.n), dim3(TB), 0, st, r.trees, r.args);
     hipLaunchKernelGGL(k_advance, dim3((r.n + TB - 1) / TB), dim3(TB), 0, st, r.move_ctr, r.n);
 }
@@ -1212,6 +1230,7 @@ void launch_replica_compact(const ReplicaLaunch<T>& r, hipStream_t st) {
     template void launch_compact<T>(const DevTree<T>&, const int*, int, hipStream_t); \
     template void launch_propose<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);   \
     template void launch_reverse<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);   \
+    template void launch_decide<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);    \
     template void launch_setup<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);     \
     template void launch_trial<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);     \
     template void launch_pair_corr<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t); \

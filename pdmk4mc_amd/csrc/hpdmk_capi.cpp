@@ -404,6 +404,7 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
                     timed(h, kKPropose, [&](hipEvent_t x, hipEvent_t y) { launch_propose(dv, a, h->stream, x, y); });
                 timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
                 timed(h, kKTrial, [&](hipEvent_t x, hipEvent_t y) { launch_trial(dv, a, h->stream, x, y); });
+                timed(h, kKScan, [&](hipEvent_t x, hipEvent_t y) { launch_decide(dv, a, h->stream, x, y); });
                 timed(h, kKUpdate, [&](hipEvent_t x, hipEvent_t y) { launch_apply(dv, a, h->stream, x, y); });
                 compact_if_due(1);
             }
@@ -568,6 +569,7 @@ int hpdmk_eval_shift_energy(hpdmk_tree t, long i, double dx, double dy, double d
             const McArgs a = mc_args(h, h->single, 0, 1, kTrialOnly, 0.0);
             launch_setup(dv, a, h->stream);
             launch_trial(dv, a, h->stream);
+            launch_decide(dv, a, h->stream);
         });
         HIPX(hipGetLastError());
         double p[5], e;
@@ -588,6 +590,7 @@ int hpdmk_update_shift(hpdmk_tree t, long i, double dx, double dy, double dz) {
             const McArgs a = mc_args(h, h->single, 0, 1, kTrialForceAccept, 0.0);
             launch_setup(dv, a, h->stream);
             launch_trial(dv, a, h->stream);
+            launch_decide(dv, a, h->stream);
             launch_apply(dv, a, h->stream);
             if (++h->since_compact >= kCompactEvery) {
                 launch_compact(dv, h->leaves, h->n_leaves, h->stream);
@@ -632,6 +635,7 @@ int hpdmk_eval_shift_energy_batch(hpdmk_tree t, int n, const long* idx, const do
                 const McArgs a = mc_args(h, h->single, m0, k, kTrialOnly, 0.0);
                 launch_setup(dv, a, h->stream);
                 launch_trial(dv, a, h->stream);
+                launch_decide(dv, a, h->stream);
             });
         }
         HIPX(hipGetLastError());
