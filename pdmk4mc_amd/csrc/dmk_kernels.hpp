@@ -141,6 +141,8 @@ struct ReplicaLaunch {
 
 template <typename T>
 void launch_replica_step(const ReplicaLaunch<T>& r, hipStream_t st);
+// move_ctr[0..n) += 1
+void launch_advance(int* move_ctr, int n, hipStream_t st);
 template <typename T>
 void launch_replica_compact(const ReplicaLaunch<T>& r, hipStream_t st);
 

@@ -1220,6 +1220,10 @@ void launch_replica_step(const ReplicaLaunch<T>& r, hipStream_t st) {
     hipLaunchKernelGGL(k_advance, dim3((r.n + TB - 1) / TB), dim3(TB), 0, st, r.move_ctr, r.n);
 }
 
+void launch_advance(int* move_ctr, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_advance, dim3((n + TB - 1) / TB), dim3(TB), 0, st, move_ctr, n);
+}
+
 template <typename T>
 void launch_replica_compact(const ReplicaLaunch<T>& r, hipStream_t st) {
     hipLaunchKernelGGL(k_compact_rep<T>, dim3(r.max_leaves, 1, r.n), dim3(TB), 0, st, r.trees, r.leaves,

@@ -397,7 +397,45 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
                 h->since_compact = 0;
             }
         };
-        if (mode == 0) {
+        if (mode == 0 && !h->profiling && n >= 2 * kPlainGraphMoves) {
+            // plain chain from a HIP graph of kPlainGraphMoves moves: the
+            // kernels read the move index from a device counter that the last
+            // node of every move advances
+            if (!h->step_ctr) h->step_ctr = dalloc<int>(1, h->owned);
+            HIPX(hipMemsetAsync(h->step_ctr, 0, sizeof(int), h->stream));
+            McArgs a = mc_args(h, mb, 0, 1, kTrialMetropolis, beta);
+            a.move_ctr = h->step_ctr;
+            a.n_moves = n;
+            auto one_move = [&] {
+                if (a.prop.rf > 0.0) launch_propose(dv, a, h->stream);
+                launch_setup(dv, a, h->stream);
+                launch_trial(dv, a, h->stream);
+                launch_decide(dv, a, h->stream);
+                launch_apply(dv, a, h->stream);
+                launch_advance(h->step_ctr, 1, h->stream);
+            };
+            struct Graph {
+                hipGraph_t g = nullptr;
+                hipGraphExec_t x = nullptr;
+                ~Graph() {
+                    if (x) (void)hipGraphExecDestroy(x);
+                    if (g) (void)hipGraphDestroy(g);
+                }
+            } gr;
+            HIPX(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < kPlainGraphMoves; ++k) one_move();
+            HIPX(hipStreamEndCapture(h->stream, &gr.g));
+            HIPX(hipGraphInstantiate(&gr.x, gr.g, nullptr, nullptr, 0));
+            int m = 0;
+            for (; m + kPlainGraphMoves <= n; m += kPlainGraphMoves) {
+                HIPX(hipGraphLaunch(gr.x, h->stream));
+                compact_if_due(kPlainGraphMoves);
+            }
+            for (; m < n; ++m) {
+                one_move();
+                compact_if_due(1);
+            }
+        } else if (mode == 0) {
             for (int m = 0; m < n; ++m) {
                 const McArgs a = mc_args(h, mb, m, 1, kTrialMetropolis, beta);
                 if (a.prop.rf > 0.0)

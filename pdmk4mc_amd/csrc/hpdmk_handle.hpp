@@ -66,6 +66,8 @@ struct MoveBuf {
 // count (at least count + 64), so a leaf would need 64 net arrivals in between
 // to run out.
 constexpr int kCompactEvery = 2048;
+// moves per captured graph of the plain chain
+constexpr int kPlainGraphMoves = 64;
 
 enum KernelKind { kKTrial = 0, kKUpdate = 1, kKCorr = 2, kKScan = 3, kKSetup = 4, kKPropose = 5, kKinds = 6 };
 
