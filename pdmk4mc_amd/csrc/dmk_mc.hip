@@ -316,54 +316,65 @@ __device__ int capacity_ok(const DevTree<T>& t, const TrialInfo& ti, int delta) 
 }
 
 // ------------------------------------------------------------------ trial
-// Sums the per-block partials of trial tr into v: window, difference,
-// res_t, res_o, particles scanned at t and o.  Callers make the partials
-// visible first (kernel boundary, or an agent-scope fence after the block
-// counter); plain loads then see them.
+// Sums of the per-block partials of trial tr: window, difference, res_t,
+// res_o, particles scanned at t and o, hard-sphere overlaps at t.  The gs
+// lanes of a contiguous lane group (g0 = rank in the group, gs a power of two
+// <= 64) take the blocks g0, g0 + gs, ... and join their sums with shuffles:
+// every lane of the group ends with the totals.  Callers make the partials
+// visible first (kernel boundary); plain loads then see them.
 template <typename T>
-__device__ void sum_trial_partials(const DevTree<T>& t, const McArgs& a, int tr, double* v) {
+__device__ void sum_trial_partials(const DevTree<T>& t, const McArgs& a, int tr, int g0, int gs, double* v) {
     const Tiles& g = a.tiles;
     const int D = t.max_depth, nL = n_levels_pw(D), per = trial_blocks(D, g);
     const double* p = a.s.part + kPartStride * (size_t)tr * per;
-    const int d0 = g.win, d1 = d0 + 2 * nL * g.diff;
-    double w = 0, df = 0, rt = 0, ro = 0, nt = 0, no = 0, ov = 0;
-    for (int b = 0; b < g.win; ++b) w += p[kPartStride * b];
-    for (int b = d0; b < d1; ++b) df += p[kPartStride * b];
-    for (int b = 0; b < g.near; ++b) {
-        rt += p[kPartStride * (d1 + b)];
-        nt += p[kPartStride * (d1 + b) + 1];
-        ov += p[kPartStride * (d1 + b) + 2];
-        ro += p[kPartStride * (d1 + g.near + b)];
-        no += p[kPartStride * (d1 + g.near + b) + 1];
+    const int d0 = g.win, d1 = d0 + 2 * nL * g.diff, d2 = d1 + g.near;
+#pragma unroll
+    for (int c = 0; c < kAccPerTrial; ++c) v[c] = 0.0;
+    for (int b = g0; b < per; b += gs) {
+        const double* q = p + kPartStride * b;
+        if (b < d0) {
+            v[0] += q[0];
+        } else if (b < d1) {
+            v[1] += q[0];
+        } else if (b < d2) {
+            v[2] += q[0];
+            v[4] += q[1];
+            v[6] += q[2];
+        } else {
+            v[3] += q[0];
+            v[5] += q[1];
+        }
     }
-    v[0] = w;
-    v[1] = df;
-    v[2] = rt;
-    v[3] = ro;
-    v[4] = nt;
-    v[5] = no;
-    v[6] = ov;
+    for (int o = 1; o < gs; o <<= 1)
+#pragma unroll
+        for (int c = 0; c < kAccPerTrial; ++c) v[c] += __shfl_xor(v[c], o);
 }
 
-// feature sums of the target's near field (learned proposal) from the partials
+// feature sums of the target's near field (learned proposal) from the
+// partials, by a lane group as above
 template <typename T>
-__device__ void sum_target_features(const DevTree<T>& t, const McArgs& a, int tr, double* fs) {
+__device__ void sum_target_features(const DevTree<T>& t, const McArgs& a, int tr, int g0, int gs, double* fs) {
     const Tiles& g = a.tiles;
     const int nL = n_levels_pw(t.max_depth), per = trial_blocks(t.max_depth, g);
     const double* p = a.s.part + kPartStride * (size_t)tr * per;
     const int d1 = g.win + 2 * nL * g.diff;
+#pragma unroll
     for (int c = 0; c < kFeatSums; ++c) fs[c] = 0.0;
-    for (int b = 0; b < g.near; ++b)
+    for (int b = g0; b < g.near; b += gs)
+#pragma unroll
         for (int c = 0; c < kFeatSums; ++c) fs[c] += p[kPartStride * (d1 + b) + 3 + c];
+    for (int o = 1; o < gs; o <<= 1)
+#pragma unroll
+        for (int c = 0; c < kFeatSums; ++c) fs[c] += __shfl_xor(fs[c], o);
 }
 
-// dE of trial tr from its partials (src/energy_shift.cpp:67) plus the optional
-// per-move parts and counts; *overlaps = particles within sigma of the target
+// dE of trial tr from its partial sums v (src/energy_shift.cpp:67) plus the
+// optional per-move parts and counts; *overlaps = particles within sigma of
+// the target
 template <typename T>
-__device__ double assemble_trial(const DevTree<T>& t, const McArgs& a, int tr, int m, int* overlaps) {
+__device__ double assemble_trial(const DevTree<T>& t, const McArgs& a, int tr, int m, const double* v,
+                                 int* overlaps) {
     const TrialInfo& ti = a.s.info[tr];
-    double v[kAccPerTrial];
-    sum_trial_partials(t, a, tr, v);
     *overlaps = (int)v[6];
     const double self = self_term(t, ti);
     const double dE = v[0] + v[1] + v[2] - v[3] - self;
@@ -524,14 +535,15 @@ __device__ __forceinline__ void decide_body(const DevTree<T>& t, const McArgs& a
     const TrialInfo& ti = a.s.info[tr];
     unsigned long long tp = wall_clock64();
     const bool learned = a.prop.learned && a.mode == kTrialMetropolis;
-    if (threadIdx.x == 0) {
-        int ovl = 0;
-        sh_dE = assemble_trial(t, a, tr, m, &ovl);
-        sh_ovl = ovl;
-        if (learned) {
-            double fs[kFeatSums];
-            sum_target_features(t, a, tr, fs);
-            features_from_sums(ti.q, fs, phi);
+    if (threadIdx.x < 64) {  // wave 0 sums the partials, lane 0 assembles
+        double v[kAccPerTrial], fs[kFeatSums];
+        sum_trial_partials(t, a, tr, threadIdx.x, 64, v);
+        if (learned) sum_target_features(t, a, tr, threadIdx.x, 64, fs);
+        if (threadIdx.x == 0) {
+            int ovl = 0;
+            sh_dE = assemble_trial(t, a, tr, m, v, &ovl);
+            sh_ovl = ovl;
+            if (learned) features_from_sums(ti.q, fs, phi);
         }
     }
     double H = 0.0;
@@ -612,10 +624,10 @@ __global__ void __launch_bounds__(TB) k_reverse(DevTree<T> t, McArgs a) {
     if (*t.err) return;
     const int tr = blockIdx.x, m = move_base(a) + tr;
     if (tr >= keff(a)) return;
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 64) {
         double fs[kFeatSums];
-        sum_target_features(t, a, tr, fs);
-        features_from_sums(a.s.info[tr].q, fs, phi);
+        sum_target_features(t, a, tr, threadIdx.x, 64, fs);
+        if (threadIdx.x == 0) features_from_sums(a.s.info[tr].q, fs, phi);
     }
     mlp_forward_block(a.prop.w, a.prop, phi, hbuf, o, mu, sig, wsh);
     if (threadIdx.x == 0) {
@@ -801,10 +813,17 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
     const bool dyn = a.move_ctr != nullptr;
     const bool flg = hs || a.prop.learned;  // pair flags present
     unsigned long long tp = wall_clock64();
-    for (int j = threadIdx.x; j < Ke; j += blockDim.x) {
+    // gs lanes per trial sum its partials (gs = 256 / K rounded to a power of two)
+    int gs = 1;
+    while (gs < 64 && gs * 2 * K <= (int)blockDim.x) gs *= 2;
+    for (int w0 = threadIdx.x; w0 < Ke * gs; w0 += blockDim.x) {
+        const int j = w0 / gs, g0 = w0 % gs;
+        double v[kAccPerTrial];
+        sum_trial_partials(t, a, j, g0, gs, v);
+        if (g0 != 0) continue;
         const int m = base + j;
         int ov = 0;
-        const double dE = assemble_trial(t, a, j, m, &ov);
+        const double dE = assemble_trial(t, a, j, m, v, &ov);
         dEs[j] = dE;
         ov0[j] = ov;
         a.ml.dE_stale[m] = dE;
