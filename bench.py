@@ -55,6 +55,33 @@ def measured_traffic(path, kernel):
     return rec.get("bytes_per_launch")
 
 
+def hard_spheres(n, packing, sigma, seed):
+    """n non-overlapping spheres (random sequential addition on a cell grid) in
+    a periodic cube at the packing fraction; charges alternating +1/-1."""
+    L = (n * np.pi / 6.0 * sigma ** 3 / packing) ** (1.0 / 3.0)
+    nc = max(1, int(L // sigma))
+    cell = L / nc
+    grid = {}
+    rng = np.random.default_rng(seed)
+    r = np.zeros((n, 3))
+    k = 0
+    while k < n:
+        x = rng.uniform(0, L, 3)
+        c = tuple((x // cell).astype(int) % nc)
+        near = [grid.get(((c[0] + a) % nc, (c[1] + b) % nc, (c[2] + e) % nc), ())
+                for a in (-1, 0, 1) for b in (-1, 0, 1) for e in (-1, 0, 1)]
+        ids = [i for lst in near for i in lst]
+        if ids:
+            dv = r[ids] - x
+            dv -= L * np.rint(dv / L)
+            if (dv * dv).sum(1).min() < sigma * sigma:
+                continue
+        r[k] = x
+        grid.setdefault(c, []).append(k)
+        k += 1
+    return r, np.where(np.arange(n) % 2 == 0, 1.0, -1.0), L
+
+
 def workload(n, L, n_moves, seed):
     rng = np.random.default_rng(seed)
     r = rng.uniform(0, L, (n, 3))
@@ -102,11 +129,17 @@ def main():
     ap.add_argument("--train-moves", type=int, default=20000)
     ap.add_argument("--train-step", type=float, default=2.0, help="uniform step of the training-data chain")
     ap.add_argument("--train-epochs", type=int, default=20)
+    ap.add_argument("--hs-n", type=int, default=20000)
+    ap.add_argument("--hs-moves", type=int, default=20000)
+    ap.add_argument("--hs-step", type=float, default=0.5, help="uniform step of the hard-sphere chain")
+    ap.add_argument("--hs-train", type=int, default=100000, help="uniform-chain moves that train the MLP")
+    ap.add_argument("--hs-train-step", type=float, default=1.0)
+    ap.add_argument("--hs-rf", type=float, default=2.0)
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--replica-moves", type=int, default=1000)
     ap.add_argument("--graph-steps", type=int, default=64)
-    ap.add_argument("--legs", default="batched,learned,plain,accuracy,cpu,copy,replicas",
-                    help="comma list of batched, learned, plain, accuracy, cpu, copy, replicas (copy = a 1 GiB "
+    ap.add_argument("--legs", default="batched,learned,plain,accuracy,cpu,copy,replicas,hardspheres",
+                    help="comma list of batched, learned, plain, accuracy, cpu, copy, replicas, hardspheres, large (copy = a 1 GiB "
                          "streaming copy: measured HBM ceiling and byte-counter calibration; replicas = "
                          "256 ragged independent systems stepped together)")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_update_batched_n1e5.json"),
@@ -406,6 +439,67 @@ def main():
                     "build_s": round(t_rb, 2), "acceptance": round(float(racc.mean()), 4)}
         print(f"# replicas: {rep_info}", file=sys.stderr, flush=True)
 
+    hs_info = None
+    if "hardspheres" in legs:
+        # charged hard spheres: N = 2e4, sigma = 1, packing 0.2, Bjerrum length 2;
+        # uniform proposal vs the MLP proposal trained on uniform-chain data
+        hr, hq, hL = hard_spheres(args.hs_n, 0.2, 1.0, args.seed + 31)
+        hparams = P.HPDMKParams(args.leaf, args.digits, hL, init=P.DIRECT)
+        rng_h = np.random.default_rng(args.seed + 32)
+
+        def hs_tree():
+            gh = P.HPDMKPtTree(hparams, hr, hq, prec)
+            gh.form_outgoing_pw()
+            gh.form_incoming_pw()
+            gh.mc_set_model(2.0, 1.0)
+            return gh
+
+        hcfg = dict(max_mean=0.3, sigma_min=0.02, sigma_max=1.0)
+        Mh = args.hs_moves
+        hi = rng_h.integers(0, args.hs_n, Mh)
+        hu = rng_h.uniform(0, 1, Mh)
+        out = {}
+        gh = hs_tree()
+        gh.mc_set_proposal(False, rf=args.hs_rf)  # training data: uniform chain, origin features recorded
+        ti = rng_h.integers(0, args.hs_n, args.hs_train)
+        td = rng_h.uniform(-args.hs_train_step, args.hs_train_step, (args.hs_train, 3))
+        gh.mc_load(ti, td, rng_h.uniform(0, 1, args.hs_train))
+        gh.mc_run(1.0, 1, batch)
+        _, tacc = gh.mc_results()
+        tdl, tft, _ = gh.mc_proposal_results()
+        gh.close()
+        w_h, l_h = P.mlp_train(tft, tdl, tacc.astype(np.float64), P.mlp_init(args.seed), lr=3e-3,
+                               epochs=args.train_epochs, batch=1024, seed=args.seed, **hcfg)
+        for name, learned in (("uniform", False), ("learned", True)):
+            gh = hs_tree()
+            if learned:
+                gh.mc_set_proposal(True, weights=w_h, rf=args.hs_rf, **hcfg)
+                dd = rng_h.normal(size=(Mh, 3))
+            else:
+                dd = rng_h.uniform(-args.hs_step, args.hs_step, (Mh, 3))
+            gh.mc_load(hi[:2000], dd[:2000], hu[:2000])
+            gh.mc_run(1.0, 1, batch)  # warm-up
+            gh.mc_load(hi, dd, hu)
+            t0 = time.perf_counter()
+            gh.mc_run(1.0, 1, batch)
+            wh = time.perf_counter() - t0
+            dEh, ah = gh.mc_results()
+            dlh = gh.mc_proposal_results()[0] if learned else dd
+            esjd = float((ah * (dlh ** 2).sum(1)).mean())
+            out[name] = {"moves_per_s": round(Mh / wh, 1), "acceptance": round(float(ah.mean()), 4),
+                         "overlap_rejections": round(float(np.isinf(dEh).mean()), 4),
+                         "esjd_per_move": round(esjd, 5), "esjd_per_s": round(esjd * Mh / wh, 1)}
+            gh.close()
+        hs_info = {"workload": f"N={args.hs_n} charged hard spheres, sigma 1, packing 0.2 (L={hL:.2f}), "
+                               f"Bjerrum length 2, digits {args.digits}, leaf {args.leaf}, batched chain",
+                   "uniform_step": args.hs_step, "feature_radius": args.hs_rf,
+                   "training": {"moves": args.hs_train, "uniform_step": args.hs_train_step,
+                                "acceptance": round(float(tacc.mean()), 4), "nll_first": round(float(l_h[0]), 4),
+                                "nll_last": round(float(l_h[-1]), 4)},
+                   **out,
+                   "esjd_per_s_learned_over_uniform": round(out["learned"]["esjd_per_s"] / out["uniform"]["esjd_per_s"], 3)}
+        print(f"# hard spheres: {hs_info}", file=sys.stderr, flush=True)
+
     copy_gbs = None
     if "copy" in legs:
         # streaming copy of 1 GiB: the measured HBM ceiling, and the
@@ -435,6 +529,7 @@ def main():
         "roofline": dict(roofline, measured_copy_gbs=round(copy_gbs, 1) if copy_gbs else None),
         "cpu_baseline": cpu,
         "replicas": rep_info,
+        "hard_spheres": hs_info,
         "accuracy": acc_info,
     }
     print(json.dumps(out))
