@@ -135,6 +135,7 @@ def main():
     ap.add_argument("--hs-train", type=int, default=100000, help="uniform-chain moves that train the MLP")
     ap.add_argument("--hs-train-step", type=float, default=1.0)
     ap.add_argument("--hs-rf", type=float, default=2.0)
+    ap.add_argument("--large-moves", type=int, default=100000)
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--replica-moves", type=int, default=1000)
     ap.add_argument("--graph-steps", type=int, default=64)
@@ -500,6 +501,48 @@ def main():
                    "esjd_per_s_learned_over_uniform": round(out["learned"]["esjd_per_s"] / out["uniform"]["esjd_per_s"], 3)}
         print(f"# hard spheres: {hs_info}", file=sys.stderr, flush=True)
 
+    large_info = None
+    if "large" in legs:
+        # N = 1e6, L = 215.4, charges 90% +-1 and 10% +-2 (neutral), digits 6:
+        # full build, then 1e5 batched shifts; dE against FP64 Ewald
+        # (s = 4, alpha = 0.1: r_c = 40, 57^3 modes) on a sample of moves
+        NL, LL = 1000000, 215.443469
+        rng_g = np.random.default_rng(args.seed + 41)
+        rg = rng_g.uniform(0, LL, (NL, 3))
+        kk = np.arange(NL)
+        qg = np.where(kk % 2 == 0, 1.0, -1.0)
+        qg[kk % 20 < 2] *= 2.0  # one +2 and one -2 in every 20: 10% of magnitude 2, neutral
+        gparams = P.HPDMKParams(args.leaf, args.digits, LL, init=P.DIRECT)
+        t0 = time.perf_counter()
+        gg = P.HPDMKPtTree(gparams, rg, qg, prec)
+        gg.form_outgoing_pw()
+        gg.form_incoming_pw()
+        Eg = gg.eval_energy()
+        t_bg = time.perf_counter() - t0
+        Mg = args.large_moves
+        gi = rng_g.integers(0, NL, Mg + 64)
+        gd = rng_g.uniform(-1, 1, (Mg + 64, 3))
+        gu = rng_g.uniform(0, 1, Mg + 64)
+        sdE = gg.eval_shift_energy_batch(gi[:64], gd[:64])
+        gg.mc_load(gi[64:], gd[64:], gu[64:])
+        t0 = time.perf_counter()
+        gg.mc_run(1.0, 1, batch)
+        wg = time.perf_counter() - t0
+        _, ag = gg.mc_results()
+        depth = gg.max_depth
+        gg.close()
+        t0 = time.perf_counter()
+        ewg = P.Ewald(LL, 4.0, 0.1, 1.0, qg, rg)
+        dE_ew = ewg.shift_chain(gi[:64], gd[:64])
+        ewg.close()
+        large_info = {"workload": "N=1e6, L=215.4, 90% +-1 / 10% +-2 neutral, digits 6, leaf 200",
+                      "levels": depth, "build_energy_s": round(t_bg, 3), "E": Eg,
+                      "moves_per_s": round(Mg / wg, 1), "moves": Mg, "acceptance": round(float(ag.mean()), 4),
+                      "dE_vs_ewald_max_rel_to_E": float(np.abs(sdE - dE_ew).max() / abs(Eg)),
+                      "ewald": {"s": 4.0, "alpha": 0.1, "sample_moves": 64,
+                                "seconds": round(time.perf_counter() - t0, 2)}}
+        print(f"# large: {large_info}", file=sys.stderr, flush=True)
+
     copy_gbs = None
     if "copy" in legs:
         # streaming copy of 1 GiB: the measured HBM ceiling, and the
@@ -530,6 +573,7 @@ def main():
         "cpu_baseline": cpu,
         "replicas": rep_info,
         "hard_spheres": hs_info,
+        "large": large_info,
         "accuracy": acc_info,
     }
     print(json.dumps(out))
