@@ -644,10 +644,49 @@ __global__ void __launch_bounds__(TB) k_reverse(DevTree<T> t, McArgs a) {
 //               when i's target box at level l is B or a neighbour of B
 //               (s = phase of the image of t_i next to B), minus the same for o_i
 //   near field: +-q_i q_j R_b(|x_j - y|) for y in {t_i, o_i}, b of x_j's leaf
+// Weighted half-grid sum of the product of two separable waves,
+//   Re sum_k W[k] a[k] conj(b[k]),  a[k] = ax[x] ay[y] az[z],  k = x + d y + d^2 z,
+//   = Re sum_{y,z} (ay conj by)[y] (az conj bz)[z] sum_x W[k] (ax conj bx)[x]:
+// d^2 nz real-times-complex products instead of forming every phase.  The
+// factor rows go to shared A/B/C; each thread takes (y, z) rows and returns
+// its share (the caller reduces over the block).
+template <typename T>
+__device__ double separable_dot(const T* W, int d, int nz, const Cplx<T>* ax, const Cplx<T>* ay,
+                                const Cplx<T>* az, const Cplx<T>* bx, const Cplx<T>* by, const Cplx<T>* bz,
+                                Cplx<T>* A, Cplx<T>* B, Cplx<T>* C) {
+    __syncthreads();  // the previous term's A/B/C are consumed
+    for (int x = threadIdx.x; x < 2 * d + nz; x += blockDim.x) {
+        const Cplx<T> u = x < d ? ax[x] : x < 2 * d ? ay[x - d] : az[x - 2 * d];
+        const Cplx<T> v = x < d ? bx[x] : x < 2 * d ? by[x - d] : bz[x - 2 * d];
+        const Cplx<T> r = {u.x * v.x + u.y * v.y, u.y * v.x - u.x * v.y};  // u conj(v)
+        if (x < d)
+            A[x] = r;
+        else if (x < 2 * d)
+            B[x - d] = r;
+        else
+            C[x - 2 * d] = r;
+    }
+    __syncthreads();
+    double acc = 0.0;
+    for (int rr = threadIdx.x; rr < d * nz; rr += blockDim.x) {
+        const int y = rr % d, z = rr / d;
+        const T* w = W + (size_t)d * y + (size_t)d * d * z;
+        T sr = 0, si = 0;
+        for (int x = 0; x < d; ++x) {
+            const T wx = w[x];
+            sr += wx * A[x].x;
+            si += wx * A[x].y;
+        }
+        const Cplx<T> bc = cmul(B[y], C[z]);
+        acc += (double)(bc.x * sr - bc.y * si);
+    }
+    return acc;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
     __shared__ double red[TB / 64];
-    __shared__ Cplx<T> rows[21 * kRowStride];
+    __shared__ Cplx<T> A[kRowStride], B[kRowStride], C[kRowStride];
     const int K = a.K;
     int p = blockIdx.x, i = 0;
     while (p >= K - 1 - i) {
@@ -656,43 +695,25 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
     }
     const int j = i + 1 + p;
     if (*t.err || j >= keff(a)) return;
-    const TrialInfo& A = a.s.info[i];
+    const TrialInfo& A_ = a.s.info[i];
     const TrialInfo& Bj = a.s.info[j];
     const int D = t.max_depth, nL = n_levels_pw(D), stride = table_stride(D);
     const Cplx<T>* ta = tabs_of<T>(a.s, stride, i);
     const Cplx<T>* tj = tabs_of<T>(a.s, stride, j);
-    // rows 0-5: j's root rows (t, o); 6-11: i's root rows
+    // window: (tau_j - omega_j) against (tau_i - omega_i), four separable terms
+    double part = 0.0;
     {
-        constexpr int R = (12 * kRowStride + TB - 1) / TB;
-        Cplx<T> v[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int q = threadIdx.x + r * TB;
-            if (q < 12 * kRowStride) v[r] = q < 6 * kRowStride ? tj[q] : ta[q - 6 * kRowStride];
-        }
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int q = threadIdx.x + r * TB;
-            if (q < 12 * kRowStride) rows[q] = v[r];
-        }
+        const int dw = t.dw, nzw = t.n_window + 1;
+        for (int ej = 0; ej < 2; ++ej)
+            for (int ei = 0; ei < 2; ++ei) {
+                const double sg = (ej == ei) ? 1.0 : -1.0;
+                part += sg * separable_dot(t.Wmat, dw, nzw, tj + root_row(ej, 0), tj + root_row(ej, 1),
+                                           tj + root_row(ej, 2), ta + root_row(ei, 0), ta + root_row(ei, 1),
+                                           ta + root_row(ei, 2), A, B, C);
+            }
+        part *= 2.0 * t.cW;
     }
-    __syncthreads();
-    double total = 0;
-    {
-        const int dw = t.dw;
-        const Cplx<T>* R = rows;
-        double part = 0;
-#pragma unroll 4
-        for (int e = threadIdx.x; e < t.Mw; e += blockDim.x) {
-            const Cplx<T> x1 = phase_at(R, R + kRowStride, R + 2 * kRowStride, dw, e);
-            const Cplx<T> x2 = phase_at(R + 3 * kRowStride, R + 4 * kRowStride, R + 5 * kRowStride, dw, e);
-            const Cplx<T> y1 = phase_at(R + 6 * kRowStride, R + 7 * kRowStride, R + 8 * kRowStride, dw, e);
-            const Cplx<T> y2 = phase_at(R + 9 * kRowStride, R + 10 * kRowStride, R + 11 * kRowStride, dw, e);
-            const T ar = x1.x - x2.x, ai = x1.y - x2.y, br = y1.x - y2.x, bi = y1.y - y2.y;
-            part += (double)((ar * br + ai * bi) * t.Wmat[e]);
-        }
-        total += block_sum(part, red) * 2.0 * t.cW;
-    }
+    // difference levels of j's paths next to i's boxes
     for (int ep = 0; ep < 2; ++ep) {
         const int lam = ep == 0 ? Bj.lam_t : Bj.lam_o;
         for (int l = 2; l <= lam - 1; ++l) {
@@ -701,58 +722,29 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
             int sh[2][3] = {{0, 0, 0}, {0, 0, 0}};
             bool on[2];
             for (int w = 0; w < 2; ++w) {  // w = 0: i's target (+), 1: i's origin (-)
-                const int lami = w == 0 ? A.lam_t : A.lam_o;
+                const int lami = w == 0 ? A_.lam_t : A_.lam_o;
                 on[w] = lami >= l;
                 if (!on[w]) continue;
-                const int bi = w == 0 ? A.path_t[l] : A.path_o[l];
+                const int bi = w == 0 ? A_.path_t[l] : A_.path_o[l];
                 for (int ax = 0; ax < 3 && on[w]; ++ax) {
                     sh[w][ax] = image_shift(t.cell[3 * bi + ax], t.cell[3 * node + ax], nl);
                     if (sh[w][ax] == 2) on[w] = false;
                 }
             }
-            if (!on[0] && !on[1]) continue;  // uniform across the block
-            // rows 12-14: j's rows at this level; 15-17 / 18-20: i's shifted target / origin rows
-            {
-                constexpr int R = (9 * kRowStride + TB - 1) / TB;
-                Cplx<T> vv[R];
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const int q = threadIdx.x + r * TB;
-                    const int rr = q / kRowStride, mm = q % kRowStride, ax = rr % 3;
-                    Cplx<T> v = {T(0), T(0)};
-                    if (q < 9 * kRowStride) {
-                        if (rr < 3)
-                            v = tj[level_row(nL, ep, li, ax, 0) + mm];
-                        else if (rr < 6 && on[0])
-                            v = ta[level_row(nL, 0, li, ax, sh[0][ax]) + mm];
-                        else if (rr >= 6 && on[1])
-                            v = ta[level_row(nL, 1, li, ax, sh[1][ax]) + mm];
-                    }
-                    vv[r] = v;
-                }
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const int q = threadIdx.x + r * TB;
-                    if (q < 9 * kRowStride) rows[12 * kRowStride + q] = vv[r];
-                }
-            }
-            __syncthreads();
             const T* Dl = t.Dmat + (size_t)li * t.Md;
-            const int dd = t.dd;
-            const Cplx<T>* R = rows + 12 * kRowStride;
-            double part = 0;
-#pragma unroll 4
-            for (int e = threadIdx.x; e < t.Md; e += blockDim.x) {
-                const Cplx<T> w = phase_at(R, R + kRowStride, R + 2 * kRowStride, dd, e);
-                // delta in = conj(tau_i s) - conj(omega_i s'); absent endpoints have zero rows
-                const Cplx<T> v0 = phase_at(R + 3 * kRowStride, R + 4 * kRowStride, R + 5 * kRowStride, dd, e);
-                const Cplx<T> v1 = phase_at(R + 6 * kRowStride, R + 7 * kRowStride, R + 8 * kRowStride, dd, e);
-                const T dr = v0.x - v1.x, di = -(v0.y - v1.y);
-                part += (double)((w.x * dr - w.y * di) * Dl[e]);
+            double lv = 0.0;
+            for (int w = 0; w < 2; ++w) {
+                if (!on[w]) continue;  // uniform across the block
+                lv += (w == 0 ? 1.0 : -1.0) *
+                      separable_dot(Dl, t.dd, t.n_diff + 1, tj + level_row(nL, ep, li, 0, 0),
+                                    tj + level_row(nL, ep, li, 1, 0), tj + level_row(nL, ep, li, 2, 0),
+                                    ta + level_row(nL, w, li, 0, sh[w][0]), ta + level_row(nL, w, li, 1, sh[w][1]),
+                                    ta + level_row(nL, w, li, 2, sh[w][2]), A, B, C);
             }
-            total += block_sum(part, red) * 2.0 * t.cD[l] * (ep == 0 ? 1.0 : -1.0);
+            part += lv * 2.0 * t.cD[l] * (ep == 0 ? 1.0 : -1.0);
         }
     }
+    double total = block_sum(part, red);
     if (threadIdx.x == 0) {
         const T eps = std::numeric_limits<T>::epsilon();
         int flags = 0;  // hard spheres: 1 t_j within sigma of o_i (counted, now gone), 2 of t_i (new)
@@ -763,7 +755,7 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
             const T b2 = (T)(bb * bb), inv_b = (T)(1.0 / bb);
             double s = 0;
             for (int w = 0; w < 2; ++w) {
-                const double* y = w == 0 ? A.t : A.o;
+                const double* y = w == 0 ? A_.t : A_.o;
                 double dx = xj[0] - y[0], dy = xj[1] - y[1], dz = xj[2] - y[2];
                 dx -= t.L * rint(dx / t.L);
                 dy -= t.L * rint(dy / t.L);
@@ -774,12 +766,12 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
                     s += (w == 0 ? 1.0 : -1.0) * (double)residual_poly_over_r<T>(sqrt(r2), inv_b, t.poly, t.n_poly);
                 if (ep == 0 && dx * dx + dy * dy + dz * dz < a.sigma2) flags |= w == 0 ? 2 : 1;
             }
-            total += (ep == 0 ? 1.0 : -1.0) * s * A.q * Bj.q;
+            total += (ep == 0 ? 1.0 : -1.0) * s * A_.q * Bj.q;
         }
         if (a.prop.learned) {  // j's proposal is stale once i is accepted: bit 4
             const double rf2 = a.prop.rf * a.prop.rf;
-            if (A.i == Bj.i || within(A.o, Bj.o, t.L, rf2) || within(A.t, Bj.o, t.L, rf2) ||
-                within(A.o, Bj.t, t.L, rf2) || within(A.t, Bj.t, t.L, rf2))
+            if (A_.i == Bj.i || within(A_.o, Bj.o, t.L, rf2) || within(A_.t, Bj.o, t.L, rf2) ||
+                within(A_.o, Bj.t, t.L, rf2) || within(A_.t, Bj.t, t.L, rf2))
                 flags |= 4;
         }
         a.s.corr[i * K + j] = total;
