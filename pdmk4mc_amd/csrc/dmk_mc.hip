@@ -672,10 +672,20 @@ __device__ double separable_dot(const T* W, int d, int nz, const Cplx<T>* ax, co
         const int y = rr % d, z = rr / d;
         const T* w = W + (size_t)d * y + (size_t)d * d * z;
         T sr = 0, si = 0;
-        for (int x = 0; x < d; ++x) {
-            const T wx = w[x];
-            sr += wx * A[x].x;
-            si += wx * A[x].y;
+        int x = 0;
+        for (; x + 8 <= d; x += 8) {  // eight independent loads in flight
+            T wv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) wv[u] = w[x + u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                sr += wv[u] * A[x + u].x;
+                si += wv[u] * A[x + u].y;
+            }
+        }
+        for (; x < d; ++x) {
+            sr += w[x] * A[x].x;
+            si += w[x] * A[x].y;
         }
         const Cplx<T> bc = cmul(B[y], C[z]);
         acc += (double)(bc.x * sr - bc.y * si);
