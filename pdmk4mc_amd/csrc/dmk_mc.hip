@@ -1002,13 +1002,17 @@ __global__ void k_advance(int* move_ctr, int n) {
 }
 
 // ------------------------------------------------------------------ apply
-// Work items: root chunks, then (rewritten array, element chunk).  For its
-// array a block lists the (move, endpoint, image) contributions in move order,
-// stages their phase rows in LDS a tile at a time, and every thread adds them
-// to its element, which it reads and writes once.
-constexpr int kApplyTile = 16;
-constexpr int kApplyPer = 8;  // elements per thread and work item
-constexpr int kApplyLv = 4;   // levels of the per-block path table (deeper levels read TrialInfo)
+// Work items: (array, block of grid rows) for the root and every rewritten
+// array; a grid row is the d elements sharing (k_y, k_z).  For its array a
+// block lists the (move, endpoint, image) contributions in move order and
+// stages their phase rows in LDS a tile at a time.  Each update is separable,
+// s X[x] Y[y] Z[z]: the lanes of a row form s Y[y] Z[z] once per move and add
+// X[x] times it to their share of the row's x; the sums go through LDS so the
+// read-modify-write of the array is coalesced, every element read and
+// written once.
+constexpr int kApplyTile = 8;
+constexpr int kApplyLv = 4;    // levels of the per-block path table (deeper levels read TrialInfo)
+constexpr int kMaxGridD = 53;  // largest 2n+1 of any digits setting
 
 template <typename T>
 __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a) {
@@ -1022,8 +1026,12 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
     const int nD = a.s.plan[0];
     if (!a.s.plan[1]) return;
     const int K = a.K, D = t.max_depth, nL = n_levels_pw(D), stride = table_stride(D);
-    constexpr int CH = TB * kApplyPer;
-    const int nWb = (t.Mw + CH - 1) / CH, nDb = (t.Md + CH - 1) / CH;
+    constexpr int LPR = sizeof(T) == 8 ? 4 : 2;  // lanes per grid row
+    constexpr int RPI = TB / LPR;                // grid rows per work item
+    constexpr int XMAX = (kMaxGridD + LPR - 1) / LPR;
+    __shared__ Cplx<T> S[RPI * kMaxGridD];       // the item's sums, row-major like the array
+    const int rows_w = t.dw * (t.n_window + 1), rows_d = t.dd * (t.n_diff + 1);
+    const int nWb = (rows_w + RPI - 1) / RPI, nDb = (rows_d + RPI - 1) / RPI;
     const int total = nWb + nD * nDb;
     if ((int)blockIdx.x >= total) return;  // no work item for this block
     for (int m = threadIdx.x; m < K; m += blockDim.x) ac_s[m] = a.s.accept[m];
@@ -1101,17 +1109,15 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
         const int nm = mcount;
         if (threadIdx.x == 0) probe_add(a.s, kPrApplyMatch, tp);
         const int d = is_root ? t.dw : t.dd;
-        const int M = is_root ? t.Mw : t.Md;
-        int idx[kApplyPer];  // packed i | j << 8 | k << 16 of each element, -1 beyond the array
-        T ar[kApplyPer], ai[kApplyPer];
+        const int nrows = is_root ? rows_w : rows_d;
+        const int part = threadIdx.x % LPR, lr = threadIdx.x / LPR, row = chunk * RPI + lr;
+        const bool on = row < nrows;
+        const int y = on ? row % d : 0, z = on ? row / d : 0;
+        const int hx = (d + LPR - 1) / LPR, x0 = part * hx;
+        const int nx = on ? max(0, min(hx, d - x0)) : 0;
+        T ar[XMAX], ai[XMAX];
 #pragma unroll
-        for (int q = 0; q < kApplyPer; ++q) {
-            const int e = chunk * CH + q * TB + threadIdx.x;
-            const int er = e / d;
-            idx[q] = e < M ? ((e % d) | ((er % d) << 8) | ((er / d) << 16)) : -1;
-            ar[q] = 0;
-            ai[q] = 0;
-        }
+        for (int x = 0; x < XMAX; ++x) ar[x] = ai[x] = T(0);
         for (int k0 = 0; k0 < nm; k0 += kApplyTile) {
             const int nt = nm - k0 < kApplyTile ? nm - k0 : kApplyTile;
             __syncthreads();
@@ -1138,33 +1144,37 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
                 }
             }
             __syncthreads();
-            for (int k = 0; k < nt; ++k) {
-                const Cplx<T>* R = rows + k * 3 * kRowStride;
-                const T sg = ((mlist[k0 + k] >> 7) & 1) ? T(-1) : T(1);
+            if (nx > 0)
+                for (int k = 0; k < nt; ++k) {
+                    const Cplx<T>* Rw = rows + k * 3 * kRowStride;
+                    const T sg = ((mlist[k0 + k] >> 7) & 1) ? T(-1) : T(1);
+                    Cplx<T> yz = cmul(Rw[kRowStride + y], Rw[2 * kRowStride + z]);
+                    yz.x *= sg;
+                    yz.y *= sg;
 #pragma unroll
-                for (int q = 0; q < kApplyPer; ++q) {
-                    if (idx[q] < 0) continue;
-                    const int ii = idx[q] & 255, jj = (idx[q] >> 8) & 255, kk = idx[q] >> 16;
-                    const Cplx<T> v = cmul(cmul(R[ii], R[kRowStride + jj]), R[2 * kRowStride + kk]);
-                    ar[q] += sg * v.x;
-                    ai[q] += sg * v.y;
+                    for (int x = 0; x < XMAX; ++x) {
+                        if (x < nx) {
+                            const Cplx<T> v = cmul(Rw[x0 + x], yz);
+                            ar[x] += v.x;
+                            ai[x] += v.y;
+                        }
+                    }
                 }
-            }
         }
         if (threadIdx.x == 0) probe_add(a.s, kPrApplyTiles, tp);
-        // read-modify-write: every load of the item first, then every store
+        // sums to LDS in the array's order, then a coalesced read-modify-write
+        __syncthreads();
+#pragma unroll
+        for (int x = 0; x < XMAX; ++x)
+            if (x < nx) S[lr * d + x0 + x] = {ar[x], ai[x]};
+        __syncthreads();
         Cplx<T>* dst = is_root ? t.root : (kind == 0 ? t.out : t.inc) + (size_t)slot * t.Md;
         const T sy = (is_root || kind == 0) ? T(1) : T(-1);  // in[C] += conj(tau s)
-        Cplx<T> cur[kApplyPer];
-#pragma unroll
-        for (int q = 0; q < kApplyPer; ++q) {
-            const int e = chunk * CH + q * TB + threadIdx.x;
-            if (e < M) cur[q] = dst[e];
-        }
-#pragma unroll
-        for (int q = 0; q < kApplyPer; ++q) {
-            const int e = chunk * CH + q * TB + threadIdx.x;
-            if (e < M) dst[e] = {cur[q].x + ar[q], cur[q].y + sy * ai[q]};
+        const int e0 = chunk * RPI * d;
+        const int ne = min(RPI, nrows - chunk * RPI) * d;
+        for (int e = threadIdx.x; e < ne; e += TB) {
+            const Cplx<T> c = dst[e0 + e], v = S[e];
+            dst[e0 + e] = {c.x + v.x, c.y + sy * v.y};
         }
         if (threadIdx.x == 0) probe_add(a.s, kPrApplyRmw, tp);
     }
