@@ -136,6 +136,7 @@ struct ReplicaLaunch {
     const int* const* leaves; // n leaf lists
     const int* n_leaves;      // n
     int n, max_depth, max_leaves, apply_blocks;
+    int dmax;     // largest grid points per axis over the replicas (shared-memory sizing)
     Tiles tiles;  // trial shape (the same for every replica)
 };
 

@@ -1014,22 +1014,35 @@ constexpr int kApplyTile = 8;
 constexpr int kApplyLv = 4;    // levels of the per-block path table (deeper levels read TrialInfo)
 constexpr int kMaxGridD = 53;  // largest 2n+1 of any digits setting
 
+// lanes per grid row, and the x values one lane holds for grids up to d
 template <typename T>
+__host__ __device__ constexpr int apply_lpr() { return sizeof(T) == 8 ? 4 : 2; }
+template <typename T>
+__host__ __device__ constexpr int apply_xmax(int d) { return (d + apply_lpr<T>() - 1) / apply_lpr<T>(); }
+// dynamic shared memory of k_apply: the item's sums and one tile of rows,
+// both at row stride dmax = max(d_w, d)
+template <typename T>
+__host__ __device__ inline size_t apply_shmem(int dmax) {
+    return sizeof(Cplx<T>) * ((size_t)(TB / apply_lpr<T>()) * dmax + (size_t)kApplyTile * 3 * dmax);
+}
+
+template <typename T, int XMAX>
 __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a) {
+    extern __shared__ char apply_dyn[];
     __shared__ int mlist[2 * kMaxBatch];
     __shared__ int wtot[TB / 64];
     __shared__ int mcount;
-    __shared__ Cplx<T> rows[kApplyTile * 3 * kRowStride];
     __shared__ int ac_s[kMaxBatch];
     __shared__ int pnode[2 * kMaxBatch * kApplyLv];  // (move, endpoint, level-2) -> box, -1 if none
     if (*t.err) return;
     const int nD = a.s.plan[0];
     if (!a.s.plan[1]) return;
     const int K = a.K, D = t.max_depth, nL = n_levels_pw(D), stride = table_stride(D);
-    constexpr int LPR = sizeof(T) == 8 ? 4 : 2;  // lanes per grid row
-    constexpr int RPI = TB / LPR;                // grid rows per work item
-    constexpr int XMAX = (kMaxGridD + LPR - 1) / LPR;
-    __shared__ Cplx<T> S[RPI * kMaxGridD];       // the item's sums, row-major like the array
+    constexpr int LPR = apply_lpr<T>();  // lanes per grid row
+    constexpr int RPI = TB / LPR;        // grid rows per work item
+    const int dmax = t.dw > t.dd ? t.dw : t.dd;
+    Cplx<T>* S = reinterpret_cast<Cplx<T>*>(apply_dyn);  // the item's sums, row-major like the array
+    Cplx<T>* rows = S + RPI * dmax;                       // tile rows at stride dmax
     const int rows_w = t.dw * (t.n_window + 1), rows_d = t.dd * (t.n_diff + 1);
     const int nWb = (rows_w + RPI - 1) / RPI, nDb = (rows_d + RPI - 1) / RPI;
     const int total = nWb + nD * nDb;
@@ -1123,13 +1136,14 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
             __syncthreads();
             {
                 // all loads of the tile are issued before the LDS stores
-                constexpr int R = (kApplyTile * 3 * kRowStride + TB - 1) / TB;
+                constexpr int R = (kApplyTile * 3 * kMaxGridD + TB - 1) / TB;
                 Cplx<T> v[R];
+                const int nq = nt * 3 * d;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const int q = threadIdx.x + r * TB;
-                    if (q < nt * 3 * kRowStride) {
-                        const int k = q / (3 * kRowStride), ax = (q / kRowStride) % 3, mm = q % kRowStride;
+                    if (q < nq) {
+                        const int k = q / (3 * d), ax = (q / d) % 3, mm = q % d;
                         const int c = mlist[k0 + k];
                         const int m = c >> 8, ep = (c >> 7) & 1, code = c & 127;
                         const int sh = ax == 0 ? code % 3 - 1 : (ax == 1 ? (code / 3) % 3 - 1 : code / 9 - 1);
@@ -1140,15 +1154,15 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const int q = threadIdx.x + r * TB;
-                    if (q < nt * 3 * kRowStride) rows[q] = v[r];
+                    if (q < nq) rows[(q / d) * dmax + q % d] = v[r];
                 }
             }
             __syncthreads();
             if (nx > 0)
                 for (int k = 0; k < nt; ++k) {
-                    const Cplx<T>* Rw = rows + k * 3 * kRowStride;
+                    const Cplx<T>* Rw = rows + k * 3 * dmax;
                     const T sg = ((mlist[k0 + k] >> 7) & 1) ? T(-1) : T(1);
-                    Cplx<T> yz = cmul(Rw[kRowStride + y], Rw[2 * kRowStride + z]);
+                    Cplx<T> yz = cmul(Rw[dmax + y], Rw[2 * dmax + z]);
                     yz.x *= sg;
                     yz.y *= sg;
 #pragma unroll
@@ -1180,19 +1194,30 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
     }
 }
 
-template <typename T>
+template <typename T, int XMAX>
 __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
-    apply_body(t, a);
+    apply_body<T, XMAX>(t, a);
 }
 
-template <typename T>
+template <typename T, int XMAX>
 __global__ void __launch_bounds__(TB) k_apply_rep(const DevTree<T>* __restrict__ ts, const McArgs* __restrict__ as) {
     const DevTree<T>& t = ts[blockIdx.z];
     const McArgs& a = as[blockIdx.z];
-    apply_body(t, a);
+    apply_body<T, XMAX>(t, a);
 }
 
+// grids up to 26 (float) or 28 (double) points per axis use the narrow variant
+template <typename T>
+constexpr int kApplyXSmall = apply_xmax<T>(sizeof(T) == 8 ? 28 : 26);
+template <typename T>
+constexpr int kApplyXLarge = apply_xmax<T>(kMaxGridD);
+
 // ------------------------------------------------------------------ launchers
+// dynamic shared memory above the default 64 KB must be allowed per kernel
+inline void allow_shmem(const void* kernel, size_t bytes) {
+    if (bytes > 65536) (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
 // e0/e1 (may be null) take the kernel's own start and end timestamps
 template <typename T>
 void launch_propose(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
@@ -1232,7 +1257,15 @@ void launch_scan(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_
 
 template <typename T>
 void launch_apply(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
-    hipExtLaunchKernelGGL(k_apply<T>, dim3(a.tiles.apply), dim3(TB), 0, st, e0, e1, 0, t, a);
+    const int dmax = t.dw > t.dd ? t.dw : t.dd;
+    const size_t sh = apply_shmem<T>(dmax);
+    if (apply_xmax<T>(dmax) <= kApplyXSmall<T>) {
+        allow_shmem((const void*)k_apply<T, kApplyXSmall<T>>, sh);
+        hipExtLaunchKernelGGL((k_apply<T, kApplyXSmall<T>>), dim3(a.tiles.apply), dim3(TB), sh, st, e0, e1, 0, t, a);
+    } else {
+        allow_shmem((const void*)k_apply<T, kApplyXLarge<T>>, sh);
+        hipExtLaunchKernelGGL((k_apply<T, kApplyXLarge<T>>), dim3(a.tiles.apply), dim3(TB), sh, st, e0, e1, 0, t, a);
+    }
 }
 
 template <typename T>
@@ -1247,7 +1280,17 @@ void launch_replica_step(const ReplicaLaunch<T>& r, hipStream_t st) {
     hipLaunchKernelGGL(k_trial_rep<T>, dim3(trial_blocks(r.max_depth, r.tiles), 1, r.n), dim3(TB), 0, st, r.trees,
                        r.args);
     hipLaunchKernelGGL(k_decide_rep<T>, rep, dim3(TB), 0, st, r.trees, r.args);
-    hipLaunchKernelGGL(k_apply_rep<T>, dim3(r.apply_blocks, 1, r.n), dim3(TB), 0, st, r.trees, r.args);
+    // the replicas share digits: one grid size (the largest) sizes the launch
+    const size_t sh = apply_shmem<T>(r.dmax);
+    if (apply_xmax<T>(r.dmax) <= kApplyXSmall<T>) {
+        allow_shmem((const void*)k_apply_rep<T, kApplyXSmall<T>>, sh);
+        hipLaunchKernelGGL((k_apply_rep<T, kApplyXSmall<T>>), dim3(r.apply_blocks, 1, r.n), dim3(TB), sh, st, r.trees,
+                           r.args);
+    } else {
+        allow_shmem((const void*)k_apply_rep<T, kApplyXLarge<T>>, sh);
+        hipLaunchKernelGGL((k_apply_rep<T, kApplyXLarge<T>>), dim3(r.apply_blocks, 1, r.n), dim3(TB), sh, st, r.trees,
+                           r.args);
+    }
     hipLaunchKernelGGL(k_advance, dim3((r.n + TB - 1) / TB), dim3(TB), 0, st, r.move_ctr, r.n);
 }
 

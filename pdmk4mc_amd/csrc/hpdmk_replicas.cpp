@@ -62,6 +62,11 @@ ReplicaLaunch<T> launch_args(ReplicaSet* s) {
     r.max_depth = s->max_depth;
     r.max_leaves = s->max_leaves;
     r.apply_blocks = s->apply_blocks;
+    r.dmax = 0;
+    for (auto& h : s->reps) {
+        const DevTree<T>& d = dev<T>(h.get());
+        r.dmax = std::max(r.dmax, std::max(d.dw, d.dd));
+    }
     r.tiles = Tiles{};
     return r;
 }
