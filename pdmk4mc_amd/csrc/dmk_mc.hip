@@ -1019,11 +1019,13 @@ template <typename T>
 __host__ __device__ constexpr int apply_lpr() { return sizeof(T) == 8 ? 4 : 2; }
 template <typename T>
 __host__ __device__ constexpr int apply_xmax(int d) { return (d + apply_lpr<T>() - 1) / apply_lpr<T>(); }
-// dynamic shared memory of k_apply: the item's sums and one tile of rows,
-// both at row stride dmax = max(d_w, d)
+// dynamic shared memory of k_apply: the item's sums and two tiles of rows
+// (double buffer), at row stride dmax = max(d_w, d), then the path table of
+// the batch's K moves
 template <typename T>
-__host__ __device__ inline size_t apply_shmem(int dmax) {
-    return sizeof(Cplx<T>) * ((size_t)(TB / apply_lpr<T>()) * dmax + (size_t)kApplyTile * 3 * dmax);
+__host__ __device__ inline size_t apply_shmem(int dmax, int K) {
+    return sizeof(Cplx<T>) * ((size_t)(TB / apply_lpr<T>()) * dmax + 2 * (size_t)kApplyTile * 3 * dmax) +
+           sizeof(int) * 2 * (size_t)K * kApplyLv;
 }
 
 template <typename T, int XMAX>
@@ -1033,7 +1035,6 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
     __shared__ int wtot[TB / 64];
     __shared__ int mcount;
     __shared__ int ac_s[kMaxBatch];
-    __shared__ int pnode[2 * kMaxBatch * kApplyLv];  // (move, endpoint, level-2) -> box, -1 if none
     if (*t.err) return;
     const int nD = a.s.plan[0];
     if (!a.s.plan[1]) return;
@@ -1042,7 +1043,8 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
     constexpr int RPI = TB / LPR;        // grid rows per work item
     const int dmax = t.dw > t.dd ? t.dw : t.dd;
     Cplx<T>* S = reinterpret_cast<Cplx<T>*>(apply_dyn);  // the item's sums, row-major like the array
-    Cplx<T>* rows = S + RPI * dmax;                       // tile rows at stride dmax
+    Cplx<T>* rows2 = S + RPI * dmax;                      // two tiles of rows at stride dmax
+    int* pnode = reinterpret_cast<int*>(rows2 + 2 * kApplyTile * 3 * dmax);  // (move, endpoint, level-2) -> box
     const int rows_w = t.dw * (t.n_window + 1), rows_d = t.dd * (t.n_diff + 1);
     const int nWb = (rows_w + RPI - 1) / RPI, nDb = (rows_d + RPI - 1) / RPI;
     const int total = nWb + nD * nDb;
@@ -1131,33 +1133,46 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
         T ar[XMAX], ai[XMAX];
 #pragma unroll
         for (int x = 0; x < XMAX; ++x) ar[x] = ai[x] = T(0);
-        for (int k0 = 0; k0 < nm; k0 += kApplyTile) {
+        // Row tiles of kApplyTile contributions, double-buffered: the global
+        // loads of tile n+1 are in flight while tile n is accumulated.
+        constexpr int R = (kApplyTile * 3 * kMaxGridD + TB - 1) / TB;
+        Cplx<T> v[R];
+        auto load_tile = [&](int k0) {
             const int nt = nm - k0 < kApplyTile ? nm - k0 : kApplyTile;
-            __syncthreads();
-            {
-                // all loads of the tile are issued before the LDS stores
-                constexpr int R = (kApplyTile * 3 * kMaxGridD + TB - 1) / TB;
-                Cplx<T> v[R];
-                const int nq = nt * 3 * d;
+            const int nq = nt * 3 * d;
 #pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const int q = threadIdx.x + r * TB;
-                    if (q < nq) {
-                        const int k = q / (3 * d), ax = (q / d) % 3, mm = q % d;
-                        const int c = mlist[k0 + k];
-                        const int m = c >> 8, ep = (c >> 7) & 1, code = c & 127;
-                        const int sh = ax == 0 ? code % 3 - 1 : (ax == 1 ? (code / 3) % 3 - 1 : code / 9 - 1);
-                        const Cplx<T>* tb = tabs_of<T>(a.s, stride, m);
-                        v[r] = tb[(is_root ? root_row(ep, ax) : level_row(nL, ep, li, ax, sh)) + mm];
-                    }
-                }
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const int q = threadIdx.x + r * TB;
-                    if (q < nq) rows[(q / d) * dmax + q % d] = v[r];
+            for (int r = 0; r < R; ++r) {
+                const int q = threadIdx.x + r * TB;
+                if (q < nq) {
+                    const int k = q / (3 * d), ax = (q / d) % 3, mm = q % d;
+                    const int c = mlist[k0 + k];
+                    const int m = c >> 8, ep = (c >> 7) & 1, code = c & 127;
+                    const int sh = ax == 0 ? code % 3 - 1 : (ax == 1 ? (code / 3) % 3 - 1 : code / 9 - 1);
+                    const Cplx<T>* tb = tabs_of<T>(a.s, stride, m);
+                    v[r] = tb[(is_root ? root_row(ep, ax) : level_row(nL, ep, li, ax, sh)) + mm];
                 }
             }
-            __syncthreads();
+        };
+        auto store_tile = [&](int k0, Cplx<T>* dstrows) {
+            const int nt = nm - k0 < kApplyTile ? nm - k0 : kApplyTile;
+            const int nq = nt * 3 * d;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int q = threadIdx.x + r * TB;
+                if (q < nq) dstrows[(q / d) * dmax + q % d] = v[r];
+            }
+        };
+        if (nm > 0) {
+            load_tile(0);
+            store_tile(0, rows2);
+        }
+        int buf = 0;
+        for (int k0 = 0; k0 < nm; k0 += kApplyTile) {
+            const int nt = nm - k0 < kApplyTile ? nm - k0 : kApplyTile;
+            const bool more = k0 + kApplyTile < nm;
+            if (more) load_tile(k0 + kApplyTile);  // in flight during the accumulation
+            __syncthreads();                        // tile k0 stored by every thread
+            const Cplx<T>* rows = rows2 + buf * kApplyTile * 3 * dmax;
             if (nx > 0)
                 for (int k = 0; k < nt; ++k) {
                     const Cplx<T>* Rw = rows + k * 3 * dmax;
@@ -1168,12 +1183,14 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
 #pragma unroll
                     for (int x = 0; x < XMAX; ++x) {
                         if (x < nx) {
-                            const Cplx<T> v = cmul(Rw[x0 + x], yz);
-                            ar[x] += v.x;
-                            ai[x] += v.y;
+                            const Cplx<T> w = cmul(Rw[x0 + x], yz);
+                            ar[x] += w.x;
+                            ai[x] += w.y;
                         }
                     }
                 }
+            if (more) store_tile(k0 + kApplyTile, rows2 + (buf ^ 1) * kApplyTile * 3 * dmax);
+            buf ^= 1;
         }
         if (threadIdx.x == 0) probe_add(a.s, kPrApplyTiles, tp);
         // sums to LDS in the array's order, then a coalesced read-modify-write
@@ -1258,7 +1275,7 @@ void launch_scan(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_
 template <typename T>
 void launch_apply(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     const int dmax = t.dw > t.dd ? t.dw : t.dd;
-    const size_t sh = apply_shmem<T>(dmax);
+    const size_t sh = apply_shmem<T>(dmax, a.K);
     if (apply_xmax<T>(dmax) <= kApplyXSmall<T>) {
         allow_shmem((const void*)k_apply<T, kApplyXSmall<T>>, sh);
         hipExtLaunchKernelGGL((k_apply<T, kApplyXSmall<T>>), dim3(a.tiles.apply), dim3(TB), sh, st, e0, e1, 0, t, a);
@@ -1281,7 +1298,7 @@ void launch_replica_step(const ReplicaLaunch<T>& r, hipStream_t st) {
                        r.args);
     hipLaunchKernelGGL(k_decide_rep<T>, rep, dim3(TB), 0, st, r.trees, r.args);
     // the replicas share digits: one grid size (the largest) sizes the launch
-    const size_t sh = apply_shmem<T>(r.dmax);
+    const size_t sh = apply_shmem<T>(r.dmax, 1);
     if (apply_xmax<T>(r.dmax) <= kApplyXSmall<T>) {
         allow_shmem((const void*)k_apply_rep<T, kApplyXSmall<T>>, sh);
         hipLaunchKernelGGL((k_apply_rep<T, kApplyXSmall<T>>), dim3(r.apply_blocks, 1, r.n), dim3(TB), sh, st, r.trees,
