@@ -1011,7 +1011,9 @@ __global__ void k_advance(int* move_ctr, int n) {
 // read-modify-write of the array is coalesced, every element read and
 // written once.
 constexpr int kApplyTile = 8;
-constexpr int kApplyLv = 4;    // levels of the per-block path table (deeper levels read TrialInfo)
+constexpr int kApplyLv = 4;     // levels of the per-block path table (deeper levels read TrialInfo)
+constexpr int kApplySplit = 4;  // an array with more than kApplyHeavy contributions is shared by
+constexpr int kApplyHeavy = 16; // kApplySplit items that add their partial sums atomically
 constexpr int kMaxGridD = 53;  // largest 2n+1 of any digits setting
 
 // lanes per grid row, and the x values one lane holds for grids up to d
@@ -1047,7 +1049,7 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
     int* pnode = reinterpret_cast<int*>(rows2 + 2 * kApplyTile * 3 * dmax);  // (move, endpoint, level-2) -> box
     const int rows_w = t.dw * (t.n_window + 1), rows_d = t.dd * (t.n_diff + 1);
     const int nWb = (rows_w + RPI - 1) / RPI, nDb = (rows_d + RPI - 1) / RPI;
-    const int total = nWb + nD * nDb;
+    const int total = (nWb + nD * nDb) * kApplySplit;
     if ((int)blockIdx.x >= total) return;  // no work item for this block
     for (int m = threadIdx.x; m < K; m += blockDim.x) ac_s[m] = a.s.accept[m];
     for (int w = threadIdx.x; w < 2 * K * kApplyLv; w += blockDim.x) {
@@ -1058,7 +1060,8 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
         pnode[w] = (l <= lam && l <= D - 2) ? (ep == 0 ? ti.path_t[l] : ti.path_o[l]) : -1;
     }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    for (int item2 = blockIdx.x; item2 < total; item2 += gridDim.x) {
+        const int item = item2 / kApplySplit, part_c = item2 % kApplySplit;  // part of the contributions
         const bool is_root = item < nWb;
         int slot = 0, kind = 2, node = 0, l = 0, li = 0, chunk;
         if (is_root) {
@@ -1121,7 +1124,14 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
                 for (int k = 0; k < (int)(blockDim.x >> 6); ++k) mcount += wtot[k];
             __syncthreads();
         }
-        const int nm = mcount;
+        const int nm_all = mcount;
+        // heavy arrays: this item takes an even share of the contributions
+        const bool heavy = nm_all > kApplyHeavy;
+        if (!heavy && part_c > 0) continue;  // uniform across the block
+        const int cbeg = heavy ? part_c * nm_all / kApplySplit : 0;
+        const int cend = heavy ? (part_c + 1) * nm_all / kApplySplit : nm_all;
+        const int nm = cend - cbeg;
+        const int* mlc = mlist + cbeg;
         if (threadIdx.x == 0) probe_add(a.s, kPrApplyMatch, tp);
         const int d = is_root ? t.dw : t.dd;
         const int nrows = is_root ? rows_w : rows_d;
@@ -1145,7 +1155,7 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
                 const int q = threadIdx.x + r * TB;
                 if (q < nq) {
                     const int k = q / (3 * d), ax = (q / d) % 3, mm = q % d;
-                    const int c = mlist[k0 + k];
+                    const int c = mlc[k0 + k];
                     const int m = c >> 8, ep = (c >> 7) & 1, code = c & 127;
                     const int sh = ax == 0 ? code % 3 - 1 : (ax == 1 ? (code / 3) % 3 - 1 : code / 9 - 1);
                     const Cplx<T>* tb = tabs_of<T>(a.s, stride, m);
@@ -1176,7 +1186,7 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
             if (nx > 0)
                 for (int k = 0; k < nt; ++k) {
                     const Cplx<T>* Rw = rows + k * 3 * dmax;
-                    const T sg = ((mlist[k0 + k] >> 7) & 1) ? T(-1) : T(1);
+                    const T sg = ((mlc[k0 + k] >> 7) & 1) ? T(-1) : T(1);
                     Cplx<T> yz = cmul(Rw[dmax + y], Rw[2 * dmax + z]);
                     yz.x *= sg;
                     yz.y *= sg;
@@ -1203,9 +1213,17 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
         const T sy = (is_root || kind == 0) ? T(1) : T(-1);  // in[C] += conj(tau s)
         const int e0 = chunk * RPI * d;
         const int ne = min(RPI, nrows - chunk * RPI) * d;
-        for (int e = threadIdx.x; e < ne; e += TB) {
-            const Cplx<T> c = dst[e0 + e], v = S[e];
-            dst[e0 + e] = {c.x + v.x, c.y + sy * v.y};
+        if (heavy) {  // shares of one array from several items
+            for (int e = threadIdx.x; e < ne; e += TB) {
+                const Cplx<T> v = S[e];
+                atomicAdd(&dst[e0 + e].x, v.x);
+                atomicAdd(&dst[e0 + e].y, sy * v.y);
+            }
+        } else {
+            for (int e = threadIdx.x; e < ne; e += TB) {
+                const Cplx<T> c = dst[e0 + e], v = S[e];
+                dst[e0 + e] = {c.x + v.x, c.y + sy * v.y};
+            }
         }
         if (threadIdx.x == 0) probe_add(a.s, kPrApplyRmw, tp);
     }
