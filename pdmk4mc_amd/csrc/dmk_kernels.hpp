@@ -73,8 +73,9 @@ struct McScratch {
     double* corr;          // kMaxBatch^2
     unsigned char* ovl;    // kMaxBatch^2 hard-sphere flags of pair (i, j): 1 t_j near o_i, 2 t_j near t_i
     unsigned char* mark;   // 2 per expansion box (self-resetting)
-    int* dest;             // compact list of rewritten arrays: 2*slot + kind (0 out, 1 in)
-    int* plan;             // [0] arrays in dest, [1] any move accepted
+    int* dest;             // rewritten arrays, 2*slot + kind (0 out, 1 in): level-2 in arrays from
+                           // the front, the others from the back (2 n_pw entries)
+    int* plan;             // [0] arrays in dest, [1] accepted moves, [2] level-2 in arrays
     unsigned long long* probe;  // optional phase timers (profiling): kProbes sums + counts
 };
 

@@ -209,13 +209,14 @@ __device__ void relocate(const DevTree<T>& t, const TrialInfo& ti, int rank) {
 
 // Marks every expansion array the accepted moves rewrite (out of each path box
 // of levels 2..min(leaf, D-2), in of the non-leaf ones and of their non-leaf
-// colleagues), compacts the marks into s.dest in slot order, and relocates
-// the moved particles between leaf slots (thread 0, moves in order).  Run by
-// one whole block.
+// colleagues) and compacts the marks into s.dest in slot order: the incoming
+// arrays of level 2 (each neighbours a quarter of the level's boxes, so a
+// batch's moves pile up there) from the front, the others from the back.
+// Relocates the moved particles between leaf slots.  Run by one whole block.
 template <typename T>
 __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K) {
-    __shared__ int wtot[TB / 64];
-    __shared__ int base_sh;
+    __shared__ int wtot[TB / 64], wtot2[TB / 64];
+    __shared__ int base_sh, base2_sh;
     __shared__ int lo_s[kMaxBatch], lt_s[kMaxBatch], ac_s[kMaxBatch], conflict[kMaxBatch];
     const int D = t.max_depth, nL = n_levels_pw(D);
     unsigned long long tp = wall_clock64();
@@ -235,14 +236,16 @@ __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K) {
         if (l > (ep == 0 ? ti.lam_t : ti.lam_o)) continue;
         const int node = ep == 0 ? ti.path_t[l] : ti.path_o[l];
         const int own = t.pw_slot[node];
+        // an incoming array of level 2 is marked 2 (colleagues share the level)
+        const unsigned char in_mark = l == 2 ? 2 : 1;
         a.s.mark[2 * own] = 1;
-        if (!t.is_leaf[node]) a.s.mark[2 * own + 1] = 1;
+        if (!t.is_leaf[node]) a.s.mark[2 * own + 1] = in_mark;
         int sl[26];
 #pragma unroll
         for (int c = 0; c < 26; ++c) sl[c] = t.col_in_slot[26 * node + c];
 #pragma unroll
         for (int c = 0; c < 26; ++c)
-            if (sl[c] >= 0) a.s.mark[2 * sl[c] + 1] = 1;
+            if (sl[c] >= 0) a.s.mark[2 * sl[c] + 1] = in_mark;
     }
     // particles that stay in their leaf move in place, all at once
     for (int m = threadIdx.x; m < K; m += blockDim.x) {
@@ -257,25 +260,33 @@ __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K) {
     if (threadIdx.x == 0) probe_add(a.s, kPrPlanMark, tp);
     const int n2 = 2 * t.n_pw_nodes;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    if (threadIdx.x == 0) base_sh = 0;
+    if (threadIdx.x == 0) base_sh = base2_sh = 0;
     __syncthreads();
     for (int c0 = 0; c0 < n2; c0 += blockDim.x) {
         const int idx = c0 + threadIdx.x;
-        const bool f = idx < n2 && a.s.mark[idx];
-        const unsigned long long bal = __ballot(f);
+        const int mk = idx < n2 ? a.s.mark[idx] : 0;
+        const bool f = mk != 0, front = mk == 2, back = mk == 1;
+        const unsigned long long bf = __ballot(front), bb = __ballot(back);
         const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-        const int pre = __popcll(bal & lt);
-        if (lane == 0) wtot[wv] = __popcll(bal);
-        __syncthreads();
-        int off = base_sh;
-        for (int k = 0; k < wv; ++k) off += wtot[k];
-        if (f) {
-            a.s.dest[off + pre] = idx;
-            a.s.mark[idx] = 0;
+        if (lane == 0) {
+            wtot[wv] = __popcll(bf);
+            wtot2[wv] = __popcll(bb);
         }
         __syncthreads();
+        int off = base_sh, off2 = base2_sh;
+        for (int k = 0; k < wv; ++k) {
+            off += wtot[k];
+            off2 += wtot2[k];
+        }
+        if (front) a.s.dest[off + __popcll(bf & lt)] = idx;
+        if (back) a.s.dest[n2 - 1 - (off2 + __popcll(bb & lt))] = idx;
+        if (f) a.s.mark[idx] = 0;
+        __syncthreads();
         if (threadIdx.x == 0)
-            for (int k = 0; k < nw; ++k) base_sh += wtot[k];
+            for (int k = 0; k < nw; ++k) {
+                base_sh += wtot[k];
+                base2_sh += wtot2[k];
+            }
         __syncthreads();
     }
     if (threadIdx.x == 0) probe_add(a.s, kPrPlanCompact, tp);
@@ -297,10 +308,11 @@ __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K) {
     for (int m = threadIdx.x; m < K; m += blockDim.x)
         if (conflict[m] > 0) t.slot_count[lt_s[m]] += conflict[m];
     if (threadIdx.x != 0) return;
-    int any = 0;
-    for (int m = 0; m < K; ++m) any |= ac_s[m];
-    a.s.plan[0] = base_sh;
-    a.s.plan[1] = any;
+    int nacc = 0;
+    for (int m = 0; m < K; ++m) nacc += ac_s[m] != 0;
+    a.s.plan[0] = base_sh + base2_sh;
+    a.s.plan[1] = nacc;
+    a.s.plan[2] = base_sh;
     probe_add(a.s, kPrPlanRelocate, tp);
 }
 
@@ -1003,17 +1015,21 @@ __global__ void k_advance(int* move_ctr, int n) {
 
 // ------------------------------------------------------------------ apply
 // Work items: (array, block of grid rows) for the root and every rewritten
-// array; a grid row is the d elements sharing (k_y, k_z).  For its array a
-// block lists the (move, endpoint, image) contributions in move order and
-// stages their phase rows in LDS a tile at a time.  Each update is separable,
-// s X[x] Y[y] Z[z]: the lanes of a row form s Y[y] Z[z] once per move and add
-// X[x] times it to their share of the row's x; the sums go through LDS so the
-// read-modify-write of the array is coalesced, every element read and
-// written once.
+// array; a grid row is the d elements sharing (k_y, k_z).  An item first
+// loads its part of the array into registers (coalesced; the read is in
+// flight meanwhile), lists the (move, endpoint, image) contributions to the
+// array in move order and stages their phase rows in LDS a tile at a time.
+// Each update is separable, s X[x] Y[y] Z[z]: the lanes of a row form
+// s Y[y] Z[z] once per contribution and add X[x] times it to their share of
+// the row's x; the sums go through LDS back to the array's order, so the
+// write is coalesced too, every element read and written once.  The root and,
+// in large batches, the level-2 incoming arrays (each neighbours a quarter of
+// the level's boxes) have their contributions split over kApplySplit items,
+// whose sums are added atomically instead.
 constexpr int kApplyTile = 8;
 constexpr int kApplyLv = 4;     // levels of the per-block path table (deeper levels read TrialInfo)
-constexpr int kApplySplit = 4;  // an array with more than kApplyHeavy contributions is shared by
-constexpr int kApplyHeavy = 16; // kApplySplit items that add their partial sums atomically
+constexpr int kApplySplit = 4;  // items sharing a heavy array
+constexpr int kApplyHeavy = 16; // contributions (expected) above which an array is split
 constexpr int kMaxGridD = 53;  // largest 2n+1 of any digits setting
 
 // lanes per grid row, and the x values one lane holds for grids up to d
@@ -1021,12 +1037,15 @@ template <typename T>
 __host__ __device__ constexpr int apply_lpr() { return sizeof(T) == 8 ? 4 : 2; }
 template <typename T>
 __host__ __device__ constexpr int apply_xmax(int d) { return (d + apply_lpr<T>() - 1) / apply_lpr<T>(); }
+// the unguarded reads of a lane end at most kApplyPad elements past the tiles
+constexpr int kApplyPad = 64;
 // dynamic shared memory of k_apply: the item's sums and two tiles of rows
-// (double buffer), at row stride dmax = max(d_w, d), then the path table of
-// the batch's K moves
+// (double buffer), at row stride dmax = max(d_w, d), kApplyPad elements of
+// padding, then the path table of the batch's K moves
 template <typename T>
 __host__ __device__ inline size_t apply_shmem(int dmax, int K) {
-    return sizeof(Cplx<T>) * ((size_t)(TB / apply_lpr<T>()) * dmax + 2 * (size_t)kApplyTile * 3 * dmax) +
+    return sizeof(Cplx<T>) *
+               ((size_t)(TB / apply_lpr<T>()) * dmax + 2 * (size_t)kApplyTile * 3 * dmax + kApplyPad) +
            sizeof(int) * 2 * (size_t)K * kApplyLv;
 }
 
@@ -1038,19 +1057,25 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
     __shared__ int mcount;
     __shared__ int ac_s[kMaxBatch];
     if (*t.err) return;
-    const int nD = a.s.plan[0];
-    if (!a.s.plan[1]) return;
+    const int nD = a.s.plan[0], nacc = a.s.plan[1], nH = a.s.plan[2];
+    if (!nacc) return;
     const int K = a.K, D = t.max_depth, nL = n_levels_pw(D), stride = table_stride(D);
     constexpr int LPR = apply_lpr<T>();  // lanes per grid row
     constexpr int RPI = TB / LPR;        // grid rows per work item
     const int dmax = t.dw > t.dd ? t.dw : t.dd;
     Cplx<T>* S = reinterpret_cast<Cplx<T>*>(apply_dyn);  // the item's sums, row-major like the array
     Cplx<T>* rows2 = S + RPI * dmax;                      // two tiles of rows at stride dmax
-    int* pnode = reinterpret_cast<int*>(rows2 + 2 * kApplyTile * 3 * dmax);  // (move, endpoint, level-2) -> box
+    int* pnode = reinterpret_cast<int*>(rows2 + 2 * kApplyTile * 3 * dmax + kApplyPad);  // (move, endpoint, level-2) -> box
     const int rows_w = t.dw * (t.n_window + 1), rows_d = t.dd * (t.n_diff + 1);
     const int nWb = (rows_w + RPI - 1) / RPI, nDb = (rows_d + RPI - 1) / RPI;
-    const int total = (nWb + nD * nDb) * kApplySplit;
+    // every accepted endpoint rewrites the root; a level-2 incoming array
+    // expects 27/64 of them
+    const int sr = 2 * nacc > kApplyHeavy ? kApplySplit : 1;
+    const int sh = 2 * nacc * 27 > 64 * kApplyHeavy ? kApplySplit : 1;
+    const int n_root = nWb * sr, n_front = nH * nDb * sh;
+    const int total = n_root + n_front + (nD - nH) * nDb;
     if ((int)blockIdx.x >= total) return;  // no work item for this block
+    const int n2 = 2 * t.n_pw_nodes;
     for (int m = threadIdx.x; m < K; m += blockDim.x) ac_s[m] = a.s.accept[m];
     for (int w = threadIdx.x; w < 2 * K * kApplyLv; w += blockDim.x) {
         const int li = w % kApplyLv, me = w / kApplyLv, m = me >> 1, ep = me & 1;
@@ -1060,25 +1085,47 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
         pnode[w] = (l <= lam && l <= D - 2) ? (ep == 0 ? ti.path_t[l] : ti.path_o[l]) : -1;
     }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int item2 = blockIdx.x; item2 < total; item2 += gridDim.x) {
-        const int item = item2 / kApplySplit, part_c = item2 % kApplySplit;  // part of the contributions
-        const bool is_root = item < nWb;
-        int slot = 0, kind = 2, node = 0, l = 0, li = 0, chunk;
-        if (is_root) {
-            chunk = item;
+    for (int it = blockIdx.x; it < total; it += gridDim.x) {
+        bool is_root = false;
+        int chunk, part_c = 0, nparts = 1, key = 0;
+        if (it < n_root) {
+            is_root = true;
+            chunk = it / sr;
+            part_c = it % sr;
+            nparts = sr;
+        } else if (it < n_root + n_front) {
+            const int r = it - n_root, per = nDb * sh;
+            key = a.s.dest[r / per];
+            chunk = (r % per) / sh;
+            part_c = r % sh;
+            nparts = sh;
         } else {
-            const int r = item - nWb;
-            const int key = a.s.dest[r / nDb];
-            slot = key >> 1;
-            kind = key & 1;
-            node = a.pn.node[slot];
-            l = a.pn.level[slot];
-            li = l - 2;
+            const int r = it - n_root - n_front;
+            key = a.s.dest[n2 - 1 - r / nDb];
             chunk = r % nDb;
         }
+        const bool heavy = nparts > 1;
+        const int slot = key >> 1, kind = is_root ? 2 : key & 1;
+        const int node = is_root ? 0 : a.pn.node[slot];
+        const int l = is_root ? 0 : a.pn.level[slot], li = l - 2;
         const int nl = 1 << l;
+        const int d = is_root ? t.dw : t.dd;
+        const int nrows = is_root ? rows_w : rows_d;
+        const int e0 = chunk * RPI * d, ne = min(RPI, nrows - chunk * RPI) * d;
+        Cplx<T>* dst = (is_root ? t.root : (kind == 0 ? t.out : t.inc) + (size_t)slot * t.Md) + e0;
+        const T sy = (is_root || kind == 0) ? T(1) : T(-1);  // in[C] += conj(tau s)
+        // the item's elements e = threadIdx.x + TB j (at most XMAX), in flight
+        // while the contributions are listed and staged
+        Cplx<T> pf[XMAX];
+        if (!heavy) {
+#pragma unroll
+            for (int j = 0; j < XMAX; ++j) {
+                const int e = threadIdx.x + TB * j;
+                if (e < ne) pf[j] = dst[e];
+            }
+        }
         unsigned long long tp = wall_clock64();
-        __syncthreads();  // previous item's lists and rows fully consumed
+        __syncthreads();  // previous item's lists, rows and sums fully consumed
         if (threadIdx.x == 0) mcount = 0;
         __syncthreads();
         for (int c0 = 0; c0 < 2 * K; c0 += blockDim.x) {
@@ -1125,16 +1172,11 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
             __syncthreads();
         }
         const int nm_all = mcount;
-        // heavy arrays: this item takes an even share of the contributions
-        const bool heavy = nm_all > kApplyHeavy;
-        if (!heavy && part_c > 0) continue;  // uniform across the block
-        const int cbeg = heavy ? part_c * nm_all / kApplySplit : 0;
-        const int cend = heavy ? (part_c + 1) * nm_all / kApplySplit : nm_all;
+        const int cbeg = part_c * nm_all / nparts, cend = (part_c + 1) * nm_all / nparts;
         const int nm = cend - cbeg;
         const int* mlc = mlist + cbeg;
         if (threadIdx.x == 0) probe_add(a.s, kPrApplyMatch, tp);
-        const int d = is_root ? t.dw : t.dd;
-        const int nrows = is_root ? rows_w : rows_d;
+        if (heavy && nm == 0) continue;  // uniform across the block
         const int part = threadIdx.x % LPR, lr = threadIdx.x / LPR, row = chunk * RPI + lr;
         const bool on = row < nrows;
         const int y = on ? row % d : 0, z = on ? row / d : 0;
@@ -1183,46 +1225,50 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
             if (more) load_tile(k0 + kApplyTile);  // in flight during the accumulation
             __syncthreads();                        // tile k0 stored by every thread
             const Cplx<T>* rows = rows2 + buf * kApplyTile * 3 * dmax;
-            if (nx > 0)
-                for (int k = 0; k < nt; ++k) {
-                    const Cplx<T>* Rw = rows + k * 3 * dmax;
-                    const T sg = ((mlc[k0 + k] >> 7) & 1) ? T(-1) : T(1);
-                    Cplx<T> yz = cmul(Rw[dmax + y], Rw[2 * dmax + z]);
-                    yz.x *= sg;
-                    yz.y *= sg;
+            // No per-x guard: a lane accumulates all XMAX values (those past its
+            // share of the row read on into the following rows or the padding
+            // after the tiles, and are never written), so the row's LDS reads
+            // are issued together instead of one branch and wait per x.
+            for (int k = 0; k < nt; ++k) {
+                const Cplx<T>* Rw = rows + k * 3 * dmax;
+                const T sg = ((mlc[k0 + k] >> 7) & 1) ? T(-1) : T(1);
+                Cplx<T> yz = cmul(Rw[dmax + y], Rw[2 * dmax + z]);
+                yz.x *= sg;
+                yz.y *= sg;
+                Cplx<T> xv[XMAX];
 #pragma unroll
-                    for (int x = 0; x < XMAX; ++x) {
-                        if (x < nx) {
-                            const Cplx<T> w = cmul(Rw[x0 + x], yz);
-                            ar[x] += w.x;
-                            ai[x] += w.y;
-                        }
-                    }
+                for (int x = 0; x < XMAX; ++x) xv[x] = Rw[x0 + x];
+#pragma unroll
+                for (int x = 0; x < XMAX; ++x) {
+                    ar[x] = fma(xv[x].x, yz.x, ar[x]);
+                    ar[x] = fma(-xv[x].y, yz.y, ar[x]);
+                    ai[x] = fma(xv[x].x, yz.y, ai[x]);
+                    ai[x] = fma(xv[x].y, yz.x, ai[x]);
                 }
+            }
             if (more) store_tile(k0 + kApplyTile, rows2 + (buf ^ 1) * kApplyTile * 3 * dmax);
             buf ^= 1;
         }
         if (threadIdx.x == 0) probe_add(a.s, kPrApplyTiles, tp);
-        // sums to LDS in the array's order, then a coalesced read-modify-write
-        __syncthreads();
+        // sums to LDS in the array's order, then a coalesced write
 #pragma unroll
         for (int x = 0; x < XMAX; ++x)
             if (x < nx) S[lr * d + x0 + x] = {ar[x], ai[x]};
         __syncthreads();
-        Cplx<T>* dst = is_root ? t.root : (kind == 0 ? t.out : t.inc) + (size_t)slot * t.Md;
-        const T sy = (is_root || kind == 0) ? T(1) : T(-1);  // in[C] += conj(tau s)
-        const int e0 = chunk * RPI * d;
-        const int ne = min(RPI, nrows - chunk * RPI) * d;
-        if (heavy) {  // shares of one array from several items
+        if (heavy) {
             for (int e = threadIdx.x; e < ne; e += TB) {
                 const Cplx<T> v = S[e];
-                atomicAdd(&dst[e0 + e].x, v.x);
-                atomicAdd(&dst[e0 + e].y, sy * v.y);
+                atomicAdd(&dst[e].x, v.x);
+                atomicAdd(&dst[e].y, sy * v.y);
             }
         } else {
-            for (int e = threadIdx.x; e < ne; e += TB) {
-                const Cplx<T> c = dst[e0 + e], v = S[e];
-                dst[e0 + e] = {c.x + v.x, c.y + sy * v.y};
+#pragma unroll
+            for (int j = 0; j < XMAX; ++j) {
+                const int e = threadIdx.x + TB * j;
+                if (e < ne) {
+                    const Cplx<T> v = S[e];
+                    dst[e] = {pf[j].x + v.x, pf[j].y + sy * v.y};
+                }
             }
         }
         if (threadIdx.x == 0) probe_add(a.s, kPrApplyRmw, tp);
@@ -1246,6 +1292,9 @@ template <typename T>
 constexpr int kApplyXSmall = apply_xmax<T>(sizeof(T) == 8 ? 28 : 26);
 template <typename T>
 constexpr int kApplyXLarge = apply_xmax<T>(kMaxGridD);
+// a lane's reads reach x0 + XMAX <= LPR XMAX elements into its last row
+static_assert(apply_lpr<float>() * apply_xmax<float>(kMaxGridD) <= kApplyPad, "apply padding");
+static_assert(apply_lpr<double>() * apply_xmax<double>(kMaxGridD) <= kApplyPad, "apply padding");
 
 // ------------------------------------------------------------------ launchers
 // dynamic shared memory above the default 64 KB must be allowed per kernel
