@@ -1022,14 +1022,15 @@ __global__ void k_advance(int* move_ctr, int n) {
 // Each update is separable, s X[x] Y[y] Z[z]: the lanes of a row form
 // s Y[y] Z[z] once per contribution and add X[x] times it to their share of
 // the row's x; the sums go through LDS back to the array's order, so the
-// write is coalesced too, every element read and written once.  The root and,
-// in large batches, the level-2 incoming arrays (each neighbours a quarter of
-// the level's boxes) have their contributions split over kApplySplit items,
-// whose sums are added atomically instead.
+// write is coalesced too, every element read and written once.  The root (in
+// batches with more than kApplyHeavy accepted endpoints) has its contributions
+// split over kApplySplit items, whose sums are added atomically instead; other
+// arrays are never split (atomic adds over whole arrays cost more than the
+// longer contribution loop).
 constexpr int kApplyTile = 8;
 constexpr int kApplyLv = 4;     // levels of the per-block path table (deeper levels read TrialInfo)
-constexpr int kApplySplit = 4;  // items sharing a heavy array
-constexpr int kApplyHeavy = 16; // contributions (expected) above which an array is split
+constexpr int kApplySplit = 4;  // items sharing the root
+constexpr int kApplyHeavy = 16; // contributions above which the root is split
 constexpr int kMaxGridD = 53;  // largest 2n+1 of any digits setting
 
 // lanes per grid row, and the x values one lane holds for grids up to d
@@ -1052,7 +1053,8 @@ __host__ __device__ inline size_t apply_shmem(int dmax, int K) {
 template <typename T, int XMAX>
 __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a) {
     extern __shared__ char apply_dyn[];
-    __shared__ int mlist[2 * kMaxBatch];
+    __shared__ int mlist[2 * kMaxBatch];     // contributions: move << 8 | endpoint << 7 | image code
+    __shared__ int mrow[3 * 2 * kMaxBatch];  // their three table rows (x, y, z), offsets into s.tabs
     __shared__ int wtot[TB / 64];
     __shared__ int mcount;
     __shared__ int ac_s[kMaxBatch];
@@ -1068,12 +1070,12 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
     int* pnode = reinterpret_cast<int*>(rows2 + 2 * kApplyTile * 3 * dmax + kApplyPad);  // (move, endpoint, level-2) -> box
     const int rows_w = t.dw * (t.n_window + 1), rows_d = t.dd * (t.n_diff + 1);
     const int nWb = (rows_w + RPI - 1) / RPI, nDb = (rows_d + RPI - 1) / RPI;
-    // every accepted endpoint rewrites the root; a level-2 incoming array
-    // expects 27/64 of them
+    // every accepted endpoint rewrites the root: its contributions are split
+    // over sr items.  The level-2 incoming arrays (front of s.dest, the most
+    // contributions after the root) come next, so they start first.
     const int sr = 2 * nacc > kApplyHeavy ? kApplySplit : 1;
-    const int sh = 2 * nacc * 27 > 64 * kApplyHeavy ? kApplySplit : 1;
-    const int n_root = nWb * sr, n_front = nH * nDb * sh;
-    const int total = n_root + n_front + (nD - nH) * nDb;
+    const int n_root = nWb * sr;
+    const int total = n_root + nD * nDb;
     if ((int)blockIdx.x >= total) return;  // no work item for this block
     const int n2 = 2 * t.n_pw_nodes;
     for (int m = threadIdx.x; m < K; m += blockDim.x) ac_s[m] = a.s.accept[m];
@@ -1093,15 +1095,9 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
             chunk = it / sr;
             part_c = it % sr;
             nparts = sr;
-        } else if (it < n_root + n_front) {
-            const int r = it - n_root, per = nDb * sh;
-            key = a.s.dest[r / per];
-            chunk = (r % per) / sh;
-            part_c = r % sh;
-            nparts = sh;
         } else {
-            const int r = it - n_root - n_front;
-            key = a.s.dest[n2 - 1 - r / nDb];
+            const int r = it - n_root, arr = r / nDb;
+            key = a.s.dest[arr < nH ? arr : n2 - 1 - (arr - nH)];
             chunk = r % nDb;
         }
         const bool heavy = nparts > 1;
@@ -1165,7 +1161,17 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
             __syncthreads();
             int off = mcount;
             for (int k = 0; k < wv; ++k) off += wtot[k];
-            if (f) mlist[off + __popcll(bal & lt)] = code;
+            if (f) {
+                const int pos = off + __popcll(bal & lt);
+                mlist[pos] = code;
+                const int m = code >> 8, ep = (code >> 7) & 1, cc = code & 127;
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) {
+                    const int sh = ax == 0 ? cc % 3 - 1 : (ax == 1 ? (cc / 3) % 3 - 1 : cc / 9 - 1);
+                    mrow[3 * pos + ax] =
+                        m * stride + (is_root ? root_row(ep, ax) : level_row(nL, ep, li, ax, sh));
+                }
+            }
             __syncthreads();
             if (threadIdx.x == 0)
                 for (int k = 0; k < (int)(blockDim.x >> 6); ++k) mcount += wtot[k];
@@ -1185,33 +1191,34 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
         T ar[XMAX], ai[XMAX];
 #pragma unroll
         for (int x = 0; x < XMAX; ++x) ar[x] = ai[x] = T(0);
-        // Row tiles of kApplyTile contributions, double-buffered: the global
-        // loads of tile n+1 are in flight while tile n is accumulated.
-        constexpr int R = (kApplyTile * 3 * kMaxGridD + TB - 1) / TB;
-        Cplx<T> v[R];
+        // Row tiles of kApplyTile contributions (3 table rows each, RW lanes
+        // per row), double-buffered: the global loads of tile n+1 are in
+        // flight while tile n is accumulated.
+        constexpr int RW = LPR * XMAX <= 32 ? 32 : 64;  // lanes per staged row (d <= LPR XMAX)
+        constexpr int RPP = TB / RW;                    // staged rows per pass
+        constexpr int R = (3 * kApplyTile + RPP - 1) / RPP;
+        const int mm = threadIdx.x % RW, rr0 = threadIdx.x / RW;
+        const Cplx<T>* tbase = reinterpret_cast<const Cplx<T>*>(a.s.tabs);
+        const int* mrc = mrow + 3 * cbeg;
+        T vr[R], vi[R];  // scalar halves: a complex array captured by the lambdas would live in scratch
         auto load_tile = [&](int k0) {
-            const int nt = nm - k0 < kApplyTile ? nm - k0 : kApplyTile;
-            const int nq = nt * 3 * d;
+            const int nr = 3 * min(nm - k0, kApplyTile);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const int q = threadIdx.x + r * TB;
-                if (q < nq) {
-                    const int k = q / (3 * d), ax = (q / d) % 3, mm = q % d;
-                    const int c = mlc[k0 + k];
-                    const int m = c >> 8, ep = (c >> 7) & 1, code = c & 127;
-                    const int sh = ax == 0 ? code % 3 - 1 : (ax == 1 ? (code / 3) % 3 - 1 : code / 9 - 1);
-                    const Cplx<T>* tb = tabs_of<T>(a.s, stride, m);
-                    v[r] = tb[(is_root ? root_row(ep, ax) : level_row(nL, ep, li, ax, sh)) + mm];
+                const int rr = rr0 + r * RPP;
+                if (rr < nr && mm < d) {
+                    const Cplx<T> c = tbase[mrc[3 * k0 + rr] + mm];
+                    vr[r] = c.x;
+                    vi[r] = c.y;
                 }
             }
         };
         auto store_tile = [&](int k0, Cplx<T>* dstrows) {
-            const int nt = nm - k0 < kApplyTile ? nm - k0 : kApplyTile;
-            const int nq = nt * 3 * d;
+            const int nr = 3 * min(nm - k0, kApplyTile);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const int q = threadIdx.x + r * TB;
-                if (q < nq) dstrows[(q / d) * dmax + q % d] = v[r];
+                const int rr = rr0 + r * RPP;
+                if (rr < nr && mm < d) dstrows[rr * dmax + mm] = {vr[r], vi[r]};
             }
         };
         if (nm > 0) {
