@@ -63,12 +63,23 @@ __host__ __device__ inline int max_trial_blocks(int max_depth) {
     return kMaxRoleBlocks * (1 + 2 * nL + 2);
 }
 
+// one trial of a batch, assembled from its partials before the decisions
+struct TrialRec {
+    double dE;       // against the batch's starting state
+    double thr;      // decision threshold: -ln(u) / beta (+ H / beta with a learned proposal)
+    int room;        // free slots of the target leaf (1 << 30 when the leaf does not change)
+    int leaf_t, leaf_o;
+    int ov;          // particles within sigma of the target
+    int pid;         // particle index
+    int pad;
+};
+
 // device scratch of the Monte Carlo kernels (sized for kMaxBatch trials)
 struct McScratch {
     TrialInfo* info;
     void* tabs;            // Cplx<T>, table_stride(D) per trial
     double* part;          // per trial and block: kPartStride values
-    unsigned* done;        // block counter per trial (self-resetting)
+    TrialRec* rec;         // per trial of a batch
     int* accept;           // per trial of the current launch
     double* corr;          // kMaxBatch^2
     unsigned char* ovl;    // kMaxBatch^2 hard-sphere flags of pair (i, j): 1 t_j near o_i, 2 t_j near t_i

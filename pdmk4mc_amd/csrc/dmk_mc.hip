@@ -1,13 +1,17 @@
 // HIP kernels (gfx950, wave64) of the Monte Carlo step:
+//   k_propose   learned proposal: features and MLP displacement per trial
 //   k_setup     per trial: endpoints, root-to-leaf paths, FP64 phase rows
-//   k_trial     per trial: dE = window + difference + near field - self
-//               (src/energy_shift.cpp:18-302); the last block assembles dE,
-//               takes the Metropolis decision and (single move) plans the update
+//   k_trial     per trial: partial sums of dE = window + difference + near
+//               field - self (src/energy_shift.cpp:18-302), one per block
+//   k_decide    single move: assembles dE, decides, plans the update
+//   k_reverse   learned proposal in a batch: reverse-move log-ratio per trial
 //   k_pair_corr per pair of trials of a batch: exact effect of an accepted
-//               earlier move on a later trial
+//               earlier move on a later trial; then one block per trial
+//               assembles its record (dE, threshold, leaf room)
 //   k_scan      one block: sequential decisions of a batch, update plan
 //   k_apply     accept/update (src/tree.cpp:447-597) over the compact list of
 //               rewritten arrays; every array element is read and written once
+//   k_compact   periodic squeeze of the holes moves leave in leaf slots
 // T is the storage/arithmetic type of the bulk (double or float); phase
 // arguments are formed in FP64 and every reduction accumulates in FP64.
 #include <hip/hip_ext.h>
@@ -342,19 +346,36 @@ __device__ void sum_trial_partials(const DevTree<T>& t, const McArgs& a, int tr,
     const int d0 = g.win, d1 = d0 + 2 * nL * g.diff, d2 = d1 + g.near;
 #pragma unroll
     for (int c = 0; c < kAccPerTrial; ++c) v[c] = 0.0;
-    for (int b = g0; b < per; b += gs) {
-        const double* q = p + kPartStride * b;
-        if (b < d0) {
-            v[0] += q[0];
-        } else if (b < d1) {
-            v[1] += q[0];
-        } else if (b < d2) {
-            v[2] += q[0];
-            v[4] += q[1];
-            v[6] += q[2];
-        } else {
-            v[3] += q[0];
-            v[5] += q[1];
+    constexpr int U = 8;  // blocks whose loads are in flight together
+    for (int b0 = g0; b0 < per; b0 += U * gs) {
+        double q0[U], q1[U], q2[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int b = b0 + u * gs;
+            q0[u] = q1[u] = q2[u] = 0.0;
+            if (b < per) {
+                const double* q = p + kPartStride * b;
+                q0[u] = q[0];
+                q1[u] = q[1];
+                q2[u] = q[2];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int b = b0 + u * gs;
+            if (b >= per) continue;
+            if (b < d0) {
+                v[0] += q0[u];
+            } else if (b < d1) {
+                v[1] += q0[u];
+            } else if (b < d2) {
+                v[2] += q0[u];
+                v[4] += q1[u];
+                v[6] += q2[u];
+            } else {
+                v[3] += q0[u];
+                v[5] += q1[u];
+            }
         }
     }
     for (int o = 1; o < gs; o <<= 1)
@@ -705,12 +726,43 @@ __device__ double separable_dot(const T* W, int d, int nz, const Cplx<T>* ax, co
     return acc;
 }
 
+// One trial's record for the decisions (wave 0 of an extra block of
+// k_pair_corr): partial sums, dE, threshold, target-leaf room.
+template <typename T>
+__device__ void assemble_record(const DevTree<T>& t, const McArgs& a, int j) {
+    double v[kAccPerTrial];
+    sum_trial_partials(t, a, j, threadIdx.x, 64, v);
+    if (threadIdx.x != 0) return;
+    const int m = move_base(a) + j;
+    TrialRec r;
+    r.dE = assemble_trial(t, a, j, m, v, &r.ov);
+    a.ml.dE_stale[m] = r.dE;
+    const TrialInfo& tj = a.s.info[j];
+    r.leaf_t = tj.path_t[tj.lam_t];
+    r.leaf_o = tj.path_o[tj.lam_o];
+    r.room = r.leaf_t == r.leaf_o ? (1 << 30) : t.slot_cap[r.leaf_t] - t.slot_count[r.leaf_t];
+    // learned proposal: lB dE - H / beta < -ln(u) / beta
+    r.thr = mh_threshold(a.ml.u[m], a.beta) + (a.prop.learned ? a.prop.H[m] / a.beta : 0.0);
+    r.pid = tj.i;
+    r.pad = 0;
+    a.s.rec[j] = r;
+}
+
+// Blocks 0 .. K-1 assemble the trial records (first, so that their chains of
+// dependent loads overlap the pair blocks); the K(K-1)/2 blocks after them
+// form the pair corrections.
 template <typename T>
 __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
     __shared__ double red[TB / 64];
     __shared__ Cplx<T> A[kRowStride], B[kRowStride], C[kRowStride];
     const int K = a.K;
-    int p = blockIdx.x, i = 0;
+    if ((int)blockIdx.x < K) {
+        const int j = blockIdx.x;
+        if (*t.err || j >= keff(a) || threadIdx.x >= 64) return;
+        assemble_record(t, a, j);
+        return;
+    }
+    int p = blockIdx.x - K, i = 0;
     while (p >= K - 1 - i) {
         p -= K - 1 - i;
         ++i;
@@ -815,7 +867,8 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
 template <typename T>
 __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
     __shared__ double dEs[kMaxBatch], thr[kMaxBatch], dEf[kMaxBatch];
-    __shared__ int leafT[kMaxBatch], leafO[kMaxBatch], accs[kMaxBatch], room[kMaxBatch], ov0[kMaxBatch];
+    __shared__ int leafT[kMaxBatch], leafO[kMaxBatch], accs[kMaxBatch], room[kMaxBatch], ov0[kMaxBatch],
+        pids[kMaxBatch];
     __shared__ double C[64 * 64];
     __shared__ unsigned char F[64 * 64];
     __shared__ int cons;
@@ -827,26 +880,15 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
     const bool dyn = a.move_ctr != nullptr;
     const bool flg = hs || a.prop.learned;  // pair flags present
     unsigned long long tp = wall_clock64();
-    // gs lanes per trial sum its partials (gs = 256 / K rounded to a power of two)
-    int gs = 1;
-    while (gs < 64 && gs * 2 * K <= (int)blockDim.x) gs *= 2;
-    for (int w0 = threadIdx.x; w0 < Ke * gs; w0 += blockDim.x) {
-        const int j = w0 / gs, g0 = w0 % gs;
-        double v[kAccPerTrial];
-        sum_trial_partials(t, a, j, g0, gs, v);
-        if (g0 != 0) continue;
-        const int m = base + j;
-        int ov = 0;
-        const double dE = assemble_trial(t, a, j, m, v, &ov);
-        dEs[j] = dE;
-        ov0[j] = ov;
-        a.ml.dE_stale[m] = dE;
-        const TrialInfo& tj = a.s.info[j];
-        leafT[j] = tj.path_t[tj.lam_t];
-        leafO[j] = tj.path_o[tj.lam_o];
-        room[j] = leafT[j] == leafO[j] ? (1 << 30) : t.slot_cap[leafT[j]] - t.slot_count[leafT[j]];
-        // learned proposal: lB dE - H / beta < -ln(u) / beta
-        thr[j] = mh_threshold(a.ml.u[m], a.beta) + (a.prop.learned ? a.prop.H[m] / a.beta : 0.0);
+    for (int j = threadIdx.x; j < Ke; j += blockDim.x) {  // records from k_pair_corr
+        const TrialRec r = a.s.rec[j];
+        dEs[j] = r.dE;
+        thr[j] = r.thr;
+        room[j] = r.room;
+        leafT[j] = r.leaf_t;
+        leafO[j] = r.leaf_o;
+        ov0[j] = r.ov;
+        pids[j] = r.pid;
     }
     const bool in_lds = K <= 64;
     if (in_lds) {
@@ -867,7 +909,52 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
     if (threadIdx.x == 0) cons = Ke > 0 ? Ke : 0;
     __syncthreads();
     if (threadIdx.x == 0) probe_add(a.s, kPrScanAssemble, tp);
-    if (threadIdx.x < 64) {
+    if (threadIdx.x < 64 && Ke <= 64) {
+        // Lane j owns trial j and keeps what the walk reads of it in
+        // registers; the owner's decision reaches the other lanes by
+        // readlane, and the pair entries of row i + 1 are read while row i
+        // is decided.
+        const int j = threadIdx.x;
+        const bool in = j < Ke;
+        double run = in ? dEs[j] : 0.0;
+        const double thr_j = in ? thr[j] : 0.0;
+        int ovr = in ? ov0[j] : 0, dcr = 0, stl = 0;
+        const int room_j = in ? room[j] : 0, lt_j = in ? leafT[j] : -1, lo_j = in ? leafO[j] : -1;
+        const int pid_j = in ? pids[j] : -1;
+        const double lB = a.lB;
+        double cn = 0.0;
+        int fn = 0;
+        if (0 < j && in) {
+            cn = in_lds ? C[j] : a.s.corr[j];
+            if (flg) fn = in_lds ? F[j] : a.s.ovl[j];
+        }
+        for (int i = 0; i < Ke; ++i) {
+            const double cc = cn;
+            const int fc = fn;
+            if (i + 1 < j && in) {
+                cn = in_lds ? C[(i + 1) * K + j] : a.s.corr[(i + 1) * K + j];
+                if (flg) fn = in_lds ? F[(i + 1) * K + j] : a.s.ovl[(i + 1) * K + j];
+            }
+            if (dyn && __builtin_amdgcn_readlane(stl, i)) {  // a stale proposal ends the step
+                if (j == 0) cons = i;
+                break;
+            }
+            const double dE = lB * run;
+            const int okl = (ovr == 0 && dE < thr_j) ? (dcr < room_j ? 1 : 2) : 0;  // 2: the target leaf is full
+            if (j == i) {
+                dEf[i] = ovr > 0 ? (double)INFINITY : dE;  // a hard-sphere overlap is an infinite energy
+                accs[i] = okl;
+            }
+            if (__builtin_amdgcn_readlane(okl, i) == 1 && j > i && in) {
+                const int lti = __builtin_amdgcn_readlane(lt_j, i);
+                run += cc;
+                dcr += (__builtin_amdgcn_readlane(lo_j, i) != lti) && (lti == lt_j);
+                if (hs) ovr += ((fc >> 1) & 1) - (fc & 1);
+                stl |= (fc >> 2) & 1;
+                if (dyn && pid_j == __builtin_amdgcn_readlane(pid_j, i)) stl = 1;  // the same particle again
+            }
+        }
+    } else if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
         double run[4];
         int dcr[4], ovr[4], stl[4];
@@ -912,7 +999,7 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
                             if (hs) ovr[k] += ((f >> 1) & 1) - (f & 1);
                             stl[k] |= (f >> 2) & 1;
                         }
-                        if (dyn && a.s.info[j].i == a.s.info[i].i) stl[k] = 1;  // the same particle again
+                        if (dyn && pids[j] == pids[i]) stl[k] = 1;  // the same particle again
                     }
                 }
             }
@@ -1337,8 +1424,8 @@ void launch_reverse(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEve
 
 template <typename T>
 void launch_pair_corr(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
-    const int np = a.K * (a.K - 1) / 2;
-    if (np > 0) hipExtLaunchKernelGGL(k_pair_corr<T>, dim3(np), dim3(TB), 0, st, e0, e1, 0, t, a);
+    const int nb = a.K + a.K * (a.K - 1) / 2;  // one record block per trial, then the pairs
+    if (nb > 0) hipExtLaunchKernelGGL(k_pair_corr<T>, dim3(nb), dim3(TB), 0, st, e0, e1, 0, t, a);
 }
 
 template <typename T>

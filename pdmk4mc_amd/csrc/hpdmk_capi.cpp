@@ -257,9 +257,8 @@ Handle* create_handle(HPDMKParams params, int precision, int n_src, const double
     const int Mw = (2 * h->pw.n_window + 1) * (2 * h->pw.n_window + 1) * (h->pw.n_window + 1);
     h->acc_root = dalloc<double>(2 * Mw, o);
     h->energy = dalloc<double>(4, o);
-    h->t_done = dalloc<unsigned>(kMaxBatch, o);
     h->sc.part = dalloc<double>(kPartStride * (size_t)kMaxBatch * max_trial_blocks(D), o);
-    h->sc.done = h->t_done;
+    h->sc.rec = dalloc<TrialRec>(kMaxBatch, o);
     h->sc.info = dalloc<TrialInfo>(kMaxBatch, o);
     h->sc.accept = dalloc<int>(kMaxBatch, o);
     h->sc.corr = dalloc<double>(kMaxBatch * kMaxBatch, o);
@@ -273,7 +272,6 @@ Handle* create_handle(HPDMKParams params, int precision, int n_src, const double
     HIPX(hipMemset(h->probe, 0, sizeof(unsigned long long) * 2 * kProbes));
     HIPX(hipMemset(h->sc.mark, 0, 2 * (size_t)h->n_pw + 2));
     HIPX(hipMemset(h->sc.plan, 0, 3 * sizeof(int)));
-    HIPX(hipMemset(h->t_done, 0, sizeof(unsigned) * kMaxBatch));
     HIPX(hipDeviceSynchronize());
     return h.release();
 }
@@ -1024,7 +1022,6 @@ int hpdmk_clear_error(hpdmk_tree t) {
     return guard([&] {
         Handle* h = H(t);
         HIPX(hipMemsetAsync(h->err, 0, sizeof(int), h->stream));
-        HIPX(hipMemsetAsync(h->t_done, 0, sizeof(unsigned) * kMaxBatch, h->stream));
         HIPX(hipStreamSynchronize(h->stream));
     });
 }

@@ -92,7 +92,6 @@ struct Handle {
     // Monte Carlo scratch (kMaxBatch trials)
     McScratch sc{};
     unsigned long long* probe = nullptr;
-    unsigned* t_done = nullptr;
     // move lists in HBM: `single` serves eval/update calls, `chain` the staged Metropolis run
     MoveBuf single, chain;
     int since_compact = 0;  // leaf changes (upper bound) since the last slot compaction
