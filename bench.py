@@ -29,6 +29,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+FP32_PEAK_TFS = 157.3  # MI355X dense FP32 (matrix and vector)
+FP64_PEAK_TFS = 78.6
 
 
 def cpu_model():
@@ -50,9 +52,8 @@ def measured_traffic(path, kernel):
             rec = json.load(f)
     except (OSError, ValueError):
         return None
-    if rec.get("kernel") != kernel:
-        return None
-    return rec.get("bytes_per_launch")
+    k = rec.get("kernels", {}).get(kernel)
+    return k.get("bytes_per_launch") if k else None
 
 
 def hard_spheres(n, packing, sigma, seed):
@@ -102,6 +103,30 @@ def trial_bytes(st, Mw, Md, bc, br, bn):
 
 def accept_bytes(st, acc, Mw, Md, bc):
     return acc * (2 * Mw * bc + 2 * (st[:, 4] + st[:, 5]) * Md * bc)
+
+
+def update_bytes_distinct(st, Mw, Md, bc):
+    """Algorithmic HBM bytes of the batched updates: every distinct expansion
+    array a batch rewrites (stats column 7, on the batch's first move) and the
+    root expansion are read and written once per batch."""
+    U = st[:, 7]
+    return int(((U > 0) * 2 * Mw * bc + 2 * U * Md * bc).sum())
+
+
+def term_flops(d, nz):
+    """One separable weighted dot over a d x d x nz half grid: d^2 nz
+    real-times-complex multiply-adds (4 flops) plus a complex product and
+    combine per (y, z) row (9 flops)."""
+    return 4 * d * d * nz + 9 * d * nz
+
+
+def pair_corr_flops(st, n_moves, K, nw, nd):
+    """Algorithmic flops of the pair corrections of a batched run of n_moves
+    in batches of K: four window terms per pair of trials plus the counted
+    difference-level terms (stats column 6)."""
+    full, rem = divmod(n_moves, K)
+    pairs = full * K * (K - 1) // 2 + rem * (rem - 1) // 2
+    return pairs * 4 * term_flops(2 * nw + 1, nw + 1) + int(st[:, 6].sum()) * term_flops(2 * nd + 1, nd + 1)
 
 
 def main():
@@ -285,26 +310,46 @@ def main():
     phases = gp.phase_times()
     gp.close()
     Mw, Md = g.dim_window, g.dim_diff
-    kb = {"trial": trial_bytes(st_p, Mw, Md, bc, br, bn).sum(),
-          "update": accept_bytes(st_p, acc_p, Mw, Md, bc).sum()}
-    dom = max(kt, key=lambda k: kt[k][0])
-    ms_tot, cnt = kt[dom]
-    avg_ms = ms_tot / max(cnt, 1)
-    per_launch = kb.get(dom, 0.0) / max(cnt, 1)
-    achieved = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    traffic = measured_traffic(args.traffic_file, dom)
-    per_kernel = {}
-    for k, b in kb.items():
-        ms_k, n_k = kt[k]
-        if n_k and ms_k > 0:
-            per_kernel[k] = {"avg_launch_us": round(ms_k / n_k * 1e3, 3), "alg_bytes_per_launch": int(b / n_k),
-                             "achieved_gbs": round(b / n_k / (ms_k / n_k * 1e-3) / 1e9, 1)}
+    nw, nd = g.n_window, g.n_diff
+    # algorithmic work of each kernel over the profiled segment: HBM bytes for
+    # the trial and update kernels, FP32 flops for the pair corrections
+    work = {"trial": ("hbm", trial_bytes(st_p, Mw, Md, bc, br, bn).sum()),
+            "update": ("hbm", update_bytes_distinct(st_p, Mw, Md, bc))}
+    update_bytes_per_move_sum = accept_bytes(st_p, acc_p, Mw, Md, bc).sum()
+    if batch > 1:
+        work["pair_corr"] = ("fp32" if prec == P.HPDMK_MIXED else "fp64",
+                             pair_corr_flops(st_p, prof_moves, batch, nw, nd))
     symbol = {"trial": "k_trial", "update": "k_apply", "pair_corr": "k_pair_corr", "scan": "k_scan",
               "setup": "k_setup"}
-    roofline = {"bound": "hbm", "kernel": dom, "kernel_symbol": symbol.get(dom, dom), "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "bytes_per_launch": int(per_launch), "avg_launch_us": round(avg_ms * 1e3, 3),
+    per_kernel = {}
+    for k, (bound, w) in work.items():
+        ms_k, n_k = kt[k]
+        if not (n_k and ms_k > 0):
+            continue
+        avg_s = ms_k / n_k * 1e-3
+        if bound == "hbm":
+            ach, peak, unit = w / n_k / avg_s / 1e9, HBM_PEAK_GBS, "GB/s"
+        else:
+            ach, peak, unit = w / n_k / avg_s / 1e12, FP32_PEAK_TFS if bound == "fp32" else FP64_PEAK_TFS, "TFLOP/s"
+        # compute-bound work is priced against the dense MFMA peak of its type
+        # (FP32: 157.3 TFLOP/s, equal to the FP32 vector peak); the pair
+        # corrections run on the vector ALUs, which "engine" states
+        per_kernel[k] = {"bound": "hbm" if bound == "hbm" else "mfma",
+                         "engine": "memory" if bound == "hbm" else bound + " VALU (no MFMA)",
+                         "avg_launch_us": round(avg_s * 1e6, 3), "work_per_launch": int(w / n_k),
+                         "achieved": round(ach, 3), "peak": peak, "unit": unit, "frac": round(ach / peak, 5)}
+    dom = max(kt, key=lambda k: kt[k][0])
+    ms_tot, cnt = kt[dom]
+    pk = per_kernel.get(dom)
+    if pk is None:  # a kernel without an algorithmic-work figure (decisions): report the update
+        dom = "update"
+        pk = per_kernel["update"]
+    traffic = measured_traffic(args.traffic_file, dom)
+    roofline = {"bound": pk["bound"], "engine": pk["engine"], "kernel": dom, "kernel_symbol": symbol.get(dom, dom),
+                "achieved": pk["achieved"], "peak": pk["peak"], "unit": pk["unit"], "frac": pk["frac"],
+                "traffic": traffic, "work_per_launch": pk["work_per_launch"], "avg_launch_us": pk["avg_launch_us"],
                 "kernel_ms": {k: [round(v[0], 3), v[1]] for k, v in kt.items()}, "per_kernel": per_kernel,
+                "update_bytes_per_launch_per_move_sum": int(update_bytes_per_move_sum / max(kt["update"][1], 1)),
                 "phase_us": phases}
 
     # plain chain (one trial kernel + one update kernel per move).  The timed

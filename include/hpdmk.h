@@ -121,10 +121,14 @@ int hpdmk_mc_set_model(hpdmk_tree tree, double bjerrum, double sigma);
 int hpdmk_mc_run(hpdmk_tree tree, double beta, int mode, int batch, double* gpu_ms);  /* batch 0: tuned */
 int hpdmk_mc_results(hpdmk_tree tree, double* dE, int* accept);
 
-/* Per-move counts of the last profiled run (6 ints per move): difference levels read
+/* Per-move counts of the last profiled run (8 ints per move): difference levels read
  * at the target and origin, near-field particles scanned at the target and
  * origin, expansion arrays an acceptance rewrites for the target and origin
- * paths.  They size the algorithmic bytes of each move. */
+ * paths, the difference-level terms of the batch's pair corrections that this
+ * move received as the later trial, and (on the first move of each batch or
+ * single-move step that accepts anything, else 0) the distinct expansion
+ * arrays that step rewrites.  They size the algorithmic bytes and flops of
+ * each move. */
 int hpdmk_mc_stats(hpdmk_tree tree, int* stats);
 
 /* Profiling level of later hpdmk_mc_run calls: 0 off; 1 kernel times

@@ -9,12 +9,15 @@
 namespace hpdmk_amd {
 
 constexpr int kMaxBatch = 256;     // trials per batched step
+constexpr int kMarkWords = 4096;   // shared-memory bitmap of rewritten arrays: 2 n_pw <= 32 kMarkWords
 constexpr int kRowStride = 64;     // stride of one 1-D phase row (2n+1 <= 53)
 constexpr int kAccPerTrial = 7;    // window, difference, res_t, res_o, scanned_t, scanned_o, overlaps_t
 // per trial block: partial sum, particles scanned, hard-sphere overlaps, and
 // the learned proposal's feature sums at the target (kFeatSums)
 constexpr int kPartStride = 3 + kFeatSums;
-constexpr int kStatsPerMove = 6;   // levels t/o, near particles t/o, rewritten arrays t/o
+constexpr int kStatsPerMove = 8;   // levels t/o, near particles t/o, rewritten arrays t/o,
+                                    // difference terms of the pair corrections (as the later trial),
+                                    // distinct arrays the move's batch rewrites (first move of a batch)
 
 // Everything later kernels need about one trial move: endpoints, charge and
 // the root-to-leaf paths of the origin o and the target t = wrap(o + d).
@@ -83,7 +86,6 @@ struct McScratch {
     int* accept;           // per trial of the current launch
     double* corr;          // kMaxBatch^2
     unsigned char* ovl;    // kMaxBatch^2 hard-sphere flags of pair (i, j): 1 t_j near o_i, 2 t_j near t_i
-    unsigned char* mark;   // 2 per expansion box (self-resetting)
     int* dest;             // rewritten arrays, 2*slot + kind (0 out, 1 in): level-2 in arrays from
                            // the front, the others from the back (2 n_pw entries)
     int* plan;             // [0] arrays in dest, [1] accepted moves, [2] level-2 in arrays

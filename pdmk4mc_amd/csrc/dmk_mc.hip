@@ -70,6 +70,10 @@ __device__ __forceinline__ void setup_body(const DevTree<T>& t, const McArgs& a)
     if (threadIdx.x == 0) {
         trial_setup(t, a.ml.idx[m], a.ml.d + 3 * m, ti);
         a.s.info[tr] = ti;
+        if (a.ml.stats) {  // filled by k_pair_corr (pair terms) and the update planner (batch arrays)
+            a.ml.stats[kStatsPerMove * m + 6] = 0;
+            a.ml.stats[kStatsPerMove * m + 7] = 0;
+        }
     }
     __syncthreads();
     const int D = t.max_depth, nL = n_levels_pw(D);
@@ -213,17 +217,22 @@ __device__ void relocate(const DevTree<T>& t, const TrialInfo& ti, int rank) {
 
 // Marks every expansion array the accepted moves rewrite (out of each path box
 // of levels 2..min(leaf, D-2), in of the non-leaf ones and of their non-leaf
-// colleagues) and compacts the marks into s.dest in slot order: the incoming
-// arrays of level 2 (each neighbours a quarter of the level's boxes, so a
-// batch's moves pile up there) from the front, the others from the back.
-// Relocates the moved particles between leaf slots.  Run by one whole block.
+// colleagues) in a shared-memory bitmap; the first marker of an array appends
+// it to s.dest: the incoming arrays of level 2 (each neighbours a quarter of
+// the level's boxes, so a batch's moves pile up there) from the front, the
+// others from the back.  The order inside each group is immaterial: every
+// array is rewritten by its own work items.  Relocates the moved particles
+// between leaf slots.  Run by one whole block.
 template <typename T>
 __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K) {
-    __shared__ int wtot[TB / 64], wtot2[TB / 64];
-    __shared__ int base_sh, base2_sh;
+    __shared__ unsigned bits[kMarkWords];
+    __shared__ int nfront, nback;
     __shared__ int lo_s[kMaxBatch], lt_s[kMaxBatch], ac_s[kMaxBatch], conflict[kMaxBatch];
     const int D = t.max_depth, nL = n_levels_pw(D);
+    const int n2 = 2 * t.n_pw_nodes;  // at most 32 kMarkWords (checked at set-up)
     unsigned long long tp = wall_clock64();
+    for (int w = threadIdx.x; w < (n2 + 31) / 32; w += blockDim.x) bits[w] = 0u;
+    if (threadIdx.x == 0) nfront = nback = 0;
     for (int m = threadIdx.x; m < K; m += blockDim.x) {
         const TrialInfo& ti = a.s.info[m];
         ac_s[m] = a.s.accept[m];
@@ -231,6 +240,15 @@ __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K) {
         lt_s[m] = ti.path_t[ti.lam_t];
     }
     __syncthreads();
+    // first marker of array idx appends it (front: an incoming array of level 2)
+    auto claim = [&](int idx, bool front) {
+        const unsigned bit = 1u << (idx & 31);
+        if (atomicOr(&bits[idx >> 5], bit) & bit) return;
+        if (front)
+            a.s.dest[atomicAdd(&nfront, 1)] = idx;
+        else
+            a.s.dest[n2 - 1 - atomicAdd(&nback, 1)] = idx;
+    };
     // one thread per (move, endpoint, level): own out, own in, colleagues' in
     for (int w = threadIdx.x; w < 2 * K * nL; w += blockDim.x) {
         const int li = w % nL, me = w / nL, m = me >> 1, ep = me & 1;
@@ -240,16 +258,15 @@ __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K) {
         if (l > (ep == 0 ? ti.lam_t : ti.lam_o)) continue;
         const int node = ep == 0 ? ti.path_t[l] : ti.path_o[l];
         const int own = t.pw_slot[node];
-        // an incoming array of level 2 is marked 2 (colleagues share the level)
-        const unsigned char in_mark = l == 2 ? 2 : 1;
-        a.s.mark[2 * own] = 1;
-        if (!t.is_leaf[node]) a.s.mark[2 * own + 1] = in_mark;
+        const bool front = l == 2;
         int sl[26];
 #pragma unroll
         for (int c = 0; c < 26; ++c) sl[c] = t.col_in_slot[26 * node + c];
+        claim(2 * own, false);
+        if (!t.is_leaf[node]) claim(2 * own + 1, front);
 #pragma unroll
         for (int c = 0; c < 26; ++c)
-            if (sl[c] >= 0) a.s.mark[2 * sl[c] + 1] = in_mark;
+            if (sl[c] >= 0) claim(2 * sl[c] + 1, front);
     }
     // particles that stay in their leaf move in place, all at once
     for (int m = threadIdx.x; m < K; m += blockDim.x) {
@@ -262,37 +279,7 @@ __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K) {
     }
     __syncthreads();
     if (threadIdx.x == 0) probe_add(a.s, kPrPlanMark, tp);
-    const int n2 = 2 * t.n_pw_nodes;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    if (threadIdx.x == 0) base_sh = base2_sh = 0;
-    __syncthreads();
-    for (int c0 = 0; c0 < n2; c0 += blockDim.x) {
-        const int idx = c0 + threadIdx.x;
-        const int mk = idx < n2 ? a.s.mark[idx] : 0;
-        const bool f = mk != 0, front = mk == 2, back = mk == 1;
-        const unsigned long long bf = __ballot(front), bb = __ballot(back);
-        const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-        if (lane == 0) {
-            wtot[wv] = __popcll(bf);
-            wtot2[wv] = __popcll(bb);
-        }
-        __syncthreads();
-        int off = base_sh, off2 = base2_sh;
-        for (int k = 0; k < wv; ++k) {
-            off += wtot[k];
-            off2 += wtot2[k];
-        }
-        if (front) a.s.dest[off + __popcll(bf & lt)] = idx;
-        if (back) a.s.dest[n2 - 1 - (off2 + __popcll(bb & lt))] = idx;
-        if (f) a.s.mark[idx] = 0;
-        __syncthreads();
-        if (threadIdx.x == 0)
-            for (int k = 0; k < nw; ++k) {
-                base_sh += wtot[k];
-                base2_sh += wtot2[k];
-            }
-        __syncthreads();
-    }
+    const int base_sh = nfront, base2_sh = nback;
     if (threadIdx.x == 0) probe_add(a.s, kPrPlanCompact, tp);
     // leaf changes (src/tree.cpp:454-463, 583-596), all in parallel: each mover
     // gets the next free slots of its target leaf in move order
@@ -317,6 +304,7 @@ __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K) {
     a.s.plan[0] = base_sh + base2_sh;
     a.s.plan[1] = nacc;
     a.s.plan[2] = base_sh;
+    if (a.ml.stats && nacc) a.ml.stats[kStatsPerMove * move_base(a) + 7] = base_sh + base2_sh;
     probe_add(a.s, kPrPlanRelocate, tp);
 }
 
@@ -677,51 +665,71 @@ __global__ void __launch_bounds__(TB) k_reverse(DevTree<T> t, McArgs a) {
 //               when i's target box at level l is B or a neighbour of B
 //               (s = phase of the image of t_i next to B), minus the same for o_i
 //   near field: +-q_i q_j R_b(|x_j - y|) for y in {t_i, o_i}, b of x_j's leaf
-// Weighted half-grid sum of the product of two separable waves,
-//   Re sum_k W[k] a[k] conj(b[k]),  a[k] = ax[x] ay[y] az[z],  k = x + d y + d^2 z,
-//   = Re sum_{y,z} (ay conj by)[y] (az conj bz)[z] sum_x W[k] (ax conj bx)[x]:
-// d^2 nz real-times-complex products instead of forming every phase.  The
-// factor rows go to shared A/B/C; each thread takes (y, z) rows and returns
-// its share (the caller reduces over the block).
-template <typename T>
-__device__ double separable_dot(const T* W, int d, int nz, const Cplx<T>* ax, const Cplx<T>* ay,
-                                const Cplx<T>* az, const Cplx<T>* bx, const Cplx<T>* by, const Cplx<T>* bz,
-                                Cplx<T>* A, Cplx<T>* B, Cplx<T>* C) {
-    __syncthreads();  // the previous term's A/B/C are consumed
-    for (int x = threadIdx.x; x < 2 * d + nz; x += blockDim.x) {
-        const Cplx<T> u = x < d ? ax[x] : x < 2 * d ? ay[x - d] : az[x - 2 * d];
-        const Cplx<T> v = x < d ? bx[x] : x < 2 * d ? by[x - d] : bz[x - 2 * d];
-        const Cplx<T> r = {u.x * v.x + u.y * v.y, u.y * v.x - u.x * v.y};  // u conj(v)
-        if (x < d)
-            A[x] = r;
-        else if (x < 2 * d)
-            B[x - d] = r;
-        else
-            C[x - 2 * d] = r;
-    }
+// Weighted half-grid sums of products of two separable waves, NC of them
+// against the same table W in one pass over it,
+//   sum_c sg[c] Re sum_k W[k] a_c[k] conj(b_c[k]),  a_c[k] = ax_c[x] ay_c[y] az_c[z],
+//   k = x + d y + d^2 z,
+//   = sum_c sg[c] Re sum_{y,z} (ay conj by)_c[y] (az conj bz)_c[z] sum_x W[k] (ax conj bx)_c[x]:
+// d^2 nz real-times-complex products per term instead of forming every phase,
+// and every element of W is loaded once for all NC terms.  rows[6c..6c+5] are
+// term c's ax, ay, az, bx, by, bz.  The factor rows go to shared A/B/C (NC
+// rows of kRowStride each); each thread takes (y, z) rows and returns its
+// share (the caller reduces over the block).
+template <typename T, int NC>
+__device__ __forceinline__ double separable_dots(const T* W, int d, int nz, const Cplx<T>* const* rows, const double* sg,
+                                 Cplx<T>* A, Cplx<T>* B, Cplx<T>* C) {
+    __syncthreads();  // the previous call's A/B/C are consumed
+    const int per = 2 * d + nz;
+#pragma unroll
+    for (int c = 0; c < NC; ++c)  // compile-time term index: the row pointers stay in registers
+        for (int x = threadIdx.x; x < per; x += blockDim.x) {
+            const Cplx<T> u = x < d ? rows[6 * c][x] : x < 2 * d ? rows[6 * c + 1][x - d] : rows[6 * c + 2][x - 2 * d];
+            const Cplx<T> v =
+                x < d ? rows[6 * c + 3][x] : x < 2 * d ? rows[6 * c + 4][x - d] : rows[6 * c + 5][x - 2 * d];
+            const Cplx<T> r = {u.x * v.x + u.y * v.y, u.y * v.x - u.x * v.y};  // u conj(v)
+            if (x < d)
+                A[c * kRowStride + x] = r;
+            else if (x < 2 * d)
+                B[c * kRowStride + x - d] = r;
+            else
+                C[c * kRowStride + x - 2 * d] = r;
+        }
     __syncthreads();
     double acc = 0.0;
     for (int rr = threadIdx.x; rr < d * nz; rr += blockDim.x) {
         const int y = rr % d, z = rr / d;
         const T* w = W + (size_t)d * y + (size_t)d * d * z;
-        T sr = 0, si = 0;
+        T sr[NC], si[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) sr[c] = si[c] = T(0);
         int x = 0;
         for (; x + 8 <= d; x += 8) {  // eight independent loads in flight
             T wv[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) wv[u] = w[x + u];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                sr += wv[u] * A[x + u].x;
-                si += wv[u] * A[x + u].y;
-            }
+            for (int u = 0; u < 8; ++u)
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    const Cplx<T> av = A[c * kRowStride + x + u];
+                    sr[c] += wv[u] * av.x;
+                    si[c] += wv[u] * av.y;
+                }
         }
         for (; x < d; ++x) {
-            sr += w[x] * A[x].x;
-            si += w[x] * A[x].y;
+            const T wv = w[x];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                const Cplx<T> av = A[c * kRowStride + x];
+                sr[c] += wv * av.x;
+                si[c] += wv * av.y;
+            }
         }
-        const Cplx<T> bc = cmul(B[y], C[z]);
-        acc += (double)(bc.x * sr - bc.y * si);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const Cplx<T> bc = cmul(B[c * kRowStride + y], C[c * kRowStride + z]);
+            acc += sg[c] * (double)(bc.x * sr[c] - bc.y * si[c]);
+        }
     }
     return acc;
 }
@@ -754,7 +762,7 @@ __device__ void assemble_record(const DevTree<T>& t, const McArgs& a, int j) {
 template <typename T>
 __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
     __shared__ double red[TB / 64];
-    __shared__ Cplx<T> A[kRowStride], B[kRowStride], C[kRowStride];
+    __shared__ Cplx<T> A[4 * kRowStride], B[4 * kRowStride], C[4 * kRowStride];
     const int K = a.K;
     if ((int)blockIdx.x < K) {
         const int j = blockIdx.x;
@@ -777,17 +785,21 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
     // window: (tau_j - omega_j) against (tau_i - omega_i), four separable terms
     double part = 0.0;
     {
-        const int dw = t.dw, nzw = t.n_window + 1;
+        const Cplx<T>* rows[24];
+        double sg[4];
         for (int ej = 0; ej < 2; ++ej)
             for (int ei = 0; ei < 2; ++ei) {
-                const double sg = (ej == ei) ? 1.0 : -1.0;
-                part += sg * separable_dot(t.Wmat, dw, nzw, tj + root_row(ej, 0), tj + root_row(ej, 1),
-                                           tj + root_row(ej, 2), ta + root_row(ei, 0), ta + root_row(ei, 1),
-                                           ta + root_row(ei, 2), A, B, C);
+                const int c = 2 * ej + ei;
+                sg[c] = (ej == ei) ? 1.0 : -1.0;
+                for (int ax = 0; ax < 3; ++ax) {
+                    rows[6 * c + ax] = tj + root_row(ej, ax);
+                    rows[6 * c + 3 + ax] = ta + root_row(ei, ax);
+                }
             }
-        part *= 2.0 * t.cW;
+        part = separable_dots<T, 4>(t.Wmat, t.dw, t.n_window + 1, rows, sg, A, B, C) * (2.0 * t.cW);
     }
     // difference levels of j's paths next to i's boxes
+    int ndots = 0;  // level terms formed (counted for the profiled runs)
     for (int ep = 0; ep < 2; ++ep) {
         const int lam = ep == 0 ? Bj.lam_t : Bj.lam_o;
         for (int l = 2; l <= lam - 1; ++l) {
@@ -795,26 +807,37 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
             const int node = ep == 0 ? Bj.path_t[l] : Bj.path_o[l];
             int sh[2][3] = {{0, 0, 0}, {0, 0, 0}};
             bool on[2];
+#pragma unroll
             for (int w = 0; w < 2; ++w) {  // w = 0: i's target (+), 1: i's origin (-)
                 const int lami = w == 0 ? A_.lam_t : A_.lam_o;
                 on[w] = lami >= l;
                 if (!on[w]) continue;
                 const int bi = w == 0 ? A_.path_t[l] : A_.path_o[l];
-                for (int ax = 0; ax < 3 && on[w]; ++ax) {
-                    sh[w][ax] = image_shift(t.cell[3 * bi + ax], t.cell[3 * node + ax], nl);
-                    if (sh[w][ax] == 2) on[w] = false;
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) {
+                    const int v = image_shift(t.cell[3 * bi + ax], t.cell[3 * node + ax], nl);
+                    sh[w][ax] = v == 2 ? 0 : v;
+                    on[w] = on[w] && v != 2;
                 }
             }
             const T* Dl = t.Dmat + (size_t)li * t.Md;
-            double lv = 0.0;
-            for (int w = 0; w < 2; ++w) {
-                if (!on[w]) continue;  // uniform across the block
-                lv += (w == 0 ? 1.0 : -1.0) *
-                      separable_dot(Dl, t.dd, t.n_diff + 1, tj + level_row(nL, ep, li, 0, 0),
-                                    tj + level_row(nL, ep, li, 1, 0), tj + level_row(nL, ep, li, 2, 0),
-                                    ta + level_row(nL, w, li, 0, sh[w][0]), ta + level_row(nL, w, li, 1, sh[w][1]),
-                                    ta + level_row(nL, w, li, 2, sh[w][2]), A, B, C);
+            // term c: j's row against i's target (w = 0, +) or origin (w = 1, -)
+            const int w0 = on[0] ? 0 : 1;  // the first term's w; the second (if any) is w = 1
+            const Cplx<T>* rows[12];
+            const double sg[2] = {w0 == 0 ? 1.0 : -1.0, -1.0};
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                rows[ax] = tj + level_row(nL, ep, li, ax, 0);
+                rows[3 + ax] = ta + level_row(nL, w0, li, ax, on[0] ? sh[0][ax] : sh[1][ax]);
+                rows[6 + ax] = rows[ax];
+                rows[9 + ax] = ta + level_row(nL, 1, li, ax, sh[1][ax]);
             }
+            double lv = 0.0;  // branches uniform across the block
+            if (on[0] && on[1])
+                lv = separable_dots<T, 2>(Dl, t.dd, t.n_diff + 1, rows, sg, A, B, C);
+            else if (on[0] || on[1])
+                lv = separable_dots<T, 1>(Dl, t.dd, t.n_diff + 1, rows, sg, A, B, C);
+            ndots += (int)on[0] + (int)on[1];
             part += lv * 2.0 * t.cD[l] * (ep == 0 ? 1.0 : -1.0);
         }
     }
@@ -849,6 +872,7 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
                 flags |= 4;
         }
         a.s.corr[i * K + j] = total;
+        if (a.ml.stats && ndots) atomicAdd(a.ml.stats + kStatsPerMove * (move_base(a) + j) + 6, ndots);
         if (a.sigma2 > 0.0 || a.prop.learned) a.s.ovl[i * K + j] = (unsigned char)flags;
     }
 }
@@ -909,7 +933,77 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
     if (threadIdx.x == 0) cons = Ke > 0 ? Ke : 0;
     __syncthreads();
     if (threadIdx.x == 0) probe_add(a.s, kPrScanAssemble, tp);
-    if (threadIdx.x < 64 && Ke <= 64) {
+    if (threadIdx.x < 64 && in_lds) {
+        // Lane j owns trial j and holds its whole column of pair corrections
+        // and flags in registers; the walk is unrolled over the (at most 64)
+        // trials, so each step is register arithmetic plus readlanes of the
+        // deciding lane.
+        const int j = threadIdx.x;
+        const bool in = j < Ke;
+        double col[64];
+        unsigned fl[16];
+        // unconditional loads (i K + j < 64 * 64 for K <= 64), so that they
+        // issue back to back; entries outside the column are zeroed after
+#pragma unroll
+        for (int i = 0; i < 64; ++i) col[i] = C[i * K + j];
+#pragma unroll
+        for (int i = 0; i < 64; ++i) col[i] = (in && i < j) ? col[i] : 0.0;
+#pragma unroll
+        for (int w = 0; w < 16; ++w) {
+            unsigned v = 0u;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) v |= (unsigned)F[(4 * w + b) * K + j] << (8 * b);
+            fl[w] = v;
+        }
+#pragma unroll
+        for (int w = 0; w < 16; ++w) {
+            unsigned keep = 0u;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) keep |= (flg && in && 4 * w + b < j) ? (0xffu << (8 * b)) : 0u;
+            fl[w] &= keep;
+        }
+        double run = in ? dEs[j] : 0.0;
+        const double thr_j = in ? thr[j] : 0.0;
+        int ovr = in ? ov0[j] : 0, dcr = 0, stl = 0;
+        const int room_j = in ? room[j] : 0, lt_j = in ? leafT[j] : -1, lo_j = in ? leafO[j] : -1;
+        const int pid_j = in ? pids[j] : -1;
+        const double lB = a.lB;
+        bool done = false;  // uniform
+        double my_dE = 0.0;
+        int my_ok = 0;
+#pragma unroll
+        for (int i = 0; i < 64; ++i) {
+            if (i >= Ke) done = true;
+            if (!done && dyn && __builtin_amdgcn_readlane(stl, i)) {  // a stale proposal ends the step
+                if (j == 0) cons = i;
+                done = true;
+            }
+            if (done) continue;
+            const double dE = lB * run;
+            const int okl = (ovr == 0 && dE < thr_j) ? (dcr < room_j ? 1 : 2) : 0;  // 2: the target leaf is full
+            if (j == i) {  // the owner keeps its outcome in registers until the walk ends
+                my_dE = ovr > 0 ? (double)INFINITY : dE;  // a hard-sphere overlap is an infinite energy
+                my_ok = okl;
+            }
+            if (__builtin_amdgcn_readlane(okl, i) == 1) {  // uniform
+                const int lti = __builtin_amdgcn_readlane(lt_j, i);
+                const int loi = __builtin_amdgcn_readlane(lo_j, i);
+                const int pidi = __builtin_amdgcn_readlane(pid_j, i);
+                if (j > i && in) {
+                    const int fc = (int)((fl[i >> 2] >> (8 * (i & 3))) & 0xffu);
+                    run += col[i];
+                    dcr += (loi != lti) && (lti == lt_j);
+                    if (hs) ovr += ((fc >> 1) & 1) - (fc & 1);
+                    stl |= (fc >> 2) & 1;
+                    if (dyn && pid_j == pidi) stl = 1;  // the same particle again
+                }
+            }
+        }
+        if (in) {
+            dEf[j] = my_dE;
+            accs[j] = my_ok;
+        }
+    } else if (threadIdx.x < 64 && Ke <= 64) {
         // Lane j owns trial j and keeps what the walk reads of it in
         // registers; the owner's decision reaches the other lanes by
         // readlane, and the pair entries of row i + 1 are read while row i

@@ -265,12 +265,12 @@ Handle* create_handle(HPDMKParams params, int precision, int n_src, const double
     h->sc.ovl = dalloc<unsigned char>(kMaxBatch * kMaxBatch, o);
     const size_t tsz = precision == HPDMK_FP64 ? sizeof(Cplx<double>) : sizeof(Cplx<float>);
     h->sc.tabs = dalloc<char>(tsz * (size_t)table_stride(D) * kMaxBatch, o);
-    h->sc.mark = dalloc<unsigned char>(2 * (size_t)h->n_pw + 2, o);
+    if (2 * (size_t)h->n_pw > 32 * (size_t)kMarkWords)
+        throw HpdmkError(HPDMK_EINVAL, "tree has more expansion boxes than the update planner's bitmap holds");
     h->sc.dest = dalloc<int>(2 * (size_t)h->n_pw + 2, o);
     h->sc.plan = dalloc<int>(3, o);
     h->probe = dalloc<unsigned long long>(2 * kProbes, o);
     HIPX(hipMemset(h->probe, 0, sizeof(unsigned long long) * 2 * kProbes));
-    HIPX(hipMemset(h->sc.mark, 0, 2 * (size_t)h->n_pw + 2));
     HIPX(hipMemset(h->sc.plan, 0, 3 * sizeof(int)));
     HIPX(hipDeviceSynchronize());
     return h.release();

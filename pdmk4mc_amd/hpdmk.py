@@ -347,7 +347,7 @@ class HPDMKPtTree:
         return dE, acc, ms
 
     def mc_stats(self):
-        st = np.zeros((self._mc_idx.size, 6), dtype=np.int32)
+        st = np.zeros((self._mc_idx.size, 8), dtype=np.int32)
         _check(load_library().hpdmk_mc_stats(self._h, st.ctypes.data_as(_ip)))
         return st
 
