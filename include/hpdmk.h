@@ -79,6 +79,13 @@ int hpdmk_tree_info(hpdmk_tree tree, int* info);
 int hpdmk_form_outgoing_pw(hpdmk_tree tree);
 /* src/planewave.cpp:99-155 */
 int hpdmk_form_incoming_pw(hpdmk_tree tree);
+/* The expansions the reference keeps in HPDMKPtTree::outgoing_pw and
+ * incoming_pw (include/tree.hpp:86-87) and its tests read directly
+ * (test/hpdmk_test.cpp:43, 84): which 0 the root outgoing expansion, 1 the
+ * outgoing and 2 the incoming expansion of node `node` (Morton pre-order, a
+ * box of levels 2..n_levels-2).  Copies the half grid (index i + j d + k d^2,
+ * k_z <= 0) into out as interleaved re/im doubles; *count = complex elements. */
+int hpdmk_get_expansion(hpdmk_tree tree, int which, int node, double* out, int* count);
 /* include/tree.hpp:105 eval_energy; parts (optional, 3 doubles): window,
  * difference, residual (src/energy.cpp:21-173) */
 int hpdmk_eval_energy(hpdmk_tree tree, double* energy, double* parts);

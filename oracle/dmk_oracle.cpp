@@ -379,11 +379,16 @@ void nudft_halfplane(int nk, double dk, const std::vector<long>& idx, const std:
 }  // namespace
 
 template <typename Real>
-void DmkTree<Real>::form_outgoing_pw() {
+void DmkTree<Real>::form_outgoing_pw_root() {
     std::fill(root_out.begin(), root_out.end(), cplx(0));
     std::vector<long> all(n_src);
     std::iota(all.begin(), all.end(), 0L);
     nudft_halfplane<Real>(pw.n_window, delta_k[0], all, r_sorted, q_sorted, root_out.data());
+}
+
+template <typename Real>
+void DmkTree<Real>::form_outgoing_pw() {
+    form_outgoing_pw_root();
     for (int l = 2; l < max_depth - 1; ++l)
         for (int nd : level_indices[l]) {
             std::fill(outgoing[nd].begin(), outgoing[nd].end(), cplx(0));

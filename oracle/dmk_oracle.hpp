@@ -122,6 +122,11 @@ struct DmkTree {
 
     int locate(Real x, Real y, Real z, std::vector<int>& path) const;  // src/tree.cpp:204-236
     void form_outgoing_pw();   // src/planewave.cpp:24-97 (DIRECT, exact root sum)
+    void form_outgoing_pw_root();   // src/planewave.cpp:24-46 (exact sum instead of FINUFFT)
+    // PROXY (src/upward_pass.cpp:206-322), oracle/proxy_oracle.cpp
+    void form_outgoing_pw_proxy();
+    std::vector<double> proxy_from_particles(int node, int order) const;  // upward_pass.cpp:109-149
+    std::vector<std::vector<double>> proxy_coeffs;  // per node, order^3 (a fastest), after PROXY
     void form_incoming_pw();   // src/planewave.cpp:99-155
     double eval_energy_window();  // src/energy.cpp:21-39
     double eval_energy_diff();    // src/energy.cpp:42-79
@@ -140,6 +145,16 @@ struct DmkTree {
     // helpers
     int find_node(int l, int cx, int cy, int cz) const;  // exact node at level l or -1
 };
+
+// ---------------------------------------------------------------- PROXY ----
+// Chebyshev machinery of include/chebychev.hpp and src/upward_pass.cpp
+// (oracle/proxy_oracle.cpp).  Matrices are row-major.
+int proxy_order(int digits);                                   // upward_pass.cpp:9-14 (+ digits 6)
+std::vector<double> cheb_nodes(int p);                         // chebychev.hpp:35-42
+std::vector<double> cheb_vandermonde(const std::vector<double>& x, int p);  // chebychev.hpp:45-60
+std::vector<double> lu_inverse(std::vector<double> A, int p);
+std::vector<double> cheb_c2p(int p, int half);                 // chebychev.hpp:180-193, 203-296
+std::vector<std::complex<double>> cheb_poly2pw(int p, int n);  // upward_pass.cpp:152-180 (conjugated)
 
 // ---------------------------------------------------------------- Ewald ----
 // src/ewald.cpp:17-308 with a cell list instead of vesin.

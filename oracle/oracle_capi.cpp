@@ -177,6 +177,66 @@ int oracle_form_pw(void* h) {
     });
 }
 
+// PROXY outgoing expansions (src/upward_pass.cpp) followed by the incoming ones
+int oracle_form_pw_proxy(void* h) {
+    return guard([&] {
+        with_tree(h, [&](auto& t) {
+            t.form_outgoing_pw_proxy();
+            t.form_incoming_pw();
+            return 0;
+        });
+    });
+}
+
+// Chebyshev coefficients of a node (order^3, a fastest): from_particles = 1
+// forms them from the node's own particles, 0 returns those the last PROXY
+// build holds (transferred from the children for non-leaf boxes above D-2)
+int oracle_proxy_coeffs(void* h, int node, int from_particles, double* out) {
+    return with_tree(h, [&](auto& t) {
+        const int p = proxy_order(t.params.digits);
+        std::vector<double> c;
+        if (from_particles)
+            c = t.proxy_from_particles(node, p);
+        else if (node < (int)t.proxy_coeffs.size())
+            c = t.proxy_coeffs[node];
+        if (c.empty()) return -1;
+        std::copy(c.begin(), c.end(), out);
+        return p;
+    });
+}
+
+// Chebyshev matrices: which 0 V^-1 (p x p), 1 c2p lower half, 2 c2p upper
+// half, 3 poly2pw (re/im interleaved, (2n+1) x p); returns the element count
+int oracle_cheb_matrix(int which, int p, int n, double* out) {
+    try {
+        std::vector<double> m;
+        if (which == 0) m = lu_inverse(cheb_vandermonde(cheb_nodes(p), p), p);
+        if (which == 1 || which == 2) m = cheb_c2p(p, which - 1);
+        if (which == 3) {
+            const auto P = cheb_poly2pw(p, n);
+            for (size_t i = 0; i < P.size(); ++i) {
+                out[2 * i] = P[i].real();
+                out[2 * i + 1] = P[i].imag();
+            }
+            return (int)P.size();
+        }
+        std::copy(m.begin(), m.end(), out);
+        return (int)m.size();
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+int oracle_proxy_order(int digits) {
+    try {
+        return proxy_order(digits);
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
 // which: 0 root outgoing (half grid), 1 node outgoing, 2 node incoming; out: interleaved re/im
 int oracle_get_pw(void* h, int which, int node, double* out) {
     return with_tree(h, [&](auto& t) {

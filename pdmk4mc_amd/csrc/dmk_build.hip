@@ -211,12 +211,17 @@ void launch_copy16(const void* src, void* dst, size_t n16, hipStream_t st) {
 }
 
 template <typename T>
-void launch_outgoing(const DevTree<T>& t, const PwNodes& pn, double* acc_root, hipStream_t st) {
+void launch_outgoing_root(const DevTree<T>& t, double* acc_root, hipStream_t st) {
     hipLaunchKernelGGL(k_zero, dim3((2 * t.Mw + TB - 1) / TB), dim3(TB), 0, st, acc_root, 2 * t.Mw);
     const int chunk = 2048;
     const int nch = (t.total_slots + chunk - 1) / chunk;
     hipLaunchKernelGGL(k_pw_root<T>, dim3((t.Mw + TB - 1) / TB, nch), dim3(TB), 0, st, t, acc_root, chunk);
     hipLaunchKernelGGL(k_store_root<T>, dim3((t.Mw + TB - 1) / TB), dim3(TB), 0, st, t, (const double*)acc_root);
+}
+
+template <typename T>
+void launch_outgoing(const DevTree<T>& t, const PwNodes& pn, double* acc_root, hipStream_t st) {
+    launch_outgoing_root(t, acc_root, st);
     if (t.n_pw_nodes > 0)
         hipLaunchKernelGGL(k_pw_level<T>, dim3((t.Md + TB - 1) / TB, t.n_pw_nodes), dim3(TB), 0, st, t, pn);
 }
@@ -239,6 +244,7 @@ void launch_energy(const DevTree<T>& t, const PwNodes& pn, const int* leaves, in
 }
 
 #define INSTANTIATE(T)                                                                                     \
+    template void launch_outgoing_root<T>(const DevTree<T>&, double*, hipStream_t);                      \
     template void launch_outgoing<T>(const DevTree<T>&, const PwNodes&, double*, hipStream_t);           \
     template void launch_incoming<T>(const DevTree<T>&, const PwNodes&, hipStream_t);                    \
     template void launch_energy<T>(const DevTree<T>&, const PwNodes&, const int*, int, double*, hipStream_t);

@@ -1,6 +1,8 @@
 // Host-side set-up of a DMK system (see dmk_host.hpp).
 #include "dmk_host.hpp"
 
+#include "../../include/hpdmk.h"
+
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -388,6 +390,50 @@ Octree build_octree(int n, const double* r, double L, int n_per_leaf) {
             T.near_img[T.near_off[i] + k] = (int8_t)nl_lists[i][k].second;
         }
     return T;
+}
+
+// ------------------------------------------------------------------ PROXY tables
+int proxy_order(int digits) {
+    if (digits <= 3) return 18;  // src/upward_pass.cpp:9-14
+    if (digits <= 6) return 28;  // an order of include/chebychev.hpp:86-96; interpolation error < 1e-7
+    throw HpdmkError(HPDMK_ENOTIMPL, "Polynomial order for requested precision not implemented");
+}
+
+ProxyTables proxy_tables(int p, int n) {
+    ProxyTables t;
+    t.p = p;
+    std::vector<double> x(p), Vi((size_t)p * p);
+    for (int i = 0; i < p; ++i) x[i] = std::cos(M_PI * ((p - i - 1) + 0.5) / p);  // ascending nodes
+    auto cheb = [p](double u, std::vector<double>& T) {
+        T.assign(p, 0.0);
+        T[0] = 1.0;
+        if (p > 1) T[1] = u;
+        for (int k = 2; k < p; ++k) T[k] = 2.0 * u * T[k - 1] - T[k - 2];
+    };
+    std::vector<double> Tv;
+    for (int i = 0; i < p; ++i) {
+        cheb(x[i], Tv);
+        for (int a = 0; a < p; ++a) Vi[(size_t)a * p + i] = (a == 0 ? 1.0 : 2.0) / p * Tv[a];
+    }
+    t.A.assign(2 * (size_t)p * p, 0.0);
+    for (int h = 0; h < 2; ++h)
+        for (int i = 0; i < p; ++i) {
+            cheb(0.5 * x[i] + (h ? 0.5 : -0.5), Tv);  // T_l at the child's nodes in parent coordinates
+            for (int l = 0; l < p; ++l)
+                for (int a = 0; a < p; ++a) t.A[(size_t)h * p * p + (size_t)l * p + a] += Vi[(size_t)a * p + i] * Tv[l];
+        }
+    const int d = 2 * n + 1;
+    t.P.assign(2 * (size_t)d * p, 0.0);
+    for (int k = 0; k < d; ++k)
+        for (int i = 0; i < p; ++i) {
+            const double th = (k - n) * (M_PI / 3.0) * x[i];
+            const double c = std::cos(th), s = -std::sin(th);
+            for (int a = 0; a < p; ++a) {
+                t.P[2 * ((size_t)k * p + a)] += c * Vi[(size_t)a * p + i];
+                t.P[2 * ((size_t)k * p + a) + 1] += s * Vi[(size_t)a * p + i];
+            }
+        }
+    return t;
 }
 
 }  // namespace hpdmk_amd

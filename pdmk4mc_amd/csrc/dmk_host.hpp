@@ -44,6 +44,18 @@ Octree build_octree(int n, const double* r, double L, int n_per_leaf);
 std::vector<double> window_table(const PswfConsts& p, double L);
 std::vector<double> difference_table(const PswfConsts& p, double L, int level);
 
+// Chebyshev tables of the PROXY initialisation (src/upward_pass.cpp,
+// include/chebychev.hpp), from the discrete orthogonality of T_a at the
+// first-kind nodes x_i: V^-1[a][i] = (2 - delta_a0) / p T_a(x_i).
+//   A[h p^2 + l p + a]: T_l(x/2 + (h ? 1/2 : -1/2)) = sum_a A[l][a] T_a(x)
+//   P[2 (k p + a) + {0,1}] = sum_i e^{-i (k - n) (pi/3) x_i} V^-1[a][i], k = 0..2n
+struct ProxyTables {
+    int p = 0;
+    std::vector<double> A, P;
+};
+int proxy_order(int digits);  // 18 for digits <= 3 (src/upward_pass.cpp:9-14), 28 for 6, else throws
+ProxyTables proxy_tables(int p, int n);
+
 struct HpdmkError : std::runtime_error {
     int code;
     HpdmkError(int c, const std::string& m) : std::runtime_error(m), code(c) {}

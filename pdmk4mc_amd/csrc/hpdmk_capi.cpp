@@ -501,6 +501,54 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
     return ms;
 }
 
+// PROXY set-up at first use: tables, coefficient storage and the box lists
+// (every box of level D-2 and every shallower leaf takes its coefficients from
+// its particles; the other boxes of levels 2..D-3 from their children)
+static void proxy_setup(Handle* h) {
+    if (h->proxy.p) return;
+    const int p = proxy_order(h->params.digits);
+    const Octree& tr = h->tree;
+    const int D = tr.max_depth;
+    const ProxyTables tb = proxy_tables(p, h->pw.n_diff);
+    ProxyArgs& pa = h->proxy;
+    auto& o = h->owned;
+    pa.A = dupload(tb.A, o);
+    pa.P = dupload(tb.P, o);
+    pa.coef = dalloc<double>((size_t)h->n_pw * p * p * p, o);
+    std::vector<int> direct, c2p;
+    std::vector<std::vector<int>> by_level(std::max(D - 4, 0));  // levels 2..D-3
+    int slot = 0;
+    for (int i = 0; i < tr.n_nodes; ++i) {
+        const int l = tr.level[i];
+        if (l < 2 || l > D - 2) continue;
+        if (l == D - 2 || tr.is_leaf[i])
+            direct.push_back(slot);
+        else
+            by_level[l - 2].push_back(slot);
+        ++slot;
+    }
+    if (by_level.size() > (size_t)kMaxLevels) throw HpdmkError(HPDMK_EINVAL, "tree too deep for the upward pass");
+    pa.n_levels = (int)by_level.size();
+    for (int li = 0; li < pa.n_levels; ++li) {
+        pa.level_off[li] = (int)c2p.size();
+        pa.level_cnt[li] = (int)by_level[li].size();
+        c2p.insert(c2p.end(), by_level[li].begin(), by_level[li].end());
+    }
+    pa.direct = dupload(direct, o);
+    pa.n_direct = (int)direct.size();
+    pa.c2p = dupload(c2p, o);
+    pa.p = p;
+}
+
+void form_outgoing(Handle* h, hipStream_t st) {
+    if (h->params.init == PROXY) {
+        proxy_setup(h);
+        dispatch(h, [&](auto& d) { launch_outgoing_proxy(d, h->pn, h->acc_root, h->proxy, st); });
+    } else {
+        dispatch(h, [&](auto& d) { launch_outgoing(d, h->pn, h->acc_root, st); });
+    }
+}
+
 }  // namespace capi
 }  // namespace hpdmk_amd
 
@@ -563,9 +611,7 @@ int hpdmk_tree_info(hpdmk_tree t, int* info) {
 int hpdmk_form_outgoing_pw(hpdmk_tree t) {
     return guard([&] {
         Handle* h = H(t);
-        if (h->params.init == PROXY && h->params.digits > 3)  // src/upward_pass.cpp:9-14
-            throw HpdmkError(HPDMK_ENOTIMPL, "Polynomial order for requested precision not implemented");
-        dispatch(h, [&](auto& d) { launch_outgoing(d, h->pn, h->acc_root, h->stream); });
+        form_outgoing(h, h->stream);
         HIPX(hipGetLastError());
         HIPX(hipStreamSynchronize(h->stream));
     });
@@ -577,6 +623,35 @@ int hpdmk_form_incoming_pw(hpdmk_tree t) {
         dispatch(h, [&](auto& d) { launch_incoming(d, h->pn, h->stream); });
         HIPX(hipGetLastError());
         HIPX(hipStreamSynchronize(h->stream));
+    });
+}
+
+int hpdmk_get_expansion(hpdmk_tree t, int which, int node, double* out, int* count) {
+    return guard([&] {
+        Handle* h = H(t);
+        if (!out || which < 0 || which > 2) throw HpdmkError(HPDMK_EINVAL, "bad expansion selector or null output");
+        const Octree& tr = h->tree;
+        int slot = -1;
+        if (which > 0) {
+            if (node < 0 || node >= tr.n_nodes || tr.level[node] < 2 || tr.level[node] > tr.max_depth - 2)
+                throw HpdmkError(HPDMK_EINVAL, "node carries no expansion");
+            slot = 0;
+            for (int i = 0; i < node; ++i) slot += tr.level[i] >= 2 && tr.level[i] <= tr.max_depth - 2;
+            if (which == 2 && tr.is_leaf[node]) throw HpdmkError(HPDMK_EINVAL, "leaves carry no incoming expansion");
+        }
+        HIPX(hipStreamSynchronize(h->stream));
+        dispatch(h, [&](auto& d) {
+            using T = std::decay_t<decltype(d.root[0].x)>;
+            const size_t M = which == 0 ? d.Mw : d.Md;
+            const Cplx<T>* src = which == 0 ? d.root : (which == 1 ? d.out : d.inc) + (size_t)slot * d.Md;
+            std::vector<Cplx<T>> v(M);
+            HIPX(hipMemcpy(v.data(), src, M * sizeof(Cplx<T>), hipMemcpyDeviceToHost));
+            for (size_t e = 0; e < M; ++e) {
+                out[2 * e] = (double)v[e].x;
+                out[2 * e + 1] = (double)v[e].y;
+            }
+            if (count) *count = (int)M;
+        });
     });
 }
 

@@ -14,6 +14,7 @@
 #include "dmk_common.hpp"
 #include "dmk_host.hpp"
 #include "dmk_kernels.hpp"
+#include "dmk_proxy.hpp"
 
 namespace hpdmk_amd {
 namespace capi {
@@ -89,6 +90,9 @@ struct Handle {
     double* acc_root = nullptr;
     double* energy = nullptr;
     int* err = nullptr;
+    // PROXY initialisation (allocated at its first use): Chebyshev tables,
+    // coefficients per expansion box, box lists
+    ProxyArgs proxy{};
     // Monte Carlo scratch (kMaxBatch trials)
     McScratch sc{};
     unsigned long long* probe = nullptr;
@@ -120,6 +124,10 @@ struct Handle {
         if (stream && own_stream) (void)hipStreamDestroy(stream);
     }
 };
+
+// outgoing expansions of a handle's system by its params.init (DIRECT or
+// PROXY), enqueued on st (src/planewave.cpp:86-97)
+void form_outgoing(Handle* h, hipStream_t st);
 
 template <typename F>
 int guard(F&& f) {

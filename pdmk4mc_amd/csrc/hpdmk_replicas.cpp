@@ -150,12 +150,8 @@ int hpdmk_replicas_form_pw(hpdmk_replicas s) {
     return guard([&] {
         ReplicaSet* set = RS(s);
         for (auto& h : set->reps) {
-            if (h->params.init == PROXY && h->params.digits > 3)  // src/upward_pass.cpp:9-14
-                throw HpdmkError(HPDMK_ENOTIMPL, "Polynomial order for requested precision not implemented");
-            dispatch(h.get(), [&](auto& d) {
-                launch_outgoing(d, h->pn, h->acc_root, set->stream);
-                launch_incoming(d, h->pn, set->stream);
-            });
+            form_outgoing(h.get(), set->stream);
+            dispatch(h.get(), [&](auto& d) { launch_incoming(d, h->pn, set->stream); });
         }
         HIPX(hipGetLastError());
         HIPX(hipStreamSynchronize(set->stream));

@@ -50,6 +50,7 @@ SIGNATURES = {
     "hpdmk_mc_run": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_int, _dp]),
     "hpdmk_mc_results": (C.c_int, [C.c_void_p, _dp, _ip]),
     "hpdmk_mc_stats": (C.c_int, [C.c_void_p, _ip]),
+    "hpdmk_get_expansion": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _ip]),
     "hpdmk_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "hpdmk_kernel_times": (C.c_int, [C.c_void_p, _dp, _ip]),
     "hpdmk_clear_error": (C.c_int, [C.c_void_p]),
@@ -232,6 +233,16 @@ class HPDMKPtTree:
 
     def form_incoming_pw(self):
         _check(load_library().hpdmk_form_incoming_pw(self._h))
+
+    def get_expansion(self, which, node=0):
+        """which 0: root outgoing; 1: outgoing of `node`; 2: incoming of `node`
+        (the reference's tree.outgoing_pw / incoming_pw, half grid)."""
+        n = self.dim_window if which == 0 else self.dim_diff
+        out = np.zeros(2 * n)
+        cnt = C.c_int(0)
+        _check(load_library().hpdmk_get_expansion(self._h, int(which), int(node), out.ctypes.data_as(_dp),
+                                                  C.byref(cnt)))
+        return out[0::2] + 1j * out[1::2]
 
     def energy_parts(self):
         e = C.c_double()

@@ -52,6 +52,10 @@ def _load(fast=False):
         "oracle_positions": (C.c_int, [C.c_void_p, _dp]),
         "oracle_form_pw": (C.c_int, [C.c_void_p]),
         "oracle_get_pw": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
+        "oracle_form_pw_proxy": (C.c_int, [C.c_void_p]),
+        "oracle_proxy_coeffs": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
+        "oracle_cheb_matrix": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp]),
+        "oracle_proxy_order": (C.c_int, [C.c_int]),
         "oracle_single_pw": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, _dp,
                                        C.c_int, _ip]),
         "oracle_eval_energy": (C.c_int, [C.c_void_p, _dp]),
@@ -153,6 +157,27 @@ def reference_test_moves(n_src, L, ntrials):
     return idx, d
 
 
+# ---------------------------------------------------------------- PROXY
+def proxy_order(digits):
+    p = lib().oracle_proxy_order(int(digits))
+    if p < 0:
+        raise RuntimeError(lib().oracle_last_error().decode())
+    return p
+
+
+def cheb_matrix(which, p, n=0):
+    """0: V^-1 (p x p); 1, 2: child-to-parent matrices of the lower and upper
+    half; 3: coefficient-to-plane-wave matrix P ((2n+1) x p, complex)."""
+    size = (2 * n + 1) * p * 2 if which == 3 else p * p
+    out = np.zeros(size)
+    cnt = lib().oracle_cheb_matrix(int(which), int(p), int(n), out.ctypes.data_as(_dp))
+    if cnt < 0:
+        raise RuntimeError(lib().oracle_last_error().decode())
+    if which == 3:
+        return (out[0::2] + 1j * out[1::2]).reshape(2 * n + 1, p)
+    return out.reshape(p, p)
+
+
 # ---------------------------------------------------------------- tree
 class OracleTree:
     """CPU restatement of HPDMKPtTree<Real> (FP64 by default)."""
@@ -200,6 +225,18 @@ class OracleTree:
 
     def form_pw(self):
         _check(self._L.oracle_form_pw(self.h), self._L)
+
+    def form_pw_proxy(self):
+        """PROXY outgoing expansions (Chebyshev upward pass), then incoming."""
+        _check(self._L.oracle_form_pw_proxy(self.h), self._L)
+
+    def proxy_coeffs(self, node, from_particles=False):
+        p = proxy_order(self.digits)
+        out = np.zeros(p ** 3)
+        n = self._L.oracle_proxy_coeffs(self.h, int(node), int(from_particles), out.ctypes.data_as(_dp))
+        if n < 0:
+            return None
+        return out.reshape(p, p, p)  # [c, b, a]: a (x) fastest
 
     def get_pw(self, which, node=0):
         dim = self.dim_window if which == 0 else self.dim_diff

@@ -1,0 +1,142 @@
+"""GPU parity of the PROXY initialisation (Chebyshev upward pass,
+src/upward_pass.cpp) against the CPU oracle's restatement.
+
+Bars (written in each test):
+* FP64: every outgoing and incoming expansion equals the oracle's PROXY
+  expansion to 1e-11 of the largest element, energies to 1e-10 of |E|;
+* the reference's PlanewaveProxy contract (test/hpdmk_test.cpp:17-127,
+  547-549): root within 1e-3 relative and boxes within 1e-3 absolute of the
+  brute-force sums;
+* PROXY and DIRECT builds agree (1e-5 of sum |q| at digits 3, 1e-7 at 6);
+* mixed precision: PROXY energy within 1e-6 of the FP64 PROXY energy;
+* trial dE after a PROXY build equals the oracle's (1e-9 absolute);
+* digits 9 PROXY is refused with HPDMK_ENOTIMPL, as the reference throws.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import pdmk4mc_amd as P
+
+pytestmark = pytest.mark.gpu
+L = 20.0
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def build(n, leaf, digits, init, precision=P.HPDMK_FP64):
+    r, q = O.reference_test_system(n, L)
+    g = P.HPDMKPtTree(P.HPDMKParams(leaf, digits, L, init=init), r, q, precision)
+    g.form_outgoing_pw()
+    g.form_incoming_pw()
+    return r, q, g
+
+
+@pytest.mark.parametrize("n,leaf,digits", [(1000, 5, 3), (3000, 6, 6), (20000, 64, 6)])
+def test_proxy_expansions_match_oracle(n, leaf, digits):
+    log("proxy expansions", n, leaf, digits)
+    r, q, g = build(n, leaf, digits, P.PROXY)
+    t = O.OracleTree(r, q, L, n_per_leaf=leaf, digits=digits)
+    t.form_pw_proxy()
+    assert g.n_nodes == t.n_nodes
+    nodes = t.nodes()
+    D = t.max_depth
+    ro, rg = t.get_pw(0), g.get_expansion(0)
+    assert np.abs(ro - rg).max() < 1e-11 * np.abs(ro).max()
+    checked = 0
+    for nd in range(t.n_nodes):
+        if not (2 <= nodes[nd, 0] <= D - 2):
+            continue
+        for which in (1, 2):
+            if which == 2 and nodes[nd, 1]:
+                continue
+            a, b = t.get_pw(which, nd), g.get_expansion(which, nd)
+            scale = max(np.abs(a).max(), 1.0)
+            assert np.abs(a - b).max() < 1e-11 * scale, (nd, which)
+            checked += 1
+    assert checked > 0
+    Eo, Eg = t.eval_energy(), g.energy_parts()
+    assert np.abs(Eg - Eo).max() < 1e-10 * abs(Eo.sum())
+
+
+def test_planewave_proxy_contract_gpu():
+    log("PlanewaveProxy contract")
+    r, q, g = build(1000, 5, 3, P.PROXY)
+    t = O.OracleTree(r, q, L, n_per_leaf=5, digits=3)
+    pos = t.positions()
+    nodes = t.nodes()
+    nw, nd_ = g.n_window, g.n_diff
+    dw, dd = 2 * nw + 1, 2 * nd_ + 1
+    rng = np.random.default_rng(3)
+    root = g.get_expansion(0)
+    dk0 = 2 * np.pi / L
+    for _ in range(10):
+        ix, iy, iz = rng.integers(0, dw), rng.integers(0, dw), rng.integers(0, nw + 1)
+        kv = np.array([ix - nw, iy - nw, iz - nw]) * dk0
+        ref = np.sum(q * np.exp(-1j * (r @ kv)))
+        got = root[ix + iy * dw + iz * dw * dw]
+        assert abs(got.real / ref.real - 1) < 1e-3 and abs(got.imag / ref.imag - 1) < 1e-3
+    D = g.max_depth
+    for nd in range(g.n_nodes):
+        lv = nodes[nd, 0]
+        if lv < 2 or lv > D - 2 or nodes[nd, 6] == 0:
+            continue
+        pw = g.get_expansion(1, nd)
+        dk = (2 * np.pi / 3) / (L / 2 ** lv)
+        inside = np.all(np.floor(pos / L * (1 << lv)).astype(int) == nodes[nd, 3:6], axis=1)
+        for _ in range(3):
+            ix, iy, iz = rng.integers(0, dd), rng.integers(0, dd), rng.integers(0, nd_ + 1)
+            kv = np.array([ix - nd_, iy - nd_, iz - nd_]) * dk
+            ref = np.sum(q[inside] * np.exp(-1j * (pos[inside] @ kv)))
+            assert abs(pw[ix + iy * dd + iz * dd * dd] - ref) < 1e-3
+
+
+@pytest.mark.parametrize("n,leaf,digits,tol", [(1000, 5, 3, 1e-5), (3000, 6, 6, 1e-7)])
+def test_proxy_agrees_with_direct(n, leaf, digits, tol):
+    log("proxy vs direct", n, leaf, digits)
+    r, q, gp = build(n, leaf, digits, P.PROXY)
+    _, _, gd = build(n, leaf, digits, P.DIRECT)
+    nodes = O.OracleTree(r, q, L, n_per_leaf=leaf, digits=digits).nodes()
+    D = gp.max_depth
+    worst = 0.0
+    for nd in range(gp.n_nodes):
+        if 2 <= nodes[nd, 0] <= D - 2:
+            worst = max(worst, np.abs(gp.get_expansion(1, nd) - gd.get_expansion(1, nd)).max())
+    assert worst < tol * np.abs(q).sum()
+    Ep, Ed = gp.eval_energy(), gd.eval_energy()
+    assert abs(Ep - Ed) < 10 * tol * abs(Ed)
+
+
+def test_proxy_mixed_precision():
+    log("proxy mixed")
+    _, _, g64 = build(20000, 64, 6, P.PROXY)
+    _, _, g32 = build(20000, 64, 6, P.PROXY, P.HPDMK_MIXED)
+    E64, E32 = g64.eval_energy(), g32.eval_energy()
+    assert abs(E32 - E64) < 1e-6 * abs(E64)
+
+
+def test_proxy_shift_energy_matches_oracle():
+    log("proxy shift")
+    r, q, g = build(3000, 6, 6, P.PROXY)
+    t = O.OracleTree(r, q, L, n_per_leaf=6, digits=6)
+    t.form_pw_proxy()
+    idx, d = O.reference_test_moves(3000, L, 10)
+    for k in range(10):
+        eo = t.eval_shift_energy(idx[k], d[k])
+        eg = g.eval_shift_energy(int(idx[k]), *d[k])
+        assert abs(eg - eo) < 1e-9, (k, eg, eo)
+        t.update_shift(idx[k], d[k])
+        g.update_shift(int(idx[k]), *d[k])
+
+
+def test_proxy_digits9_not_implemented():
+    r, q = O.reference_test_system(1000, L)
+    g = P.HPDMKPtTree(P.HPDMKParams(10, 9, L, init=P.PROXY), r, q)
+    with pytest.raises(P.HPDMKError) as e:
+        g.form_outgoing_pw()
+    assert e.value.code == -5  # HPDMK_ENOTIMPL
