@@ -164,7 +164,7 @@ def main():
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--replica-moves", type=int, default=1000)
     ap.add_argument("--graph-steps", type=int, default=64)
-    ap.add_argument("--legs", default="batched,learned,plain,accuracy,cpu,copy,replicas,hardspheres",
+    ap.add_argument("--legs", default="batched,learned,plain,accuracy,cpu,copy,replicas,hardspheres,build",
                     help="comma list of batched, learned, plain, accuracy, cpu, copy, replicas, hardspheres, large (copy = a 1 GiB "
                          "streaming copy: measured HBM ceiling and byte-counter calibration; replicas = "
                          "256 ragged independent systems stepped together)")
@@ -588,6 +588,40 @@ def main():
                                 "seconds": round(time.perf_counter() - t0, 2)}}
         print(f"# large: {large_info}", file=sys.stderr, flush=True)
 
+    build_info = None
+    if "build" in legs:
+        # full builds of the N = 1e5 system: outgoing expansions (DIRECT on
+        # FP32 MFMA with FP64 sums in mixed mode, FP64 VALU in FP64 mode, or
+        # the Chebyshev PROXY pass), incoming expansions, total energy
+        build_info = {"workload": f"N={args.n}, L={args.L}, leaf {args.leaf}", "runs": []}
+        for init, prc, dg in ((P.DIRECT, P.HPDMK_MIXED, 6), (P.DIRECT, P.HPDMK_FP64, 6), (P.PROXY, P.HPDMK_MIXED, 6),
+                              (P.PROXY, P.HPDMK_FP64, 6), (P.DIRECT, P.HPDMK_MIXED, 3), (P.PROXY, P.HPDMK_MIXED, 3)):
+            gb = P.HPDMKPtTree(P.HPDMKParams(args.leaf, dg, args.L, init=init), r, q, prc)
+            gb.form_outgoing_pw()  # first call: one-time set-up of the build's tables and work lists
+            t0 = time.perf_counter()
+            gb.form_outgoing_pw()
+            t_out = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            gb.form_incoming_pw()
+            t_in = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            Eb = gb.eval_energy()
+            t_e = time.perf_counter() - t0
+            run = {"init": "direct" if init == P.DIRECT else "proxy",
+                   "precision": "mixed" if prc == P.HPDMK_MIXED else "fp64", "digits": dg,
+                   "outgoing_ms": round(t_out * 1e3, 3), "incoming_ms": round(t_in * 1e3, 3),
+                   "energy_ms": round(t_e * 1e3, 3), "E": Eb}
+            if init == P.DIRECT and prc == P.HPDMK_MIXED:
+                # algorithmic flops of the plane-wave sums: 8 per particle per
+                # half-grid mode, root plus every expansion level
+                nwb, ndb, Db = gb.n_window, gb.n_diff, gb.max_depth
+                modes = gb.dim_window + (Db - 3) * gb.dim_diff
+                run["pw_gflop"] = round(8.0 * args.n * modes / 1e9, 3)
+                run["pw_tflops_incl_launch"] = round(8.0 * args.n * modes / t_out / 1e12, 2)
+            build_info["runs"].append(run)
+            gb.close()
+        print(f"# build: {build_info}", file=sys.stderr, flush=True)
+
     copy_gbs = None
     if "copy" in legs:
         # streaming copy of 1 GiB: the measured HBM ceiling, and the
@@ -619,6 +653,7 @@ def main():
         "replicas": rep_info,
         "hard_spheres": hs_info,
         "large": large_info,
+        "build": build_info,
         "accuracy": acc_info,
     }
     print(json.dumps(out))

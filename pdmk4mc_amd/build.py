@@ -7,7 +7,7 @@ import subprocess
 PKG = pathlib.Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libhpdmk_amd.so"
-SOURCES = ["dmk_mc.hip", "dmk_build.hip", "dmk_proxy.hip", "ewald.hip", "mlp_train.hip", "hpdmk_capi.cpp", "hpdmk_replicas.cpp",
+SOURCES = ["dmk_mc.hip", "dmk_build.hip", "dmk_proxy.hip", "dmk_nudft_mfma.hip", "ewald.hip", "mlp_train.hip", "hpdmk_capi.cpp", "hpdmk_replicas.cpp",
            "hpdmk_ewald.cpp", "hpdmk_mlp.cpp", "dmk_host.cpp"]
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -540,10 +540,52 @@ static void proxy_setup(Handle* h) {
     pa.p = p;
 }
 
+// Work items of the MFMA build: every expansion box and the root, each as
+// runs of at most kMfmaLeavesPerItem of its descendant leaves; boxes with more
+// than one run sum through an FP64 buffer
+static void mfma_setup(Handle* h) {
+    if (h->mfma_ready) return;
+    const Octree& tr = h->tree;
+    const int D = tr.max_depth;
+    std::vector<int> leaf_pos(tr.n_nodes, -1);
+    int nlf = 0;
+    for (int i = 0; i < tr.n_nodes; ++i)
+        if (tr.is_leaf[i]) leaf_pos[i] = nlf++;
+    std::vector<PwWork> work;
+    std::vector<int> acc_slots;
+    auto add_runs = [&](int slot, int node, int lo, int hi) {
+        const int runs = (hi - lo + kMfmaLeavesPerItem - 1) / kMfmaLeavesPerItem;
+        int acc = -1;
+        if (slot >= 0 && runs > 1) {
+            acc = (int)acc_slots.size();
+            acc_slots.push_back(slot);
+        }
+        if (runs == 0) work.push_back({slot, node, lo, lo, acc});
+        for (int r = 0; r < runs; ++r)
+            work.push_back({slot, node, lo + r * kMfmaLeavesPerItem, std::min(hi, lo + (r + 1) * kMfmaLeavesPerItem), acc});
+    };
+    add_runs(-1, 0, 0, nlf);  // root
+    int slot = 0;
+    for (int i = 0; i < tr.n_nodes; ++i) {
+        if (tr.level[i] < 2 || tr.level[i] > D - 2) continue;
+        add_runs(slot++, i, leaf_pos[tr.first_leaf[i]], leaf_pos[tr.last_leaf[i]] + 1);
+    }
+    auto& o = h->owned;
+    h->mfma.work = dupload(work, o);
+    h->mfma.n_work = (int)work.size();
+    h->mfma.acc_slots = dupload(acc_slots, o);
+    h->mfma.n_acc = (int)acc_slots.size();
+    h->mfma.acc_box = dalloc<double>(2 * (size_t)h->d32.Md * std::max<size_t>(acc_slots.size(), 1), o);
+    h->mfma_ready = true;
+}
+
 void form_outgoing(Handle* h, hipStream_t st) {
     if (h->params.init == PROXY) {
         proxy_setup(h);
         dispatch(h, [&](auto& d) { launch_outgoing_proxy(d, h->pn, h->acc_root, h->proxy, st); });
+    } else if (h->precision == HPDMK_MIXED && h->pw.n_diff <= 12 && h->pw.n_window <= 12) {
+        mfma_setup(h);  // FP32 MFMA contractions with FP64 sums (digits <= 6)
+        launch_outgoing_mfma(h->d32, h->leaves, h->mfma, h->acc_root, st);
     } else {
         dispatch(h, [&](auto& d) { launch_outgoing(d, h->pn, h->acc_root, st); });
     }

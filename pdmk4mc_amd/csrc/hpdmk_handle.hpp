@@ -14,6 +14,7 @@
 #include "dmk_common.hpp"
 #include "dmk_host.hpp"
 #include "dmk_kernels.hpp"
+#include "dmk_nudft_mfma.hpp"
 #include "dmk_proxy.hpp"
 
 namespace hpdmk_amd {
@@ -93,6 +94,9 @@ struct Handle {
     // PROXY initialisation (allocated at its first use): Chebyshev tables,
     // coefficients per expansion box, box lists
     ProxyArgs proxy{};
+    // mixed-precision DIRECT build on MFMA (allocated at its first use)
+    PwMfmaPlan mfma{};
+    bool mfma_ready = false;
     // Monte Carlo scratch (kMaxBatch trials)
     McScratch sc{};
     unsigned long long* probe = nullptr;
