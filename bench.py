@@ -316,11 +316,12 @@ def main():
     work = {"trial": ("hbm", trial_bytes(st_p, Mw, Md, bc, br, bn).sum()),
             "update": ("hbm", update_bytes_distinct(st_p, Mw, Md, bc))}
     update_bytes_per_move_sum = accept_bytes(st_p, acc_p, Mw, Md, bc).sum()
-    if batch > 1:
+    k_batch = tiles["batch"] if (batch == 0 and tiles) else batch  # batch 0: the tuned size
+    if k_batch > 1:
         work["pair_corr"] = ("fp32" if prec == P.HPDMK_MIXED else "fp64",
-                             pair_corr_flops(st_p, prof_moves, batch, nw, nd))
+                             pair_corr_flops(st_p, prof_moves, k_batch, nw, nd))
     symbol = {"trial": "k_trial", "update": "k_apply", "pair_corr": "k_pair_corr", "scan": "k_scan",
-              "setup": "k_setup"}
+              "setup": "k_setup", "records": "k_records", "propose": "k_propose"}
     per_kernel = {}
     for k, (bound, w) in work.items():
         ms_k, n_k = kt[k]

@@ -23,6 +23,7 @@ namespace hpdmk_amd {
 
 constexpr int kMaxLevels = 16;
 constexpr int kMaxPoly = 25;
+constexpr int kNodeAtLevels = 8;  // dense cell -> node tables up to this many levels (2.4M entries)
 
 template <typename T>
 struct Cplx {
@@ -52,6 +53,8 @@ struct DevTree {
     const int* colleague;
     const int* col_in_slot;  // 26 per node: expansion slot of a non-leaf colleague, else -1
     const int* pw_slot;
+    const int* node_at;  // levels 0..D-1 when D <= kNodeAtLevels: node of each cell, -1 below a leaf
+                         // (level l at offset (8^l - 1) / 7, cell index cx + 2^l cy + 4^l cz); else null
     const int* near_off;
     const int* near_ids;
     const int8_t* near_img;

@@ -80,9 +80,31 @@ __device__ __forceinline__ int cell_at(double x, double L, int l) {
     return (int)c;
 }
 
-// descend from the root to the leaf containing (x,y,z); path[l] = node at level l
+// descend from the root to the leaf containing (x,y,z); path[l] = node at level l.
+// With the dense per-level cell tables the D lookups are independent loads
+// (one memory round trip); otherwise a chain of child lookups.
 template <typename T>
 __device__ int locate(const DevTree<T>& t, double x, double y, double z, int* path) {
+    if (t.node_at) {
+        int nodes[kNodeAtLevels];
+#pragma unroll
+        for (int l = 0; l < kNodeAtLevels; ++l) {
+            nodes[l] = -1;
+            if (l < t.max_depth) {
+                const int off = ((1 << (3 * l)) - 1) / 7;
+                const int cx = cell_at(x, t.L, l), cy = cell_at(y, t.L, l), cz = cell_at(z, t.L, l);
+                nodes[l] = t.node_at[off + cx + (cy << l) + (cz << (2 * l))];
+            }
+        }
+        int lam = 0;
+#pragma unroll
+        for (int l = 0; l < kNodeAtLevels; ++l)
+            if (nodes[l] >= 0) {
+                path[l] = nodes[l];
+                lam = l;
+            }
+        return lam;
+    }
     int nd = 0, l = 0;
     path[0] = 0;
     while (!t.is_leaf[nd]) {

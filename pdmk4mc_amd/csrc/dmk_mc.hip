@@ -80,9 +80,10 @@ __device__ __forceinline__ void setup_body(const DevTree<T>& t, const McArgs& a)
     const int stride = table_stride(D);
     Cplx<T>* tb = tabs_of<T>(a.s, stride, tr);
     const int rows = 6 + 2 * nL * 9;
-    for (int w = threadIdx.x; w < rows * kRowStride; w += blockDim.x) {
-        const int r = w / kRowStride, mm = w % kRowStride;
-        int ep, axis, n, len;
+    // one thread per row: e^{-i dk c} by one FP64 sincos, its powers by
+    // complex multiplication (FP64), then the row e^{-i (m - n) dk c}
+    for (int r = threadIdx.x; r < rows; r += blockDim.x) {
+        int ep, axis, n;
         double c, dk;
         if (r < 6) {
             ep = r / 3;
@@ -102,12 +103,20 @@ __device__ __forceinline__ void setup_body(const DevTree<T>& t, const McArgs& a)
             dk = t.delta_k[2 + li];
             c = (ep == 0 ? ti.t[axis] : ti.o[axis]) + (s1 - 1) * t.L;
         }
-        len = axis == 2 ? n + 1 : 2 * n + 1;
-        if (mm >= len) continue;
-        double sn, cs;
-        sincos((double)(mm - n) * (dk * c), &sn, &cs);
+        Cplx<T>* row = tb + r * kRowStride;
         const double scale = axis == 0 ? ti.q : 1.0;
-        tb[r * kRowStride + mm] = {(T)(cs * scale), (T)(-sn * scale)};
+        double sn, cs;
+        sincos(dk * c, &sn, &cs);
+        const double br = cs, bi = -sn;  // e^{-i dk c}
+        double wr = scale, wi = 0.0;     // scale e^{-i m dk c}, m = 0, 1, ...
+        row[n] = {(T)wr, (T)0};
+        for (int m = 1; m <= n; ++m) {
+            const double tr = wr * br - wi * bi;
+            wi = wr * bi + wi * br;
+            wr = tr;
+            if (axis != 2) row[n + m] = {(T)wr, (T)wi};  // the z row stops at k_z = 0
+            row[n - m] = {(T)wr, (T)(-wi)};
+        }
     }
 }
 

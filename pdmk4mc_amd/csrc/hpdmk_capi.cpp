@@ -214,7 +214,19 @@ Handle* create_handle(HPDMKParams params, int precision, int n_src, const double
     int* dleaf = dupload(leaf_of, o);
     h->err = dalloc<int>(1, o);
     HIPX(hipMemset(h->err, 0, sizeof(int)));
+    // dense cell -> node table of every level (one round of independent loads per locate)
+    const int* node_at = nullptr;
+    if (D <= kNodeAtLevels) {
+        std::vector<int> tab((((size_t)1 << (3 * D)) - 1) / 7, -1);
+        for (int i = 0; i < nn; ++i) {
+            const int l = tr.level[i];
+            const size_t off = (((size_t)1 << (3 * l)) - 1) / 7;
+            tab[off + tr.cell[3 * i] + ((size_t)tr.cell[3 * i + 1] << l) + ((size_t)tr.cell[3 * i + 2] << (2 * l))] = i;
+        }
+        node_at = dupload(tab, o);
+    }
     auto set_ptrs = [&](auto& d) {
+        d.node_at = node_at;
         d.child = child;
         d.level = level;
         d.is_leaf = isl;

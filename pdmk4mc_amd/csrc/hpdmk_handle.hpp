@@ -117,8 +117,8 @@ struct Handle {
     double sigma = 0.0;
     // profiling
     int profiling = 0;  // 0 off, 1 kernel events + per-move counts, 2 + phase timers
-    double kms[kKinds] = {0, 0, 0, 0, 0, 0};
-    int kcount[kKinds] = {0, 0, 0, 0, 0, 0};
+    double kms[kKinds] = {};
+    int kcount[kKinds] = {};
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
 
