@@ -8,7 +8,7 @@
 // a complex GEMM with M = d rows (two 16-row tiles), N = d (n+1) columns
 // (16-column tiles) and K = the box's particles.  Each complex tile product
 // is four v_mfma_f32_16x16x4f32 (re re, -im im, re im, im re).  The 1-D
-// phase rows of 16 particles at a time are formed in FP64 (one sincos per
+// phase rows of 64 particles at a time are formed in FP64 (one sincos per
 // particle and axis, powers by complex multiplication) and staged in LDS as
 // FP32; the FP32 tile sums are added into FP64 accumulators every 16
 // particles, so the sums accumulate in FP64.
@@ -28,8 +28,8 @@ namespace {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kKC = 16;        // particles per staged chunk (four K-steps of 4)
-constexpr int kFlush = 1;      // chunks between FP64 flushes (16 particles)
+constexpr int kKC = 64;        // particles per staged chunk (sixteen K-steps of 4)
+constexpr int kFlushK = 4;     // K-steps between FP64 flushes (16 particles)
 constexpr int kRowPad = 32;    // padded length of the x and y rows (d <= 25, two M tiles)
 constexpr int kZPad = 16;      // padded length of the z rows (n + 1 <= 13)
 constexpr int kMaxLeaves = 256;  // leaves of one work item staged in LDS
@@ -87,8 +87,7 @@ __global__ void __launch_bounds__(64 * NWV) k_pw_mfma(DevTree<float> t, const in
 #pragma unroll
             for (int r = 0; r < 4; ++r) sr[u][mt][r] = si[u][mt][r] = 0.0;
         }
-    int chunk = 0;
-    for (int base = 0; base < total; base += kKC, ++chunk) {
+    for (int base = 0; base < total; base += kKC) {
         __syncthreads();  // the previous chunk's rows are consumed
         if (threadIdx.x < kKC) {  // slot of the chunk's particle (binary search in the prefix)
             const int u = base + threadIdx.x;
@@ -140,8 +139,8 @@ __global__ void __launch_bounds__(64 * NWV) k_pw_mfma(DevTree<float> t, const in
             }
         }
         __syncthreads();
-#pragma unroll
-        for (int ks = 0; ks < kKC / 4; ++ks) {
+        const int nks = min(kKC, total - base + 3) / 4;  // K-steps holding particles (uniform)
+        for (int ks = 0; ks < nks; ++ks) {
             const int p = 4 * ks + kq;
             const float2 a0 = X[p][col_in], a1 = X[p][16 + col_in];  // rows i = col_in (+16) of this K slice
 #pragma unroll
@@ -161,20 +160,20 @@ __global__ void __launch_bounds__(64 * NWV) k_pw_mfma(DevTree<float> t, const in
                 ai[u][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.x, yz.y, ai[u][1], 0, 0, 0);
                 ai[u][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.y, yz.x, ai[u][1], 0, 0, 0);
             }
-        }
-        if ((chunk + 1) % kFlush == 0 || base + kKC >= total) {  // FP32 tile sums into FP64
+            if ((ks + 1) % kFlushK == 0 || ks + 1 == nks) {  // FP32 tile sums into FP64 (uniform)
 #pragma unroll
-            for (int u = 0; u < TPW; ++u)
+                for (int u = 0; u < TPW; ++u)
 #pragma unroll
-                for (int mt = 0; mt < 2; ++mt) {
+                    for (int mt = 0; mt < 2; ++mt) {
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        sr[u][mt][r] += (double)ar[u][mt][r];
-                        si[u][mt][r] += (double)ai[u][mt][r];
+                        for (int r = 0; r < 4; ++r) {
+                            sr[u][mt][r] += (double)ar[u][mt][r];
+                            si[u][mt][r] += (double)ai[u][mt][r];
+                        }
+                        ar[u][mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+                        ai[u][mt] = f32x4{0.f, 0.f, 0.f, 0.f};
                     }
-                    ar[u][mt] = f32x4{0.f, 0.f, 0.f, 0.f};
-                    ai[u][mt] = f32x4{0.f, 0.f, 0.f, 0.f};
-                }
+            }
         }
     }
     // D layout of v_mfma_f32_16x16x4f32: lane holds rows 4 (lane / 16) + r of column lane % 16
