@@ -168,7 +168,7 @@ def main():
                     help="comma list of batched, learned, plain, accuracy, cpu, copy, replicas, hardspheres, large (copy = a 1 GiB "
                          "streaming copy: measured HBM ceiling and byte-counter calibration; replicas = "
                          "256 ragged independent systems stepped together)")
-    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_update_batched_n1e5.json"),
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_mc_batched_n1e5.json"),
                     help="measured HBM bytes per launch of the dominant kernel (from rocprofv3 --pmc)")
     args = ap.parse_args()
     legs = set(args.legs.split(","))
