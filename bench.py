@@ -83,6 +83,24 @@ def hard_spheres(n, packing, sigma, seed):
     return r, np.where(np.arange(n) % 2 == 0, 1.0, -1.0), L
 
 
+def tau_int(x, c=5.0):
+    """Integrated autocorrelation time of the series x (in samples) with
+    Sokal's automatic window: the smallest W with W >= c tau(W)."""
+    x = np.asarray(x, dtype=float) - np.mean(x)
+    n = len(x)
+    if n < 8 or not np.any(x):
+        return float("nan")
+    f = np.fft.rfft(x, 2 * n)
+    acf = np.fft.irfft(f * np.conj(f))[:n]
+    acf /= acf[0]
+    tau = 0.5
+    for w in range(1, n):
+        tau += acf[w]
+        if w >= c * tau:
+            return float(tau)
+    return float(tau)
+
+
 def workload(n, L, n_moves, seed):
     rng = np.random.default_rng(seed)
     r = rng.uniform(0, L, (n, 3))
@@ -161,10 +179,13 @@ def main():
     ap.add_argument("--hs-train-step", type=float, default=1.0)
     ap.add_argument("--hs-rf", type=float, default=2.0)
     ap.add_argument("--large-moves", type=int, default=100000)
+    ap.add_argument("--ac-n", type=int, default=4000, help="particles of the autocorrelation system")
+    ap.add_argument("--acm-sweeps", type=int, default=60, help="sweeps of the main-workload autocorrelation chains")
+    ap.add_argument("--ac-sweeps", type=int, default=400, help="sweeps of each autocorrelation chain")
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--replica-moves", type=int, default=1000)
     ap.add_argument("--graph-steps", type=int, default=64)
-    ap.add_argument("--legs", default="batched,learned,plain,accuracy,cpu,copy,replicas,hardspheres,build",
+    ap.add_argument("--legs", default="batched,learned,plain,accuracy,cpu,copy,replicas,hardspheres,build,autocorr",
                     help="comma list of batched, learned, plain, accuracy, cpu, copy, replicas, hardspheres, large (copy = a 1 GiB "
                          "streaming copy: measured HBM ceiling and byte-counter calibration; replicas = "
                          "256 ragged independent systems stepped together)")
@@ -285,6 +306,39 @@ def main():
               f"{acc_l.mean():.3f}, ESJD/move {esjd_l:.4f}; training {learned_info['training']}",
               file=sys.stderr, flush=True)
 
+    # energy autocorrelation on this workload: chains of acm_sweeps sweeps with
+    # the uniform and the learned proposal, energy sampled once per sweep
+    # (the plain chain is the same Markov chain as the batched uniform one)
+    acm = None
+    if "autocorr_main" in legs and "learned" in legs:
+        acm = {"sweeps": args.acm_sweeps}
+        Ma = args.acm_sweeps * args.n
+        rng_m = np.random.default_rng(args.seed + 61)
+        for name, learned in (("uniform", False), ("learned", True)):
+            gm_ = P.HPDMKPtTree(params, r, q, prec)
+            gm_.form_outgoing_pw()
+            gm_.form_incoming_pw()
+            if tiles:
+                gm_.set_tiles(**tiles)
+            mi = rng_m.integers(0, args.n, Ma)
+            mu = rng_m.uniform(0, 1, Ma)
+            if learned:
+                gm_.mc_set_proposal(True, weights=w, rf=args.rf, **cfgp)
+                md = rng_m.normal(size=(Ma, 3))
+            else:
+                md = rng_m.uniform(-1.0, 1.0, (Ma, 3))
+            gm_.mc_load(mi, md, mu)
+            t0 = time.perf_counter()
+            gm_.mc_run(1.0, 1, batch)
+            wm = time.perf_counter() - t0
+            dEm, am = gm_.mc_results()
+            gm_.close()
+            e_trace = np.cumsum(np.where(am > 0, dEm, 0.0))[args.n - 1::args.n]
+            tau = tau_int(e_trace)
+            acm[name] = {"moves_per_s": round(Ma / wm, 1), "tau_int_sweeps": round(tau, 3),
+                         "decorrelated_samples_per_s": round(Ma / wm / args.n / (2 * tau), 4)}
+        print(f"# autocorr (main workload): {acm}", file=sys.stderr, flush=True)
+
     # per-kernel times on a profiled segment: start/end timestamps of every
     # dispatch (level 1: no device-side timers in the kernels)
     prof_moves = min(M, 4000)
@@ -395,6 +449,12 @@ def main():
         gpp.close()
         print(f"# plain: {plain:.0f} moves/s, acceptance {acc_pl.mean():.3f}, ESJD/move {esjd_pl:.4f}; "
               f"headline/plain = {value / plain:.2f}", file=sys.stderr, flush=True)
+        if acm:  # same Markov chain as the batched uniform one: same tau per sweep
+            tau_u = acm["uniform"]["tau_int_sweeps"]
+            acm["plain"] = {"moves_per_s": round(plain, 1), "tau_int_sweeps": tau_u,
+                            "decorrelated_samples_per_s": round(plain / args.n / (2 * tau_u), 4)}
+            acm["learned_over_plain"] = round(acm["learned"]["decorrelated_samples_per_s"] /
+                                              acm["plain"]["decorrelated_samples_per_s"], 3)
 
     # accuracy of the arithmetic mode: trial dE of this precision vs the FP64 path on the same state
     acc_info = None
@@ -426,9 +486,30 @@ def main():
                          "dE_mixed_vs_ewald_max_rel_to_E": float(np.abs(em - dE_ew).max() / abs(E_ew)),
                          "dE_mixed_vs_ewald_mean_rel_to_E": float(np.abs(em - dE_ew).mean() / abs(E_ew)),
                          "dE_fp64_vs_ewald_max_rel_to_E": float(np.abs(e64 - dE_ew).max() / abs(E_ew))})
-        print(f"# accuracy: {acc_info}", file=sys.stderr, flush=True)
         g64.close()
         gm.close()
+        # the same errors at every tolerance of the workload (digits 3, 6, 9):
+        # mixed precision (FP32 bulk, FP64 phases and sums; digits 3 and 6,
+        # like the reference's float instantiation, src/tree.cpp:19-29) and FP64
+        per_digits = {}
+        for dg in (3, 6, 9):
+            row = {}
+            for name, prc in (("mixed", P.HPDMK_MIXED), ("fp64", P.HPDMK_FP64)):
+                if prc == P.HPDMK_MIXED and dg > 6:
+                    row[name] = "not supported (FP32 storage: digits 3 and 6)"
+                    continue
+                gx = P.HPDMKPtTree(P.HPDMKParams(args.leaf, dg, args.L, init=P.DIRECT), r, q, prc)
+                gx.form_outgoing_pw()
+                gx.form_incoming_pw()
+                ex = gx.eval_shift_energy_batch(sidx, sd)
+                Ex = gx.eval_energy()
+                gx.close()
+                row[name] = {"E_vs_ewald_rel": float(abs(Ex - E_ew) / abs(E_ew)),
+                             "dE_vs_ewald_max_rel_to_E": float(np.abs(ex - dE_ew).max() / abs(E_ew)),
+                             "dE_vs_ewald_mean_rel_to_E": float(np.abs(ex - dE_ew).mean() / abs(E_ew))}
+            per_digits[str(dg)] = row
+        acc_info["per_digits"] = per_digits
+        print(f"# accuracy: {acc_info}", file=sys.stderr, flush=True)
 
     cpu = None
     if "cpu" in legs:
@@ -547,6 +628,70 @@ def main():
                    "esjd_per_s_learned_over_uniform": round(out["learned"]["esjd_per_s"] / out["uniform"]["esjd_per_s"], 3)}
         print(f"# hard spheres: {hs_info}", file=sys.stderr, flush=True)
 
+    ac_info = None
+    if "autocorr" in legs:
+        # energy autocorrelation of the chain: charged hard spheres, N = 4000,
+        # sigma 1, packing 0.2, Bjerrum length 2, digits 6, leaf 32; uniform
+        # proposal vs the MLP proposal trained on this system's uniform chain.
+        # The energy is sampled once per sweep (N moves); decorrelated
+        # samples per second = sweeps per second / (2 tau_int).
+        an = args.ac_n
+        ar_, aq, aL = hard_spheres(an, 0.2, 1.0, args.seed + 51)
+        aparams = P.HPDMKParams(32, args.digits, aL, init=P.DIRECT)
+        rng_a = np.random.default_rng(args.seed + 52)
+        acfg = dict(max_mean=0.3, sigma_min=0.02, sigma_max=1.0)
+        arf = 2.0
+
+        def ac_tree():
+            ga = P.HPDMKPtTree(aparams, ar_, aq, prec)
+            ga.form_outgoing_pw()
+            ga.form_incoming_pw()
+            ga.mc_set_model(2.0, 1.0)
+            return ga
+
+        ga = ac_tree()
+        ga.mc_set_proposal(False, rf=arf)
+        nt = 50 * an
+        ti = rng_a.integers(0, an, nt)
+        ga.mc_load(ti, rng_a.uniform(-1.0, 1.0, (nt, 3)), rng_a.uniform(0, 1, nt))
+        ga.mc_run(1.0, 1, batch)
+        _, tacc = ga.mc_results()
+        tdl, tft, _ = ga.mc_proposal_results()
+        ga.close()
+        w_a, _ = P.mlp_train(tft, tdl, tacc.astype(np.float64), P.mlp_init(args.seed), lr=3e-3,
+                             epochs=args.train_epochs, batch=1024, seed=args.seed, **acfg)
+        Ma = args.ac_sweeps * an
+        ac_out = {}
+        for name, learned in (("uniform", False), ("learned", True)):
+            ga = ac_tree()
+            ai = rng_a.integers(0, an, Ma)
+            au = rng_a.uniform(0, 1, Ma)
+            if learned:
+                ga.mc_set_proposal(True, weights=w_a, rf=arf, **acfg)
+                ad = rng_a.normal(size=(Ma, 3))
+            else:
+                ad = rng_a.uniform(-args.hs_step, args.hs_step, (Ma, 3))
+            ga.mc_load(ai[:20 * an], ad[:20 * an], au[:20 * an])
+            ga.mc_run(1.0, 1, batch)  # equilibration: 20 sweeps
+            ga.mc_load(ai, ad, au)
+            t0 = time.perf_counter()
+            ga.mc_run(1.0, 1, batch)
+            wa = time.perf_counter() - t0
+            dEa, aa = ga.mc_results()
+            ga.close()
+            e_trace = np.cumsum(np.where(aa > 0, dEa, 0.0))[an - 1::an]  # energy after each sweep (offset dropped)
+            tau = tau_int(e_trace)
+            sweeps_s = Ma / wa / an
+            ac_out[name] = {"moves_per_s": round(Ma / wa, 1), "acceptance": round(float(aa.mean()), 4),
+                            "sweeps": args.ac_sweeps, "tau_int_sweeps": round(tau, 3),
+                            "decorrelated_samples_per_s": round(sweeps_s / (2.0 * tau), 2)}
+        ac_info = {"workload": f"N={an} charged hard spheres, sigma 1, packing 0.2 (L={aL:.2f}), Bjerrum length 2, "
+                               f"digits {args.digits}, leaf 32, batched chain; energy sampled every sweep",
+                   **ac_out,
+                   "decorrelated_learned_over_uniform": round(ac_out["learned"]["decorrelated_samples_per_s"] /
+                                                              ac_out["uniform"]["decorrelated_samples_per_s"], 3)}
+        print(f"# autocorr: {ac_info}", file=sys.stderr, flush=True)
+
     large_info = None
     if "large" in legs:
         # N = 1e6, L = 215.4, charges 90% +-1 and 10% +-2 (neutral), digits 6:
@@ -655,6 +800,8 @@ def main():
         "hard_spheres": hs_info,
         "large": large_info,
         "build": build_info,
+        "autocorr": ac_info,
+        "autocorr_main": acm,
         "accuracy": acc_info,
     }
     print(json.dumps(out))
