@@ -86,6 +86,16 @@ int hpdmk_form_incoming_pw(hpdmk_tree tree);
  * box of levels 2..n_levels-2).  Copies the half grid (index i + j d + k d^2,
  * k_z <= 0) into out as interleaved re/im doubles; *count = complex elements. */
 int hpdmk_get_expansion(hpdmk_tree tree, int which, int node, double* out, int* count);
+/* Rebuilds the octree of a system from its current particle positions into a
+ * new handle: fresh topology, leaf slots and box lists, expansions formed when
+ * the old system had them formed, and the Monte Carlo settings carried over
+ * (model, proposal with its MLP weights, launch shape, profiling level; move
+ * lists are not: load them again).  The old handle stays valid; destroy it
+ * with hpdmk_tree_destroy.  No counterpart in the reference, which never
+ * re-refines (src/tree.cpp:447-597 keeps the topology of the first build):
+ * this is the remedy for HPDMK_ECAPACITY after long runs and gives the energy
+ * of a fresh build to check the incremental one against. */
+int hpdmk_tree_rebuild(hpdmk_tree tree, hpdmk_tree* rebuilt);
 /* include/tree.hpp:105 eval_energy; parts (optional, 3 doubles): window,
  * difference, residual (src/energy.cpp:21-173) */
 int hpdmk_eval_energy(hpdmk_tree tree, double* energy, double* parts);

@@ -82,6 +82,19 @@ class HPDMKPtTree {
         return r;
     }
 
+    // Rebuilds the octree from the current positions (hpdmk_tree_rebuild): the
+    // object then holds the rebuilt system; expansions stay formed as before.
+    // No counterpart in the reference, which keeps the first topology.
+    void rebuild() {
+        hpdmk_tree nh = nullptr;
+        check(hpdmk_tree_rebuild(h_, &nh));
+        hpdmk_tree_destroy(h_);
+        h_ = nh;
+        int info[10];
+        check(hpdmk_tree_info(h_, info));
+        n_levels_ = info[1];
+    }
+
     hpdmk_tree handle() const { return h_; }
 
   private:

@@ -513,6 +513,31 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
     return ms;
 }
 
+// current positions (caller's order, N x 3) and charges of a system
+void snapshot(Handle* h, std::vector<double>& r, std::vector<double>& q) {
+    const int n = h->n_src;
+    r.assign(3 * (size_t)n, 0.0);
+    q.assign(n, 0.0);
+    HIPX(hipStreamSynchronize(h->stream));
+    dispatch(h, [&](auto& dv) {
+        const int S = dv.total_slots;
+        std::vector<double> x(S), y(S), z(S), c(S);
+        std::vector<int> orig(S);
+        HIPX(hipMemcpy(x.data(), dv.px, sizeof(double) * S, hipMemcpyDeviceToHost));
+        HIPX(hipMemcpy(y.data(), dv.py, sizeof(double) * S, hipMemcpyDeviceToHost));
+        HIPX(hipMemcpy(z.data(), dv.pz, sizeof(double) * S, hipMemcpyDeviceToHost));
+        HIPX(hipMemcpy(c.data(), dv.pq, sizeof(double) * S, hipMemcpyDeviceToHost));
+        HIPX(hipMemcpy(orig.data(), dv.porig, sizeof(int) * S, hipMemcpyDeviceToHost));
+        for (int s = 0; s < S; ++s)
+            if (orig[s] >= 0) {
+                r[3 * orig[s]] = x[s];
+                r[3 * orig[s] + 1] = y[s];
+                r[3 * orig[s] + 2] = z[s];
+                q[orig[s]] = c[s];
+            }
+    });
+}
+
 // PROXY set-up at first use: tables, coefficient storage and the box lists
 // (every box of level D-2 and every shallower leaf takes its coefficients from
 // its particles; the other boxes of levels 2..D-3 from their children)
@@ -668,6 +693,7 @@ int hpdmk_form_outgoing_pw(hpdmk_tree t) {
         form_outgoing(h, h->stream);
         HIPX(hipGetLastError());
         HIPX(hipStreamSynchronize(h->stream));
+        h->formed_out = true;
     });
 }
 
@@ -677,6 +703,43 @@ int hpdmk_form_incoming_pw(hpdmk_tree t) {
         dispatch(h, [&](auto& d) { launch_incoming(d, h->pn, h->stream); });
         HIPX(hipGetLastError());
         HIPX(hipStreamSynchronize(h->stream));
+        h->formed_in = true;
+    });
+}
+
+int hpdmk_tree_rebuild(hpdmk_tree t, hpdmk_tree* out) {
+    return guard([&] {
+        Handle* h = H(t);
+        if (!out) throw HpdmkError(HPDMK_EINVAL, "null argument");
+        *out = nullptr;
+        std::vector<double> r, q;
+        snapshot(h, r, q);
+        std::unique_ptr<Handle> nh(create_handle(h->params, h->precision, h->n_src, r.data(), q.data(), nullptr));
+        nh->tiles = h->tiles;
+        nh->tuned_batch = h->tuned_batch;
+        nh->bjerrum = h->bjerrum;
+        nh->sigma = h->sigma;
+        nh->profiling = h->profiling;
+        nh->prop_learned = h->prop_learned;
+        nh->prop_rf = std::min(h->prop_rf, h->params.L / (double)(1 << (nh->tree.max_depth - 1)));
+        nh->prop_max_mean = h->prop_max_mean;
+        nh->prop_smin = h->prop_smin;
+        nh->prop_smax = h->prop_smax;
+        if (h->mlp_w) {
+            nh->mlp_w = dalloc<float>(kMlpParams, nh->owned);
+            HIPX(hipMemcpy(nh->mlp_w, h->mlp_w, sizeof(float) * kMlpParams, hipMemcpyDeviceToDevice));
+        }
+        if (h->formed_out) {
+            form_outgoing(nh.get(), nh->stream);
+            nh->formed_out = true;
+        }
+        if (h->formed_in) {
+            dispatch(nh.get(), [&](auto& d) { launch_incoming(d, nh->pn, nh->stream); });
+            nh->formed_in = true;
+        }
+        HIPX(hipGetLastError());
+        HIPX(hipStreamSynchronize(nh->stream));
+        *out = nh.release();
     });
 }
 
@@ -1025,24 +1088,8 @@ int hpdmk_autotune(hpdmk_tree t, int n_moves, const long* idx, const double* d, 
         if (n_moves < 1 || !idx || !d || !u) throw HpdmkError(HPDMK_EINVAL, "empty or null move list");
         // a twin of the current state takes the timing runs; this system is untouched
         const int n = h->n_src;
-        std::vector<double> r(3 * (size_t)n), q(n);
-        dispatch(h, [&](auto& dv) {
-            const int S = dv.total_slots;
-            std::vector<double> x(S), y(S), z(S), c(S);
-            std::vector<int> orig(S);
-            HIPX(hipMemcpy(x.data(), dv.px, sizeof(double) * S, hipMemcpyDeviceToHost));
-            HIPX(hipMemcpy(y.data(), dv.py, sizeof(double) * S, hipMemcpyDeviceToHost));
-            HIPX(hipMemcpy(z.data(), dv.pz, sizeof(double) * S, hipMemcpyDeviceToHost));
-            HIPX(hipMemcpy(c.data(), dv.pq, sizeof(double) * S, hipMemcpyDeviceToHost));
-            HIPX(hipMemcpy(orig.data(), dv.porig, sizeof(int) * S, hipMemcpyDeviceToHost));
-            for (int s = 0; s < S; ++s)
-                if (orig[s] >= 0) {
-                    r[3 * orig[s]] = x[s];
-                    r[3 * orig[s] + 1] = y[s];
-                    r[3 * orig[s] + 2] = z[s];
-                    q[orig[s]] = c[s];
-                }
-        });
+        std::vector<double> r, q;
+        snapshot(h, r, q);
         std::unique_ptr<Handle> tw(create_handle(h->params, h->precision, n, r.data(), q.data(), nullptr));
         tw->bjerrum = h->bjerrum;
         tw->sigma = h->sigma;

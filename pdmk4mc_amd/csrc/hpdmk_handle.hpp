@@ -94,6 +94,8 @@ struct Handle {
     // PROXY initialisation (allocated at its first use): Chebyshev tables,
     // coefficients per expansion box, box lists
     ProxyArgs proxy{};
+    // which expansions the caller formed (a rebuild forms the same)
+    bool formed_out = false, formed_in = false;
     // mixed-precision DIRECT build on MFMA (allocated at its first use)
     PwMfmaPlan mfma{};
     bool mfma_ready = false;
@@ -132,6 +134,8 @@ struct Handle {
 // outgoing expansions of a handle's system by its params.init (DIRECT or
 // PROXY), enqueued on st (src/planewave.cpp:86-97)
 void form_outgoing(Handle* h, hipStream_t st);
+// current positions (caller's order) and charges, copied to the host
+void snapshot(Handle* h, std::vector<double>& r, std::vector<double>& q);
 
 template <typename F>
 int guard(F&& f) {

@@ -38,6 +38,7 @@ SIGNATURES = {
     "hpdmk_tree_create": (C.c_int, [HPDMKParams, C.c_int, _dp, _dp, C.POINTER(C.c_void_p)]),
     "hpdmk_tree_create_ex": (C.c_int, [HPDMKParams, C.c_int, C.c_int, _dp, _dp, C.POINTER(C.c_void_p)]),
     "hpdmk_tree_destroy": (C.c_int, [C.c_void_p]),
+    "hpdmk_tree_rebuild": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "hpdmk_tree_info": (C.c_int, [C.c_void_p, _ip]),
     "hpdmk_form_outgoing_pw": (C.c_int, [C.c_void_p]),
     "hpdmk_form_incoming_pw": (C.c_int, [C.c_void_p]),
@@ -209,6 +210,20 @@ class HPDMKPtTree:
          self.n_pw_boxes, self.max_near, self.n_leaves, self.n_slots) = [int(v) for v in info]
 
     # ---- lifecycle
+    def rebuild(self):
+        """A new system with the octree rebuilt from the current positions
+        (fresh leaf slots; expansions formed as in this one; Monte Carlo
+        settings carried over, move lists not).  This system stays usable."""
+        h = C.c_void_p()
+        _check(load_library().hpdmk_tree_rebuild(self._h, C.byref(h)))
+        g = HPDMKPtTree.__new__(HPDMKPtTree)
+        g.params, g.n_src, g.precision, g._owned, g._h = self.params, self.n_src, self.precision, True, h
+        info = np.zeros(10, dtype=np.int32)
+        _check(load_library().hpdmk_tree_info(h, info.ctypes.data_as(_ip)))
+        (g.n_nodes, g.max_depth, g.n_window, g.n_diff, g.dim_window, g.dim_diff,
+         g.n_pw_boxes, g.max_near, g.n_leaves, g.n_slots) = [int(v) for v in info]
+        return g
+
     def close(self):
         if getattr(self, "_h", None):
             if self._owned:
