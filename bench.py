@@ -83,6 +83,39 @@ def hard_spheres(n, packing, sigma, seed):
     return r, np.where(np.arange(n) % 2 == 0, 1.0, -1.0), L
 
 
+def write_reference_csv(args, g, plain_info, build_info):
+    """Rows in the schemas of the reference's data/dmk_energyshift_runtime.csv
+    (benchmark/dmk_energyshift_runtime.cpp:89-101) and data/dmk_energy_runtime.csv
+    (benchmark/dmk_energy_runtime.cpp), so that its plot scripts read these
+    numbers beside its own.  GPU rows: n_threads = 0; times in seconds;
+    time_shift = set-up + trial + decision kernels per move and time_update =
+    update kernel per move of the plain chain (time_res: not split, NaN);
+    energy rows from the build leg (window/difference/residual not split)."""
+    os.makedirs(args.csv_dir, exist_ok=True)
+    p = os.path.join(args.csv_dir, "dmk_energyshift_runtime.csv")
+    if plain_info:
+        km = plain_info["profile"]["kernel_ms"]
+        nm = max(km["trial"][1], 1)
+        shift = (km["setup"][0] + km["trial"][0] + km["scan"][0]) / nm * 1e-3
+        upd = km["update"][0] / nm * 1e-3
+        new = not os.path.exists(p)
+        with open(p, "a") as f:
+            if new:
+                f.write("n_src,n_src_per_leaf,digits,L,depth,n_threads,time_update,time_shift,time_res\n")
+            f.write(f"{args.n},{args.leaf},{args.digits},{args.L},{g.max_depth},0,{upd:.6g},{shift:.6g},NaN\n")
+    p = os.path.join(args.csv_dir, "dmk_energy_runtime.csv")
+    if build_info:
+        new = not os.path.exists(p)
+        with open(p, "a") as f:
+            if new:
+                f.write("n_src,n_src_per_leaf,eps,L,time_outgoing_pw,time_incoming_pw,time_window,"
+                        "time_difference,time_residual,time_total\n")
+            for run in build_info["runs"]:
+                tot = (run["outgoing_ms"] + run["incoming_ms"] + run["energy_ms"]) * 1e-3
+                f.write(f"{args.n},{args.leaf},{10.0 ** -run['digits']:g},{args.L},{run['outgoing_ms'] * 1e-3:.6g},"
+                        f"{run['incoming_ms'] * 1e-3:.6g},NaN,NaN,NaN,{tot:.6g}\n")
+
+
 def tau_int(x, c=5.0):
     """Integrated autocorrelation time of the series x (in samples) with
     Sokal's automatic window: the smallest W with W >= c tau(W)."""
@@ -181,6 +214,9 @@ def main():
     ap.add_argument("--large-moves", type=int, default=100000)
     ap.add_argument("--ac-n", type=int, default=4000, help="particles of the autocorrelation system")
     ap.add_argument("--acm-sweeps", type=int, default=60, help="sweeps of the main-workload autocorrelation chains")
+    ap.add_argument("--csv-dir", default=None,
+                    help="also append rows in the reference's data/*.csv schemas (dmk_energyshift_runtime.csv, "
+                         "dmk_energy_runtime.csv) to this directory")
     ap.add_argument("--ac-sweeps", type=int, default=400, help="sweeps of each autocorrelation chain")
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--replica-moves", type=int, default=1000)
@@ -804,6 +840,8 @@ def main():
         "autocorr_main": acm,
         "accuracy": acc_info,
     }
+    if args.csv_dir:
+        write_reference_csv(args, g, plain_info, build_info)
     print(json.dumps(out))
 
 

@@ -169,6 +169,13 @@ int hpdmk_phase_times(hpdmk_tree tree, double* us, int* count);
  * hold 9 ints per node, written in Morton pre-order as: level, is_leaf,
  * parent, cell x, cell y, cell z, particles, same-level neighbours, near-list
  * length.  particle_leaf (optional, n_src ints) receives each particle's leaf. */
+/* src/utils.cpp:14-28 read_particle_info: a text file whose first line is a
+ * header and whose other lines hold "x y z q" (lines that do not parse as four
+ * numbers are skipped).  Writes the first min(capacity, count) particles as
+ * r_src (N x 3) and charge, and the file's particle count to *count (call with
+ * capacity 0 to size the arrays).  Host only; no GPU needed. */
+int hpdmk_read_particle_info(const char* path, int capacity, double* r_src, double* charge, int* count);
+
 int hpdmk_plan_tree(HPDMKParams params, int n_src, const double* r_src, int* n_nodes, int* nodes,
                     int* particle_leaf);
 /* PSWF constants of a digits setting: c, lambda, C0, diff0, psi0(0),

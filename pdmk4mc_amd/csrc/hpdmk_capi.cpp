@@ -7,7 +7,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <fstream>
 #include <memory>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -920,6 +922,31 @@ int hpdmk_kernel_times(hpdmk_tree t, double* ms, int* count) {
             if (ms) ms[k] = h->kms[k];
             if (count) count[k] = h->kcount[k];
         }
+    });
+}
+
+int hpdmk_read_particle_info(const char* path, int capacity, double* r, double* q, int* count) {
+    return guard([&] {
+        if (!path || !count || capacity < 0 || (capacity > 0 && (!r || !q)))
+            throw HpdmkError(HPDMK_EINVAL, "null argument or negative capacity");
+        std::ifstream in(path);
+        if (!in) throw HpdmkError(HPDMK_EINVAL, std::string("cannot open ") + path);
+        std::string line;
+        std::getline(in, line);  // header
+        int n = 0;
+        while (std::getline(in, line)) {
+            std::istringstream ss(line);
+            double x, y, z, c;
+            if (!(ss >> x >> y >> z >> c)) continue;
+            if (n < capacity) {
+                r[3 * n] = x;
+                r[3 * n + 1] = y;
+                r[3 * n + 2] = z;
+                q[n] = c;
+            }
+            ++n;
+        }
+        *count = n;
     });
 }
 

@@ -39,6 +39,7 @@ SIGNATURES = {
     "hpdmk_tree_create_ex": (C.c_int, [HPDMKParams, C.c_int, C.c_int, _dp, _dp, C.POINTER(C.c_void_p)]),
     "hpdmk_tree_destroy": (C.c_int, [C.c_void_p]),
     "hpdmk_tree_rebuild": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "hpdmk_read_particle_info": (C.c_int, [C.c_char_p, C.c_int, _dp, _dp, _ip]),
     "hpdmk_tree_info": (C.c_int, [C.c_void_p, _ip]),
     "hpdmk_form_outgoing_pw": (C.c_int, [C.c_void_p]),
     "hpdmk_form_incoming_pw": (C.c_int, [C.c_void_p]),
@@ -111,6 +112,19 @@ def _check(rc):
 def _d(a, n=None):
     a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
     return a, a.ctypes.data_as(_dp)
+
+
+def read_particle_info(path):
+    """src/utils.cpp:14-28: (r (N x 3), q) from an "x y z q" text file with
+    one header line."""
+    L = load_library()
+    n = C.c_int(0)
+    _check(L.hpdmk_read_particle_info(str(path).encode(), 0, None, None, C.byref(n)))
+    r = np.zeros((n.value, 3))
+    q = np.zeros(n.value)
+    _check(L.hpdmk_read_particle_info(str(path).encode(), n.value, r.ctypes.data_as(_dp), q.ctypes.data_as(_dp),
+                                      C.byref(n)))
+    return r, q
 
 
 def plan_tree(params, r_src):

@@ -81,3 +81,17 @@ def test_cpp_class_header_compiles_and_caller_is_built():
     # argument errors are reported before any device call
     res = subprocess.run([str(CPP_CHECK)], capture_output=True, text=True, timeout=60)
     assert res.returncode == 2 and "usage" in res.stderr
+
+
+def test_read_particle_info_follows_reference(tmp_path):
+    """src/utils.cpp:14-28: the first line is a header; every other line that
+    parses as four numbers is a particle (extra fields ignored, other lines
+    skipped)."""
+    f = tmp_path / "particles.txt"
+    f.write_text("x y z q\n1 2 3 -1\n  4.5\t5.5 6.5 1.0\nnot a particle\n7 8 9\n1e-3 2e-3 3e-3 2 extra\n")
+    r, q = P.read_particle_info(f)
+    assert r.shape == (3, 3)
+    assert np.array_equal(r, [[1, 2, 3], [4.5, 5.5, 6.5], [1e-3, 2e-3, 3e-3]])
+    assert np.array_equal(q, [-1.0, 1.0, 2.0])
+    with pytest.raises(P.HPDMKError):
+        P.read_particle_info(tmp_path / "missing.txt")
