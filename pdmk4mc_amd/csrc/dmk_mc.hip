@@ -152,11 +152,23 @@ __device__ __forceinline__ void propose_body(const DevTree<T>& t, const McArgs& 
     const double rf2 = P.rf * P.rf;
     double fs[kFeatSums] = {0, 0, 0, 0, 0, 0, 0};
     const int off = t.near_off[leaf], cnt = t.near_off[leaf + 1] - off;
-    for (int k = 0; k < cnt; ++k) {
-        const int nb = t.near_ids[off + k];
-        const int code = t.near_img[off + k];
-        const double sx = (code % 3 - 1) * t.L, sy = ((code / 3) % 3 - 1) * t.L, sz = (code / 9 - 1) * t.L;
-        {  // the leaf's box (at its image) must reach into the ball of radius rf
+    // near-list leaves whose box (at its image) reaches into the ball of
+    // radius rf: tested TB entries at a time in parallel, the reaching ones
+    // compacted in list order; then their particles in one flat parallel loop
+    __shared__ int rs0[TB], rn[TB], rpre[TB + 1], wcnt[TB / 64];
+    __shared__ double rsh[TB][3];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int base = 0; base < cnt; base += TB) {
+        const int k = base + threadIdx.x;
+        bool reach = false;
+        int s0 = 0, n = 0;
+        double sx = 0, sy = 0, sz = 0;
+        if (k < cnt) {
+            const int nb = t.near_ids[off + k];
+            const int code = t.near_img[off + k];
+            sx = (code % 3 - 1) * t.L;
+            sy = ((code / 3) % 3 - 1) * t.L;
+            sz = (code / 9 - 1) * t.L;
             const double bs = t.boxsize[t.level[nb]];
             const double lo[3] = {t.cell[3 * nb] * bs + sx, t.cell[3 * nb + 1] * bs + sy, t.cell[3 * nb + 2] * bs + sz};
             const double x[3] = {ox, oy, oz};
@@ -165,16 +177,45 @@ __device__ __forceinline__ void propose_body(const DevTree<T>& t, const McArgs& 
                 const double e = fmax(fmax(lo[c] - x[c], x[c] - (lo[c] + bs)), 0.0);
                 g2 += e * e;
             }
-            if (g2 >= rf2) continue;  // uniform across the block
+            reach = g2 < rf2;
+            if (reach) {
+                s0 = t.slot_start[nb];
+                n = t.slot_count[nb];
+            }
         }
-        const int s0 = t.slot_start[nb], n = t.slot_count[nb];
-        for (int p = threadIdx.x; p < n; p += blockDim.x) {
-            const int s = s0 + p;
+        const unsigned long long bal = __ballot(reach);
+        const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        if (lane == 0) wcnt[wv] = __popcll(bal);
+        __syncthreads();
+        int pos = __popcll(bal & lt);
+        for (int w = 0; w < wv; ++w) pos += wcnt[w];
+        if (reach) {
+            rs0[pos] = s0;
+            rn[pos] = n;
+            rsh[pos][0] = sx;
+            rsh[pos][1] = sy;
+            rsh[pos][2] = sz;
+        }
+        __syncthreads();
+        int nr = 0;
+        for (int w = 0; w < TB / 64; ++w) nr += wcnt[w];
+        if (threadIdx.x == 0) {
+            rpre[0] = 0;
+            for (int r = 0; r < nr; ++r) rpre[r + 1] = rpre[r] + rn[r];
+        }
+        __syncthreads();
+        const int total = rpre[nr];
+        for (int u = threadIdx.x; u < total; u += blockDim.x) {
+            int r = 0;
+            while (rpre[r + 1] <= u) ++r;  // nr is small (the leaves around one point)
+            const int s = rs0[r] + (u - rpre[r]);
             if (s == own || t.porig[s] < 0) continue;
-            const double ex = ox - (t.px[s] + sx), ey = oy - (t.py[s] + sy), ez = oz - (t.pz[s] + sz);
+            const double ex = ox - (t.px[s] + rsh[r][0]), ey = oy - (t.py[s] + rsh[r][1]),
+                         ez = oz - (t.pz[s] + rsh[r][2]);
             const double r2 = ex * ex + ey * ey + ez * ez;
             if (r2 > 0.0 && r2 < rf2) feature_pair(ex, ey, ez, r2, t.pq[s], P.rf, fs);
         }
+        __syncthreads();  // the lists are consumed before the next pass
     }
     block_sum_n<kFeatSums>(fs, redn);
     if (threadIdx.x == 0) {
