@@ -552,14 +552,37 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
         const int own = (hs || ft) ? t.slot_of[ti.i] : -1;
         const double rf2 = a.prop.rf * a.prop.rf;
         double acc = 0;
-        for (int k = r % NB; k < cnt; k += NB) {
-            const int nb = t.near_ids[off + k];
-            const int code = t.near_img[off + k];
-            const double sx = (code % 3 - 1) * t.L, sy = ((code / 3) % 3 - 1) * t.L, sz = (code / 9 - 1) * t.L;
-            const int s0 = t.slot_start[nb], n = t.slot_count[nb];
-            nscan += n;
-            for (int p = threadIdx.x; p < n; p += blockDim.x) {
-                const int s = s0 + p;
+        // this block's near-list entries k = r % NB + NB j, 64 at a time: their
+        // slot ranges are loaded in parallel, then all their particles form
+        // one flat loop (no chain of dependent loads per entry)
+        __shared__ int es0[64], epre[65];
+        __shared__ double esh[64][3];
+        const int nk = cnt > r % NB ? (cnt - r % NB + NB - 1) / NB : 0;
+        for (int j0 = 0; j0 < nk; j0 += 64) {
+            const int ne = min(64, nk - j0);
+            if (threadIdx.x < ne) {
+                const int k = r % NB + NB * (j0 + threadIdx.x);
+                const int nb = t.near_ids[off + k];
+                const int code = t.near_img[off + k];
+                es0[threadIdx.x] = t.slot_start[nb];
+                epre[threadIdx.x + 1] = t.slot_count[nb];
+                esh[threadIdx.x][0] = (code % 3 - 1) * t.L;
+                esh[threadIdx.x][1] = ((code / 3) % 3 - 1) * t.L;
+                esh[threadIdx.x][2] = (code / 9 - 1) * t.L;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                epre[0] = 0;
+                for (int e = 1; e <= ne; ++e) epre[e] += epre[e - 1];
+            }
+            __syncthreads();
+            const int total = epre[ne];
+            nscan += total;
+            int e = 0;
+            for (int u = threadIdx.x; u < total; u += blockDim.x) {
+                while (epre[e + 1] <= u) ++e;
+                const int s = es0[e] + (u - epre[e]);
+                const double sx = esh[e][0], sy = esh[e][1], sz = esh[e][2];
                 const double ex = cx - (t.px[s] + sx), ey = cy - (t.py[s] + sy), ez = cz - (t.pz[s] + sz);
                 const T dx = (T)ex, dy = (T)ey, dz = (T)ez;
                 const T r2 = dx * dx + dy * dy + dz * dz;
@@ -571,6 +594,7 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
                     if (r2d > 0.0 && r2d < rf2) feature_pair(ex, ey, ez, r2d, t.pq[s], a.prop.rf, fsum);
                 }
             }
+            __syncthreads();  // the entry lists are consumed before the next pass
         }
         part = acc * ti.q;
     }
