@@ -290,15 +290,6 @@ __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K) {
         lt_s[m] = ti.path_t[ti.lam_t];
     }
     __syncthreads();
-    // first marker of array idx appends it (front: an incoming array of level 2)
-    auto claim = [&](int idx, bool front) {
-        const unsigned bit = 1u << (idx & 31);
-        if (atomicOr(&bits[idx >> 5], bit) & bit) return;
-        if (front)
-            a.s.dest[atomicAdd(&nfront, 1)] = idx;
-        else
-            a.s.dest[n2 - 1 - atomicAdd(&nback, 1)] = idx;
-    };
     // one thread per (move, endpoint, level): own out, own in, colleagues' in
     for (int w = threadIdx.x; w < 2 * K * nL; w += blockDim.x) {
         const int li = w % nL, me = w / nL, m = me >> 1, ep = me & 1;
@@ -309,14 +300,29 @@ __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K) {
         const int node = ep == 0 ? ti.path_t[l] : ti.path_o[l];
         const int own = t.pw_slot[node];
         const bool front = l == 2;
-        int sl[26];
+        // the 28 candidate arrays (own out, own in, colleagues' in): every bit
+        // is set first (independent LDS atomics in flight together), then the
+        // first markers append
+        int ix[28];
+        ix[0] = 2 * own;
+        ix[1] = t.is_leaf[node] ? -1 : 2 * own + 1;
 #pragma unroll
-        for (int c = 0; c < 26; ++c) sl[c] = t.col_in_slot[26 * node + c];
-        claim(2 * own, false);
-        if (!t.is_leaf[node]) claim(2 * own + 1, front);
+        for (int c = 0; c < 26; ++c) {
+            const int sc = t.col_in_slot[26 * node + c];
+            ix[2 + c] = sc >= 0 ? 2 * sc + 1 : -1;
+        }
+        unsigned old[28];
 #pragma unroll
-        for (int c = 0; c < 26; ++c)
-            if (sl[c] >= 0) claim(2 * sl[c] + 1, front);
+        for (int c = 0; c < 28; ++c)
+            old[c] = ix[c] >= 0 ? atomicOr(&bits[ix[c] >> 5], 1u << (ix[c] & 31)) : ~0u;
+#pragma unroll
+        for (int c = 0; c < 28; ++c) {
+            if (ix[c] < 0 || (old[c] & (1u << (ix[c] & 31)))) continue;
+            if (c > 0 && front)
+                a.s.dest[atomicAdd(&nfront, 1)] = ix[c];
+            else
+                a.s.dest[n2 - 1 - atomicAdd(&nback, 1)] = ix[c];
+        }
     }
     // particles that stay in their leaf move in place, all at once
     for (int m = threadIdx.x; m < K; m += blockDim.x) {
@@ -333,17 +339,36 @@ __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K) {
     if (threadIdx.x == 0) probe_add(a.s, kPrPlanCompact, tp);
     // leaf changes (src/tree.cpp:454-463, 583-596), all in parallel: each mover
     // gets the next free slots of its target leaf in move order
-    for (int m = threadIdx.x; m < K; m += blockDim.x) {
-        int rank = 0, total = 0;
-        const bool mover = ac_s[m] && lo_s[m] != lt_s[m];
-        if (mover)
-            for (int k = 0; k < K; ++k)
-                if (ac_s[k] && lo_s[k] != lt_s[k] && lt_s[k] == lt_s[m]) {
+    if (K > 8 && K <= 64) {  // one wave: lane m compares its target leaf with every lane's by readlane
+        if (threadIdx.x < 64) {
+            const int m = threadIdx.x;
+            const bool mover = m < K && ac_s[m] && lo_s[m] != lt_s[m];
+            const int key = mover ? lt_s[m] : -1;
+            int rank = 0, total = 0;
+#pragma unroll
+            for (int k = 0; k < 64; ++k) {
+                const int kk = __builtin_amdgcn_readlane(key, k);
+                if (mover && kk == key) {
                     rank += k < m;
                     ++total;
                 }
-        conflict[m] = mover ? (rank == 0 ? total : 0) : -1;  // first mover into a leaf carries the count
-        if (mover) relocate(t, a.s.info[m], rank);
+            }
+            if (m < K) conflict[m] = mover ? (rank == 0 ? total : 0) : -1;  // first mover carries the count
+            if (mover) relocate(t, a.s.info[m], rank);
+        }
+    } else {
+        for (int m = threadIdx.x; m < K; m += blockDim.x) {
+            int rank = 0, total = 0;
+            const bool mover = ac_s[m] && lo_s[m] != lt_s[m];
+            if (mover)
+                for (int k = 0; k < K; ++k)
+                    if (ac_s[k] && lo_s[k] != lt_s[k] && lt_s[k] == lt_s[m]) {
+                        rank += k < m;
+                        ++total;
+                    }
+            conflict[m] = mover ? (rank == 0 ? total : 0) : -1;  // first mover into a leaf carries the count
+            if (mover) relocate(t, a.s.info[m], rank);
+        }
     }
     __syncthreads();
     for (int m = threadIdx.x; m < K; m += blockDim.x)
