@@ -171,13 +171,25 @@ def term_flops(d, nz):
     return 4 * d * d * nz + 9 * d * nz
 
 
-def pair_corr_flops(st, n_moves, K, nw, nd):
-    """Algorithmic flops of the pair corrections of a batched run of n_moves
-    in batches of K: four window terms per pair of trials plus the counted
-    difference-level terms (stats column 6)."""
+GRAM_MAX_K = 64  # mixed-precision batches of 2..64: the window pair terms run on the matrix cores (k_trial)
+
+
+def pair_corr_flops(st, n_moves, K, nw, nd, gram):
+    """Algorithmic flops of the pair-correction kernel over a batched run of
+    n_moves in batches of K: the counted difference-level terms (stats column
+    6), plus four window terms per pair of trials unless the window part runs
+    as the Gram product of k_trial (gram)."""
     full, rem = divmod(n_moves, K)
     pairs = full * K * (K - 1) // 2 + rem * (rem - 1) // 2
-    return pairs * 4 * term_flops(2 * nw + 1, nw + 1) + int(st[:, 6].sum()) * term_flops(2 * nd + 1, nd + 1)
+    window = 0 if gram else pairs * 4 * term_flops(2 * nw + 1, nw + 1)
+    return window + int(st[:, 6].sum()) * term_flops(2 * nd + 1, nd + 1)
+
+
+def gram_mflop(K, Mw):
+    """FP32 matrix-core flops of the window Gram product of one batch of K
+    trials: the 16 x 16 tiles i <= j over 2 Mw real components."""
+    nt = (K + 15) // 16
+    return nt * (nt + 1) // 2 * 256 * 2 * (2 * Mw) / 1e6
 
 
 def main():
@@ -407,9 +419,10 @@ def main():
             "update": ("hbm", update_bytes_distinct(st_p, Mw, Md, bc))}
     update_bytes_per_move_sum = accept_bytes(st_p, acc_p, Mw, Md, bc).sum()
     k_batch = tiles["batch"] if (batch == 0 and tiles) else batch  # batch 0: the tuned size
+    gram = prec == P.HPDMK_MIXED and 2 <= k_batch <= GRAM_MAX_K
     if k_batch > 1:
         work["pair_corr"] = ("fp32" if prec == P.HPDMK_MIXED else "fp64",
-                             pair_corr_flops(st_p, prof_moves, k_batch, nw, nd))
+                             pair_corr_flops(st_p, prof_moves, k_batch, nw, nd, gram))
     symbol = {"trial": "k_trial", "update": "k_apply", "pair_corr": "k_pair_corr", "scan": "k_scan",
               "setup": "k_setup", "records": "k_records", "propose": "k_propose"}
     per_kernel = {}
@@ -441,6 +454,7 @@ def main():
                 "traffic": traffic, "work_per_launch": pk["work_per_launch"], "avg_launch_us": pk["avg_launch_us"],
                 "kernel_ms": {k: [round(v[0], 3), v[1]] for k, v in kt.items()}, "per_kernel": per_kernel,
                 "update_bytes_per_launch_per_move_sum": int(update_bytes_per_move_sum / max(kt["update"][1], 1)),
+                "window_gram_mflop_per_launch": round(gram_mflop(k_batch, Mw), 2) if gram else None,
                 "phase_us": phases}
 
     # plain chain (one trial kernel + one update kernel per move).  The timed

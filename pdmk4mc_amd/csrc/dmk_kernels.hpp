@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "dmk_common.hpp"
 #include "dmk_proposal.hpp"
 
@@ -77,6 +79,15 @@ struct TrialRec {
     int pad;
 };
 
+// Window part of the pair corrections as a Gram product on the matrix cores
+// (FP32 tables): extra blocks of k_trial, each over kGramModes window modes
+// in LDS chunks of kGramSW, for batches of 2..kGramMaxK trials.
+constexpr int kGramMaxK = 64;
+constexpr int kGramSW = 16;
+constexpr int kGramChunks = 2;
+constexpr int kGramModes = kGramSW * kGramChunks;
+__host__ __device__ inline int gram_blocks(int Mw) { return (Mw + kGramModes - 1) / kGramModes; }
+
 // device scratch of the Monte Carlo kernels (sized for kMaxBatch trials)
 struct McScratch {
     TrialInfo* info;
@@ -85,6 +96,8 @@ struct McScratch {
     TrialRec* rec;         // per trial of a batch
     int* accept;           // per trial of the current launch
     double* corr;          // kMaxBatch^2
+    double* gram;          // FP32 tables, batches of at most kGramMaxK: window pair terms,
+                           // gram_blocks(Mw) shares per pair (null: the pair blocks form them)
     unsigned char* ovl;    // kMaxBatch^2 hard-sphere flags of pair (i, j): 1 t_j near o_i, 2 t_j near t_i
     int* dest;             // rewritten arrays, 2*slot + kind (0 out, 1 in): level-2 in arrays from
                            // the front, the others from the back (2 n_pw entries)
@@ -139,6 +152,20 @@ struct McArgs {
     Tiles tiles;
     Proposal prop;  // learned proposal (Metropolis and batch modes)
 };
+
+// the batch's window pair terms come from the Gram blocks of k_trial
+template <typename T>
+__host__ __device__ inline bool use_gram(const McArgs& a) {
+    return std::is_same<T, float>::value && a.s.gram != nullptr && a.mode == kTrialBatch && a.K >= 2 &&
+           a.K <= kGramMaxK;
+}
+// rows of Gram blocks in front of the trial rows of k_trial's grid
+template <typename T>
+__host__ __device__ inline int gram_rows(const DevTree<T>& t, const McArgs& a) {
+    if (!use_gram<T>(a)) return 0;
+    const int tb = trial_blocks(t.max_depth, a.tiles);
+    return (gram_blocks(t.Mw) + tb - 1) / tb;
+}
 
 // One Metropolis step of every replica of a set (one move each): per-replica
 // device state and arguments live in device arrays indexed by blockIdx.z.

@@ -502,7 +502,99 @@ __device__ double assemble_trial(const DevTree<T>& t, const McArgs& a, int tr, i
     return dE;
 }
 
-// grid (trial_blocks(D, tiles), K).  Block roles inside a trial (each strides over its share):
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Window part of the pair corrections of a batch on the matrix cores:
+//   G[i][j] = sum_k W_k Re(Phi_j(k) conj Phi_i(k)),
+// Phi = tau - omega of a trial (its root-level phases at the target minus
+// those at the origin, charge in the x-row), over the Mw window modes: the
+// window term of trial j (include/vecops.hpp:118-145) taken against the change
+// an accepted move i makes to the root expansion.  Block g takes modes [kGramModes g, kGramModes
+// (g + 1)) in chunks of kGramSW: each chunk's Phi of every trial goes to LDS
+// as real rows (index r = 2 m + part), and every 16 x 16 tile i <= j is a
+// chain of v_mfma_f32_16x16x4f32 with A[i][r] = P_i(r), B[r][j] = W P_j(r);
+// each chunk's FP32 tile is added into FP64.  gram[(i K + j) G + g] holds
+// block g's share of pair i < j; the pair blocks add the G shares in order.
+__device__ void gram_body(const DevTree<float>& t, const McArgs& a, int g) {
+    constexpr int RS = 2 * kGramSW + 4;  // LDS row stride in floats: 16 rows land 4 banks apart
+    __shared__ float P[kGramMaxK * RS];
+    __shared__ float Wc[kGramSW];
+    const int G = gram_blocks(t.Mw), K = a.K, Ke = keff(a);
+    if (g >= G || Ke < 2) return;  // uniform; no pair without two trials
+    const int stride = table_stride(t.max_depth), dw = t.dw;
+    const int nt = (K + 15) / 16, ntile = nt * (nt + 1) / 2;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, li = lane & 15, kq = lane >> 4;
+    // this wave's tiles: wv, wv + 4, wv + 8 of the upper triangle (it <= jt)
+    int tI[3], tJ[3];
+    bool tv[3];
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+        int idx = wv + 4 * u, it = 0;
+        tv[u] = idx < ntile;
+        while (idx >= nt - it && it < nt) {
+            idx -= nt - it;
+            ++it;
+        }
+        tI[u] = it;
+        tJ[u] = it + idx;
+    }
+    double sum[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+    for (int c = 0; c < kGramChunks; ++c) {
+        const int mb = g * kGramModes + c * kGramSW;
+        if (mb >= t.Mw) break;  // uniform
+        __syncthreads();        // the previous chunk is consumed
+        // every load of the chunk in flight at once: indices clamped into
+        // range, out-of-range entries zeroed after the loads
+#pragma unroll
+        for (int it = 0; it < kGramMaxK * kGramSW / TB; ++it) {
+            const int e = threadIdx.x + it * TB;
+            const int tr = e / kGramSW, mm = e % kGramSW, m = mb + mm;
+            const Cplx<float>* tb = tabs_of<float>(a.s, stride, min(tr, Ke - 1));
+            const int mc = min(m, t.Mw - 1);
+            const Cplx<float> ta = phase_at(tb + root_row(0, 0), tb + root_row(0, 1), tb + root_row(0, 2), dw, mc);
+            const Cplx<float> om = phase_at(tb + root_row(1, 0), tb + root_row(1, 1), tb + root_row(1, 2), dw, mc);
+            const bool ok = tr < Ke && m < t.Mw;
+            if (tr < nt * 16) {
+                P[tr * RS + 2 * mm] = ok ? ta.x - om.x : 0.f;
+                P[tr * RS + 2 * mm + 1] = ok ? ta.y - om.y : 0.f;
+            }
+        }
+        if (threadIdx.x < kGramSW) {
+            const int m = mb + threadIdx.x;
+            Wc[threadIdx.x] = m < t.Mw ? t.Wmat[m] : 0.f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            if (!tv[u]) continue;  // uniform per wave
+            f32x4 d = {0.f, 0.f, 0.f, 0.f};
+            const float* pa = P + (tI[u] * 16 + li) * RS;
+            const float* pb = P + (tJ[u] * 16 + li) * RS;
+#pragma unroll
+            for (int ks = 0; ks < 2 * kGramSW / 4; ++ks) {
+                const int r = 4 * ks + kq;
+                d = __builtin_amdgcn_mfma_f32_16x16x4f32(pa[r], pb[r] * Wc[r >> 1], d, 0, 0, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sum[u][r] += (double)d[r];
+        }
+    }
+    // D layout of v_mfma_f32_16x16x4f32: lane holds rows 4 kq + r of column li
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+        if (!tv[u]) continue;
+        const int gj = tJ[u] * 16 + li;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int gi = tI[u] * 16 + 4 * kq + r;
+            if (gi < gj && gj < Ke) a.s.gram[((size_t)gi * K + gj) * G + g] = sum[u][r];
+        }
+    }
+}
+
+// grid (trial_blocks(D, tiles), gram_rows + K).  With use_gram the first
+// gram_rows rows are Gram blocks (window pair terms, above); then per trial,
+// block roles (each strides over its share):
 //   [0, win)                               window
 //   [.., + 2*nL*diff)                      difference (endpoint, level, share)
 //   [.., + 2*near)                         near field (endpoint, share of the near list)
@@ -512,7 +604,12 @@ template <typename T>
 __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a) {
     __shared__ double red[TB / 64];
     if (*t.err) return;
-    const int tr = blockIdx.y, b = blockIdx.x, m = move_base(a) + tr;
+    const int gy = gram_rows(t, a);
+    if ((int)blockIdx.y < gy) {
+        if constexpr (std::is_same<T, float>::value) gram_body(t, a, blockIdx.y * gridDim.x + blockIdx.x);
+        return;
+    }
+    const int tr = blockIdx.y - gy, b = blockIdx.x, m = move_base(a) + tr;
     if (tr >= keff(a)) return;
     const TrialInfo& ti = a.s.info[tr];
     const int D = t.max_depth, nL = n_levels_pw(D), per = trial_blocks(D, a.tiles);
@@ -773,27 +870,12 @@ __global__ void __launch_bounds__(TB) k_reverse(DevTree<T> t, McArgs a) {
 // and every element of W is loaded once for all NC terms.  rows[6c..6c+5] are
 // term c's ax, ay, az, bx, by, bz.  The factor rows go to shared A/B/C (NC
 // rows of kRowStride each); each thread takes (y, z) rows and returns its
-// share (the caller reduces over the block).
+// share (the caller reduces over the block).  dot_rows is the second phase
+// (A/B/C formed, kRowStride apart per term); separable_dots forms the rows
+// from the six row pointers per term and calls it.
 template <typename T, int NC>
-__device__ __forceinline__ double separable_dots(const T* W, int d, int nz, const Cplx<T>* const* rows, const double* sg,
-                                 Cplx<T>* A, Cplx<T>* B, Cplx<T>* C) {
-    __syncthreads();  // the previous call's A/B/C are consumed
-    const int per = 2 * d + nz;
-#pragma unroll
-    for (int c = 0; c < NC; ++c)  // compile-time term index: the row pointers stay in registers
-        for (int x = threadIdx.x; x < per; x += blockDim.x) {
-            const Cplx<T> u = x < d ? rows[6 * c][x] : x < 2 * d ? rows[6 * c + 1][x - d] : rows[6 * c + 2][x - 2 * d];
-            const Cplx<T> v =
-                x < d ? rows[6 * c + 3][x] : x < 2 * d ? rows[6 * c + 4][x - d] : rows[6 * c + 5][x - 2 * d];
-            const Cplx<T> r = {u.x * v.x + u.y * v.y, u.y * v.x - u.x * v.y};  // u conj(v)
-            if (x < d)
-                A[c * kRowStride + x] = r;
-            else if (x < 2 * d)
-                B[c * kRowStride + x - d] = r;
-            else
-                C[c * kRowStride + x - 2 * d] = r;
-        }
-    __syncthreads();
+__device__ __forceinline__ double dot_rows(const T* W, int d, int nz, const Cplx<T>* A, const Cplx<T>* B,
+                                           const Cplx<T>* C, const double* sg) {
     double acc = 0.0;
     for (int rr = threadIdx.x; rr < d * nz; rr += blockDim.x) {
         const int y = rr % d, z = rr / d;
@@ -833,6 +915,29 @@ __device__ __forceinline__ double separable_dots(const T* W, int d, int nz, cons
     return acc;
 }
 
+template <typename T, int NC>
+__device__ __forceinline__ double separable_dots(const T* W, int d, int nz, const Cplx<T>* const* rows, const double* sg,
+                                 Cplx<T>* A, Cplx<T>* B, Cplx<T>* C) {
+    __syncthreads();  // the previous call's A/B/C are consumed
+    const int per = 2 * d + nz;
+#pragma unroll
+    for (int c = 0; c < NC; ++c)  // compile-time term index: the row pointers stay in registers
+        for (int x = threadIdx.x; x < per; x += blockDim.x) {
+            const Cplx<T> u = x < d ? rows[6 * c][x] : x < 2 * d ? rows[6 * c + 1][x - d] : rows[6 * c + 2][x - 2 * d];
+            const Cplx<T> v =
+                x < d ? rows[6 * c + 3][x] : x < 2 * d ? rows[6 * c + 4][x - d] : rows[6 * c + 5][x - 2 * d];
+            const Cplx<T> r = {u.x * v.x + u.y * v.y, u.y * v.x - u.x * v.y};  // u conj(v)
+            if (x < d)
+                A[c * kRowStride + x] = r;
+            else if (x < 2 * d)
+                B[c * kRowStride + x - d] = r;
+            else
+                C[c * kRowStride + x - 2 * d] = r;
+        }
+    __syncthreads();
+    return dot_rows<T, NC>(W, d, nz, A, B, C, sg);
+}
+
 // One trial's record for the decisions (wave 0 of an extra block of
 // k_pair_corr): partial sums, dE, threshold, target-leaf room.
 template <typename T>
@@ -861,7 +966,7 @@ __device__ void assemble_record(const DevTree<T>& t, const McArgs& a, int j) {
 template <typename T>
 __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
     __shared__ double red[TB / 64];
-    __shared__ Cplx<T> A[4 * kRowStride], B[4 * kRowStride], C[4 * kRowStride];
+    __shared__ Cplx<T> A[8 * kRowStride], B[8 * kRowStride], C[8 * kRowStride];
     const int K = a.K;
     if ((int)blockIdx.x < K) {
         const int j = blockIdx.x;
@@ -881,9 +986,14 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
     const int D = t.max_depth, nL = n_levels_pw(D), stride = table_stride(D);
     const Cplx<T>* ta = tabs_of<T>(a.s, stride, i);
     const Cplx<T>* tj = tabs_of<T>(a.s, stride, j);
-    // window: (tau_j - omega_j) against (tau_i - omega_i), four separable terms
+    // window: (tau_j - omega_j) against (tau_i - omega_i): the Gram blocks'
+    // shares in order, or four separable terms
     double part = 0.0;
-    {
+    if (use_gram<T>(a)) {
+        const int G = gram_blocks(t.Mw);
+        for (int g = threadIdx.x; g < G; g += blockDim.x) part += a.s.gram[((size_t)i * K + j) * G + g];
+        part *= 2.0 * t.cW;
+    } else {
         const Cplx<T>* rows[24];
         double sg[4];
         for (int ej = 0; ej < 2; ++ej)
@@ -897,82 +1007,108 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
             }
         part = separable_dots<T, 4>(t.Wmat, t.dw, t.n_window + 1, rows, sg, A, B, C) * (2.0 * t.cW);
     }
-    // difference levels of j's paths next to i's boxes
-    int ndots = 0;  // level terms formed (counted for the profiled runs)
-    for (int ep = 0; ep < 2; ++ep) {
-        const int lam = ep == 0 ? Bj.lam_t : Bj.lam_o;
-        for (int l = 2; l <= lam - 1; ++l) {
-            const int li = l - 2, nl = 1 << l;
-            const int node = ep == 0 ? Bj.path_t[l] : Bj.path_o[l];
-            int sh[2][3] = {{0, 0, 0}, {0, 0, 0}};
-            bool on[2];
-#pragma unroll
-            for (int w = 0; w < 2; ++w) {  // w = 0: i's target (+), 1: i's origin (-)
-                const int lami = w == 0 ? A_.lam_t : A_.lam_o;
-                on[w] = lami >= l;
-                if (!on[w]) continue;
+    // difference levels of j's paths next to i's boxes.  Term slot
+    // s = 2 (ep nL + li) + w: j's endpoint ep (0 target +, 1 origin -) at level
+    // l = 2 + li against i's endpoint w (0 target +, 1 origin -); on when l is
+    // above j's leaf level, i's box exists at level l and (as the periodic
+    // image shifted by sh) is B or a neighbour of j's box B.  Up to four
+    // (ep, li) groups per pass: one thread per (slot, axis) finds the image
+    // shifts, then every on slot's factor rows are formed at once and the dots
+    // taken group by group (three barriers per pass).
+    __shared__ int shf[8 * 3];  // (slot of the pass, axis): image shift, 2 = off
+    int ndots = 0;              // level terms formed (counted for the profiled runs)
+    const int dd = t.dd, nzd = t.n_diff + 1, perd = 2 * dd + nzd;
+    for (int g0 = 0; g0 < 2 * nL; g0 += 4) {
+        const int ng = min(4, 2 * nL - g0);
+        __syncthreads();  // the previous pass's (or the window's) LDS is consumed
+        if ((int)threadIdx.x < 6 * ng) {
+            const int ax = threadIdx.x % 3, w = (threadIdx.x / 3) & 1, gg = g0 + threadIdx.x / 6;
+            const int ep = gg / nL, l = 2 + gg % nL;
+            const int lam = ep == 0 ? Bj.lam_t : Bj.lam_o;
+            const int lami = w == 0 ? A_.lam_t : A_.lam_o;
+            int v = 2;
+            if (l <= lam - 1 && lami >= l) {
+                const int node = ep == 0 ? Bj.path_t[l] : Bj.path_o[l];
                 const int bi = w == 0 ? A_.path_t[l] : A_.path_o[l];
-#pragma unroll
-                for (int ax = 0; ax < 3; ++ax) {
-                    const int v = image_shift(t.cell[3 * bi + ax], t.cell[3 * node + ax], nl);
-                    sh[w][ax] = v == 2 ? 0 : v;
-                    on[w] = on[w] && v != 2;
-                }
+                v = image_shift(t.cell[3 * bi + ax], t.cell[3 * node + ax], 1 << l);
             }
+            shf[threadIdx.x] = v;
+        }
+        __syncthreads();
+        unsigned on = 0;  // the same in every thread
+        for (int q = 0; q < 2 * ng; ++q)
+            if (shf[3 * q] != 2 && shf[3 * q + 1] != 2 && shf[3 * q + 2] != 2) on |= 1u << q;
+        if (!on) continue;
+        // factor rows u conj(v) of the on slots: u = j's row, v = i's (shifted) row
+        for (int e = threadIdx.x; e < 2 * ng * perd; e += blockDim.x) {
+            const int q = e / perd, x = e % perd;
+            if (!((on >> q) & 1u)) continue;
+            const int gg = g0 + q / 2, w = q & 1, ep = gg / nL, li = gg % nL;
+            const int ax = x < dd ? 0 : x < 2 * dd ? 1 : 2, xi = x - ax * dd;
+            const Cplx<T> u = tj[level_row(nL, ep, li, ax, 0) + xi];
+            const Cplx<T> v = ta[level_row(nL, w, li, ax, shf[3 * q + ax]) + xi];
+            const Cplx<T> r = {u.x * v.x + u.y * v.y, u.y * v.x - u.x * v.y};
+            Cplx<T>* dst = ax == 0 ? A : ax == 1 ? B : C;
+            dst[q * kRowStride + xi] = r;
+        }
+        __syncthreads();
+        for (int gq = 0; gq < ng; ++gq) {
+            const unsigned m2 = (on >> (2 * gq)) & 3u;
+            if (!m2) continue;  // uniform
+            const int gg = g0 + gq, ep = gg / nL, li = gg % nL;
             const T* Dl = t.Dmat + (size_t)li * t.Md;
-            // term c: j's row against i's target (w = 0, +) or origin (w = 1, -)
-            const int w0 = on[0] ? 0 : 1;  // the first term's w; the second (if any) is w = 1
-            const Cplx<T>* rows[12];
-            const double sg[2] = {w0 == 0 ? 1.0 : -1.0, -1.0};
-#pragma unroll
-            for (int ax = 0; ax < 3; ++ax) {
-                rows[ax] = tj + level_row(nL, ep, li, ax, 0);
-                rows[3 + ax] = ta + level_row(nL, w0, li, ax, on[0] ? sh[0][ax] : sh[1][ax]);
-                rows[6 + ax] = rows[ax];
-                rows[9 + ax] = ta + level_row(nL, 1, li, ax, sh[1][ax]);
+            const int o = (2 * gq + (m2 == 2u ? 1 : 0)) * kRowStride;  // first on slot
+            double lv;
+            if (m2 == 3u) {
+                const double sg[2] = {1.0, -1.0};
+                lv = dot_rows<T, 2>(Dl, dd, nzd, A + o, B + o, C + o, sg);
+            } else {
+                const double sg[1] = {m2 == 1u ? 1.0 : -1.0};
+                lv = dot_rows<T, 1>(Dl, dd, nzd, A + o, B + o, C + o, sg);
             }
-            double lv = 0.0;  // branches uniform across the block
-            if (on[0] && on[1])
-                lv = separable_dots<T, 2>(Dl, t.dd, t.n_diff + 1, rows, sg, A, B, C);
-            else if (on[0] || on[1])
-                lv = separable_dots<T, 1>(Dl, t.dd, t.n_diff + 1, rows, sg, A, B, C);
-            ndots += (int)on[0] + (int)on[1];
-            part += lv * 2.0 * t.cD[l] * (ep == 0 ? 1.0 : -1.0);
+            ndots += __popc(m2);
+            part += lv * 2.0 * t.cD[2 + li] * (ep == 0 ? 1.0 : -1.0);
         }
     }
-    double total = block_sum(part, red);
-    if (threadIdx.x == 0) {
-        const T eps = std::numeric_limits<T>::epsilon();
-        int flags = 0;  // hard spheres: 1 t_j within sigma of o_i (counted, now gone), 2 of t_i (new)
-        for (int ep = 0; ep < 2; ++ep) {
-            const int lam = ep == 0 ? Bj.lam_t : Bj.lam_o;
-            const double* xj = ep == 0 ? Bj.t : Bj.o;
-            const double bb = t.boxsize[lam];
-            const T b2 = (T)(bb * bb), inv_b = (T)(1.0 / bb);
-            double s = 0;
-            for (int w = 0; w < 2; ++w) {
-                const double* y = w == 0 ? A_.t : A_.o;
-                double dx = xj[0] - y[0], dy = xj[1] - y[1], dz = xj[2] - y[2];
-                dx -= t.L * rint(dx / t.L);
-                dy -= t.L * rint(dy / t.L);
-                dz -= t.L * rint(dz / t.L);
-                const T fx = (T)dx, fy = (T)dy, fz = (T)dz;
-                const T r2 = fx * fx + fy * fy + fz * fz;
-                if (r2 > eps && r2 < b2)
-                    s += (w == 0 ? 1.0 : -1.0) * (double)residual_poly_over_r<T>(sqrt(r2), inv_b, t.poly, t.n_poly);
-                if (ep == 0 && dx * dx + dy * dy + dz * dz < a.sigma2) flags |= w == 0 ? 2 : 1;
-            }
-            total += (ep == 0 ? 1.0 : -1.0) * s * A_.q * Bj.q;
+    // near field: four threads of the last wave take one (ep, w) pair each
+    // (+-q_i q_j R_b(|x_j - y|), b of x_j's leaf box), a fifth the flags; their
+    // work overlaps the other waves' dots
+    const int q = threadIdx.x - (TB - 64);
+    if (q >= 0 && q < 4) {
+        const int ep = q >> 1, w = q & 1;
+        const double* xj = ep == 0 ? Bj.t : Bj.o;
+        const double* y = w == 0 ? A_.t : A_.o;
+        const double bb = t.boxsize[ep == 0 ? Bj.lam_t : Bj.lam_o];
+        const T b2 = (T)(bb * bb), inv_b = (T)(1.0 / bb);
+        double dx = xj[0] - y[0], dy = xj[1] - y[1], dz = xj[2] - y[2];
+        dx -= t.L * rint(dx / t.L);
+        dy -= t.L * rint(dy / t.L);
+        dz -= t.L * rint(dz / t.L);
+        const T fx = (T)dx, fy = (T)dy, fz = (T)dz;
+        const T r2 = fx * fx + fy * fy + fz * fz;
+        if (r2 > std::numeric_limits<T>::epsilon() && r2 < b2)
+            part += ((ep == w) ? 1.0 : -1.0) * A_.q * Bj.q *
+                    (double)residual_poly_over_r<T>(sqrt(r2), inv_b, t.poly, t.n_poly);
+    } else if (q == 4 && (a.sigma2 > 0.0 || a.prop.learned)) {
+        // hard spheres: 1 t_j within sigma of o_i (counted, now gone), 2 of t_i (new);
+        // learned proposal: 4 j's proposal is stale once i is accepted
+        int flags = 0;
+        if (a.sigma2 > 0.0) {
+            if (within(Bj.t, A_.o, t.L, a.sigma2)) flags |= 1;
+            if (within(Bj.t, A_.t, t.L, a.sigma2)) flags |= 2;
         }
-        if (a.prop.learned) {  // j's proposal is stale once i is accepted: bit 4
+        if (a.prop.learned) {
             const double rf2 = a.prop.rf * a.prop.rf;
             if (A_.i == Bj.i || within(A_.o, Bj.o, t.L, rf2) || within(A_.t, Bj.o, t.L, rf2) ||
                 within(A_.o, Bj.t, t.L, rf2) || within(A_.t, Bj.t, t.L, rf2))
                 flags |= 4;
         }
+        a.s.ovl[i * K + j] = (unsigned char)flags;
+    }
+    const double total = block_sum(part, red);
+    if (threadIdx.x == 0) {
         a.s.corr[i * K + j] = total;
         if (a.ml.stats && ndots) atomicAdd(a.ml.stats + kStatsPerMove * (move_base(a) + j) + 6, ndots);
-        if (a.sigma2 > 0.0 || a.prop.learned) a.s.ovl[i * K + j] = (unsigned char)flags;
     }
 }
 
@@ -1602,7 +1738,8 @@ void launch_setup(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent
 
 template <typename T>
 void launch_trial(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
-    hipExtLaunchKernelGGL(k_trial<T>, dim3(trial_blocks(t.max_depth, a.tiles), a.K), dim3(TB), 0, st, e0, e1, 0, t, a);
+    hipExtLaunchKernelGGL(k_trial<T>, dim3(trial_blocks(t.max_depth, a.tiles), gram_rows(t, a) + a.K), dim3(TB), 0,
+                          st, e0, e1, 0, t, a);
 }
 
 template <typename T>

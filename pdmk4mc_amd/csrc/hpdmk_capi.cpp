@@ -276,6 +276,7 @@ Handle* create_handle(HPDMKParams params, int precision, int n_src, const double
     h->sc.info = dalloc<TrialInfo>(kMaxBatch, o);
     h->sc.accept = dalloc<int>(kMaxBatch, o);
     h->sc.corr = dalloc<double>(kMaxBatch * kMaxBatch, o);
+    h->sc.gram = precision == HPDMK_FP64 ? nullptr : dalloc<double>((size_t)kGramMaxK * kGramMaxK * gram_blocks(Mw), o);
     h->sc.ovl = dalloc<unsigned char>(kMaxBatch * kMaxBatch, o);
     const size_t tsz = precision == HPDMK_FP64 ? sizeof(Cplx<double>) : sizeof(Cplx<float>);
     h->sc.tabs = dalloc<char>(tsz * (size_t)table_stride(D) * kMaxBatch, o);

@@ -118,7 +118,7 @@ def test_metropolis_chain_fp64_matches_oracle(digits):
 
 
 @pytest.mark.parametrize("precision", [P.HPDMK_FP64, P.HPDMK_MIXED])
-@pytest.mark.parametrize("batch", [8, 64])
+@pytest.mark.parametrize("batch", [8, 40, 64, 128])  # mixed, 2..64: window pair terms on the matrix cores
 def test_batched_chain_equals_plain(precision, batch):
     log("batched vs plain", precision, batch)
     r, q, g1 = make(2000, 40, 6, precision, seed=3)
