@@ -513,8 +513,9 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // (g + 1)) in chunks of kGramSW: each chunk's Phi of every trial goes to LDS
 // as real rows (index r = 2 m + part), and every 16 x 16 tile i <= j is a
 // chain of v_mfma_f32_16x16x4f32 with A[i][r] = P_i(r), B[r][j] = W P_j(r);
-// each chunk's FP32 tile is added into FP64.  gram[(i K + j) G + g] holds
-// block g's share of pair i < j; the pair blocks add the G shares in order.
+// each chunk's FP32 tile is added into FP64.  gram[(i G + g) K + j] holds
+// block g's share of pair i < j (a tile row's 16 columns are one 128-byte
+// store); the pair blocks add the G shares in order.
 __device__ void gram_body(const DevTree<float>& t, const McArgs& a, int g) {
     constexpr int RS = 2 * kGramSW + 4;  // LDS row stride in floats: 16 rows land 4 banks apart
     __shared__ float P[kGramMaxK * RS];
@@ -587,7 +588,7 @@ __device__ void gram_body(const DevTree<float>& t, const McArgs& a, int g) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int gi = tI[u] * 16 + 4 * kq + r;
-            if (gi < gj && gj < Ke) a.s.gram[((size_t)gi * K + gj) * G + g] = sum[u][r];
+            if (gi < gj && gj < Ke) a.s.gram[((size_t)gi * G + g) * K + gj] = sum[u][r];
         }
     }
 }
@@ -991,7 +992,7 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
     double part = 0.0;
     if (use_gram<T>(a)) {
         const int G = gram_blocks(t.Mw);
-        for (int g = threadIdx.x; g < G; g += blockDim.x) part += a.s.gram[((size_t)i * K + j) * G + g];
+        for (int g = threadIdx.x; g < G; g += blockDim.x) part += a.s.gram[((size_t)i * G + g) * K + j];
         part *= 2.0 * t.cW;
     } else {
         const Cplx<T>* rows[24];
