@@ -153,6 +153,10 @@ struct McArgs {
     Proposal prop;  // learned proposal (Metropolis and batch modes)
 };
 
+// a batch of at most 64 trials moves its particles in the last block of
+// k_apply (one wave ranks the movers); other steps move them in the plan
+__host__ __device__ inline bool apply_relocates(const McArgs& a) { return a.mode == kTrialBatch && a.K <= 64; }
+
 // the batch's window pair terms come from the Gram blocks of k_trial
 template <typename T>
 __host__ __device__ inline bool use_gram(const McArgs& a) {

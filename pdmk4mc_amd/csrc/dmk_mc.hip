@@ -246,11 +246,9 @@ __global__ void __launch_bounds__(TB) k_propose(DevTree<T> t, McArgs a) {
 // same leaf) and its old slot becomes a hole (q = 0, porig = -1) until the
 // next compaction (k_compact).
 template <typename T>
-__device__ void relocate(const DevTree<T>& t, const TrialInfo& ti, int rank) {
-    const int lt = ti.path_t[ti.lam_t];
+__device__ void relocate_to(const DevTree<T>& t, const TrialInfo& ti, int lt, int ns) {
     const int i = ti.i;
     const int s = t.slot_of[i];
-    const int ns = t.slot_start[lt] + t.slot_count[lt] + rank;
     t.px[ns] = ti.t[0];
     t.py[ns] = ti.t[1];
     t.pz[ns] = ti.t[2];
@@ -264,6 +262,47 @@ __device__ void relocate(const DevTree<T>& t, const TrialInfo& ti, int rank) {
     t.pq[s] = 0;
     t.porig[s] = -1;
 }
+template <typename T>
+__device__ void relocate(const DevTree<T>& t, const TrialInfo& ti, int rank) {
+    const int lt = ti.path_t[ti.lam_t];
+    relocate_to(t, ti, lt, t.slot_start[lt] + t.slot_count[lt] + rank);
+}
+
+// Particle moves of an accepted batch of at most 64 trials (src/tree.cpp:454-463,
+// 583-596), run by one wave: lane m takes trial m; a move inside its leaf
+// rewrites the position in place, a mover gets the next free slot of its
+// target leaf in move order (its rank among the batch's movers into that
+// leaf, by readlane), and the first mover into a leaf adds the count.
+template <typename T>
+__device__ void relocate_batch(const DevTree<T>& t, const McArgs& a) {
+    const int m = threadIdx.x;
+    if (m >= 64) return;
+    const bool in = m < a.K && a.s.accept[m];
+    const TrialInfo& ti = a.s.info[in ? m : 0];
+    const int lo = in ? ti.path_o[ti.lam_o] : -1, lt = in ? ti.path_t[ti.lam_t] : -1;
+    const bool mover = in && lo != lt;
+    const int key = mover ? lt : -1;
+    int rank = 0, total = 0;
+#pragma unroll
+    for (int k = 0; k < 64; ++k) {
+        const int kk = __builtin_amdgcn_readlane(key, k);
+        if (mover && kk == key) {
+            rank += k < m;
+            ++total;
+        }
+    }
+    if (in && !mover) {
+        const int s = t.slot_of[ti.i];
+        t.px[s] = ti.t[0];
+        t.py[s] = ti.t[1];
+        t.pz[s] = ti.t[2];
+    }
+    if (mover) {
+        const int cnt = t.slot_count[lt];  // every mover's load precedes the count's store (data dependence)
+        relocate_to(t, ti, lt, t.slot_start[lt] + cnt + rank);
+        if (rank == 0) t.slot_count[lt] = cnt + total;
+    }
+}
 
 // Marks every expansion array the accepted moves rewrite (out of each path box
 // of levels 2..min(leaf, D-2), in of the non-leaf ones and of their non-leaf
@@ -274,7 +313,7 @@ __device__ void relocate(const DevTree<T>& t, const TrialInfo& ti, int rank) {
 // array is rewritten by its own work items.  Relocates the moved particles
 // between leaf slots.  Run by one whole block.
 template <typename T>
-__device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K) {
+__device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K, bool reloc = true) {
     __shared__ unsigned bits[kMarkWords];
     __shared__ int nfront, nback;
     __shared__ int lo_s[kMaxBatch], lt_s[kMaxBatch], ac_s[kMaxBatch], conflict[kMaxBatch];
@@ -325,7 +364,7 @@ __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K) {
         }
     }
     // particles that stay in their leaf move in place, all at once
-    for (int m = threadIdx.x; m < K; m += blockDim.x) {
+    for (int m = threadIdx.x; m < K && reloc; m += blockDim.x) {
         if (!ac_s[m] || lt_s[m] != lo_s[m]) continue;
         const TrialInfo& ti = a.s.info[m];
         const int s = t.slot_of[ti.i];
@@ -338,8 +377,10 @@ __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K) {
     const int base_sh = nfront, base2_sh = nback;
     if (threadIdx.x == 0) probe_add(a.s, kPrPlanCompact, tp);
     // leaf changes (src/tree.cpp:454-463, 583-596), all in parallel: each mover
-    // gets the next free slots of its target leaf in move order
-    if (K > 8 && K <= 64) {  // one wave: lane m compares its target leaf with every lane's by readlane
+    // gets the next free slots of its target leaf in move order (unless the
+    // update kernel moves the particles: relocate_batch)
+    if (!reloc) {
+    } else if (K > 8 && K <= 64) {  // one wave: lane m compares its target leaf with every lane's by readlane
         if (threadIdx.x < 64) {
             const int m = threadIdx.x;
             const bool mover = m < K && ac_s[m] && lo_s[m] != lt_s[m];
@@ -371,7 +412,7 @@ __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K) {
         }
     }
     __syncthreads();
-    for (int m = threadIdx.x; m < K; m += blockDim.x)
+    for (int m = threadIdx.x; m < K && reloc; m += blockDim.x)
         if (conflict[m] > 0) t.slot_count[lt_s[m]] += conflict[m];
     if (threadIdx.x != 0) return;
     int nacc = 0;
@@ -1353,7 +1394,7 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
         a.s.accept[j] = ok;  // undecided and unused trials apply nothing
     }
     __syncthreads();
-    plan_update(t, a, K);
+    plan_update(t, a, K, !apply_relocates(a));
     if (dyn && threadIdx.x == 0) *a.move_ctr = base + nc;
 }
 
@@ -1478,6 +1519,12 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
     if (*t.err) return;
     const int nD = a.s.plan[0], nacc = a.s.plan[1], nH = a.s.plan[2];
     if (!nacc) return;
+    const bool rb = apply_relocates(a);  // block 0 moves the particles, the others rewrite arrays
+    if (rb && blockIdx.x == 0) {
+        relocate_batch(t, a);
+        return;
+    }
+    const int bid = blockIdx.x - (rb ? 1 : 0), nblk = gridDim.x - (rb ? 1 : 0);
     const int K = a.K, D = t.max_depth, nL = n_levels_pw(D), stride = table_stride(D);
     constexpr int LPR = apply_lpr<T>();  // lanes per grid row
     constexpr int RPI = TB / LPR;        // grid rows per work item
@@ -1493,7 +1540,7 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
     const int sr = 2 * nacc > kApplyHeavy ? kApplySplit : 1;
     const int n_root = nWb * sr;
     const int total = n_root + nD * nDb;
-    if ((int)blockIdx.x >= total) return;  // no work item for this block
+    if (bid >= total) return;  // no work item for this block
     const int n2 = 2 * t.n_pw_nodes;
     for (int m = threadIdx.x; m < K; m += blockDim.x) ac_s[m] = a.s.accept[m];
     for (int w = threadIdx.x; w < 2 * K * kApplyLv; w += blockDim.x) {
@@ -1504,7 +1551,7 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
         pnode[w] = (l <= lam && l <= D - 2) ? (ep == 0 ? ti.path_t[l] : ti.path_o[l]) : -1;
     }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int it = blockIdx.x; it < total; it += gridDim.x) {
+    for (int it = bid; it < total; it += nblk) {
         bool is_root = false;
         int chunk, part_c = 0, nparts = 1, key = 0;
         if (it < n_root) {
@@ -1768,12 +1815,13 @@ template <typename T>
 void launch_apply(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     const int dmax = t.dw > t.dd ? t.dw : t.dd;
     const size_t sh = apply_shmem<T>(dmax, a.K);
+    const int nb = a.tiles.apply + (apply_relocates(a) ? 1 : 0);  // + the particle-move block
     if (apply_xmax<T>(dmax) <= kApplyXSmall<T>) {
         allow_shmem((const void*)k_apply<T, kApplyXSmall<T>>, sh);
-        hipExtLaunchKernelGGL((k_apply<T, kApplyXSmall<T>>), dim3(a.tiles.apply), dim3(TB), sh, st, e0, e1, 0, t, a);
+        hipExtLaunchKernelGGL((k_apply<T, kApplyXSmall<T>>), dim3(nb), dim3(TB), sh, st, e0, e1, 0, t, a);
     } else {
         allow_shmem((const void*)k_apply<T, kApplyXLarge<T>>, sh);
-        hipExtLaunchKernelGGL((k_apply<T, kApplyXLarge<T>>), dim3(a.tiles.apply), dim3(TB), sh, st, e0, e1, 0, t, a);
+        hipExtLaunchKernelGGL((k_apply<T, kApplyXLarge<T>>), dim3(nb), dim3(TB), sh, st, e0, e1, 0, t, a);
     }
 }
 
