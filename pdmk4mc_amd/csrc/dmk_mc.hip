@@ -283,7 +283,7 @@ __device__ void relocate_batch(const DevTree<T>& t, const McArgs& a) {
     const bool mover = in && lo != lt;
     const int key = mover ? lt : -1;
     int rank = 0, total = 0;
-#pragma unroll
+#pragma unroll 2  // few scalar registers live at once (the update kernel's loop keeps its own)
     for (int k = 0; k < 64; ++k) {
         const int kk = __builtin_amdgcn_readlane(key, k);
         if (mover && kk == key) {
@@ -1508,7 +1508,7 @@ __host__ __device__ inline size_t apply_shmem(int dmax, int K) {
            sizeof(int) * 2 * (size_t)K * kApplyLv;
 }
 
-template <typename T, int XMAX>
+template <typename T, int XMAX, bool RB = false>
 __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a) {
     extern __shared__ char apply_dyn[];
     __shared__ int mlist[2 * kMaxBatch];     // contributions: move << 8 | endpoint << 7 | image code
@@ -1519,12 +1519,16 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
     if (*t.err) return;
     const int nD = a.s.plan[0], nacc = a.s.plan[1], nH = a.s.plan[2];
     if (!nacc) return;
-    const bool rb = apply_relocates(a);  // block 0 moves the particles, the others rewrite arrays
-    if (rb && blockIdx.x == 0) {
-        relocate_batch(t, a);
-        return;
+    // RB (a batch, apply_relocates): block 0 moves the particles, the others
+    // rewrite arrays; a separate instantiation, so that the single-move update
+    // compiles without it
+    if constexpr (RB) {
+        if (blockIdx.x == 0) {
+            relocate_batch(t, a);
+            return;
+        }
     }
-    const int bid = blockIdx.x - (rb ? 1 : 0), nblk = gridDim.x - (rb ? 1 : 0);
+    const int bid = blockIdx.x - (RB ? 1 : 0), nblk = gridDim.x - (RB ? 1 : 0);
     const int K = a.K, D = t.max_depth, nL = n_levels_pw(D), stride = table_stride(D);
     constexpr int LPR = apply_lpr<T>();  // lanes per grid row
     constexpr int RPI = TB / LPR;        // grid rows per work item
@@ -1746,9 +1750,9 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
     }
 }
 
-template <typename T, int XMAX>
+template <typename T, int XMAX, bool RB>
 __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
-    apply_body<T, XMAX>(t, a);
+    apply_body<T, XMAX, RB>(t, a);
 }
 
 template <typename T, int XMAX>
@@ -1815,13 +1819,22 @@ template <typename T>
 void launch_apply(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     const int dmax = t.dw > t.dd ? t.dw : t.dd;
     const size_t sh = apply_shmem<T>(dmax, a.K);
-    const int nb = a.tiles.apply + (apply_relocates(a) ? 1 : 0);  // + the particle-move block
+    const bool rb = apply_relocates(a);
+    const int nb = a.tiles.apply + (rb ? 1 : 0);  // + the particle-move block
+    auto go = [&](auto kernel) {
+        allow_shmem((const void*)kernel, sh);
+        hipExtLaunchKernelGGL(kernel, dim3(nb), dim3(TB), sh, st, e0, e1, 0, t, a);
+    };
     if (apply_xmax<T>(dmax) <= kApplyXSmall<T>) {
-        allow_shmem((const void*)k_apply<T, kApplyXSmall<T>>, sh);
-        hipExtLaunchKernelGGL((k_apply<T, kApplyXSmall<T>>), dim3(nb), dim3(TB), sh, st, e0, e1, 0, t, a);
+        if (rb)
+            go(k_apply<T, kApplyXSmall<T>, true>);
+        else
+            go(k_apply<T, kApplyXSmall<T>, false>);
     } else {
-        allow_shmem((const void*)k_apply<T, kApplyXLarge<T>>, sh);
-        hipExtLaunchKernelGGL((k_apply<T, kApplyXLarge<T>>), dim3(nb), dim3(TB), sh, st, e0, e1, 0, t, a);
+        if (rb)
+            go(k_apply<T, kApplyXLarge<T>, true>);
+        else
+            go(k_apply<T, kApplyXLarge<T>, false>);
     }
 }
 
