@@ -1508,7 +1508,7 @@ __host__ __device__ inline size_t apply_shmem(int dmax, int K) {
            sizeof(int) * 2 * (size_t)K * kApplyLv;
 }
 
-template <typename T, int XMAX, bool RB = false>
+template <typename T, int XMAX, bool RB = false, bool MM = true>
 __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a) {
     extern __shared__ char apply_dyn[];
     __shared__ int mlist[2 * kMaxBatch];     // contributions: move << 8 | endpoint << 7 | image code
@@ -1659,6 +1659,26 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
         T ar[XMAX], ai[XMAX];
 #pragma unroll
         for (int x = 0; x < XMAX; ++x) ar[x] = ai[x] = T(0);
+        // FP32 with d <= 32 (MM: not for the replica sets, whose one-move
+        // items have a contribution or two): the item's sums are a complex
+        // product on the matrix cores, S[r][x] = sum_c P_c[r] X_c[x] with
+        // P_c[r] = s_c Y_c[y(r)] Z_c[z(r)] (RPI rows by the contributions, by 32
+        // x values), four v_mfma_f32_16x16x4f32 per complex 16 x 16 x 4 step;
+        // wave wv owns row tiles 2 wv and 2 wv + 1 and both column tiles.
+        constexpr bool MF = MM && std::is_same<T, float>::value && LPR * XMAX <= 32 && RPI == 128;
+        const int li16 = lane & 15, kq = lane >> 4;
+        int oy[2], oz[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int rg = chunk * RPI + (2 * wv + u) * 16 + li16;
+            oy[u] = rg < nrows ? rg % d : 0;
+            oz[u] = rg < nrows ? rg / d : 0;
+        }
+        f32x4 mr[2][2], mi[2][2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct) mr[u][ct] = mi[u][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
         // Row tiles of kApplyTile contributions (3 table rows each, RW lanes
         // per row), double-buffered: the global loads of tile n+1 are in
         // flight while tile n is accumulated.
@@ -1704,21 +1724,53 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
             // share of the row read on into the following rows or the padding
             // after the tiles, and are never written), so the row's LDS reads
             // are issued together instead of one branch and wait per x.
-            for (int k = 0; k < nt; ++k) {
-                const Cplx<T>* Rw = rows + k * 3 * dmax;
-                const T sg = ((mlc[k0 + k] >> 7) & 1) ? T(-1) : T(1);
-                Cplx<T> yz = cmul(Rw[dmax + y], Rw[2 * dmax + z]);
-                yz.x *= sg;
-                yz.y *= sg;
-                Cplx<T> xv[XMAX];
+            if constexpr (MF) {
+                for (int c0 = 0; c0 < nt; c0 += 4) {  // uniform; lane group kq takes contribution c0 + kq
+                    const int k = c0 + kq;
+                    const bool kv = k < nt;
+                    const Cplx<T>* Rw = rows + (kv ? k : 0) * 3 * dmax;
+                    const float sg = (kv ? ((mlc[k0 + k] >> 7) & 1) ? -1.f : 1.f : 0.f);  // 0: past the tile's end
+                    float pr[2], pim[2], xr[2], xim[2];
 #pragma unroll
-                for (int x = 0; x < XMAX; ++x) xv[x] = Rw[x0 + x];
+                    for (int u = 0; u < 2; ++u) {
+                        const Cplx<T> p = cmul(Rw[dmax + oy[u]], Rw[2 * dmax + oz[u]]);
+                        pr[u] = sg * p.x;
+                        pim[u] = sg * p.y;
+                    }
 #pragma unroll
-                for (int x = 0; x < XMAX; ++x) {
-                    ar[x] = fma(xv[x].x, yz.x, ar[x]);
-                    ar[x] = fma(-xv[x].y, yz.y, ar[x]);
-                    ai[x] = fma(xv[x].x, yz.y, ai[x]);
-                    ai[x] = fma(xv[x].y, yz.x, ai[x]);
+                    for (int ct = 0; ct < 2; ++ct) {
+                        const int x = ct * 16 + li16;
+                        const Cplx<T> xv = Rw[x < d ? x : 0];
+                        xr[ct] = x < d ? xv.x : 0.f;
+                        xim[ct] = x < d ? xv.y : 0.f;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 2; ++u)
+#pragma unroll
+                        for (int ct = 0; ct < 2; ++ct) {
+                            mr[u][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[u], xr[ct], mr[u][ct], 0, 0, 0);
+                            mr[u][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(-pim[u], xim[ct], mr[u][ct], 0, 0, 0);
+                            mi[u][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[u], xim[ct], mi[u][ct], 0, 0, 0);
+                            mi[u][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(pim[u], xr[ct], mi[u][ct], 0, 0, 0);
+                        }
+                }
+            } else {
+                for (int k = 0; k < nt; ++k) {
+                    const Cplx<T>* Rw = rows + k * 3 * dmax;
+                    const T sg = ((mlc[k0 + k] >> 7) & 1) ? T(-1) : T(1);
+                    Cplx<T> yz = cmul(Rw[dmax + y], Rw[2 * dmax + z]);
+                    yz.x *= sg;
+                    yz.y *= sg;
+                    Cplx<T> xv[XMAX];
+#pragma unroll
+                    for (int x = 0; x < XMAX; ++x) xv[x] = Rw[x0 + x];
+#pragma unroll
+                    for (int x = 0; x < XMAX; ++x) {
+                        ar[x] = fma(xv[x].x, yz.x, ar[x]);
+                        ar[x] = fma(-xv[x].y, yz.y, ar[x]);
+                        ai[x] = fma(xv[x].x, yz.y, ai[x]);
+                        ai[x] = fma(xv[x].y, yz.x, ai[x]);
+                    }
                 }
             }
             if (more) store_tile(k0 + kApplyTile, rows2 + (buf ^ 1) * kApplyTile * 3 * dmax);
@@ -1726,9 +1778,22 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
         }
         if (threadIdx.x == 0) probe_add(a.s, kPrApplyTiles, tp);
         // sums to LDS in the array's order, then a coalesced write
+        if constexpr (MF) {
+            // D layout of v_mfma_f32_16x16x4f32: lane holds rows 4 kq + q of column li16
 #pragma unroll
-        for (int x = 0; x < XMAX; ++x)
-            if (x < nx) S[lr * d + x0 + x] = {ar[x], ai[x]};
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int lq = (2 * wv + u) * 16 + 4 * kq + q, x = ct * 16 + li16;
+                        if (chunk * RPI + lq < nrows && x < d) S[lq * d + x] = {mr[u][ct][q], mi[u][ct][q]};
+                    }
+        } else {
+#pragma unroll
+            for (int x = 0; x < XMAX; ++x)
+                if (x < nx) S[lr * d + x0 + x] = {ar[x], ai[x]};
+        }
         __syncthreads();
         if (heavy) {
             for (int e = threadIdx.x; e < ne; e += TB) {
@@ -1759,7 +1824,7 @@ template <typename T, int XMAX>
 __global__ void __launch_bounds__(TB) k_apply_rep(const DevTree<T>* __restrict__ ts, const McArgs* __restrict__ as) {
     const DevTree<T>& t = ts[blockIdx.z];
     const McArgs& a = as[blockIdx.z];
-    apply_body<T, XMAX>(t, a);
+    apply_body<T, XMAX, false, false>(t, a);
 }
 
 // grids up to 26 (float) or 28 (double) points per axis use the narrow variant
