@@ -876,23 +876,33 @@ __device__ __forceinline__ bool within(const double* x, const double* y, double 
 // H = log q(-delta | phi(t)) - log q(delta | phi(o)) from the target feature
 // sums of its trial blocks (kernel boundary: no cross-block fences).  One
 // block per trial.
+// Called by every thread of the block; H is stored and returned in thread 0.
 template <typename T>
-__global__ void __launch_bounds__(TB) k_reverse(DevTree<T> t, McArgs a) {
+__device__ double reverse_logratio(const DevTree<T>& t, const McArgs& a, int tr, int m) {
     __shared__ double phi[kFeat], hbuf[2 * kHid], o[kOut], mu[3], sig[3];
     __shared__ float wsh[kMlpParams];
-    if (*t.err) return;
-    const int tr = blockIdx.x, m = move_base(a) + tr;
-    if (tr >= keff(a)) return;
     if (threadIdx.x < 64) {
         double fs[kFeatSums];
         sum_target_features(t, a, tr, threadIdx.x, 64, fs);
         if (threadIdx.x == 0) features_from_sums(a.s.info[tr].q, fs, phi);
     }
     mlp_forward_block(a.prop.w, a.prop, phi, hbuf, o, mu, sig, wsh);
+    double H = 0.0;
     if (threadIdx.x == 0) {
         const double back[3] = {-a.prop.dprop[3 * m], -a.prop.dprop[3 * m + 1], -a.prop.dprop[3 * m + 2]};
-        a.prop.H[m] = gauss_logq(back, mu, sig) - a.prop.logq[m];
+        H = gauss_logq(back, mu, sig) - a.prop.logq[m];
+        a.prop.H[m] = H;
     }
+    return H;
+}
+
+// single-trial steps; batches take H in k_pair_corr's record blocks
+template <typename T>
+__global__ void __launch_bounds__(TB) k_reverse(DevTree<T> t, McArgs a) {
+    if (*t.err) return;
+    const int tr = blockIdx.x, m = move_base(a) + tr;
+    if (tr >= keff(a)) return;
+    reverse_logratio(t, a, tr, m);
 }
 
 // ------------------------------------------------------------------ batched trials
@@ -981,9 +991,10 @@ __device__ __forceinline__ double separable_dots(const T* W, int d, int nz, cons
 }
 
 // One trial's record for the decisions (wave 0 of an extra block of
-// k_pair_corr): partial sums, dE, threshold, target-leaf room.
+// k_pair_corr): partial sums, dE, threshold (H: the learned proposal's
+// log-ratio), target-leaf room.
 template <typename T>
-__device__ void assemble_record(const DevTree<T>& t, const McArgs& a, int j) {
+__device__ void assemble_record(const DevTree<T>& t, const McArgs& a, int j, double H) {
     double v[kAccPerTrial];
     sum_trial_partials(t, a, j, threadIdx.x, 64, v);
     if (threadIdx.x != 0) return;
@@ -996,7 +1007,7 @@ __device__ void assemble_record(const DevTree<T>& t, const McArgs& a, int j) {
     r.leaf_o = tj.path_o[tj.lam_o];
     r.room = r.leaf_t == r.leaf_o ? (1 << 30) : t.slot_cap[r.leaf_t] - t.slot_count[r.leaf_t];
     // learned proposal: lB dE - H / beta < -ln(u) / beta
-    r.thr = mh_threshold(a.ml.u[m], a.beta) + (a.prop.learned ? a.prop.H[m] / a.beta : 0.0);
+    r.thr = mh_threshold(a.ml.u[m], a.beta) + (a.prop.learned ? H / a.beta : 0.0);
     r.pid = tj.i;
     r.pad = 0;
     a.s.rec[j] = r;
@@ -1012,8 +1023,15 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
     const int K = a.K;
     if ((int)blockIdx.x < K) {
         const int j = blockIdx.x;
-        if (*t.err || j >= keff(a) || threadIdx.x >= 64) return;
-        assemble_record(t, a, j);
+        if (*t.err || j >= keff(a)) return;  // uniform
+        __shared__ double Hs;
+        if (a.prop.learned) {  // the reverse-move log-ratio first (the whole block)
+            const double H = reverse_logratio(t, a, j, move_base(a) + j);
+            if (threadIdx.x == 0) Hs = H;
+            __syncthreads();
+        }
+        if (threadIdx.x >= 64) return;
+        assemble_record(t, a, j, a.prop.learned ? Hs : 0.0);
         return;
     }
     int p = blockIdx.x - K, i = 0;

@@ -475,10 +475,12 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
                     timed(h, kKPropose, [&](hipEvent_t x, hipEvent_t y) { launch_propose(dv, a, h->stream, x, y); });
                     timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
                     timed(h, kKTrial, [&](hipEvent_t x, hipEvent_t y) { launch_trial(dv, a, h->stream, x, y); });
-                    timed(h, kKPropose, [&](hipEvent_t x, hipEvent_t y) { launch_reverse(dv, a, h->stream, x, y); });
-                    if (batch > 1)
+                    if (batch > 1)  // its record blocks take the reverse-move log-ratios
                         timed(h, kKCorr,
                               [&](hipEvent_t x, hipEvent_t y) { launch_pair_corr(dv, a, h->stream, x, y); });
+                    else
+                        timed(h, kKPropose,
+                              [&](hipEvent_t x, hipEvent_t y) { launch_reverse(dv, a, h->stream, x, y); });
                     timed(h, kKScan, [&](hipEvent_t x, hipEvent_t y) { launch_scan(dv, a, h->stream, x, y); });
                     timed(h, kKUpdate, [&](hipEvent_t x, hipEvent_t y) { launch_apply(dv, a, h->stream, x, y); });
                     compact_if_due(batch);
