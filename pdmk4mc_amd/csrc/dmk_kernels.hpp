@@ -207,6 +207,10 @@ void launch_reverse(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEve
 template <typename T>
 void launch_setup(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0 = nullptr,
                   hipEvent_t e1 = nullptr);
+// launch_propose then launch_setup, as one kernel
+template <typename T>
+void launch_propose_setup(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0 = nullptr,
+                          hipEvent_t e1 = nullptr);
 template <typename T>
 void launch_trial(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0 = nullptr,
                   hipEvent_t e1 = nullptr);

@@ -240,6 +240,15 @@ __global__ void __launch_bounds__(TB) k_propose(DevTree<T> t, McArgs a) {
     propose_body(t, a);
 }
 
+// the proposal (or the features) and the set-up of each trial in one launch:
+// the set-up's thread 0 reads the displacement the proposal's thread 0 wrote
+template <typename T>
+__global__ void __launch_bounds__(TB) k_propose_setup(DevTree<T> t, McArgs a) {
+    propose_body(t, a);
+    __syncthreads();
+    setup_body(t, a);
+}
+
 // ------------------------------------------------------------------ update plan
 // Moves an accepted particle whose leaf changes: it is appended to the
 // target leaf at slot_count + rank (rank = earlier movers of the batch into the
@@ -1867,6 +1876,11 @@ void launch_propose(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEve
 }
 
 template <typename T>
+void launch_propose_setup(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+    hipExtLaunchKernelGGL(k_propose_setup<T>, dim3(a.K), dim3(TB), 0, st, e0, e1, 0, t, a);
+}
+
+template <typename T>
 void launch_setup(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     hipExtLaunchKernelGGL(k_setup<T>, dim3(a.K), dim3(TB), 0, st, e0, e1, 0, t, a);
 }
@@ -1963,6 +1977,7 @@ void launch_replica_compact(const ReplicaLaunch<T>& r, hipStream_t st) {
     template void launch_reverse<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);   \
     template void launch_decide<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);    \
     template void launch_setup<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);     \
+    template void launch_propose_setup<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t); \
     template void launch_trial<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);     \
     template void launch_pair_corr<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t); \
     template void launch_scan<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);      \

@@ -420,8 +420,10 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
             a.move_ctr = h->step_ctr;
             a.n_moves = n;
             auto one_move = [&] {
-                if (a.prop.rf > 0.0) launch_propose(dv, a, h->stream);
-                launch_setup(dv, a, h->stream);
+                if (a.prop.rf > 0.0)
+                    launch_propose_setup(dv, a, h->stream);
+                else
+                    launch_setup(dv, a, h->stream);
                 launch_trial(dv, a, h->stream);
                 launch_decide(dv, a, h->stream);
                 launch_apply(dv, a, h->stream);
@@ -452,8 +454,10 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
             for (int m = 0; m < n; ++m) {
                 const McArgs a = mc_args(h, mb, m, 1, kTrialMetropolis, beta);
                 if (a.prop.rf > 0.0)
-                    timed(h, kKPropose, [&](hipEvent_t x, hipEvent_t y) { launch_propose(dv, a, h->stream, x, y); });
-                timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
+                    timed(h, kKPropose,
+                          [&](hipEvent_t x, hipEvent_t y) { launch_propose_setup(dv, a, h->stream, x, y); });
+                else
+                    timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
                 timed(h, kKTrial, [&](hipEvent_t x, hipEvent_t y) { launch_trial(dv, a, h->stream, x, y); });
                 timed(h, kKScan, [&](hipEvent_t x, hipEvent_t y) { launch_decide(dv, a, h->stream, x, y); });
                 timed(h, kKUpdate, [&](hipEvent_t x, hipEvent_t y) { launch_apply(dv, a, h->stream, x, y); });
@@ -472,8 +476,8 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
                     McArgs a = mc_args(h, mb, 0, batch, kTrialBatch, beta);
                     a.move_ctr = h->step_ctr;
                     a.n_moves = n;
-                    timed(h, kKPropose, [&](hipEvent_t x, hipEvent_t y) { launch_propose(dv, a, h->stream, x, y); });
-                    timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
+                    timed(h, kKPropose,
+                          [&](hipEvent_t x, hipEvent_t y) { launch_propose_setup(dv, a, h->stream, x, y); });
                     timed(h, kKTrial, [&](hipEvent_t x, hipEvent_t y) { launch_trial(dv, a, h->stream, x, y); });
                     if (batch > 1)  // its record blocks take the reverse-move log-ratios
                         timed(h, kKCorr,
@@ -496,8 +500,10 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
                 const int m0 = bt.first, k = bt.second;
                 const McArgs a = mc_args(h, mb, m0, k, kTrialBatch, beta);
                 if (a.prop.rf > 0.0)  // features only (uniform proposals)
-                    timed(h, kKPropose, [&](hipEvent_t x, hipEvent_t y) { launch_propose(dv, a, h->stream, x, y); });
-                timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
+                    timed(h, kKPropose,
+                          [&](hipEvent_t x, hipEvent_t y) { launch_propose_setup(dv, a, h->stream, x, y); });
+                else
+                    timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
                 timed(h, kKTrial, [&](hipEvent_t x, hipEvent_t y) { launch_trial(dv, a, h->stream, x, y); });
                 if (k > 1)
                     timed(h, kKCorr,
