@@ -1192,7 +1192,10 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
 // radius of its origin or target (its features, hence its proposal density,
 // changed).  That trial starts the next step; the counter advances by the
 // number decided, so the chain is the plain chain.
-template <typename T>
+// HS: hard spheres may be on; LR: a learned proposal or a step counter may
+// be attached.  The walk of the uniform chain without hard spheres compiles
+// without the flag, overlap and staleness bookkeeping.
+template <typename T, bool HS, bool LR>
 __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
     __shared__ double dEs[kMaxBatch], thr[kMaxBatch], dEf[kMaxBatch];
     __shared__ int leafT[kMaxBatch], leafO[kMaxBatch], accs[kMaxBatch], room[kMaxBatch], ov0[kMaxBatch],
@@ -1204,9 +1207,9 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
     const int K = a.K;             // stride of the pair tables
     const int base = move_base(a);
     const int Ke = keff(a);        // trials of this step
-    const bool hs = a.sigma2 > 0.0;
-    const bool dyn = a.move_ctr != nullptr;
-    const bool flg = hs || a.prop.learned;  // pair flags present
+    const bool hs = HS && a.sigma2 > 0.0;
+    const bool dyn = LR && a.move_ctr != nullptr;
+    const bool flg = hs || (LR && a.prop.learned);  // pair flags present
     unsigned long long tp = wall_clock64();
     for (int j = threadIdx.x; j < Ke; j += blockDim.x) {  // records from k_pair_corr
         const TrialRec r = a.s.rec[j];
@@ -1909,7 +1912,15 @@ void launch_pair_corr(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipE
 
 template <typename T>
 void launch_scan(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
-    hipExtLaunchKernelGGL(k_scan<T>, dim3(1), dim3(TB), 0, st, e0, e1, 0, t, a);
+    const bool hs = a.sigma2 > 0.0, lr = a.prop.learned || a.move_ctr != nullptr;
+    if (hs && lr)
+        hipExtLaunchKernelGGL((k_scan<T, true, true>), dim3(1), dim3(TB), 0, st, e0, e1, 0, t, a);
+    else if (hs)
+        hipExtLaunchKernelGGL((k_scan<T, true, false>), dim3(1), dim3(TB), 0, st, e0, e1, 0, t, a);
+    else if (lr)
+        hipExtLaunchKernelGGL((k_scan<T, false, true>), dim3(1), dim3(TB), 0, st, e0, e1, 0, t, a);
+    else
+        hipExtLaunchKernelGGL((k_scan<T, false, false>), dim3(1), dim3(TB), 0, st, e0, e1, 0, t, a);
 }
 
 template <typename T>
