@@ -111,8 +111,9 @@ def test_learned_chain_matches_oracle():
 
 @pytest.mark.parametrize("precision", [P.HPDMK_FP64, P.HPDMK_MIXED])
 @pytest.mark.parametrize("batch", [8, 64])
-def test_batched_learned_chain_equals_plain(precision, batch):
-    log("batched learned", precision, batch)
+@pytest.mark.parametrize("spheres", [True, False])  # the decision scan has a variant for each
+def test_batched_learned_chain_equals_plain(precision, batch, spheres):
+    log("batched learned", precision, batch, spheres)
     r, q, L = hard_sphere_system(2000, 0.2, 1.0, seed=11)
     w = P.mlp_init(7, scale=1.0)
     runs = []
@@ -120,7 +121,8 @@ def test_batched_learned_chain_equals_plain(precision, batch):
         g = P.HPDMKPtTree(P.HPDMKParams(16, 6, L, init=P.DIRECT), r, q, precision)
         g.form_outgoing_pw()
         g.form_incoming_pw()
-        g.mc_set_model(2.0, 1.0)
+        if spheres:
+            g.mc_set_model(2.0, 1.0)
         g.mc_set_proposal(learned=True, weights=w, rf=2.0, **CFG)
         rng = np.random.default_rng(12)
         idx, z, u = rng.integers(0, 2000, 600), rng.normal(size=(600, 3)), rng.uniform(0, 1, 600)
