@@ -479,12 +479,8 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
                     timed(h, kKPropose,
                           [&](hipEvent_t x, hipEvent_t y) { launch_propose_setup(dv, a, h->stream, x, y); });
                     timed(h, kKTrial, [&](hipEvent_t x, hipEvent_t y) { launch_trial(dv, a, h->stream, x, y); });
-                    if (batch > 1)  // its record blocks take the reverse-move log-ratios
-                        timed(h, kKCorr,
-                              [&](hipEvent_t x, hipEvent_t y) { launch_pair_corr(dv, a, h->stream, x, y); });
-                    else
-                        timed(h, kKPropose,
-                              [&](hipEvent_t x, hipEvent_t y) { launch_reverse(dv, a, h->stream, x, y); });
+                    // record blocks (dE, threshold with the reverse-move log-ratio), then the pairs
+                    timed(h, kKCorr, [&](hipEvent_t x, hipEvent_t y) { launch_pair_corr(dv, a, h->stream, x, y); });
                     timed(h, kKScan, [&](hipEvent_t x, hipEvent_t y) { launch_scan(dv, a, h->stream, x, y); });
                     timed(h, kKUpdate, [&](hipEvent_t x, hipEvent_t y) { launch_apply(dv, a, h->stream, x, y); });
                     compact_if_due(batch);
@@ -505,9 +501,8 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
                 else
                     timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
                 timed(h, kKTrial, [&](hipEvent_t x, hipEvent_t y) { launch_trial(dv, a, h->stream, x, y); });
-                if (k > 1)
-                    timed(h, kKCorr,
-                          [&](hipEvent_t x, hipEvent_t y) { launch_pair_corr(dv, a, h->stream, x, y); });
+                // the record blocks run for a batch of one too (k_scan reads its record)
+                timed(h, kKCorr, [&](hipEvent_t x, hipEvent_t y) { launch_pair_corr(dv, a, h->stream, x, y); });
                 timed(h, kKScan, [&](hipEvent_t x, hipEvent_t y) { launch_scan(dv, a, h->stream, x, y); });
                 timed(h, kKUpdate, [&](hipEvent_t x, hipEvent_t y) { launch_apply(dv, a, h->stream, x, y); });
                 compact_if_due(k);

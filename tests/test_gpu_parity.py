@@ -134,6 +134,29 @@ def test_batched_chain_equals_plain(precision, batch):
         assert np.abs(g1.energy_parts() - g2.energy_parts()).max() < 1e-9
 
 
+@pytest.mark.parametrize("precision", [P.HPDMK_FP64, P.HPDMK_MIXED])
+def test_batched_chain_single_trial_batches(precision):
+    """Moves of one particle in a row cut the batch at every repeat: batches
+    of one trial (and a batch size of 1) take the plain chain's decisions."""
+    log("batched single-trial batches", precision)
+    r, q, g1 = make(2000, 40, 6, precision, seed=4)
+    _, _, g2 = make(2000, 40, 6, precision, seed=4)
+    _, _, g3 = make(2000, 40, 6, precision, seed=4)
+    idx, d, u = _moves(2000, 240, 6)
+    idx[1::2] = idx[0::2]  # every particle moves twice in a row
+    idx[100:110] = idx[100]
+    dE1, acc1, _ = g1.mc_chain(idx, d, u, beta=1.0, mode=0)
+    dE2, acc2, _ = g2.mc_chain(idx, d, u, beta=1.0, mode=1, batch=16)
+    dE3, acc3, _ = g3.mc_chain(idx, d, u, beta=1.0, mode=1, batch=1)
+    tol = 1e-9 if precision == P.HPDMK_FP64 else 2e-4
+    for dE, acc in ((dE2, acc2), (dE3, acc3)):
+        assert np.abs(dE1 - dE).max() < tol
+        assert (acc1 == acc).mean() > (0.999 if precision == P.HPDMK_FP64 else 0.99)
+    if precision == P.HPDMK_FP64:
+        assert np.abs(g1.positions() - g2.positions()).max() < 1e-12
+        assert np.abs(g1.positions() - g3.positions()).max() < 1e-12
+
+
 def test_batch_eval_equals_single_eval():
     r, q, g = make(1000, 10, 6)
     idx, d = O.reference_test_moves(1000, L, 10)
