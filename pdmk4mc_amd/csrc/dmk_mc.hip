@@ -1,16 +1,21 @@
 // HIP kernels (gfx950, wave64) of the Monte Carlo step:
 //   k_propose   learned proposal: features and MLP displacement per trial
 //   k_setup     per trial: endpoints, root-to-leaf paths, FP64 phase rows
+//               (k_propose_setup: both in one launch)
 //   k_trial     per trial: partial sums of dE = window + difference + near
-//               field - self (src/energy_shift.cpp:18-302), one per block
+//               field - self (src/energy_shift.cpp:18-302), one per block;
+//               for a batch also the window pair terms (Gram blocks, MFMA)
 //   k_decide    single move: assembles dE, decides, plans the update
-//   k_reverse   learned proposal in a batch: reverse-move log-ratio per trial
-//   k_pair_corr per pair of trials of a batch: exact effect of an accepted
-//               earlier move on a later trial; then one block per trial
-//               assembles its record (dE, threshold, leaf room)
+//   k_reverse   learned proposal, single-trial steps: reverse-move log-ratio
+//   k_pair_corr one block per trial assembles its record (dE, threshold with
+//               the learned proposal's reverse-move log-ratio, leaf room);
+//               then per pair of trials of a batch: exact effect of an
+//               accepted earlier move on a later trial
 //   k_scan      one block: sequential decisions of a batch, update plan
 //   k_apply     accept/update (src/tree.cpp:447-597) over the compact list of
-//               rewritten arrays; every array element is read and written once
+//               rewritten arrays; every array element is read and written
+//               once (FP32 sums on the matrix cores); for a batch, block 0
+//               moves the particles
 //   k_compact   periodic squeeze of the holes moves leave in leaf slots
 // T is the storage/arithmetic type of the bulk (double or float); phase
 // arguments are formed in FP64 and every reduction accumulates in FP64.
