@@ -1027,6 +1027,8 @@ __device__ void assemble_record(const DevTree<T>& t, const McArgs& a, int j, dou
     a.s.rec[j] = r;
 }
 
+constexpr int kPairTB = 128;  // threads per pair-correction block (two waves: twice the resident blocks of four)
+
 // Blocks 0 .. K-1 assemble the trial records (first, so that their chains of
 // dependent loads overlap the pair blocks); the K(K-1)/2 blocks after them
 // form the pair corrections.
@@ -1147,7 +1149,7 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
     // near field: four threads of the last wave take one (ep, w) pair each
     // (+-q_i q_j R_b(|x_j - y|), b of x_j's leaf box), a fifth the flags; their
     // work overlaps the other waves' dots
-    const int q = threadIdx.x - (TB - 64);
+    const int q = threadIdx.x - ((int)blockDim.x - 64);
     if (q >= 0 && q < 4) {
         const int ep = q >> 1, w = q & 1;
         const double* xj = ep == 0 ? Bj.t : Bj.o;
@@ -1912,7 +1914,7 @@ void launch_reverse(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEve
 template <typename T>
 void launch_pair_corr(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     const int nb = a.K + a.K * (a.K - 1) / 2;  // one record block per trial, then the pairs
-    if (nb > 0) hipExtLaunchKernelGGL(k_pair_corr<T>, dim3(nb), dim3(TB), 0, st, e0, e1, 0, t, a);
+    if (nb > 0) hipExtLaunchKernelGGL(k_pair_corr<T>, dim3(nb), dim3(kPairTB), 0, st, e0, e1, 0, t, a);
 }
 
 template <typename T>

@@ -79,7 +79,7 @@ __host__ __device__ inline double gauss_logq(const double* delta, const double* 
     return v;
 }
 
-// FP64 forward pass by all threads of a block (at least 256): the weights are
+// FP64 forward pass by all threads of a block (at least 64): the weights are
 // staged in shared memory (wsh, kMlpParams floats), every unit is computed by
 // 8 lanes (partial dot products joined with shuffles).  phi (8), hbuf
 // (2 kHid) and o (kOut) are shared; on return mu/sig (3 each, shared) are set.
@@ -93,9 +93,9 @@ __device__ inline void mlp_forward_block(const float* w, const Proposal& p, cons
     const float* b2 = W2 + kHid * kHid;
     const float* W3 = b2 + kHid;
     const float* b3 = W3 + kHid * kOut;
-    const int unit = t >> 3, part = t & 7;  // 8 lanes per unit
+    const int part = t & 7, nu = blockDim.x >> 3;  // 8 lanes per unit, units strided by the block
     __syncthreads();
-    if (unit < kHid) {
+    for (int unit = t >> 3; unit < kHid; unit += nu) {
         double s = (double)W1[part * kHid + unit] * phi[part];  // kFeat = 8: one input per lane
         s += __shfl_xor(s, 1);
         s += __shfl_xor(s, 2);
@@ -103,7 +103,7 @@ __device__ inline void mlp_forward_block(const float* w, const Proposal& p, cons
         if (part == 0) hbuf[unit] = tanh(s + (double)b1[unit]);
     }
     __syncthreads();
-    if (unit < kHid) {
+    for (int unit = t >> 3; unit < kHid; unit += nu) {
         double s = 0.0;
         for (int h = part; h < kHid; h += 8) s += (double)W2[h * kHid + unit] * hbuf[h];
         s += __shfl_xor(s, 1);
@@ -112,6 +112,7 @@ __device__ inline void mlp_forward_block(const float* w, const Proposal& p, cons
         if (part == 0) hbuf[kHid + unit] = tanh(s + (double)b2[unit]);
     }
     __syncthreads();
+    const int unit = t >> 3;
     if (unit < kOut) {
         double s = 0.0;
         for (int h = part; h < kHid; h += 8) s += (double)W3[h * kOut + unit] * hbuf[kHid + h];
