@@ -200,10 +200,6 @@ void launch_propose(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEve
 template <typename T>
 void launch_decide(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0 = nullptr,
                    hipEvent_t e1 = nullptr);
-// learned proposal in a batch: the reverse-move log-ratio of every trial
-template <typename T>
-void launch_reverse(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0 = nullptr,
-                    hipEvent_t e1 = nullptr);
 template <typename T>
 void launch_setup(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0 = nullptr,
                   hipEvent_t e1 = nullptr);

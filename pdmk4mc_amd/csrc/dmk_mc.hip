@@ -6,7 +6,6 @@
 //               field - self (src/energy_shift.cpp:18-302), one per block;
 //               for a batch also the window pair terms (Gram blocks, MFMA)
 //   k_decide    single move: assembles dE, decides, plans the update
-//   k_reverse   learned proposal, single-trial steps: reverse-move log-ratio
 //   k_pair_corr one block per trial assembles its record (dE, threshold with
 //               the learned proposal's reverse-move log-ratio, leaf room);
 //               then per pair of trials of a batch: exact effect of an
@@ -886,11 +885,12 @@ __device__ __forceinline__ bool within(const double* x, const double* y, double 
     return s < r2max;
 }
 
-// Learned proposal in a batch: each trial's proposal log-ratio
+// Learned proposal in a batch: trial tr's proposal log-ratio
 // H = log q(-delta | phi(t)) - log q(delta | phi(o)) from the target feature
-// sums of its trial blocks (kernel boundary: no cross-block fences).  One
-// block per trial.
-// Called by every thread of the block; H is stored and returned in thread 0.
+// sums of its trial blocks (read after the trial kernel's boundary: no
+// cross-block fences), by one block (k_pair_corr's record block of the
+// trial).  Called by every thread of the block; H is stored and returned in
+// thread 0.
 template <typename T>
 __device__ double reverse_logratio(const DevTree<T>& t, const McArgs& a, int tr, int m) {
     __shared__ double phi[kFeat], hbuf[2 * kHid], o[kOut], mu[3], sig[3];
@@ -910,14 +910,6 @@ __device__ double reverse_logratio(const DevTree<T>& t, const McArgs& a, int tr,
     return H;
 }
 
-// single-trial steps; batches take H in k_pair_corr's record blocks
-template <typename T>
-__global__ void __launch_bounds__(TB) k_reverse(DevTree<T> t, McArgs a) {
-    if (*t.err) return;
-    const int tr = blockIdx.x, m = move_base(a) + tr;
-    if (tr >= keff(a)) return;
-    reverse_logratio(t, a, tr, m);
-}
 
 // ------------------------------------------------------------------ batched trials
 // Effect of accepted move i on trial j (i < j, distinct particles): exactly
@@ -1907,11 +1899,6 @@ void launch_decide(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEven
 }
 
 template <typename T>
-void launch_reverse(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
-    hipExtLaunchKernelGGL(k_reverse<T>, dim3(a.K), dim3(TB), 0, st, e0, e1, 0, t, a);
-}
-
-template <typename T>
 void launch_pair_corr(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     const int nb = a.K + a.K * (a.K - 1) / 2;  // one record block per trial, then the pairs
     if (nb > 0) hipExtLaunchKernelGGL(k_pair_corr<T>, dim3(nb), dim3(kPairTB), 0, st, e0, e1, 0, t, a);
@@ -1992,7 +1979,6 @@ void launch_replica_compact(const ReplicaLaunch<T>& r, hipStream_t st) {
 #define INSTANTIATE(T)                                                                \
     template void launch_compact<T>(const DevTree<T>&, const int*, int, hipStream_t); \
     template void launch_propose<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);   \
-    template void launch_reverse<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);   \
     template void launch_decide<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);    \
     template void launch_setup<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);     \
     template void launch_propose_setup<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t); \
