@@ -192,10 +192,6 @@ void launch_advance(int* move_ctr, int n, hipStream_t st);
 template <typename T>
 void launch_replica_compact(const ReplicaLaunch<T>& r, hipStream_t st);
 
-// features at the origins and, for the learned proposal, the displacements
-template <typename T>
-void launch_propose(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0 = nullptr,
-                    hipEvent_t e1 = nullptr);
 // single moves and trial-only batches: dE, decision, update plan
 template <typename T>
 void launch_decide(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0 = nullptr,
@@ -203,7 +199,8 @@ void launch_decide(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEven
 template <typename T>
 void launch_setup(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0 = nullptr,
                   hipEvent_t e1 = nullptr);
-// launch_propose then launch_setup, as one kernel
+// features at the origins and, for the learned proposal, the displacements;
+// then the set-up of launch_setup, in one kernel
 template <typename T>
 void launch_propose_setup(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0 = nullptr,
                           hipEvent_t e1 = nullptr);

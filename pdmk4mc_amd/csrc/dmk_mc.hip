@@ -1,7 +1,7 @@
 // HIP kernels (gfx950, wave64) of the Monte Carlo step:
-//   k_propose   learned proposal: features and MLP displacement per trial
 //   k_setup     per trial: endpoints, root-to-leaf paths, FP64 phase rows
-//               (k_propose_setup: both in one launch)
+//   k_propose_setup  the same after the proposal: features at the origin and,
+//               learned, the MLP displacement
 //   k_trial     per trial: partial sums of dE = window + difference + near
 //               field - self (src/energy_shift.cpp:18-302), one per block;
 //               for a batch also the window pair terms (Gram blocks, MFMA)
@@ -237,11 +237,6 @@ __device__ __forceinline__ void propose_body(const DevTree<T>& t, const McArgs& 
         }
         P.logq[m] = gauss_logq(dl, mu, sig);
     }
-}
-
-template <typename T>
-__global__ void __launch_bounds__(TB) k_propose(DevTree<T> t, McArgs a) {
-    propose_body(t, a);
 }
 
 // the proposal (or the features) and the set-up of each trial in one launch:
@@ -1873,11 +1868,6 @@ inline void allow_shmem(const void* kernel, size_t bytes) {
 
 // e0/e1 (may be null) take the kernel's own start and end timestamps
 template <typename T>
-void launch_propose(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
-    hipExtLaunchKernelGGL(k_propose<T>, dim3(a.K), dim3(TB), 0, st, e0, e1, 0, t, a);
-}
-
-template <typename T>
 void launch_propose_setup(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     hipExtLaunchKernelGGL(k_propose_setup<T>, dim3(a.K), dim3(TB), 0, st, e0, e1, 0, t, a);
 }
@@ -1978,7 +1968,6 @@ void launch_replica_compact(const ReplicaLaunch<T>& r, hipStream_t st) {
 
 #define INSTANTIATE(T)                                                                \
     template void launch_compact<T>(const DevTree<T>&, const int*, int, hipStream_t); \
-    template void launch_propose<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);   \
     template void launch_decide<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);    \
     template void launch_setup<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);     \
     template void launch_propose_setup<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t); \
