@@ -135,6 +135,23 @@ def test_batched_chain_equals_plain(precision, batch):
 
 
 @pytest.mark.parametrize("precision", [P.HPDMK_FP64, P.HPDMK_MIXED])
+def test_batched_chain_equals_plain_digits3(precision):
+    """Digits 3: smaller window and difference grids (the Gram blocks' mode
+    count, the update's matrix-core tiles) on the same chain test."""
+    log("batched vs plain, digits 3", precision)
+    r, q, g1 = make(2000, 40, 3, precision, seed=5)
+    _, _, g2 = make(2000, 40, 3, precision, seed=5)
+    idx, d, u = _moves(2000, 400, 7)
+    dE1, acc1, _ = g1.mc_chain(idx, d, u, beta=1.0, mode=0)
+    dE2, acc2, _ = g2.mc_chain(idx, d, u, beta=1.0, mode=1, batch=64)
+    tol = 1e-9 if precision == P.HPDMK_FP64 else 2e-4
+    assert np.abs(dE1 - dE2).max() < tol
+    assert (acc1 == acc2).mean() > (0.999 if precision == P.HPDMK_FP64 else 0.99)
+    if precision == P.HPDMK_FP64:
+        assert np.abs(g1.positions() - g2.positions()).max() < 1e-12
+
+
+@pytest.mark.parametrize("precision", [P.HPDMK_FP64, P.HPDMK_MIXED])
 def test_batched_chain_single_trial_batches(precision):
     """Moves of one particle in a row cut the batch at every repeat: batches
     of one trial (and a batch size of 1) take the plain chain's decisions."""
