@@ -1195,7 +1195,8 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
     __shared__ int leafT[kMaxBatch], leafO[kMaxBatch], accs[kMaxBatch], room[kMaxBatch], ov0[kMaxBatch],
         pids[kMaxBatch];
     __shared__ double C[64 * 64];
-    __shared__ unsigned char F[64 * 64];
+    __shared__ unsigned Fw[64 * 64 / 4];  // pair flags, read as words
+    unsigned char* F = reinterpret_cast<unsigned char*>(Fw);
     __shared__ int cons;
     if (*t.err) return;
     const int K = a.K;             // stride of the pair tables
@@ -1217,19 +1218,32 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
     }
     const bool in_lds = K <= 64;
     if (in_lds) {
+        // the corrections and (four bytes per load) the pair flags: all of a
+        // thread's loads in flight together, then the LDS stores
+        const unsigned* ow = reinterpret_cast<const unsigned*>(a.s.ovl);
+        const int nw = flg ? (K * K + 3) / 4 : 0;  // <= 1024; the last word stays inside the kMaxBatch^2 flags
         double v[16];
+        unsigned fv[4];
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int q = threadIdx.x + r * TB;
             if (q < K * K) v[r] = a.s.corr[q];
         }
 #pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int q = threadIdx.x + r * TB;
+            if (q < nw) fv[r] = ow[q];
+        }
+#pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int q = threadIdx.x + r * TB;
             if (q < K * K) C[q] = v[r];
         }
-        if (flg)
-            for (int q = threadIdx.x; q < K * K; q += blockDim.x) F[q] = a.s.ovl[q];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int q = threadIdx.x + r * TB;
+            if (q < nw) Fw[q] = fv[r];
+        }
     }
     if (threadIdx.x == 0) cons = Ke > 0 ? Ke : 0;
     __syncthreads();
