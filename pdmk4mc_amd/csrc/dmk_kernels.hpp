@@ -153,6 +153,13 @@ struct McArgs {
     Proposal prop;  // learned proposal (Metropolis and batch modes)
 };
 
+// single-move steps of launch_trial/launch_decide (plain chain,
+// update_shift): the trial kernel's last row of blocks marks the update plan,
+// the decision moves the particle
+__host__ __device__ inline bool premark(const McArgs& a) {
+    return a.K == 1 && (a.mode == kTrialMetropolis || a.mode == kTrialForceAccept);
+}
+
 // a batch of at most 64 trials moves its particles in the last block of
 // k_apply (one wave ranks the movers); other steps move them in the plan
 __host__ __device__ inline bool apply_relocates(const McArgs& a) { return a.mode == kTrialBatch && a.K <= 64; }

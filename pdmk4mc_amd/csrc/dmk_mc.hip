@@ -4,8 +4,9 @@
 //               learned, the MLP displacement
 //   k_trial     per trial: partial sums of dE = window + difference + near
 //               field - self (src/energy_shift.cpp:18-302), one per block;
-//               for a batch also the window pair terms (Gram blocks, MFMA)
-//   k_decide    single move: assembles dE, decides, plans the update
+//               for a batch also the window pair terms (Gram blocks, MFMA);
+//               for a single move also the update plan's marking (last row)
+//   k_decide    single move: assembles dE, decides, moves the particle
 //   k_pair_corr one block per trial assembles its record (dE, threshold with
 //               the learned proposal's reverse-move log-ratio, leaf room);
 //               then per pair of trials of a batch: exact effect of an
@@ -312,23 +313,103 @@ __device__ void relocate_batch(const DevTree<T>& t, const McArgs& a) {
     }
 }
 
-// Marks every expansion array the accepted moves rewrite (out of each path box
-// of levels 2..min(leaf, D-2), in of the non-leaf ones and of their non-leaf
-// colleagues) in a shared-memory bitmap; the first marker of an array appends
-// it to s.dest: the incoming arrays of level 2 (each neighbours a quarter of
-// the level's boxes, so a batch's moves pile up there) from the front, the
-// others from the back.  The order inside each group is immaterial: every
-// array is rewritten by its own work items.  Relocates the moved particles
-// between leaf slots.  Run by one whole block.
+// Marks the arrays one accepted endpoint rewrites at level l (out of its box,
+// in of the box when it is not a leaf and of its non-leaf colleagues) in the
+// shared-memory bitmap; the first marker of an array appends it to s.dest: the
+// incoming arrays of level 2 (each neighbours a quarter of the level's boxes,
+// so a batch's moves pile up there) from the front, the others from the back.
+// The order inside each group is immaterial: every array is rewritten by its
+// own work items.
 template <typename T>
+__device__ __forceinline__ void mark_arrays(const DevTree<T>& t, const McArgs& a, const TrialInfo& ti, int ep, int l,
+                                            unsigned* bits, int* nfront, int* nback, int n2) {
+    if (l > (ep == 0 ? ti.lam_t : ti.lam_o)) return;
+    const int node = ep == 0 ? ti.path_t[l] : ti.path_o[l];
+    const int own = t.pw_slot[node];
+    const bool front = l == 2;
+    // the 28 candidate arrays (own out, own in, colleagues' in): every bit is
+    // set first (independent LDS atomics in flight together), then the first
+    // markers append
+    int ix[28];
+    ix[0] = 2 * own;
+    ix[1] = t.is_leaf[node] ? -1 : 2 * own + 1;
+#pragma unroll
+    for (int c = 0; c < 26; ++c) {
+        const int sc = t.col_in_slot[26 * node + c];
+        ix[2 + c] = sc >= 0 ? 2 * sc + 1 : -1;
+    }
+    unsigned old[28];
+#pragma unroll
+    for (int c = 0; c < 28; ++c) old[c] = ix[c] >= 0 ? atomicOr(&bits[ix[c] >> 5], 1u << (ix[c] & 31)) : ~0u;
+#pragma unroll
+    for (int c = 0; c < 28; ++c) {
+        if (ix[c] < 0 || (old[c] & (1u << (ix[c] & 31)))) continue;
+        if (c > 0 && front)
+            a.s.dest[atomicAdd(nfront, 1)] = ix[c];
+        else
+            a.s.dest[n2 - 1 - atomicAdd(nback, 1)] = ix[c];
+    }
+}
+
+// Single-move steps (premark): the update plan's arrays, marked by the trial
+// kernel's last row before the decision as if the move were accepted (they
+// depend on the two paths only); a rejection leaves the accepted count at
+// zero.  One block; the bitmap is the launch's dynamic shared memory,
+// (2 n_pw + 31) / 32 words, so that the trial blocks keep their occupancy.
+template <typename T>
+__device__ void mark_single(const DevTree<T>& t, const McArgs& a) {
+    extern __shared__ unsigned mark_bits[];
+    __shared__ int nfront, nback;
+    const int nL = n_levels_pw(t.max_depth), n2 = 2 * t.n_pw_nodes;
+    unsigned long long tp = wall_clock64();
+    for (int w = threadIdx.x; w < (n2 + 31) / 32; w += blockDim.x) mark_bits[w] = 0u;
+    if (threadIdx.x == 0) nfront = nback = 0;
+    __syncthreads();
+    // one thread per (endpoint, level, candidate array): few registers in the
+    // trial kernel, every candidate's loads in flight together
+    const TrialInfo& ti = a.s.info[0];
+    for (int w = threadIdx.x; w < 2 * nL * 28; w += blockDim.x) {
+        const int c = w % 28, el = w / 28, ep = el / nL, l = 2 + el % nL;
+        if (l > (ep == 0 ? ti.lam_t : ti.lam_o)) continue;
+        const int node = ep == 0 ? ti.path_t[l] : ti.path_o[l];
+        int ix;
+        if (c == 0) {
+            ix = 2 * t.pw_slot[node];
+        } else if (c == 1) {
+            ix = t.is_leaf[node] ? -1 : 2 * t.pw_slot[node] + 1;
+        } else {
+            const int sc = t.col_in_slot[26 * node + c - 2];
+            ix = sc >= 0 ? 2 * sc + 1 : -1;
+        }
+        if (ix < 0 || (atomicOr(&mark_bits[ix >> 5], 1u << (ix & 31)) & (1u << (ix & 31)))) continue;
+        if (c > 0 && l == 2)
+            a.s.dest[atomicAdd(&nfront, 1)] = ix;
+        else
+            a.s.dest[n2 - 1 - atomicAdd(&nback, 1)] = ix;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a.s.plan[0] = nfront + nback;
+        a.s.plan[2] = nfront;
+        probe_add(a.s, kPrPlanMark, tp);
+    }
+}
+
+// The update plan of a step: marks every array the accepted moves rewrite
+// (mark_arrays) and relocates the moved particles between leaf slots.  Run by
+// one whole block.  MARKED: mark_single already marked the arrays (and wrote
+// their counts); this block only moves the particle and writes the accepted
+// count.
+template <typename T, bool MARKED = false>
 __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K, bool reloc = true) {
-    __shared__ unsigned bits[kMarkWords];
+    __shared__ unsigned bits[MARKED ? 1 : kMarkWords];
     __shared__ int nfront, nback;
     __shared__ int lo_s[kMaxBatch], lt_s[kMaxBatch], ac_s[kMaxBatch], conflict[kMaxBatch];
     const int D = t.max_depth, nL = n_levels_pw(D);
     const int n2 = 2 * t.n_pw_nodes;  // at most 32 kMarkWords (checked at set-up)
     unsigned long long tp = wall_clock64();
-    for (int w = threadIdx.x; w < (n2 + 31) / 32; w += blockDim.x) bits[w] = 0u;
+    if (!MARKED)
+        for (int w = threadIdx.x; w < (n2 + 31) / 32; w += blockDim.x) bits[w] = 0u;
     if (threadIdx.x == 0) nfront = nback = 0;
     for (int m = threadIdx.x; m < K; m += blockDim.x) {
         const TrialInfo& ti = a.s.info[m];
@@ -338,38 +419,9 @@ __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K, bool re
     }
     __syncthreads();
     // one thread per (move, endpoint, level): own out, own in, colleagues' in
-    for (int w = threadIdx.x; w < 2 * K * nL; w += blockDim.x) {
+    for (int w = threadIdx.x; w < (MARKED ? 0 : 2 * K * nL); w += blockDim.x) {
         const int li = w % nL, me = w / nL, m = me >> 1, ep = me & 1;
-        if (!ac_s[m]) continue;
-        const TrialInfo& ti = a.s.info[m];
-        const int l = 2 + li;
-        if (l > (ep == 0 ? ti.lam_t : ti.lam_o)) continue;
-        const int node = ep == 0 ? ti.path_t[l] : ti.path_o[l];
-        const int own = t.pw_slot[node];
-        const bool front = l == 2;
-        // the 28 candidate arrays (own out, own in, colleagues' in): every bit
-        // is set first (independent LDS atomics in flight together), then the
-        // first markers append
-        int ix[28];
-        ix[0] = 2 * own;
-        ix[1] = t.is_leaf[node] ? -1 : 2 * own + 1;
-#pragma unroll
-        for (int c = 0; c < 26; ++c) {
-            const int sc = t.col_in_slot[26 * node + c];
-            ix[2 + c] = sc >= 0 ? 2 * sc + 1 : -1;
-        }
-        unsigned old[28];
-#pragma unroll
-        for (int c = 0; c < 28; ++c)
-            old[c] = ix[c] >= 0 ? atomicOr(&bits[ix[c] >> 5], 1u << (ix[c] & 31)) : ~0u;
-#pragma unroll
-        for (int c = 0; c < 28; ++c) {
-            if (ix[c] < 0 || (old[c] & (1u << (ix[c] & 31)))) continue;
-            if (c > 0 && front)
-                a.s.dest[atomicAdd(&nfront, 1)] = ix[c];
-            else
-                a.s.dest[n2 - 1 - atomicAdd(&nback, 1)] = ix[c];
-        }
+        if (ac_s[m]) mark_arrays(t, a, a.s.info[m], ep, 2 + li, bits, &nfront, &nback, n2);
     }
     // particles that stay in their leaf move in place, all at once
     for (int m = threadIdx.x; m < K && reloc; m += blockDim.x) {
@@ -381,8 +433,9 @@ __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K, bool re
         t.pz[s] = ti.t[2];
     }
     __syncthreads();
-    if (threadIdx.x == 0) probe_add(a.s, kPrPlanMark, tp);
-    const int base_sh = nfront, base2_sh = nback;
+    if (threadIdx.x == 0 && !MARKED) probe_add(a.s, kPrPlanMark, tp);
+    const int base_sh = MARKED ? a.s.plan[2] : nfront;
+    const int base2_sh = MARKED ? a.s.plan[0] - base_sh : nback;
     if (threadIdx.x == 0) probe_add(a.s, kPrPlanCompact, tp);
     // leaf changes (src/tree.cpp:454-463, 583-596), all in parallel: each mover
     // gets the next free slots of its target leaf in move order (unless the
@@ -425,9 +478,11 @@ __device__ void plan_update(const DevTree<T>& t, const McArgs& a, int K, bool re
     if (threadIdx.x != 0) return;
     int nacc = 0;
     for (int m = 0; m < K; ++m) nacc += ac_s[m] != 0;
-    a.s.plan[0] = base_sh + base2_sh;
+    if (!MARKED) {
+        a.s.plan[0] = base_sh + base2_sh;
+        a.s.plan[2] = base_sh;
+    }
     a.s.plan[1] = nacc;
-    a.s.plan[2] = base_sh;
     if (a.ml.stats && nacc) a.ml.stats[kStatsPerMove * move_base(a) + 7] = base_sh + base2_sh;
     probe_add(a.s, kPrPlanRelocate, tp);
 }
@@ -650,7 +705,7 @@ __device__ void gram_body(const DevTree<float>& t, const McArgs& a, int g) {
 //   [.., + 2*near)                         near field (endpoint, share of the near list)
 // Every block stores its partial sums; k_decide (single moves) or k_scan
 // (batches) assembles them after the kernel ends.
-template <typename T>
+template <typename T, bool PM = false>
 __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a) {
     __shared__ double red[TB / 64];
     if (*t.err) return;
@@ -658,6 +713,12 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
     if ((int)blockIdx.y < gy) {
         if constexpr (std::is_same<T, float>::value) gram_body(t, a, blockIdx.y * gridDim.x + blockIdx.x);
         return;
+    }
+    if constexpr (PM) {  // premark(a): the update plan's marking, before the decision
+        if ((int)blockIdx.y == gy + a.K) {
+            if (blockIdx.x == 0 && keff(a) > 0) mark_single(t, a);
+            return;
+        }
     }
     const int tr = blockIdx.y - gy, b = blockIdx.x, m = move_base(a) + tr;
     if (tr >= keff(a)) return;
@@ -791,7 +852,7 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
 // from the partials of its trial blocks (kernel boundary: no cross-block
 // fences), adds the learned proposal's log-ratio, takes the Metropolis
 // decision and, for a single move, plans the update.  One block per trial.
-template <typename T>
+template <typename T, bool PM>
 __device__ __forceinline__ void decide_body(const DevTree<T>& t, const McArgs& a) {
     __shared__ double sh_dE, phi[kFeat], hbuf[2 * kHid], o[kOut], mu[3], sig[3];
     __shared__ float wsh[kMlpParams];
@@ -840,23 +901,29 @@ __device__ __forceinline__ void decide_body(const DevTree<T>& t, const McArgs& a
     }
     if (a.mode == kTrialOnly) return;
     __syncthreads();
-    plan_update(t, a, 1);  // single-move step: this block plans the update
+    // single-move step: this block plans the update; with PM (launch_trial
+    // added the marking row) the trial kernel marked the arrays and this block
+    // moves the particle and writes the accepted count
+    if (PM && premark(a))
+        plan_update<T, true>(t, a, 1);
+    else
+        plan_update(t, a, 1);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(TB) k_decide(DevTree<T> t, McArgs a) {
-    decide_body(t, a);
+    decide_body<T, true>(t, a);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(TB) k_decide_rep(const DevTree<T>* __restrict__ ts,
                                                    const McArgs* __restrict__ as) {
-    decide_body(ts[blockIdx.z], as[blockIdx.z]);
+    decide_body<T, false>(ts[blockIdx.z], as[blockIdx.z]);
 }
 
-template <typename T>
+template <typename T, bool PM>  // PM: with the marking row (single-move steps)
 __global__ void __launch_bounds__(TB) k_trial(DevTree<T> t, McArgs a) {
-    trial_body(t, a);
+    trial_body<T, PM>(t, a);
 }
 
 // replica blockIdx.z; the grid is sized for the deepest replica and the
@@ -1893,8 +1960,15 @@ void launch_setup(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent
 
 template <typename T>
 void launch_trial(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
-    hipExtLaunchKernelGGL(k_trial<T>, dim3(trial_blocks(t.max_depth, a.tiles), gram_rows(t, a) + a.K), dim3(TB), 0,
-                          st, e0, e1, 0, t, a);
+    // Gram rows, trial rows, the marking row (premark: its bitmap is the dynamic shared memory)
+    const bool pm = premark(a);
+    const int gy = gram_rows(t, a) + a.K + (pm ? 1 : 0);
+    const size_t sh = pm ? sizeof(unsigned) * ((2 * (size_t)t.n_pw_nodes + 31) / 32) : 0;
+    const dim3 grid(trial_blocks(t.max_depth, a.tiles), gy);
+    if (pm)
+        hipExtLaunchKernelGGL((k_trial<T, true>), grid, dim3(TB), sh, st, e0, e1, 0, t, a);
+    else
+        hipExtLaunchKernelGGL((k_trial<T, false>), grid, dim3(TB), 0, st, e0, e1, 0, t, a);
 }
 
 template <typename T>
@@ -1953,6 +2027,8 @@ template <typename T>
 void launch_replica_step(const ReplicaLaunch<T>& r, hipStream_t st) {
     const dim3 rep(1, 1, r.n);
     hipLaunchKernelGGL(k_setup_rep<T>, rep, dim3(TB), 0, st, r.trees, r.args);
+    // one trial row: with 256 replicas the trial kernel fills the chip, so the
+    // decision plans the whole update (a marking row measured slower)
     hipLaunchKernelGGL(k_trial_rep<T>, dim3(trial_blocks(r.max_depth, r.tiles), 1, r.n), dim3(TB), 0, st, r.trees,
                        r.args);
     hipLaunchKernelGGL(k_decide_rep<T>, rep, dim3(TB), 0, st, r.trees, r.args);
