@@ -93,6 +93,7 @@ Handle* create_handle(HPDMKParams params, int precision, int n_src, const double
     } catch (const HpdmkError& e) {
         throw HpdmkError(HPDMK_EINVAL, e.what());
     }
+    HIPX(hipGetDevice(&h->device));
     h->tree = build_octree(n_src, r_src, params.L, params.n_per_leaf);
     h->n_src = n_src;
     const Octree& tr = h->tree;
@@ -323,6 +324,13 @@ void upload_moves(Handle* h, MoveBuf& b, int n, const long* idx, const double* d
     b.n = n;
 }
 
+// The decisions compare dE with -ln(u) / beta (and add H / beta for a learned
+// proposal): beta must be a finite positive number.
+void check_beta(double beta) {
+    if (!(beta > 0.0) || !std::isfinite(beta))
+        throw HpdmkError(HPDMK_EINVAL, "beta must be finite and > 0");
+}
+
 McArgs mc_args(Handle* h, const MoveBuf& b, int move0, int K, int mode, double beta) {
     McArgs a{};
     a.s = h->sc;
@@ -367,6 +375,7 @@ McArgs mc_args(Handle* h, const MoveBuf& b, int move0, int K, int mode, double b
 // Runs the loaded move list as a Metropolis chain (mode 0 plain, 1 batched;
 // batch 0 = the tuned batch size); returns the GPU time in ms.
 double run_chain(Handle* h, double beta, int mode, int batch) {
+    check_beta(beta);
     MoveBuf& mb = h->chain;
     const int n = mb.n;
     if (n <= 0) throw HpdmkError(HPDMK_EINVAL, "no moves loaded (hpdmk_mc_load)");
@@ -726,8 +735,16 @@ int hpdmk_tree_rebuild(hpdmk_tree t, hpdmk_tree* out) {
         nh->bjerrum = h->bjerrum;
         nh->sigma = h->sigma;
         nh->profiling = h->profiling;
+        // the hard-sphere overlap search and the proposal features read the
+        // target leaf's near list, which reaches one box of the finest level:
+        // a deeper rebuilt tree may no longer cover sigma or rf
+        const double finest = h->params.L / (double)(1 << (nh->tree.max_depth - 1));
+        if (h->sigma > finest || h->prop_rf > finest)
+            throw HpdmkError(HPDMK_EINVAL, "the rebuilt tree's finest box (" + std::to_string(finest) +
+                                               ") is smaller than the hard-sphere diameter or the proposal's "
+                                               "feature radius; rebuild with a larger n_per_leaf");
         nh->prop_learned = h->prop_learned;
-        nh->prop_rf = std::min(h->prop_rf, h->params.L / (double)(1 << (nh->tree.max_depth - 1)));
+        nh->prop_rf = h->prop_rf;
         nh->prop_max_mean = h->prop_max_mean;
         nh->prop_smin = h->prop_smin;
         nh->prop_smax = h->prop_smax;
@@ -1115,6 +1132,7 @@ int hpdmk_get_tiles(hpdmk_tree t, int* out) {
 int hpdmk_autotune(hpdmk_tree t, int n_moves, const long* idx, const double* d, const double* u, double beta,
                    int* out, double* best_ms) {
     return guard([&] {
+        check_beta(beta);
         Handle* h = H(t);
         if (n_moves < 1 || !idx || !d || !u) throw HpdmkError(HPDMK_EINVAL, "empty or null move list");
         // a twin of the current state takes the timing runs; this system is untouched

@@ -81,6 +81,7 @@ struct Handle {
     int n_src = 0, n_leaves = 0, n_pw = 0;
     double Q = 0;
     hipStream_t stream = nullptr;
+    int device = 0;          // HIP device current at create (the handle's memory lives there)
     bool own_stream = true;  // false when the stream belongs to a replica set
     bool borrowed = false;   // member of a replica set: destroyed with the set
     std::vector<void*> owned;
@@ -198,6 +199,8 @@ void timed(Handle* h, int kind, L&& launch) {
 Handle* create_handle(HPDMKParams params, int precision, int n_src, const double* r_src, const double* charge,
                       hipStream_t stream);
 void ensure_moves(Handle* h, MoveBuf& b, int n);
+// throws HPDMK_EINVAL unless beta is finite and > 0
+void check_beta(double beta);
 void upload_moves(Handle* h, MoveBuf& b, int n, const long* idx, const double* d, const double* u);
 McArgs mc_args(Handle* h, const MoveBuf& b, int move0, int K, int mode, double beta);
 double run_chain(Handle* h, double beta, int mode, int batch);

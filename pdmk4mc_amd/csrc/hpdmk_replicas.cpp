@@ -176,6 +176,7 @@ int hpdmk_replicas_mc_run(hpdmk_replicas s, double beta, int graph_steps, double
         const int n = set->n_moves;
         if (n <= 0) throw HpdmkError(HPDMK_EINVAL, "no moves loaded (hpdmk_replicas_mc_load)");
         if (graph_steps < 0) throw HpdmkError(HPDMK_EINVAL, "graph_steps must be >= 0");
+        check_beta(beta);
         std::vector<McArgs> args;
         for (auto& h : set->reps)
             if (h->prop_rf > 0.0)

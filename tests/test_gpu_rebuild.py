@@ -92,3 +92,47 @@ def test_rebuild_recovers_capacity_overflow():
     g2.update_shift(0, 3.0, 0.0, 0.0)  # the rebuilt system takes moves again
     g.close()
     g2.close()
+
+
+def test_rebuild_refuses_deeper_tree_under_hard_core():
+    """A crowded leaf makes the rebuilt tree deeper; its finest box is then
+    smaller than a hard-sphere diameter that the first tree allowed, and the
+    overlap search (the target leaf's near list) would miss contacts: the
+    rebuild must refuse instead of clipping silently."""
+    log("rebuild refused under a hard core wider than the new finest box")
+    n = 2000
+    r, q = O.reference_test_system(n, L)
+    g = P.HPDMKPtTree(P.HPDMKParams(64, 3, L, init=P.DIRECT), r, q)
+    finest = L / 2 ** (g.max_depth - 1)
+    g.mc_set_model(1.0, 0.9 * finest)
+    pos = g.positions()
+    target = np.array([1.0, 1.0, 1.0])
+    for i in range(60):  # 60 arrivals fit the leaf's slots (count + 64) and exceed the leaf cap
+        d = target + 0.2 * np.random.default_rng(i).uniform(-1, 1, 3) - pos[i]
+        g.update_shift(i, *d)
+    with pytest.raises(P.HPDMKError) as e:
+        g.rebuild()
+    assert e.value.code == -1, e.value  # HPDMK_EINVAL
+    g.mc_set_model(1.0, 0.0)
+    g2 = g.rebuild()  # without the hard core the deeper tree is fine
+    assert g2.max_depth > g.max_depth
+    g.close()
+    g2.close()
+
+
+def test_chain_rejects_bad_beta():
+    log("beta must be finite and > 0")
+    n = 1000
+    r, q = O.reference_test_system(n, L)
+    g = P.HPDMKPtTree(P.HPDMKParams(64, 3, L, init=P.DIRECT), r, q)
+    g.form_outgoing_pw()
+    g.form_incoming_pw()
+    idx = np.arange(8)
+    d = np.full((8, 3), 0.1)
+    u = np.full(8, 0.5)
+    for beta in (0.0, -1.0, float("nan"), float("inf")):
+        for mode in (0, 1):
+            with pytest.raises(P.HPDMKError) as e:
+                g.mc_chain(idx, d, u, beta=beta, mode=mode)
+            assert e.value.code == -1
+    g.close()
