@@ -116,6 +116,16 @@ int hpdmk_get_positions(hpdmk_tree tree, double* r_src);
 
 /* n independent trial moves evaluated against the same state (no update). */
 int hpdmk_eval_shift_energy_batch(hpdmk_tree tree, int n, const long* idx, const double* d, double* dE);
+/* The same without any device allocation: no counterpart in the reference;
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The move list and the
+ * energies are staged in a fixed device array of the library (144 KiB per
+ * GPU, held until the process ends), so n is at most 4096 (HPDMK_EINVAL
+ * beyond).  Every call on one GPU shares that array: calls that overlap on
+ * one GPU (several threads, or several handles on different streams) race
+ * and must be serialised by the caller.  The handle's own state was
+ * allocated at create. */
+int hpdmk_eval_shift_energy_batch_noalloc(hpdmk_tree tree, int n, const long* idx, const double* d, double* dE);
 
 /* Metropolis chain on the device.  Move k proposes particle idx[k] displaced
  * by d[3k..3k+2] and is accepted when u[k] < exp(-beta dE).

@@ -48,6 +48,7 @@ SIGNATURES = {
     "hpdmk_update_shift": (C.c_int, [C.c_void_p, C.c_long, C.c_double, C.c_double, C.c_double]),
     "hpdmk_get_positions": (C.c_int, [C.c_void_p, _dp]),
     "hpdmk_eval_shift_energy_batch": (C.c_int, [C.c_void_p, C.c_int, _lp, _dp, _dp]),
+    "hpdmk_eval_shift_energy_batch_noalloc": (C.c_int, [C.c_void_p, C.c_int, _lp, _dp, _dp]),
     "hpdmk_mc_load": (C.c_int, [C.c_void_p, C.c_int, _lp, _dp, _dp]),
     "hpdmk_mc_run": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_int, _dp]),
     "hpdmk_mc_results": (C.c_int, [C.c_void_p, _dp, _ip]),
