@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
+from chain_check import MIXED_DE_BOUND, assert_same_chain, thresholds
 import pdmk4mc_amd as P
 
 pytestmark = pytest.mark.gpu
@@ -126,12 +127,13 @@ def test_batched_chain_equals_plain(precision, batch):
     idx, d, u = _moves(2000, 400, 5)
     dE1, acc1, _ = g1.mc_chain(idx, d, u, beta=1.0, mode=0)
     dE2, acc2, _ = g2.mc_chain(idx, d, u, beta=1.0, mode=1, batch=batch)
-    tol = 1e-9 if precision == P.HPDMK_FP64 else 2e-4
-    assert np.abs(dE1 - dE2).max() < tol
-    assert (acc1 == acc2).mean() > (0.999 if precision == P.HPDMK_FP64 else 0.99)
     if precision == P.HPDMK_FP64:
+        assert (acc1 == acc2).all()
+        assert np.abs(dE1 - dE2).max() < 1e-9
         assert np.abs(g1.positions() - g2.positions()).max() < 1e-12
         assert np.abs(g1.energy_parts() - g2.energy_parts()).max() < 1e-9
+    else:
+        assert_same_chain(dE1, acc1, dE2, acc2, thresholds(u), MIXED_DE_BOUND)
 
 
 @pytest.mark.parametrize("precision", [P.HPDMK_FP64, P.HPDMK_MIXED])
@@ -144,11 +146,12 @@ def test_batched_chain_equals_plain_digits3(precision):
     idx, d, u = _moves(2000, 400, 7)
     dE1, acc1, _ = g1.mc_chain(idx, d, u, beta=1.0, mode=0)
     dE2, acc2, _ = g2.mc_chain(idx, d, u, beta=1.0, mode=1, batch=64)
-    tol = 1e-9 if precision == P.HPDMK_FP64 else 2e-4
-    assert np.abs(dE1 - dE2).max() < tol
-    assert (acc1 == acc2).mean() > (0.999 if precision == P.HPDMK_FP64 else 0.99)
     if precision == P.HPDMK_FP64:
+        assert (acc1 == acc2).all()
+        assert np.abs(dE1 - dE2).max() < 1e-9
         assert np.abs(g1.positions() - g2.positions()).max() < 1e-12
+    else:
+        assert_same_chain(dE1, acc1, dE2, acc2, thresholds(u), MIXED_DE_BOUND)
 
 
 @pytest.mark.parametrize("precision", [P.HPDMK_FP64, P.HPDMK_MIXED])
@@ -165,10 +168,12 @@ def test_batched_chain_single_trial_batches(precision):
     dE1, acc1, _ = g1.mc_chain(idx, d, u, beta=1.0, mode=0)
     dE2, acc2, _ = g2.mc_chain(idx, d, u, beta=1.0, mode=1, batch=16)
     dE3, acc3, _ = g3.mc_chain(idx, d, u, beta=1.0, mode=1, batch=1)
-    tol = 1e-9 if precision == P.HPDMK_FP64 else 2e-4
     for dE, acc in ((dE2, acc2), (dE3, acc3)):
-        assert np.abs(dE1 - dE).max() < tol
-        assert (acc1 == acc).mean() > (0.999 if precision == P.HPDMK_FP64 else 0.99)
+        if precision == P.HPDMK_FP64:
+            assert (acc1 == acc).all()
+            assert np.abs(dE1 - dE).max() < 1e-9
+        else:
+            assert_same_chain(dE1, acc1, dE, acc, thresholds(u), MIXED_DE_BOUND)
     if precision == P.HPDMK_FP64:
         assert np.abs(g1.positions() - g2.positions()).max() < 1e-12
         assert np.abs(g1.positions() - g3.positions()).max() < 1e-12
