@@ -101,7 +101,10 @@ struct McScratch {
     unsigned char* ovl;    // kMaxBatch^2 hard-sphere flags of pair (i, j): 1 t_j near o_i, 2 t_j near t_i
     int* dest;             // rewritten arrays, 2*slot + kind (0 out, 1 in): level-2 in arrays from
                            // the front, the others from the back (2 n_pw entries)
-    int* plan;             // [0] arrays in dest, [1] accepted moves, [2] level-2 in arrays
+    int* dinfo;            // fused single-move chain: per dest entry the array's level and box cell,
+                           // l | cx << 4 | cy << 13 | cz << 22 (dest then in marking order)
+    int* plan;             // [0] arrays in dest, [1] accepted moves, [2] level-2 in arrays,
+                           // [3] single move: the target leaf has room (marking row)
     unsigned long long* probe;  // optional phase timers (profiling): kProbes sums + counts
 };
 
@@ -135,12 +138,16 @@ struct McArgs {
     McScratch s;
     MoveList ml;
     PwNodes pn;
-    int move0;   // first move of this launch in the move list
-    // when set, the first move is *move_ctr (replica steps replayed from a
+    int move0;   // first move of this launch in the move list (after *move_ctr when set)
+    // when set, the first move is *move_ctr + move0 (steps replayed from a
     // graph; batched learned steps, whose scan advances it) and a launch takes
-    // at most n_moves - *move_ctr trials
+    // at most n_moves - *move_ctr - move0 trials
     int* move_ctr;
     int n_moves;
+    // fused single-move chain: move m's trial state (TrialInfo, phase rows)
+    // lives in slot m & 1; the update kernel decides from the trial partials,
+    // moves the particle and prepares move m + 1 in the other slot
+    int ring;
     int K;       // trials in this launch (at most)
     int mode;    // TrialMode
     double beta;
@@ -194,8 +201,8 @@ struct ReplicaLaunch {
 
 template <typename T>
 void launch_replica_step(const ReplicaLaunch<T>& r, hipStream_t st);
-// move_ctr[0..n) += 1
-void launch_advance(int* move_ctr, int n, hipStream_t st);
+// move_ctr[0..n) += inc
+void launch_advance(int* move_ctr, int n, hipStream_t st, int inc = 1);
 template <typename T>
 void launch_replica_compact(const ReplicaLaunch<T>& r, hipStream_t st);
 

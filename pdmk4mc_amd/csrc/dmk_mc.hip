@@ -59,31 +59,20 @@ __device__ __forceinline__ int image_shift(int ca, int cb, int nl) {
 }
 
 // first move of this launch: the step counter when one is attached
-__device__ __forceinline__ int move_base(const McArgs& a) { return a.move_ctr ? *a.move_ctr : a.move0; }
+__device__ __forceinline__ int move_base(const McArgs& a) { return (a.move_ctr ? *a.move_ctr : 0) + a.move0; }
 // trials of this launch: with a step counter, the moves left (at most K)
 __device__ __forceinline__ int keff(const McArgs& a) {
-    return a.move_ctr ? min(a.K, a.n_moves - *a.move_ctr) : a.K;
+    return a.move_ctr ? min(a.K, a.n_moves - move_base(a)) : a.K;
 }
+// scratch slot (TrialInfo, phase rows) of trial tr of move m: the ring slot
+// m & 1 in the fused single-move chain, else the trial's index
+__device__ __forceinline__ int trial_slot(const McArgs& a, int tr, int m) { return a.ring ? (m & 1) : tr; }
 
 // ------------------------------------------------------------------ setup
+// FP64-formed phase rows of a trial into tb, by the whole block
 template <typename T>
-__device__ __forceinline__ void setup_body(const DevTree<T>& t, const McArgs& a) {
-    __shared__ TrialInfo ti;
-    if (*t.err) return;
-    const int tr = blockIdx.x, m = move_base(a) + tr;
-    if (tr >= keff(a)) return;
-    if (threadIdx.x == 0) {
-        trial_setup(t, a.ml.idx[m], a.ml.d + 3 * m, ti);
-        a.s.info[tr] = ti;
-        if (a.ml.stats) {  // filled by k_pair_corr (pair terms) and the update planner (batch arrays)
-            a.ml.stats[kStatsPerMove * m + 6] = 0;
-            a.ml.stats[kStatsPerMove * m + 7] = 0;
-        }
-    }
-    __syncthreads();
+__device__ void form_rows(const DevTree<T>& t, const TrialInfo& ti, Cplx<T>* tb) {
     const int D = t.max_depth, nL = n_levels_pw(D);
-    const int stride = table_stride(D);
-    Cplx<T>* tb = tabs_of<T>(a.s, stride, tr);
     const int rows = 6 + 2 * nL * 9;
     // one thread per row: e^{-i dk c} by one FP64 sincos, its powers by
     // complex multiplication (FP64), then the row e^{-i (m - n) dk c}
@@ -123,6 +112,30 @@ __device__ __forceinline__ void setup_body(const DevTree<T>& t, const McArgs& a)
             row[n - m] = {(T)wr, (T)(-wi)};
         }
     }
+}
+
+// trial state of move m (paths, endpoints) into slot sl, by thread 0 of the
+// block; the caller syncs, then form_rows
+template <typename T>
+__device__ __forceinline__ void setup_move(const DevTree<T>& t, const McArgs& a, int m, int sl, TrialInfo& ti) {
+    trial_setup(t, a.ml.idx[m], a.ml.d + 3 * m, ti);
+    a.s.info[sl] = ti;
+    if (a.ml.stats) {  // filled by k_pair_corr (pair terms) and the update planner (batch arrays)
+        a.ml.stats[kStatsPerMove * m + 6] = 0;
+        a.ml.stats[kStatsPerMove * m + 7] = 0;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void setup_body(const DevTree<T>& t, const McArgs& a) {
+    __shared__ TrialInfo ti;
+    if (*t.err) return;
+    const int tr = blockIdx.x, m = move_base(a) + tr;
+    if (tr >= keff(a)) return;
+    const int sl = trial_slot(a, tr, m);
+    if (threadIdx.x == 0) setup_move(t, a, m, sl, ti);
+    __syncthreads();
+    form_rows(t, ti, tabs_of<T>(a.s, table_stride(t.max_depth), sl));
 }
 
 template <typename T>
@@ -367,7 +380,11 @@ __device__ void mark_single(const DevTree<T>& t, const McArgs& a) {
     __syncthreads();
     // one thread per (endpoint, level, candidate array): few registers in the
     // trial kernel, every candidate's loads in flight together
-    const TrialInfo& ti = a.s.info[0];
+    const TrialInfo& ti = a.s.info[trial_slot(a, 0, move_base(a))];
+    if (threadIdx.x == 0) {  // room in the target leaf (read before the update kernel moves anything)
+        const int lt = ti.path_t[ti.lam_t], lo = ti.path_o[ti.lam_o];
+        a.s.plan[3] = lt == lo || t.slot_count[lt] < t.slot_cap[lt];
+    }
     for (int w = threadIdx.x; w < 2 * nL * 28; w += blockDim.x) {
         const int c = w % 28, el = w / 28, ep = el / nL, l = 2 + el % nL;
         if (l > (ep == 0 ? ti.lam_t : ti.lam_o)) continue;
@@ -382,10 +399,28 @@ __device__ void mark_single(const DevTree<T>& t, const McArgs& a) {
             ix = sc >= 0 ? 2 * sc + 1 : -1;
         }
         if (ix < 0 || (atomicOr(&mark_bits[ix >> 5], 1u << (ix & 31)) & (1u << (ix & 31)))) continue;
-        if (c > 0 && l == 2)
+        if (a.ring) {
+            // fused chain: one list in marking order, with the array box's
+            // level and cell (the endpoint's cell at level l, shifted by the
+            // colleague offset and wrapped) so that the update kernel finds
+            // every contribution without table loads
+            const double* x = ep == 0 ? ti.t : ti.o;
+            const int nl = 1 << l;
+            int cc[3] = {cell_at(x[0], t.L, l), cell_at(x[1], t.L, l), cell_at(x[2], t.L, l)};
+            if (c >= 2) {
+                const int j = c - 2 < 13 ? c - 2 : c - 1;  // offset (dx+1) + 3(dy+1) + 9(dz+1), 13 skipped
+                cc[0] = (cc[0] + j % 3 - 1 + nl) % nl;
+                cc[1] = (cc[1] + (j / 3) % 3 - 1 + nl) % nl;
+                cc[2] = (cc[2] + j / 9 - 1 + nl) % nl;
+            }
+            const int pos = atomicAdd(&nfront, 1);
+            a.s.dest[pos] = ix;
+            a.s.dinfo[pos] = l | (cc[0] << 4) | (cc[1] << 13) | (cc[2] << 22);
+        } else if (c > 0 && l == 2) {
             a.s.dest[atomicAdd(&nfront, 1)] = ix;
-        else
+        } else {
             a.s.dest[n2 - 1 - atomicAdd(&nback, 1)] = ix;
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -722,9 +757,10 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
     }
     const int tr = blockIdx.y - gy, b = blockIdx.x, m = move_base(a) + tr;
     if (tr >= keff(a)) return;
-    const TrialInfo& ti = a.s.info[tr];
+    const int sl = trial_slot(a, tr, m);
+    const TrialInfo& ti = a.s.info[sl];
     const int D = t.max_depth, nL = n_levels_pw(D), per = trial_blocks(D, a.tiles);
-    const Cplx<T>* tb = tabs_of<T>(a.s, table_stride(D), tr);
+    const Cplx<T>* tb = tabs_of<T>(a.s, table_stride(D), sl);
     double part = 0.0;
     int nscan = 0, novl = 0;
     double fsum[kFeatSums] = {0, 0, 0, 0, 0, 0, 0};
@@ -1571,9 +1607,9 @@ __global__ void __launch_bounds__(TB) k_compact_rep(const DevTree<T>* __restrict
 }
 
 // next move of every replica (graph-replayable step counter)
-__global__ void k_advance(int* move_ctr, int n) {
+__global__ void k_advance(int* move_ctr, int n, int inc) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < n) move_ctr[r] += 1;
+    if (r < n) move_ctr[r] += inc;
 }
 
 // ------------------------------------------------------------------ apply
@@ -1925,6 +1961,217 @@ __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
     apply_body<T, XMAX, RB>(t, a);
 }
 
+// ------------------------------------------------------------------ fused single-move update
+// The update kernel of the fused single-move chain (a.ring; src/tree.cpp:447-597).
+// Every block sums the trial kernel's partials in the same order and takes
+// the same Metropolis decision (dE < -ln(u) / beta, no overlap, room in the
+// target leaf as the marking row found it).  Block 0 writes the move's dE and
+// decision, moves the particle and sets up move m + 1 in the other ring slot
+// (its particle's data are loaded while the decision is pending; a move of
+// the same particle is redone from the target).  The other blocks rewrite one
+// chunk of kUpd1J * TB elements of the root or of a marked array each: the
+// contributions of the two endpoints follow from their box cells at the
+// array's level (computed from the positions) and the array box's cell (from
+// the marking row), and the array's elements and the phase-row entries are
+// loaded before the decision is known; only the write-back waits for it.
+constexpr int kUpd1J = 4;
+
+// wave 0: the partial sums of the move's trial blocks (lane b loads block b,
+// b + 64, ...), reduced per role; every lane ends with the totals
+template <typename T>
+__device__ __forceinline__ void upd1_partials(const DevTree<T>& t, const McArgs& a, double* v) {
+    sum_trial_partials(t, a, 0, threadIdx.x, 64, v);
+}
+
+template <typename T>
+__device__ __forceinline__ int upd1_decide(const DevTree<T>& t, const McArgs& a, const TrialInfo& ti, int m,
+                                           const double* v, bool writer) {
+    const int ovl = (int)v[6];
+    double dE;
+    if (writer) {
+        int o2;
+        dE = assemble_trial(t, a, m & 1, m, v, &o2);  // also the optional parts and counts
+    } else {
+        dE = v[0] + v[1] + v[2] - v[3] - self_term(t, ti);
+    }
+    dE = ovl > 0 ? (double)INFINITY : a.lB * dE;  // a hard-sphere overlap is an infinite energy
+    int ok = ovl == 0 && dE < mh_threshold(a.ml.u[m], a.beta);
+    if (ok && !a.s.plan[3]) {  // the target leaf is full: the chain stops here
+        ok = 0;
+        if (writer) atomicOr(t.err, (int)kErrLeafOverflow);
+    }
+    if (writer) {
+        a.ml.dE[m] = dE;
+        if (a.ml.accept_out) a.ml.accept_out[m] = ok;
+    }
+    return ok;
+}
+
+template <typename T>
+__device__ void upd1_block0(const DevTree<T>& t, const McArgs& a) {
+    __shared__ TrialInfo nxt;
+    __shared__ int ok_sh;
+    const int m = move_base(a), sl = m & 1;
+    const TrialInfo& ti = a.s.info[sl];
+    const bool has_next = m + 1 < a.n_moves;
+    // move m + 1 from the current state, while the decision is pending
+    if (threadIdx.x == 64 && has_next) trial_setup(t, a.ml.idx[m + 1], a.ml.d + 3 * (m + 1), nxt);
+    if (threadIdx.x < 64) {
+        double v[kAccPerTrial];
+        upd1_partials(t, a, v);
+        if (threadIdx.x == 0) ok_sh = upd1_decide(t, a, ti, m, v, true);
+    }
+    __syncthreads();
+    const int ok = ok_sh;
+    if (threadIdx.x == 0) {
+        if (ok) {  // src/tree.cpp:454-463, 583-596
+            const int lt = ti.path_t[ti.lam_t], lo = ti.path_o[ti.lam_o];
+            if (lt == lo) {
+                const int s = t.slot_of[ti.i];
+                t.px[s] = ti.t[0];
+                t.py[s] = ti.t[1];
+                t.pz[s] = ti.t[2];
+            } else {
+                relocate(t, ti, 0);
+                t.slot_count[lt] += 1;
+            }
+        }
+        if (has_next) {
+            if (ok && nxt.i == ti.i) {  // the same particle moves again, from this move's target
+                for (int c = 0; c < 3; ++c) {
+                    nxt.o[c] = ti.t[c];
+                    nxt.t[c] = wrap(nxt.o[c] + nxt.dv[c], t.L);
+                }
+                nxt.lam_o = ti.lam_t;
+                for (int l = 0; l < kMaxLevels; ++l) nxt.path_o[l] = ti.path_t[l];
+                nxt.lam_t = locate(t, nxt.t[0], nxt.t[1], nxt.t[2], nxt.path_t);
+            }
+            a.s.info[sl ^ 1] = nxt;
+            if (a.ml.stats) {
+                a.ml.stats[kStatsPerMove * (m + 1) + 6] = 0;
+                a.ml.stats[kStatsPerMove * (m + 1) + 7] = 0;
+            }
+        }
+    }
+    __syncthreads();
+    if (has_next) form_rows(t, nxt, tabs_of<T>(a.s, table_stride(t.max_depth), sl ^ 1));
+}
+
+template <typename T>
+__device__ void upd1_body(const DevTree<T>& t, const McArgs& a) {
+    __shared__ int ok_sh;
+    if (*t.err) return;
+    if (blockIdx.x == 0) {
+        upd1_block0(t, a);
+        return;
+    }
+    const int m = move_base(a), sl = m & 1;
+    const TrialInfo& ti = a.s.info[sl];
+    const int nD = a.s.plan[0];
+    constexpr int CH = kUpd1J * TB;  // elements per item
+    const int nR = (t.Mw + CH - 1) / CH, nA = (t.Md + CH - 1) / CH;
+    const int it = blockIdx.x - 1;
+    const bool have = it < nR + nD * nA;  // uniform
+    const int D = t.max_depth, nL = n_levels_pw(D);
+    const Cplx<T>* tb = tabs_of<T>(a.s, table_stride(D), sl);
+    bool is_root = true;
+    int chunk = it, key = 0, l = 0, ccell[3] = {0, 0, 0};
+    if (have && it >= nR) {
+        const int r = it - nR, arr = r / nA;
+        chunk = r % nA;
+        key = a.s.dest[arr];
+        const int di = a.s.dinfo[arr];
+        is_root = false;
+        l = di & 15;
+        ccell[0] = (di >> 4) & 511;
+        ccell[1] = (di >> 13) & 511;
+        ccell[2] = (di >> 22) & 511;
+    }
+    const int slot = key >> 1, kind = is_root ? 2 : key & 1;
+    const int d = is_root ? t.dw : t.dd, M = is_root ? t.Mw : t.Md;
+    Cplx<T>* dst = is_root ? t.root : (kind == 0 ? t.out : t.inc) + (size_t)slot * t.Md;
+    // contribution of endpoint ep (0 target, 1 origin): its rows at the
+    // image shift of its box next to the array's box, sign -1 at the origin,
+    // 0 when the endpoint does not touch the array (rows then any valid ones)
+    const Cplx<T>* rx[2];
+    const Cplx<T>* ry[2];
+    const Cplx<T>* rz[2];
+    T sg[2];
+#pragma unroll
+    for (int ep = 0; ep < 2; ++ep) {
+        rx[ep] = tb + root_row(ep, 0);
+        ry[ep] = tb + root_row(ep, 1);
+        rz[ep] = tb + root_row(ep, 2);
+        sg[ep] = have && is_root ? (ep == 0 ? T(1) : T(-1)) : T(0);
+        if (!have || is_root || l > (ep == 0 ? ti.lam_t : ti.lam_o)) continue;
+        const double* x = ep == 0 ? ti.t : ti.o;
+        const int nl = 1 << l;
+        int sh[3];
+        bool same = true, adj = true;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int bc = cell_at(x[c], t.L, l);
+            same = same && bc == ccell[c];
+            sh[c] = image_shift(bc, ccell[c], nl);
+            adj = adj && sh[c] != 2;
+        }
+        if (!(same || (kind == 1 && adj))) continue;
+        const int li = l - 2;
+        rx[ep] = tb + level_row(nL, ep, li, 0, same ? 0 : sh[0]);
+        ry[ep] = tb + level_row(nL, ep, li, 1, same ? 0 : sh[1]);
+        rz[ep] = tb + level_row(nL, ep, li, 2, same ? 0 : sh[2]);
+        sg[ep] = ep == 0 ? T(1) : T(-1);
+    }
+    // the chunk's elements and their phase-row entries, all loads in flight
+    // together; then the trial's partial sums (wave 0) and the decision
+    Cplx<T> pf[kUpd1J], px_[kUpd1J][2], py_[kUpd1J][2], pz_[kUpd1J][2];
+#pragma unroll
+    for (int j = 0; j < kUpd1J; ++j) {
+        const int e = min(chunk * CH + threadIdx.x + TB * j, M - 1);
+        const int x = e % d, r = e / d, y = r % d, z = r / d;
+        pf[j] = dst[have ? e : 0];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            px_[j][c] = rx[c][x];
+            py_[j][c] = ry[c][y];
+            pz_[j][c] = rz[c][z];
+        }
+    }
+    double v[kAccPerTrial];
+    if (threadIdx.x < 64) upd1_partials(t, a, v);
+    Cplx<T> val[kUpd1J];
+#pragma unroll
+    for (int j = 0; j < kUpd1J; ++j) {
+        val[j] = {T(0), T(0)};
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            if (sg[c] == T(0)) continue;  // rows of an absent contribution may be unformed
+            Cplx<T> yz = cmul(py_[j][c], pz_[j][c]);
+            yz.x *= sg[c];
+            yz.y *= sg[c];
+            const Cplx<T> xv = px_[j][c];
+            val[j].x = fma(xv.x, yz.x, val[j].x);
+            val[j].x = fma(-xv.y, yz.y, val[j].x);
+            val[j].y = fma(xv.x, yz.y, val[j].y);
+            val[j].y = fma(xv.y, yz.x, val[j].y);
+        }
+    }
+    if (threadIdx.x == 0) ok_sh = upd1_decide(t, a, ti, m, v, false);
+    __syncthreads();
+    if (!ok_sh || !have) return;
+    const T sy = kind == 1 ? T(-1) : T(1);  // in[C] += conj(tau s)
+#pragma unroll
+    for (int j = 0; j < kUpd1J; ++j) {
+        const int e = chunk * CH + threadIdx.x + TB * j;
+        if (e < M) dst[e] = {pf[j].x + val[j].x, pf[j].y + sy * val[j].y};
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TB) k_update1(DevTree<T> t, McArgs a) {
+    upd1_body(t, a);
+}
+
 template <typename T, int XMAX>
 __global__ void __launch_bounds__(TB) k_apply_rep(const DevTree<T>* __restrict__ ts, const McArgs* __restrict__ as) {
     const DevTree<T>& t = ts[blockIdx.z];
@@ -2005,6 +2252,15 @@ void launch_apply(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent
         allow_shmem((const void*)kernel, sh);
         hipExtLaunchKernelGGL(kernel, dim3(nb), dim3(TB), sh, st, e0, e1, 0, t, a);
     };
+    if (a.ring) {
+        // block 0, then items: the root's chunks and every marked array's (at
+        // most 2 endpoints x levels x 28 arrays)
+        constexpr int CH = kUpd1J * TB;
+        const int nL = t.max_depth >= 4 ? t.max_depth - 3 : 0;
+        const int items = (t.Mw + CH - 1) / CH + 2 * nL * 28 * ((t.Md + CH - 1) / CH);
+        hipExtLaunchKernelGGL(k_update1<T>, dim3(1 + items), dim3(TB), 0, st, e0, e1, 0, t, a);
+        return;
+    }
     if (apply_xmax<T>(dmax) <= kApplyXSmall<T>) {
         if (rb)
             go(k_apply<T, kApplyXSmall<T>, true>);
@@ -2043,11 +2299,11 @@ void launch_replica_step(const ReplicaLaunch<T>& r, hipStream_t st) {
         hipLaunchKernelGGL((k_apply_rep<T, kApplyXLarge<T>>), dim3(r.apply_blocks, 1, r.n), dim3(TB), sh, st, r.trees,
                            r.args);
     }
-    hipLaunchKernelGGL(k_advance, dim3((r.n + TB - 1) / TB), dim3(TB), 0, st, r.move_ctr, r.n);
+    hipLaunchKernelGGL(k_advance, dim3((r.n + TB - 1) / TB), dim3(TB), 0, st, r.move_ctr, r.n, 1);
 }
 
-void launch_advance(int* move_ctr, int n, hipStream_t st) {
-    hipLaunchKernelGGL(k_advance, dim3((n + TB - 1) / TB), dim3(TB), 0, st, move_ctr, n);
+void launch_advance(int* move_ctr, int n, hipStream_t st, int inc) {
+    hipLaunchKernelGGL(k_advance, dim3((n + TB - 1) / TB), dim3(TB), 0, st, move_ctr, n, inc);
 }
 
 template <typename T>

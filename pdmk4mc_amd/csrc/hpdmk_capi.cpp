@@ -284,10 +284,11 @@ Handle* create_handle(HPDMKParams params, int precision, int n_src, const double
     if (2 * (size_t)h->n_pw > 32 * (size_t)kMarkWords)
         throw HpdmkError(HPDMK_EINVAL, "tree has more expansion boxes than the update planner's bitmap holds");
     h->sc.dest = dalloc<int>(2 * (size_t)h->n_pw + 2, o);
-    h->sc.plan = dalloc<int>(3, o);
+    h->sc.dinfo = dalloc<int>(2 * (size_t)h->n_pw + 2, o);
+    h->sc.plan = dalloc<int>(4, o);
     h->probe = dalloc<unsigned long long>(2 * kProbes, o);
     HIPX(hipMemset(h->probe, 0, sizeof(unsigned long long) * 2 * kProbes));
-    HIPX(hipMemset(h->sc.plan, 0, 3 * sizeof(int)));
+    HIPX(hipMemset(h->sc.plan, 0, 4 * sizeof(int)));
     HIPX(hipDeviceSynchronize());
     return h.release();
 }
@@ -419,7 +420,53 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
                 h->since_compact = 0;
             }
         };
-        if (mode == 0 && !h->profiling && n >= 2 * kPlainGraphMoves) {
+        if (mode == 0 && !(h->prop_rf > 0.0) && h->tree.max_depth <= kFusedMaxDepth) {
+            // fused single-move chain: per move a trial kernel (with the update
+            // plan's marking row) and an update kernel whose blocks take the
+            // decision themselves; the update kernel's block 0 moves the
+            // particle and sets up the next move, so only the first move needs
+            // the set-up kernel.  Runs of kPlainGraphMoves moves are replayed
+            // from a HIP graph whose kernels read the run's first move from a
+            // device counter (advanced once per run).
+            McArgs a = mc_args(h, mb, 0, 1, kTrialMetropolis, beta);
+            a.ring = 1;
+            a.n_moves = n;
+            timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
+            int m = 0;
+            if (!h->profiling && n >= 2 * kPlainGraphMoves) {
+                if (!h->step_ctr) h->step_ctr = dalloc<int>(1, h->owned);
+                HIPX(hipMemsetAsync(h->step_ctr, 0, sizeof(int), h->stream));
+                McArgs g = a;
+                g.move_ctr = h->step_ctr;
+                struct Graph {
+                    hipGraph_t g = nullptr;
+                    hipGraphExec_t x = nullptr;
+                    ~Graph() {
+                        if (x) (void)hipGraphExecDestroy(x);
+                        if (g) (void)hipGraphDestroy(g);
+                    }
+                } gr;
+                HIPX(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+                for (int k = 0; k < kPlainGraphMoves; ++k) {
+                    g.move0 = k;
+                    launch_trial(dv, g, h->stream);
+                    launch_apply(dv, g, h->stream);
+                }
+                launch_advance(h->step_ctr, 1, h->stream, kPlainGraphMoves);
+                HIPX(hipStreamEndCapture(h->stream, &gr.g));
+                HIPX(hipGraphInstantiate(&gr.x, gr.g, nullptr, nullptr, 0));
+                for (; m + kPlainGraphMoves <= n; m += kPlainGraphMoves) {
+                    HIPX(hipGraphLaunch(gr.x, h->stream));
+                    compact_if_due(kPlainGraphMoves);
+                }
+            }
+            for (; m < n; ++m) {
+                a.move0 = m;
+                timed(h, kKTrial, [&](hipEvent_t x, hipEvent_t y) { launch_trial(dv, a, h->stream, x, y); });
+                timed(h, kKUpdate, [&](hipEvent_t x, hipEvent_t y) { launch_apply(dv, a, h->stream, x, y); });
+                compact_if_due(1);
+            }
+        } else if (mode == 0 && !h->profiling && n >= 2 * kPlainGraphMoves) {
             // plain chain from a HIP graph of kPlainGraphMoves moves: the
             // kernels read the move index from a device counter that the last
             // node of every move advances

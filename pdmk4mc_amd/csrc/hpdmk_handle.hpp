@@ -70,6 +70,9 @@ struct MoveBuf {
 constexpr int kCompactEvery = 2048;
 // moves per captured graph of the plain chain
 constexpr int kPlainGraphMoves = 64;
+// the fused single-move chain packs box cells in 9 bits (levels <= 9, i.e.
+// trees of at most 11 levels); deeper trees run the four-kernel chain
+constexpr int kFusedMaxDepth = 11;
 
 enum KernelKind { kKTrial = 0, kKUpdate = 1, kKCorr = 2, kKScan = 3, kKSetup = 4, kKPropose = 5, kKinds = 6 };
 
