@@ -180,16 +180,21 @@ __device__ void trial_setup(const DevTree<T>& t, int i, const double* d, TrialIn
 }
 
 // self-pair correction q^2 R_{b_2}(d_min) of src/energy_shift.cpp:33-41, :58
+// The nearest of the 27 images of o (shifts -1, 0, 1 per axis) is the
+// per-axis nearest one, so its distance comes from three per-axis minima
+// (the same sum, hence the same value, as the minimum over all 27).
 template <typename T>
 __device__ double self_term(const DevTree<T>& t, const TrialInfo& ti) {
     double dr = sqrt(ti.dv[0] * ti.dv[0] + ti.dv[1] * ti.dv[1] + ti.dv[2] * ti.dv[2]);
-    for (int mx = -1; mx <= 1; ++mx)
-        for (int my = -1; my <= 1; ++my)
-            for (int mz = -1; mz <= 1; ++mz) {
-                const double ax = ti.t[0] - (ti.o[0] + mx * t.L), ay = ti.t[1] - (ti.o[1] + my * t.L),
-                             az = ti.t[2] - (ti.o[2] + mz * t.L);
-                dr = fmin(dr, sqrt(ax * ax + ay * ay + az * az));
-            }
+    double sq[3];
+    for (int c = 0; c < 3; ++c) {
+        sq[c] = INFINITY;
+        for (int k = -1; k <= 1; ++k) {
+            const double e = ti.t[c] - (ti.o[c] + k * t.L);
+            sq[c] = fmin(sq[c], e * e);
+        }
+    }
+    dr = fmin(dr, sqrt(sq[0] + sq[1] + sq[2]));
     if (dr == 0) return 0.0;
     const double b2 = t.boxsize[2];
     const double u = (dr / b2 - 0.5) * 2.0;

@@ -29,6 +29,14 @@ struct TrialInfo {
     int i;
     int lam_o, lam_t;  // leaf levels
     int path_o[kMaxLevels], path_t[kMaxLevels];
+    // fused single-move chain only (prep_body): expansion slot of each path
+    // box (-1 outside levels 2..min(leaf level, D-2)) and the near list of
+    // each endpoint's leaf; index 0 target, 1 origin
+    int pws[2][kMaxLevels];
+    int noff[2], ncnt[2];
+    int m;        // the move's index in the move list
+    double self;  // its self-pair correction q^2 R_{b_2}(d_min) (src/energy_shift.cpp:58)
+    double thr;   // its Metropolis threshold -ln(u) / beta
 };
 
 // Phase rows of one trial (Cplx<T>, kRowStride apart):
@@ -59,6 +67,10 @@ struct Tiles {
     int win = 2, diff = 8, near = 8, apply = 1024;
 };
 constexpr int kMaxRoleBlocks = 16;  // upper bound of win, diff, near (scratch sizing)
+// partial sums per lane of the fused single-move update's decision wave: it
+// reads at most 64 kUpd1P trial blocks (304 at the deepest fused tree, 11
+// levels, with the largest tiles)
+constexpr int kUpd1P = 6;
 __host__ __device__ inline int trial_blocks(int max_depth, const Tiles& g) {
     const int nL = max_depth >= 4 ? max_depth - 3 : 0;
     return g.win + 2 * nL * g.diff + 2 * g.near;
@@ -145,8 +157,10 @@ struct McArgs {
     int* move_ctr;
     int n_moves;
     // fused single-move chain: move m's trial state (TrialInfo, phase rows)
-    // lives in slot m & 1; the update kernel decides from the trial partials,
-    // moves the particle and prepares move m + 1 in the other slot
+    // lives in slot m & 1 = move0 & 1 (a graph's first move, *move_ctr, is
+    // even), so kernels find it without reading the counter; the update
+    // kernel decides from the trial partials and moves the particle; the
+    // trial kernel prepares move m + 1 in the other slot
     int ring;
     int K;       // trials in this launch (at most)
     int mode;    // TrialMode

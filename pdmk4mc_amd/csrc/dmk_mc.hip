@@ -67,6 +67,8 @@ __device__ __forceinline__ int keff(const McArgs& a) {
 // scratch slot (TrialInfo, phase rows) of trial tr of move m: the ring slot
 // m & 1 in the fused single-move chain, else the trial's index
 __device__ __forceinline__ int trial_slot(const McArgs& a, int tr, int m) { return a.ring ? (m & 1) : tr; }
+// the fused chain's ring slot of this launch's move (no counter load)
+__device__ __forceinline__ int ring_slot(const McArgs& a) { return a.move0 & 1; }
 
 // ------------------------------------------------------------------ setup
 // FP64-formed phase rows of a trial into tb, by the whole block
@@ -126,12 +128,66 @@ __device__ __forceinline__ void setup_move(const DevTree<T>& t, const McArgs& a,
     }
 }
 
+// Fused single-move chain: the whole trial state of move m in ring slot sl,
+// by one block: paths, the path boxes' expansion slots, the endpoints' near
+// lists and the phase rows.  prev (may be null): the particle was moved by
+// the previous move and sits at prev->t (its leaf path is prev's target path).
+template <typename T>
+__device__ void prep_body(const DevTree<T>& t, const McArgs& a, int m, int sl, const TrialInfo* prev) {
+    __shared__ TrialInfo ti;
+    const int D = t.max_depth;
+    if (threadIdx.x == 0) {
+        if (prev) {
+            ti.i = prev->i;
+            ti.q = prev->q;
+            for (int c = 0; c < 3; ++c) {
+                ti.o[c] = prev->t[c];
+                ti.dv[c] = a.ml.d[3 * m + c];
+                ti.t[c] = wrap(ti.o[c] + ti.dv[c], t.L);
+            }
+            ti.lam_o = prev->lam_t;
+            for (int l = 0; l < kMaxLevels; ++l) ti.path_o[l] = prev->path_t[l];
+            ti.lam_t = locate(t, ti.t[0], ti.t[1], ti.t[2], ti.path_t);
+        } else {
+            trial_setup(t, a.ml.idx[m], a.ml.d + 3 * m, ti);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * kMaxLevels) {
+        const int ep = threadIdx.x / kMaxLevels, l = threadIdx.x % kMaxLevels;
+        const int lam = ep == 0 ? ti.lam_t : ti.lam_o;
+        ti.pws[ep][l] = (l >= 2 && l <= lam && l <= D - 2) ? t.pw_slot[ep == 0 ? ti.path_t[l] : ti.path_o[l]] : -1;
+    } else if (threadIdx.x == 64 || threadIdx.x == 65) {
+        const int ep = threadIdx.x - 64;
+        const int leaf = ep == 0 ? ti.path_t[ti.lam_t] : ti.path_o[ti.lam_o];
+        const int o0 = t.near_off[leaf];
+        ti.noff[ep] = o0;
+        ti.ncnt[ep] = t.near_off[leaf + 1] - o0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ti.m = m;
+        ti.self = self_term(t, ti);
+        ti.thr = mh_threshold(a.ml.u[m], a.beta);
+        a.s.info[sl] = ti;
+        if (a.ml.stats) {
+            a.ml.stats[kStatsPerMove * m + 6] = 0;
+            a.ml.stats[kStatsPerMove * m + 7] = 0;
+        }
+    }
+    form_rows(t, ti, tabs_of<T>(a.s, table_stride(D), sl));
+}
+
 template <typename T>
 __device__ __forceinline__ void setup_body(const DevTree<T>& t, const McArgs& a) {
     __shared__ TrialInfo ti;
     if (*t.err) return;
     const int tr = blockIdx.x, m = move_base(a) + tr;
     if (tr >= keff(a)) return;
+    if (a.ring) {
+        prep_body(t, a, m, m & 1, nullptr);
+        return;
+    }
     const int sl = trial_slot(a, tr, m);
     if (threadIdx.x == 0) setup_move(t, a, m, sl, ti);
     __syncthreads();
@@ -380,7 +436,7 @@ __device__ void mark_single(const DevTree<T>& t, const McArgs& a) {
     __syncthreads();
     // one thread per (endpoint, level, candidate array): few registers in the
     // trial kernel, every candidate's loads in flight together
-    const TrialInfo& ti = a.s.info[trial_slot(a, 0, move_base(a))];
+    const TrialInfo& ti = a.s.info[a.ring ? ring_slot(a) : 0];
     if (threadIdx.x == 0) {  // room in the target leaf (read before the update kernel moves anything)
         const int lt = ti.path_t[ti.lam_t], lo = ti.path_o[ti.lam_o];
         a.s.plan[3] = lt == lo || t.slot_count[lt] < t.slot_cap[lt];
@@ -754,10 +810,18 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
             if (blockIdx.x == 0 && keff(a) > 0) mark_single(t, a);
             return;
         }
+        // fused chain: the next move's trial state from the current state (the
+        // update kernel redoes it when this move relocates the same particle)
+        if ((int)blockIdx.y == gy + a.K + 1) {
+            const int mn = a.s.info[ring_slot(a)].m + 1;
+            if (blockIdx.x == 0 && mn < a.n_moves) prep_body(t, a, mn, mn & 1, nullptr);
+            return;
+        }
     }
-    const int tr = blockIdx.y - gy, b = blockIdx.x, m = move_base(a) + tr;
-    if (tr >= keff(a)) return;
-    const int sl = trial_slot(a, tr, m);
+    const int tr = blockIdx.y - gy, b = blockIdx.x;
+    // the fused chain's graphs hold valid moves only, and its slot needs no counter
+    if (!a.ring && tr >= keff(a)) return;
+    const int sl = a.ring ? ring_slot(a) : tr;
     const TrialInfo& ti = a.s.info[sl];
     const int D = t.max_depth, nL = n_levels_pw(D), per = trial_blocks(D, a.tiles);
     const Cplx<T>* tb = tabs_of<T>(a.s, table_stride(D), sl);
@@ -786,7 +850,8 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
         const int lam = ep == 0 ? ti.lam_t : ti.lam_o;
         if (l <= lam - 1) {
             const int node = ep == 0 ? ti.path_t[l] : ti.path_o[l];
-            const Cplx<T>* in = t.inc + (size_t)t.pw_slot[node] * t.Md;
+            const int ps = a.ring ? ti.pws[ep][l] : t.pw_slot[node];
+            const Cplx<T>* in = t.inc + (size_t)ps * t.Md;
             const T* Dl = t.Dmat + (size_t)li * t.Md;
             const Cplx<T>* X = tb + level_row(nL, ep, li, 0, 0);
             const Cplx<T>* Y = tb + level_row(nL, ep, li, 1, 0);
@@ -807,7 +872,8 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
         const int ep = r / NB;
         const int lam = ep == 0 ? ti.lam_t : ti.lam_o;
         const int leaf = ep == 0 ? ti.path_t[lam] : ti.path_o[lam];
-        const int off = t.near_off[leaf], cnt = t.near_off[leaf + 1] - off;
+        const int off = a.ring ? ti.noff[ep] : t.near_off[leaf];
+        const int cnt = a.ring ? ti.ncnt[ep] : t.near_off[leaf + 1] - off;
         const double cx = ep == 0 ? ti.t[0] : ti.o[0], cy = ep == 0 ? ti.t[1] : ti.o[1],
                      cz = ep == 0 ? ti.t[2] : ti.o[2];
         const double bb = t.boxsize[lam];
@@ -1976,27 +2042,82 @@ __global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
 // loaded before the decision is known; only the write-back waits for it.
 constexpr int kUpd1J = 4;
 
-// wave 0: the partial sums of the move's trial blocks (lane b loads block b,
-// b + 64, ...), reduced per role; every lane ends with the totals
+// wave 0: the partial sums of the move's trial blocks.  Lane b loads blocks
+// b, b + 64, ... (issued early, into registers: kUpd1P per lane, enough for
+// every launch shape of trees the fused chain runs) and upd1_reduce adds them
+// per role in block order; every lane ends with the totals.
+struct Upd1Part {
+    double q0[kUpd1P], q1[kUpd1P], q2[kUpd1P];
+};
+template <typename T>
+__device__ __forceinline__ void upd1_load(const DevTree<T>& t, const McArgs& a, Upd1Part& P) {
+    const int per = trial_blocks(t.max_depth, a.tiles), lane = threadIdx.x & 63;
+#pragma unroll
+    for (int u = 0; u < kUpd1P; ++u) {
+        const int b = lane + 64 * u;
+        const double* q = a.s.part + kPartStride * (size_t)(b < per ? b : 0);
+        P.q0[u] = b < per ? q[0] : 0.0;
+        P.q1[u] = b < per ? q[1] : 0.0;
+        P.q2[u] = b < per ? q[2] : 0.0;
+    }
+}
+template <typename T>
+__device__ __forceinline__ void upd1_reduce(const DevTree<T>& t, const McArgs& a, const Upd1Part& P, double* v) {
+    const Tiles& g = a.tiles;
+    const int nL = n_levels_pw(t.max_depth), per = trial_blocks(t.max_depth, g), lane = threadIdx.x & 63;
+    const int d0 = g.win, d1 = d0 + 2 * nL * g.diff, d2 = d1 + g.near;
+#pragma unroll
+    for (int c = 0; c < kAccPerTrial; ++c) v[c] = 0.0;
+#pragma unroll
+    for (int u = 0; u < kUpd1P; ++u) {
+        const int b = lane + 64 * u;
+        if (b >= per) continue;
+        if (b < d0) {
+            v[0] += P.q0[u];
+        } else if (b < d1) {
+            v[1] += P.q0[u];
+        } else if (b < d2) {
+            v[2] += P.q0[u];
+            v[4] += P.q1[u];
+            v[6] += P.q2[u];
+        } else {
+            v[3] += P.q0[u];
+            v[5] += P.q1[u];
+        }
+    }
+    for (int o = 1; o < 64; o <<= 1)
+#pragma unroll
+        for (int c = 0; c < kAccPerTrial; ++c) v[c] += __shfl_xor(v[c], o);
+}
 template <typename T>
 __device__ __forceinline__ void upd1_partials(const DevTree<T>& t, const McArgs& a, double* v) {
-    sum_trial_partials(t, a, 0, threadIdx.x, 64, v);
+    Upd1Part P;
+    upd1_load(t, a, P);
+    upd1_reduce(t, a, P, v);
 }
 
+// the decision from the partial sums v, the trial state's precomputed
+// self-pair term and threshold, and the marking row's room flag
 template <typename T>
 __device__ __forceinline__ int upd1_decide(const DevTree<T>& t, const McArgs& a, const TrialInfo& ti, int m,
-                                           const double* v, bool writer) {
+                                           const double* v, int room, bool writer) {
     const int ovl = (int)v[6];
-    double dE;
-    if (writer) {
+    double dE = v[0] + v[1] + v[2] - v[3] - ti.self;  // src/energy_shift.cpp:67
+    if (writer && a.ml.parts) {
+        double* p = a.ml.parts + 5 * m;
+        p[0] = v[0];
+        p[1] = v[1];
+        p[2] = v[2];
+        p[3] = v[3];
+        p[4] = ti.self;
+    }
+    if (writer && a.ml.stats) {
         int o2;
-        dE = assemble_trial(t, a, m & 1, m, v, &o2);  // also the optional parts and counts
-    } else {
-        dE = v[0] + v[1] + v[2] - v[3] - self_term(t, ti);
+        (void)assemble_trial(t, a, m & 1, m, v, &o2);  // the per-move counts
     }
     dE = ovl > 0 ? (double)INFINITY : a.lB * dE;  // a hard-sphere overlap is an infinite energy
-    int ok = ovl == 0 && dE < mh_threshold(a.ml.u[m], a.beta);
-    if (ok && !a.s.plan[3]) {  // the target leaf is full: the chain stops here
+    int ok = ovl == 0 && dE < ti.thr;
+    if (ok && !room) {  // the target leaf is full: the chain stops here
         ok = 0;
         if (writer) atomicOr(t.err, (int)kErrLeafOverflow);
     }
@@ -2009,52 +2130,55 @@ __device__ __forceinline__ int upd1_decide(const DevTree<T>& t, const McArgs& a,
 
 template <typename T>
 __device__ void upd1_block0(const DevTree<T>& t, const McArgs& a) {
-    __shared__ TrialInfo nxt;
-    __shared__ int ok_sh;
-    const int m = move_base(a), sl = m & 1;
+    __shared__ int ok_sh, redo;
+    const int sl = ring_slot(a);
     const TrialInfo& ti = a.s.info[sl];
+    const int m = ti.m;
     const bool has_next = m + 1 < a.n_moves;
-    // move m + 1 from the current state, while the decision is pending
-    if (threadIdx.x == 64 && has_next) trial_setup(t, a.ml.idx[m + 1], a.ml.d + 3 * (m + 1), nxt);
+    // loads of the relocation and of the next move's particle, in flight with the partials
+    const int lt = ti.path_t[ti.lam_t], lo = ti.path_o[ti.lam_o];
+    int s_i = 0, cnt_t = 0, st_t = 0, nxt_i = -1, room = 0;
+    if (threadIdx.x == 0) {
+        room = a.s.plan[3];
+        s_i = t.slot_of[ti.i];
+        st_t = t.slot_start[lt];
+        cnt_t = t.slot_count[lt];
+        nxt_i = has_next ? a.s.info[sl ^ 1].i : -1;  // set up by the trial kernel
+    }
     if (threadIdx.x < 64) {
         double v[kAccPerTrial];
         upd1_partials(t, a, v);
-        if (threadIdx.x == 0) ok_sh = upd1_decide(t, a, ti, m, v, true);
-    }
-    __syncthreads();
-    const int ok = ok_sh;
-    if (threadIdx.x == 0) {
-        if (ok) {  // src/tree.cpp:454-463, 583-596
-            const int lt = ti.path_t[ti.lam_t], lo = ti.path_o[ti.lam_o];
-            if (lt == lo) {
-                const int s = t.slot_of[ti.i];
-                t.px[s] = ti.t[0];
-                t.py[s] = ti.t[1];
-                t.pz[s] = ti.t[2];
-            } else {
-                relocate(t, ti, 0);
-                t.slot_count[lt] += 1;
-            }
-        }
-        if (has_next) {
-            if (ok && nxt.i == ti.i) {  // the same particle moves again, from this move's target
-                for (int c = 0; c < 3; ++c) {
-                    nxt.o[c] = ti.t[c];
-                    nxt.t[c] = wrap(nxt.o[c] + nxt.dv[c], t.L);
+        if (threadIdx.x == 0) {
+            const int ok = upd1_decide(t, a, ti, m, v, room, true);
+            ok_sh = ok;
+            if (ok) {  // src/tree.cpp:454-463, 583-596
+                if (lt == lo) {
+                    t.px[s_i] = ti.t[0];
+                    t.py[s_i] = ti.t[1];
+                    t.pz[s_i] = ti.t[2];
+                } else {  // appended to the target leaf; its old slot becomes a hole
+                    const int ns = st_t + cnt_t;
+                    t.px[ns] = ti.t[0];
+                    t.py[ns] = ti.t[1];
+                    t.pz[ns] = ti.t[2];
+                    t.pq[ns] = ti.q;
+                    t.porig[ns] = ti.i;
+                    t.slot_of[ti.i] = ns;
+                    t.leaf_of[ti.i] = lt;
+                    t.px[s_i] = 0;
+                    t.py[s_i] = 0;
+                    t.pz[s_i] = 0;
+                    t.pq[s_i] = 0;
+                    t.porig[s_i] = -1;
+                    t.slot_count[lt] = cnt_t + 1;
                 }
-                nxt.lam_o = ti.lam_t;
-                for (int l = 0; l < kMaxLevels; ++l) nxt.path_o[l] = ti.path_t[l];
-                nxt.lam_t = locate(t, nxt.t[0], nxt.t[1], nxt.t[2], nxt.path_t);
             }
-            a.s.info[sl ^ 1] = nxt;
-            if (a.ml.stats) {
-                a.ml.stats[kStatsPerMove * (m + 1) + 6] = 0;
-                a.ml.stats[kStatsPerMove * (m + 1) + 7] = 0;
-            }
+            redo = ok && nxt_i == ti.i;
         }
     }
     __syncthreads();
-    if (has_next) form_rows(t, nxt, tabs_of<T>(a.s, table_stride(t.max_depth), sl ^ 1));
+    // the next move is the same particle's, now at this move's target
+    if (redo) prep_body(t, a, m + 1, sl ^ 1, &ti);
 }
 
 template <typename T>
@@ -2065,13 +2189,17 @@ __device__ void upd1_body(const DevTree<T>& t, const McArgs& a) {
         upd1_block0(t, a);
         return;
     }
-    const int m = move_base(a), sl = m & 1;
+    const int sl = ring_slot(a);
     const TrialInfo& ti = a.s.info[sl];
-    const int nD = a.s.plan[0];
+    const int m = ti.m;
+    Upd1Part P;
+    if (threadIdx.x < 64) upd1_load(t, a, P);  // in flight with everything below
+    const int nD = a.s.plan[0], room = a.s.plan[3];
     constexpr int CH = kUpd1J * TB;  // elements per item
     const int nR = (t.Mw + CH - 1) / CH, nA = (t.Md + CH - 1) / CH;
     const int it = blockIdx.x - 1;
     const bool have = it < nR + nD * nA;  // uniform
+    if (!have) return;  // the grid is sized for the most arrays a move can mark
     const int D = t.max_depth, nL = n_levels_pw(D);
     const Cplx<T>* tb = tabs_of<T>(a.s, table_stride(D), sl);
     bool is_root = true;
@@ -2138,7 +2266,7 @@ __device__ void upd1_body(const DevTree<T>& t, const McArgs& a) {
         }
     }
     double v[kAccPerTrial];
-    if (threadIdx.x < 64) upd1_partials(t, a, v);
+    if (threadIdx.x < 64) upd1_reduce(t, a, P, v);
     Cplx<T> val[kUpd1J];
 #pragma unroll
     for (int j = 0; j < kUpd1J; ++j) {
@@ -2156,14 +2284,17 @@ __device__ void upd1_body(const DevTree<T>& t, const McArgs& a) {
             val[j].y = fma(xv.y, yz.x, val[j].y);
         }
     }
-    if (threadIdx.x == 0) ok_sh = upd1_decide(t, a, ti, m, v, false);
+    if (threadIdx.x == 0) ok_sh = upd1_decide(t, a, ti, m, v, room, false);
     __syncthreads();
     if (!ok_sh || !have) return;
     const T sy = kind == 1 ? T(-1) : T(1);  // in[C] += conj(tau s)
 #pragma unroll
     for (int j = 0; j < kUpd1J; ++j) {
         const int e = chunk * CH + threadIdx.x + TB * j;
-        if (e < M) dst[e] = {pf[j].x + val[j].x, pf[j].y + sy * val[j].y};
+        if (e < M) {  // streaming stores: the next kernel reads them from memory, not from this L2
+            __builtin_nontemporal_store(pf[j].x + val[j].x, &dst[e].x);
+            __builtin_nontemporal_store(pf[j].y + sy * val[j].y, &dst[e].y);
+        }
     }
 }
 
@@ -2209,7 +2340,7 @@ template <typename T>
 void launch_trial(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     // Gram rows, trial rows, the marking row (premark: its bitmap is the dynamic shared memory)
     const bool pm = premark(a);
-    const int gy = gram_rows(t, a) + a.K + (pm ? 1 : 0);
+    const int gy = gram_rows(t, a) + a.K + (pm ? 1 : 0) + (pm && a.ring ? 1 : 0);
     const size_t sh = pm ? sizeof(unsigned) * ((2 * (size_t)t.n_pw_nodes + 31) / 32) : 0;
     const dim3 grid(trial_blocks(t.max_depth, a.tiles), gy);
     if (pm)

@@ -420,7 +420,8 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
                 h->since_compact = 0;
             }
         };
-        if (mode == 0 && !(h->prop_rf > 0.0) && h->tree.max_depth <= kFusedMaxDepth) {
+        if (mode == 0 && !(h->prop_rf > 0.0) && h->tree.max_depth <= kFusedMaxDepth &&
+            trial_blocks(h->tree.max_depth, h->tiles) <= 64 * kUpd1P) {
             // fused single-move chain: per move a trial kernel (with the update
             // plan's marking row) and an update kernel whose blocks take the
             // decision themselves; the update kernel's block 0 moves the

@@ -70,6 +70,7 @@ struct MoveBuf {
 constexpr int kCompactEvery = 2048;
 // moves per captured graph of the plain chain
 constexpr int kPlainGraphMoves = 64;
+static_assert(kPlainGraphMoves % 2 == 0, "the fused chain's ring slot is the graph-local move's parity");
 // the fused single-move chain packs box cells in 9 bits (levels <= 9, i.e.
 // trees of at most 11 levels); deeper trees run the four-kernel chain
 constexpr int kFusedMaxDepth = 11;
