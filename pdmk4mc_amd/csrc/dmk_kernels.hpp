@@ -39,6 +39,14 @@ struct TrialInfo {
     double thr;   // its Metropolis threshold -ln(u) / beta
 };
 
+// Fused single-move chain: a move two ahead, prefetched by the update kernel
+// of move m for the trial kernel of move m + 1 (which sets up move m + 2):
+// its particle and where that particle is after move m.
+struct NextMove {
+    double o[3], dv[3], q, u;
+    int i, m;
+};
+
 // Phase rows of one trial (Cplx<T>, kRowStride apart):
 //   root:  [endpoint][axis]                     e^{-i (m-n_w) dk_0 x_a}
 //   level: [endpoint][level-2][axis][shift+1]    e^{-i (m-n_d) dk_l (x_a + shift L)}
@@ -71,6 +79,9 @@ constexpr int kMaxRoleBlocks = 16;  // upper bound of win, diff, near (scratch s
 // reads at most 64 kUpd1P trial blocks (304 at the deepest fused tree, 11
 // levels, with the largest tiles)
 constexpr int kUpd1P = 6;
+// near-list entries of the next move that the fused update's block 0 writes:
+// 4 per thread of its 256, so the fused chain needs 2 max_near <= kUpd1Near
+constexpr int kUpd1Near = 4 * 256;
 __host__ __device__ inline int trial_blocks(int max_depth, const Tiles& g) {
     const int nL = max_depth >= 4 ? max_depth - 3 : 0;
     return g.win + 2 * nL * g.diff + 2 * g.near;
@@ -103,6 +114,10 @@ __host__ __device__ inline int gram_blocks(int Mw) { return (Mw + kGramModes - 1
 // device scratch of the Monte Carlo kernels (sized for kMaxBatch trials)
 struct McScratch {
     TrialInfo* info;
+    // fused single-move chain: near-field entries of each ring slot's move,
+    // [slot][endpoint][max_near] x {slot start, particle count, image code, leaf}
+    int4* near;
+    NextMove* nm;          // fused chain: [2], the move m + 2 prefetched by move m's update (slot m & 1)
     void* tabs;            // Cplx<T>, table_stride(D) per trial
     double* part;          // per trial and block: kPartStride values
     TrialRec* rec;         // per trial of a batch

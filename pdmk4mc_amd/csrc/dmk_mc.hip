@@ -132,12 +132,25 @@ __device__ __forceinline__ void setup_move(const DevTree<T>& t, const McArgs& a,
 // by one block: paths, the path boxes' expansion slots, the endpoints' near
 // lists and the phase rows.  prev (may be null): the particle was moved by
 // the previous move and sits at prev->t (its leaf path is prev's target path).
+// nm (may be null): the move's particle as prefetched (position after the
+// previous move), so that only the paths remain to be found.
 template <typename T>
-__device__ void prep_body(const DevTree<T>& t, const McArgs& a, int m, int sl, const TrialInfo* prev) {
+__device__ void prep_body(const DevTree<T>& t, const McArgs& a, int m, int sl, const TrialInfo* prev,
+                          const NextMove* nm = nullptr) {
     __shared__ TrialInfo ti;
     const int D = t.max_depth;
     if (threadIdx.x == 0) {
-        if (prev) {
+        if (nm && !prev) {
+            ti.i = nm->i;
+            ti.q = nm->q;
+            for (int c = 0; c < 3; ++c) {
+                ti.o[c] = nm->o[c];
+                ti.dv[c] = nm->dv[c];
+                ti.t[c] = wrap(ti.o[c] + ti.dv[c], t.L);
+            }
+            ti.lam_o = locate(t, ti.o[0], ti.o[1], ti.o[2], ti.path_o);
+            ti.lam_t = locate(t, ti.t[0], ti.t[1], ti.t[2], ti.path_t);
+        } else if (prev) {
             ti.i = prev->i;
             ti.q = prev->q;
             for (int c = 0; c < 3; ++c) {
@@ -168,7 +181,7 @@ __device__ void prep_body(const DevTree<T>& t, const McArgs& a, int m, int sl, c
     if (threadIdx.x == 0) {
         ti.m = m;
         ti.self = self_term(t, ti);
-        ti.thr = mh_threshold(a.ml.u[m], a.beta);
+        ti.thr = mh_threshold(nm && !prev ? nm->u : a.ml.u[m], a.beta);
         a.s.info[sl] = ti;
         if (a.ml.stats) {
             a.ml.stats[kStatsPerMove * m + 6] = 0;
@@ -178,14 +191,53 @@ __device__ void prep_body(const DevTree<T>& t, const McArgs& a, int m, int sl, c
     form_rows(t, ti, tabs_of<T>(a.s, table_stride(D), sl));
 }
 
+// Fused chain: the near-field entries of the move in ring slot sl (its
+// TrialInfo written), by a whole block: slot range and image code of every
+// near-list leaf of both endpoints.  Counts are read now; with moved >= 0 the
+// caller has just appended a particle to leaf `moved` (count + 1) in the
+// same kernel.
+template <typename T>
+__device__ void fill_near(const DevTree<T>& t, const McArgs& a, const TrialInfo& ti, int sl, int moved, int moved_cnt) {
+    const int mx = t.max_near;
+    for (int w = threadIdx.x; w < 2 * mx; w += blockDim.x) {
+        const int ep = w / mx, k = w % mx;
+        if (k >= ti.ncnt[ep]) continue;
+        const int nb = t.near_ids[ti.noff[ep] + k];
+        const int code = t.near_img[ti.noff[ep] + k];
+        const int cnt = nb == moved ? moved_cnt : t.slot_count[nb];
+        a.s.near[(size_t)(2 * sl + ep) * mx + k] = make_int4(t.slot_start[nb], cnt, code, nb);
+    }
+}
+
+// the record of move m (its particle from the current state) into nm slot sl
+template <typename T>
+__device__ __forceinline__ void load_next_move(const DevTree<T>& t, const McArgs& a, int m, NextMove& r) {
+    const int i = a.ml.idx[m], s = t.slot_of[i];
+    r.i = i;
+    r.m = m;
+    r.q = t.pq[s];
+    r.o[0] = t.px[s];
+    r.o[1] = t.py[s];
+    r.o[2] = t.pz[s];
+    for (int c = 0; c < 3; ++c) r.dv[c] = a.ml.d[3 * m + c];
+    r.u = a.ml.u[m];
+}
+
 template <typename T>
 __device__ __forceinline__ void setup_body(const DevTree<T>& t, const McArgs& a) {
     __shared__ TrialInfo ti;
     if (*t.err) return;
     const int tr = blockIdx.x, m = move_base(a) + tr;
     if (tr >= keff(a)) return;
-    if (a.ring) {
+    if (a.ring) {  // a run's first move (or the next after a compaction) and its successor's particle
+        if (threadIdx.x == 64 && m + 1 < a.n_moves) {
+            NextMove r;
+            load_next_move(t, a, m + 1, r);
+            a.s.nm[(m + 1) & 1] = r;
+        }
         prep_body(t, a, m, m & 1, nullptr);
+        __syncthreads();
+        fill_near(t, a, a.s.info[m & 1], m & 1, -1, 0);
         return;
     }
     const int sl = trial_slot(a, tr, m);
@@ -814,7 +866,7 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
         // update kernel redoes it when this move relocates the same particle)
         if ((int)blockIdx.y == gy + a.K + 1) {
             const int mn = a.s.info[ring_slot(a)].m + 1;
-            if (blockIdx.x == 0 && mn < a.n_moves) prep_body(t, a, mn, mn & 1, nullptr);
+            if (blockIdx.x == 0 && mn < a.n_moves) prep_body(t, a, mn, mn & 1, nullptr, &a.s.nm[mn & 1]);
             return;
         }
     }
@@ -893,14 +945,23 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
         __shared__ int es0[64], epre[65];
         __shared__ double esh[64][3];
         const int nk = cnt > r % NB ? (cnt - r % NB + NB - 1) / NB : 0;
+        const int4* ent = a.s.near + (size_t)(2 * sl + ep) * t.max_near;  // fused chain: prepared entries
         for (int j0 = 0; j0 < nk; j0 += 64) {
             const int ne = min(64, nk - j0);
             if (threadIdx.x < ne) {
                 const int k = r % NB + NB * (j0 + threadIdx.x);
-                const int nb = t.near_ids[off + k];
-                const int code = t.near_img[off + k];
-                es0[threadIdx.x] = t.slot_start[nb];
-                epre[threadIdx.x + 1] = t.slot_count[nb];
+                int code;
+                if (a.ring) {
+                    const int4 e4 = ent[k];
+                    es0[threadIdx.x] = e4.x;
+                    epre[threadIdx.x + 1] = e4.y;
+                    code = e4.z;
+                } else {
+                    const int nb = t.near_ids[off + k];
+                    code = t.near_img[off + k];
+                    es0[threadIdx.x] = t.slot_start[nb];
+                    epre[threadIdx.x + 1] = t.slot_count[nb];
+                }
                 esh[threadIdx.x][0] = (code % 3 - 1) * t.L;
                 esh[threadIdx.x][1] = ((code / 3) % 3 - 1) * t.L;
                 esh[threadIdx.x][2] = (code / 9 - 1) * t.L;
@@ -913,20 +974,40 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
             __syncthreads();
             const int total = epre[ne];
             nscan += total;
+            // two particles per pass (u and u + TB): both loads in flight together
             int e = 0;
-            for (int u = threadIdx.x; u < total; u += blockDim.x) {
-                while (epre[e + 1] <= u) ++e;
-                const int s = es0[e] + (u - epre[e]);
-                const double sx = esh[e][0], sy = esh[e][1], sz = esh[e][2];
-                const double ex = cx - (t.px[s] + sx), ey = cy - (t.py[s] + sy), ez = cz - (t.pz[s] + sz);
-                const T dx = (T)ex, dy = (T)ey, dz = (T)ez;
-                const T r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 > eps && r2 < b2)
-                    acc += (double)((T)t.pq[s] * residual_poly_over_r<T>(sqrt(r2), inv_b, t.poly, t.n_poly));
-                if (hs && s != own && t.porig[s] >= 0 && ex * ex + ey * ey + ez * ez < a.sigma2) ++novl;
-                if (ft && s != own && t.porig[s] >= 0) {
-                    const double r2d = ex * ex + ey * ey + ez * ez;
-                    if (r2d > 0.0 && r2d < rf2) feature_pair(ex, ey, ez, r2d, t.pq[s], a.prop.rf, fsum);
+            for (int u0 = threadIdx.x; u0 < total; u0 += 2 * TB) {
+                int sl2[2];
+                double px2[2], py2[2], pz2[2], pq2[2], shx[2], shy[2], shz[2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int u = u0 + h * TB;
+                    const bool in = u < total;
+                    while (in && epre[e + 1] <= u) ++e;  // u increases: the entry index only moves forward
+                    sl2[h] = in ? es0[e] + (u - epre[e]) : -1;
+                    const int sv = in ? sl2[h] : es0[0];
+                    px2[h] = t.px[sv];
+                    py2[h] = t.py[sv];
+                    pz2[h] = t.pz[sv];
+                    pq2[h] = t.pq[sv];
+                    shx[h] = esh[e][0];
+                    shy[h] = esh[e][1];
+                    shz[h] = esh[e][2];
+                }
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int s = sl2[h];
+                    if (s < 0) continue;
+                    const double ex = cx - (px2[h] + shx[h]), ey = cy - (py2[h] + shy[h]), ez = cz - (pz2[h] + shz[h]);
+                    const T dx = (T)ex, dy = (T)ey, dz = (T)ez;
+                    const T r2 = dx * dx + dy * dy + dz * dz;
+                    if (r2 > eps && r2 < b2)
+                        acc += (double)((T)pq2[h] * residual_poly_over_r<T>(sqrt(r2), inv_b, t.poly, t.n_poly));
+                    if (hs && s != own && t.porig[s] >= 0 && ex * ex + ey * ey + ez * ez < a.sigma2) ++novl;
+                    if (ft && s != own && t.porig[s] >= 0) {
+                        const double r2d = ex * ex + ey * ey + ez * ez;
+                        if (r2d > 0.0 && r2d < rf2) feature_pair(ex, ey, ez, r2d, pq2[h], a.prop.rf, fsum);
+                    }
                 }
             }
             __syncthreads();  // the entry lists are consumed before the next pass
@@ -2137,6 +2218,7 @@ __device__ void upd1_block0(const DevTree<T>& t, const McArgs& a) {
     const bool has_next = m + 1 < a.n_moves;
     // loads of the relocation and of the next move's particle, in flight with the partials
     const int lt = ti.path_t[ti.lam_t], lo = ti.path_o[ti.lam_o];
+    __shared__ int moved_sh, mcnt_sh;
     int s_i = 0, cnt_t = 0, st_t = 0, nxt_i = -1, room = 0;
     if (threadIdx.x == 0) {
         room = a.s.plan[3];
@@ -2145,6 +2227,30 @@ __device__ void upd1_block0(const DevTree<T>& t, const McArgs& a) {
         cnt_t = t.slot_count[lt];
         nxt_i = has_next ? a.s.info[sl ^ 1].i : -1;  // set up by the trial kernel
     }
+    // move m + 2's particle (for the trial kernel after next), in flight with the decision
+    const bool has_nn = m + 2 < a.n_moves;
+    NextMove r2;
+    if (threadIdx.x == 64 && has_nn) load_next_move(t, a, m + 2, r2);
+    // the next move's near-list leaves (ids, codes, slot ranges), in flight with the decision
+    const TrialInfo& tn = a.s.info[sl ^ 1];
+    const int mx = t.max_near;
+    constexpr int NPT = kUpd1Near / TB;  // entries per thread (2 max_near <= kUpd1Near, checked by the host)
+    int e_nb[NPT], e_code[NPT], e_st[NPT], e_cnt[NPT];
+#pragma unroll
+    for (int u = 0; u < NPT; ++u) {
+        const int w = threadIdx.x + TB * u, ep = w / mx, k = w % mx;
+        e_nb[u] = -1;
+        if (has_next && w < 2 * mx && k < tn.ncnt[ep]) {
+            e_nb[u] = t.near_ids[tn.noff[ep] + k];
+            e_code[u] = t.near_img[tn.noff[ep] + k];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < NPT; ++u)
+        if (e_nb[u] >= 0) {
+            e_st[u] = t.slot_start[e_nb[u]];
+            e_cnt[u] = t.slot_count[e_nb[u]];  // before this kernel's relocation (patched below)
+        }
     if (threadIdx.x < 64) {
         double v[kAccPerTrial];
         upd1_partials(t, a, v);
@@ -2174,11 +2280,30 @@ __device__ void upd1_block0(const DevTree<T>& t, const McArgs& a) {
                 }
             }
             redo = ok && nxt_i == ti.i;
+            moved_sh = ok && lt != lo ? lt : -1;
+            mcnt_sh = cnt_t + 1;
         }
     }
     __syncthreads();
-    // the next move is the same particle's, now at this move's target
-    if (redo) prep_body(t, a, m + 1, sl ^ 1, &ti);
+    if (threadIdx.x == 64 && has_nn) {
+        if (r2.i == ti.i && ok_sh)  // this move's particle: now at its target
+            for (int c = 0; c < 3; ++c) r2.o[c] = ti.t[c];
+        a.s.nm[m & 1] = r2;  // (m + 2) & 1
+    }
+    if (redo) {  // the next move is the same particle's, now at this move's target
+        prep_body(t, a, m + 1, sl ^ 1, &ti);
+        __syncthreads();
+        fill_near(t, a, a.s.info[sl ^ 1], sl ^ 1, moved_sh, mcnt_sh);
+        return;
+    }
+    const int moved = moved_sh, mcnt = mcnt_sh;
+#pragma unroll
+    for (int u = 0; u < NPT; ++u) {
+        const int w = threadIdx.x + TB * u, ep = w / mx, k = w % mx;
+        if (e_nb[u] >= 0)
+            a.s.near[(size_t)(2 * (sl ^ 1) + ep) * mx + k] =
+                make_int4(e_st[u], e_nb[u] == moved ? mcnt : e_cnt[u], e_code[u], e_nb[u]);
+    }
 }
 
 template <typename T>
@@ -2291,10 +2416,7 @@ __device__ void upd1_body(const DevTree<T>& t, const McArgs& a) {
 #pragma unroll
     for (int j = 0; j < kUpd1J; ++j) {
         const int e = chunk * CH + threadIdx.x + TB * j;
-        if (e < M) {  // streaming stores: the next kernel reads them from memory, not from this L2
-            __builtin_nontemporal_store(pf[j].x + val[j].x, &dst[e].x);
-            __builtin_nontemporal_store(pf[j].y + sy * val[j].y, &dst[e].y);
-        }
+        if (e < M) dst[e] = {pf[j].x + val[j].x, pf[j].y + sy * val[j].y};
     }
 }
 

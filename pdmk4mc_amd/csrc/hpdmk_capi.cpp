@@ -285,6 +285,8 @@ Handle* create_handle(HPDMKParams params, int precision, int n_src, const double
         throw HpdmkError(HPDMK_EINVAL, "tree has more expansion boxes than the update planner's bitmap holds");
     h->sc.dest = dalloc<int>(2 * (size_t)h->n_pw + 2, o);
     h->sc.dinfo = dalloc<int>(2 * (size_t)h->n_pw + 2, o);
+    h->sc.near = dalloc<int4>(4 * (size_t)std::max(max_near, 1), o);
+    h->sc.nm = dalloc<NextMove>(2, o);
     h->sc.plan = dalloc<int>(4, o);
     h->probe = dalloc<unsigned long long>(2 * kProbes, o);
     HIPX(hipMemset(h->probe, 0, sizeof(unsigned long long) * 2 * kProbes));
@@ -421,7 +423,7 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
             }
         };
         if (mode == 0 && !(h->prop_rf > 0.0) && h->tree.max_depth <= kFusedMaxDepth &&
-            trial_blocks(h->tree.max_depth, h->tiles) <= 64 * kUpd1P) {
+            trial_blocks(h->tree.max_depth, h->tiles) <= 64 * kUpd1P && 2 * dv.max_near <= kUpd1Near) {
             // fused single-move chain: per move a trial kernel (with the update
             // plan's marking row) and an update kernel whose blocks take the
             // decision themselves; the update kernel's block 0 moves the
@@ -433,6 +435,19 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
             a.ring = 1;
             a.n_moves = n;
             timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
+            // a slot compaction moves particles inside their leaves: the next
+            // move's prepared near-field entries are formed again after it
+            auto compact_fused = [&](int moved, int next) {
+                h->since_compact += moved;
+                if (h->since_compact < kCompactEvery) return;
+                launch_compact(dv, h->leaves, h->n_leaves, h->stream);
+                h->since_compact = 0;
+                if (next < n) {
+                    McArgs s = a;
+                    s.move0 = next;
+                    launch_setup(dv, s, h->stream);
+                }
+            };
             int m = 0;
             if (!h->profiling && n >= 2 * kPlainGraphMoves) {
                 if (!h->step_ctr) h->step_ctr = dalloc<int>(1, h->owned);
@@ -458,14 +473,14 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
                 HIPX(hipGraphInstantiate(&gr.x, gr.g, nullptr, nullptr, 0));
                 for (; m + kPlainGraphMoves <= n; m += kPlainGraphMoves) {
                     HIPX(hipGraphLaunch(gr.x, h->stream));
-                    compact_if_due(kPlainGraphMoves);
+                    compact_fused(kPlainGraphMoves, m + kPlainGraphMoves);
                 }
             }
             for (; m < n; ++m) {
                 a.move0 = m;
                 timed(h, kKTrial, [&](hipEvent_t x, hipEvent_t y) { launch_trial(dv, a, h->stream, x, y); });
                 timed(h, kKUpdate, [&](hipEvent_t x, hipEvent_t y) { launch_apply(dv, a, h->stream, x, y); });
-                compact_if_due(1);
+                compact_fused(1, m + 1);
             }
         } else if (mode == 0 && !h->profiling && n >= 2 * kPlainGraphMoves) {
             // plain chain from a HIP graph of kPlainGraphMoves moves: the
