@@ -230,7 +230,7 @@ int hpdmk_mlp_train(int n, const double* feat, const double* delta, const double
 
 /* ---- launch shape and tile autotuner (no reference counterpart) --------
  * tiles[4]: blocks per trial for the window sum, per (endpoint, difference
- * level), per endpoint's near field (each 1..16), and blocks of the update
+ * level), per endpoint's near field (each 1..32), and blocks of the update
  * kernel; batch: trials per batched step used when hpdmk_mc_run gets batch 0.
  * The chain's results do not depend on them (up to FP summation order). */
 int hpdmk_set_tiles(hpdmk_tree tree, const int* tiles, int batch);

@@ -35,6 +35,7 @@ struct TrialInfo {
     int pws[2][kMaxLevels];
     int noff[2], ncnt[2];
     int m;        // the move's index in the move list
+    double u;     // its Metropolis uniform
     double self;  // its self-pair correction q^2 R_{b_2}(d_min) (src/energy_shift.cpp:58)
     double thr;   // its Metropolis threshold -ln(u) / beta
 };
@@ -74,10 +75,10 @@ enum TrialMode : int {
 struct Tiles {
     int win = 2, diff = 8, near = 8, apply = 1024;
 };
-constexpr int kMaxRoleBlocks = 16;  // upper bound of win, diff, near (scratch sizing)
+constexpr int kMaxRoleBlocks = 32;  // upper bound of win, diff, near (scratch sizing)
 // partial sums per lane of the fused single-move update's decision wave: it
-// reads at most 64 kUpd1P trial blocks (304 at the deepest fused tree, 11
-// levels, with the largest tiles)
+// reads at most 64 kUpd1P trial blocks (the host runs launch shapes with more
+// through the four-kernel chain)
 constexpr int kUpd1P = 6;
 // near-list entries of the next move that the fused update's block 0 writes:
 // 4 per thread of its 256, so the fused chain needs 2 max_near <= kUpd1Near
@@ -115,7 +116,8 @@ __host__ __device__ inline int gram_blocks(int Mw) { return (Mw + kGramModes - 1
 struct McScratch {
     TrialInfo* info;
     // fused single-move chain: near-field entries of each ring slot's move,
-    // [slot][endpoint][max_near] x {slot start, particle count, image code, leaf}
+    // [slot][endpoint][1 + max_near]: the count, then {slot start, particle
+    // count, image code, leaf} per near-list leaf
     int4* near;
     NextMove* nm;          // fused chain: [2], the move m + 2 prefetched by move m's update (slot m & 1)
     void* tabs;            // Cplx<T>, table_stride(D) per trial

@@ -26,6 +26,8 @@
 
 namespace hpdmk_amd {
 
+constexpr int kMaxGridD = 53;  // largest 2n+1 of any digits setting
+
 template <typename T>
 __device__ __forceinline__ Cplx<T>* tabs_of(const McScratch& s, int stride, int tr) {
     return static_cast<Cplx<T>*>(s.tabs) + (size_t)tr * stride;
@@ -134,9 +136,11 @@ __device__ __forceinline__ void setup_move(const DevTree<T>& t, const McArgs& a,
 // the previous move and sits at prev->t (its leaf path is prev's target path).
 // nm (may be null): the move's particle as prefetched (position after the
 // previous move), so that only the paths remain to be found.
+// rows: also the self-pair term, the threshold and the phase rows (else the
+// update kernel's row block forms them: upd1_rows)
 template <typename T>
 __device__ void prep_body(const DevTree<T>& t, const McArgs& a, int m, int sl, const TrialInfo* prev,
-                          const NextMove* nm = nullptr) {
+                          const NextMove* nm = nullptr, bool rows = true) {
     __shared__ TrialInfo ti;
     const int D = t.max_depth;
     if (threadIdx.x == 0) {
@@ -180,15 +184,18 @@ __device__ void prep_body(const DevTree<T>& t, const McArgs& a, int m, int sl, c
     __syncthreads();
     if (threadIdx.x == 0) {
         ti.m = m;
-        ti.self = self_term(t, ti);
-        ti.thr = mh_threshold(nm && !prev ? nm->u : a.ml.u[m], a.beta);
+        ti.u = nm && !prev ? nm->u : a.ml.u[m];
+        if (rows) {
+            ti.self = self_term(t, ti);
+            ti.thr = mh_threshold(ti.u, a.beta);
+        }
         a.s.info[sl] = ti;
         if (a.ml.stats) {
             a.ml.stats[kStatsPerMove * m + 6] = 0;
             a.ml.stats[kStatsPerMove * m + 7] = 0;
         }
     }
-    form_rows(t, ti, tabs_of<T>(a.s, table_stride(D), sl));
+    if (rows) form_rows(t, ti, tabs_of<T>(a.s, table_stride(D), sl));
 }
 
 // Fused chain: the near-field entries of the move in ring slot sl (its
@@ -196,16 +203,19 @@ __device__ void prep_body(const DevTree<T>& t, const McArgs& a, int m, int sl, c
 // near-list leaf of both endpoints.  Counts are read now; with moved >= 0 the
 // caller has just appended a particle to leaf `moved` (count + 1) in the
 // same kernel.
+// Layout of each (slot, endpoint) list: max_near + 1 int4, the first holding
+// the entry count (so a reader needs no other state to find its entries).
 template <typename T>
 __device__ void fill_near(const DevTree<T>& t, const McArgs& a, const TrialInfo& ti, int sl, int moved, int moved_cnt) {
     const int mx = t.max_near;
+    if (threadIdx.x < 2) a.s.near[(size_t)(2 * sl + threadIdx.x) * (mx + 1)] = make_int4(ti.ncnt[threadIdx.x], 0, 0, 0);
     for (int w = threadIdx.x; w < 2 * mx; w += blockDim.x) {
         const int ep = w / mx, k = w % mx;
         if (k >= ti.ncnt[ep]) continue;
         const int nb = t.near_ids[ti.noff[ep] + k];
         const int code = t.near_img[ti.noff[ep] + k];
         const int cnt = nb == moved ? moved_cnt : t.slot_count[nb];
-        a.s.near[(size_t)(2 * sl + ep) * mx + k] = make_int4(t.slot_start[nb], cnt, code, nb);
+        a.s.near[(size_t)(2 * sl + ep) * (mx + 1) + 1 + k] = make_int4(t.slot_start[nb], cnt, code, nb);
     }
 }
 
@@ -866,7 +876,7 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
         // update kernel redoes it when this move relocates the same particle)
         if ((int)blockIdx.y == gy + a.K + 1) {
             const int mn = a.s.info[ring_slot(a)].m + 1;
-            if (blockIdx.x == 0 && mn < a.n_moves) prep_body(t, a, mn, mn & 1, nullptr, &a.s.nm[mn & 1]);
+            if (blockIdx.x == 0 && mn < a.n_moves) prep_body(t, a, mn, mn & 1, nullptr, &a.s.nm[mn & 1], false);
             return;
         }
     }
@@ -924,8 +934,14 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
         const int ep = r / NB;
         const int lam = ep == 0 ? ti.lam_t : ti.lam_o;
         const int leaf = ep == 0 ? ti.path_t[lam] : ti.path_o[lam];
-        const int off = a.ring ? ti.noff[ep] : t.near_off[leaf];
-        const int cnt = a.ring ? ti.ncnt[ep] : t.near_off[leaf + 1] - off;
+        // fused chain: the prepared entries (count first); this block's first
+        // 64 entries load together with the count
+        const int4* ent = a.s.near + (size_t)(2 * sl + ep) * (t.max_near + 1) + 1;
+        int4 pre = make_int4(0, 0, 0, 0);
+        if (a.ring && threadIdx.x < 64 && r % NB + NB * (int)threadIdx.x < t.max_near)
+            pre = ent[r % NB + NB * threadIdx.x];
+        const int off = a.ring ? 0 : t.near_off[leaf];
+        const int cnt = a.ring ? ent[-1].x : t.near_off[leaf + 1] - off;
         const double cx = ep == 0 ? ti.t[0] : ti.o[0], cy = ep == 0 ? ti.t[1] : ti.o[1],
                      cz = ep == 0 ? ti.t[2] : ti.o[2];
         const double bb = t.boxsize[lam];
@@ -945,14 +961,13 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
         __shared__ int es0[64], epre[65];
         __shared__ double esh[64][3];
         const int nk = cnt > r % NB ? (cnt - r % NB + NB - 1) / NB : 0;
-        const int4* ent = a.s.near + (size_t)(2 * sl + ep) * t.max_near;  // fused chain: prepared entries
         for (int j0 = 0; j0 < nk; j0 += 64) {
             const int ne = min(64, nk - j0);
             if (threadIdx.x < ne) {
                 const int k = r % NB + NB * (j0 + threadIdx.x);
                 int code;
                 if (a.ring) {
-                    const int4 e4 = ent[k];
+                    const int4 e4 = j0 == 0 ? pre : ent[k];
                     es0[threadIdx.x] = e4.x;
                     epre[threadIdx.x + 1] = e4.y;
                     code = e4.z;
@@ -1777,7 +1792,6 @@ constexpr int kApplyTile = 8;
 constexpr int kApplyLv = 4;     // levels of the per-block path table (deeper levels read TrialInfo)
 constexpr int kApplySplit = 4;  // items sharing the root
 constexpr int kApplyHeavy = 16; // contributions above which the root is split
-constexpr int kMaxGridD = 53;  // largest 2n+1 of any digits setting
 
 // lanes per grid row, and the x values one lane holds for grids up to d
 template <typename T>
@@ -2297,13 +2311,41 @@ __device__ void upd1_block0(const DevTree<T>& t, const McArgs& a) {
         return;
     }
     const int moved = moved_sh, mcnt = mcnt_sh;
+    if (has_next && threadIdx.x < 2)
+        a.s.near[(size_t)(2 * (sl ^ 1) + threadIdx.x) * (mx + 1)] = make_int4(tn.ncnt[threadIdx.x], 0, 0, 0);
 #pragma unroll
     for (int u = 0; u < NPT; ++u) {
         const int w = threadIdx.x + TB * u, ep = w / mx, k = w % mx;
         if (e_nb[u] >= 0)
-            a.s.near[(size_t)(2 * (sl ^ 1) + ep) * mx + k] =
+            a.s.near[(size_t)(2 * (sl ^ 1) + ep) * (mx + 1) + 1 + k] =
                 make_int4(e_st[u], e_nb[u] == moved ? mcnt : e_cnt[u], e_code[u], e_nb[u]);
     }
+}
+
+// Block 1 of the fused update: the self-pair term, threshold and phase rows
+// of move m + 1 (whose paths the trial kernel found), beside the array
+// items.  When this move relocates the next move's particle, block 0 sets
+// the next move up again (with rows) and this block does nothing.
+template <typename T>
+__device__ void upd1_rows(const DevTree<T>& t, const McArgs& a) {
+    __shared__ int skip;
+    const int sl = ring_slot(a);
+    const TrialInfo& ti = a.s.info[sl];
+    const int m = ti.m;
+    if (m + 1 >= a.n_moves) return;  // uniform
+    TrialInfo& tn = a.s.info[sl ^ 1];
+    if (threadIdx.x < 64) {
+        double v[kAccPerTrial];
+        upd1_partials(t, a, v);
+        if (threadIdx.x == 0) skip = upd1_decide(t, a, ti, m, v, a.s.plan[3], false) && tn.i == ti.i;
+    }
+    __syncthreads();
+    if (skip) return;
+    if (threadIdx.x == 0) {
+        tn.self = self_term(t, tn);
+        tn.thr = mh_threshold(tn.u, a.beta);
+    }
+    form_rows(t, tn, tabs_of<T>(a.s, table_stride(t.max_depth), sl ^ 1));
 }
 
 template <typename T>
@@ -2314,6 +2356,10 @@ __device__ void upd1_body(const DevTree<T>& t, const McArgs& a) {
         upd1_block0(t, a);
         return;
     }
+    if (blockIdx.x == 1) {
+        upd1_rows(t, a);
+        return;
+    }
     const int sl = ring_slot(a);
     const TrialInfo& ti = a.s.info[sl];
     const int m = ti.m;
@@ -2322,7 +2368,7 @@ __device__ void upd1_body(const DevTree<T>& t, const McArgs& a) {
     const int nD = a.s.plan[0], room = a.s.plan[3];
     constexpr int CH = kUpd1J * TB;  // elements per item
     const int nR = (t.Mw + CH - 1) / CH, nA = (t.Md + CH - 1) / CH;
-    const int it = blockIdx.x - 1;
+    const int it = blockIdx.x - 2;
     const bool have = it < nR + nD * nA;  // uniform
     if (!have) return;  // the grid is sized for the most arrays a move can mark
     const int D = t.max_depth, nL = n_levels_pw(D);
@@ -2506,12 +2552,12 @@ void launch_apply(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent
         hipExtLaunchKernelGGL(kernel, dim3(nb), dim3(TB), sh, st, e0, e1, 0, t, a);
     };
     if (a.ring) {
-        // block 0, then items: the root's chunks and every marked array's (at
-        // most 2 endpoints x levels x 28 arrays)
+        // block 0, the row block, then items: the root's chunks and every
+        // marked array's (at most 2 endpoints x levels x 28 arrays)
         constexpr int CH = kUpd1J * TB;
         const int nL = t.max_depth >= 4 ? t.max_depth - 3 : 0;
         const int items = (t.Mw + CH - 1) / CH + 2 * nL * 28 * ((t.Md + CH - 1) / CH);
-        hipExtLaunchKernelGGL(k_update1<T>, dim3(1 + items), dim3(TB), 0, st, e0, e1, 0, t, a);
+        hipExtLaunchKernelGGL(k_update1<T>, dim3(2 + items), dim3(TB), 0, st, e0, e1, 0, t, a);
         return;
     }
     if (apply_xmax<T>(dmax) <= kApplyXSmall<T>) {

@@ -285,7 +285,7 @@ Handle* create_handle(HPDMKParams params, int precision, int n_src, const double
         throw HpdmkError(HPDMK_EINVAL, "tree has more expansion boxes than the update planner's bitmap holds");
     h->sc.dest = dalloc<int>(2 * (size_t)h->n_pw + 2, o);
     h->sc.dinfo = dalloc<int>(2 * (size_t)h->n_pw + 2, o);
-    h->sc.near = dalloc<int4>(4 * (size_t)std::max(max_near, 1), o);
+    h->sc.near = dalloc<int4>(4 * ((size_t)max_near + 1), o);
     h->sc.nm = dalloc<NextMove>(2, o);
     h->sc.plan = dalloc<int>(4, o);
     h->probe = dalloc<unsigned long long>(2 * kProbes, o);
@@ -1170,7 +1170,7 @@ int hpdmk_set_tiles(hpdmk_tree t, const int* tiles, int batch) {
         if (!tiles) throw HpdmkError(HPDMK_EINVAL, "null tiles");
         for (int k = 0; k < 3; ++k)
             if (tiles[k] < 1 || tiles[k] > kMaxRoleBlocks)
-                throw HpdmkError(HPDMK_EINVAL, "blocks per trial role must be in [1, 16]");
+                throw HpdmkError(HPDMK_EINVAL, "blocks per trial role must be in [1, 32]");
         if (tiles[3] < 1 || tiles[3] > 65535) throw HpdmkError(HPDMK_EINVAL, "update blocks must be in [1, 65535]");
         if (batch < 1 || batch > kMaxBatch) throw HpdmkError(HPDMK_EINVAL, "batch must be in [1, 256]");
         h->tiles.win = tiles[0];

@@ -59,7 +59,7 @@ def test_autotune_keeps_state_and_chain():
     sidx, sd, su = moves(4000, 512, 9)
     tiles, ms = g.autotune(sidx, sd, su)
     log("tuned", tiles, ms)
-    assert ms > 0 and 1 <= tiles["batch"] <= 256 and 1 <= tiles["near"] <= 16
+    assert ms > 0 and 1 <= tiles["batch"] <= 256 and 1 <= tiles["near"] <= 32
     assert g.tiles() == tiles
     assert np.array_equal(g.positions(), x0) and abs(g.eval_energy() - E0) < 1e-12 * abs(E0)  # FP64 atomics: summation order
     idx, d, u = moves(4000, 500, 10)
@@ -73,6 +73,6 @@ def test_bad_tiles_rejected():
     with pytest.raises(P.HPDMKError):
         g.set_tiles(win=0)
     with pytest.raises(P.HPDMKError):
-        g.set_tiles(near=17)
+        g.set_tiles(near=33)
     with pytest.raises(P.HPDMKError):
         g.set_tiles(batch=300)
