@@ -35,9 +35,9 @@ struct TrialInfo {
     int pws[2][kMaxLevels];
     int noff[2], ncnt[2];
     int m;        // the move's index in the move list
+    int rs;       // tabs slot of its phase rows and McScratch::st slot of its self term and threshold:
+                  // m & 1, or 2 + (m & 1) when set up again after the previous move moved its particle
     double u;     // its Metropolis uniform
-    double self;  // its self-pair correction q^2 R_{b_2}(d_min) (src/energy_shift.cpp:58)
-    double thr;   // its Metropolis threshold -ln(u) / beta
 };
 
 // Fused single-move chain: a move two ahead, prefetched by the update kernel
@@ -120,6 +120,8 @@ struct McScratch {
     // count, image code, leaf} per near-list leaf
     int4* near;
     NextMove* nm;          // fused chain: [2], the move m + 2 prefetched by move m's update (slot m & 1)
+    double2* st;           // fused chain: [4], self-pair correction q^2 R_{b_2}(d_min)
+                           // (src/energy_shift.cpp:58) and threshold -ln(u) / beta per rows slot
     void* tabs;            // Cplx<T>, table_stride(D) per trial
     double* part;          // per trial and block: kPartStride values
     TrialRec* rec;         // per trial of a batch

@@ -73,13 +73,20 @@ __device__ __forceinline__ int trial_slot(const McArgs& a, int tr, int m) { retu
 __device__ __forceinline__ int ring_slot(const McArgs& a) { return a.move0 & 1; }
 
 // ------------------------------------------------------------------ setup
-// FP64-formed phase rows of a trial into tb, by the whole block
+// FP64-formed phase rows of a trial into tb, by the whole block.  Row r of
+// grid half-width n holds scale e^{-i (k - n) dk c} for k = 0..2n (the z row
+// stops at k_z = 0; the x row carries the charge).  Two phases: one FP64
+// sincos per row gives its base e^{-i dk c} (in LDS); then one thread per
+// (row, run of kRowRun powers) raises the base to the run's first power by
+// squaring and multiplies on (FP64 complex arithmetic throughout).
+constexpr int kRowRun = 4;
+constexpr int kMaxRows = 6 + 2 * (kMaxLevels - 3) * 9;
 template <typename T>
 __device__ void form_rows(const DevTree<T>& t, const TrialInfo& ti, Cplx<T>* tb) {
+    __shared__ double bre[kMaxRows], bim[kMaxRows], bsc[kMaxRows];
+    __shared__ int bn[kMaxRows];
     const int D = t.max_depth, nL = n_levels_pw(D);
     const int rows = 6 + 2 * nL * 9;
-    // one thread per row: e^{-i dk c} by one FP64 sincos, its powers by
-    // complex multiplication (FP64), then the row e^{-i (m - n) dk c}
     for (int r = threadIdx.x; r < rows; r += blockDim.x) {
         int ep, axis, n;
         double c, dk;
@@ -96,24 +103,48 @@ __device__ void form_rows(const DevTree<T>& t, const TrialInfo& ti, Cplx<T>* tb)
             const int li = (rr / 9) % nL;
             ep = rr / (9 * nL);
             const int lam = ep == 0 ? ti.lam_t : ti.lam_o;
-            if (2 + li > lam) continue;  // this endpoint has no box at that level
-            n = t.n_diff;
+            n = 2 + li > lam ? -1 : t.n_diff;  // -1: this endpoint has no box at that level
             dk = t.delta_k[2 + li];
             c = (ep == 0 ? ti.t[axis] : ti.o[axis]) + (s1 - 1) * t.L;
         }
-        Cplx<T>* row = tb + r * kRowStride;
-        const double scale = axis == 0 ? ti.q : 1.0;
         double sn, cs;
         sincos(dk * c, &sn, &cs);
-        const double br = cs, bi = -sn;  // e^{-i dk c}
-        double wr = scale, wi = 0.0;     // scale e^{-i m dk c}, m = 0, 1, ...
-        row[n] = {(T)wr, (T)0};
-        for (int m = 1; m <= n; ++m) {
+        bre[r] = cs;
+        bim[r] = -sn;  // e^{-i dk c}
+        bsc[r] = axis == 0 ? ti.q : 1.0;
+        bn[r] = n;
+    }
+    __syncthreads();
+    const int runs = (kMaxGridD / 2 + 1 + kRowRun - 1) / kRowRun;  // powers 0..n of the widest grid
+    for (int w = threadIdx.x; w < rows * runs; w += blockDim.x) {
+        const int r = w / runs, k0 = (w % runs) * kRowRun, n = bn[r];
+        if (k0 > n) continue;
+        const double br = bre[r], bi = bim[r];
+        // base^k0 by squaring
+        double pr = 1.0, pi = 0.0, qr = br, qi = bi;
+        for (int e = k0; e > 0; e >>= 1) {
+            if (e & 1) {
+                const double tr = pr * qr - pi * qi;
+                pi = pr * qi + pi * qr;
+                pr = tr;
+            }
+            const double sr = qr * qr - qi * qi;
+            qi = 2.0 * qr * qi;
+            qr = sr;
+        }
+        double wr = bsc[r] * pr, wi = bsc[r] * pi;
+        Cplx<T>* row = tb + r * kRowStride;
+        const bool zrow = r < 6 ? r % 3 == 2 : ((r - 6) / 3) % 3 == 2;
+        for (int k = k0; k < k0 + kRowRun && k <= n; ++k) {
+            if (k == 0) {
+                row[n] = {(T)wr, (T)0};
+            } else {
+                if (!zrow) row[n + k] = {(T)wr, (T)wi};  // the z row stops at k_z = 0
+                row[n - k] = {(T)wr, (T)(-wi)};
+            }
             const double tr = wr * br - wi * bi;
             wi = wr * bi + wi * br;
             wr = tr;
-            if (axis != 2) row[n + m] = {(T)wr, (T)wi};  // the z row stops at k_z = 0
-            row[n - m] = {(T)wr, (T)(-wi)};
         }
     }
 }
@@ -182,20 +213,21 @@ __device__ void prep_body(const DevTree<T>& t, const McArgs& a, int m, int sl, c
         ti.ncnt[ep] = t.near_off[leaf + 1] - o0;
     }
     __syncthreads();
+    // every thread knows the rows slot itself (thread 0's writes below are
+    // not waited for before the rows are formed)
+    const int rslot = prev ? 2 + (m & 1) : (m & 1);  // a redo keeps off the slot the row block may be writing
     if (threadIdx.x == 0) {
         ti.m = m;
         ti.u = nm && !prev ? nm->u : a.ml.u[m];
-        if (rows) {
-            ti.self = self_term(t, ti);
-            ti.thr = mh_threshold(ti.u, a.beta);
-        }
+        ti.rs = rslot;
+        if (rows) a.s.st[rslot] = make_double2(self_term(t, ti), mh_threshold(ti.u, a.beta));
         a.s.info[sl] = ti;
         if (a.ml.stats) {
             a.ml.stats[kStatsPerMove * m + 6] = 0;
             a.ml.stats[kStatsPerMove * m + 7] = 0;
         }
     }
-    if (rows) form_rows(t, ti, tabs_of<T>(a.s, table_stride(D), sl));
+    if (rows) form_rows(t, ti, tabs_of<T>(a.s, table_stride(D), rslot));
 }
 
 // Fused chain: the near-field entries of the move in ring slot sl (its
@@ -886,7 +918,7 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
     const int sl = a.ring ? ring_slot(a) : tr;
     const TrialInfo& ti = a.s.info[sl];
     const int D = t.max_depth, nL = n_levels_pw(D), per = trial_blocks(D, a.tiles);
-    const Cplx<T>* tb = tabs_of<T>(a.s, table_stride(D), sl);
+    const Cplx<T>* tb = tabs_of<T>(a.s, table_stride(D), a.ring ? ti.rs : sl);
     double part = 0.0;
     int nscan = 0, novl = 0;
     double fsum[kFeatSums] = {0, 0, 0, 0, 0, 0, 0};
@@ -2197,21 +2229,22 @@ template <typename T>
 __device__ __forceinline__ int upd1_decide(const DevTree<T>& t, const McArgs& a, const TrialInfo& ti, int m,
                                            const double* v, int room, bool writer) {
     const int ovl = (int)v[6];
-    double dE = v[0] + v[1] + v[2] - v[3] - ti.self;  // src/energy_shift.cpp:67
+    const double2 st = a.s.st[ti.rs];  // self-pair term, threshold
+    double dE = v[0] + v[1] + v[2] - v[3] - st.x;  // src/energy_shift.cpp:67
     if (writer && a.ml.parts) {
         double* p = a.ml.parts + 5 * m;
         p[0] = v[0];
         p[1] = v[1];
         p[2] = v[2];
         p[3] = v[3];
-        p[4] = ti.self;
+        p[4] = st.x;
     }
     if (writer && a.ml.stats) {
         int o2;
         (void)assemble_trial(t, a, m & 1, m, v, &o2);  // the per-move counts
     }
     dE = ovl > 0 ? (double)INFINITY : a.lB * dE;  // a hard-sphere overlap is an infinite energy
-    int ok = ovl == 0 && dE < ti.thr;
+    int ok = ovl == 0 && dE < st.y;
     if (ok && !room) {  // the target leaf is full: the chain stops here
         ok = 0;
         if (writer) atomicOr(t.err, (int)kErrLeafOverflow);
@@ -2324,28 +2357,18 @@ __device__ void upd1_block0(const DevTree<T>& t, const McArgs& a) {
 
 // Block 1 of the fused update: the self-pair term, threshold and phase rows
 // of move m + 1 (whose paths the trial kernel found), beside the array
-// items.  When this move relocates the next move's particle, block 0 sets
-// the next move up again (with rows) and this block does nothing.
+// items and without waiting for the decision.  When this move relocates the
+// next move's particle, block 0 sets the next move up again into rows slot
+// 2 + ((m + 1) & 1), and what this block wrote goes unused.
 template <typename T>
 __device__ void upd1_rows(const DevTree<T>& t, const McArgs& a) {
-    __shared__ int skip;
     const int sl = ring_slot(a);
-    const TrialInfo& ti = a.s.info[sl];
-    const int m = ti.m;
-    if (m + 1 >= a.n_moves) return;  // uniform
-    TrialInfo& tn = a.s.info[sl ^ 1];
-    if (threadIdx.x < 64) {
-        double v[kAccPerTrial];
-        upd1_partials(t, a, v);
-        if (threadIdx.x == 0) skip = upd1_decide(t, a, ti, m, v, a.s.plan[3], false) && tn.i == ti.i;
-    }
-    __syncthreads();
-    if (skip) return;
-    if (threadIdx.x == 0) {
-        tn.self = self_term(t, tn);
-        tn.thr = mh_threshold(tn.u, a.beta);
-    }
-    form_rows(t, tn, tabs_of<T>(a.s, table_stride(t.max_depth), sl ^ 1));
+    if (a.s.info[sl].m + 1 >= a.n_moves) return;  // uniform
+    const TrialInfo& tn = a.s.info[sl ^ 1];
+    const int rs = sl ^ 1;  // (m + 1) & 1
+    if (threadIdx.x == TB - 1)  // a thread that forms no row base
+        a.s.st[rs] = make_double2(self_term(t, tn), mh_threshold(tn.u, a.beta));
+    form_rows(t, tn, tabs_of<T>(a.s, table_stride(t.max_depth), rs));
 }
 
 template <typename T>
@@ -2372,7 +2395,7 @@ __device__ void upd1_body(const DevTree<T>& t, const McArgs& a) {
     const bool have = it < nR + nD * nA;  // uniform
     if (!have) return;  // the grid is sized for the most arrays a move can mark
     const int D = t.max_depth, nL = n_levels_pw(D);
-    const Cplx<T>* tb = tabs_of<T>(a.s, table_stride(D), sl);
+    const Cplx<T>* tb = tabs_of<T>(a.s, table_stride(D), ti.rs);
     bool is_root = true;
     int chunk = it, key = 0, l = 0, ccell[3] = {0, 0, 0};
     if (have && it >= nR) {

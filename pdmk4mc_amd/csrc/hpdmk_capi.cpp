@@ -287,6 +287,7 @@ Handle* create_handle(HPDMKParams params, int precision, int n_src, const double
     h->sc.dinfo = dalloc<int>(2 * (size_t)h->n_pw + 2, o);
     h->sc.near = dalloc<int4>(4 * ((size_t)max_near + 1), o);
     h->sc.nm = dalloc<NextMove>(2, o);
+    h->sc.st = dalloc<double2>(4, o);
     h->sc.plan = dalloc<int>(4, o);
     h->probe = dalloc<unsigned long long>(2 * kProbes, o);
     HIPX(hipMemset(h->probe, 0, sizeof(unsigned long long) * 2 * kProbes));
@@ -422,8 +423,9 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
                 h->since_compact = 0;
             }
         };
+        const Tiles ftiles = h->tiles_set ? h->tiles : kPlainTiles;
         if (mode == 0 && !(h->prop_rf > 0.0) && h->tree.max_depth <= kFusedMaxDepth &&
-            trial_blocks(h->tree.max_depth, h->tiles) <= 64 * kUpd1P && 2 * dv.max_near <= kUpd1Near) {
+            trial_blocks(h->tree.max_depth, ftiles) <= 64 * kUpd1P && 2 * dv.max_near <= kUpd1Near) {
             // fused single-move chain: per move a trial kernel (with the update
             // plan's marking row) and an update kernel whose blocks take the
             // decision themselves; the update kernel's block 0 moves the
@@ -434,6 +436,7 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
             McArgs a = mc_args(h, mb, 0, 1, kTrialMetropolis, beta);
             a.ring = 1;
             a.n_moves = n;
+            a.tiles = ftiles;
             timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
             // a slot compaction moves particles inside their leaves: the next
             // move's prepared near-field entries are formed again after it
@@ -794,6 +797,7 @@ int hpdmk_tree_rebuild(hpdmk_tree t, hpdmk_tree* out) {
         snapshot(h, r, q);
         std::unique_ptr<Handle> nh(create_handle(h->params, h->precision, h->n_src, r.data(), q.data(), nullptr));
         nh->tiles = h->tiles;
+        nh->tiles_set = h->tiles_set;
         nh->tuned_batch = h->tuned_batch;
         nh->bjerrum = h->bjerrum;
         nh->sigma = h->sigma;
@@ -1173,6 +1177,7 @@ int hpdmk_set_tiles(hpdmk_tree t, const int* tiles, int batch) {
                 throw HpdmkError(HPDMK_EINVAL, "blocks per trial role must be in [1, 32]");
         if (tiles[3] < 1 || tiles[3] > 65535) throw HpdmkError(HPDMK_EINVAL, "update blocks must be in [1, 65535]");
         if (batch < 1 || batch > kMaxBatch) throw HpdmkError(HPDMK_EINVAL, "batch must be in [1, 256]");
+        h->tiles_set = true;
         h->tiles.win = tiles[0];
         h->tiles.diff = tiles[1];
         h->tiles.near = tiles[2];
@@ -1261,6 +1266,7 @@ int hpdmk_autotune(hpdmk_tree t, int n_moves, const long* idx, const double* d, 
             consider(g, best_batch);
         }
         h->tiles = best;
+        h->tiles_set = true;
         h->tuned_batch = best_batch;
         if (out) {
             out[0] = best.win;

@@ -74,6 +74,9 @@ static_assert(kPlainGraphMoves % 2 == 0, "the fused chain's ring slot is the gra
 // the fused single-move chain packs box cells in 9 bits (levels <= 9, i.e.
 // trees of at most 11 levels); deeper trees run the four-kernel chain
 constexpr int kFusedMaxDepth = 11;
+// launch shape of the fused single-move chain unless the caller chose one:
+// more near-field blocks than a batch wants (one trial fills few CUs)
+constexpr Tiles kPlainTiles{4, 8, 32, 1024};
 
 enum KernelKind { kKTrial = 0, kKUpdate = 1, kKCorr = 2, kKScan = 3, kKSetup = 4, kKPropose = 5, kKinds = 6 };
 
@@ -113,6 +116,7 @@ struct Handle {
     // launch shape of the trial and update kernels, and the batch size used
     // when hpdmk_mc_run is given batch 0 (hpdmk_set_tiles, hpdmk_autotune)
     Tiles tiles{};
+    bool tiles_set = false;  // set by hpdmk_set_tiles / hpdmk_autotune (else the fused chain uses kPlainTiles)
     int tuned_batch = 64;
     // proposal of hpdmk_mc_run (hpdmk_mc_set_proposal)
     int prop_learned = 0;
