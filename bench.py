@@ -13,9 +13,13 @@ Headline: trial moves/s of the batched chain with the learned proposal
 exact Metropolis-Hastings ratio).  Beside it: the batched chain with uniform
 displacements U[-1,1]^3, the plain one-trial-at-a-time GPU chain (uniform),
 their acceptance and mean squared jump per move, the roofline of the
-dominant kernel, the accuracy against FP64 Ewald, 256 ragged replicas, and
-the CPU baseline (the oracle, i.e. the CPU restatement of the reference path,
-one thread, on a bounded sample of the same chain).
+dominant kernel (profiled on the headline chain), the accuracy against FP64
+Ewald, 256 ragged replicas, the N = 1e6 system, the energy autocorrelation
+(decorrelated samples/s), and the CPU baseline: the oracle (the CPU
+restatement of the reference path, built with the reference's flags) on this
+host, as independent chains on every core (cpu_baseline.value) and as one
+chain on one thread, plus a sanity anchor at the reference's own benchmark
+point, each on a bounded sample of moves.
 """
 import argparse
 import json
@@ -208,6 +212,12 @@ def main():
     ap.add_argument("--tune-moves", type=int, default=4000)
     ap.add_argument("--plain-moves", type=int, default=3000)
     ap.add_argument("--cpu-moves", type=int, default=12000)
+    ap.add_argument("--cpu-par-moves", type=int, default=1500,
+                    help="moves of each independent chain of the all-core CPU baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the all-core CPU baseline (0: the process's CPUs, capped by OMP_NUM_THREADS)")
+    ap.add_argument("--anchor-moves", type=int, default=3000,
+                    help="moves of the CPU sanity anchor at the reference's benchmark point")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--rf", type=float, default=2.5, help="feature radius of the learned proposal")
@@ -233,7 +243,8 @@ def main():
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--replica-moves", type=int, default=1000)
     ap.add_argument("--graph-steps", type=int, default=64)
-    ap.add_argument("--legs", default="batched,learned,plain,accuracy,cpu,copy,replicas,hardspheres,build,autocorr",
+    ap.add_argument("--legs", default="batched,learned,plain,accuracy,cpu,copy,replicas,hardspheres,build,autocorr,"
+                                      "autocorr_main,large",
                     help="comma list of batched, learned, plain, accuracy, cpu, copy, replicas, hardspheres, large (copy = a 1 GiB "
                          "streaming copy: measured HBM ceiling and byte-counter calibration; replicas = "
                          "256 ragged independent systems stepped together)")
@@ -303,6 +314,8 @@ def main():
     # the same timed protocol with displacements drawn from the MLP
     learned_info = None
     value_uniform = value
+    wall_headline, acc_headline = wall, acc_b
+    learned_w = None
     if "learned" in legs:
         cfgp = dict(max_mean=args.max_mean, sigma_min=args.sigma_min, sigma_max=args.sigma_max)
         rng_l = np.random.default_rng(args.seed + 21)
@@ -341,6 +354,8 @@ def main():
         dl_l, _, H_l = gl.mc_proposal_results()
         gl.close()
         value = moves / wall_l
+        wall_headline, acc_headline = wall_l, acc_l
+        learned_w = w
         esjd_l = float((acc_l * (dl_l ** 2).sum(1)).mean())
         learned_info = {
             "value": round(value, 2), "unit": "trial moves/s", "gpu_ms": round(gl_ms, 2), "acceptance": round(float(acc_l.mean()), 4),
@@ -387,10 +402,12 @@ def main():
                          "decorrelated_samples_per_s": round(Ma / wm / args.n / (2 * tau), 4)}
         print(f"# autocorr (main workload): {acm}", file=sys.stderr, flush=True)
 
-    # per-kernel times on a profiled segment: start/end timestamps of every
-    # dispatch (level 1: no device-side timers in the kernels)
+    # per-kernel times of the headline chain (the learned proposal when that
+    # leg ran) on a profiled segment: start/end timestamps of every dispatch
+    # (level 1: no device-side timers in the kernels)
     prof_moves = min(M, 4000)
     psl = slice(n_total - prof_moves, n_total)
+    prof_z = np.random.default_rng(args.seed + 23).normal(size=(prof_moves, 3))
 
     def profiled(level):
         gp = P.HPDMKPtTree(params, r, q, prec)
@@ -399,7 +416,12 @@ def main():
         if tiles:
             gp.set_tiles(**tiles)
         gp.set_profiling(level)
-        gp.mc_load(idx[psl], d[psl], u[psl])
+        if learned_w is not None:
+            gp.mc_set_proposal(True, weights=learned_w, rf=args.rf, max_mean=args.max_mean,
+                               sigma_min=args.sigma_min, sigma_max=args.sigma_max)
+            gp.mc_load(idx[psl], prof_z, u[psl])
+        else:
+            gp.mc_load(idx[psl], d[psl], u[psl])
         gp.mc_run(1.0, 1, batch)
         return gp
 
@@ -449,9 +471,20 @@ def main():
         dom = "update"
         pk = per_kernel["update"]
     traffic = measured_traffic(args.traffic_file, dom)
+    traffic_src = None
+    if traffic is not None:
+        try:
+            with open(args.traffic_file) as f:
+                rec = json.load(f)
+            traffic_src = {"file": os.path.relpath(args.traffic_file, ROOT), "measured": rec.get("date"),
+                           "workload": rec.get("workload"), "note": "replayed from the committed rocprofv3 --pmc "
+                           "summary, not measured in this run"}
+        except (OSError, ValueError):
+            pass
     roofline = {"bound": pk["bound"], "engine": pk["engine"], "kernel": dom, "kernel_symbol": symbol.get(dom, dom),
                 "achieved": pk["achieved"], "peak": pk["peak"], "unit": pk["unit"], "frac": pk["frac"],
-                "traffic": traffic, "work_per_launch": pk["work_per_launch"], "avg_launch_us": pk["avg_launch_us"],
+                "traffic": traffic, "traffic_source": traffic_src,
+                "chain": "batched + learned (headline)" if learned_w is not None else "batched uniform", "work_per_launch": pk["work_per_launch"], "avg_launch_us": pk["avg_launch_us"],
                 "kernel_ms": {k: [round(v[0], 3), v[1]] for k, v in kt.items()}, "per_kernel": per_kernel,
                 "update_bytes_per_launch_per_move_sum": int(update_bytes_per_move_sum / max(kt["update"][1], 1)),
                 "window_gram_mflop_per_launch": round(gram_mflop(k_batch, Mw), 2) if gram else None,
@@ -563,22 +596,89 @@ def main():
 
     cpu = None
     if "cpu" in legs:
+        # The CPU restatement of the reference path (the oracle, built with the
+        # reference's -O3 -ffast-math), on this host's cores:
+        #   mode 1: one chain on one thread (benchmark/dmk_energyshift_runtime.cpp:44-91);
+        #   mode 2: independent chains of the same workload on every core
+        #           (threads; each chain is a separate oracle tree);
+        #   anchor: the reference's own benchmark point (digits 3, FP32, leaf
+        #           200, density 200, N = 8e4, relocations U[0, L)^3, every
+        #           move applied) against data/dmk_energyshift_runtime.csv:22-31.
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as O
+        from concurrent.futures import ThreadPoolExecutor
 
         S = args.cpu_moves
         t = O.OracleTree(r, q, args.L, n_per_leaf=args.leaf, digits=args.digits, precision=64, fast=True)
         t.form_pw()
         csl = slice(n_total, n_total + S)
         dE_c, acc_c, t2 = t.mc_chain(idx[csl], d[csl], u[csl], 1.0)
-        cpu = {"value": round(S / t2.sum(), 2), "unit": "trial moves/s", "cores": 1, "kind": "port",
-               "sample": f"{S} moves of a chain from the initial state (same system and move distribution), "
-                         "FP64, oracle built -O3 -ffast-math, one thread",
-               "cpu_model": cpu_model(),
-               "eval_us": round(t2[0] / S * 1e6, 2), "update_us_per_accept": round(t2[1] / max(1, acc_c.sum()) * 1e6, 2)}
+        t.close()
+        single = {"value": round(S / t2.sum(), 2), "unit": "trial moves/s", "cores": 1,
+                  "sample": f"{S} moves of one chain from the initial state (the metric's system and move "
+                            "distribution), FP64",
+                  "eval_us": round(t2[0] / S * 1e6, 2),
+                  "update_us_per_accept": round(t2[1] / max(1, acc_c.sum()) * 1e6, 2)}
         if acc_plain_ref is not None:
             n_cmp = len(acc_plain_ref)
-            cpu["decisions_equal_to_plain_gpu"] = f"{int((acc_c[:n_cmp] == acc_plain_ref).sum())}/{n_cmp}"
+            single["decisions_equal_to_plain_gpu"] = f"{int((acc_c[:n_cmp] == acc_plain_ref).sum())}/{n_cmp}"
+        try:
+            avail = len(os.sched_getaffinity(0))
+        except AttributeError:
+            avail = os.cpu_count() or 1
+        omp = int(os.environ.get("OMP_NUM_THREADS") or 0)
+        P_thr = args.cpu_threads or (min(avail, omp) if omp > 0 else avail)
+        Sp = args.cpu_par_moves
+
+        def build_chain(k):
+            tk = O.OracleTree(r, q, args.L, n_per_leaf=args.leaf, digits=args.digits, precision=64, fast=True)
+            tk.form_pw()
+            return tk
+
+        def run_chain_k(k, tk):
+            rk = np.random.default_rng(args.seed + 1000 + k)
+            ik = rk.integers(0, args.n, Sp)
+            dk = rk.uniform(-1.0, 1.0, (Sp, 3))
+            uk = rk.uniform(0.0, 1.0, Sp)
+            return tk.mc_chain(ik, dk, uk, 1.0)
+
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(P_thr) as ex:
+            trees = list(ex.map(build_chain, range(P_thr)))
+        t_build_par = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(P_thr) as ex:
+            list(ex.map(lambda k: run_chain_k(k, trees[k]), range(P_thr)))
+        wall_par = time.perf_counter() - t0
+        for tk in trees:
+            tk.close()
+        del trees
+        cpu = {"value": round(P_thr * Sp / wall_par, 2), "unit": "trial moves/s", "cores": P_thr, "kind": "port",
+               "sample": f"{P_thr} independent chains (one per thread) of {Sp} moves each on the metric's system "
+                         "(N=1e5, L=100, digits 6, leaf 200, dx~U[-1,1]^3, beta=1), FP64, the oracle built with "
+                         "-O3 -ffast-math; trees built beforehand in parallel "
+                         f"({t_build_par:.1f} s, not timed)",
+               "host": {"cpu_model": cpu_model(), "nproc": os.cpu_count(), "affinity": avail,
+                        "OMP_NUM_THREADS": omp or None},
+               "single_thread": single}
+        # sanity anchor at the reference's own benchmark point
+        Na, rho = 80000, 200.0
+        La = (Na / rho) ** (1.0 / 3.0)
+        ra, qa = O.reference_test_system(Na, La)
+        ta = O.OracleTree(ra, qa, La, n_per_leaf=200, digits=3, precision=32, fast=True)
+        ta.form_pw()
+        rk = np.random.default_rng(args.seed + 77)
+        Sa = args.anchor_moves
+        ia = rk.integers(0, Na, Sa)
+        da = rk.uniform(0.0, La, (Sa, 3))
+        _, aa, ta2 = ta.mc_chain(ia, da, np.zeros(Sa), 1.0)  # u = 0: every move applied, like the reference loop
+        ta.close()
+        cpu["anchor"] = {"workload": f"N={Na}, L={La:.4f} (density 200), digits 3, FP32, leaf 200, "
+                                     f"relocations U[0,L)^3, {Sa} moves, every move applied",
+                         "eval_us": round(ta2[0] / Sa * 1e6, 2),
+                         "update_us": round(ta2[1] / max(1, aa.sum()) * 1e6, 2),
+                         "reference_us": {"eval": 41.4, "update": 22.5,
+                                          "source": "data/dmk_energyshift_runtime.csv:22-31 (1 thread, CPU unstated)"}}
         print(f"# cpu: {cpu}", file=sys.stderr, flush=True)
 
     rep_info = None
@@ -745,8 +845,9 @@ def main():
     large_info = None
     if "large" in legs:
         # N = 1e6, L = 215.4, charges 90% +-1 and 10% +-2 (neutral), digits 6:
-        # full build, then 1e5 batched shifts; dE against FP64 Ewald
-        # (s = 4, alpha = 0.1: r_c = 40, 57^3 modes) on a sample of moves
+        # full build, then 1e5 batched shifts; E and the dE of 1e4 sampled
+        # moves (1e4 particles) against FP64 Ewald (s = 5, alpha = 0.08:
+        # r_c = 62.5, 57^2 x 29 half-cube modes)
         NL, LL = 1000000, 215.443469
         rng_g = np.random.default_rng(args.seed + 41)
         rg = rng_g.uniform(0, LL, (NL, 3))
@@ -761,11 +862,12 @@ def main():
         Eg = gg.eval_energy()
         t_bg = time.perf_counter() - t0
         Mg = args.large_moves
-        gi = rng_g.integers(0, NL, Mg + 64)
-        gd = rng_g.uniform(-1, 1, (Mg + 64, 3))
-        gu = rng_g.uniform(0, 1, Mg + 64)
-        sdE = gg.eval_shift_energy_batch(gi[:64], gd[:64])
-        gg.mc_load(gi[64:], gd[64:], gu[64:])
+        Ns = 10000
+        gi = rng_g.integers(0, NL, Mg + Ns)
+        gd = rng_g.uniform(-1, 1, (Mg + Ns, 3))
+        gu = rng_g.uniform(0, 1, Mg + Ns)
+        sdE = gg.eval_shift_energy_batch(gi[:Ns], gd[:Ns])
+        gg.mc_load(gi[Ns:], gd[Ns:], gu[Ns:])
         t0 = time.perf_counter()
         gg.mc_run(1.0, 1, batch)
         wg = time.perf_counter() - t0
@@ -773,14 +875,17 @@ def main():
         depth = gg.max_depth
         gg.close()
         t0 = time.perf_counter()
-        ewg = P.Ewald(LL, 4.0, 0.1, 1.0, qg, rg)
-        dE_ew = ewg.shift_chain(gi[:64], gd[:64])
+        ewg = P.Ewald(LL, 5.0, 0.08, 1.0, qg, rg)
+        E_ewg = ewg.compute_energy()
+        dE_ew = ewg.shift_chain(gi[:Ns], gd[:Ns])
         ewg.close()
         large_info = {"workload": "N=1e6, L=215.4, 90% +-1 / 10% +-2 neutral, digits 6, leaf 200",
                       "levels": depth, "build_energy_s": round(t_bg, 3), "E": Eg,
                       "moves_per_s": round(Mg / wg, 1), "moves": Mg, "acceptance": round(float(ag.mean()), 4),
-                      "dE_vs_ewald_max_rel_to_E": float(np.abs(sdE - dE_ew).max() / abs(Eg)),
-                      "ewald": {"s": 4.0, "alpha": 0.1, "sample_moves": 64,
+                      "E_vs_ewald_rel": float(abs(Eg - E_ewg) / abs(E_ewg)),
+                      "dE_vs_ewald_max_rel_to_E": float(np.abs(sdE - dE_ew).max() / abs(E_ewg)),
+                      "dE_vs_ewald_mean_rel_to_E": float(np.abs(sdE - dE_ew).mean() / abs(E_ewg)),
+                      "ewald": {"s": 5.0, "alpha": 0.08, "sample_moves": Ns, "E": E_ewg,
                                 "seconds": round(time.perf_counter() - t0, 2)}}
         print(f"# large: {large_info}", file=sys.stderr, flush=True)
 
@@ -828,7 +933,7 @@ def main():
     out = {
         "metric": "dE trial moves/sec at 1e-6 rel err, N=1e5; batched+learned vs plain GPU",
         "value": round(value, 2), "unit": "trial moves/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 3), "higher_is_better": True,
+        "warmup": args.warmup, "ms_per_step": round(wall_headline / args.steps * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64acc" if prec == P.HPDMK_MIXED else "f64",
         "data": "synthetic",
         "config": {"workload": "N=1e5 periodic cube L=100, +-1 alternating charges, digits 6, leaf 200, "
@@ -843,7 +948,15 @@ def main():
         "batched_uniform": {"value": round(value_uniform, 2), "unit": "trial moves/s",
                             "acceptance": round(float(acc_b.mean()), 4)},
         "learned": learned_info,
-        "acceptance": round(float(acc_b.mean()), 4),
+        "acceptance": round(float(acc_headline.mean()), 4),
+        "moves_per_step": M,
+        # the metric's "1e-6 rel err" is the tolerance setting (digits 6); the
+        # measured errors against FP64 Ewald on this system are these
+        "error_vs_ewald": ({"tolerance": 10.0 ** -args.digits,
+                            "E_rel": acc_info.get("E_mixed_vs_ewald_rel"),
+                            "dE_max_rel_to_E": acc_info.get("dE_mixed_vs_ewald_max_rel_to_E"),
+                            "dE_mean_rel_to_E": acc_info.get("dE_mixed_vs_ewald_mean_rel_to_E"),
+                            "sample_moves": acc_info.get("sample")} if acc_info and "ewald" in acc_info else None),
         "roofline": dict(roofline, measured_copy_gbs=round(copy_gbs, 1) if copy_gbs else None),
         "cpu_baseline": cpu,
         "replicas": rep_info,
