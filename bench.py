@@ -855,6 +855,11 @@ def main():
         qg = np.where(kk % 2 == 0, 1.0, -1.0)
         qg[kk % 20 < 2] *= 2.0  # one +2 and one -2 in every 20: 10% of magnitude 2, neutral
         gparams = P.HPDMKParams(args.leaf, args.digits, LL, init=P.DIRECT)
+        P.plan_tree_device(gparams, rg)
+        _, _, ms_gdev = P.plan_tree_device(gparams, rg)
+        t0 = time.perf_counter()
+        P.plan_tree(gparams, rg)
+        ms_ghost = (time.perf_counter() - t0) * 1e3
         t0 = time.perf_counter()
         gg = P.HPDMKPtTree(gparams, rg, qg, prec)
         gg.form_outgoing_pw()
@@ -881,6 +886,7 @@ def main():
         ewg.close()
         large_info = {"workload": "N=1e6, L=215.4, 90% +-1 / 10% +-2 neutral, digits 6, leaf 200",
                       "levels": depth, "build_energy_s": round(t_bg, 3), "E": Eg,
+                      "octree_ms": {"device": round(ms_gdev, 3), "host": round(ms_ghost, 3)},
                       "moves_per_s": round(Mg / wg, 1), "moves": Mg, "acceptance": round(float(ag.mean()), 4),
                       "E_vs_ewald_rel": float(abs(Eg - E_ewg) / abs(E_ewg)),
                       "dE_vs_ewald_max_rel_to_E": float(np.abs(sdE - dE_ew).max() / abs(E_ewg)),
@@ -895,6 +901,13 @@ def main():
         # FP32 MFMA with FP64 sums in mixed mode, FP64 VALU in FP64 mode, or
         # the Chebyshev PROXY pass), incoming expansions, total energy
         build_info = {"workload": f"N={args.n}, L={args.L}, leaf {args.leaf}", "runs": []}
+        # octree topology: the device build (hpdmk_tree_create's) and the host
+        # build, including the copies of the coordinates in and the tables out
+        P.plan_tree_device(params, r)  # first call: code objects
+        _, _, ms_dev = P.plan_tree_device(params, r)
+        t0 = time.perf_counter()
+        P.plan_tree(params, r)
+        build_info["octree_ms"] = {"device": round(ms_dev, 3), "host": round((time.perf_counter() - t0) * 1e3, 3)}
         for init, prc, dg in ((P.DIRECT, P.HPDMK_MIXED, 6), (P.DIRECT, P.HPDMK_FP64, 6), (P.PROXY, P.HPDMK_MIXED, 6),
                               (P.PROXY, P.HPDMK_FP64, 6), (P.DIRECT, P.HPDMK_MIXED, 3), (P.PROXY, P.HPDMK_MIXED, 3)):
             gb = P.HPDMKPtTree(P.HPDMKParams(args.leaf, dg, args.L, init=init), r, q, prc)

@@ -188,6 +188,13 @@ int hpdmk_read_particle_info(const char* path, int capacity, double* r_src, doub
 
 int hpdmk_plan_tree(HPDMKParams params, int n_src, const double* r_src, int* n_nodes, int* nodes,
                     int* particle_leaf);
+/* The same octree built on the GPU (Morton keys, radix sort, refinement,
+ * periodic 2:1 balance, neighbour and near lists; the build hpdmk_tree_create
+ * uses), node for node equal to hpdmk_plan_tree's.  *ms (optional): host wall
+ * time of the build including the copies.  HPDMK_ENOTIMPL for trees of more
+ * than 20 levels (hpdmk_tree_create then builds them on the host). */
+int hpdmk_plan_tree_device(HPDMKParams params, int n_src, const double* r_src, int* n_nodes, int* nodes,
+                           int* particle_leaf, double* ms);
 /* PSWF constants of a digits setting: c, lambda, C0, diff0, psi0(0),
  * n_window, n_diff (out[7]). */
 int hpdmk_pswf_constants(int digits, double* out);

@@ -8,7 +8,7 @@ PKG = pathlib.Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libhpdmk_amd.so"
 SOURCES = ["dmk_mc.hip", "dmk_build.hip", "dmk_proxy.hip", "dmk_nudft_mfma.hip", "ewald.hip", "mlp_train.hip", "hpdmk_capi.cpp", "hpdmk_replicas.cpp",
-           "hpdmk_ewald.cpp", "hpdmk_mlp.cpp", "dmk_host.cpp", "hpdmk_noalloc.hip"]
+           "hpdmk_ewald.cpp", "hpdmk_mlp.cpp", "dmk_host.cpp", "hpdmk_noalloc.hip", "dmk_octree.hip"]
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-std=c++17", "-O3", "--offload-arch=gfx950", "-fPIC", "-shared",

@@ -13,6 +13,9 @@
 #include <string>
 #include <vector>
 
+#include <chrono>
+
+#include "dmk_octree.hpp"
 #include "hpdmk_handle.hpp"
 
 using namespace hpdmk_amd;
@@ -94,17 +97,20 @@ Handle* create_handle(HPDMKParams params, int precision, int n_src, const double
         throw HpdmkError(HPDMK_EINVAL, e.what());
     }
     HIPX(hipGetDevice(&h->device));
-    h->tree = build_octree(n_src, r_src, params.L, params.n_per_leaf);
-    h->n_src = n_src;
-    const Octree& tr = h->tree;
-    const int D = tr.max_depth;
-    if (D > kMaxLevels - 1) throw HpdmkError(HPDMK_ERUNTIME, "tree deeper than supported");
     if (stream) {
         h->stream = stream;
         h->own_stream = false;
     } else {
         HIPX(hipStreamCreate(&h->stream));
     }
+    // the octree topology on the GPU (the host build only for trees deeper
+    // than the device build's pre-order keys hold)
+    h->tree = build_octree_device(n_src, r_src, params.L, params.n_per_leaf, h->stream);
+    if (h->tree.n_nodes == 0) h->tree = build_octree(n_src, r_src, params.L, params.n_per_leaf);
+    h->n_src = n_src;
+    const Octree& tr = h->tree;
+    const int D = tr.max_depth;
+    if (D > kMaxLevels - 1) throw HpdmkError(HPDMK_ERUNTIME, "tree deeper than supported");
     auto& o = h->owned;
 
     // ---- slots: leaf B owns [start, start + cap)
@@ -1038,12 +1044,9 @@ int hpdmk_read_particle_info(const char* path, int capacity, double* r, double* 
     });
 }
 
-int hpdmk_plan_tree(HPDMKParams params, int n_src, const double* r_src, int* n_nodes, int* nodes,
-                    int* particle_leaf) {
-    return guard([&] {
-        if (!r_src || !n_nodes) throw HpdmkError(HPDMK_EINVAL, "null argument");
-        if (params.n_per_leaf < 1) throw HpdmkError(HPDMK_EINVAL, "n_per_leaf must be positive");
-        const Octree tr = build_octree(n_src, r_src, params.L, params.n_per_leaf);
+namespace {
+// the per-node summary of hpdmk_plan_tree / hpdmk_plan_tree_device
+void write_plan(const Octree& tr, int* n_nodes, int* nodes, int* particle_leaf) {
         *n_nodes = tr.n_nodes;
         if (nodes) {
             std::vector<int> sub(tr.n_nodes, 0);
@@ -1074,6 +1077,35 @@ int hpdmk_plan_tree(HPDMKParams params, int n_src, const double* r_src, int* n_n
                 if (tr.is_leaf[i])
                     for (int k = 0; k < tr.leaf_count[i]; ++k) particle_leaf[tr.order[pos++]] = i;
         }
+}
+}  // namespace
+
+int hpdmk_plan_tree(HPDMKParams params, int n_src, const double* r_src, int* n_nodes, int* nodes,
+                    int* particle_leaf) {
+    return guard([&] {
+        if (!r_src || !n_nodes) throw HpdmkError(HPDMK_EINVAL, "null argument");
+        if (params.n_per_leaf < 1) throw HpdmkError(HPDMK_EINVAL, "n_per_leaf must be positive");
+        write_plan(build_octree(n_src, r_src, params.L, params.n_per_leaf), n_nodes, nodes, particle_leaf);
+    });
+}
+
+int hpdmk_plan_tree_device(HPDMKParams params, int n_src, const double* r_src, int* n_nodes, int* nodes,
+                           int* particle_leaf, double* ms) {
+    return guard([&] {
+        if (!r_src || !n_nodes) throw HpdmkError(HPDMK_EINVAL, "null argument");
+        if (params.n_per_leaf < 1) throw HpdmkError(HPDMK_EINVAL, "n_per_leaf must be positive");
+        struct S {
+            hipStream_t s = nullptr;
+            ~S() {
+                if (s) (void)hipStreamDestroy(s);
+            }
+        } st;
+        HIPX(hipStreamCreate(&st.s));
+        const auto t0 = std::chrono::steady_clock::now();
+        const Octree tr = build_octree_device(n_src, r_src, params.L, params.n_per_leaf, st.s);
+        if (ms) *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (tr.n_nodes == 0) throw HpdmkError(HPDMK_ENOTIMPL, "tree deeper than the device octree build supports");
+        write_plan(tr, n_nodes, nodes, particle_leaf);
     });
 }
 

@@ -41,6 +41,7 @@ SIGNATURES = {
     "hpdmk_tree_rebuild": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "hpdmk_read_particle_info": (C.c_int, [C.c_char_p, C.c_int, _dp, _dp, _ip]),
     "hpdmk_tree_info": (C.c_int, [C.c_void_p, _ip]),
+    "hpdmk_plan_tree_device": (C.c_int, [HPDMKParams, C.c_int, _dp, _ip, _ip, _ip, _dp]),
     "hpdmk_form_outgoing_pw": (C.c_int, [C.c_void_p]),
     "hpdmk_form_incoming_pw": (C.c_int, [C.c_void_p]),
     "hpdmk_eval_energy": (C.c_int, [C.c_void_p, _dp, _dp]),
@@ -139,6 +140,21 @@ def plan_tree(params, r_src):
     leaf = np.zeros(n, dtype=np.int32)
     _check(L.hpdmk_plan_tree(params, n, pr, C.byref(nn), nodes.ctypes.data_as(_ip), leaf.ctypes.data_as(_ip)))
     return nodes, leaf
+
+
+def plan_tree_device(params, r_src):
+    """The same octree plan built on the GPU: (nodes[n,9], particle_leaf[n_src], ms)."""
+    L = load_library()
+    r, pr = _d(r_src)
+    n = r.size // 3
+    nn = C.c_int()
+    ms = C.c_double()
+    _check(L.hpdmk_plan_tree_device(params, n, pr, C.byref(nn), None, None, None))
+    nodes = np.zeros((nn.value, 9), dtype=np.int32)
+    leaf = np.zeros(n, dtype=np.int32)
+    _check(L.hpdmk_plan_tree_device(params, n, pr, C.byref(nn), nodes.ctypes.data_as(_ip), leaf.ctypes.data_as(_ip),
+                                    C.byref(ms)))
+    return nodes, leaf, ms.value
 
 
 def pswf_constants(digits):
