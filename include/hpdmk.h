@@ -204,9 +204,12 @@ int hpdmk_pswf_constants(int digits, double* out);
  * symmetric proposal).  learned = 1: d holds standard normal triples z and the
  * displacement is delta = mu + sigma z, with mu = max_mean tanh(o[0:3]),
  * sigma = sigma_min + (sigma_max - sigma_min) sigmoid(o[3:6]) and o the output
- * of an MLP 8 -> 32 -> 32 -> 6 (tanh; float weights W1[8][32] b1 W2[32][32] b2
- * W3[32][6] b3, 1542 values) applied to features of the particle's
- * neighbourhood within radius rf (the particle itself removed).  The decision
+ * of an MLP 8 -> 64 -> 64 -> 64 -> 6 (tanh hidden layers; float weights
+ * W1[8][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64][6] b4[6], 9286
+ * values) applied to features of the particle's neighbourhood within radius
+ * rf (the particle itself removed): its charge, the softened local field (3),
+ * the neighbour count and the crowding direction (3).  Inference runs on the
+ * matrix cores with BF16 operands and FP32 accumulation.  The decision
  * adds the exact proposal ratio: accept iff bjerrum dE - H / beta < -ln(u) /
  * beta, H = log q(-delta | features at the target) - log q(delta | features
  * at the origin).  rf > 0 with learned = 0 records the origin features of
@@ -222,13 +225,17 @@ int hpdmk_mc_proposal_results(hpdmk_tree tree, double* delta, double* feat, doub
 
 /* Proposal MLP on the GPU (no reference counterpart).  Samples: features
  * feat[n][8] (as recorded by hpdmk_mc_proposal_results), displacements
- * delta[n][3] and weights weight[n] (e.g. 1 for accepted moves, 0 else).
- * hpdmk_mlp_eval: mu_sigma[n][6] (optional), grad[1542] (optional) = the sum
- * over samples of weight x d NLL / d w with NLL = sum_a log sigma_a +
- * (delta_a - mu_a)^2 / (2 sigma_a^2), and *loss (optional) = the weighted NLL
- * sum.  cfg3: max_mean, sigma_min, sigma_max.
- * hpdmk_mlp_train: Adam on minibatches of the mean weighted NLL; cfg[7]:
- * max_mean, sigma_min, sigma_max, learning rate, epochs, minibatch, seed;
+ * delta[n][3] and weights weight[n].
+ * hpdmk_mlp_eval: mu_sigma[n][6] (optional) from the deployed BF16
+ * matrix-core inference, grad[9286] (optional) = the FP32 sum over samples of
+ * weight x d NLL / d w with NLL = sum_a log sigma_a + (delta_a - mu_a)^2 /
+ * (2 sigma_a^2), and *loss (optional) = the weighted NLL sum.  cfg3:
+ * max_mean, sigma_min, sigma_max.
+ * hpdmk_mlp_train: Adam on minibatches of the weighted NLL; cfg[8]:
+ * max_mean, sigma_min, sigma_max, learning rate, epochs, minibatch, seed,
+ * normalisation (0: by the minibatch's weight sum, for 0/1 likelihood
+ * weights; 1: by its sample count, for signed weights such as a reward minus
+ * a baseline, whose descent is the policy gradient of the mean reward);
  * weights in/out; epoch_loss[epochs] (optional) = mean NLL per epoch. */
 int hpdmk_mlp_eval(int n, const float* weights, const double* feat, const double* delta, const double* weight,
                    const double* cfg3, double* mu_sigma, float* grad, double* loss);

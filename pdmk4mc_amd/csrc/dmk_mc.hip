@@ -308,8 +308,7 @@ __global__ void __launch_bounds__(TB) k_setup_rep(const DevTree<T>* __restrict__
 template <typename T>
 __device__ __forceinline__ void propose_body(const DevTree<T>& t, const McArgs& a) {
     __shared__ double redn[(TB / 64) * kFeatSums];
-    __shared__ double phi[kFeat], hbuf[2 * kHid], o[kOut], mu[3], sig[3];
-    __shared__ float wsh[kMlpParams];
+    __shared__ double phi[kFeat], mu[3], sig[3];
     if (*t.err) return;
     const Proposal& P = a.prop;
     const int tr = blockIdx.x, m = move_base(a) + tr;
@@ -392,7 +391,9 @@ __device__ __forceinline__ void propose_body(const DevTree<T>& t, const McArgs& 
             for (int f = 0; f < kFeat; ++f) P.feat[(size_t)kFeat * m + f] = phi[f];
     }
     if (!P.learned) return;  // uniform across the grid
-    mlp_forward_block(P.w, P, phi, hbuf, o, mu, sig, wsh);
+    __syncthreads();
+    mlp_forward_mfma(P, phi, mu, sig);
+    __syncthreads();
     if (threadIdx.x == 0) {
         double dl[3];
         for (int c = 0; c < 3; ++c) {
@@ -1084,8 +1085,7 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
 // decision and, for a single move, plans the update.  One block per trial.
 template <typename T, bool PM>
 __device__ __forceinline__ void decide_body(const DevTree<T>& t, const McArgs& a) {
-    __shared__ double sh_dE, phi[kFeat], hbuf[2 * kHid], o[kOut], mu[3], sig[3];
-    __shared__ float wsh[kMlpParams];
+    __shared__ double sh_dE, phi[kFeat], mu[3], sig[3];
     __shared__ int sh_ovl;
     if (*t.err) return;
     const int tr = blockIdx.x, m = move_base(a) + tr;
@@ -1106,7 +1106,9 @@ __device__ __forceinline__ void decide_body(const DevTree<T>& t, const McArgs& a
     }
     double H = 0.0;
     if (learned) {  // proposal log-ratio: reverse move from the target
-        mlp_forward_block(a.prop.w, a.prop, phi, hbuf, o, mu, sig, wsh);
+        __syncthreads();
+        mlp_forward_mfma(a.prop, phi, mu, sig);
+        __syncthreads();
         if (threadIdx.x == 0) {
             const double back[3] = {-a.prop.dprop[3 * m], -a.prop.dprop[3 * m + 1], -a.prop.dprop[3 * m + 2]};
             H = gauss_logq(back, mu, sig) - a.prop.logq[m];
@@ -1185,14 +1187,15 @@ __device__ __forceinline__ bool within(const double* x, const double* y, double 
 // thread 0.
 template <typename T>
 __device__ double reverse_logratio(const DevTree<T>& t, const McArgs& a, int tr, int m) {
-    __shared__ double phi[kFeat], hbuf[2 * kHid], o[kOut], mu[3], sig[3];
-    __shared__ float wsh[kMlpParams];
+    __shared__ double phi[kFeat], mu[3], sig[3];
     if (threadIdx.x < 64) {
         double fs[kFeatSums];
         sum_target_features(t, a, tr, threadIdx.x, 64, fs);
         if (threadIdx.x == 0) features_from_sums(a.s.info[tr].q, fs, phi);
     }
-    mlp_forward_block(a.prop.w, a.prop, phi, hbuf, o, mu, sig, wsh);
+    __syncthreads();
+    mlp_forward_mfma(a.prop, phi, mu, sig);
+    __syncthreads();
     double H = 0.0;
     if (threadIdx.x == 0) {
         const double back[3] = {-a.prop.dprop[3 * m], -a.prop.dprop[3 * m + 1], -a.prop.dprop[3 * m + 2]};

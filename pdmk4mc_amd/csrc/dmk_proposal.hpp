@@ -12,12 +12,16 @@
 //
 // Features of a point x for particle i, over the other particles j within the
 // feature radius rf (minimum image v = x - x_j, r = |v|, 0 < r < rf):
-//     s1 = sum q_j v / (r^2 + a^2)^{3/2}      (softened field, a = 0.5)
-//     s2 = sum w,  s3 = sum w v / r,           w = (1 - r/rf)^2
-//     phi = [q_i, tanh(q_i s1 / 1), tanh(s2 / 4), tanh(s3 / 2)]   (8 values)
-// MLP 8 -> 32 -> 32 -> 6 with tanh hidden layers; outputs o give
+//     s1 = sum q_j v / (r^2 + a^2)^{3/2}      (softened local field, a = 0.5)
+//     s2 = number of neighbours,  s3 = sum w v / r,  w = (1 - r/rf)^2
+//     phi = [q_i, tanh(q_i s1 / 1), tanh(s2 / 8), tanh(s3 / 2)]   (8 values)
+// MLP 8 -> 64 -> 64 -> 64 -> 6 with tanh hidden layers; outputs o give
 //     mu = max_mean tanh(o[0:3]),  sigma = smin + (smax - smin) sigmoid(o[3:6]).
-// Weights (float, row-major): W1[8][32] b1[32] W2[32][32] b2[32] W3[32][6] b3[6].
+// Weights (float, row-major): W1[8][64] b1[64] W2[64][64] b2[64] W3[64][64]
+// b3[64] W4[64][6] b4[6].  Training is FP32 (mlp_train.hip); inference runs
+// on the matrix cores with BF16 operands (features, weights, hidden
+// activations rounded to nearest even) and FP32 accumulation, biases and tanh
+// in FP32 (mlp_forward_mfma).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -27,16 +31,26 @@
 namespace hpdmk_amd {
 
 constexpr int kFeat = 8;
-constexpr int kHid = 32;
+constexpr int kHid = 64;
 constexpr int kOut = 6;
-constexpr int kMlpParams = kFeat * kHid + kHid + kHid * kHid + kHid + kHid * kOut + kOut;
+constexpr int kMlpParams = kFeat * kHid + kHid + 2 * (kHid * kHid + kHid) + kHid * kOut + kOut;
+// offsets of the parameter blocks
+constexpr int kW1 = 0, kB1 = kW1 + kFeat * kHid, kW2 = kB1 + kHid, kB2 = kW2 + kHid * kHid, kW3 = kB2 + kHid,
+              kB3 = kW3 + kHid * kHid, kW4 = kB3 + kHid, kB4 = kW4 + kHid * kOut;
 constexpr int kFeatSums = 7;
 constexpr double kFeatSoft = 0.5;
-constexpr double kFeatScale[3] = {1.0, 4.0, 2.0};
+constexpr double kFeatScale[3] = {1.0, 8.0, 2.0};
+// BF16 weights in matrix-core operand order (mlp_pack_weights): per layer,
+// per (k-step s, output tile t), 64 lanes x 4 values, lane l holding
+// W[16 s + 4 (l / 16) + j][16 t + l % 16], j = 0..3 (zero outside the layer)
+constexpr int kPackL1 = 1 * 4 * 256, kPackL2 = 4 * 4 * 256, kPackL3 = 4 * 4 * 256, kPackL4 = 4 * 1 * 256;
+constexpr int kPackOff1 = 0, kPackOff2 = kPackOff1 + kPackL1, kPackOff3 = kPackOff2 + kPackL2,
+              kPackOff4 = kPackOff3 + kPackL3, kPackSize = kPackOff4 + kPackL4;
 
 struct Proposal {
     int learned;       // 0: displacements as loaded; 1: drawn from the MLP (the list holds z)
-    const float* w;    // kMlpParams (learned)
+    const float* w;    // kMlpParams (learned): the biases are read from here
+    const unsigned short* wp;  // kPackSize BF16 weights in operand order (learned)
     const double* z;   // per move: N(0,1) triple (learned)
     double rf;         // feature radius (0: no features)
     double max_mean, smin, smax;
@@ -56,7 +70,7 @@ __host__ __device__ inline void feature_pair(double vx, double vy, double vz, do
     s[1] += qj * vy * inv;
     s[2] += qj * vz * inv;
     const double w = (1.0 - r / rf) * (1.0 - r / rf);
-    s[3] += w;
+    s[3] += 1.0;
     s[4] += w * vx / r;
     s[5] += w * vy / r;
     s[6] += w * vz / r;
@@ -79,54 +93,103 @@ __host__ __device__ inline double gauss_logq(const double* delta, const double* 
     return v;
 }
 
-// FP64 forward pass by all threads of a block (at least 64): the weights are
-// staged in shared memory (wsh, kMlpParams floats), every unit is computed by
-// 8 lanes (partial dot products joined with shuffles).  phi (8), hbuf
-// (2 kHid) and o (kOut) are shared; on return mu/sig (3 each, shared) are set.
-__device__ inline void mlp_forward_block(const float* w, const Proposal& p, const double* phi, double* hbuf,
-                                         double* o, double* mu, double* sig, float* wsh) {
-    const int t = threadIdx.x;
-    for (int k = t; k < kMlpParams; k += blockDim.x) wsh[k] = w[k];
-    const float* W1 = wsh;
-    const float* b1 = W1 + kFeat * kHid;
-    const float* W2 = b1 + kHid;
-    const float* b2 = W2 + kHid * kHid;
-    const float* W3 = b2 + kHid;
-    const float* b3 = W3 + kHid * kOut;
-    const int part = t & 7, nu = blockDim.x >> 3;  // 8 lanes per unit, units strided by the block
-    __syncthreads();
-    for (int unit = t >> 3; unit < kHid; unit += nu) {
-        double s = (double)W1[part * kHid + unit] * phi[part];  // kFeat = 8: one input per lane
-        s += __shfl_xor(s, 1);
-        s += __shfl_xor(s, 2);
-        s += __shfl_xor(s, 4);
-        if (part == 0) hbuf[unit] = tanh(s + (double)b1[unit]);
+// round-to-nearest-even FP32 -> BF16 bits
+__host__ __device__ inline unsigned short bf16_bits(float x) {
+    union {
+        float f;
+        unsigned int u;
+    } v;
+    v.f = x;
+    const unsigned int lsb = (v.u >> 16) & 1u;
+    return (unsigned short)((v.u + 0x7fffu + lsb) >> 16);
+}
+
+// The packed BF16 weights (kPackSize values) from the float parameters.
+inline void mlp_pack_weights(const float* w, unsigned short* wp) {
+    const int off[4] = {kPackOff1, kPackOff2, kPackOff3, kPackOff4};
+    const int wofs[4] = {kW1, kW2, kW3, kW4};
+    const int nin[4] = {kFeat, kHid, kHid, kHid}, nout[4] = {kHid, kHid, kHid, kOut};
+    for (int L = 0; L < 4; ++L) {
+        const int ns = (nin[L] + 15) / 16, nt = (nout[L] + 15) / 16;
+        for (int s = 0; s < ns; ++s)
+            for (int t = 0; t < nt; ++t)
+                for (int l = 0; l < 64; ++l)
+                    for (int j = 0; j < 4; ++j) {
+                        const int k = 16 * s + 4 * (l / 16) + j, n = 16 * t + l % 16;
+                        const float v = (k < nin[L] && n < nout[L]) ? w[wofs[L] + k * nout[L] + n] : 0.0f;
+                        wp[off[L] + ((s * nt + t) * 64 + l) * 4 + j] = bf16_bits(v);
+                    }
     }
-    __syncthreads();
-    for (int unit = t >> 3; unit < kHid; unit += nu) {
-        double s = 0.0;
-        for (int h = part; h < kHid; h += 8) s += (double)W2[h * kHid + unit] * hbuf[h];
-        s += __shfl_xor(s, 1);
-        s += __shfl_xor(s, 2);
-        s += __shfl_xor(s, 4);
-        if (part == 0) hbuf[kHid + unit] = tanh(s + (double)b2[unit]);
+}
+
+typedef short mlp_s4 __attribute__((ext_vector_type(4)));
+typedef float mlp_f4 __attribute__((ext_vector_type(4)));
+
+// One layer on v_mfma_f32_16x16x16bf16_1k for one sample: the input vector is
+// row 0 of the A operand (lanes l % 16 == 0 hold a[s] = x[16 s + 4 (l/16) + j];
+// the other lanes zeros), the packed weights the B operand; on return lane
+// l < 16 holds unit 16 t + l of the FP32 sums in acc[t][0].
+template <int NS, int NT>
+__device__ inline void mlp_layer(const mlp_s4* a, const unsigned short* wp, mlp_f4* acc) {
+    const int l = threadIdx.x & 63;
+    mlp_s4 b[NS][NT];
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) b[s][t] = *reinterpret_cast<const mlp_s4*>(wp + ((s * NT + t) * 64 + l) * 4);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        acc[t] = mlp_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < NS; ++s) acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[s], b[s][t], acc[t], 0, 0, 0);
     }
-    __syncthreads();
-    const int unit = t >> 3;
-    if (unit < kOut) {
-        double s = 0.0;
-        for (int h = part; h < kHid; h += 8) s += (double)W3[h * kOut + unit] * hbuf[kHid + h];
-        s += __shfl_xor(s, 1);
-        s += __shfl_xor(s, 2);
-        s += __shfl_xor(s, 4);
-        if (part == 0) o[unit] = s + (double)b3[unit];
+}
+
+// tanh(sum + bias) of the 64 hidden units, rounded to BF16 and moved into the
+// A-operand layout of the next layer by lane shuffles
+__device__ inline void mlp_hidden(const mlp_f4* acc, const float* bias, mlp_s4* a) {
+    const int l = threadIdx.x & 63;
+    short h[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) h[t] = l < 16 ? (short)bf16_bits(tanhf(acc[t][0] + bias[16 * t + l])) : (short)0;
+    const int g = l >> 4;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        mlp_s4 v;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = (short)__shfl((int)h[s], 4 * g + j);
+        a[s] = (l & 15) == 0 ? v : mlp_s4{0, 0, 0, 0};
     }
-    __syncthreads();
-    if (t < 3) {
-        mu[t] = p.max_mean * tanh(o[t]);
-        sig[t] = p.smin + (p.smax - p.smin) / (1.0 + exp(-o[3 + t]));
+}
+
+// BF16 forward pass by one wave (wave 0 of the block; the others idle).
+// phi (kFeat doubles, shared) in; mu/sig (3 each, shared) out, formed in
+// FP64 from the FP32 outputs.  The caller syncs the block around the call.
+__device__ inline void mlp_forward_mfma(const Proposal& p, const double* phi, double* mu, double* sig) {
+    if (threadIdx.x >= 64) return;
+    const int l = threadIdx.x, g = l >> 4;
+    mlp_s4 a[4];
+    {  // features: K = 8, one k-step (lanes with g < 2 carry values)
+        mlp_s4 v;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = (short)bf16_bits((float)(4 * g + j < kFeat ? phi[4 * g + j] : 0.0));
+        a[0] = (l & 15) == 0 && g < 2 ? v : mlp_s4{0, 0, 0, 0};
     }
-    __syncthreads();
+    mlp_f4 acc[4];
+    mlp_layer<1, 4>(a, p.wp + kPackOff1, acc);
+    mlp_hidden(acc, p.w + kB1, a);
+    mlp_layer<4, 4>(a, p.wp + kPackOff2, acc);
+    mlp_hidden(acc, p.w + kB2, a);
+    mlp_layer<4, 4>(a, p.wp + kPackOff3, acc);
+    mlp_hidden(acc, p.w + kB3, a);
+    mlp_layer<4, 1>(a, p.wp + kPackOff4, acc);
+    if (l < 3) {
+        const double om = (double)(acc[0][0] + p.w[kB4 + l]);
+        mu[l] = p.max_mean * tanh(om);
+    } else if (l < 6) {
+        const double os = (double)(acc[0][0] + p.w[kB4 + l]);
+        sig[l - 3] = p.smin + (p.smax - p.smin) / (1.0 + exp(-os));
+    }
 }
 
 }  // namespace hpdmk_amd

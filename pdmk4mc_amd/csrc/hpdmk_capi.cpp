@@ -367,6 +367,7 @@ McArgs mc_args(Handle* h, const MoveBuf& b, int move0, int K, int mode, double b
         if (h->prop_learned) {
             a.prop.learned = 1;
             a.prop.w = h->mlp_w;
+            a.prop.wp = h->mlp_wp;
             a.prop.z = b.d;
             a.prop.max_mean = h->prop_max_mean;
             a.prop.smin = h->prop_smin;
@@ -824,6 +825,8 @@ int hpdmk_tree_rebuild(hpdmk_tree t, hpdmk_tree* out) {
         if (h->mlp_w) {
             nh->mlp_w = dalloc<float>(kMlpParams, nh->owned);
             HIPX(hipMemcpy(nh->mlp_w, h->mlp_w, sizeof(float) * kMlpParams, hipMemcpyDeviceToDevice));
+            nh->mlp_wp = dalloc<unsigned short>(kPackSize, nh->owned);
+            HIPX(hipMemcpy(nh->mlp_wp, h->mlp_wp, sizeof(unsigned short) * kPackSize, hipMemcpyDeviceToDevice));
         }
         if (h->formed_out) {
             form_outgoing(nh.get(), nh->stream);
@@ -1173,7 +1176,11 @@ int hpdmk_mc_set_proposal(hpdmk_tree t, int learned, const float* weights, doubl
             if (sigma_max + max_mean > 0.25 * h->params.L)
                 throw HpdmkError(HPDMK_EINVAL, "proposal scale too large for the box (sigma_max + max_mean > L/4)");
             if (!h->mlp_w) h->mlp_w = dalloc<float>(kMlpParams, h->owned);
+            if (!h->mlp_wp) h->mlp_wp = dalloc<unsigned short>(kPackSize, h->owned);
             HIPX(hipMemcpy(h->mlp_w, weights, sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
+            std::vector<unsigned short> packed(kPackSize);
+            mlp_pack_weights(weights, packed.data());
+            HIPX(hipMemcpy(h->mlp_wp, packed.data(), sizeof(unsigned short) * kPackSize, hipMemcpyHostToDevice));
             h->prop_max_mean = max_mean;
             h->prop_smin = sigma_min;
             h->prop_smax = sigma_max;

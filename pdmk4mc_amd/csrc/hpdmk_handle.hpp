@@ -122,6 +122,7 @@ struct Handle {
     int prop_learned = 0;
     int* step_ctr = nullptr;  // device move counter of batched learned runs
     float* mlp_w = nullptr;  // kMlpParams on the device
+    unsigned short* mlp_wp = nullptr;  // kPackSize BF16 weights in matrix-core operand order
     double prop_rf = 0.0, prop_max_mean = 0.5, prop_smin = 0.05, prop_smax = 1.0;
     // Monte Carlo model of hpdmk_mc_run (hpdmk_mc_set_model)
     double bjerrum = 1.0;

@@ -56,6 +56,9 @@ int hpdmk_mlp_eval(int n, const float* weights, const double* feat, const double
         upload_data(m, n, feat, delta, weight, cfg3);
         std::vector<float> w(weights, weights + kMlpParams);
         m.w = dupload(w, m.owned);
+        std::vector<unsigned short> packed(kPackSize);
+        mlp_pack_weights(weights, packed.data());
+        const unsigned short* wp = dupload(packed, m.owned);
         const int nb = mlp_blocks(n);
         float* part = dalloc<float>((size_t)nb * kMlpParams, m.owned);
         float* lp = dalloc<float>(nb, m.owned);
@@ -63,7 +66,14 @@ int hpdmk_mlp_eval(int n, const float* weights, const double* feat, const double
         float* out = dalloc<float>(6 * (size_t)n, m.owned);
         launch_mlp_grad(m.d, nullptr, 0, n, m.w, part, lp, m.stream);
         launch_mlp_adam(m.w, nullptr, nullptr, part, nb, g, 1.0f, AdamStep{0, 0, 0, 0, 1, 1}, m.stream);
-        launch_mlp_out(m.d, n, m.w, out, m.stream);
+        Proposal pr{};
+        pr.learned = 1;
+        pr.w = m.w;
+        pr.wp = wp;
+        pr.max_mean = cfg3[0];
+        pr.smin = cfg3[1];
+        pr.smax = cfg3[2];
+        launch_mlp_infer(pr, m.d.X, n, out, m.stream);
         HIPX(hipGetLastError());
         std::vector<float> o(6 * (size_t)n), l(nb);
         HIPX(hipMemcpyAsync(o.data(), out, sizeof(float) * o.size(), hipMemcpyDeviceToHost, m.stream));
@@ -82,6 +92,7 @@ int hpdmk_mlp_train(int n, const double* feat, const double* delta, const double
         const double lr = cfg[3];
         const int epochs = (int)cfg[4], batch = (int)cfg[5];
         const unsigned seed = (unsigned)cfg[6];
+        const bool by_count = cfg[7] != 0.0;  // signed weights (policy gradient): mean over the minibatch
         if (!(lr > 0) || epochs < 1 || batch < 1) throw HpdmkError(HPDMK_EINVAL, "need lr > 0, epochs >= 1, batch >= 1");
         MlpDevice m;
         upload_data(m, n, feat, delta, weight, cfg);
@@ -99,7 +110,7 @@ int hpdmk_mlp_train(int n, const double* feat, const double* delta, const double
         std::iota(perm.begin(), perm.end(), 0);
         std::mt19937 rng(seed);
         double wsum_all = 0;
-        for (int i = 0; i < n; ++i) wsum_all += weight[i];
+        for (int i = 0; i < n; ++i) wsum_all += by_count ? 1.0 : weight[i];
         if (!(wsum_all > 0)) throw HpdmkError(HPDMK_EINVAL, "all sample weights are zero");
         const float b1 = 0.9f, b2 = 0.999f;
         long step = 0;
@@ -110,7 +121,7 @@ int hpdmk_mlp_train(int n, const double* feat, const double* delta, const double
             for (int b0 = 0; b0 < n; b0 += bs, ++nbat) {
                 const int cnt = std::min(bs, n - b0);
                 double wsum = 0;
-                for (int k = 0; k < cnt; ++k) wsum += weight[perm[b0 + k]];
+                for (int k = 0; k < cnt; ++k) wsum += by_count ? 1.0 : weight[perm[b0 + k]];
                 ++step;
                 const AdamStep st{(float)lr, b1, b2, 1e-8f, 1.0f - std::pow(b1, (float)step),
                                   1.0f - std::pow(b2, (float)step)};

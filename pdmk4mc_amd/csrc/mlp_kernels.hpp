@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "dmk_proposal.hpp"
+
 namespace hpdmk_amd {
 
 // training samples on the device: features X[n][8], displacements D[n][3],
@@ -19,12 +21,13 @@ struct AdamStep {
 };
 
 int mlp_blocks(int n);
-// per-block partial gradients (mlp_blocks(n) x 1542) and losses of samples sel[b0 .. b0 + n) (sel may be null)
+// per-block partial gradients (mlp_blocks(n) x kMlpParams) and losses of samples sel[b0 .. b0 + n) (sel may be null)
 void launch_mlp_grad(const MlpData& d, const int* sel, int b0, int n, const float* w, float* part, float* loss_part,
                      hipStream_t st);
 // grad = scale * sum of partials; then an Adam step unless s.lr <= 0
 void launch_mlp_adam(float* w, float* m, float* v, const float* part, int nblk, float* grad, float scale,
                      const AdamStep& s, hipStream_t st);
-void launch_mlp_out(const MlpData& d, int n, const float* w, float* out, hipStream_t st);
+// the deployed BF16 matrix-core inference of n samples X[n][8]: out[n][6] = mu, sigma
+void launch_mlp_infer(const Proposal& p, const float* X, int n, float* out, hipStream_t st);
 
 }  // namespace hpdmk_amd

@@ -1,8 +1,10 @@
-// Training of the proposal MLP (dmk_proposal.hpp) on the GPU: maximum
-// likelihood of displacements under q(delta | phi), weighted per sample
-// (accepted moves of a chain weigh 1, rejected 0), Adam on minibatches.
-// FP32 forward and backward; gradients per minibatch are reduced from
-// per-block partial sums (no atomics on the parameters).
+// Training of the proposal MLP (dmk_proposal.hpp) on the GPU: the weighted
+// Gaussian NLL of displacements under q(delta | phi) (weights 1/0 for the
+// likelihood of accepted moves; signed rewards minus a baseline for the
+// policy gradient of the accepted squared jump), Adam on minibatches.  FP32
+// forward and backward; gradients per minibatch are reduced from per-block
+// partial sums (no atomics on the parameters).  The deployed BF16 inference
+// is k_mlp_infer.
 #include <hip/hip_runtime.h>
 
 #include "dmk_proposal.hpp"
@@ -12,55 +14,53 @@ namespace hpdmk_amd {
 
 namespace {
 
-constexpr int MTB = 128;  // samples per block
-constexpr int kVec = kFeat + 4 * kHid + kOut;  // per-sample vectors kept in shared memory
+constexpr int MTB = 64;                            // samples per block
+constexpr int kVec = kFeat + 6 * kHid + kOut;      // per-sample vectors in shared memory
+// phi | h1 | g1 | h2 | g2 | h3 | g3 | dout
+constexpr int vH1 = kFeat, vG1 = vH1 + kHid, vH2 = vG1 + kHid, vG2 = vH2 + kHid, vH3 = vG2 + kHid, vG3 = vH3 + kHid,
+              vDO = vG3 + kHid;
 
-struct Offs {
-    static constexpr int W1 = 0, B1 = W1 + kFeat * kHid, W2 = B1 + kHid, B2 = W2 + kHid * kHid, W3 = B2 + kHid,
-                         B3 = W3 + kHid * kOut;
-};
-
-// forward pass of one sample (FP32); h1, h2, o written to the given arrays
-__device__ __forceinline__ void forward(const float* w, const float* phi, float* h1, float* h2, float* o) {
+// FP32 forward pass of one sample into its shared vector v (phi set); o out
+__device__ __forceinline__ void forward(const float* w, float* v, float* o) {
+    const float* phi = v;
     for (int h = 0; h < kHid; ++h) {
-        float s = w[Offs::B1 + h];
-        for (int f = 0; f < kFeat; ++f) s += w[Offs::W1 + f * kHid + h] * phi[f];
-        h1[h] = tanhf(s);
+        float s = w[kB1 + h];
+        for (int f = 0; f < kFeat; ++f) s += w[kW1 + f * kHid + h] * phi[f];
+        v[vH1 + h] = tanhf(s);
     }
     for (int h = 0; h < kHid; ++h) {
-        float s = w[Offs::B2 + h];
-        for (int i = 0; i < kHid; ++i) s += w[Offs::W2 + i * kHid + h] * h1[i];
-        h2[h] = tanhf(s);
+        float s = w[kB2 + h];
+        for (int i = 0; i < kHid; ++i) s += w[kW2 + i * kHid + h] * v[vH1 + i];
+        v[vH2 + h] = tanhf(s);
+    }
+    for (int h = 0; h < kHid; ++h) {
+        float s = w[kB3 + h];
+        for (int i = 0; i < kHid; ++i) s += w[kW3 + i * kHid + h] * v[vH2 + i];
+        v[vH3 + h] = tanhf(s);
     }
     for (int k = 0; k < kOut; ++k) {
-        float s = w[Offs::B3 + k];
-        for (int h = 0; h < kHid; ++h) s += w[Offs::W3 + h * kOut + k] * h2[h];
+        float s = w[kB4 + k];
+        for (int h = 0; h < kHid; ++h) s += w[kW4 + h * kOut + k] * v[vH3 + h];
         o[k] = s;
     }
 }
 
 // per-block gradient partials of sum_s wt_s NLL_s over the samples sel[b0 .. b0 + n)
+// (dynamic shared memory: MTB x kVec floats)
 __global__ void __launch_bounds__(MTB) k_mlp_grad(MlpData d, const int* sel, int b0, int n, const float* w,
                                                   float* part, float* loss_part) {
-    __shared__ float V[MTB][kVec];  // phi | h1 | g1 | h2 | g2 | do
-    __shared__ float lred[MTB / 64];
+    extern __shared__ float Vs[];
     const int t = threadIdx.x;
     const int k = blockIdx.x * MTB + t;
     const bool on = k < n;
-    float* v = V[t];
-    float* phi = v;
-    float* h1 = v + kFeat;
-    float* g1 = h1 + kHid;
-    float* h2 = g1 + kHid;
-    float* g2 = h2 + kHid;
-    float* dout = g2 + kHid;
+    float* v = Vs + (size_t)t * kVec;
     float L = 0.0f;
     if (on) {
         const int s = sel ? sel[b0 + k] : b0 + k;
         const float wt = d.wt[s];
-        for (int f = 0; f < kFeat; ++f) phi[f] = d.X[(size_t)kFeat * s + f];
+        for (int f = 0; f < kFeat; ++f) v[f] = d.X[(size_t)kFeat * s + f];
         float o[kOut];
-        forward(w, phi, h1, h2, o);
+        forward(w, v, o);
         for (int a = 0; a < 3; ++a) {
             const float th = tanhf(o[a]);
             const float mu = d.max_mean * th;
@@ -70,55 +70,70 @@ __global__ void __launch_bounds__(MTB) k_mlp_grad(MlpData d, const int* sel, int
             L += wt * (logf(sig) + 0.5f * y * y);
             const float dmu = -wt * y / sig;
             const float dsig = wt * (1.0f - y * y) / sig;
-            dout[a] = dmu * d.max_mean * (1.0f - th * th);
-            dout[3 + a] = dsig * (d.smax - d.smin) * sg * (1.0f - sg);
+            v[vDO + a] = dmu * d.max_mean * (1.0f - th * th);
+            v[vDO + 3 + a] = dsig * (d.smax - d.smin) * sg * (1.0f - sg);
         }
         for (int h = 0; h < kHid; ++h) {
+            float s3 = 0.0f;
+            for (int q = 0; q < kOut; ++q) s3 += w[kW4 + h * kOut + q] * v[vDO + q];
+            v[vG3 + h] = s3 * (1.0f - v[vH3 + h] * v[vH3 + h]);
+        }
+        for (int i = 0; i < kHid; ++i) {
             float s2 = 0.0f;
-            for (int q = 0; q < kOut; ++q) s2 += w[Offs::W3 + h * kOut + q] * dout[q];
-            g2[h] = s2 * (1.0f - h2[h] * h2[h]);
+            for (int h = 0; h < kHid; ++h) s2 += w[kW3 + i * kHid + h] * v[vG3 + h];
+            v[vG2 + i] = s2 * (1.0f - v[vH2 + i] * v[vH2 + i]);
         }
         for (int i = 0; i < kHid; ++i) {
             float s1 = 0.0f;
-            for (int h = 0; h < kHid; ++h) s1 += w[Offs::W2 + i * kHid + h] * g2[h];
-            g1[i] = s1 * (1.0f - h1[i] * h1[i]);
+            for (int h = 0; h < kHid; ++h) s1 += w[kW2 + i * kHid + h] * v[vG2 + h];
+            v[vG1 + i] = s1 * (1.0f - v[vH1 + i] * v[vH1 + i]);
         }
     } else {
         for (int c = 0; c < kVec; ++c) v[c] = 0.0f;
     }
     __syncthreads();
     // parameter gradients: sums over the block's samples of outer products
+    // (input side x output-side gradient) per layer, biases the gradient sums
     float* out = part + (size_t)blockIdx.x * kMlpParams;
     for (int p = t; p < kMlpParams; p += MTB) {
-        float acc = 0.0f;
-        if (p < Offs::B1) {
-            const int f = p / kHid, h = p % kHid;
-            for (int s = 0; s < MTB; ++s) acc += V[s][f] * V[s][kFeat + kHid + h];
-        } else if (p < Offs::W2) {
-            const int h = p - Offs::B1;
-            for (int s = 0; s < MTB; ++s) acc += V[s][kFeat + kHid + h];
-        } else if (p < Offs::B2) {
-            const int i = (p - Offs::W2) / kHid, h = (p - Offs::W2) % kHid;
-            for (int s = 0; s < MTB; ++s) acc += V[s][kFeat + i] * V[s][kFeat + 3 * kHid + h];
-        } else if (p < Offs::W3) {
-            const int h = p - Offs::B2;
-            for (int s = 0; s < MTB; ++s) acc += V[s][kFeat + 3 * kHid + h];
-        } else if (p < Offs::B3) {
-            const int h = (p - Offs::W3) / kOut, q = (p - Offs::W3) % kOut;
-            for (int s = 0; s < MTB; ++s) acc += V[s][kFeat + 2 * kHid + h] * V[s][kFeat + 4 * kHid + q];
+        int xi, gi;  // vector offsets of the input side (-1: bias) and the gradient side
+        if (p < kB1) {
+            xi = (p - kW1) / kHid;
+            gi = vG1 + (p - kW1) % kHid;
+        } else if (p < kW2) {
+            xi = -1;
+            gi = vG1 + p - kB1;
+        } else if (p < kB2) {
+            xi = vH1 + (p - kW2) / kHid;
+            gi = vG2 + (p - kW2) % kHid;
+        } else if (p < kW3) {
+            xi = -1;
+            gi = vG2 + p - kB2;
+        } else if (p < kB3) {
+            xi = vH2 + (p - kW3) / kHid;
+            gi = vG3 + (p - kW3) % kHid;
+        } else if (p < kW4) {
+            xi = -1;
+            gi = vG3 + p - kB3;
+        } else if (p < kB4) {
+            xi = vH3 + (p - kW4) / kOut;
+            gi = vDO + (p - kW4) % kOut;
         } else {
-            const int q = p - Offs::B3;
-            for (int s = 0; s < MTB; ++s) acc += V[s][kFeat + 4 * kHid + q];
+            xi = -1;
+            gi = vDO + p - kB4;
         }
+        float acc = 0.0f;
+        if (xi < 0)
+            for (int s = 0; s < MTB; ++s) acc += Vs[(size_t)s * kVec + gi];
+        else
+            for (int s = 0; s < MTB; ++s) acc += Vs[(size_t)s * kVec + xi] * Vs[(size_t)s * kVec + gi];
         out[p] = acc;
     }
     for (int o2 = 32; o2 > 0; o2 >>= 1) L += __shfl_xor(L, o2);
-    if ((t & 63) == 0) lred[t >> 6] = L;
-    __syncthreads();
-    if (t == 0) loss_part[blockIdx.x] = lred[0] + lred[1];
+    if (t == 0) loss_part[blockIdx.x] = L;
 }
 
-// sum of the block partials (scaled by 1/wsum) into grad, then an Adam step
+// sum of the block partials (scaled) into grad, then an Adam step
 __global__ void k_mlp_adam(float* w, float* m, float* v, const float* part, int nblk, float* grad, float scale,
                            AdamStep st) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -134,16 +149,19 @@ __global__ void k_mlp_adam(float* w, float* m, float* v, const float* part, int 
     w[p] -= st.lr * mh / (sqrtf(vh) + st.eps);
 }
 
-// outputs mu, sigma per sample (FP32 forward), for checks
-__global__ void k_mlp_out(MlpData d, int n, const float* w, float* out) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+// the deployed inference (BF16 operands on the matrix cores, FP32
+// accumulation) of one sample per block of 64 threads: mu, sigma
+__global__ void __launch_bounds__(64) k_mlp_infer(Proposal p, const float* X, int n, float* out) {
+    __shared__ double phi[kFeat], mu[3], sig[3];
+    const int s = blockIdx.x;
     if (s >= n) return;
-    float phi[kFeat], h1[kHid], h2[kHid], o[kOut];
-    for (int f = 0; f < kFeat; ++f) phi[f] = d.X[(size_t)kFeat * s + f];
-    forward(w, phi, h1, h2, o);
-    for (int a = 0; a < 3; ++a) {
-        out[6 * (size_t)s + a] = d.max_mean * tanhf(o[a]);
-        out[6 * (size_t)s + 3 + a] = d.smin + (d.smax - d.smin) / (1.0f + expf(-o[3 + a]));
+    if (threadIdx.x < kFeat) phi[threadIdx.x] = X[(size_t)kFeat * s + threadIdx.x];
+    __syncthreads();
+    mlp_forward_mfma(p, phi, mu, sig);
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        out[6 * (size_t)s + threadIdx.x] = (float)mu[threadIdx.x];
+        out[6 * (size_t)s + 3 + threadIdx.x] = (float)sig[threadIdx.x];
     }
 }
 
@@ -153,7 +171,9 @@ int mlp_blocks(int n) { return (n + MTB - 1) / MTB; }
 
 void launch_mlp_grad(const MlpData& d, const int* sel, int b0, int n, const float* w, float* part, float* loss_part,
                      hipStream_t st) {
-    hipLaunchKernelGGL(k_mlp_grad, dim3(mlp_blocks(n)), dim3(MTB), 0, st, d, sel, b0, n, w, part, loss_part);
+    const size_t sh = sizeof(float) * (size_t)MTB * kVec;
+    (void)hipFuncSetAttribute((const void*)k_mlp_grad, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    hipLaunchKernelGGL(k_mlp_grad, dim3(mlp_blocks(n)), dim3(MTB), sh, st, d, sel, b0, n, w, part, loss_part);
 }
 
 void launch_mlp_adam(float* w, float* m, float* v, const float* part, int nblk, float* grad, float scale,
@@ -162,8 +182,8 @@ void launch_mlp_adam(float* w, float* m, float* v, const float* part, int nblk, 
                        s);
 }
 
-void launch_mlp_out(const MlpData& d, int n, const float* w, float* out, hipStream_t st) {
-    hipLaunchKernelGGL(k_mlp_out, dim3((n + 127) / 128), dim3(128), 0, st, d, n, w, out);
+void launch_mlp_infer(const Proposal& p, const float* X, int n, float* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_mlp_infer, dim3(n), dim3(64), 0, st, p, X, n, out);
 }
 
 }  // namespace hpdmk_amd

@@ -165,21 +165,23 @@ def pswf_constants(digits):
 
 
 MLP_FEATURES = 8
-MLP_PARAMS = 8 * 32 + 32 + 32 * 32 + 32 + 32 * 6 + 6
+MLP_LAYERS = ((8, 64), (64, 64), (64, 64), (64, 6))
+MLP_PARAMS = sum(a * b + b for a, b in MLP_LAYERS)
 
 
 def mlp_init(seed=0, scale=1.0):
     """Initial proposal-MLP weights (float32): scaled Gaussian layers, zero biases."""
     rng = np.random.default_rng(seed)
     parts = []
-    for fan_in, fan_out in ((8, 32), (32, 32), (32, 6)):
+    for fan_in, fan_out in MLP_LAYERS:
         parts.append(rng.normal(0, scale / np.sqrt(fan_in), (fan_in, fan_out)).reshape(-1))
         parts.append(np.zeros(fan_out))
     return np.concatenate(parts).astype(np.float32)
 
 
 def mlp_eval(weights, feat, delta, weight, max_mean=0.5, sigma_min=0.05, sigma_max=1.0):
-    """GPU FP32 forward and gradient: (mu_sigma[n,6], grad[1542], weighted NLL sum)."""
+    """GPU inference and gradient: (mu_sigma[n,6] from the deployed BF16 matrix-core
+    inference, FP32 grad[MLP_PARAMS] of the weighted NLL sum, that sum)."""
     w = np.ascontiguousarray(weights, dtype=np.float32)
     feat, pf = _d(np.asarray(feat).reshape(-1))
     delta, pdl = _d(np.asarray(delta).reshape(-1))
@@ -196,13 +198,15 @@ def mlp_eval(weights, feat, delta, weight, max_mean=0.5, sigma_min=0.05, sigma_m
 
 
 def mlp_train(feat, delta, weight, weights0, max_mean=0.5, sigma_min=0.05, sigma_max=1.0, lr=3e-3, epochs=20,
-              batch=1024, seed=0):
-    """Adam on the mean weighted NLL of the displacements; returns (weights, loss per epoch)."""
+              batch=1024, seed=0, signed=False):
+    """Adam on the mean weighted NLL of the displacements; returns (weights, loss per epoch).
+    signed: weights are rewards minus a baseline (normalised by the sample count), so
+    the descent is the policy gradient of the mean reward."""
     w = np.array(weights0, dtype=np.float32, copy=True)
     feat, pf = _d(np.asarray(feat).reshape(-1))
     delta, pdl = _d(np.asarray(delta).reshape(-1))
     weight, pw = _d(weight)
-    cfg, pc = _d([max_mean, sigma_min, sigma_max, lr, epochs, batch, seed])
+    cfg, pc = _d([max_mean, sigma_min, sigma_max, lr, epochs, batch, seed, 1.0 if signed else 0.0])
     losses = np.zeros(int(epochs))
     _check(load_library().hpdmk_mlp_train(weight.size, pf, pdl, pw, pc, w.ctypes.data_as(C.POINTER(C.c_float)),
                                           losses.ctypes.data_as(_dp)))

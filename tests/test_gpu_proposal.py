@@ -1,11 +1,18 @@
 """Learned proposal on the GPU against the oracle (oracle/mlp_oracle.py and the
 DMK oracle).
 
-Bars: recorded origin features equal the oracle's to 1e-12; the FP32 MLP
-forward pass and weighted-NLL gradient equal the FP64 oracle's to 1e-5
-(outputs) and 1e-4 relative (gradient); training lowers the NLL of data with
-a learnable mean; the FP64 learned chain takes the oracle chain's decisions,
-with displacements to 1e-12 and dE and the proposal log-ratio H to 1e-9.
+Bars: recorded origin features equal the oracle's to 1e-12; the deployed
+BF16 matrix-core inference equals the oracle's emulation of it (BF16
+operands, FP32 sums) to 1e-5 on at least 90% of samples and to 0.02 on all
+(the sums' FP32 rounding order is the hardware's, so a sum can fall on the
+other side of a BF16 rounding boundary), and the FP64 forward pass to BF16
+precision (0.05); the FP32 weighted-NLL gradient equals the FP64 oracle's to
+1e-4 relative; training lowers the NLL of data with a learnable mean; the
+FP64 learned chain, replayed move by move by the oracle on the GPU's own
+trajectory: the oracle's displacement (from the same z) within 1e-5 on at
+least 90% of moves and 0.02 on all, H within 2e-3 (0.1 where a flip moved
+the proposal), dE at the GPU's displacement to 1e-9, and every decision the
+oracle's unless the decision variable lies within 1e-3 of its threshold.
 """
 import pathlib
 import sys
@@ -63,8 +70,13 @@ def test_mlp_forward_and_gradient_match_oracle():
     delta = rng.normal(0, 0.4, (1000, 3))
     wt = rng.integers(0, 2, 1000).astype(float)
     ms, g, loss = P.mlp_eval(w, phi, delta, wt, **CFG)
+    mub, sigb = M.forward_bf16(w, phi, CFG["max_mean"], CFG["sigma_min"], CFG["sigma_max"])
+    err = np.maximum(np.abs(ms[:, :3] - mub).max(1), np.abs(ms[:, 3:] - sigb).max(1))
+    log(f"BF16 inference vs emulation: median {np.median(err):.2e}, max {err.max():.2e}, "
+        f"beyond 1e-5: {(err > 1e-5).mean():.3f}")
+    assert (err <= 1e-5).mean() >= 0.9 and err.max() < 0.02
     mu, sig, _ = M.forward(w.astype(np.float64), phi, CFG["max_mean"], CFG["sigma_min"], CFG["sigma_max"])
-    assert np.abs(ms[:, :3] - mu).max() < 1e-5 and np.abs(ms[:, 3:] - sig).max() < 1e-5
+    assert np.abs(ms[:, :3] - mu).max() < 0.05 and np.abs(ms[:, 3:] - sig).max() < 0.05
     lo, go = M.nll_and_grad(w.astype(np.float64), phi, delta, wt, CFG["max_mean"], CFG["sigma_min"], CFG["sigma_max"])
     assert abs(loss - lo) < 1e-5 * abs(lo)
     assert np.abs(g - go).max() < 1e-4 * np.abs(go).max()
@@ -100,13 +112,22 @@ def test_learned_chain_matches_oracle():
     dlg, _, Hg = g.mc_proposal_results()
     t = O.OracleTree(r, q, L, n_per_leaf=leaf, digits=6)
     t.form_pw()
-    dEo, acco, dlo, Ho = M.learned_chain(t, r, q, L, idx, z, u, 1.0, w.astype(np.float64), rf, CFG["max_mean"],
-                                         CFG["sigma_min"], CFG["sigma_max"], bjerrum=2.0, sigma=1.0)
-    assert (accg == acco).all()
-    assert np.abs(dlg - dlo).max() < 1e-12
-    assert np.abs(Hg - Ho).max() < 1e-9
+    dlo, Ho, dEo = M.replay_learned_chain(t, r, q, L, idx, z, dlg, accg, 1.0, w, rf, CFG["max_mean"],
+                                          CFG["sigma_min"], CFG["sigma_max"], bjerrum=2.0, sigma=1.0)
+    ed = np.abs(dlg - dlo).max(1)
+    eh = np.abs(Hg - Ho)
+    log(f"replay: delta beyond 1e-5 {(ed > 1e-5).mean():.3f} max {ed.max():.2e}; H median {np.median(eh):.2e} "
+        f"max {eh.max():.2e}")
+    assert (ed <= 1e-5).mean() >= 0.9 and ed.max() < 0.02
+    flip = ed > 1e-5
+    assert eh[~flip].max(initial=0.0) < 2e-3 and eh.max() < 0.1
     fin = np.isfinite(dEo)
     assert (np.isfinite(dEg) == fin).all() and np.abs(dEg[fin] - dEo[fin]).max() < 1e-9
+    thr = -np.log(u)
+    x = np.where(fin, dEo, np.inf) - Ho
+    clear = np.abs(x - thr) > 1e-3 + eh
+    assert (accg[clear] == (x < thr)[clear]).all()
+    log("moves decided within the tie band:", int((~clear).sum()))
     assert 5 < accg.sum() < 145
 
 
