@@ -9,8 +9,10 @@ accept/update).  Moves and random numbers are in HBM before the timed region
 starts.
 
 Headline: trial moves/s of the batched chain with the learned proposal
-(displacements drawn from an MLP trained on the GPU from a uniform chain,
-exact Metropolis-Hastings ratio).  Beside it: the batched chain with uniform
+(displacements drawn from an MLP trained on the GPU: warm start on a
+uniform chain, then policy-gradient rounds on the expected squared jump;
+exact Metropolis-Hastings ratio), judged against the uniform proposal with
+its step tuned on the same system.  Beside it: the batched chain with uniform
 displacements U[-1,1]^3, the plain one-trial-at-a-time GPU chain (uniform),
 their acceptance and mean squared jump per move, the roofline of the
 dominant kernel (profiled on the headline chain), the accuracy against FP64
@@ -138,6 +140,34 @@ def tau_int(x, c=5.0):
     return float(tau)
 
 
+def energy_autocorrelation(g, n, sweeps, learned, step, w, cfg, seed, batch, chunk=20):
+    """Energy after every sweep (n moves) of a batched chain on g (uniform
+    step `step`, or the learned proposal with weights w), in chunks of
+    `chunk` sweeps; returns tau_int in sweeps (and of each half of the run),
+    moves/s and decorrelated samples/s = sweeps/s / (2 tau)."""
+    from pdmk4mc_amd import proposal as PR
+    rng = np.random.default_rng(seed)
+    es, secs, E, acc_sum = [], 0.0, 0.0, 0.0
+    for k in range(0, sweeps, chunk):
+        m = min(chunk, sweeps - k) * n
+        if learned:
+            dE, acc, _, _, _, _, sec = PR.run_learned(g, n, w, cfg, m, 1.0, batch, rng)
+        else:
+            dE, acc, _, _, sec = PR.run_uniform(g, n, step, m, 1.0, batch, rng)
+        secs += sec
+        acc_sum += acc.sum()
+        c = np.cumsum(np.where(acc > 0, dE, 0.0))
+        es.append(E + c[n - 1::n])
+        E += c[-1]
+    e = np.concatenate(es)
+    h = len(e) // 2
+    tau = tau_int(e)
+    mps = sweeps * n / secs
+    return {"moves_per_s": round(mps, 1), "acceptance": round(float(acc_sum) / (sweeps * n), 4), "sweeps": sweeps,
+            "tau_int_sweeps": round(tau, 3), "tau_int_halves": [round(tau_int(e[:h]), 3), round(tau_int(e[h:]), 3)],
+            "decorrelated_samples_per_s": round(mps / n / (2.0 * tau), 4)}
+
+
 def workload(n, L, n_moves, seed):
     rng = np.random.default_rng(seed)
     r = rng.uniform(0, L, (n, 3))
@@ -224,26 +254,30 @@ def main():
     ap.add_argument("--max-mean", type=float, default=0.5)
     ap.add_argument("--sigma-min", type=float, default=0.05)
     ap.add_argument("--sigma-max", type=float, default=1.5)
-    ap.add_argument("--train-moves", type=int, default=20000)
-    ap.add_argument("--train-step", type=float, default=2.0, help="uniform step of the training-data chain")
+    ap.add_argument("--train-moves", type=int, default=100000, help="uniform-chain moves of the warm start")
+    ap.add_argument("--train-step", type=float, default=2.0, help="uniform step of the warm-start chain")
     ap.add_argument("--train-epochs", type=int, default=20)
+    ap.add_argument("--pg-rounds", type=int, default=6, help="policy-gradient rounds after the warm start")
+    ap.add_argument("--pg-moves", type=int, default=50000, help="learned-chain moves per policy-gradient round")
+    ap.add_argument("--judge-steps", default="0.5,1,2,3,4,6", help="uniform steps the judge scans (N=1e5)")
     ap.add_argument("--hs-n", type=int, default=20000)
-    ap.add_argument("--hs-moves", type=int, default=20000)
-    ap.add_argument("--hs-step", type=float, default=0.5, help="uniform step of the hard-sphere chain")
-    ap.add_argument("--hs-train", type=int, default=100000, help="uniform-chain moves that train the MLP")
+    ap.add_argument("--hs-moves", type=int, default=200000)
+    ap.add_argument("--hs-train", type=int, default=100000, help="uniform-chain moves of the warm start")
     ap.add_argument("--hs-train-step", type=float, default=1.0)
+    ap.add_argument("--hs-pg-rounds", type=int, default=20)
     ap.add_argument("--hs-rf", type=float, default=2.0)
+    ap.add_argument("--hs-judge-steps", default="1,1.5,2,2.5,3",
+                    help="uniform steps the judge scans by energy autocorrelation (hard spheres)")
+    ap.add_argument("--hs-sweeps", type=int, default=300, help="sweeps of each hard-sphere autocorrelation chain")
     ap.add_argument("--large-moves", type=int, default=100000)
-    ap.add_argument("--ac-n", type=int, default=4000, help="particles of the autocorrelation system")
     ap.add_argument("--acm-sweeps", type=int, default=60, help="sweeps of the main-workload autocorrelation chains")
     ap.add_argument("--csv-dir", default=None,
                     help="also append rows in the reference's data/*.csv schemas (dmk_energyshift_runtime.csv, "
                          "dmk_energy_runtime.csv) to this directory")
-    ap.add_argument("--ac-sweeps", type=int, default=400, help="sweeps of each autocorrelation chain")
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--replica-moves", type=int, default=1000)
     ap.add_argument("--graph-steps", type=int, default=64)
-    ap.add_argument("--legs", default="batched,learned,plain,accuracy,cpu,copy,replicas,hardspheres,build,autocorr,"
+    ap.add_argument("--legs", default="batched,learned,plain,accuracy,cpu,copy,replicas,hardspheres,build,"
                                       "autocorr_main,large",
                     help="comma list of batched, learned, plain, accuracy, cpu, copy, replicas, hardspheres, large (copy = a 1 GiB "
                          "streaming copy: measured HBM ceiling and byte-counter calibration; replicas = "
@@ -260,6 +294,7 @@ def main():
         sys.exit(2)
 
     import pdmk4mc_amd as P
+    from pdmk4mc_amd import proposal as PR
 
     prec = P.HPDMK_MIXED if args.precision == "mixed" else P.HPDMK_FP64
     bc, br, bn = (8, 4, 16) if prec == P.HPDMK_MIXED else (16, 8, 32)
@@ -309,33 +344,29 @@ def main():
     print(f"# batched: {moves} moves in {wall:.3f}s (gpu {gpu_ms / 1e3:.3f}s) = {value:.0f} moves/s, "
           f"acceptance {acc_b.mean():.3f}", file=sys.stderr, flush=True)
 
-    # batched chain with the learned proposal: training data from a broad
-    # uniform chain (origin features recorded), MLP trained on the GPU, then
-    # the same timed protocol with displacements drawn from the MLP
+    # batched chain with the learned proposal: trained on the GPU on a twin of
+    # this system (warm start by maximum likelihood on a uniform chain, then
+    # policy-gradient rounds on the learned chain's expected squared jump),
+    # then the same timed protocol with displacements drawn from the MLP.
+    # The judge: the uniform proposal with its step tuned on the same twin.
     learned_info = None
     value_uniform = value
     wall_headline, acc_headline = wall, acc_b
     learned_w = None
+    judge_step = 1.0
     if "learned" in legs:
         cfgp = dict(max_mean=args.max_mean, sigma_min=args.sigma_min, sigma_max=args.sigma_max)
-        rng_l = np.random.default_rng(args.seed + 21)
-        Nc = args.train_moves
-        ci = rng_l.integers(0, args.n, Nc)
-        cd = rng_l.uniform(-args.train_step, args.train_step, (Nc, 3))
-        cu = rng_l.uniform(0.0, 1.0, Nc)
         t0 = time.time()
         gc = P.HPDMKPtTree(params, r, q, prec)
         gc.form_outgoing_pw()
         gc.form_incoming_pw()
-        gc.mc_set_proposal(False, rf=args.rf)
-        gc.mc_load(ci, cd, cu)
-        gc.mc_run(1.0, 1, args.batch)
-        _, cacc = gc.mc_results()
-        cdl, cft, _ = gc.mc_proposal_results()
-        gc.close()
-        w, losses = P.mlp_train(cft, cdl, cacc.astype(np.float64), P.mlp_init(args.seed), lr=3e-3,
-                                epochs=args.train_epochs, batch=1024, seed=args.seed, **cfgp)
+        w, hist = PR.train_proposal(gc, args.n, dict(rf=args.rf, **cfgp), warm_step=args.train_step,
+                                    warm_moves=args.train_moves, rounds=args.pg_rounds, round_moves=args.pg_moves,
+                                    warm_epochs=args.train_epochs, seed=args.seed)
         t_train = time.time() - t0
+        judge_step, judge = PR.tune_uniform(gc, args.n, [float(x) for x in args.judge_steps.split(",")], 50000,
+                                            seed=args.seed + 24)
+        gc.close()
         gl = P.HPDMKPtTree(params, r, q, prec)
         gl.form_outgoing_pw()
         gl.form_incoming_pw()
@@ -350,56 +381,49 @@ def main():
         t0 = time.perf_counter()
         gl_ms = gl.mc_run(1.0, 1, batch)
         wall_l = time.perf_counter() - t0
-        _, acc_l = gl.mc_results()
+        dE_l, acc_l = gl.mc_results()
         dl_l, _, H_l = gl.mc_proposal_results()
         gl.close()
         value = moves / wall_l
         wall_headline, acc_headline = wall_l, acc_l
         learned_w = w
-        esjd_l = float((acc_l * (dl_l ** 2).sum(1)).mean())
+        esjd_l = PR.esjd(dE_l, H_l, dl_l, 1.0)
+        jb = judge[judge_step]
         learned_info = {
             "value": round(value, 2), "unit": "trial moves/s", "gpu_ms": round(gl_ms, 2), "acceptance": round(float(acc_l.mean()), 4),
             "esjd_per_move": round(esjd_l, 5), "esjd_per_s": round(esjd_l * value, 1),
             "mean_abs_logratio": round(float(np.abs(H_l).mean()), 4),
             "proposal": dict(rf=args.rf, **cfgp),
-            "training": {"moves": Nc, "uniform_step": args.train_step, "acceptance": round(float(cacc.mean()), 4),
-                         "epochs": args.train_epochs, "nll_first": round(float(losses[0]), 4),
-                         "nll_last": round(float(losses[-1]), 4), "seconds": round(t_train, 2)}}
+            "training": {"warm_start": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in hist[0].items()},
+                         "warm_uniform_step": args.train_step,
+                         "policy_gradient": [{"round": h["round"], "esjd_per_move": round(h["esjd_per_move"], 4),
+                                              "acceptance": round(h["acceptance"], 4)} for h in hist[1:]],
+                         "seconds": round(t_train, 2)},
+            "judge": {"tuned_uniform_step": judge_step,
+                      "scan": {str(k): {kk: round(vv, 4) for kk, vv in v.items()} for k, v in judge.items()},
+                      "esjd_per_move_learned_over_tuned_uniform": round(esjd_l / jb["esjd_per_move"], 3),
+                      "esjd_per_s_learned_over_tuned_uniform": round(esjd_l * value / jb["esjd_per_s"], 3)}}
         print(f"# batched+learned: {moves} moves in {wall_l:.3f}s = {value:.0f} moves/s, acceptance "
-              f"{acc_l.mean():.3f}, ESJD/move {esjd_l:.4f}; training {learned_info['training']}",
+              f"{acc_l.mean():.3f}, ESJD/move {esjd_l:.4f}; judge: tuned uniform step {judge_step}, "
+              f"ESJD/move {jb['esjd_per_move']:.4f} at {jb['moves_per_s']:.0f} moves/s",
               file=sys.stderr, flush=True)
 
     # energy autocorrelation on this workload: chains of acm_sweeps sweeps with
-    # the uniform and the learned proposal, energy sampled once per sweep
-    # (the plain chain is the same Markov chain as the batched uniform one)
+    # the judge's tuned uniform step and with the learned proposal, energy
+    # sampled once per sweep (the plain chain at that step is the same Markov
+    # chain as the batched uniform one)
     acm = None
     if "autocorr_main" in legs and "learned" in legs:
-        acm = {"sweeps": args.acm_sweeps}
-        Ma = args.acm_sweeps * args.n
-        rng_m = np.random.default_rng(args.seed + 61)
-        for name, learned in (("uniform", False), ("learned", True)):
+        acm = {"sweeps": args.acm_sweeps, "uniform_step": judge_step}
+        for name in ("uniform", "learned"):
             gm_ = P.HPDMKPtTree(params, r, q, prec)
             gm_.form_outgoing_pw()
             gm_.form_incoming_pw()
             if tiles:
                 gm_.set_tiles(**tiles)
-            mi = rng_m.integers(0, args.n, Ma)
-            mu = rng_m.uniform(0, 1, Ma)
-            if learned:
-                gm_.mc_set_proposal(True, weights=w, rf=args.rf, **cfgp)
-                md = rng_m.normal(size=(Ma, 3))
-            else:
-                md = rng_m.uniform(-1.0, 1.0, (Ma, 3))
-            gm_.mc_load(mi, md, mu)
-            t0 = time.perf_counter()
-            gm_.mc_run(1.0, 1, batch)
-            wm = time.perf_counter() - t0
-            dEm, am = gm_.mc_results()
+            acm[name] = energy_autocorrelation(gm_, args.n, args.acm_sweeps, name == "learned", judge_step, learned_w,
+                                               dict(rf=args.rf, **cfgp), args.seed + 61, batch)
             gm_.close()
-            e_trace = np.cumsum(np.where(am > 0, dEm, 0.0))[args.n - 1::args.n]
-            tau = tau_int(e_trace)
-            acm[name] = {"moves_per_s": round(Ma / wm, 1), "tau_int_sweeps": round(tau, 3),
-                         "decorrelated_samples_per_s": round(Ma / wm / args.n / (2 * tau), 4)}
         print(f"# autocorr (main workload): {acm}", file=sys.stderr, flush=True)
 
     # per-kernel times of the headline chain (the learned proposal when that
@@ -532,10 +556,20 @@ def main():
         gpp.close()
         print(f"# plain: {plain:.0f} moves/s, acceptance {acc_pl.mean():.3f}, ESJD/move {esjd_pl:.4f}; "
               f"headline/plain = {value / plain:.2f}", file=sys.stderr, flush=True)
-        if acm:  # same Markov chain as the batched uniform one: same tau per sweep
+        if acm:  # the plain chain at the judge's step: the batched uniform chain's tau per sweep
+            gpj = P.HPDMKPtTree(params, r, q, prec)
+            gpj.form_outgoing_pw()
+            gpj.form_incoming_pw()
+            gpj.mc_load(idx[wsl], d[wsl] * judge_step, u[wsl])
+            gpj.mc_run(1.0, 0)
+            gpj.mc_load(idx[psl], d[psl] * judge_step, u[psl])
+            t0 = time.perf_counter()
+            gpj.mc_run(1.0, 0)
+            plain_j = args.plain_moves / (time.perf_counter() - t0)
+            gpj.close()
             tau_u = acm["uniform"]["tau_int_sweeps"]
-            acm["plain"] = {"moves_per_s": round(plain, 1), "tau_int_sweeps": tau_u,
-                            "decorrelated_samples_per_s": round(plain / args.n / (2 * tau_u), 4)}
+            acm["plain"] = {"moves_per_s": round(plain_j, 1), "uniform_step": judge_step, "tau_int_sweeps": tau_u,
+                            "decorrelated_samples_per_s": round(plain_j / args.n / (2 * tau_u), 4)}
             acm["learned_over_plain"] = round(acm["learned"]["decorrelated_samples_per_s"] /
                                               acm["plain"]["decorrelated_samples_per_s"], 3)
 
@@ -719,128 +753,63 @@ def main():
 
     hs_info = None
     if "hardspheres" in legs:
-        # charged hard spheres: N = 2e4, sigma = 1, packing 0.2, Bjerrum length 2;
-        # uniform proposal vs the MLP proposal trained on uniform-chain data
+        # the learned-proposal workload: N = 2e4 charged hard spheres, sigma 1,
+        # packing 0.2, Bjerrum length 2, digits 6.  One chain, equilibrated
+        # (20 sweeps), trains the MLP (warm start on hs_train uniform moves,
+        # then policy-gradient rounds); the judge is the uniform proposal at
+        # the step with the most decorrelated energy samples per second; the
+        # learned and judge chains then run on from the same state.
         hr, hq, hL = hard_spheres(args.hs_n, 0.2, 1.0, args.seed + 31)
         hparams = P.HPDMKParams(args.leaf, args.digits, hL, init=P.DIRECT)
+        gh = P.HPDMKPtTree(hparams, hr, hq, prec)
+        gh.form_outgoing_pw()
+        gh.form_incoming_pw()
+        gh.mc_set_model(2.0, 1.0)
+        hcfg = dict(rf=args.hs_rf, max_mean=0.5, sigma_min=0.02, sigma_max=2.0)
         rng_h = np.random.default_rng(args.seed + 32)
-
-        def hs_tree():
-            gh = P.HPDMKPtTree(hparams, hr, hq, prec)
-            gh.form_outgoing_pw()
-            gh.form_incoming_pw()
-            gh.mc_set_model(2.0, 1.0)
-            return gh
-
-        hcfg = dict(max_mean=0.3, sigma_min=0.02, sigma_max=1.0)
-        Mh = args.hs_moves
-        hi = rng_h.integers(0, args.hs_n, Mh)
-        hu = rng_h.uniform(0, 1, Mh)
+        PR.run_uniform(gh, args.hs_n, 0.5, 20 * args.hs_n, 1.0, batch, rng_h)
+        t0 = time.time()
+        w_h, hist_h = PR.train_proposal(gh, args.hs_n, hcfg, warm_step=args.hs_train_step, warm_moves=args.hs_train,
+                                        rounds=args.hs_pg_rounds, round_moves=100000, warm_epochs=args.train_epochs,
+                                        seed=args.seed + 33)
+        t_train_h = time.time() - t0
+        scan = {}
+        for s_ in [float(x) for x in args.hs_judge_steps.split(",")]:
+            scan[s_] = energy_autocorrelation(gh, args.hs_n, args.hs_sweeps, False, s_, None, hcfg,
+                                              args.seed + 34, batch)
+        best_s = max(scan, key=lambda k: scan[k]["decorrelated_samples_per_s"])
+        lac = energy_autocorrelation(gh, args.hs_n, args.hs_sweeps, True, 0.0, w_h, hcfg, args.seed + 35, batch)
         out = {}
-        gh = hs_tree()
-        gh.mc_set_proposal(False, rf=args.hs_rf)  # training data: uniform chain, origin features recorded
-        ti = rng_h.integers(0, args.hs_n, args.hs_train)
-        td = rng_h.uniform(-args.hs_train_step, args.hs_train_step, (args.hs_train, 3))
-        gh.mc_load(ti, td, rng_h.uniform(0, 1, args.hs_train))
-        gh.mc_run(1.0, 1, batch)
-        _, tacc = gh.mc_results()
-        tdl, tft, _ = gh.mc_proposal_results()
-        gh.close()
-        w_h, l_h = P.mlp_train(tft, tdl, tacc.astype(np.float64), P.mlp_init(args.seed), lr=3e-3,
-                               epochs=args.train_epochs, batch=1024, seed=args.seed, **hcfg)
-        for name, learned in (("uniform", False), ("learned", True)):
-            gh = hs_tree()
-            if learned:
-                gh.mc_set_proposal(True, weights=w_h, rf=args.hs_rf, **hcfg)
-                dd = rng_h.normal(size=(Mh, 3))
+        for name in ("uniform", "learned"):
+            if name == "learned":
+                dEh, ah, dlh, Hh, _, _, sec = PR.run_learned(gh, args.hs_n, w_h, hcfg, args.hs_moves, 1.0, batch,
+                                                             rng_h)
             else:
-                dd = rng_h.uniform(-args.hs_step, args.hs_step, (Mh, 3))
-            gh.mc_load(hi[:2000], dd[:2000], hu[:2000])
-            gh.mc_run(1.0, 1, batch)  # warm-up
-            gh.mc_load(hi, dd, hu)
-            t0 = time.perf_counter()
-            gh.mc_run(1.0, 1, batch)
-            wh = time.perf_counter() - t0
-            dEh, ah = gh.mc_results()
-            dlh = gh.mc_proposal_results()[0] if learned else dd
-            esjd = float((ah * (dlh ** 2).sum(1)).mean())
-            out[name] = {"moves_per_s": round(Mh / wh, 1), "acceptance": round(float(ah.mean()), 4),
+                dEh, ah, dlh, _, sec = PR.run_uniform(gh, args.hs_n, best_s, args.hs_moves, 1.0, batch, rng_h)
+                Hh = np.zeros(len(dEh))
+            e = PR.esjd(dEh, Hh, dlh, 1.0)
+            out[name] = {"moves_per_s": round(args.hs_moves / sec, 1), "acceptance": round(float(ah.mean()), 4),
                          "overlap_rejections": round(float(np.isinf(dEh).mean()), 4),
-                         "esjd_per_move": round(esjd, 5), "esjd_per_s": round(esjd * Mh / wh, 1)}
-            gh.close()
+                         "esjd_per_move": round(e, 5), "esjd_per_s": round(e * args.hs_moves / sec, 1)}
+        gh.close()
+        ub = scan[best_s]
         hs_info = {"workload": f"N={args.hs_n} charged hard spheres, sigma 1, packing 0.2 (L={hL:.2f}), "
                                f"Bjerrum length 2, digits {args.digits}, leaf {args.leaf}, batched chain",
-                   "uniform_step": args.hs_step, "feature_radius": args.hs_rf,
-                   "training": {"moves": args.hs_train, "uniform_step": args.hs_train_step,
-                                "acceptance": round(float(tacc.mean()), 4), "nll_first": round(float(l_h[0]), 4),
-                                "nll_last": round(float(l_h[-1]), 4)},
+                   "feature_radius": args.hs_rf, "proposal": hcfg,
+                   "training": {"warm_start": {k: (round(v, 4) if isinstance(v, float) else v)
+                                               for k, v in hist_h[0].items()},
+                                "warm_uniform_step": args.hs_train_step,
+                                "policy_gradient_esjd_per_move": [round(h["esjd_per_move"], 4) for h in hist_h[1:]],
+                                "seconds": round(t_train_h, 2)},
+                   "judge": {"uniform_step": best_s, "scan": {str(k): v for k, v in scan.items()}},
+                   "autocorr_learned": lac,
+                   "decorrelated_per_sweep_learned_over_tuned_uniform": round(ub["tau_int_sweeps"] / lac["tau_int_sweeps"], 3),
+                   "decorrelated_per_s_learned_over_tuned_uniform": round(lac["decorrelated_samples_per_s"] /
+                                                                          ub["decorrelated_samples_per_s"], 3),
                    **out,
-                   "esjd_per_s_learned_over_uniform": round(out["learned"]["esjd_per_s"] / out["uniform"]["esjd_per_s"], 3)}
+                   "esjd_per_s_learned_over_tuned_uniform": round(out["learned"]["esjd_per_s"] /
+                                                                  out["uniform"]["esjd_per_s"], 3)}
         print(f"# hard spheres: {hs_info}", file=sys.stderr, flush=True)
-
-    ac_info = None
-    if "autocorr" in legs:
-        # energy autocorrelation of the chain: charged hard spheres, N = 4000,
-        # sigma 1, packing 0.2, Bjerrum length 2, digits 6, leaf 32; uniform
-        # proposal vs the MLP proposal trained on this system's uniform chain.
-        # The energy is sampled once per sweep (N moves); decorrelated
-        # samples per second = sweeps per second / (2 tau_int).
-        an = args.ac_n
-        ar_, aq, aL = hard_spheres(an, 0.2, 1.0, args.seed + 51)
-        aparams = P.HPDMKParams(32, args.digits, aL, init=P.DIRECT)
-        rng_a = np.random.default_rng(args.seed + 52)
-        acfg = dict(max_mean=0.3, sigma_min=0.02, sigma_max=1.0)
-        arf = 2.0
-
-        def ac_tree():
-            ga = P.HPDMKPtTree(aparams, ar_, aq, prec)
-            ga.form_outgoing_pw()
-            ga.form_incoming_pw()
-            ga.mc_set_model(2.0, 1.0)
-            return ga
-
-        ga = ac_tree()
-        ga.mc_set_proposal(False, rf=arf)
-        nt = 50 * an
-        ti = rng_a.integers(0, an, nt)
-        ga.mc_load(ti, rng_a.uniform(-1.0, 1.0, (nt, 3)), rng_a.uniform(0, 1, nt))
-        ga.mc_run(1.0, 1, batch)
-        _, tacc = ga.mc_results()
-        tdl, tft, _ = ga.mc_proposal_results()
-        ga.close()
-        w_a, _ = P.mlp_train(tft, tdl, tacc.astype(np.float64), P.mlp_init(args.seed), lr=3e-3,
-                             epochs=args.train_epochs, batch=1024, seed=args.seed, **acfg)
-        Ma = args.ac_sweeps * an
-        ac_out = {}
-        for name, learned in (("uniform", False), ("learned", True)):
-            ga = ac_tree()
-            ai = rng_a.integers(0, an, Ma)
-            au = rng_a.uniform(0, 1, Ma)
-            if learned:
-                ga.mc_set_proposal(True, weights=w_a, rf=arf, **acfg)
-                ad = rng_a.normal(size=(Ma, 3))
-            else:
-                ad = rng_a.uniform(-args.hs_step, args.hs_step, (Ma, 3))
-            ga.mc_load(ai[:20 * an], ad[:20 * an], au[:20 * an])
-            ga.mc_run(1.0, 1, batch)  # equilibration: 20 sweeps
-            ga.mc_load(ai, ad, au)
-            t0 = time.perf_counter()
-            ga.mc_run(1.0, 1, batch)
-            wa = time.perf_counter() - t0
-            dEa, aa = ga.mc_results()
-            ga.close()
-            e_trace = np.cumsum(np.where(aa > 0, dEa, 0.0))[an - 1::an]  # energy after each sweep (offset dropped)
-            tau = tau_int(e_trace)
-            sweeps_s = Ma / wa / an
-            ac_out[name] = {"moves_per_s": round(Ma / wa, 1), "acceptance": round(float(aa.mean()), 4),
-                            "sweeps": args.ac_sweeps, "tau_int_sweeps": round(tau, 3),
-                            "decorrelated_samples_per_s": round(sweeps_s / (2.0 * tau), 2)}
-        ac_info = {"workload": f"N={an} charged hard spheres, sigma 1, packing 0.2 (L={aL:.2f}), Bjerrum length 2, "
-                               f"digits {args.digits}, leaf 32, batched chain; energy sampled every sweep",
-                   **ac_out,
-                   "decorrelated_learned_over_uniform": round(ac_out["learned"]["decorrelated_samples_per_s"] /
-                                                              ac_out["uniform"]["decorrelated_samples_per_s"], 3)}
-        print(f"# autocorr: {ac_info}", file=sys.stderr, flush=True)
 
     large_info = None
     if "large" in legs:
@@ -976,7 +945,6 @@ def main():
         "hard_spheres": hs_info,
         "large": large_info,
         "build": build_info,
-        "autocorr": ac_info,
         "autocorr_main": acm,
         "accuracy": acc_info,
     }

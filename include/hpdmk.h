@@ -222,6 +222,11 @@ int hpdmk_mc_set_proposal(hpdmk_tree tree, int learned, const float* weights, do
 /* Of the last hpdmk_mc_run (each optional): delta[3n] displacements used,
  * feat[8n] features at the origins (rf > 0), logratio[n] H (learned). */
 int hpdmk_mc_proposal_results(hpdmk_tree tree, double* delta, double* feat, double* logratio);
+/* Of the last hpdmk_mc_run with the learned proposal: feat_t[8n] the features
+ * at each move's target (the particle removed, the rest as the move saw
+ * them), i.e. the input of its reverse-move density (training data of the
+ * policy gradient through H).  No reference counterpart. */
+int hpdmk_mc_target_features(hpdmk_tree tree, double* feat_t);
 
 /* Proposal MLP on the GPU (no reference counterpart).  Samples: features
  * feat[n][8] (as recorded by hpdmk_mc_proposal_results), displacements

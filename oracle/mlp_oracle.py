@@ -157,11 +157,12 @@ def replay_learned_chain(tree, r, q, L, idx, z, delta_dev, acc_dev, beta, w, rf,
     oracle forms the features, the proposal (BF16 inference emulation) and
     the DMK energy change at the displacement the GPU used, from the state
     the GPU's earlier decisions left, and applies the GPU's decision.
-    Returns (delta, H, dE reported at the GPU's displacement) as the oracle
-    computes them; the caller checks the GPU's decisions against them."""
+    Returns (delta, H, dE reported at the GPU's displacement, features at the
+    target) as the oracle computes them; the caller checks the GPU's
+    decisions against them."""
     pos = np.array(r, dtype=np.float64, copy=True)
     n = len(idx)
-    dl, Hs, dE = np.zeros((n, 3)), np.zeros(n), np.zeros(n)
+    dl, Hs, dE, ftt = np.zeros((n, 3)), np.zeros(n), np.zeros(n), np.zeros((n, N_FEAT))
     for k in range(n):
         i = int(idx[k])
         o = pos[i].copy()
@@ -169,7 +170,8 @@ def replay_learned_chain(tree, r, q, L, idx, z, delta_dev, acc_dev, beta, w, rf,
         dl[k] = mu[0] + sg[0] * z[k]
         d = np.asarray(delta_dev[k], dtype=np.float64)
         t = np.mod(o + d, L)
-        mu2, sg2 = forward_bf16(w, features(pos, q, L, i, t, rf), max_mean, smin, smax)
+        ftt[k] = features(pos, q, L, i, t, rf)
+        mu2, sg2 = forward_bf16(w, ftt[k], max_mean, smin, smax)
         Hs[k] = gauss_logq(-d, mu2[0], sg2[0]) - gauss_logq(d, mu[0], sg[0])
         overlap = False
         if sigma > 0:
@@ -183,4 +185,4 @@ def replay_learned_chain(tree, r, q, L, idx, z, delta_dev, acc_dev, beta, w, rf,
         if acc_dev[k]:
             tree.update_shift(i, d)
             pos[i] = t
-    return dl, Hs, dE
+    return dl, Hs, dE, ftt

@@ -376,11 +376,10 @@ __device__ __forceinline__ void propose_body(const DevTree<T>& t, const McArgs& 
             int r = 0;
             while (rpre[r + 1] <= u) ++r;  // nr is small (the leaves around one point)
             const int s = rs0[r] + (u - rpre[r]);
-            if (s == own || t.porig[s] < 0) continue;
             const double ex = ox - (t.px[s] + rsh[r][0]), ey = oy - (t.py[s] + rsh[r][1]),
                          ez = oz - (t.pz[s] + rsh[r][2]);
             const double r2 = ex * ex + ey * ey + ez * ez;
-            if (r2 > 0.0 && r2 < rf2) feature_pair(ex, ey, ez, r2, t.pq[s], P.rf, fs);
+            if (r2 > 0.0 && r2 < rf2 && s != own && t.porig[s] >= 0) feature_pair(ex, ey, ez, r2, t.pq[s], P.rf, fs);
         }
         __syncthreads();  // the lists are consumed before the next pass
     }
@@ -1051,11 +1050,11 @@ __device__ __forceinline__ void trial_body(const DevTree<T>& t, const McArgs& a)
                     const T r2 = dx * dx + dy * dy + dz * dz;
                     if (r2 > eps && r2 < b2)
                         acc += (double)((T)pq2[h] * residual_poly_over_r<T>(sqrt(r2), inv_b, t.poly, t.n_poly));
-                    if (hs && s != own && t.porig[s] >= 0 && ex * ex + ey * ey + ez * ez < a.sigma2) ++novl;
-                    if (ft && s != own && t.porig[s] >= 0) {
-                        const double r2d = ex * ex + ey * ey + ez * ez;
-                        if (r2d > 0.0 && r2d < rf2) feature_pair(ex, ey, ez, r2d, pq2[h], a.prop.rf, fsum);
-                    }
+                    // distances first: the hole test (porig) loads only for close particles
+                    const double r2d = ex * ex + ey * ey + ez * ez;
+                    if (hs && r2d < a.sigma2 && s != own && t.porig[s] >= 0) ++novl;
+                    if (ft && r2d > 0.0 && r2d < rf2 && s != own && t.porig[s] >= 0)
+                        feature_pair(ex, ey, ez, r2d, pq2[h], a.prop.rf, fsum);
                 }
             }
             __syncthreads();  // the entry lists are consumed before the next pass
@@ -1101,7 +1100,11 @@ __device__ __forceinline__ void decide_body(const DevTree<T>& t, const McArgs& a
             int ovl = 0;
             sh_dE = assemble_trial(t, a, tr, m, v, &ovl);
             sh_ovl = ovl;
-            if (learned) features_from_sums(ti.q, fs, phi);
+            if (learned) {
+                features_from_sums(ti.q, fs, phi);
+                if (a.prop.feat_t)
+                    for (int f = 0; f < kFeat; ++f) a.prop.feat_t[(size_t)kFeat * m + f] = phi[f];
+            }
         }
     }
     double H = 0.0;
@@ -1191,7 +1194,11 @@ __device__ double reverse_logratio(const DevTree<T>& t, const McArgs& a, int tr,
     if (threadIdx.x < 64) {
         double fs[kFeatSums];
         sum_target_features(t, a, tr, threadIdx.x, 64, fs);
-        if (threadIdx.x == 0) features_from_sums(a.s.info[tr].q, fs, phi);
+        if (threadIdx.x == 0) {
+            features_from_sums(a.s.info[tr].q, fs, phi);
+            if (a.prop.feat_t)
+                for (int f = 0; f < kFeat; ++f) a.prop.feat_t[(size_t)kFeat * m + f] = phi[f];
+        }
     }
     __syncthreads();
     mlp_forward_mfma(a.prop, phi, mu, sig);

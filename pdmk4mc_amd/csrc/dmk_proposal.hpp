@@ -57,6 +57,7 @@ struct Proposal {
     double* dprop;     // per move: displacement used (learned)
     double* logq;      // per move: log q(delta | phi(o)) (learned)
     double* feat;      // per move: phi(o) (kFeat), when rf > 0
+    double* feat_t;    // per move: phi(t) (kFeat), learned
     double* H;         // per move: proposal log-ratio (learned)
 };
 
@@ -69,11 +70,12 @@ __host__ __device__ inline void feature_pair(double vx, double vy, double vz, do
     s[0] += qj * vx * inv;
     s[1] += qj * vy * inv;
     s[2] += qj * vz * inv;
-    const double w = (1.0 - r / rf) * (1.0 - r / rf);
+    const double ir = 1.0 / r, x = 1.0 - r / rf;
+    const double wr = x * x * ir;
     s[3] += 1.0;
-    s[4] += w * vx / r;
-    s[5] += w * vy / r;
-    s[6] += w * vz / r;
+    s[4] += wr * vx;
+    s[5] += wr * vy;
+    s[6] += wr * vz;
 }
 
 __host__ __device__ inline void features_from_sums(double q, const double* s, double* phi) {

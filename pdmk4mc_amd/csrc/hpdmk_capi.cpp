@@ -317,6 +317,7 @@ void ensure_moves(Handle* h, MoveBuf& b, int n) {
     b.dprop = dalloc<double>(3 * (size_t)c, o);
     b.logq = dalloc<double>(c, o);
     b.feat = dalloc<double>(kFeat * (size_t)c, o);
+    b.feat_t = dalloc<double>(kFeat * (size_t)c, o);
     b.H = dalloc<double>(c, o);
     b.cap = c;
 }
@@ -368,6 +369,7 @@ McArgs mc_args(Handle* h, const MoveBuf& b, int move0, int K, int mode, double b
             a.prop.learned = 1;
             a.prop.w = h->mlp_w;
             a.prop.wp = h->mlp_wp;
+            a.prop.feat_t = b.feat_t;
             a.prop.z = b.d;
             a.prop.max_mean = h->prop_max_mean;
             a.prop.smin = h->prop_smin;
@@ -1203,6 +1205,17 @@ int hpdmk_mc_proposal_results(hpdmk_tree t, double* delta, double* feat, double*
         }
         if (feat) HIPX(hipMemcpyAsync(feat, mb.feat, sizeof(double) * kFeat * n, hipMemcpyDeviceToHost, h->stream));
         if (logratio) HIPX(hipMemcpyAsync(logratio, mb.H, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+        HIPX(hipStreamSynchronize(h->stream));
+    });
+}
+
+int hpdmk_mc_target_features(hpdmk_tree t, double* feat_t) {
+    return guard([&] {
+        Handle* h = H(t);
+        const MoveBuf& mb = h->chain;
+        if (!feat_t) throw HpdmkError(HPDMK_EINVAL, "null output");
+        if (!h->prop_learned) throw HpdmkError(HPDMK_EINVAL, "target features are recorded by the learned proposal only");
+        HIPX(hipMemcpyAsync(feat_t, mb.feat_t, sizeof(double) * kFeat * mb.n, hipMemcpyDeviceToHost, h->stream));
         HIPX(hipStreamSynchronize(h->stream));
     });
 }

@@ -56,10 +56,12 @@ struct MoveBuf {
     double* parts = nullptr;
     int* acc = nullptr;
     int* stats = nullptr;
-    // proposal: displacement used, log q, features at the origin, log-ratio
+    // proposal: displacement used, log q, features at the origin and at the
+    // target (learned), log-ratio
     double* dprop = nullptr;
     double* logq = nullptr;
     double* feat = nullptr;
+    double* feat_t = nullptr;
     double* H = nullptr;
 };
 

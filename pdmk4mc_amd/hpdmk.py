@@ -69,6 +69,7 @@ SIGNATURES = {
     "hpdmk_mc_set_proposal": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_double, C.c_double,
                                         C.c_double, C.c_double]),
     "hpdmk_mc_proposal_results": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "hpdmk_mc_target_features": (C.c_int, [C.c_void_p, _dp]),
     "hpdmk_mlp_eval": (C.c_int, [C.c_int, C.POINTER(C.c_float), _dp, _dp, _dp, _dp, _dp, C.POINTER(C.c_float), _dp]),
     "hpdmk_mlp_train": (C.c_int, [C.c_int, _dp, _dp, _dp, _dp, C.POINTER(C.c_float), _dp]),
     "hpdmk_set_tiles": (C.c_int, [C.c_void_p, _ip, C.c_int]),
@@ -367,6 +368,13 @@ class HPDMKPtTree:
         _check(load_library().hpdmk_mc_proposal_results(self._h, dl.ctypes.data_as(_dp), ft.ctypes.data_as(_dp),
                                                         H.ctypes.data_as(_dp)))
         return dl, ft, H
+
+    def mc_target_features(self):
+        """Features at each move's target of the last learned-proposal run
+        (the input of the reverse-move density)."""
+        ft = np.zeros((self._mc_idx.size, MLP_FEATURES))
+        _check(load_library().hpdmk_mc_target_features(self._h, ft.ctypes.data_as(_dp)))
+        return ft
 
     def set_tiles(self, win=2, diff=8, near=8, apply=1024, batch=64):
         t = np.array([win, diff, near, apply], dtype=np.int32)

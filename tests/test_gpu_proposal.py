@@ -12,7 +12,10 @@ FP64 learned chain, replayed move by move by the oracle on the GPU's own
 trajectory: the oracle's displacement (from the same z) within 1e-5 on at
 least 90% of moves and 0.02 on all, H within 2e-3 (0.1 where a flip moved
 the proposal), dE at the GPU's displacement to 1e-9, and every decision the
-oracle's unless the decision variable lies within 1e-3 of its threshold.
+oracle's unless the decision variable lies within 1e-3 of its threshold,
+and the recorded target features the oracle's to 1e-12.  Policy-gradient
+training (pdmk4mc_amd/proposal.py) on a hard-sphere chain raises the
+expected squared jump of its warm start.
 """
 import pathlib
 import sys
@@ -23,6 +26,7 @@ import pytest
 import oracle_lib as O
 from chain_check import MIXED_DE_BOUND, assert_same_chain, thresholds
 import pdmk4mc_amd as P
+from pdmk4mc_amd import proposal as PR
 from hs_system import hard_sphere_system
 
 sys.path.insert(0, str(pathlib.Path(__file__).resolve().parents[1] / "oracle"))
@@ -110,10 +114,12 @@ def test_learned_chain_matches_oracle():
     idx, z, u = rng.integers(0, 2000, 150), rng.normal(size=(150, 3)), rng.uniform(0, 1, 150)
     dEg, accg, _ = g.mc_chain(idx, z, u, beta=1.0, mode=0)
     dlg, _, Hg = g.mc_proposal_results()
+    fttg = g.mc_target_features()
     t = O.OracleTree(r, q, L, n_per_leaf=leaf, digits=6)
     t.form_pw()
-    dlo, Ho, dEo = M.replay_learned_chain(t, r, q, L, idx, z, dlg, accg, 1.0, w, rf, CFG["max_mean"],
+    dlo, Ho, dEo, ftto = M.replay_learned_chain(t, r, q, L, idx, z, dlg, accg, 1.0, w, rf, CFG["max_mean"],
                                           CFG["sigma_min"], CFG["sigma_max"], bjerrum=2.0, sigma=1.0)
+    assert np.abs(fttg - ftto).max() < 1e-12  # features at the target: the reverse density's input
     ed = np.abs(dlg - dlo).max(1)
     eh = np.abs(Hg - Ho)
     log(f"replay: delta beyond 1e-5 {(ed > 1e-5).mean():.3f} max {ed.max():.2e}; H median {np.median(eh):.2e} "
@@ -165,3 +171,27 @@ def test_batched_learned_chain_equals_plain(precision, batch, spheres):
         k = assert_same_chain(dE1, a1, dE2, a2, thresholds(u), MIXED_DE_BOUND, score_ref=dE1 - H1)
         assert np.abs(d1[:k] - d2[:k]).max(initial=0.0) < 1e-5
         assert np.abs(H1[:k] - H2[:k]).max(initial=0.0) < MIXED_DE_BOUND
+
+
+def test_policy_gradient_training_raises_expected_jump():
+    log("policy gradient")
+    r, q, L = hard_sphere_system(3000, 0.2, 1.0, seed=13)
+    n = len(q)
+    g = P.HPDMKPtTree(P.HPDMKParams(32, 6, L, init=P.DIRECT), r, q, P.HPDMK_MIXED)
+    g.form_outgoing_pw()
+    g.form_incoming_pw()
+    g.mc_set_model(2.0, 1.0)
+    cfg = dict(rf=2.0, max_mean=0.5, sigma_min=0.02, sigma_max=2.0)
+    rng = np.random.default_rng(14)
+    PR.run_uniform(g, n, 0.5, 30 * n, 1.0, 64, rng)  # equilibration
+    w, hist = PR.train_proposal(g, n, cfg, warm_step=1.0, warm_moves=30000, rounds=8, round_moves=30000,
+                                lr=3e-4, pg_batch=8192, seed=15)
+    for h in hist:
+        log({k: (round(v, 4) if isinstance(v, float) else v) for k, v in h.items()})
+    w_warm, _ = PR.train_proposal(g, n, cfg, warm_step=1.0, warm_moves=30000, rounds=0, seed=15)
+    js = {}
+    for name, ww in (("warm", w_warm), ("trained", w)):
+        out = PR.run_learned(g, n, ww, cfg, 60000, 1.0, 64, rng)
+        js[name] = PR.esjd(out[0], out[3], out[2], 1.0)
+    log("expected squared jump per move", js)
+    assert js["trained"] > 1.05 * js["warm"]
