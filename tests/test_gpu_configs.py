@@ -19,15 +19,27 @@ the FP64 Ewald reporter.
   200): FP64 replica chains equal their own FP64 plain chains (four replicas:
   smallest, largest, two more), the smallest equals the oracle's chain, and
   every mixed replica chain equals its FP64 replica chain up to near ties.
+* N = 2e4 charged hard spheres (sigma 1, packing 0.2, Bjerrum length 2,
+  digits 6, leaf 200) with the learned proposal: the FP64 plain chain
+  replayed move by move by the oracle (DMK oracle energies, the oracle's
+  features and BF16 inference emulation) with the bars of
+  tests/test_gpu_proposal.py, and the FP64 batched chain equal to it
+  (decisions, displacements, H, dE, positions).
 """
 import sys
 
 import numpy as np
 import pytest
 
+import pathlib
+
 import oracle_lib as O
 import pdmk4mc_amd as P
 from chain_check import MIXED_DE_BOUND, assert_same_chain, thresholds
+from hs_system import hard_sphere_system
+
+sys.path.insert(0, str(pathlib.Path(__file__).resolve().parents[1] / "oracle"))
+import mlp_oracle as MO  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 CONTRACT_6 = 5e-5  # 5 * 10^(1 - digits), test/hpdmk_test.cpp:465
@@ -216,3 +228,50 @@ def test_256_ragged_replicas():
     for k in range(len(systems)):
         assert_same_chain(dE64[k], acc64[k], dEm[k], accm[k], thresholds(u[k]), MIXED_DE_BOUND)
     assert 0.5 < acc64.mean() < 0.95
+
+
+def test_n2e4_hard_spheres_learned_chain():
+    log("N=2e4 hard spheres: build")
+    r, q, L = hard_sphere_system(20000, 0.2, 1.0, seed=2031)
+    n = len(q)
+    params = P.HPDMKParams(200, 6, L, init=P.DIRECT)
+    cfg = dict(max_mean=0.5, sigma_min=0.02, sigma_max=2.0)
+    rf = 2.0
+    w = P.mlp_init(2032, scale=1.0)
+    rng = np.random.default_rng(2033)
+    m = 300
+    idx, z, u = rng.integers(0, n, m), rng.normal(size=(m, 3)), rng.uniform(0, 1, m)
+    runs = []
+    for mode in (0, 1):
+        g = built(params, r, q, P.HPDMK_FP64)
+        g.mc_set_model(2.0, 1.0)
+        g.mc_set_proposal(learned=True, weights=w, rf=rf, **cfg)
+        dE, acc, _ = g.mc_chain(idx, z, u, beta=1.0, mode=mode, batch=64)
+        dl, _, H = g.mc_proposal_results()
+        runs.append((dE, acc, dl, H, g.mc_target_features(), g.positions()))
+        g.close()
+    (dEg, accg, dlg, Hg, fttg, xg), (dEb, accb, dlb, Hb, _, xb) = runs
+    assert (accb == accg).all()
+    assert np.abs(dlb - dlg).max() < 1e-12 and np.abs(Hb - Hg).max() < 1e-9
+    fin = np.isfinite(dEg)
+    assert (np.isfinite(dEb) == fin).all() and np.abs(dEb[fin] - dEg[fin]).max() < 1e-9
+    assert np.abs(xb - xg).max() < 1e-12
+    log("N=2e4 hard spheres: oracle replay")
+    t = O.OracleTree(r, q, L, n_per_leaf=200, digits=6, precision=64, fast=True)
+    t.form_pw()
+    dlo, Ho, dEo, ftto = MO.replay_learned_chain(t, r, q, L, idx, z, dlg, accg, 1.0, w, rf, cfg["max_mean"],
+                                                 cfg["sigma_min"], cfg["sigma_max"], bjerrum=2.0, sigma=1.0)
+    assert np.abs(fttg - ftto).max() < 1e-12
+    ed = np.abs(dlg - dlo).max(1)
+    eh = np.abs(Hg - Ho)
+    log(f"replay: delta beyond 1e-5 {(ed > 1e-5).mean():.3f} max {ed.max():.2e}; H max {eh.max():.2e}")
+    assert (ed <= 1e-5).mean() >= 0.9 and ed.max() < 0.02
+    assert eh[ed <= 1e-5].max(initial=0.0) < 2e-3 and eh.max() < 0.1
+    fin = np.isfinite(dEo)
+    assert (np.isfinite(dEg) == fin).all() and np.abs(dEg[fin] - dEo[fin]).max() < 1e-9
+    thr = -np.log(u)
+    x = np.where(fin, dEo, np.inf) - Ho
+    clear = np.abs(x - thr) > 1e-3 + eh
+    assert (accg[clear] == (x < thr)[clear]).all()
+    log("accepted", int(accg.sum()), "overlaps", int((~fin).sum()), "tie band", int((~clear).sum()))
+    assert 5 < accg.sum() < m - 5 and (~fin).sum() > 5
