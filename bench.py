@@ -276,7 +276,8 @@ def main():
                          "dmk_energy_runtime.csv) to this directory")
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--replica-moves", type=int, default=1000)
-    ap.add_argument("--graph-steps", type=int, default=64)
+    ap.add_argument("--graph-steps", type=int, default=1000,
+                    help="replica steps per captured graph (>= --replica-moves: the whole run is one launch)")
     ap.add_argument("--legs", default="batched,learned,plain,accuracy,cpu,copy,replicas,hardspheres,build,"
                                       "autocorr_main,large",
                     help="comma list of batched, learned, plain, accuracy, cpu, copy, replicas, hardspheres, large (copy = a 1 GiB "
@@ -731,12 +732,12 @@ def main():
         rs.form_pw()
         t_rb = time.perf_counter() - t0
         Mr = args.replica_moves
-        ridx = np.stack([rng.integers(0, len(q), Mr + 100) for _, q in systems])
-        rd = rng.uniform(-1.0, 1.0, (len(systems), Mr + 100, 3))
-        ru = rng.uniform(0, 1, (len(systems), Mr + 100))
-        rs.mc_load(ridx[:, :100], rd[:, :100], ru[:, :100])
-        rs.mc_run(1.0, args.graph_steps)  # warm-up (graph instantiation, first touches)
-        rs.mc_load(ridx[:, 100:], rd[:, 100:], ru[:, 100:])
+        ridx = np.stack([rng.integers(0, len(q), 2 * Mr) for _, q in systems])
+        rd = rng.uniform(-1.0, 1.0, (len(systems), 2 * Mr, 3))
+        ru = rng.uniform(0, 1, (len(systems), 2 * Mr))
+        rs.mc_load(ridx[:, :Mr], rd[:, :Mr], ru[:, :Mr])
+        rs.mc_run(1.0, args.graph_steps)  # warm-up: graph capture and instantiation, first touches
+        rs.mc_load(ridx[:, Mr:], rd[:, Mr:], ru[:, Mr:])
         t0 = time.perf_counter()
         r_ms = rs.mc_run(1.0, args.graph_steps)
         r_wall = time.perf_counter() - t0
@@ -747,7 +748,9 @@ def main():
                     "workload": f"{len(systems)} replicas, N_r ~ U{{2000..20000}}, L_r = (N_r/0.1)^(1/3), +-1, "
                                 f"digits {args.digits}, leaf {args.leaf}, {Mr} moves each",
                     "n_total": int(sum(len(q) for _, q in systems)), "levels": [min(depths), max(depths)],
-                    "graph_steps": args.graph_steps, "gpu_ms": round(r_ms, 2), "wall_s": round(r_wall, 3),
+                    "graph_steps": args.graph_steps,
+                    "single_graph_launch": args.graph_steps >= Mr,
+                    "gpu_ms": round(r_ms, 2), "wall_s": round(r_wall, 3),
                     "build_s": round(t_rb, 2), "acceptance": round(float(racc.mean()), 4)}
         print(f"# replicas: {rep_info}", file=sys.stderr, flush=True)
 

@@ -286,7 +286,10 @@ int hpdmk_replicas_form_pw(hpdmk_replicas set);
  * index), d[3 (r * n_moves + k) + a], u[r * n_moves + k]. */
 int hpdmk_replicas_mc_load(hpdmk_replicas set, int n_moves, const long* idx, const double* d, const double* u);
 /* Runs every replica's chain; graph_steps > 1 captures that many steps into
- * one HIP graph and replays it. */
+ * one HIP graph and replays it; graph_steps >= n_moves captures the whole
+ * run (every step of every replica and the periodic slot compactions) into
+ * one graph that runs as a single launch (kept instantiated for later runs
+ * of as many moves). */
 int hpdmk_replicas_mc_run(hpdmk_replicas set, double beta, int graph_steps, double* gpu_ms);
 /* dE and decisions, replica-major like the move list */
 int hpdmk_replicas_mc_results(hpdmk_replicas set, double* dE, int* accept);

@@ -1150,10 +1150,17 @@ __global__ void __launch_bounds__(TB) k_decide(DevTree<T> t, McArgs a) {
     decide_body<T, true>(t, a);
 }
 
+// The replica's decision is its step's last reader of the move counter (the
+// update reads the plan), so thread 0 advances the counter when the block is
+// done: every other thread read it before the block's first barrier, or
+// returned early with it (no move left, or an error) where a later value
+// changes nothing.
 template <typename T>
 __global__ void __launch_bounds__(TB) k_decide_rep(const DevTree<T>* __restrict__ ts,
                                                    const McArgs* __restrict__ as) {
-    decide_body<T, false>(ts[blockIdx.z], as[blockIdx.z]);
+    const McArgs& a = as[blockIdx.z];
+    decide_body<T, false>(ts[blockIdx.z], a);
+    if (threadIdx.x == 0) *a.move_ctr += 1;
 }
 
 template <typename T, bool PM>  // PM: with the marking row (single-move steps)
@@ -1852,7 +1859,7 @@ __host__ __device__ inline size_t apply_shmem(int dmax, int K) {
            sizeof(int) * 2 * (size_t)K * kApplyLv;
 }
 
-template <typename T, int XMAX, bool RB = false, bool MM = true>
+template <typename T, int XMAX, bool RB = false>
 __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a) {
     extern __shared__ char apply_dyn[];
     __shared__ int mlist[2 * kMaxBatch];     // contributions: move << 8 | endpoint << 7 | image code
@@ -2003,13 +2010,12 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
         T ar[XMAX], ai[XMAX];
 #pragma unroll
         for (int x = 0; x < XMAX; ++x) ar[x] = ai[x] = T(0);
-        // FP32 with d <= 32 (MM: not for the replica sets, whose one-move
-        // items have a contribution or two): the item's sums are a complex
+        // FP32 with d <= 32: the item's sums are a complex
         // product on the matrix cores, S[r][x] = sum_c P_c[r] X_c[x] with
         // P_c[r] = s_c Y_c[y(r)] Z_c[z(r)] (RPI rows by the contributions, by 32
         // x values), four v_mfma_f32_16x16x4f32 per complex 16 x 16 x 4 step;
         // wave wv owns row tiles 2 wv and 2 wv + 1 and both column tiles.
-        constexpr bool MF = MM && std::is_same<T, float>::value && LPR * XMAX <= 32 && RPI == 128;
+        constexpr bool MF = std::is_same<T, float>::value && LPR * XMAX <= 32 && RPI == 128;
         const int li16 = lane & 15, kq = lane >> 4;
         int oy[2], oz[2];
 #pragma unroll
@@ -2504,11 +2510,117 @@ __global__ void __launch_bounds__(TB) k_update1(DevTree<T> t, McArgs a) {
     upd1_body(t, a);
 }
 
-template <typename T, int XMAX>
-__global__ void __launch_bounds__(TB) k_apply_rep(const DevTree<T>* __restrict__ ts, const McArgs* __restrict__ as) {
+// The update of a replica step: one move per replica (replica blockIdx.z),
+// so an element of a rewritten array takes at most two contributions, the
+// target's and the origin's, straight from the trial's phase rows.  Work
+// items are chunks of kRep1J TB consecutive elements: the root array, then
+// the arrays the decision listed (s.dest); block b takes items b, b +
+// gridDim.x, ...  Per item the block stages the x rows of both endpoints and
+// the products s Y[y] Z[z] of the grid rows the chunk spans in LDS; each
+// thread then adds X_t[x] (s Y Z)_t + X_o[x] (s Y Z)_o to its elements
+// (conjugated for an incoming array), whose loads were issued first.
+constexpr int kRep1J = 8;                      // elements per thread and item
+constexpr int kRep1Rows = kRep1J * TB / 4 + 2;  // grid rows a chunk spans (grid rows hold >= 5 elements)
+template <typename T>
+__global__ void __launch_bounds__(TB) k_apply1_rep(const DevTree<T>* __restrict__ ts, const McArgs* __restrict__ as) {
+    __shared__ Cplx<T> sx[2][kMaxGridD], syz[2][kRep1Rows];
+    __shared__ T ssg[2];
     const DevTree<T>& t = ts[blockIdx.z];
     const McArgs& a = as[blockIdx.z];
-    apply_body<T, XMAX, false, false>(t, a);
+    if (*t.err) return;
+    const int nD = a.s.plan[0], nacc = a.s.plan[1], nH = a.s.plan[2];
+    if (!nacc) return;  // uniform: rejected, or no move left
+    constexpr int CH = kRep1J * TB;  // elements per item
+    const int nR = (t.Mw + CH - 1) / CH, nA = (t.Md + CH - 1) / CH;
+    const int total = nR + nD * nA;
+    const TrialInfo& ti = a.s.info[0];
+    const int D = t.max_depth, nL = n_levels_pw(D), n2 = 2 * t.n_pw_nodes;
+    const Cplx<T>* tb = tabs_of<T>(a.s, table_stride(D), 0);
+    for (int it = blockIdx.x; it < total; it += gridDim.x) {
+        const bool is_root = it < nR;
+        int chunk = it, key = 0;
+        if (!is_root) {
+            const int r = it - nR, arr = r / nA;
+            chunk = r % nA;
+            key = a.s.dest[arr < nH ? arr : n2 - 1 - (arr - nH)];
+        }
+        const int slot = key >> 1, kind = is_root ? 2 : key & 1;
+        const int node = is_root ? 0 : a.pn.node[slot];
+        const int l = is_root ? 0 : a.pn.level[slot];
+        const int d = is_root ? t.dw : t.dd, M = is_root ? t.Mw : t.Md;
+        Cplx<T>* dst = is_root ? t.root : (kind == 0 ? t.out : t.inc) + (size_t)slot * t.Md;
+        const int e0 = chunk * CH;
+        Cplx<T> pf[kRep1J];
+#pragma unroll
+        for (int j = 0; j < kRep1J; ++j) pf[j] = dst[min(e0 + (int)threadIdx.x + TB * j, M - 1)];
+        // rows of endpoint ep (0 target, 1 origin) at the image shift of its
+        // box next to the array's box; sign -1 at the origin, 0 when the
+        // endpoint does not touch the array
+        const Cplx<T>* rx[2];
+        const Cplx<T>* ry[2];
+        const Cplx<T>* rz[2];
+        T sg[2];
+#pragma unroll
+        for (int ep = 0; ep < 2; ++ep) {
+            rx[ep] = tb + root_row(ep, 0);
+            ry[ep] = tb + root_row(ep, 1);
+            rz[ep] = tb + root_row(ep, 2);
+            sg[ep] = is_root ? (ep == 0 ? T(1) : T(-1)) : T(0);
+            if (is_root || l > (ep == 0 ? ti.lam_t : ti.lam_o)) continue;
+            const double* x = ep == 0 ? ti.t : ti.o;
+            const int nl = 1 << l;
+            int sh[3];
+            bool same = true, adj = true;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int bc = cell_at(x[c], t.L, l);
+                const int cc = t.cell[3 * node + c];
+                same = same && bc == cc;
+                sh[c] = image_shift(bc, cc, nl);
+                adj = adj && sh[c] != 2;
+            }
+            if (!(same || (kind == 1 && adj))) continue;
+            const int li = l - 2;
+            rx[ep] = tb + level_row(nL, ep, li, 0, same ? 0 : sh[0]);
+            ry[ep] = tb + level_row(nL, ep, li, 1, same ? 0 : sh[1]);
+            rz[ep] = tb + level_row(nL, ep, li, 2, same ? 0 : sh[2]);
+            sg[ep] = ep == 0 ? T(1) : T(-1);
+        }
+        const int r0 = e0 / d, nr = min((min(e0 + CH, M) - 1) / d - r0 + 1, kRep1Rows);
+        __syncthreads();  // the previous item's staged rows are consumed
+        for (int k = threadIdx.x; k < 2 * (d + nr); k += TB) {
+            const int ep = k & 1, w = k >> 1;
+            if (w < d) {
+                sx[ep][w] = rx[ep][w];
+            } else {
+                const int row = r0 + w - d, y = row % d, z = row / d;
+                Cplx<T> yz = cmul(ry[ep][y], rz[ep][z]);
+                yz.x *= sg[ep];
+                yz.y *= sg[ep];
+                syz[ep][w - d] = yz;
+            }
+        }
+        if (threadIdx.x < 2) ssg[threadIdx.x] = sg[threadIdx.x];
+        __syncthreads();
+        const T sy = kind == 1 ? T(-1) : T(1);  // in[C] += conj(tau s)
+#pragma unroll
+        for (int j = 0; j < kRep1J; ++j) {
+            const int e = e0 + threadIdx.x + TB * j;
+            const int x = e % d, rr = e / d - r0;
+            Cplx<T> val = {T(0), T(0)};
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                if (ssg[c] == T(0)) continue;
+                const Cplx<T> yz = syz[c][rr < nr ? rr : 0];
+                const Cplx<T> xv = sx[c][x];
+                val.x = fma(xv.x, yz.x, val.x);
+                val.x = fma(-xv.y, yz.y, val.x);
+                val.y = fma(xv.x, yz.y, val.y);
+                val.y = fma(xv.y, yz.x, val.y);
+            }
+            if (e < M) dst[e] = {pf[j].x + val.x, pf[j].y + sy * val.y};
+        }
+    }
 }
 
 // grids up to 26 (float) or 28 (double) points per axis use the narrow variant
@@ -2620,18 +2732,8 @@ void launch_replica_step(const ReplicaLaunch<T>& r, hipStream_t st) {
     hipLaunchKernelGGL(k_trial_rep<T>, dim3(trial_blocks(r.max_depth, r.tiles), 1, r.n), dim3(TB), 0, st, r.trees,
                        r.args);
     hipLaunchKernelGGL(k_decide_rep<T>, rep, dim3(TB), 0, st, r.trees, r.args);
-    // the replicas share digits: one grid size (the largest) sizes the launch
-    const size_t sh = apply_shmem<T>(r.dmax, 1);
-    if (apply_xmax<T>(r.dmax) <= kApplyXSmall<T>) {
-        allow_shmem((const void*)k_apply_rep<T, kApplyXSmall<T>>, sh);
-        hipLaunchKernelGGL((k_apply_rep<T, kApplyXSmall<T>>), dim3(r.apply_blocks, 1, r.n), dim3(TB), sh, st, r.trees,
-                           r.args);
-    } else {
-        allow_shmem((const void*)k_apply_rep<T, kApplyXLarge<T>>, sh);
-        hipLaunchKernelGGL((k_apply_rep<T, kApplyXLarge<T>>), dim3(r.apply_blocks, 1, r.n), dim3(TB), sh, st, r.trees,
-                           r.args);
-    }
-    hipLaunchKernelGGL(k_advance, dim3((r.n + TB - 1) / TB), dim3(TB), 0, st, r.move_ctr, r.n, 1);
+    hipLaunchKernelGGL(k_apply1_rep<T>, dim3(r.apply_blocks, 1, r.n), dim3(TB), 0, st, r.trees, r.args);
+    // the move counters advance in k_decide_rep
 }
 
 void launch_advance(int* move_ctr, int n, hipStream_t st, int inc) {

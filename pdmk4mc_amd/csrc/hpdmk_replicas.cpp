@@ -35,9 +35,16 @@ struct ReplicaSet {
     int* n_leaves = nullptr;
     int max_depth = 0, max_leaves = 0, apply_blocks = 0;
     int n_moves = 0;
+    // the instantiated whole-run graph and its step count: its kernels take
+    // only set-level pointers (the moves and the chain's arguments are read
+    // from device memory at run time), so a later run of as many steps
+    // launches it again
+    hipGraphExec_t run_exec = nullptr;
+    int run_steps = 0;
 
     ~ReplicaSet() {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (run_exec) (void)hipGraphExecDestroy(run_exec);
         reps.clear();
         for (void* p : owned) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
@@ -111,9 +118,10 @@ int hpdmk_replicas_create(int n_rep, const HPDMKParams* params, int precision, c
             lv.push_back(h->leaves);
             nl.push_back(h->n_leaves);
         }
-        // update blocks per replica: the root alone (2 chunks) without
-        // difference levels, else the rewritten level arrays in 2048-element chunks
-        s->apply_blocks = s->max_depth >= 4 ? 64 : 4;
+        // update blocks per replica (k_apply1_rep, 2048-element work items):
+        // the root alone without difference levels, else the root and the
+        // rewritten level arrays of the move (32 measured best at digits 6)
+        s->apply_blocks = s->max_depth >= 4 ? 32 : 4;
         const size_t tsz = precision == HPDMK_FP64 ? sizeof(DevTree<double>) : sizeof(DevTree<float>);
         s->trees = dalloc<char>(tsz * n_rep, s->owned);
         s->args = dalloc<McArgs>(n_rep, s->owned);
@@ -220,31 +228,55 @@ int hpdmk_replicas_mc_run(hpdmk_replicas s, double beta, int graph_steps, double
                 else
                     launch_replica_compact(launch_args<float>(set), set->stream);
             };
-            // steps run as replays of one captured graph of G steps; the rest
-            // (and every step when graph_steps <= 1) are launched directly
-            const int G = std::min(std::min(graph_steps, n), kReplicaCompactEvery);
-            if (G > 1) {
-                HIPX(hipStreamBeginCapture(set->stream, hipStreamCaptureModeThreadLocal));
-                step(G);
-                HIPX(hipStreamEndCapture(set->stream, &graph));
-                HIPX(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-            }
-            HIPX(hipEventRecord(e0, set->stream));
-            int done = 0, since = 0;
-            while (done < n) {
-                int k = std::min(n - done, kReplicaCompactEvery - since);
-                if (exec && k >= G) {
-                    HIPX(hipGraphLaunch(exec, set->stream));
-                    k = G;
-                } else {
+            // the run: steps with a slot compaction after every
+            // kReplicaCompactEvery; `chunk` issues k consecutive steps
+            auto schedule = [&](auto&& chunk) {
+                int done = 0, since = 0;
+                while (done < n) {
+                    const int k = chunk(std::min(n - done, kReplicaCompactEvery - since));
+                    done += k;
+                    since += k;
+                    if (since >= kReplicaCompactEvery) {
+                        compact();
+                        since = 0;
+                    }
+                }
+            };
+            if (graph_steps >= n && n > 1) {
+                // the whole run captured into one graph: a single launch
+                if (!set->run_exec || set->run_steps != n) {
+                    if (set->run_exec) (void)hipGraphExecDestroy(set->run_exec);
+                    set->run_exec = nullptr;
+                    HIPX(hipStreamBeginCapture(set->stream, hipStreamCaptureModeThreadLocal));
+                    schedule([&](int k) {
+                        step(k);
+                        return k;
+                    });
+                    HIPX(hipStreamEndCapture(set->stream, &graph));
+                    HIPX(hipGraphInstantiate(&set->run_exec, graph, nullptr, nullptr, 0));
+                    set->run_steps = n;
+                }
+                HIPX(hipEventRecord(e0, set->stream));
+                HIPX(hipGraphLaunch(set->run_exec, set->stream));
+            } else {
+                // steps run as replays of one captured graph of G steps; the
+                // rest (and every step when graph_steps <= 1) are launched directly
+                const int G = std::min(std::min(graph_steps, n), kReplicaCompactEvery);
+                if (G > 1) {
+                    HIPX(hipStreamBeginCapture(set->stream, hipStreamCaptureModeThreadLocal));
+                    step(G);
+                    HIPX(hipStreamEndCapture(set->stream, &graph));
+                    HIPX(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+                }
+                HIPX(hipEventRecord(e0, set->stream));
+                schedule([&](int k) {
+                    if (exec && k >= G) {
+                        HIPX(hipGraphLaunch(exec, set->stream));
+                        return G;
+                    }
                     step(k);
-                }
-                done += k;
-                since += k;
-                if (since >= kReplicaCompactEvery) {
-                    compact();
-                    since = 0;
-                }
+                    return k;
+                });
             }
             HIPX(hipGetLastError());
             HIPX(hipEventRecord(e1, set->stream));

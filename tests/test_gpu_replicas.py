@@ -79,11 +79,11 @@ def test_replica_chains_equal_single_chains(precision):
 
 
 def test_replica_matches_oracle_and_graph_equals_direct():
-    log("replicas: oracle, graph vs direct, compaction")
+    log("replicas: oracle, graph vs direct vs whole-run graph, compaction")
     params, systems = ragged(4, 7, lo=500, hi=1500)
     idx, d, u = moves(systems, 700, 8)  # > 512 steps: one slot compaction inside the run
     out = []
-    for gs in (64, 0):
+    for gs in (64, 0, 700):  # 700: the whole run, compaction included, as one graph launch
         rs = P.HPDMKReplicas(params, systems, P.HPDMK_FP64)
         rs.form_pw()
         rs.mc_load(idx, d, u)
@@ -91,9 +91,10 @@ def test_replica_matches_oracle_and_graph_equals_direct():
         out.append(rs.mc_results() + ([rs.tree(k).positions() for k in range(rs.n)],))
         E = [rs.tree(k).eval_energy() for k in range(rs.n)]
         rs.close()
-    (dE1, acc1, x1), (dE2, acc2, x2) = out
-    assert (acc1 == acc2).all() and np.abs(dE1 - dE2).max() < 1e-9
-    assert max(np.abs(a - b).max() for a, b in zip(x1, x2)) < 1e-12
+    (dE1, acc1, x1), (dE2, acc2, x2), (dE3, acc3, x3) = out
+    for dEk, acck, xk in ((dE2, acc2, x2), (dE3, acc3, x3)):
+        assert (acc1 == acck).all() and np.abs(dE1 - dEk).max() < 1e-9
+        assert max(np.abs(a - b).max() for a, b in zip(x1, xk)) < 1e-12
     r, q = systems[0]
     t = O.OracleTree(r, q, params[0].L, n_per_leaf=40, digits=6)
     t.form_pw()
