@@ -23,8 +23,10 @@
 //  * orders: digits <= 3 use order 18 (upward_pass.cpp:9-14); digits 6 use
 //    order 28, one of the orders the reference's polynomial calculator
 //    provides (chebychev.hpp:86-96) and whose interpolation error of the
-//    digits-6 plane waves is below 1e-7; higher digits throw like the
-//    reference.
+//    digits-6 plane waves is below 1e-7; digits 9 use order 44: the
+//    calculator's next order, 38, interpolates the digits-9 plane waves
+//    (|k| b/2 up to 19 pi/3) only to about 1e-7, order 44 to below 1e-10;
+//    higher digits throw like the reference.
 #include <cmath>
 #include <stdexcept>
 
@@ -35,6 +37,7 @@ namespace oracle {
 int proxy_order(int digits) {
     if (digits <= 3) return 18;
     if (digits <= 6) return 28;
+    if (digits <= 9) return 44;
     throw std::runtime_error("Polynomial order for requested precision not implemented");
 }
 

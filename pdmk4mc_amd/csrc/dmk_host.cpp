@@ -396,6 +396,9 @@ Octree build_octree(int n, const double* r, double L, int n_per_leaf) {
 int proxy_order(int digits) {
     if (digits <= 3) return 18;  // src/upward_pass.cpp:9-14
     if (digits <= 6) return 28;  // an order of include/chebychev.hpp:86-96; interpolation error < 1e-7
+    // digits 9: order 44 interpolates the plane waves (|k| b/2 up to 19 pi/3)
+    // to 7e-12; the calculator's 38 only to 3e-8
+    if (digits <= 9) return 44;
     throw HpdmkError(HPDMK_ENOTIMPL, "Polynomial order for requested precision not implemented");
 }
 

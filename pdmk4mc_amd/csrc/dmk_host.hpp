@@ -53,7 +53,7 @@ struct ProxyTables {
     int p = 0;
     std::vector<double> A, P;
 };
-int proxy_order(int digits);  // 18 for digits <= 3 (src/upward_pass.cpp:9-14), 28 for 6, else throws
+int proxy_order(int digits);  // 18 for digits <= 3 (src/upward_pass.cpp:9-14), 28 for 6, 44 for 9, else throws
 ProxyTables proxy_tables(int p, int n);
 
 struct HpdmkError : std::runtime_error {

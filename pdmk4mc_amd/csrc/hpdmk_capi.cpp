@@ -628,42 +628,105 @@ void snapshot(Handle* h, std::vector<double>& r, std::vector<double>& q) {
     });
 }
 
-// PROXY set-up at first use: tables, coefficient storage and the box lists
+// PROXY set-up at first use: tables, coefficient storage, the box lists
 // (every box of level D-2 and every shallower leaf takes its coefficients from
-// its particles; the other boxes of levels 2..D-3 from their children)
+// its particles; the other boxes of levels 2..D-3 from their children) and the
+// scratch of the GEMM passes, sized for chunks of about 48 MB
 static void proxy_setup(Handle* h) {
     if (h->proxy.p) return;
     const int p = proxy_order(h->params.digits);
     const Octree& tr = h->tree;
     const int D = tr.max_depth;
-    const ProxyTables tb = proxy_tables(p, h->pw.n_diff);
+    const int n = h->pw.n_diff, d = 2 * n + 1, nz = n + 1;
+    const ProxyTables tb = proxy_tables(p, n);
     ProxyArgs& pa = h->proxy;
     auto& o = h->owned;
+    const size_t p2 = (size_t)p * p, p3 = p2 * p;
+    // stacked real matrices of the GEMM passes
+    std::vector<double> az8(8 * p2), px(2 * (size_t)d * p), py(4 * (size_t)d * p), pz(4 * (size_t)nz * p);
+    for (int l3 = 0; l3 < p; ++l3)
+        for (int ci = 0; ci < 8; ++ci)
+            for (int c = 0; c < p; ++c) az8[(size_t)l3 * 8 * p + ci * p + c] = tb.A[((ci >> 2) & 1) * p2 + l3 * p + c];
+    auto pre = [&](int k, int a) { return tb.P[2 * ((size_t)k * p + a)]; };
+    auto pim = [&](int k, int a) { return tb.P[2 * ((size_t)k * p + a) + 1]; };
+    for (int k = 0; k < d; ++k)
+        for (int a = 0; a < p; ++a) {
+            px[(size_t)k * p + a] = pre(k, a);
+            px[(size_t)(d + k) * p + a] = pim(k, a);
+        }
+    auto complex_block = [&](std::vector<double>& m, int rows) {  // [[re, -im], [im, re]], rows k < `rows`
+        for (int k = 0; k < rows; ++k)
+            for (int b = 0; b < p; ++b) {
+                m[(size_t)k * 2 * p + b] = pre(k, b);
+                m[(size_t)k * 2 * p + p + b] = -pim(k, b);
+                m[(size_t)(rows + k) * 2 * p + b] = pim(k, b);
+                m[(size_t)(rows + k) * 2 * p + p + b] = pre(k, b);
+            }
+    };
+    complex_block(py, d);
+    complex_block(pz, nz);
     pa.A = dupload(tb.A, o);
-    pa.P = dupload(tb.P, o);
-    pa.coef = dalloc<double>((size_t)h->n_pw * p * p * p, o);
-    std::vector<int> direct, c2p;
-    std::vector<std::vector<int>> by_level(std::max(D - 4, 0));  // levels 2..D-3
+    pa.Az8 = dupload(az8, o);
+    pa.Px = dupload(px, o);
+    pa.Py = dupload(py, o);
+    pa.Pz = dupload(pz, o);
+    pa.coef = dalloc<double>((size_t)h->n_pw * p3, o);
+    std::vector<int> slot_of(tr.n_nodes, -1), direct;
     int slot = 0;
     for (int i = 0; i < tr.n_nodes; ++i) {
         const int l = tr.level[i];
         if (l < 2 || l > D - 2) continue;
-        if (l == D - 2 || tr.is_leaf[i])
-            direct.push_back(slot);
-        else
-            by_level[l - 2].push_back(slot);
+        slot_of[i] = slot;
+        if (l == D - 2 || tr.is_leaf[i]) direct.push_back(slot);
         ++slot;
-    }
-    if (by_level.size() > (size_t)kMaxLevels) throw HpdmkError(HPDMK_EINVAL, "tree too deep for the upward pass");
-    pa.n_levels = (int)by_level.size();
-    for (int li = 0; li < pa.n_levels; ++li) {
-        pa.level_off[li] = (int)c2p.size();
-        pa.level_cnt[li] = (int)by_level[li].size();
-        c2p.insert(c2p.end(), by_level[li].begin(), by_level[li].end());
     }
     pa.direct = dupload(direct, o);
     pa.n_direct = (int)direct.size();
-    pa.c2p = dupload(c2p, o);
+    // child -> parent chunks, deepest parent level first
+    const size_t budget = (size_t)48 << 20;
+    const int per_chunk = (int)std::max<size_t>(1, budget / (8 * p3 * sizeof(double)));
+    size_t max_ch = 0;
+    pa.c2p.clear();
+    for (int l = D - 3; l >= 2; --l) {
+        std::vector<int> parents;
+        for (int i = 0; i < tr.n_nodes; ++i)
+            if (tr.level[i] == l && !tr.is_leaf[i]) parents.push_back(i);
+        for (size_t c0 = 0; c0 < parents.size(); c0 += per_chunk) {
+            const size_t c1 = std::min(parents.size(), c0 + per_chunk);
+            std::vector<int> par, ch, ox, oy, seg;
+            for (size_t k = c0; k < c1; ++k) {
+                const int nd = parents[k];
+                par.push_back(slot_of[nd]);
+                for (int ci = 0; ci < 8; ++ci) {
+                    const int c = tr.child[8 * nd + ci];
+                    if (c < 0 || slot_of[c] < 0) {
+                        seg.push_back(-1);
+                        continue;
+                    }
+                    seg.push_back((int)ch.size());
+                    ch.push_back(slot_of[c]);
+                    ox.push_back(ci & 1);
+                    oy.push_back((ci >> 1) & 1);
+                }
+            }
+            ProxyChunk pc;
+            pc.n_par = (int)par.size();
+            pc.n_ch = (int)ch.size();
+            pc.par = dupload(par, o);
+            pc.ch = dupload(ch, o);
+            pc.ox = dupload(ox, o);
+            pc.oy = dupload(oy, o);
+            pc.seg = dupload(seg, o);
+            pa.c2p.push_back(pc);
+            max_ch = std::max(max_ch, ch.size());
+        }
+    }
+    pa.n = n, pa.d = d, pa.nz = nz;
+    const size_t y1 = 2 * (size_t)d * p2, y2 = 2 * (size_t)d * d * p, y3 = 2 * (size_t)nz * d * d;
+    pa.pw_chunk = (int)std::max<size_t>(1, std::min<size_t>(std::max(h->n_pw, 1), budget / (y1 * sizeof(double))));
+    pa.s1 = dalloc<double>(std::max(max_ch * p3, pa.pw_chunk * y1), o);
+    pa.s2 = dalloc<double>(std::max(max_ch * p3, pa.pw_chunk * y2), o);
+    pa.s3 = dalloc<double>(pa.pw_chunk * y3, o);
     pa.p = p;
 }
 

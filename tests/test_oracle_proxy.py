@@ -13,7 +13,8 @@
 * PlanewaveProxy (test/hpdmk_test.cpp:17-127, 547-549): the PROXY expansions
   against brute-force sums, root to 1e-3 relative and boxes to 1e-3
   absolute, plus agreement with DIRECT;
-* the digits-6 extension (order 28): expansions and energies agree with DIRECT.
+* the digits-6 and digits-9 extensions (orders 28 and 44): expansions and
+  energies agree with DIRECT.
 """
 import numpy as np
 import pytest
@@ -34,7 +35,7 @@ def cheb_T(p, x):
     return T
 
 
-@pytest.mark.parametrize("p", [18, 28])
+@pytest.mark.parametrize("p", [18, 28, 44])
 def test_vandermonde_inverse_closed_form(p):
     x = np.cos(np.pi * ((p - np.arange(p) - 1) + 0.5) / p)
     closed = (2.0 - (np.arange(p) == 0)) [:, None] / p * cheb_T(p, x)  # [a, i]
@@ -42,7 +43,7 @@ def test_vandermonde_inverse_closed_form(p):
     assert np.abs(Vi - closed).max() < 1e-12
 
 
-@pytest.mark.parametrize("p", [18, 28])
+@pytest.mark.parametrize("p", [18, 28, 44])
 def test_child_to_parent_identity(p):
     x = np.random.default_rng(1).uniform(-1, 1, 50)
     Tx = cheb_T(p, x)
@@ -52,7 +53,7 @@ def test_child_to_parent_identity(p):
         assert np.abs(lhs - A @ Tx).max() < 1e-9 * p
 
 
-@pytest.mark.parametrize("p,n,tol", [(18, 6, 1e-6), (28, 12, 1e-6)])
+@pytest.mark.parametrize("p,n,tol", [(18, 6, 1e-6), (28, 12, 1e-6), (44, 19, 1e-10)])
 def test_poly2pw_interpolates_plane_waves(p, n, tol):
     P = O.cheb_matrix(3, p, n)
     x = np.random.default_rng(2).uniform(-1, 1, 400)
@@ -64,11 +65,12 @@ def test_poly2pw_interpolates_plane_waves(p, n, tol):
 def test_proxy_order_table():
     assert O.proxy_order(3) == 18  # src/upward_pass.cpp:9-14
     assert O.proxy_order(6) == 28
+    assert O.proxy_order(9) == 44
     with pytest.raises(RuntimeError):
-        O.proxy_order(9)
+        O.proxy_order(12)
 
 
-@pytest.mark.parametrize("n,leaf,digits", [(1000, 5, 3), (3000, 6, 6)])
+@pytest.mark.parametrize("n,leaf,digits", [(1000, 5, 3), (3000, 6, 6), (1000, 5, 9)])
 def test_transfer_equals_direct_coefficients(n, leaf, digits):
     r, q = O.reference_test_system(n, L)
     t = O.OracleTree(r, q, L, n_per_leaf=leaf, digits=digits)
@@ -130,7 +132,7 @@ def test_planewave_proxy_contract():
     assert nboxes > 0
 
 
-@pytest.mark.parametrize("n,leaf,digits,tol", [(1000, 5, 3, 1e-5), (3000, 6, 6, 1e-7)])
+@pytest.mark.parametrize("n,leaf,digits,tol", [(1000, 5, 3, 1e-5), (3000, 6, 6, 1e-7), (1000, 5, 9, 1e-10)])
 def test_proxy_matches_direct(n, leaf, digits, tol):
     """PROXY expansions against DIRECT ones (relative to the box's total |q|),
     and the energies of the two builds."""
