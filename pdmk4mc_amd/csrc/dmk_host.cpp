@@ -402,7 +402,7 @@ int proxy_order(int digits) {
     throw HpdmkError(HPDMK_ENOTIMPL, "Polynomial order for requested precision not implemented");
 }
 
-ProxyTables proxy_tables(int p, int n) {
+ProxyTables proxy_tables(int p, int n, double hw) {
     ProxyTables t;
     t.p = p;
     std::vector<double> x(p), Vi((size_t)p * p);
@@ -429,7 +429,7 @@ ProxyTables proxy_tables(int p, int n) {
     t.P.assign(2 * (size_t)d * p, 0.0);
     for (int k = 0; k < d; ++k)
         for (int i = 0; i < p; ++i) {
-            const double th = (k - n) * (M_PI / 3.0) * x[i];
+            const double th = (k - n) * hw * x[i];
             const double c = std::cos(th), s = -std::sin(th);
             for (int a = 0; a < p; ++a) {
                 t.P[2 * ((size_t)k * p + a)] += c * Vi[(size_t)a * p + i];

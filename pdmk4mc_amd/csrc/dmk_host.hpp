@@ -3,6 +3,7 @@
 // window/difference kernel tables that are uploaded to HBM.
 #pragma once
 
+#include <cmath>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -54,7 +55,9 @@ struct ProxyTables {
     std::vector<double> A, P;
 };
 int proxy_order(int digits);  // 18 for digits <= 3 (src/upward_pass.cpp:9-14), 28 for 6, 44 for 9, else throws
-ProxyTables proxy_tables(int p, int n);
+// A: child-to-parent matrices; P[k][a] (complex) = sum_i e^{-i (k - n) hw x_i} (V^-1)[a][i]
+// over the Chebyshev nodes x_i: hw = (k step) x (half the box size), pi / 3 on every level
+ProxyTables proxy_tables(int p, int n, double hw = M_PI / 3.0);
 
 struct HpdmkError : std::runtime_error {
     int code;

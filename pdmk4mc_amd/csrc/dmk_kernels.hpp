@@ -12,6 +12,7 @@ namespace hpdmk_amd {
 
 constexpr int kMaxBatch = 256;     // trials per batched step
 constexpr int kMarkWords = 4096;   // shared-memory bitmap of rewritten arrays: 2 n_pw <= 32 kMarkWords
+constexpr int kMaxGridD = 53;  // largest 2n+1 of any digits setting
 constexpr int kRowStride = 64;     // stride of one 1-D phase row (2n+1 <= 53)
 constexpr int kAccPerTrial = 7;    // window, difference, res_t, res_o, scanned_t, scanned_o, overlaps_t
 // per trial block: partial sum, particles scanned, hard-sphere overlaps, and

@@ -26,7 +26,6 @@
 
 namespace hpdmk_amd {
 
-constexpr int kMaxGridD = 53;  // largest 2n+1 of any digits setting
 
 template <typename T>
 __device__ __forceinline__ Cplx<T>* tabs_of(const McScratch& s, int stride, int tr) {

@@ -20,6 +20,10 @@
 //                 times [X_re; X_im].  The z pass of the transfer sums the
 //                 eight children of a parent in one GEMM of depth 8p.
 //   k_pw_out      the de-centring phase and the store of the expansions
+//   k_root_sum    the root expansion from the level-2 boxes' coefficients at
+//                 the root's wavenumbers (their plane waves, shifted to the
+//                 boxes' centres and summed), when every particle lies in a
+//                 level-2 box; else the root comes from the particles
 //
 // Layouts: coefficients of pw slot s at coef + s p^3 (a fastest); the
 // matrices of ProxyArgs (dmk_proxy.hpp).
@@ -77,15 +81,17 @@ __device__ __forceinline__ long gemm_row(const GemmOp& o, int b0, int r) {
 
 }  // namespace
 
-// Batched GEMM on the FP64 matrix cores: block (batch entry, 64-column tile,
-// 32-row tile), four waves of 16 x 32 each (two 16 x 16 tiles); operands
-// read from memory (L2) four K-steps at a time, all loads in flight before
-// the products.
+// Batched GEMM on the FP64 matrix cores: block (batch entry, column tile,
+// row tile), four waves of 16 x 32 each (two 16 x 16 tiles), WM of them
+// along the rows (block tile 16 WM x 128 / WM); operands read from memory
+// (L2) four K-steps at a time, all loads in flight before the products.
+template <int WM>
 __global__ void __launch_bounds__(256) k_gemm64(GemmPass g) {
+    constexpr int BM = 16 * WM, BN = 32 * (4 / WM);
     const int b = blockIdx.x, b0 = b / g.inner, b1 = b - b0 * g.inner;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
-    const int m0 = blockIdx.z * 32 + (wv & 1) * 16, n0 = blockIdx.y * 64 + (wv >> 1) * 32;
+    const int m0 = blockIdx.z * BM + (wv % WM) * 16, n0 = blockIdx.y * BN + (wv / WM) * 32;
     const double* A = g.a.p + gemm_base(g.a, b0, b1);
     const double* B = g.b.p + gemm_base(g.b, b0, b1);
     double* C = g.c.p + gemm_base(g.c, b0, b1);
@@ -131,7 +137,10 @@ __global__ void __launch_bounds__(256) k_gemm64(GemmPass g) {
 
 void launch_gemm64(const GemmPass& g, hipStream_t st) {
     if (g.batch <= 0 || g.M <= 0 || g.N <= 0) return;
-    hipLaunchKernelGGL(k_gemm64, dim3(g.batch, (g.N + 63) / 64, (g.M + 31) / 32), dim3(256), 0, st, g);
+    if (g.N <= 32)  // narrow products: 64 x 32 block tiles
+        hipLaunchKernelGGL(k_gemm64<4>, dim3(g.batch, (g.N + 31) / 32, (g.M + 63) / 64), dim3(256), 0, st, g);
+    else
+        hipLaunchKernelGGL(k_gemm64<2>, dim3(g.batch, (g.N + 63) / 64, (g.M + 31) / 32), dim3(256), 0, st, g);
 }
 
 // particles -> coefficients.  Block (box, 256-column slice of b + p c): wave
@@ -233,27 +242,62 @@ __global__ void __launch_bounds__(256) k_proxy_leaf(DevTree<T> t, PwNodes pn, co
 template <typename T>
 __global__ void __launch_bounds__(256) k_pw_out(DevTree<T> t, PwNodes pn, const double* __restrict__ s3, int n,
                                                 int b0) {
+    __shared__ double2 ph[3][kMaxGridD];  // e^{-i dk (k - n) c} per axis
     const int slot = b0 + blockIdx.x;
     const int node = pn.node[slot];
     const int l = pn.level[slot];
     const int d = 2 * n + 1, nz = n + 1, d2 = d * d;
     const double dk = t.delta_k[l];
-    const double cx = box_centre(t, node, 0), cy = box_centre(t, node, 1), cz = box_centre(t, node, 2);
+    for (int k = threadIdx.x; k < 3 * d; k += blockDim.x) {
+        const int ax = k / d, kk = k - ax * d;
+        double s, c;
+        sincos(dk * (kk - n) * box_centre(t, node, ax), &s, &c);
+        ph[ax][kk] = make_double2(c, -s);
+    }
+    __syncthreads();
     const double* Y = s3 + (size_t)blockIdx.x * 2 * nz * d2;
     Cplx<T>* out = t.out + (size_t)slot * t.Md;
     for (int e = threadIdx.x; e < nz * d2; e += blockDim.x) {
         const int kz = e / d2, kxy = e - kz * d2, kx = kxy % d, ky = kxy / d;
-        double sxy, cxy, sz, czz;
-        sincos(dk * ((kx - n) * cx + (ky - n) * cy), &sxy, &cxy);
-        sincos(dk * (kz - n) * cz, &sz, &czz);
-        const double pr = cxy * czz - sxy * sz, pi = -(sxy * czz + cxy * sz);  // e^{-i k.c}
+        const double2 px = ph[0][kx], py = ph[1][ky], pz = ph[2][kz];
+        const double xyr = px.x * py.x - px.y * py.y, xyi = px.x * py.y + px.y * py.x;
+        const double pr = xyr * pz.x - xyi * pz.y, pi = xyr * pz.y + xyi * pz.x;  // e^{-i k.c}
         const double ar = Y[e], ai = Y[nz * d2 + e];
         out[e] = {(T)(ar * pr - ai * pi), (T)(ar * pi + ai * pr)};
     }
 }
 
+// the root expansion from the level-2 boxes' plane waves (z pass results in
+// s3, planes re, im of nzw dw^2 per box): root[k] = sum_b e^{-i k.c_b} Y_b[k]
+// over the boxes in slot order, in FP64, then stored in the tree's precision
+template <typename T>
+__global__ void __launch_bounds__(256) k_root_sum(DevTree<T> t, PwNodes pn, const int* __restrict__ slots, int nb,
+                                                  const double* __restrict__ s3, double* __restrict__ acc) {
+    const int nw = t.n_window, dw = t.dw, nzw = nw + 1, d2 = dw * dw;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nzw * d2) return;
+    const int kz = e / d2, kxy = e - kz * d2, kx = kxy % dw, ky = kxy / dw;
+    const double dk = t.delta_k[0];
+    double sr = 0.0, si = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        const int node = pn.node[slots[b]];
+        double s, c;
+        sincos(dk * ((kx - nw) * box_centre(t, node, 0) + (ky - nw) * box_centre(t, node, 1) +
+                     (kz - nw) * box_centre(t, node, 2)),
+               &s, &c);
+        const double* Y = s3 + (size_t)b * 2 * nzw * d2;
+        const double ar = Y[e], ai = Y[nzw * d2 + e];
+        sr += ar * c + ai * s;  // (ar + i ai) e^{-i theta}
+        si += ai * c - ar * s;
+    }
+    acc[2 * e] = sr;
+    acc[2 * e + 1] = si;
+    t.root[e] = {(T)sr, (T)si};
+}
+
 template <int P, typename T>
-static void launch_proxy_p(const DevTree<T>& t, const PwNodes& pn, const ProxyArgs& pa, hipStream_t st) {
+static void launch_proxy_p(const DevTree<T>& t, const PwNodes& pn, double* acc_root, const ProxyArgs& pa,
+                           hipStream_t st) {
     constexpr int CB = 256;
     if (pa.n_direct > 0)
         hipLaunchKernelGGL((k_proxy_leaf<P, T>), dim3(pa.n_direct, (P * P + CB - 1) / CB), dim3(256), 0, st, t, pn,
@@ -281,30 +325,42 @@ static void launch_proxy_p(const DevTree<T>& t, const PwNodes& pn, const ProxyAr
         z.c.p = pa.coef, z.c.ld = p2, z.c.tab = c.par, z.c.s0 = p3;
         launch_gemm64(z, st);
     }
-    // coefficients -> plane waves, boxes in chunks of pa.pw_chunk
-    const int d = pa.d, nz = pa.nz;
-    const long d2 = (long)d * d;
-    for (int b0 = 0; b0 < t.n_pw_nodes; b0 += pa.pw_chunk) {
-        const int nb = std::min(pa.pw_chunk, t.n_pw_nodes - b0);
-        GemmPass x;  // Y1[(plane d + kx) + 2d (b + p c)] = sum_a C[a + p (b + p c)] Px[plane d + kx][a]
+    // coefficients -> plane waves: the x, y and z passes of nb boxes (slots
+    // tab[i], or b0 + i) with the stacked matrices of grid d, nz into s3
+    auto pw3 = [&](const int* tab, int b0, int nb, const double* Mx, const double* My, const double* Mz, int d,
+                   int nz) {
+        const long d2 = (long)d * d;
+        GemmPass x;  // Y1[(plane d + kx) + 2d (b + p c)] = sum_a C[a + p (b + p c)] Mx[plane d + kx][a]
         x.M = (int)p2, x.K = p, x.N = 2 * d, x.batch = nb;
-        x.a.p = pa.coef + (size_t)b0 * p3, x.a.ld = p, x.a.s0 = p3;
-        x.b.p = const_cast<double*>(pa.Px), x.b.ld = p, x.b.col = 1;
+        x.a.p = pa.coef + (size_t)b0 * p3, x.a.ld = p, x.a.tab = tab, x.a.s0 = p3;
+        x.b.p = const_cast<double*>(Mx), x.b.ld = p, x.b.col = 1;
         x.c.p = pa.s1, x.c.ld = 2 * d, x.c.s0 = 2 * d * p2;
         launch_gemm64(x, st);
-        GemmPass y;  // Y2[kx + d ky + d^2 c + d^2 p plane] = sum_{plane', b} Py[plane d + ky][plane' p + b] Y1[plane' d + kx + 2d (b + p c)]
+        GemmPass y;  // Y2[kx + d ky + d^2 c + d^2 p plane] = sum_{plane', b} My[plane d + ky][plane' p + b] Y1[plane' d + kx + 2d (b + p c)]
         y.M = 2 * d, y.K = 2 * p, y.N = d, y.inner = p, y.batch = nb * p;
-        y.a.p = const_cast<double*>(pa.Py), y.a.ld = 2 * p;
+        y.a.p = const_cast<double*>(My), y.a.ld = 2 * p;
         y.b.p = pa.s1, y.b.ld = 2 * d, y.b.s0 = 2 * d * p2, y.b.s1 = 2 * d * p, y.b.seg = p, y.b.nseg = 2,
         y.b.sstride = d;
         y.c.p = pa.s2, y.c.ld = d, y.c.s0 = 2 * d2 * p, y.c.s1 = d2, y.c.seg = d, y.c.nseg = 2, y.c.sstride = d2 * p;
         launch_gemm64(y, st);
-        GemmPass z;  // Y3[kxy + d^2 kz + d^2 nz plane] = sum_{plane', c} Pz[plane nz + kz][plane' p + c] Y2[kxy + d^2 c + d^2 p plane']
+        GemmPass z;  // Y3[kxy + d^2 kz + d^2 nz plane] = sum_{plane', c} Mz[plane nz + kz][plane' p + c] Y2[kxy + d^2 c + d^2 p plane']
         z.M = 2 * nz, z.K = 2 * p, z.N = (int)d2, z.batch = nb;
-        z.a.p = const_cast<double*>(pa.Pz), z.a.ld = 2 * p;
+        z.a.p = const_cast<double*>(Mz), z.a.ld = 2 * p;
         z.b.p = pa.s2, z.b.ld = d2, z.b.s0 = 2 * d2 * p, z.b.seg = p, z.b.nseg = 2, z.b.sstride = d2 * p;
         z.c.p = pa.s3, z.c.ld = d2, z.c.s0 = 2 * d2 * nz, z.c.seg = nz, z.c.nseg = 2, z.c.sstride = d2 * nz;
         launch_gemm64(z, st);
+    };
+    // the root from the level-2 boxes, at the root's wavenumbers
+    if (pa.n_root > 0) {
+        const int dw = 2 * pa.nw + 1, nzw = pa.nw + 1;
+        pw3(pa.root_slots, 0, pa.n_root, pa.Prx, pa.Pry, pa.Prz, dw, nzw);
+        hipLaunchKernelGGL(k_root_sum<T>, dim3((nzw * dw * dw + 255) / 256), dim3(256), 0, st, t, pn, pa.root_slots,
+                           pa.n_root, pa.s3, acc_root);
+    }
+    // every expansion box, in chunks of pa.pw_chunk
+    for (int b0 = 0; b0 < t.n_pw_nodes; b0 += pa.pw_chunk) {
+        const int nb = std::min(pa.pw_chunk, t.n_pw_nodes - b0);
+        pw3(nullptr, b0, nb, pa.Px, pa.Py, pa.Pz, pa.d, pa.nz);
         hipLaunchKernelGGL(k_pw_out<T>, dim3(nb), dim3(256), 0, st, t, pn, pa.s3, pa.n, b0);
     }
 }
@@ -312,13 +368,13 @@ static void launch_proxy_p(const DevTree<T>& t, const PwNodes& pn, const ProxyAr
 template <typename T>
 void launch_outgoing_proxy(const DevTree<T>& t, const PwNodes& pn, double* acc_root, const ProxyArgs& pa,
                            hipStream_t st) {
-    launch_outgoing_root(t, acc_root, st);
+    if (pa.n_root == 0) launch_outgoing_root(t, acc_root, st);  // from the particles
     if (pa.p == 18)
-        launch_proxy_p<18>(t, pn, pa, st);  // digits 3
+        launch_proxy_p<18>(t, pn, acc_root, pa, st);  // digits 3
     else if (pa.p == 28)
-        launch_proxy_p<28>(t, pn, pa, st);  // digits 6
+        launch_proxy_p<28>(t, pn, acc_root, pa, st);  // digits 6
     else
-        launch_proxy_p<44>(t, pn, pa, st);  // digits 9
+        launch_proxy_p<44>(t, pn, acc_root, pa, st);  // digits 9
 }
 
 template void launch_outgoing_proxy<double>(const DevTree<double>&, const PwNodes&, double*, const ProxyArgs&,

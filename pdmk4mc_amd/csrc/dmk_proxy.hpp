@@ -48,11 +48,20 @@ struct ProxyChunk {
 struct ProxyArgs {
     int p = 0;                   // Chebyshev order: 18 (digits 3), 28 (digits 6), 44 (digits 9)
     int n = 0, d = 0, nz = 0;    // plane waves: k = -n..n per axis (d = 2n + 1), kz <= 0 (nz = n + 1)
+    int nw = 0;                  // root (window) plane waves: k = -nw..nw per axis
     const double* A = nullptr;   // child-to-parent matrices, lower then upper half, A[h p^2 + l p + a]
     const double* Az8 = nullptr; // p x 8p: [A_{z(0)} .. A_{z(7)}], z(ci) = ci >> 2 (the z pass of all octants)
     const double* Px = nullptr;  // 2d x p: rows (re, kx) then (im, kx) of the coefficient-to-plane-wave matrix
     const double* Py = nullptr;  // 2d x 2p: [[P_re, -P_im], [P_im, P_re]]
     const double* Pz = nullptr;  // 2nz x 2p: the same for kz = 0..n
+    // the root from the level-2 boxes: the coefficient-to-plane-wave matrix
+    // at the root's wavenumbers (k - nw) 2 pi / L over a box of size L / 4,
+    // stacked like Px, Py, Pz; used when every particle lies in a level-2 box
+    const double* Prx = nullptr;
+    const double* Pry = nullptr;
+    const double* Prz = nullptr;
+    const int* root_slots = nullptr;  // pw slots of the level-2 boxes
+    int n_root = 0;                   // 0: the root from the particles
     double* coef = nullptr;      // p^3 per pw slot, a fastest
     double* s1 = nullptr;        // scratch: x-pass results (c2p: chunk children p^3; pw: chunk boxes 2d p^2)
     double* s2 = nullptr;        // scratch: y-pass results (c2p: chunk children p^3; pw: chunk boxes 2 d^2 p)

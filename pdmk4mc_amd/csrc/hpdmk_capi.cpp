@@ -647,24 +647,28 @@ static void proxy_setup(Handle* h) {
     for (int l3 = 0; l3 < p; ++l3)
         for (int ci = 0; ci < 8; ++ci)
             for (int c = 0; c < p; ++c) az8[(size_t)l3 * 8 * p + ci * p + c] = tb.A[((ci >> 2) & 1) * p2 + l3 * p + c];
-    auto pre = [&](int k, int a) { return tb.P[2 * ((size_t)k * p + a)]; };
-    auto pim = [&](int k, int a) { return tb.P[2 * ((size_t)k * p + a) + 1]; };
-    for (int k = 0; k < d; ++k)
-        for (int a = 0; a < p; ++a) {
-            px[(size_t)k * p + a] = pre(k, a);
-            px[(size_t)(d + k) * p + a] = pim(k, a);
-        }
-    auto complex_block = [&](std::vector<double>& m, int rows) {  // [[re, -im], [im, re]], rows k < `rows`
-        for (int k = 0; k < rows; ++k)
-            for (int b = 0; b < p; ++b) {
-                m[(size_t)k * 2 * p + b] = pre(k, b);
-                m[(size_t)k * 2 * p + p + b] = -pim(k, b);
-                m[(size_t)(rows + k) * 2 * p + b] = pim(k, b);
-                m[(size_t)(rows + k) * 2 * p + p + b] = pre(k, b);
+    // stacked planes of a complex d x p matrix P: x (2d x p, rows re then
+    // im) and [[re, -im], [im, re]] of its first `rows` rows (2 rows x 2p)
+    auto stack_x = [p](const std::vector<double>& P, int dd, std::vector<double>& m) {
+        for (int k = 0; k < dd; ++k)
+            for (int a = 0; a < p; ++a) {
+                m[(size_t)k * p + a] = P[2 * ((size_t)k * p + a)];
+                m[(size_t)(dd + k) * p + a] = P[2 * ((size_t)k * p + a) + 1];
             }
     };
-    complex_block(py, d);
-    complex_block(pz, nz);
+    auto complex_block = [p](const std::vector<double>& P, std::vector<double>& m, int rows) {
+        for (int k = 0; k < rows; ++k)
+            for (int b = 0; b < p; ++b) {
+                const double re = P[2 * ((size_t)k * p + b)], im = P[2 * ((size_t)k * p + b) + 1];
+                m[(size_t)k * 2 * p + b] = re;
+                m[(size_t)k * 2 * p + p + b] = -im;
+                m[(size_t)(rows + k) * 2 * p + b] = im;
+                m[(size_t)(rows + k) * 2 * p + p + b] = re;
+            }
+    };
+    stack_x(tb.P, d, px);
+    complex_block(tb.P, py, d);
+    complex_block(tb.P, pz, nz);
     pa.A = dupload(tb.A, o);
     pa.Az8 = dupload(az8, o);
     pa.Px = dupload(px, o);
@@ -724,9 +728,39 @@ static void proxy_setup(Handle* h) {
     pa.n = n, pa.d = d, pa.nz = nz;
     const size_t y1 = 2 * (size_t)d * p2, y2 = 2 * (size_t)d * d * p, y3 = 2 * (size_t)nz * d * d;
     pa.pw_chunk = (int)std::max<size_t>(1, std::min<size_t>(std::max(h->n_pw, 1), budget / (y1 * sizeof(double))));
-    pa.s1 = dalloc<double>(std::max(max_ch * p3, pa.pw_chunk * y1), o);
-    pa.s2 = dalloc<double>(std::max(max_ch * p3, pa.pw_chunk * y2), o);
-    pa.s3 = dalloc<double>(pa.pw_chunk * y3, o);
+    size_t n1 = std::max(max_ch * p3, pa.pw_chunk * y1), n2 = std::max(max_ch * p3, pa.pw_chunk * y2),
+           n3 = pa.pw_chunk * y3;
+    // the root from the level-2 boxes when every particle lies in one (no
+    // leaf above level 2): root wavenumbers (k - nw) 2 pi / L over boxes of
+    // size L / 4, i.e. hw = pi / 4
+    bool root_ok = D - 2 >= 2;
+    for (int i = 0; i < tr.n_nodes && root_ok; ++i)
+        if (tr.level[i] < 2 && tr.is_leaf[i]) root_ok = false;
+    pa.nw = h->pw.n_window;
+    pa.n_root = 0;
+    if (root_ok) {
+        const int nw = pa.nw, dw = 2 * nw + 1, nzw = nw + 1;
+        const ProxyTables tr2 = proxy_tables(p, nw, M_PI / 4.0);
+        std::vector<double> prx(2 * (size_t)dw * p), pry(4 * (size_t)dw * p), prz(4 * (size_t)nzw * p);
+        stack_x(tr2.P, dw, prx);
+        complex_block(tr2.P, pry, dw);
+        complex_block(tr2.P, prz, nzw);
+        pa.Prx = dupload(prx, o);
+        pa.Pry = dupload(pry, o);
+        pa.Prz = dupload(prz, o);
+        std::vector<int> rs;
+        for (int i = 0; i < tr.n_nodes; ++i)
+            if (tr.level[i] == 2) rs.push_back(slot_of[i]);
+        pa.root_slots = dupload(rs, o);
+        pa.n_root = (int)rs.size();
+        const size_t nr = rs.size();
+        n1 = std::max(n1, nr * 2 * dw * p2);
+        n2 = std::max(n2, nr * 2 * (size_t)dw * dw * p);
+        n3 = std::max(n3, nr * 2 * (size_t)nzw * dw * dw);
+    }
+    pa.s1 = dalloc<double>(n1, o);
+    pa.s2 = dalloc<double>(n2, o);
+    pa.s3 = dalloc<double>(n3, o);
     pa.p = p;
 }
 
