@@ -273,22 +273,30 @@ __global__ void __launch_bounds__(256) k_pw_out(DevTree<T> t, PwNodes pn, const 
 template <typename T>
 __global__ void __launch_bounds__(256) k_root_sum(DevTree<T> t, PwNodes pn, const int* __restrict__ slots, int nb,
                                                   const double* __restrict__ s3, double* __restrict__ acc) {
+    // e^{-i dk (k - nw) c} for the four level-2 cell centres c = (i + 1/2) L / 4 of an axis
+    __shared__ double2 ph[4][kMaxGridD];
     const int nw = t.n_window, dw = t.dw, nzw = nw + 1, d2 = dw * dw;
+    const double dk = t.delta_k[0], b2 = t.boxsize[2];
+    for (int k = threadIdx.x; k < 4 * dw; k += blockDim.x) {
+        const int ci = k / dw, kk = k - ci * dw;
+        double s, c;
+        sincos(dk * (kk - nw) * ((ci + 0.5) * b2), &s, &c);
+        ph[ci][kk] = make_double2(c, -s);
+    }
+    __syncthreads();
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nzw * d2) return;
     const int kz = e / d2, kxy = e - kz * d2, kx = kxy % dw, ky = kxy / dw;
-    const double dk = t.delta_k[0];
     double sr = 0.0, si = 0.0;
     for (int b = 0; b < nb; ++b) {
         const int node = pn.node[slots[b]];
-        double s, c;
-        sincos(dk * ((kx - nw) * box_centre(t, node, 0) + (ky - nw) * box_centre(t, node, 1) +
-                     (kz - nw) * box_centre(t, node, 2)),
-               &s, &c);
+        const double2 px = ph[t.cell[3 * node]][kx], py = ph[t.cell[3 * node + 1]][ky], pz = ph[t.cell[3 * node + 2]][kz];
+        const double xyr = px.x * py.x - px.y * py.y, xyi = px.x * py.y + px.y * py.x;
+        const double pr = xyr * pz.x - xyi * pz.y, pi = xyr * pz.y + xyi * pz.x;  // e^{-i k.c_b}
         const double* Y = s3 + (size_t)b * 2 * nzw * d2;
         const double ar = Y[e], ai = Y[nzw * d2 + e];
-        sr += ar * c + ai * s;  // (ar + i ai) e^{-i theta}
-        si += ai * c - ar * s;
+        sr += ar * pr - ai * pi;
+        si += ar * pi + ai * pr;
     }
     acc[2 * e] = sr;
     acc[2 * e + 1] = si;
