@@ -10,7 +10,8 @@ starts.
 
 Headline: trial moves/s of the batched chain with the learned proposal
 (displacements drawn from an MLP trained on the GPU: warm start on a
-uniform chain, then policy-gradient rounds on the expected squared jump;
+uniform chain, then policy-gradient rounds on the expected squared energy
+(or displacement) jump;
 exact Metropolis-Hastings ratio), judged against the uniform proposal with
 its step tuned on the same system.  Beside it: the batched chain with uniform
 displacements U[-1,1]^3, the plain one-trial-at-a-time GPU chain (uniform),
@@ -259,12 +260,16 @@ def main():
     ap.add_argument("--train-epochs", type=int, default=20)
     ap.add_argument("--pg-rounds", type=int, default=6, help="policy-gradient rounds after the warm start")
     ap.add_argument("--pg-moves", type=int, default=50000, help="learned-chain moves per policy-gradient round")
+    ap.add_argument("--pg-objective", choices=["energy", "distance"], default="energy",
+                    help="policy-gradient reward: accepted squared energy jump (the judge is the energy "
+                         "autocorrelation) or squared displacement")
     ap.add_argument("--judge-steps", default="0.5,1,2,3,4,6", help="uniform steps the judge scans (N=1e5)")
     ap.add_argument("--hs-n", type=int, default=20000)
     ap.add_argument("--hs-moves", type=int, default=200000)
     ap.add_argument("--hs-train", type=int, default=100000, help="uniform-chain moves of the warm start")
     ap.add_argument("--hs-train-step", type=float, default=1.0)
-    ap.add_argument("--hs-pg-rounds", type=int, default=20)
+    ap.add_argument("--hs-pg-rounds", type=int, default=10)
+    ap.add_argument("--hs-equil-sweeps", type=int, default=100, help="uniform sweeps (step 1) before training")
     ap.add_argument("--hs-rf", type=float, default=2.0)
     ap.add_argument("--hs-judge-steps", default="1,1.5,2,2.5,3",
                     help="uniform steps the judge scans by energy autocorrelation (hard spheres)")
@@ -363,7 +368,7 @@ def main():
         gc.form_incoming_pw()
         w, hist = PR.train_proposal(gc, args.n, dict(rf=args.rf, **cfgp), warm_step=args.train_step,
                                     warm_moves=args.train_moves, rounds=args.pg_rounds, round_moves=args.pg_moves,
-                                    warm_epochs=args.train_epochs, seed=args.seed)
+                                    warm_epochs=args.train_epochs, seed=args.seed, objective=args.pg_objective)
         t_train = time.time() - t0
         judge_step, judge = PR.tune_uniform(gc, args.n, [float(x) for x in args.judge_steps.split(",")], 50000,
                                             seed=args.seed + 24)
@@ -397,7 +402,9 @@ def main():
             "proposal": dict(rf=args.rf, **cfgp),
             "training": {"warm_start": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in hist[0].items()},
                          "warm_uniform_step": args.train_step,
-                         "policy_gradient": [{"round": h["round"], "esjd_per_move": round(h["esjd_per_move"], 4),
+                         "objective": args.pg_objective,
+                         "policy_gradient": [{"round": h["round"], "objective_per_move": round(h["objective_per_move"], 4),
+                                              "esjd_per_move": round(h["esjd_per_move"], 4),
                                               "acceptance": round(h["acceptance"], 4)} for h in hist[1:]],
                          "seconds": round(t_train, 2)},
             "judge": {"tuned_uniform_step": judge_step,
@@ -758,7 +765,7 @@ def main():
     if "hardspheres" in legs:
         # the learned-proposal workload: N = 2e4 charged hard spheres, sigma 1,
         # packing 0.2, Bjerrum length 2, digits 6.  One chain, equilibrated
-        # (20 sweeps), trains the MLP (warm start on hs_train uniform moves,
+        # (hs_equil_sweeps uniform sweeps of step 1), trains the MLP (warm start on hs_train uniform moves,
         # then policy-gradient rounds); the judge is the uniform proposal at
         # the step with the most decorrelated energy samples per second; the
         # learned and judge chains then run on from the same state.
@@ -770,11 +777,11 @@ def main():
         gh.mc_set_model(2.0, 1.0)
         hcfg = dict(rf=args.hs_rf, max_mean=0.5, sigma_min=0.02, sigma_max=2.0)
         rng_h = np.random.default_rng(args.seed + 32)
-        PR.run_uniform(gh, args.hs_n, 0.5, 20 * args.hs_n, 1.0, batch, rng_h)
+        PR.run_uniform(gh, args.hs_n, 1.0, args.hs_equil_sweeps * args.hs_n, 1.0, batch, rng_h)  # equilibration
         t0 = time.time()
         w_h, hist_h = PR.train_proposal(gh, args.hs_n, hcfg, warm_step=args.hs_train_step, warm_moves=args.hs_train,
                                         rounds=args.hs_pg_rounds, round_moves=100000, warm_epochs=args.train_epochs,
-                                        seed=args.seed + 33)
+                                        seed=args.seed + 33, objective=args.pg_objective)
         t_train_h = time.time() - t0
         scan = {}
         for s_ in [float(x) for x in args.hs_judge_steps.split(",")]:
@@ -802,7 +809,9 @@ def main():
                    "training": {"warm_start": {k: (round(v, 4) if isinstance(v, float) else v)
                                                for k, v in hist_h[0].items()},
                                 "warm_uniform_step": args.hs_train_step,
-                                "policy_gradient_esjd_per_move": [round(h["esjd_per_move"], 4) for h in hist_h[1:]],
+                                "objective": args.pg_objective,
+                                "policy_gradient_objective_per_move": [round(h["objective_per_move"], 4)
+                                                                       for h in hist_h[1:]],
                                 "seconds": round(t_train_h, 2)},
                    "judge": {"uniform_step": best_s, "scan": {str(k): v for k, v in scan.items()}},
                    "autocorr_learned": lac,
