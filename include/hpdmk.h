@@ -171,7 +171,9 @@ int hpdmk_kernel_times(hpdmk_tree tree, double* ms, int* count);
  * 0 batch scan assembly, 1 batch decisions, 2 update marking, 3 update
  * compaction, 4 particle relocation, 5 single-move finalisation, 6 update
  * contribution lists, 7 update accumulation, 8 update write-back (per work
- * item).  us[9], count[9]. */
+ * item), 9 batch decisions' set-up (masks, known outcomes), 10 the walk over
+ * the batch, 11 the union of the accepted trials' update arrays (part of 2).
+ * us[12], count[12]. */
 int hpdmk_phase_times(hpdmk_tree tree, double* us, int* count);
 
 /* Host-only inspection (no device): the octree the library would build.

@@ -12,6 +12,11 @@ namespace hpdmk_amd {
 
 constexpr int kMaxBatch = 256;     // trials per batched step
 constexpr int kMarkWords = 4096;   // shared-memory bitmap of rewritten arrays: 2 n_pw <= 32 kMarkWords
+// candidate arrays of one trial's update (own out, own in, 26 colleagues' in per
+// endpoint and expansion level), listed ahead of the decisions for batches of at
+// most 64 trials in trees of at most kScanLv expansion levels
+constexpr int kScanLv = 4;
+constexpr int kScanCand = 2 * kScanLv * 64 * 28;  // 16-bit entries, [endpoint, level][trial][28]
 constexpr int kMaxGridD = 53;  // largest 2n+1 of any digits setting
 constexpr int kRowStride = 64;     // stride of one 1-D phase row (2n+1 <= 53)
 constexpr int kAccPerTrial = 7;    // window, difference, res_t, res_o, scanned_t, scanned_o, overlaps_t
@@ -138,12 +143,14 @@ struct McScratch {
     int* plan;             // [0] arrays in dest, [1] accepted moves, [2] level-2 in arrays,
                            // [3] single move: the target leaf has room (marking row)
     unsigned long long* probe;  // optional phase timers (profiling): kProbes sums + counts
+    unsigned short* cand = nullptr;  // batches of <= 64: candidate arrays of each trial's update
+                                     // ((ep nL + li) 64 + trial) 28 + c, 0xffff none
 };
 
 // phase timers of the decision/plan path (wall_clock64 ticks, summed over launches)
 enum Probe : int {
     kPrScanAssemble = 0, kPrScanDecide, kPrPlanMark, kPrPlanCompact, kPrPlanRelocate, kPrTrialFinal,
-    kPrApplyMatch, kPrApplyTiles, kPrApplyRmw,
+    kPrApplyMatch, kPrApplyTiles, kPrApplyRmw, kPrScanPrep, kPrScanWalk, kPrPlanUnite,
     kProbes
 };
 

@@ -1327,6 +1327,42 @@ __device__ void assemble_record(const DevTree<T>& t, const McArgs& a, int j, dou
     a.s.rec[j] = r;
 }
 
+// Batches of at most 64 trials in trees of at most kScanLv expansion levels:
+// k_pair_corr's record blocks list the arrays each trial's update would
+// rewrite (as if accepted) and k_scan unites the lists of the accepted ones.
+template <typename T>
+__device__ __forceinline__ bool scan_fastmark(const DevTree<T>& t, const McArgs& a) {
+    return a.s.cand != nullptr && a.K <= 64 && n_levels_pw(t.max_depth) <= kScanLv &&
+           2 * t.n_pw_nodes <= 4 * kMarkWords;  // a byte mark per array (and 16-bit array ids)
+}
+
+// Candidate arrays of trial j's update into s.cand (entry ((ep nL + li) 64 +
+// j) 28 + c: endpoint ep at level 2 + li, c = 0 own out, 1 own in (not a
+// leaf), 2 + k the k-th colleague's in (non-leaf colleagues), 0xffff where
+// there is none), by the threads th, th + nth, ... of a block.
+template <typename T>
+__device__ void list_candidates(const DevTree<T>& t, const McArgs& a, int j, int th, int nth) {
+    const TrialInfo& ti = a.s.info[j];
+    const int nL = n_levels_pw(t.max_depth);
+    for (int w = th; w < 2 * nL * 28; w += nth) {
+        const int c = w % 28, el = w / 28, ep = el / nL, l = 2 + el % nL;
+        int ix = 0xffff;
+        if (l <= (ep == 0 ? ti.lam_t : ti.lam_o)) {
+            const int node = ep == 0 ? ti.path_t[l] : ti.path_o[l];
+            if (c == 0) {
+                ix = 2 * t.pw_slot[node];
+            } else if (c == 1) {
+                ix = t.is_leaf[node] ? -1 : 2 * t.pw_slot[node] + 1;
+            } else {
+                const int sc = t.col_in_slot[26 * node + c - 2];
+                ix = sc >= 0 ? 2 * sc + 1 : 0xffff;
+            }
+            if (ix < 0) ix = 0xffff;
+        }
+        a.s.cand[(el * 64 + j) * 28 + c] = (unsigned short)ix;
+    }
+}
+
 constexpr int kPairTB = 128;  // threads per pair-correction block (two waves: twice the resident blocks of four)
 
 // Blocks 0 .. K-1 assemble the trial records (first, so that their chains of
@@ -1346,7 +1382,10 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
             if (threadIdx.x == 0) Hs = H;
             __syncthreads();
         }
-        if (threadIdx.x >= 64) return;
+        if (threadIdx.x >= 64) {  // the other waves list the update's candidate arrays
+            if (scan_fastmark(t, a)) list_candidates(t, a, j, threadIdx.x - 64, blockDim.x - 64);
+            return;
+        }
         assemble_record(t, a, j, a.prop.learned ? Hs : 0.0);
         return;
     }
@@ -1488,6 +1527,16 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
     }
 }
 
+// r0 plus the entries of c whose bit is set in mask, in a fixed order (four
+// interleaved partial sums, then joined): the batch walk forms every dE it
+// compares and reports with this one function.
+__device__ __forceinline__ double masked_sum(double r0, const double* c, unsigned long long mask) {
+    double p[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 64; ++i) p[i & 3] += ((mask >> i) & 1ull) ? c[i] : 0.0;
+    return r0 + ((p[0] + p[1]) + (p[2] + p[3]));
+}
+
 // Batch decisions: dE_j = stale_j + sum over the accepted i < j of corr[i][j],
 // taken in order by one wave (lane l owns trials l, l+64, ...: it keeps their
 // running corrected dE and the appends into their target leaves); then the
@@ -1511,6 +1560,12 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
     __shared__ unsigned Fw[64 * 64 / 4];  // pair flags, read as words
     unsigned char* F = reinterpret_cast<unsigned char*>(Fw);
     __shared__ int cons;
+    // update plan from the trials' candidate lists (scan_fastmark)
+    __shared__ __align__(16) unsigned short cl[kScanCand];
+    __shared__ unsigned mbits[kMarkWords];
+    __shared__ int accf[64], wtf[TB / 64], wtb[TB / 64];
+    __shared__ unsigned short pmask[4][4][64];  // (mask kind, quarter of the earlier trials, trial)
+    __shared__ double pabs[4][64];
     if (*t.err) return;
     const int K = a.K;             // stride of the pair tables
     const int base = move_base(a);
@@ -1518,9 +1573,12 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
     const bool hs = HS && a.sigma2 > 0.0;
     const bool dyn = LR && a.move_ctr != nullptr;
     const bool flg = hs || (LR && a.prop.learned);  // pair flags present
+    const int nL = n_levels_pw(t.max_depth), n2 = 2 * t.n_pw_nodes;
+    const bool fm = scan_fastmark(t, a);  // implies K <= 64
     unsigned long long tp = wall_clock64();
-    for (int j = threadIdx.x; j < Ke; j += blockDim.x) {  // records from k_pair_corr
-        const TrialRec r = a.s.rec[j];
+    if (fm)  // the byte marks of the update plan
+        for (int w = threadIdx.x; w < (n2 + 3) / 4; w += blockDim.x) mbits[w] = 0u;
+    auto put_rec = [&](int j, const TrialRec& r) {
         dEs[j] = r.dE;
         thr[j] = r.thr;
         room[j] = r.room;
@@ -1528,15 +1586,32 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
         leafO[j] = r.leaf_o;
         ov0[j] = r.ov;
         pids[j] = r.pid;
-    }
+    };
     const bool in_lds = K <= 64;
+    if (!in_lds)
+        for (int j = threadIdx.x; j < Ke; j += blockDim.x) put_rec(j, a.s.rec[j]);  // records from k_pair_corr
     if (in_lds) {
-        // the corrections and (four bytes per load) the pair flags: all of a
-        // thread's loads in flight together, then the LDS stores
+        // the records (one per thread), the corrections, (four bytes per
+        // load) the pair flags and the candidate arrays: every load in flight
+        // together, then the LDS stores
+        TrialRec rec;
+        if ((int)threadIdx.x < Ke) rec = a.s.rec[threadIdx.x];
         const unsigned* ow = reinterpret_cast<const unsigned*>(a.s.ovl);
         const int nw = flg ? (K * K + 3) / 4 : 0;  // <= 1024; the last word stays inside the kMaxBatch^2 flags
         double v[16];
         unsigned fv[4];
+        // and the batch's candidate arrays (k_pair_corr's record blocks), int4
+        // at a time: the levels in use, all 64 trial rows of each (a trial
+        // past the step's end is never read)
+        constexpr int CL = kScanCand * 2 / 16 / TB;
+        static_assert(kScanCand * 2 == CL * TB * 16, "whole int4 rounds");
+        // unconditional loads (no branch between them): from a buffer of at
+        // least CL TB int4 whether or not the candidates are listed
+        const int n4 = fm ? 2 * nL * 64 * 28 * 2 / 16 : 0;
+        const int4* csrc = fm ? reinterpret_cast<const int4*>(a.s.cand) : reinterpret_cast<const int4*>(a.s.corr);
+        int4 cv[CL];
+#pragma unroll
+        for (int r = 0; r < CL; ++r) cv[r] = csrc[threadIdx.x + r * TB];
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int q = threadIdx.x + r * TB;
@@ -1547,6 +1622,8 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
             const int q = threadIdx.x + r * TB;
             if (q < nw) fv[r] = ow[q];
         }
+        asm volatile("" ::: "memory");  // the loads above issue before any store below
+        if ((int)threadIdx.x < Ke) put_rec(threadIdx.x, rec);
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int q = threadIdx.x + r * TB;
@@ -1557,79 +1634,116 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
             const int q = threadIdx.x + r * TB;
             if (q < nw) Fw[q] = fv[r];
         }
+        int4* cdst = reinterpret_cast<int4*>(cl);
+#pragma unroll
+        for (int r = 0; r < CL; ++r)
+            if ((int)threadIdx.x + r * TB < n4) cdst[threadIdx.x + r * TB] = cv[r];
     }
     if (threadIdx.x == 0) cons = Ke > 0 ? Ke : 0;
     __syncthreads();
     if (threadIdx.x == 0) probe_add(a.s, kPrScanAssemble, tp);
+    if (in_lds) {
+        // for lane j of the walk, over the earlier trials i < j (thread (j, q)
+        // takes i = 16 q .. 16 q + 15): i's acceptance removes an overlap
+        // (flag 1) or adds one (flag 2), makes j's proposal stale (flag 4, or
+        // the same particle), moves a particle into j's target leaf; and the
+        // sum of the corrections' magnitudes
+        const int j = threadIdx.x & 63, q = threadIdx.x >> 6;
+        const bool in = j < Ke;
+        const int lt_j = in ? leafT[j] : -1, pid_j = in ? pids[j] : -1;
+        unsigned r_rm = 0u, r_add = 0u, r_st = 0u, r_lf = 0u;
+        double sa = 0.0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int i = 16 * q + u;  // i K + j < 64 * 64: every read is in bounds, used when i < j < Ke
+            const bool e = in && i < j;
+            const int fc = flg ? (int)F[i * K + j] : 0;
+            const int lti = leafT[i], loi = leafO[i], pdi = pids[i];
+            const double cv = C[i * K + j];
+            const unsigned b = e ? 1u << u : 0u;
+            r_rm |= (fc & 1) ? b : 0u;
+            r_add |= (fc & 2) ? b : 0u;
+            r_st |= ((fc & 4) || pdi == pid_j) ? b : 0u;
+            r_lf |= (loi != lti && lti == lt_j) ? b : 0u;
+            sa += e ? fabs(cv) : 0.0;
+        }
+        pmask[0][q][j] = (unsigned short)r_rm;
+        pmask[1][q][j] = (unsigned short)r_add;
+        pmask[2][q][j] = (unsigned short)r_st;
+        pmask[3][q][j] = (unsigned short)r_lf;
+        pabs[q][j] = sa;
+        __syncthreads();
+    }
     if (threadIdx.x < 64 && in_lds) {
-        // Lane j owns trial j and holds its whole column of pair corrections
-        // and flags in registers; the walk is unrolled over the (at most 64)
-        // trials, so each step is register arithmetic plus readlanes of the
-        // deciding lane.
+        // Lane j owns trial j and holds its column of pair corrections in
+        // registers.  Trial j's dE given the accepted set S is its stale dE
+        // plus the corrections of the accepted trials before it (masked_sum:
+        // one fixed order); the counts an acceptance changes come from the
+        // masks (hard-sphere overlaps removed or added, movers into the target
+        // leaf, stale proposals).  A trial whose outcome no set of earlier
+        // acceptances can change (dE farther from its threshold than the sum
+        // of its corrections' magnitudes, no flag, room for every earlier
+        // mover) is known before the walk, which visits the others in move
+        // order: the same outcomes as deciding the trials one by one.
         const int j = threadIdx.x;
         const bool in = j < Ke;
         double col[64];
-        unsigned fl[16];
         // unconditional loads (i K + j < 64 * 64 for K <= 64), so that they
         // issue back to back; entries outside the column are zeroed after
 #pragma unroll
         for (int i = 0; i < 64; ++i) col[i] = C[i * K + j];
 #pragma unroll
         for (int i = 0; i < 64; ++i) col[i] = (in && i < j) ? col[i] : 0.0;
+        unsigned long long m_rm = 0ull, m_add = 0ull, m_st = 0ull, m_lf = 0ull;
+        double amax = 0.0;
 #pragma unroll
-        for (int w = 0; w < 16; ++w) {
-            unsigned v = 0u;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) v |= (unsigned)F[(4 * w + b) * K + j] << (8 * b);
-            fl[w] = v;
+        for (int q = 0; q < 4; ++q) {
+            m_rm |= (unsigned long long)pmask[0][q][j] << (16 * q);
+            m_add |= (unsigned long long)pmask[1][q][j] << (16 * q);
+            m_st |= (unsigned long long)pmask[2][q][j] << (16 * q);
+            m_lf |= (unsigned long long)pmask[3][q][j] << (16 * q);
+            amax += pabs[q][j];
         }
-#pragma unroll
-        for (int w = 0; w < 16; ++w) {
-            unsigned keep = 0u;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) keep |= (flg && in && 4 * w + b < j) ? (0xffu << (8 * b)) : 0u;
-            fl[w] &= keep;
-        }
-        double run = in ? dEs[j] : 0.0;
-        const double thr_j = in ? thr[j] : 0.0;
-        int ovr = in ? ov0[j] : 0, dcr = 0, stl = 0;
-        const int room_j = in ? room[j] : 0, lt_j = in ? leafT[j] : -1, lo_j = in ? leafO[j] : -1;
-        const int pid_j = in ? pids[j] : -1;
-        const double lB = a.lB;
-        bool done = false;  // uniform
-        double my_dE = 0.0;
-        int my_ok = 0;
-#pragma unroll
-        for (int i = 0; i < 64; ++i) {
-            if (i >= Ke) done = true;
-            if (!done && dyn && __builtin_amdgcn_readlane(stl, i)) {  // a stale proposal ends the step
-                if (j == 0) cons = i;
-                done = true;
+        const double run0 = in ? dEs[j] : 0.0, thr_j = in ? thr[j] : 0.0, lB = a.lB;
+        const int ov_j = in ? ov0[j] : 0, room_j = in ? room[j] : 0;
+        // every sum of a subset of the corrections (each addition rounded)
+        // stays within dl of the stale dE
+        const double dl = amax + 1e-13 * (fabs(run0) + amax);
+        const double hi = lB * (run0 + dl), lo = lB * (run0 - dl);
+        const bool e_acc = hi + 1e-15 * fabs(hi) < thr_j, e_rej = lo - 1e-15 * fabs(lo) >= thr_j;
+        const bool fixed_counts = (!hs || (m_add | m_rm) == 0ull) && __popcll(m_lf) < room_j && (!dyn || m_st == 0ull);
+        const bool known = fixed_counts && (ov_j > 0 || e_acc || e_rej);
+        const bool known_ok = ov_j == 0 && e_acc;
+        unsigned long long tw = wall_clock64();
+        if (j == 0) probe_add(a.s, kPrScanPrep, tp);
+        unsigned long long U = __ballot(in && !known);        // uniform: decided in the walk
+        unsigned long long S = __ballot(in && known && known_ok);  // uniform: accepted (known ones from the start)
+        int nc = Ke > 0 ? Ke : 0;  // uniform: trials decided (a stale proposal ends the step)
+        int my_ok = known_ok ? 1 : 0;
+        while (U) {
+            const int i = (int)__builtin_ctzll(U);  // uniform
+            U &= U - 1ull;
+            const unsigned long long Sb = S & ((1ull << i) - 1ull);
+            if (dyn && __builtin_amdgcn_readlane((int)((m_st & Sb) != 0ull), i)) {
+                nc = i;
+                break;
             }
-            if (done) continue;
-            const double dE = lB * run;
+            const int ovr = ov_j + (hs ? __popcll(m_add & Sb) - __popcll(m_rm & Sb) : 0);
+            const int dcr = __popcll(m_lf & Sb);
+            const double dE = lB * masked_sum(run0, col, Sb);
             const int okl = (ovr == 0 && dE < thr_j) ? (dcr < room_j ? 1 : 2) : 0;  // 2: the target leaf is full
-            if (j == i) {  // the owner keeps its outcome in registers until the walk ends
-                my_dE = ovr > 0 ? (double)INFINITY : dE;  // a hard-sphere overlap is an infinite energy
-                my_ok = okl;
-            }
-            if (__builtin_amdgcn_readlane(okl, i) == 1) {  // uniform
-                const int lti = __builtin_amdgcn_readlane(lt_j, i);
-                const int loi = __builtin_amdgcn_readlane(lo_j, i);
-                const int pidi = __builtin_amdgcn_readlane(pid_j, i);
-                if (j > i && in) {
-                    const int fc = (int)((fl[i >> 2] >> (8 * (i & 3))) & 0xffu);
-                    run += col[i];
-                    dcr += (loi != lti) && (lti == lt_j);
-                    if (hs) ovr += ((fc >> 1) & 1) - (fc & 1);
-                    stl |= (fc >> 2) & 1;
-                    if (dyn && pid_j == pidi) stl = 1;  // the same particle again
-                }
-            }
+            if (j == i) my_ok = okl;
+            if (__builtin_amdgcn_readlane(okl, i) == 1) S |= 1ull << i;
         }
+        if (nc < 64) S &= (1ull << nc) - 1ull;
         if (in) {
-            dEf[j] = my_dE;
+            const int ovr = ov_j + (hs ? __popcll(m_add & S) - __popcll(m_rm & S) : 0);
+            dEf[j] = ovr > 0 ? (double)INFINITY : lB * masked_sum(run0, col, S);  // an overlap is an infinite energy
             accs[j] = my_ok;
+        }
+        if (j == 0) {
+            cons = nc;
+            probe_add(a.s, kPrScanWalk, tw);
         }
     } else if (threadIdx.x < 64 && Ke <= 64) {
         // Lane j owns trial j and keeps what the walk reads of it in
@@ -1743,9 +1857,75 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
             if (a.ml.accept_out) a.ml.accept_out[m] = ok;
         }
         a.s.accept[j] = ok;  // undecided and unused trials apply nothing
+        if (fm) accf[j] = ok;
     }
     __syncthreads();
-    plan_update(t, a, K, !apply_relocates(a));
+    if (fm) {
+        // the plan's arrays: every accepted trial's candidates marked in a
+        // byte per array (2: a level-2 in array, 1: any other; an array
+        // always gets the same mark), then one pass over the marks lists them
+        // in s.dest, level-2 in arrays from the front and the others from the
+        // back
+        unsigned char* mk = reinterpret_cast<unsigned char*>(mbits);
+        for (int w = threadIdx.x; w < 2 * nc * nL; w += blockDim.x) {
+            const int m = w / (2 * nL), el = w % (2 * nL);
+            if (!accf[m]) continue;
+            const unsigned short* lst = cl + (el * 64 + m) * 28;
+            const unsigned char lv2 = el % nL == 0 ? 2 : 1;
+            int ix[28];
+#pragma unroll
+            for (int c = 0; c < 28; ++c) ix[c] = lst[c];
+#pragma unroll
+            for (int c = 0; c < 28; ++c)
+                if (ix[c] != 0xffff) mk[ix[c]] = c > 0 ? lv2 : (unsigned char)1;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) probe_add(a.s, kPrPlanUnite, tp);
+        const int per = (n2 + TB - 1) / TB, e0 = threadIdx.x * per, e1 = min(e0 + per, n2);
+        int nf = 0, nb = 0;
+        for (int e = e0; e < e1; ++e) {
+            nf += mk[e] == 2;
+            nb += mk[e] == 1;
+        }
+        int of = nf, ob = nb;  // inclusive scans over the block
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int uf = __shfl_up(of, o), ub = __shfl_up(ob, o);
+            if (lane >= o) {
+                of += uf;
+                ob += ub;
+            }
+        }
+        if (lane == 63) {
+            wtf[wv] = of;
+            wtb[wv] = ob;
+        }
+        __syncthreads();
+        int bf = 0, bb = 0, tf = 0, tb = 0;
+#pragma unroll
+        for (int w = 0; w < TB / 64; ++w) {
+            bf += w < wv ? wtf[w] : 0;
+            bb += w < wv ? wtb[w] : 0;
+            tf += wtf[w];
+            tb += wtb[w];
+        }
+        of += bf - nf;  // exclusive offsets
+        ob += bb - nb;
+        for (int e = e0; e < e1; ++e) {
+            if (mk[e] == 2) a.s.dest[of++] = e;
+            else if (mk[e] == 1) a.s.dest[n2 - 1 - ob++] = e;
+        }
+        if (threadIdx.x == 0) {
+            a.s.plan[0] = tf + tb;
+            a.s.plan[2] = tf;
+            probe_add(a.s, kPrPlanMark, tp);
+        }
+        __syncthreads();
+        plan_update<T, true>(t, a, K, !apply_relocates(a));
+    } else {
+        plan_update(t, a, K, !apply_relocates(a));
+    }
     if (dyn && threadIdx.x == 0) *a.move_ctr = base + nc;
 }
 

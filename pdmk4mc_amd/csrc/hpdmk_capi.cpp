@@ -295,6 +295,7 @@ Handle* create_handle(HPDMKParams params, int precision, int n_src, const double
     h->sc.nm = dalloc<NextMove>(2, o);
     h->sc.st = dalloc<double2>(4, o);
     h->sc.plan = dalloc<int>(4, o);
+    h->sc.cand = dalloc<unsigned short>(kScanCand, o);
     h->probe = dalloc<unsigned long long>(2 * kProbes, o);
     HIPX(hipMemset(h->probe, 0, sizeof(unsigned long long) * 2 * kProbes));
     HIPX(hipMemset(h->sc.plan, 0, 4 * sizeof(int)));

@@ -431,11 +431,11 @@ class HPDMKPtTree:
         return dict(zip(["trial", "update", "pair_corr", "scan", "setup", "propose"], zip(ms.tolist(), cnt.tolist())))
 
     def phase_times(self):
-        us = np.zeros(9)
-        cnt = np.zeros(9, dtype=np.int32)
+        us = np.zeros(12)
+        cnt = np.zeros(12, dtype=np.int32)
         _check(load_library().hpdmk_phase_times(self._h, us.ctypes.data_as(_dp), cnt.ctypes.data_as(_ip)))
         names = ["scan_assemble", "scan_decide", "plan_mark", "plan_compact", "plan_relocate", "trial_final",
-                 "apply_lists", "apply_accumulate", "apply_writeback"]
+                 "apply_lists", "apply_accumulate", "apply_writeback", "scan_prep", "scan_walk", "plan_unite"]
         return {n: (round(float(u), 2), int(c)) for n, u, c in zip(names, us, cnt)}
 
     def clear_error(self):
