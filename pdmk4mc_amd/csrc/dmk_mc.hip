@@ -302,19 +302,35 @@ __global__ void __launch_bounds__(TB) k_setup_rep(const DevTree<T>* __restrict__
 
 // ------------------------------------------------------------------ propose
 // Features of each trial's particle at its origin (particle removed) and, for
-// the learned proposal, its displacement delta = mu + sigma z and log q.
-// One block per trial.
+// the learned proposal, its displacement delta = mu + sigma z and log q;
+// into ti (shared, by thread 0) the trial's particle, origin and
+// displacement, and (by thread 64, while the features are summed) the
+// origin's root-to-leaf path.  One block per trial; returns false (uniformly)
+// when the block has no trial.
 template <typename T>
-__device__ __forceinline__ void propose_body(const DevTree<T>& t, const McArgs& a) {
-    __shared__ double redn[(TB / 64) * kFeatSums];
+__device__ __forceinline__ bool propose_body(const DevTree<T>& t, const McArgs& a, TrialInfo& ti) {
     __shared__ double phi[kFeat], mu[3], sig[3];
-    if (*t.err) return;
+    // the learned network's packed weights and biases, staged while the features are summed
+    __shared__ __align__(16) unsigned short wps[kPackSize];
+    __shared__ float bsh[kBiasPack];
+    if (*t.err) return false;
     const Proposal& P = a.prop;
     const int tr = blockIdx.x, m = move_base(a) + tr;
-    if (tr >= keff(a)) return;
+    if (tr >= keff(a)) return false;
     const int i = a.ml.idx[m];
     const int own = t.slot_of[i], leaf = t.leaf_of[i];
     const double ox = t.px[own], oy = t.py[own], oz = t.pz[own], q = t.pq[own];
+    if (threadIdx.x == 0) {
+        ti.i = i;
+        ti.q = q;
+        ti.o[0] = ox;
+        ti.o[1] = oy;
+        ti.o[2] = oz;
+        if (!P.learned)
+            for (int c = 0; c < 3; ++c) ti.dv[c] = a.ml.d[3 * m + c];
+    } else if (threadIdx.x == 64) {
+        ti.lam_o = locate(t, ox, oy, oz, ti.path_o);
+    }
     const double rf2 = P.rf * P.rf;
     double fs[kFeatSums] = {0, 0, 0, 0, 0, 0, 0};
     const int off = t.near_off[leaf], cnt = t.near_off[leaf + 1] - off;
@@ -382,33 +398,68 @@ __device__ __forceinline__ void propose_body(const DevTree<T>& t, const McArgs& 
         }
         __syncthreads();  // the lists are consumed before the next pass
     }
-    block_sum_n<kFeatSums>(fs, redn);
-    if (threadIdx.x == 0) {
-        features_from_sums(q, fs, phi);
-        if (P.feat)
-            for (int f = 0; f < kFeat; ++f) P.feat[(size_t)kFeat * m + f] = phi[f];
-    }
-    if (!P.learned) return;  // uniform across the grid
+    // staged now: the weights' loads land while the features are reduced
+    // (issued earlier, they would delay the feature loads, which wait in order)
+    if (P.learned) mlp_stage(P, wps, bsh);
+    // the feature sums: every thread's partial sums through LDS, kFeatSums x 8
+    // threads add 32 each, kFeatSums threads the eight; then one thread per feature
+    __shared__ double fpart[kFeatSums][TB], fp8[kFeatSums][8], fsum[kFeatSums];
+#pragma unroll
+    for (int c = 0; c < kFeatSums; ++c) fpart[c][threadIdx.x] = fs[c];
     __syncthreads();
-    mlp_forward_mfma(P, phi, mu, sig);
+    if (threadIdx.x < kFeatSums * 8) {
+        const int c = threadIdx.x / 8, k = threadIdx.x % 8;
+        double v = 0.0;
+        for (int u = 0; u < TB / 8; ++u) v += fpart[c][k * (TB / 8) + u];
+        fp8[c][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kFeatSums) {
+        double v = 0.0;
+        for (int k = 0; k < 8; ++k) v += fp8[threadIdx.x][k];
+        fsum[threadIdx.x] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kFeat) {
+        phi[threadIdx.x] = feature_from_sums(threadIdx.x, q, fsum);
+        if (P.feat) P.feat[(size_t)kFeat * m + threadIdx.x] = phi[threadIdx.x];
+    }
+    if (!P.learned) return true;  // uniform across the grid
+    __syncthreads();
+    mlp_forward_mfma(P, phi, mu, sig, wps, bsh);
     __syncthreads();
     if (threadIdx.x == 0) {
         double dl[3];
         for (int c = 0; c < 3; ++c) {
             dl[c] = mu[c] + sig[c] * P.z[3 * m + c];
             P.dprop[3 * m + c] = dl[c];
+            ti.dv[c] = dl[c];
         }
         P.logq[m] = gauss_logq(dl, mu, sig);
     }
+    return true;
 }
 
 // the proposal (or the features) and the set-up of each trial in one launch:
-// the set-up's thread 0 reads the displacement the proposal's thread 0 wrote
+// the set-up completes the trial state the proposal began (the target and its
+// path), then forms the phase rows (src/energy_shift.cpp:18-51)
 template <typename T>
 __global__ void __launch_bounds__(TB) k_propose_setup(DevTree<T> t, McArgs a) {
-    propose_body(t, a);
+    __shared__ TrialInfo ti;
+    if (!propose_body(t, a, ti)) return;
     __syncthreads();
-    setup_body(t, a);
+    const int tr = blockIdx.x, m = move_base(a) + tr, sl = trial_slot(a, tr, m);
+    if (threadIdx.x == 0) {
+        for (int c = 0; c < 3; ++c) ti.t[c] = wrap(ti.o[c] + ti.dv[c], t.L);
+        ti.lam_t = locate(t, ti.t[0], ti.t[1], ti.t[2], ti.path_t);
+        a.s.info[sl] = ti;
+        if (a.ml.stats) {  // filled by k_pair_corr (pair terms) and the update planner (batch arrays)
+            a.ml.stats[kStatsPerMove * m + 6] = 0;
+            a.ml.stats[kStatsPerMove * m + 7] = 0;
+        }
+    }
+    __syncthreads();
+    form_rows(t, ti, tabs_of<T>(a.s, table_stride(t.max_depth), sl));
 }
 
 // ------------------------------------------------------------------ update plan
@@ -2016,7 +2067,7 @@ __global__ void k_advance(int* move_ctr, int n, int inc) {
 // split over kApplySplit items, whose sums are added atomically instead; other
 // arrays are never split (atomic adds over whole arrays cost more than the
 // longer contribution loop).
-constexpr int kApplyTile = 8;
+constexpr int kApplyTile = 16;
 constexpr int kApplyLv = 4;     // levels of the per-block path table (deeper levels read TrialInfo)
 constexpr int kApplySplit = 4;  // items sharing the root
 constexpr int kApplyHeavy = 16; // contributions above which the root is split
@@ -2028,14 +2079,18 @@ template <typename T>
 __host__ __device__ constexpr int apply_xmax(int d) { return (d + apply_lpr<T>() - 1) / apply_lpr<T>(); }
 // the unguarded reads of a lane end at most kApplyPad elements past the tiles
 constexpr int kApplyPad = 64;
-// dynamic shared memory of k_apply: the item's sums and two tiles of rows
-// (double buffer), at row stride dmax = max(d_w, d), kApplyPad elements of
-// padding, then the path table of the batch's K moves
+// dynamic shared memory of k_apply: two tiles of rows (double buffer) at row
+// stride dmax = max(d_w, d) and kApplyPad elements of padding, sharing their
+// space with the item's sums (written after the last tile is consumed), then
+// the path table of the batch's K moves
+template <typename T>
+__host__ __device__ inline size_t apply_rows_elems(int dmax) {
+    const size_t sums = (size_t)(TB / apply_lpr<T>()) * dmax, tiles = 2 * (size_t)kApplyTile * 3 * dmax + kApplyPad;
+    return sums > tiles ? sums : tiles;
+}
 template <typename T>
 __host__ __device__ inline size_t apply_shmem(int dmax, int K) {
-    return sizeof(Cplx<T>) *
-               ((size_t)(TB / apply_lpr<T>()) * dmax + 2 * (size_t)kApplyTile * 3 * dmax + kApplyPad) +
-           sizeof(int) * 2 * (size_t)K * kApplyLv;
+    return sizeof(Cplx<T>) * apply_rows_elems<T>(dmax) + sizeof(int) * 2 * (size_t)K * kApplyLv;
 }
 
 template <typename T, int XMAX, bool RB = false>
@@ -2064,8 +2119,8 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
     constexpr int RPI = TB / LPR;        // grid rows per work item
     const int dmax = t.dw > t.dd ? t.dw : t.dd;
     Cplx<T>* S = reinterpret_cast<Cplx<T>*>(apply_dyn);  // the item's sums, row-major like the array
-    Cplx<T>* rows2 = S + RPI * dmax;                      // two tiles of rows at stride dmax
-    int* pnode = reinterpret_cast<int*>(rows2 + 2 * kApplyTile * 3 * dmax + kApplyPad);  // (move, endpoint, level-2) -> box
+    Cplx<T>* rows2 = S;                                   // two tiles of rows at stride dmax (before the sums)
+    int* pnode = reinterpret_cast<int*>(S + apply_rows_elems<T>(dmax));  // (move, endpoint, level-2) -> box
     const int rows_w = t.dw * (t.n_window + 1), rows_d = t.dd * (t.n_diff + 1);
     const int nWb = (rows_w + RPI - 1) / RPI, nDb = (rows_d + RPI - 1) / RPI;
     // every accepted endpoint rewrites the root: its contributions are split
@@ -2306,6 +2361,7 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
             buf ^= 1;
         }
         if (threadIdx.x == 0) probe_add(a.s, kPrApplyTiles, tp);
+        __syncthreads();  // the last tile is consumed: its space takes the sums
         // sums to LDS in the array's order, then a coalesced write
         if constexpr (MF) {
             // D layout of v_mfma_f32_16x16x4f32: lane holds rows 4 kq + q of column li16
@@ -2344,8 +2400,14 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
     }
 }
 
+// waves per SIMD the compiler keeps registers for: three for the FP32 grids
+// of digits <= 6 (three blocks per CU, as many as the LDS holds), the
+// compiler's choice otherwise (wider grids and FP64 keep more per thread)
+template <typename T, int XMAX>
+constexpr int apply_min_waves() { return std::is_same<T, float>::value && XMAX <= 13 ? 3 : 1; }
+
 template <typename T, int XMAX, bool RB>
-__global__ void __launch_bounds__(TB) k_apply(DevTree<T> t, McArgs a) {
+__global__ void __launch_bounds__(TB, (apply_min_waves<T, XMAX>())) k_apply(DevTree<T> t, McArgs a) {
     apply_body<T, XMAX, RB>(t, a);
 }
 

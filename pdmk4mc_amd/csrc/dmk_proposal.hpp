@@ -46,6 +46,7 @@ constexpr double kFeatScale[3] = {1.0, 8.0, 2.0};
 constexpr int kPackL1 = 1 * 4 * 256, kPackL2 = 4 * 4 * 256, kPackL3 = 4 * 4 * 256, kPackL4 = 4 * 1 * 256;
 constexpr int kPackOff1 = 0, kPackOff2 = kPackOff1 + kPackL1, kPackOff3 = kPackOff2 + kPackL2,
               kPackOff4 = kPackOff3 + kPackL3, kPackSize = kPackOff4 + kPackL4;
+static_assert(kPackSize % 8 == 0, "the packed weights copy as int4");
 
 struct Proposal {
     int learned;       // 0: displacements as loaded; 1: drawn from the MLP (the list holds z)
@@ -83,6 +84,13 @@ __host__ __device__ inline void features_from_sums(double q, const double* s, do
     for (int a = 0; a < 3; ++a) phi[1 + a] = tanh(q * s[a] / kFeatScale[0]);
     phi[4] = tanh(s[3] / kFeatScale[1]);
     for (int a = 0; a < 3; ++a) phi[5 + a] = tanh(s[4 + a] / kFeatScale[2]);
+}
+// feature f of features_from_sums alone (the same expression)
+__host__ __device__ inline double feature_from_sums(int f, double q, const double* s) {
+    if (f == 0) return q;
+    if (f < 4) return tanh(q * s[f - 1] / kFeatScale[0]);
+    if (f == 4) return tanh(s[3] / kFeatScale[1]);
+    return tanh(s[f - 1] / kFeatScale[2]);
 }
 
 // log N(delta; mu, diag sigma^2)
@@ -164,11 +172,31 @@ __device__ inline void mlp_hidden(const mlp_f4* acc, const float* bias, mlp_s4* 
     }
 }
 
+// the biases b1 b2 b3 b4 one after the other (a block's LDS copy)
+constexpr int kBiasPack = 3 * kHid + kOut;
+
+// A block's LDS copy of the packed BF16 weights and the biases, loaded by
+// every thread of the block (no barrier: the caller syncs before the MLP).
+__device__ inline void mlp_stage(const Proposal& p, unsigned short* wp_s, float* b_s) {
+    const int4* src = reinterpret_cast<const int4*>(p.wp);
+    int4* dst = reinterpret_cast<int4*>(wp_s);
+    for (int k = threadIdx.x; k < kPackSize / 8; k += blockDim.x) dst[k] = src[k];
+    for (int k = threadIdx.x; k < kBiasPack; k += blockDim.x)
+        b_s[k] = k < 3 * kHid ? p.w[kB1 + (k / kHid) * (kHid * kHid + kHid) + k % kHid] : p.w[kB4 + k - 3 * kHid];
+}
+
 // BF16 forward pass by one wave (wave 0 of the block; the others idle).
 // phi (kFeat doubles, shared) in; mu/sig (3 each, shared) out, formed in
 // FP64 from the FP32 outputs.  The caller syncs the block around the call.
-__device__ inline void mlp_forward_mfma(const Proposal& p, const double* phi, double* mu, double* sig) {
+// wp_s / b_s: the block's LDS copy of the weights (mlp_stage), else p's.
+__device__ inline void mlp_forward_mfma(const Proposal& p, const double* phi, double* mu, double* sig,
+                                        const unsigned short* wp_s = nullptr, const float* b_s = nullptr) {
     if (threadIdx.x >= 64) return;
+    const unsigned short* wp = wp_s ? wp_s : p.wp;
+    const float* b1 = b_s ? b_s : p.w + kB1;
+    const float* b2 = b_s ? b_s + kHid : p.w + kB2;
+    const float* b3 = b_s ? b_s + 2 * kHid : p.w + kB3;
+    const float* b4 = b_s ? b_s + 3 * kHid : p.w + kB4;
     const int l = threadIdx.x, g = l >> 4;
     mlp_s4 a[4];
     {  // features: K = 8, one k-step (lanes with g < 2 carry values)
@@ -178,18 +206,18 @@ __device__ inline void mlp_forward_mfma(const Proposal& p, const double* phi, do
         a[0] = (l & 15) == 0 && g < 2 ? v : mlp_s4{0, 0, 0, 0};
     }
     mlp_f4 acc[4];
-    mlp_layer<1, 4>(a, p.wp + kPackOff1, acc);
-    mlp_hidden(acc, p.w + kB1, a);
-    mlp_layer<4, 4>(a, p.wp + kPackOff2, acc);
-    mlp_hidden(acc, p.w + kB2, a);
-    mlp_layer<4, 4>(a, p.wp + kPackOff3, acc);
-    mlp_hidden(acc, p.w + kB3, a);
-    mlp_layer<4, 1>(a, p.wp + kPackOff4, acc);
+    mlp_layer<1, 4>(a, wp + kPackOff1, acc);
+    mlp_hidden(acc, b1, a);
+    mlp_layer<4, 4>(a, wp + kPackOff2, acc);
+    mlp_hidden(acc, b2, a);
+    mlp_layer<4, 4>(a, wp + kPackOff3, acc);
+    mlp_hidden(acc, b3, a);
+    mlp_layer<4, 1>(a, wp + kPackOff4, acc);
     if (l < 3) {
-        const double om = (double)(acc[0][0] + p.w[kB4 + l]);
+        const double om = (double)(acc[0][0] + b4[l]);
         mu[l] = p.max_mean * tanh(om);
     } else if (l < 6) {
-        const double os = (double)(acc[0][0] + p.w[kB4 + l]);
+        const double os = (double)(acc[0][0] + b4[l]);
         sig[l - 3] = p.smin + (p.smax - p.smin) / (1.0 + exp(-os));
     }
 }
