@@ -75,6 +75,10 @@ struct DevTree {
     Cplx<T>* inc;
     const T* Wmat;
     const T* Dmat;
+    // the difference tables folded over k_x -> -k_x and k_y -> -k_y (they depend
+    // on |k| only): per level (n+1)^3 entries, x' + (n+1) (y' + (n+1) z) holding
+    // Dmat at x = n + x', y = n + y' (pair corrections)
+    const T* Dfold;
     // ---- sticky error word (0 = ok); set by kernels, read by the host
     int* err;
 };

@@ -1483,7 +1483,8 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
     // taken group by group (three barriers per pass).
     __shared__ int shf[8 * 3];  // (slot of the pass, axis): image shift, 2 = off
     int ndots = 0;              // level terms formed (counted for the profiled runs)
-    const int dd = t.dd, nzd = t.n_diff + 1, perd = 2 * dd + nzd;
+    // over the folded tables (Dfold): x and y rows folded, A'[x'] = A[n + x'] + A[n - x']
+    const int nd = t.n_diff, n1 = nd + 1, nzd = nd + 1, perd = 2 * n1 + nzd;
     for (int g0 = 0; g0 < 2 * nL; g0 += 4) {
         const int ng = min(4, 2 * nL - g0);
         __syncthreads();  // the previous pass's (or the window's) LDS is consumed
@@ -1505,32 +1506,59 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
         for (int q = 0; q < 2 * ng; ++q)
             if (shf[3 * q] != 2 && shf[3 * q + 1] != 2 && shf[3 * q + 2] != 2) on |= 1u << q;
         if (!on) continue;
-        // factor rows u conj(v) of the on slots: u = j's row, v = i's (shifted) row
-        for (int e = threadIdx.x; e < 2 * ng * perd; e += blockDim.x) {
-            const int q = e / perd, x = e % perd;
-            if (!((on >> q) & 1u)) continue;
-            const int gg = g0 + q / 2, w = q & 1, ep = gg / nL, li = gg % nL;
-            const int ax = x < dd ? 0 : x < 2 * dd ? 1 : 2, xi = x - ax * dd;
-            const Cplx<T> u = tj[level_row(nL, ep, li, ax, 0) + xi];
-            const Cplx<T> v = ta[level_row(nL, w, li, ax, shf[3 * q + ax]) + xi];
-            const Cplx<T> r = {u.x * v.x + u.y * v.y, u.y * v.x - u.x * v.y};
-            Cplx<T>* dst = ax == 0 ? A : ax == 1 ? B : C;
-            dst[q * kRowStride + xi] = r;
+        // factor rows u conj(v) of the on slots (u = j's row, v = i's shifted
+        // row), the x and y rows folded at the grid centre n: a thread's
+        // entries FR at a time, all their loads in flight together (indices
+        // clamped into the rows; the off slots and the unfolded z rows add
+        // nothing)
+        constexpr int FR = 4;
+        const int nfe = 2 * ng * perd;
+        for (int e0 = threadIdx.x; e0 < nfe; e0 += FR * blockDim.x) {
+            Cplx<T> u1[FR], v1[FR], u2[FR], v2[FR];
+#pragma unroll
+            for (int k = 0; k < FR; ++k) {
+                const int e = min(e0 + k * (int)blockDim.x, nfe - 1), q = e / perd, x = e % perd;
+                const int gg = g0 + q / 2, w = q & 1, ep = gg / nL, li = gg % nL;
+                const int ax = x < n1 ? 0 : x < 2 * n1 ? 1 : 2, xi = x - ax * n1;
+                const int sh = shf[3 * q + ax];
+                const Cplx<T>* ru = tj + level_row(nL, ep, li, ax, 0);
+                const Cplx<T>* rv = ta + level_row(nL, w, li, ax, sh == 2 ? 0 : sh);
+                const int x1 = ax == 2 ? xi : nd + xi, x2 = ax == 2 ? xi : nd - xi;
+                u1[k] = ru[x1];
+                v1[k] = rv[x1];
+                u2[k] = ru[x2];
+                v2[k] = rv[x2];
+            }
+#pragma unroll
+            for (int k = 0; k < FR; ++k) {
+                const int e = e0 + k * (int)blockDim.x;
+                if (e >= nfe) continue;
+                const int q = e / perd, x = e % perd;
+                if (!((on >> q) & 1u)) continue;
+                const int ax = x < n1 ? 0 : x < 2 * n1 ? 1 : 2, xi = x - ax * n1;
+                Cplx<T> r = {u1[k].x * v1[k].x + u1[k].y * v1[k].y, u1[k].y * v1[k].x - u1[k].x * v1[k].y};
+                if (ax < 2 && xi > 0) {  // the mirrored entry of an x or y row
+                    r.x += u2[k].x * v2[k].x + u2[k].y * v2[k].y;
+                    r.y += u2[k].y * v2[k].x - u2[k].x * v2[k].y;
+                }
+                Cplx<T>* dst = ax == 0 ? A : ax == 1 ? B : C;
+                dst[q * kRowStride + xi] = r;
+            }
         }
         __syncthreads();
         for (int gq = 0; gq < ng; ++gq) {
             const unsigned m2 = (on >> (2 * gq)) & 3u;
             if (!m2) continue;  // uniform
             const int gg = g0 + gq, ep = gg / nL, li = gg % nL;
-            const T* Dl = t.Dmat + (size_t)li * t.Md;
+            const T* Dl = t.Dfold + (size_t)li * n1 * n1 * nzd;
             const int o = (2 * gq + (m2 == 2u ? 1 : 0)) * kRowStride;  // first on slot
             double lv;
             if (m2 == 3u) {
                 const double sg[2] = {1.0, -1.0};
-                lv = dot_rows<T, 2>(Dl, dd, nzd, A + o, B + o, C + o, sg);
+                lv = dot_rows<T, 2>(Dl, n1, nzd, A + o, B + o, C + o, sg);
             } else {
                 const double sg[1] = {m2 == 1u ? 1.0 : -1.0};
-                lv = dot_rows<T, 1>(Dl, dd, nzd, A + o, B + o, C + o, sg);
+                lv = dot_rows<T, 1>(Dl, n1, nzd, A + o, B + o, C + o, sg);
             }
             ndots += __popc(m2);
             part += lv * 2.0 * t.cD[2 + li] * (ep == 0 ? 1.0 : -1.0);

@@ -67,10 +67,25 @@ void fill_dev(Handle* h, DevTree<T>& d, const std::vector<std::vector<double>>& 
               const std::vector<double>& wtab) {
     auto& o = h->owned;
     std::vector<T> W(wtab.begin(), wtab.end());
-    std::vector<T> D;
+    std::vector<T> D, F;
     for (auto& v : dtabs) D.insert(D.end(), v.begin(), v.end());
+    // folded tables: the four entries k_x = +-x' dk, k_y = +-y' dk are equal
+    // (checked), so sum_x sum_y D A[x] B[y] = sum_x' sum_y' Dfold A'[x'] B'[y']
+    // with A'[x'] = A[n + x'] + A[n - x'] (x' > 0)
+    const int n = d.n_diff, dd = 2 * n + 1, n1 = n + 1;
+    for (auto& v : dtabs)
+        for (int z = 0; z < n1; ++z)
+            for (int y = 0; y < n1; ++y)
+                for (int x = 0; x < n1; ++x) {
+                    auto at = [&](int i, int j) { return v[((size_t)z * dd + j) * dd + i]; };
+                    const double e = at(n + x, n + y);
+                    if (at(n - x, n + y) != e || at(n + x, n - y) != e || at(n - x, n - y) != e)
+                        throw HpdmkError(HPDMK_ERUNTIME, "difference table is not symmetric in k_x, k_y");
+                    F.push_back((T)e);
+                }
     d.Wmat = dupload(W, o);
     d.Dmat = dupload(D, o);
+    d.Dfold = dupload(F, o);
     d.root = dalloc<Cplx<T>>(d.Mw, o);
     d.out = dalloc<Cplx<T>>((size_t)d.n_pw_nodes * d.Md, o);
     d.inc = dalloc<Cplx<T>>((size_t)d.n_pw_nodes * d.Md, o);
