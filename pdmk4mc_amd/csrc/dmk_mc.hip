@@ -2428,15 +2428,303 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
     }
 }
 
-// waves per SIMD the compiler keeps registers for: three for the FP32 grids
-// of digits <= 6 (three blocks per CU, as many as the LDS holds), the
-// compiler's choice otherwise (wider grids and FP64 keep more per thread)
+// The update on the matrix cores (FP32 grids up to 32 points per axis).
+// Work items: the root's row blocks (its contributions split over items when
+// many), each row block of the level-2 incoming arrays (front of s.dest: the
+// most contributions), and every other rewritten array as one item over all
+// its row blocks, so that its contributions are listed and their rows staged
+// once.  Per row block of 128 grid rows: the sums S[r][x] = sum_c P_c[r] X_c[x]
+// (P_c[r] = s_c Y_c[y(r)] Z_c[z(r)]) as a complex (rows by contributions by x)
+// product, four v_mfma_f32_16x16x4f32 per complex 16 x 16 x 4 step, wave wv
+// owning row tiles 2 wv and 2 wv + 1 and both column tiles; each lane adds
+// its accumulators straight to the array elements it holds in the D layout
+// (loaded before the products), so the write back needs no LDS pass.
+constexpr int kApplyMfRows = 128;
+__host__ __device__ inline size_t apply_shmem_mf(int dmax, int K) {
+    return sizeof(Cplx<float>) * (2 * (size_t)kApplyTile * 3 * dmax + kApplyPad) +
+           sizeof(int) * 2 * (size_t)K * kApplyLv;
+}
+template <bool RB>
+__device__ __forceinline__ void apply_body_mf(const DevTree<float>& t, const McArgs& a) {
+    extern __shared__ char apply_dyn[];
+    __shared__ int mlist[2 * kMaxBatch];     // contributions: move << 8 | endpoint << 7 | image code
+    __shared__ int mrow[3 * 2 * kMaxBatch];  // their three table rows (x, y, z), offsets into s.tabs
+    __shared__ int wtot[TB / 64];
+    __shared__ int mcount;
+    __shared__ int ac_s[kMaxBatch];
+    if (*t.err) return;
+    const int nD = a.s.plan[0], nacc = a.s.plan[1], nH = a.s.plan[2];
+    if (!nacc) return;
+    if constexpr (RB) {
+        if (blockIdx.x == 0) {
+            relocate_batch(t, a);
+            return;
+        }
+    }
+    const int bid = blockIdx.x - (RB ? 1 : 0), nblk = gridDim.x - (RB ? 1 : 0);
+    const int K = a.K, D = t.max_depth, nL = n_levels_pw(D), stride = table_stride(D);
+    constexpr int RPI = kApplyMfRows;
+    const int dmax = t.dw > t.dd ? t.dw : t.dd;
+    Cplx<float>* rows2 = reinterpret_cast<Cplx<float>*>(apply_dyn);  // two tiles of rows at stride dmax
+    int* pnode = reinterpret_cast<int*>(rows2 + 2 * kApplyTile * 3 * dmax + kApplyPad);  // (move, endpoint, level-2) -> box
+    const int rows_w = t.dw * (t.n_window + 1), rows_d = t.dd * (t.n_diff + 1);
+    const int nWb = (rows_w + RPI - 1) / RPI, nDb = (rows_d + RPI - 1) / RPI;
+    const int sr = 2 * nacc > kApplyHeavy ? kApplySplit : 1;
+    const int n_root = nWb * sr, nF = nH * nDb;
+    const int total = n_root + nF + (nD - nH);
+    if (bid >= total) return;  // no work item for this block
+    const int n2 = 2 * t.n_pw_nodes;
+    for (int m = threadIdx.x; m < K; m += blockDim.x) ac_s[m] = a.s.accept[m];
+    for (int w = threadIdx.x; w < 2 * K * kApplyLv; w += blockDim.x) {
+        const int li = w % kApplyLv, me = w / kApplyLv, m = me >> 1, ep = me & 1;
+        const TrialInfo& ti = a.s.info[m];
+        const int lam = ep == 0 ? ti.lam_t : ti.lam_o;
+        const int l = 2 + li;
+        pnode[w] = (l <= lam && l <= D - 2) ? (ep == 0 ? ti.path_t[l] : ti.path_o[l]) : -1;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, li16 = lane & 15, kq = lane >> 4;
+    for (int it = bid; it < total; it += nblk) {
+        bool is_root = false;
+        int c_lo, c_hi, part_c = 0, nparts = 1, key = 0;
+        if (it < n_root) {
+            is_root = true;
+            c_lo = it / sr;
+            c_hi = c_lo + 1;
+            part_c = it % sr;
+            nparts = sr;
+        } else if (it < n_root + nF) {
+            const int r = it - n_root;
+            key = a.s.dest[r / nDb];
+            c_lo = r % nDb;
+            c_hi = c_lo + 1;
+        } else {
+            key = a.s.dest[n2 - 1 - (it - n_root - nF)];
+            c_lo = 0;
+            c_hi = nDb;
+        }
+        const bool heavy = nparts > 1;
+        const int slot = key >> 1, kind = is_root ? 2 : key & 1;
+        const int node = is_root ? 0 : a.pn.node[slot];
+        const int l = is_root ? 0 : a.pn.level[slot], li = l - 2;
+        const int nl = 1 << l;
+        const int d = is_root ? t.dw : t.dd;
+        const int nrows = is_root ? rows_w : rows_d;
+        Cplx<float>* arr = is_root ? t.root : (kind == 0 ? t.out : t.inc) + (size_t)slot * t.Md;
+        const float sy = (is_root || kind == 0) ? 1.f : -1.f;  // in[C] += conj(tau s)
+        // the first row block's elements this lane updates (D layout), in
+        // flight while the contributions are listed and staged
+        Cplx<float> pf[2][2][4], pn[2][2][4];  // this row block's elements, the next one's
+        auto load_block = [&](int chunk, Cplx<float> (&dst)[2][2][4]) {
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int row = chunk * RPI + (2 * wv + u) * 16 + 4 * kq + q, x = ct * 16 + li16;
+                        const bool ok = row < nrows && x < d;
+                        dst[u][ct][q] = arr[ok ? row * d + x : 0];
+                    }
+        };
+        if (!heavy) load_block(c_lo, pf);
+        unsigned long long tp = wall_clock64();
+        __syncthreads();  // previous item's lists and rows fully consumed
+        if (threadIdx.x == 0) mcount = 0;
+        __syncthreads();
+        for (int c0 = 0; c0 < 2 * K; c0 += blockDim.x) {
+            const int w = c0 + threadIdx.x;
+            int code = -1;
+            if (w < 2 * K) {
+                const int m = w >> 1, ep = w & 1;
+                if (ac_s[m]) {
+                    if (is_root) {
+                        code = 13;
+                    } else {
+                        int B;
+                        if (li < kApplyLv) {
+                            B = pnode[(2 * m + ep) * kApplyLv + li];
+                        } else {
+                            const TrialInfo& ti = a.s.info[m];
+                            B = l <= (ep == 0 ? ti.lam_t : ti.lam_o) ? (ep == 0 ? ti.path_t[l] : ti.path_o[l]) : -1;
+                        }
+                        if (B >= 0) {
+                            if (B == node) {
+                                code = 13;
+                            } else if (kind == 1) {
+                                const int px = image_shift(t.cell[3 * B], t.cell[3 * node], nl);
+                                const int py = image_shift(t.cell[3 * B + 1], t.cell[3 * node + 1], nl);
+                                const int pz = image_shift(t.cell[3 * B + 2], t.cell[3 * node + 2], nl);
+                                if (px != 2 && py != 2 && pz != 2) code = (px + 1) + 3 * (py + 1) + 9 * (pz + 1);
+                            }
+                        }
+                    }
+                }
+                if (code >= 0) code |= (m << 8) | (ep << 7);
+            }
+            const bool f = code >= 0;
+            const unsigned long long bal = __ballot(f);
+            const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+            if (lane == 0) wtot[wv] = __popcll(bal);
+            __syncthreads();
+            int off = mcount;
+            for (int k = 0; k < wv; ++k) off += wtot[k];
+            if (f) {
+                const int pos = off + __popcll(bal & lt);
+                mlist[pos] = code;
+                const int m = code >> 8, ep = (code >> 7) & 1, cc = code & 127;
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) {
+                    const int sh = ax == 0 ? cc % 3 - 1 : (ax == 1 ? (cc / 3) % 3 - 1 : cc / 9 - 1);
+                    mrow[3 * pos + ax] =
+                        m * stride + (is_root ? root_row(ep, ax) : level_row(nL, ep, li, ax, sh));
+                }
+            }
+            __syncthreads();
+            if (threadIdx.x == 0)
+                for (int k = 0; k < (int)(blockDim.x >> 6); ++k) mcount += wtot[k];
+            __syncthreads();
+        }
+        const int nm_all = mcount;
+        const int cbeg = part_c * nm_all / nparts, cend = (part_c + 1) * nm_all / nparts;
+        const int nm = cend - cbeg;
+        const int* mlc = mlist + cbeg;
+        if (threadIdx.x == 0) probe_add(a.s, kPrApplyMatch, tp);
+        if (heavy && nm == 0) continue;  // uniform across the block
+        // row tiles of kApplyTile contributions (3 table rows each, 32 lanes
+        // per row), double-buffered: the global loads of tile n+1 are in flight
+        // while tile n is accumulated; an item with one tile stages it once for
+        // all its row blocks
+        constexpr int RW = 32, RPP = TB / RW, R = (3 * kApplyTile + RPP - 1) / RPP;
+        const int mm = threadIdx.x % RW, rr0 = threadIdx.x / RW;
+        const Cplx<float>* tbase = reinterpret_cast<const Cplx<float>*>(a.s.tabs);
+        const int* mrc = mrow + 3 * cbeg;
+        float vr[R], vi[R];
+        auto load_tile = [&](int k0) {
+            const int nr = 3 * min(nm - k0, kApplyTile);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int rr = rr0 + r * RPP;
+                if (rr < nr && mm < d) {
+                    const Cplx<float> c = tbase[mrc[3 * k0 + rr] + mm];
+                    vr[r] = c.x;
+                    vi[r] = c.y;
+                }
+            }
+        };
+        auto store_tile = [&](int k0, Cplx<float>* dstrows) {
+            const int nr = 3 * min(nm - k0, kApplyTile);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int rr = rr0 + r * RPP;
+                if (rr < nr && mm < d) dstrows[rr * dmax + mm] = {vr[r], vi[r]};
+            }
+        };
+        const bool one_tile = nm <= kApplyTile;
+        for (int chunk = c_lo; chunk < c_hi; ++chunk) {
+            if (chunk > c_lo && !one_tile) __syncthreads();  // the previous block's tiles are consumed
+            if (!heavy && chunk + 1 < c_hi) load_block(chunk + 1, pn);  // in flight during this block
+            int oy[2], oz[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int rg = chunk * RPI + (2 * wv + u) * 16 + li16;
+                oy[u] = rg < nrows ? rg % d : 0;
+                oz[u] = rg < nrows ? rg / d : 0;
+            }
+            f32x4 mr[2][2], mi[2][2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct) mr[u][ct] = mi[u][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+            if (chunk == c_lo || !one_tile) {
+                load_tile(0);
+                store_tile(0, rows2);
+            }
+            int buf = 0;
+            for (int k0 = 0; k0 < nm; k0 += kApplyTile) {
+                const int nt = nm - k0 < kApplyTile ? nm - k0 : kApplyTile;
+                const bool more = k0 + kApplyTile < nm;
+                if (more) load_tile(k0 + kApplyTile);  // in flight during the accumulation
+                __syncthreads();                        // tile k0 stored by every thread
+                const Cplx<float>* rows = rows2 + buf * kApplyTile * 3 * dmax;
+                for (int c0 = 0; c0 < nt; c0 += 4) {  // uniform; lane group kq takes contribution c0 + kq
+                    const int k = c0 + kq;
+                    const bool kv = k < nt;
+                    const Cplx<float>* Rw = rows + (kv ? k : 0) * 3 * dmax;
+                    const float sg = (kv ? ((mlc[k0 + k] >> 7) & 1) ? -1.f : 1.f : 0.f);  // 0: past the tile's end
+                    float pr[2], pim[2], xr[2], xim[2];
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) {
+                        const Cplx<float> p = cmul(Rw[dmax + oy[u]], Rw[2 * dmax + oz[u]]);
+                        pr[u] = sg * p.x;
+                        pim[u] = sg * p.y;
+                    }
+#pragma unroll
+                    for (int ct = 0; ct < 2; ++ct) {
+                        const int x = ct * 16 + li16;
+                        const Cplx<float> xv = Rw[x < d ? x : 0];
+                        xr[ct] = x < d ? xv.x : 0.f;
+                        xim[ct] = x < d ? xv.y : 0.f;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 2; ++u)
+#pragma unroll
+                        for (int ct = 0; ct < 2; ++ct) {
+                            mr[u][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[u], xr[ct], mr[u][ct], 0, 0, 0);
+                            mr[u][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(-pim[u], xim[ct], mr[u][ct], 0, 0, 0);
+                            mi[u][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[u], xim[ct], mi[u][ct], 0, 0, 0);
+                            mi[u][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(pim[u], xr[ct], mi[u][ct], 0, 0, 0);
+                        }
+                }
+                if (more) store_tile(k0 + kApplyTile, rows2 + (buf ^ 1) * kApplyTile * 3 * dmax);
+                buf ^= 1;
+            }
+            if (threadIdx.x == 0) probe_add(a.s, kPrApplyTiles, tp);
+            // D layout of v_mfma_f32_16x16x4f32: lane holds rows 4 kq + q of column li16
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int row = chunk * RPI + (2 * wv + u) * 16 + 4 * kq + q, x = ct * 16 + li16;
+                        if (row >= nrows || x >= d) continue;
+                        Cplx<float>* e = arr + row * d + x;
+                        if (heavy) {
+                            atomicAdd(&e->x, mr[u][ct][q]);
+                            atomicAdd(&e->y, sy * mi[u][ct][q]);
+                        } else {
+                            *e = {pf[u][ct][q].x + mr[u][ct][q], pf[u][ct][q].y + sy * mi[u][ct][q]};
+                        }
+                    }
+            if (threadIdx.x == 0) probe_add(a.s, kPrApplyRmw, tp);
+            if (!heavy && chunk + 1 < c_hi)
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+#pragma unroll
+                    for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) pf[u][ct][q] = pn[u][ct][q];
+        }
+    }
+}
+
+// waves per SIMD the compiler keeps registers for: two for the FP32 grids of
+// digits <= 6 (the matrix-core update fits in 184 registers without spills;
+// three waves spill and gain under 1 %), the compiler's choice otherwise
+// (wider grids and FP64 keep more per thread)
 template <typename T, int XMAX>
-constexpr int apply_min_waves() { return std::is_same<T, float>::value && XMAX <= 13 ? 3 : 1; }
+constexpr int apply_min_waves() { return std::is_same<T, float>::value && XMAX <= 13 ? 2 : 1; }
+
+// the FP32 grids of digits <= 6 take the matrix-core update (apply_body_mf)
+template <typename T, int XMAX>
+__host__ __device__ constexpr bool apply_uses_mf() { return std::is_same<T, float>::value && apply_lpr<T>() * XMAX <= 32; }
 
 template <typename T, int XMAX, bool RB>
 __global__ void __launch_bounds__(TB, (apply_min_waves<T, XMAX>())) k_apply(DevTree<T> t, McArgs a) {
-    apply_body<T, XMAX, RB>(t, a);
+    if constexpr (apply_uses_mf<T, XMAX>())
+        apply_body_mf<RB>(t, a);
+    else
+        apply_body<T, XMAX, RB>(t, a);
 }
 
 // ------------------------------------------------------------------ fused single-move update
@@ -2958,7 +3246,8 @@ void launch_scan(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_
 template <typename T>
 void launch_apply(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     const int dmax = t.dw > t.dd ? t.dw : t.dd;
-    const size_t sh = apply_shmem<T>(dmax, a.K);
+    const bool mf = apply_xmax<T>(dmax) <= kApplyXSmall<T> && apply_uses_mf<T, kApplyXSmall<T>>();
+    const size_t sh = mf ? apply_shmem_mf(dmax, a.K) : apply_shmem<T>(dmax, a.K);
     const bool rb = apply_relocates(a);
     const int nb = a.tiles.apply + (rb ? 1 : 0);  // + the particle-move block
     auto go = [&](auto kernel) {
