@@ -59,6 +59,9 @@ __device__ __forceinline__ int image_shift(int ca, int cb, int nl) {
     return 2;
 }
 
+// a box cell below 1024 per axis in one int (x | y << 10 | z << 20)
+__device__ __forceinline__ int pack_cell(int cx, int cy, int cz) { return cx | (cy << 10) | (cz << 20); }
+
 // first move of this launch: the step counter when one is attached
 __device__ __forceinline__ int move_base(const McArgs& a) { return (a.move_ctr ? *a.move_ctr : 0) + a.move0; }
 // trials of this launch: with a step counter, the moves left (at most K)
@@ -2272,25 +2275,6 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
         T ar[XMAX], ai[XMAX];
 #pragma unroll
         for (int x = 0; x < XMAX; ++x) ar[x] = ai[x] = T(0);
-        // FP32 with d <= 32: the item's sums are a complex
-        // product on the matrix cores, S[r][x] = sum_c P_c[r] X_c[x] with
-        // P_c[r] = s_c Y_c[y(r)] Z_c[z(r)] (RPI rows by the contributions, by 32
-        // x values), four v_mfma_f32_16x16x4f32 per complex 16 x 16 x 4 step;
-        // wave wv owns row tiles 2 wv and 2 wv + 1 and both column tiles.
-        constexpr bool MF = std::is_same<T, float>::value && LPR * XMAX <= 32 && RPI == 128;
-        const int li16 = lane & 15, kq = lane >> 4;
-        int oy[2], oz[2];
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const int rg = chunk * RPI + (2 * wv + u) * 16 + li16;
-            oy[u] = rg < nrows ? rg % d : 0;
-            oz[u] = rg < nrows ? rg / d : 0;
-        }
-        f32x4 mr[2][2], mi[2][2];
-#pragma unroll
-        for (int u = 0; u < 2; ++u)
-#pragma unroll
-            for (int ct = 0; ct < 2; ++ct) mr[u][ct] = mi[u][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
         // Row tiles of kApplyTile contributions (3 table rows each, RW lanes
         // per row), double-buffered: the global loads of tile n+1 are in
         // flight while tile n is accumulated.
@@ -2336,53 +2320,21 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
             // share of the row read on into the following rows or the padding
             // after the tiles, and are never written), so the row's LDS reads
             // are issued together instead of one branch and wait per x.
-            if constexpr (MF) {
-                for (int c0 = 0; c0 < nt; c0 += 4) {  // uniform; lane group kq takes contribution c0 + kq
-                    const int k = c0 + kq;
-                    const bool kv = k < nt;
-                    const Cplx<T>* Rw = rows + (kv ? k : 0) * 3 * dmax;
-                    const float sg = (kv ? ((mlc[k0 + k] >> 7) & 1) ? -1.f : 1.f : 0.f);  // 0: past the tile's end
-                    float pr[2], pim[2], xr[2], xim[2];
+            for (int k = 0; k < nt; ++k) {
+                const Cplx<T>* Rw = rows + k * 3 * dmax;
+                const T sg = ((mlc[k0 + k] >> 7) & 1) ? T(-1) : T(1);
+                Cplx<T> yz = cmul(Rw[dmax + y], Rw[2 * dmax + z]);
+                yz.x *= sg;
+                yz.y *= sg;
+                Cplx<T> xv[XMAX];
 #pragma unroll
-                    for (int u = 0; u < 2; ++u) {
-                        const Cplx<T> p = cmul(Rw[dmax + oy[u]], Rw[2 * dmax + oz[u]]);
-                        pr[u] = sg * p.x;
-                        pim[u] = sg * p.y;
-                    }
+                for (int x = 0; x < XMAX; ++x) xv[x] = Rw[x0 + x];
 #pragma unroll
-                    for (int ct = 0; ct < 2; ++ct) {
-                        const int x = ct * 16 + li16;
-                        const Cplx<T> xv = Rw[x < d ? x : 0];
-                        xr[ct] = x < d ? xv.x : 0.f;
-                        xim[ct] = x < d ? xv.y : 0.f;
-                    }
-#pragma unroll
-                    for (int u = 0; u < 2; ++u)
-#pragma unroll
-                        for (int ct = 0; ct < 2; ++ct) {
-                            mr[u][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[u], xr[ct], mr[u][ct], 0, 0, 0);
-                            mr[u][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(-pim[u], xim[ct], mr[u][ct], 0, 0, 0);
-                            mi[u][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[u], xim[ct], mi[u][ct], 0, 0, 0);
-                            mi[u][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(pim[u], xr[ct], mi[u][ct], 0, 0, 0);
-                        }
-                }
-            } else {
-                for (int k = 0; k < nt; ++k) {
-                    const Cplx<T>* Rw = rows + k * 3 * dmax;
-                    const T sg = ((mlc[k0 + k] >> 7) & 1) ? T(-1) : T(1);
-                    Cplx<T> yz = cmul(Rw[dmax + y], Rw[2 * dmax + z]);
-                    yz.x *= sg;
-                    yz.y *= sg;
-                    Cplx<T> xv[XMAX];
-#pragma unroll
-                    for (int x = 0; x < XMAX; ++x) xv[x] = Rw[x0 + x];
-#pragma unroll
-                    for (int x = 0; x < XMAX; ++x) {
-                        ar[x] = fma(xv[x].x, yz.x, ar[x]);
-                        ar[x] = fma(-xv[x].y, yz.y, ar[x]);
-                        ai[x] = fma(xv[x].x, yz.y, ai[x]);
-                        ai[x] = fma(xv[x].y, yz.x, ai[x]);
-                    }
+                for (int x = 0; x < XMAX; ++x) {
+                    ar[x] = fma(xv[x].x, yz.x, ar[x]);
+                    ar[x] = fma(-xv[x].y, yz.y, ar[x]);
+                    ai[x] = fma(xv[x].x, yz.y, ai[x]);
+                    ai[x] = fma(xv[x].y, yz.x, ai[x]);
                 }
             }
             if (more) store_tile(k0 + kApplyTile, rows2 + (buf ^ 1) * kApplyTile * 3 * dmax);
@@ -2391,22 +2343,9 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
         if (threadIdx.x == 0) probe_add(a.s, kPrApplyTiles, tp);
         __syncthreads();  // the last tile is consumed: its space takes the sums
         // sums to LDS in the array's order, then a coalesced write
-        if constexpr (MF) {
-            // D layout of v_mfma_f32_16x16x4f32: lane holds rows 4 kq + q of column li16
 #pragma unroll
-            for (int u = 0; u < 2; ++u)
-#pragma unroll
-                for (int ct = 0; ct < 2; ++ct)
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int lq = (2 * wv + u) * 16 + 4 * kq + q, x = ct * 16 + li16;
-                        if (chunk * RPI + lq < nrows && x < d) S[lq * d + x] = {mr[u][ct][q], mi[u][ct][q]};
-                    }
-        } else {
-#pragma unroll
-            for (int x = 0; x < XMAX; ++x)
-                if (x < nx) S[lr * d + x0 + x] = {ar[x], ai[x]};
-        }
+        for (int x = 0; x < XMAX; ++x)
+            if (x < nx) S[lr * d + x0 + x] = {ar[x], ai[x]};
         __syncthreads();
         if (heavy) {
             for (int e = threadIdx.x; e < ne; e += TB) {
@@ -2435,10 +2374,11 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
 // its row blocks, so that its contributions are listed and their rows staged
 // once.  Per row block of 128 grid rows: the sums S[r][x] = sum_c P_c[r] X_c[x]
 // (P_c[r] = s_c Y_c[y(r)] Z_c[z(r)]) as a complex (rows by contributions by x)
-// product, four v_mfma_f32_16x16x4f32 per complex 16 x 16 x 4 step, wave wv
+// product, three v_mfma_f32_16x16x4f32 per complex 16 x 16 x 4 step, wave wv
 // owning row tiles 2 wv and 2 wv + 1 and both column tiles; each lane adds
 // its accumulators straight to the array elements it holds in the D layout
-// (loaded before the products), so the write back needs no LDS pass.
+// (loaded while the first tile is staged), so the write back needs no LDS
+// pass.
 constexpr int kApplyMfRows = 128;
 __host__ __device__ inline size_t apply_shmem_mf(int dmax, int K) {
     return sizeof(Cplx<float>) * (2 * (size_t)kApplyTile * 3 * dmax + kApplyPad) +
@@ -2452,9 +2392,9 @@ __device__ __forceinline__ void apply_body_mf(const DevTree<float>& t, const McA
     __shared__ int wtot[TB / 64];
     __shared__ int mcount;
     __shared__ int ac_s[kMaxBatch];
-    if (*t.err) return;
-    const int nD = a.s.plan[0], nacc = a.s.plan[1], nH = a.s.plan[2];
-    if (!nacc) return;
+    // the error flag and the plan in one round trip
+    const int err = *t.err, nD = a.s.plan[0], nacc = a.s.plan[1], nH = a.s.plan[2];
+    if (err || !nacc) return;
     if constexpr (RB) {
         if (blockIdx.x == 0) {
             relocate_batch(t, a);
@@ -2466,7 +2406,8 @@ __device__ __forceinline__ void apply_body_mf(const DevTree<float>& t, const McA
     constexpr int RPI = kApplyMfRows;
     const int dmax = t.dw > t.dd ? t.dw : t.dd;
     Cplx<float>* rows2 = reinterpret_cast<Cplx<float>*>(apply_dyn);  // two tiles of rows at stride dmax
-    int* pnode = reinterpret_cast<int*>(rows2 + 2 * kApplyTile * 3 * dmax + kApplyPad);  // (move, endpoint, level-2) -> box
+    // (move, endpoint, level - 2) -> packed cell of the endpoint's box, -1 past its leaf
+    int* pcell = reinterpret_cast<int*>(rows2 + 2 * kApplyTile * 3 * dmax + kApplyPad);
     const int rows_w = t.dw * (t.n_window + 1), rows_d = t.dd * (t.n_diff + 1);
     const int nWb = (rows_w + RPI - 1) / RPI, nDb = (rows_d + RPI - 1) / RPI;
     const int sr = 2 * nacc > kApplyHeavy ? kApplySplit : 1;
@@ -2475,12 +2416,17 @@ __device__ __forceinline__ void apply_body_mf(const DevTree<float>& t, const McA
     if (bid >= total) return;  // no work item for this block
     const int n2 = 2 * t.n_pw_nodes;
     for (int m = threadIdx.x; m < K; m += blockDim.x) ac_s[m] = a.s.accept[m];
+    // the endpoints' path boxes by their cells (loaded here, while the
+    // first item's array and its box are looked up)
     for (int w = threadIdx.x; w < 2 * K * kApplyLv; w += blockDim.x) {
         const int li = w % kApplyLv, me = w / kApplyLv, m = me >> 1, ep = me & 1;
         const TrialInfo& ti = a.s.info[m];
         const int lam = ep == 0 ? ti.lam_t : ti.lam_o;
         const int l = 2 + li;
-        pnode[w] = (l <= lam && l <= D - 2) ? (ep == 0 ? ti.path_t[l] : ti.path_o[l]) : -1;
+        const bool on = l <= lam && l <= D - 2;
+        const int B = on ? (ep == 0 ? ti.path_t[l] : ti.path_o[l]) : 0;
+        const int cx = t.cell[3 * B], cy = t.cell[3 * B + 1], cz = t.cell[3 * B + 2];
+        pcell[w] = on ? pack_cell(cx, cy, cz) : -1;
     }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, li16 = lane & 15, kq = lane >> 4;
     for (int it = bid; it < total; it += nblk) {
@@ -2504,16 +2450,24 @@ __device__ __forceinline__ void apply_body_mf(const DevTree<float>& t, const McA
         }
         const bool heavy = nparts > 1;
         const int slot = key >> 1, kind = is_root ? 2 : key & 1;
-        const int node = is_root ? 0 : a.pn.node[slot];
-        const int l = is_root ? 0 : a.pn.level[slot], li = l - 2;
-        const int nl = 1 << l;
+        int l = 0, ncx = 0, ncy = 0, ncz = 0;  // the array box's level and cell
+        if (!is_root) {
+            const int node = a.pn.node[slot];
+            l = a.pn.level[slot];
+            ncx = t.cell[3 * node];
+            ncy = t.cell[3 * node + 1];
+            ncz = t.cell[3 * node + 2];
+        }
+        const int li = l - 2, nl = 1 << l;
         const int d = is_root ? t.dw : t.dd;
         const int nrows = is_root ? rows_w : rows_d;
         Cplx<float>* arr = is_root ? t.root : (kind == 0 ? t.out : t.inc) + (size_t)slot * t.Md;
         const float sy = (is_root || kind == 0) ? 1.f : -1.f;  // in[C] += conj(tau s)
-        // the first row block's elements this lane updates (D layout), in
-        // flight while the contributions are listed and staged
-        Cplx<float> pf[2][2][4], pn[2][2][4];  // this row block's elements, the next one's
+        // the elements this lane updates (D layout) of row blocks c and c + 1,
+        // the first two loaded with the first tile's rows: loads complete in
+        // issue order, so loads issued before the lists would hold up the
+        // lists, which need their data first
+        Cplx<float> pf[2][2][4], pn[2][2][4];
         auto load_block = [&](int chunk, Cplx<float> (&dst)[2][2][4]) {
 #pragma unroll
             for (int u = 0; u < 2; ++u)
@@ -2526,7 +2480,6 @@ __device__ __forceinline__ void apply_body_mf(const DevTree<float>& t, const McA
                         dst[u][ct][q] = arr[ok ? row * d + x : 0];
                     }
         };
-        if (!heavy) load_block(c_lo, pf);
         unsigned long long tp = wall_clock64();
         __syncthreads();  // previous item's lists and rows fully consumed
         if (threadIdx.x == 0) mcount = 0;
@@ -2540,20 +2493,29 @@ __device__ __forceinline__ void apply_body_mf(const DevTree<float>& t, const McA
                     if (is_root) {
                         code = 13;
                     } else {
-                        int B;
+                        bool on;  // the endpoint has a box at level l, of cell (bx, by, bz)
+                        int bx, by, bz;
                         if (li < kApplyLv) {
-                            B = pnode[(2 * m + ep) * kApplyLv + li];
+                            const int pc = pcell[(2 * m + ep) * kApplyLv + li];
+                            on = pc >= 0;
+                            bx = pc & 1023;
+                            by = (pc >> 10) & 1023;
+                            bz = pc >> 20;
                         } else {
                             const TrialInfo& ti = a.s.info[m];
-                            B = l <= (ep == 0 ? ti.lam_t : ti.lam_o) ? (ep == 0 ? ti.path_t[l] : ti.path_o[l]) : -1;
+                            on = l <= (ep == 0 ? ti.lam_t : ti.lam_o);
+                            const int B = on ? (ep == 0 ? ti.path_t[l] : ti.path_o[l]) : 0;
+                            bx = t.cell[3 * B];
+                            by = t.cell[3 * B + 1];
+                            bz = t.cell[3 * B + 2];
                         }
-                        if (B >= 0) {
-                            if (B == node) {
+                        if (on) {
+                            if (bx == ncx && by == ncy && bz == ncz) {
                                 code = 13;
                             } else if (kind == 1) {
-                                const int px = image_shift(t.cell[3 * B], t.cell[3 * node], nl);
-                                const int py = image_shift(t.cell[3 * B + 1], t.cell[3 * node + 1], nl);
-                                const int pz = image_shift(t.cell[3 * B + 2], t.cell[3 * node + 2], nl);
+                                const int px = image_shift(bx, ncx, nl);
+                                const int py = image_shift(by, ncy, nl);
+                                const int pz = image_shift(bz, ncz, nl);
                                 if (px != 2 && py != 2 && pz != 2) code = (px + 1) + 3 * (py + 1) + 9 * (pz + 1);
                             }
                         }
@@ -2622,7 +2584,6 @@ __device__ __forceinline__ void apply_body_mf(const DevTree<float>& t, const McA
         const bool one_tile = nm <= kApplyTile;
         for (int chunk = c_lo; chunk < c_hi; ++chunk) {
             if (chunk > c_lo && !one_tile) __syncthreads();  // the previous block's tiles are consumed
-            if (!heavy && chunk + 1 < c_hi) load_block(chunk + 1, pn);  // in flight during this block
             int oy[2], oz[2];
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
@@ -2630,13 +2591,17 @@ __device__ __forceinline__ void apply_body_mf(const DevTree<float>& t, const McA
                 oy[u] = rg < nrows ? rg % d : 0;
                 oz[u] = rg < nrows ? rg / d : 0;
             }
-            f32x4 mr[2][2], mi[2][2];
+            f32x4 mr[2][2], mi[2][2], m3[2][2];
 #pragma unroll
             for (int u = 0; u < 2; ++u)
 #pragma unroll
-                for (int ct = 0; ct < 2; ++ct) mr[u][ct] = mi[u][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+                for (int ct = 0; ct < 2; ++ct) mr[u][ct] = mi[u][ct] = m3[u][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
             if (chunk == c_lo || !one_tile) {
                 load_tile(0);
+                if (chunk == c_lo && !heavy) {
+                    load_block(c_lo, pf);
+                    if (c_lo + 1 < c_hi) load_block(c_lo + 1, pn);
+                }
                 store_tile(0, rows2);
             }
             int buf = 0;
@@ -2665,21 +2630,53 @@ __device__ __forceinline__ void apply_body_mf(const DevTree<float>& t, const McA
                         xr[ct] = x < d ? xv.x : 0.f;
                         xim[ct] = x < d ? xv.y : 0.f;
                     }
+                    // complex products with three real ones (Gauss):
+                    // K1 = (pr + pi) xr, K2 = pr (xi - xr), K3 = pi (xr + xi),
+                    // re = K1 - K3, im = K1 + K2, each K summed on its own
 #pragma unroll
                     for (int u = 0; u < 2; ++u)
 #pragma unroll
                         for (int ct = 0; ct < 2; ++ct) {
-                            mr[u][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[u], xr[ct], mr[u][ct], 0, 0, 0);
-                            mr[u][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(-pim[u], xim[ct], mr[u][ct], 0, 0, 0);
-                            mi[u][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[u], xim[ct], mi[u][ct], 0, 0, 0);
-                            mi[u][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(pim[u], xr[ct], mi[u][ct], 0, 0, 0);
+                            mr[u][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[u] + pim[u], xr[ct], mr[u][ct], 0, 0, 0);
+                            mi[u][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[u], xim[ct] - xr[ct], mi[u][ct], 0, 0, 0);
+                            m3[u][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(pim[u], xr[ct] + xim[ct], m3[u][ct], 0, 0, 0);
                         }
                 }
                 if (more) store_tile(k0 + kApplyTile, rows2 + (buf ^ 1) * kApplyTile * 3 * dmax);
                 buf ^= 1;
             }
             if (threadIdx.x == 0) probe_add(a.s, kPrApplyTiles, tp);
-            // D layout of v_mfma_f32_16x16x4f32: lane holds rows 4 kq + q of column li16
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct) {  // mr, mi, m3 hold K1, K2, K3
+                    mi[u][ct] = mr[u][ct] + mi[u][ct];
+                    mr[u][ct] = mr[u][ct] - m3[u][ct];
+                }
+            // D layout of v_mfma_f32_16x16x4f32: lane holds rows 4 kq + q of
+            // column li16.  A light item adds its elements to the sums, moves
+            // the next row block's elements up and issues the loads of the
+            // one after before its stores, so that no wait for elements
+            // waits for stores as well.
+            if (!heavy) {
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+#pragma unroll
+                    for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            mr[u][ct][q] = pf[u][ct][q].x + mr[u][ct][q];
+                            mi[u][ct][q] = pf[u][ct][q].y + sy * mi[u][ct][q];
+                        }
+                if (chunk + 1 < c_hi)
+#pragma unroll
+                    for (int u = 0; u < 2; ++u)
+#pragma unroll
+                        for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) pf[u][ct][q] = pn[u][ct][q];
+                if (chunk + 2 < c_hi) load_block(chunk + 2, pn);
+            }
 #pragma unroll
             for (int u = 0; u < 2; ++u)
 #pragma unroll
@@ -2693,17 +2690,10 @@ __device__ __forceinline__ void apply_body_mf(const DevTree<float>& t, const McA
                             atomicAdd(&e->x, mr[u][ct][q]);
                             atomicAdd(&e->y, sy * mi[u][ct][q]);
                         } else {
-                            *e = {pf[u][ct][q].x + mr[u][ct][q], pf[u][ct][q].y + sy * mi[u][ct][q]};
+                            *e = {mr[u][ct][q], mi[u][ct][q]};
                         }
                     }
             if (threadIdx.x == 0) probe_add(a.s, kPrApplyRmw, tp);
-            if (!heavy && chunk + 1 < c_hi)
-#pragma unroll
-                for (int u = 0; u < 2; ++u)
-#pragma unroll
-                    for (int ct = 0; ct < 2; ++ct)
-#pragma unroll
-                        for (int q = 0; q < 4; ++q) pf[u][ct][q] = pn[u][ct][q];
         }
     }
 }
