@@ -1449,20 +1449,45 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
         ++i;
     }
     const int j = i + 1 + p;
-    if (*t.err || j >= keff(a)) return;
+    // the error flag and the step counter are read with the trial records
+    // below and tested after them (their slots are valid memory either way)
+    const bool skip = *t.err || j >= keff(a);
     const TrialInfo& A_ = a.s.info[i];
     const TrialInfo& Bj = a.s.info[j];
     const int D = t.max_depth, nL = n_levels_pw(D), stride = table_stride(D);
     const Cplx<T>* ta = tabs_of<T>(a.s, stride, i);
     const Cplx<T>* tj = tabs_of<T>(a.s, stride, j);
     // window: (tau_j - omega_j) against (tau_i - omega_i): the Gram blocks'
-    // shares in order, or four separable terms
+    // shares (a thread's first two loaded here and added at the end, so that
+    // no wait for them precedes the level terms' loads), or four separable
+    // terms
     double part = 0.0;
-    if (use_gram<T>(a)) {
-        const int G = gram_blocks(t.Mw);
-        for (int g = threadIdx.x; g < G; g += blockDim.x) part += a.s.gram[((size_t)i * G + g) * K + j];
-        part *= 2.0 * t.cW;
-    } else {
+    const int G = use_gram<T>(a) ? gram_blocks(t.Mw) : 0;
+    const double* gsh = a.s.gram + (size_t)i * G * K + j;
+    const double gs0 = (int)threadIdx.x < G ? gsh[(size_t)threadIdx.x * K] : 0.0;
+    const double gs1 = (int)(threadIdx.x + blockDim.x) < G ? gsh[(size_t)(threadIdx.x + blockDim.x) * K] : 0.0;
+    // difference levels of j's paths next to i's boxes.  Group (ep, li): j's
+    // endpoint ep (0 target +, 1 origin -) at level l = 2 + li; slot w of the
+    // group: i's endpoint w (0 target +, 1 origin -); on when l is above j's
+    // leaf level, i's box exists at level l and (as the periodic image
+    // shifted by sh) is B or a neighbour of j's box B.  The image shifts of
+    // every (group, slot, axis) first, one thread each, from the endpoints'
+    // cells (computed from the positions, as locate finds the path boxes).
+    __shared__ int shf[2 * kMaxLevels * 6];  // ((ep nL + li) 2 + w) 3 + axis: image shift, 2 = off
+    for (int tt = threadIdx.x; tt < 12 * nL; tt += blockDim.x) {
+        const int ax = tt % 3, w = (tt / 3) & 1, gg = tt / 6;
+        const int ep = gg / nL, l = 2 + gg % nL;
+        const int lam = ep == 0 ? Bj.lam_t : Bj.lam_o;
+        const int lami = w == 0 ? A_.lam_t : A_.lam_o;
+        int v = 2;
+        if (l <= lam - 1 && lami >= l) {
+            const double xj = ep == 0 ? Bj.t[ax] : Bj.o[ax], xi = w == 0 ? A_.t[ax] : A_.o[ax];
+            v = image_shift(cell_at(xi, t.L, l), cell_at(xj, t.L, l), 1 << l);
+        }
+        shf[tt] = v;
+    }
+    if (skip) return;  // uniform
+    if (!G) {
         const Cplx<T>* rows[24];
         double sg[4];
         for (int ej = 0; ej < 2; ++ej)
@@ -1476,44 +1501,29 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
             }
         part = separable_dots<T, 4>(t.Wmat, t.dw, t.n_window + 1, rows, sg, A, B, C) * (2.0 * t.cW);
     }
-    // difference levels of j's paths next to i's boxes.  Term slot
-    // s = 2 (ep nL + li) + w: j's endpoint ep (0 target +, 1 origin -) at level
-    // l = 2 + li against i's endpoint w (0 target +, 1 origin -); on when l is
-    // above j's leaf level, i's box exists at level l and (as the periodic
-    // image shifted by sh) is B or a neighbour of j's box B.  Up to four
-    // (ep, li) groups per pass: one thread per (slot, axis) finds the image
-    // shifts, then every on slot's factor rows are formed at once and the dots
-    // taken group by group (three barriers per pass).
-    __shared__ int shf[8 * 3];  // (slot of the pass, axis): image shift, 2 = off
-    int ndots = 0;              // level terms formed (counted for the profiled runs)
+    // Passes over pairs of levels (2 pp, 2 pp + 1), four groups each
+    // (gq: ep = gq & 1, li = 2 pp + gq / 2), slot q = 2 gq + w of A/B/C:
+    // every slot's factor rows are formed at once, then the dots taken level
+    // by level, the four slots of a level in one pass over its table (two
+    // barriers per pass).
+    int ndots = 0;  // level terms formed (counted for the profiled runs)
     // over the folded tables (Dfold): x and y rows folded, A'[x'] = A[n + x'] + A[n - x']
     const int nd = t.n_diff, n1 = nd + 1, nzd = nd + 1, perd = 2 * n1 + nzd;
-    for (int g0 = 0; g0 < 2 * nL; g0 += 4) {
-        const int ng = min(4, 2 * nL - g0);
-        __syncthreads();  // the previous pass's (or the window's) LDS is consumed
-        if ((int)threadIdx.x < 6 * ng) {
-            const int ax = threadIdx.x % 3, w = (threadIdx.x / 3) & 1, gg = g0 + threadIdx.x / 6;
-            const int ep = gg / nL, l = 2 + gg % nL;
-            const int lam = ep == 0 ? Bj.lam_t : Bj.lam_o;
-            const int lami = w == 0 ? A_.lam_t : A_.lam_o;
-            int v = 2;
-            if (l <= lam - 1 && lami >= l) {
-                const int node = ep == 0 ? Bj.path_t[l] : Bj.path_o[l];
-                const int bi = w == 0 ? A_.path_t[l] : A_.path_o[l];
-                v = image_shift(t.cell[3 * bi + ax], t.cell[3 * node + ax], 1 << l);
-            }
-            shf[threadIdx.x] = v;
-        }
-        __syncthreads();
+    for (int pp = 0; 2 * pp < nL; ++pp) {
+        const int ng = 2 * pp + 1 < nL ? 4 : 2;
+        __syncthreads();  // the shifts written; the previous pass's (or the window's) LDS consumed
         unsigned on = 0;  // the same in every thread
-        for (int q = 0; q < 2 * ng; ++q)
-            if (shf[3 * q] != 2 && shf[3 * q + 1] != 2 && shf[3 * q + 2] != 2) on |= 1u << q;
+        for (int q = 0; q < 2 * ng; ++q) {
+            const int gq = q >> 1, w = q & 1, gg = (gq & 1) * nL + 2 * pp + (gq >> 1);
+            const int* sh3 = shf + (2 * gg + w) * 3;
+            if (sh3[0] != 2 && sh3[1] != 2 && sh3[2] != 2) on |= 1u << q;
+        }
         if (!on) continue;
         // factor rows u conj(v) of the on slots (u = j's row, v = i's shifted
-        // row), the x and y rows folded at the grid centre n: a thread's
-        // entries FR at a time, all their loads in flight together (indices
-        // clamped into the rows; the off slots and the unfolded z rows add
-        // nothing)
+        // row; zero rows for the off slots), the x and y rows folded at the
+        // grid centre n: a thread's entries FR at a time, all their loads in
+        // flight together (indices clamped into the rows; the unfolded z rows
+        // add nothing)
         constexpr int FR = 4;
         const int nfe = 2 * ng * perd;
         for (int e0 = threadIdx.x; e0 < nfe; e0 += FR * blockDim.x) {
@@ -1521,9 +1531,9 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
 #pragma unroll
             for (int k = 0; k < FR; ++k) {
                 const int e = min(e0 + k * (int)blockDim.x, nfe - 1), q = e / perd, x = e % perd;
-                const int gg = g0 + q / 2, w = q & 1, ep = gg / nL, li = gg % nL;
+                const int gq = q >> 1, w = q & 1, ep = gq & 1, li = 2 * pp + (gq >> 1);
                 const int ax = x < n1 ? 0 : x < 2 * n1 ? 1 : 2, xi = x - ax * n1;
-                const int sh = shf[3 * q + ax];
+                const int sh = shf[(2 * (ep * nL + li) + w) * 3 + ax];
                 const Cplx<T>* ru = tj + level_row(nL, ep, li, ax, 0);
                 const Cplx<T>* rv = ta + level_row(nL, w, li, ax, sh == 2 ? 0 : sh);
                 const int x1 = ax == 2 ? xi : nd + xi, x2 = ax == 2 ? xi : nd - xi;
@@ -1537,34 +1547,28 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
                 const int e = e0 + k * (int)blockDim.x;
                 if (e >= nfe) continue;
                 const int q = e / perd, x = e % perd;
-                if (!((on >> q) & 1u)) continue;
                 const int ax = x < n1 ? 0 : x < 2 * n1 ? 1 : 2, xi = x - ax * n1;
                 Cplx<T> r = {u1[k].x * v1[k].x + u1[k].y * v1[k].y, u1[k].y * v1[k].x - u1[k].x * v1[k].y};
                 if (ax < 2 && xi > 0) {  // the mirrored entry of an x or y row
                     r.x += u2[k].x * v2[k].x + u2[k].y * v2[k].y;
                     r.y += u2[k].y * v2[k].x - u2[k].x * v2[k].y;
                 }
+                if (!((on >> q) & 1u)) r = {T(0), T(0)};  // an off slot adds nothing to its level's dot
                 Cplx<T>* dst = ax == 0 ? A : ax == 1 ? B : C;
                 dst[q * kRowStride + xi] = r;
             }
         }
         __syncthreads();
-        for (int gq = 0; gq < ng; ++gq) {
-            const unsigned m2 = (on >> (2 * gq)) & 3u;
-            if (!m2) continue;  // uniform
-            const int gg = g0 + gq, ep = gg / nL, li = gg % nL;
+        for (int lq = 0; 2 * lq < ng; ++lq) {
+            // slots 4 lq + c, c = 2 ep + w: sign + when ep and w agree
+            const unsigned m4 = (on >> (4 * lq)) & 15u;
+            if (!m4) continue;  // uniform
+            const int li = 2 * pp + lq, o = 4 * lq * kRowStride;
             const T* Dl = t.Dfold + (size_t)li * n1 * n1 * nzd;
-            const int o = (2 * gq + (m2 == 2u ? 1 : 0)) * kRowStride;  // first on slot
-            double lv;
-            if (m2 == 3u) {
-                const double sg[2] = {1.0, -1.0};
-                lv = dot_rows<T, 2>(Dl, n1, nzd, A + o, B + o, C + o, sg);
-            } else {
-                const double sg[1] = {m2 == 1u ? 1.0 : -1.0};
-                lv = dot_rows<T, 1>(Dl, n1, nzd, A + o, B + o, C + o, sg);
-            }
-            ndots += __popc(m2);
-            part += lv * 2.0 * t.cD[2 + li] * (ep == 0 ? 1.0 : -1.0);
+            const double sg[4] = {1.0, -1.0, -1.0, 1.0};
+            const double lv = dot_rows<T, 4>(Dl, n1, nzd, A + o, B + o, C + o, sg);
+            ndots += __popc(m4);
+            part += lv * 2.0 * t.cD[2 + li];
         }
     }
     // near field: four threads of the last wave take one (ep, w) pair each
@@ -1601,6 +1605,11 @@ __global__ void __launch_bounds__(TB) k_pair_corr(DevTree<T> t, McArgs a) {
                 flags |= 4;
         }
         a.s.ovl[i * K + j] = (unsigned char)flags;
+    }
+    if (G) {
+        double gw = gs0 + gs1;
+        for (int g = threadIdx.x + 2 * blockDim.x; g < G; g += blockDim.x) gw += gsh[(size_t)g * K];
+        part += gw * (2.0 * t.cW);
     }
     const double total = block_sum(part, red);
     if (threadIdx.x == 0) {
