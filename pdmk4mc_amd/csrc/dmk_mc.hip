@@ -1216,8 +1216,10 @@ __global__ void __launch_bounds__(TB) k_decide_rep(const DevTree<T>* __restrict_
     if (threadIdx.x == 0) *a.move_ctr += 1;
 }
 
+// six waves per SIMD (at most 80 registers): the role blocks are chains of
+// dependent loads, more of them resident beats more registers each
 template <typename T, bool PM>  // PM: with the marking row (single-move steps)
-__global__ void __launch_bounds__(TB) k_trial(DevTree<T> t, McArgs a) {
+__global__ void __launch_bounds__(TB, 6) k_trial(DevTree<T> t, McArgs a) {
     trial_body<T, PM>(t, a);
 }
 
