@@ -189,6 +189,10 @@ struct McArgs {
     // kernel decides from the trial partials and moves the particle; the
     // trial kernel prepares move m + 1 in the other slot
     int ring;
+    // batched chains whose update kernel also sets up the next step
+    // (launch_apply_next): the scan moves the accepted particles, so that the
+    // next step's set-up reads final positions from the update's first block on
+    int scan_reloc;
     int K;       // trials in this launch (at most)
     int mode;    // TrialMode
     double beta;
@@ -208,9 +212,12 @@ __host__ __device__ inline bool premark(const McArgs& a) {
     return a.K == 1 && (a.mode == kTrialMetropolis || a.mode == kTrialForceAccept);
 }
 
-// a batch of at most 64 trials moves its particles in the last block of
-// k_apply (one wave ranks the movers); other steps move them in the plan
-__host__ __device__ inline bool apply_relocates(const McArgs& a) { return a.mode == kTrialBatch && a.K <= 64; }
+// a batch of at most 64 trials moves its particles in the first block of
+// k_apply (one wave ranks the movers), unless its scan does (scan_reloc);
+// other steps move them in the plan
+__host__ __device__ inline bool apply_relocates(const McArgs& a) {
+    return a.mode == kTrialBatch && a.K <= 64 && !a.scan_reloc;
+}
 
 // the batch's window pair terms come from the Gram blocks of k_trial
 template <typename T>
@@ -254,6 +261,12 @@ void launch_decide(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEven
 template <typename T>
 void launch_setup(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent_t e0 = nullptr,
                   hipEvent_t e1 = nullptr);
+// the update of a batched step (a, scan_reloc set) and the set-up of the next
+// (an: its trials' state in the other buffers of the pair; prop: the
+// proposal kernel's set-up, else the uniform one) in one launch
+template <typename T>
+void launch_apply_next(const DevTree<T>& t, const McArgs& a, const McArgs& an, bool prop, hipStream_t st,
+                       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // features at the origins and, for the learned proposal, the displacements;
 // then the set-up of launch_setup, in one kernel
 template <typename T>

@@ -21,6 +21,8 @@
 // arguments are formed in FP64 and every reduction accumulates in FP64.
 #include <hip/hip_ext.h>
 
+#include <stdexcept>
+
 #include "dmk_device.hpp"
 #include "dmk_kernels.hpp"
 
@@ -268,10 +270,10 @@ __device__ __forceinline__ void load_next_move(const DevTree<T>& t, const McArgs
 }
 
 template <typename T>
-__device__ __forceinline__ void setup_body(const DevTree<T>& t, const McArgs& a) {
+__device__ __forceinline__ void setup_body(const DevTree<T>& t, const McArgs& a, int tr) {
     __shared__ TrialInfo ti;
     if (*t.err) return;
-    const int tr = blockIdx.x, m = move_base(a) + tr;
+    const int m = move_base(a) + tr;
     if (tr >= keff(a)) return;
     if (a.ring) {  // a run's first move (or the next after a compaction) and its successor's particle
         if (threadIdx.x == 64 && m + 1 < a.n_moves) {
@@ -292,7 +294,7 @@ __device__ __forceinline__ void setup_body(const DevTree<T>& t, const McArgs& a)
 
 template <typename T>
 __global__ void __launch_bounds__(TB) k_setup(DevTree<T> t, McArgs a) {
-    setup_body(t, a);
+    setup_body(t, a, blockIdx.x);
 }
 
 // one trial per replica: replica blockIdx.z
@@ -300,25 +302,37 @@ template <typename T>
 __global__ void __launch_bounds__(TB) k_setup_rep(const DevTree<T>* __restrict__ ts, const McArgs* __restrict__ as) {
     const DevTree<T>& t = ts[blockIdx.z];
     const McArgs& a = as[blockIdx.z];
-    setup_body(t, a);
+    setup_body(t, a, blockIdx.x);
 }
 
 // ------------------------------------------------------------------ propose
+// LDS of the proposal (propose_body's big, in bytes): the packed weights,
+// the biases, then the near leaves' image shifts of the neighbour pass, whose
+// space the feature partial sums take over for the reduction
+constexpr int kProposeWpsBytes = 2 * kPackSize;
+constexpr int kProposeBshBytes = (4 * kBiasPack + 15) / 16 * 16;
+constexpr int kProposeLds = kProposeWpsBytes + kProposeBshBytes + 8 * TB * (kFeatSums > 3 ? kFeatSums : 3);
+static_assert(kProposeWpsBytes % 16 == 0, "16-byte aligned regions");
+
 // Features of each trial's particle at its origin (particle removed) and, for
 // the learned proposal, its displacement delta = mu + sigma z and log q;
 // into ti (shared, by thread 0) the trial's particle, origin and
 // displacement, and (by thread 64, while the features are summed) the
 // origin's root-to-leaf path.  One block per trial; returns false (uniformly)
-// when the block has no trial.
+// when the block has no trial (tr: the trial of this block).
 template <typename T>
-__device__ __forceinline__ bool propose_body(const DevTree<T>& t, const McArgs& a, TrialInfo& ti) {
+__device__ __forceinline__ bool propose_body(const DevTree<T>& t, const McArgs& a, TrialInfo& ti, int tr, char* big) {
     __shared__ double phi[kFeat], mu[3], sig[3];
-    // the learned network's packed weights and biases, staged while the features are summed
-    __shared__ __align__(16) unsigned short wps[kPackSize];
-    __shared__ float bsh[kBiasPack];
+    // the learned network's packed weights and biases, staged while the
+    // features are summed, and the near leaves' image shifts, whose space the
+    // feature partial sums take over after the neighbour pass (kProposeLds bytes of big)
+    unsigned short* wps = reinterpret_cast<unsigned short*>(big);
+    float* bsh = reinterpret_cast<float*>(big + kProposeWpsBytes);
+    double (*rsh)[3] = reinterpret_cast<double (*)[3]>(big + kProposeWpsBytes + kProposeBshBytes);
+    double (*fpart)[TB] = reinterpret_cast<double (*)[TB]>(big + kProposeWpsBytes + kProposeBshBytes);
     if (*t.err) return false;
     const Proposal& P = a.prop;
-    const int tr = blockIdx.x, m = move_base(a) + tr;
+    const int m = move_base(a) + tr;
     if (tr >= keff(a)) return false;
     const int i = a.ml.idx[m];
     const int own = t.slot_of[i], leaf = t.leaf_of[i];
@@ -341,7 +355,6 @@ __device__ __forceinline__ bool propose_body(const DevTree<T>& t, const McArgs& 
     // radius rf: tested TB entries at a time in parallel, the reaching ones
     // compacted in list order; then their particles in one flat parallel loop
     __shared__ int rs0[TB], rn[TB], rpre[TB + 1], wcnt[TB / 64];
-    __shared__ double rsh[TB][3];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int base = 0; base < cnt; base += TB) {
         const int k = base + threadIdx.x;
@@ -406,7 +419,7 @@ __device__ __forceinline__ bool propose_body(const DevTree<T>& t, const McArgs& 
     if (P.learned) mlp_stage(P, wps, bsh);
     // the feature sums: every thread's partial sums through LDS, kFeatSums x 8
     // threads add 32 each, kFeatSums threads the eight; then one thread per feature
-    __shared__ double fpart[kFeatSums][TB], fp8[kFeatSums][8], fsum[kFeatSums];
+    __shared__ double fp8[kFeatSums][8], fsum[kFeatSums];
 #pragma unroll
     for (int c = 0; c < kFeatSums; ++c) fpart[c][threadIdx.x] = fs[c];
     __syncthreads();
@@ -447,11 +460,11 @@ __device__ __forceinline__ bool propose_body(const DevTree<T>& t, const McArgs& 
 // the set-up completes the trial state the proposal began (the target and its
 // path), then forms the phase rows (src/energy_shift.cpp:18-51)
 template <typename T>
-__global__ void __launch_bounds__(TB) k_propose_setup(DevTree<T> t, McArgs a) {
+__device__ __forceinline__ void propose_setup_body(const DevTree<T>& t, const McArgs& a, int tr, char* big) {
     __shared__ TrialInfo ti;
-    if (!propose_body(t, a, ti)) return;
+    if (!propose_body(t, a, ti, tr, big)) return;
     __syncthreads();
-    const int tr = blockIdx.x, m = move_base(a) + tr, sl = trial_slot(a, tr, m);
+    const int m = move_base(a) + tr, sl = trial_slot(a, tr, m);
     if (threadIdx.x == 0) {
         for (int c = 0; c < 3; ++c) ti.t[c] = wrap(ti.o[c] + ti.dv[c], t.L);
         ti.lam_t = locate(t, ti.t[0], ti.t[1], ti.t[2], ti.path_t);
@@ -463,6 +476,12 @@ __global__ void __launch_bounds__(TB) k_propose_setup(DevTree<T> t, McArgs a) {
     }
     __syncthreads();
     form_rows(t, ti, tabs_of<T>(a.s, table_stride(t.max_depth), sl));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TB) k_propose_setup(DevTree<T> t, McArgs a) {
+    __shared__ __align__(16) char big[kProposeLds];
+    propose_setup_body(t, a, blockIdx.x, big);
 }
 
 // ------------------------------------------------------------------ update plan
@@ -1659,6 +1678,11 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
     __shared__ int accf[64], wtf[TB / 64], wtb[TB / 64];
     __shared__ unsigned short pmask[4][4][64];  // (mask kind, quarter of the earlier trials, trial)
     __shared__ double pabs[4][64];
+    // particle moves made here (scan_reloc, K <= 64): per trial the target,
+    // the particle's slot and charge and its target leaf's first slot and
+    // count, loaded by the second wave while the first walks the batch
+    __shared__ double rv_t[64][3], rv_q[64];
+    __shared__ int rv_s[64], rv_start[64], rv_cnt[64];
     if (*t.err) return;
     const int K = a.K;             // stride of the pair tables
     const int base = move_base(a);
@@ -1766,6 +1790,21 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
         pmask[3][q][j] = (unsigned short)r_lf;
         pabs[q][j] = sa;
         __syncthreads();
+    }
+    const bool rl = a.scan_reloc && K <= 64 && a.mode == kTrialBatch;
+    if (rl && threadIdx.x >= 64 && threadIdx.x < 128) {
+        const int j = threadIdx.x - 64;
+        if (j < Ke) {
+            const TrialInfo& ti = a.s.info[j];
+            const int lt = leafT[j], s = t.slot_of[pids[j]];
+            rv_t[j][0] = ti.t[0];
+            rv_t[j][1] = ti.t[1];
+            rv_t[j][2] = ti.t[2];
+            rv_start[j] = t.slot_start[lt];
+            rv_cnt[j] = t.slot_count[lt];
+            rv_s[j] = s;
+            rv_q[j] = t.pq[s];
+        }
     }
     if (threadIdx.x < 64 && in_lds) {
         // Lane j owns trial j and holds its column of pair corrections in
@@ -2015,11 +2054,52 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
             probe_add(a.s, kPrPlanMark, tp);
         }
         __syncthreads();
-        plan_update<T, true>(t, a, K, !apply_relocates(a));
+        plan_update<T, true>(t, a, K, !apply_relocates(a) && !rl);
     } else {
-        plan_update(t, a, K, !apply_relocates(a));
+        plan_update(t, a, K, !apply_relocates(a) && !rl);
     }
     if (dyn && threadIdx.x == 0) *a.move_ctr = base + nc;
+    if (rl && threadIdx.x < 64) {
+        // the accepted moves, last, so that no barrier waits for their
+        // stores (src/tree.cpp:454-463, 583-596): a move inside its leaf
+        // rewrites the position in place, a mover takes the next free slot of
+        // its target leaf in move order (its rank among the batch's movers
+        // into that leaf) and leaves a hole behind; the first mover into a
+        // leaf writes the count
+        const int m = threadIdx.x;
+        const bool in = m < nc && accs[m] == 1;
+        const int lt = in ? leafT[m] : -1;
+        const bool mover = in && leafO[m] != lt;
+        const int key = mover ? lt : -1;
+        int rank = 0, total = 0;
+#pragma unroll 4
+        for (int k = 0; k < 64; ++k) {
+            const int kk = __builtin_amdgcn_readlane(key, k);
+            if (mover && kk == key) {
+                rank += k < m;
+                ++total;
+            }
+        }
+        if (in) {
+            const int s = rv_s[m], i = pids[m];
+            const int ns = mover ? rv_start[m] + rv_cnt[m] + rank : s;
+            t.px[ns] = rv_t[m][0];
+            t.py[ns] = rv_t[m][1];
+            t.pz[ns] = rv_t[m][2];
+            if (mover) {
+                t.pq[ns] = rv_q[m];
+                t.porig[ns] = i;
+                t.slot_of[i] = ns;
+                t.leaf_of[i] = lt;
+                t.px[s] = 0;
+                t.py[s] = 0;
+                t.pz[s] = 0;
+                t.pq[s] = 0;
+                t.porig[s] = -1;
+                if (rank == 0) t.slot_count[lt] = rv_cnt[m] + total;
+            }
+        }
+    }
 }
 
 // Packs the live particles of every leaf (porig >= 0) to the front of its
@@ -2136,7 +2216,7 @@ __host__ __device__ inline size_t apply_shmem(int dmax, int K) {
 }
 
 template <typename T, int XMAX, bool RB = false>
-__device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a) {
+__device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a, int lead = 0) {
     extern __shared__ char apply_dyn[];
     __shared__ int mlist[2 * kMaxBatch];     // contributions: move << 8 | endpoint << 7 | image code
     __shared__ int mrow[3 * 2 * kMaxBatch];  // their three table rows (x, y, z), offsets into s.tabs
@@ -2150,12 +2230,13 @@ __device__ __forceinline__ void apply_body(const DevTree<T>& t, const McArgs& a)
     // rewrite arrays; a separate instantiation, so that the single-move update
     // compiles without it
     if constexpr (RB) {
-        if (blockIdx.x == 0) {
+        if ((int)blockIdx.x == lead) {
             relocate_batch(t, a);
             return;
         }
     }
-    const int bid = blockIdx.x - (RB ? 1 : 0), nblk = gridDim.x - (RB ? 1 : 0);
+    // blocks before lead have another role (the next step's set-up)
+    const int bid = (int)blockIdx.x - lead - (RB ? 1 : 0), nblk = (int)gridDim.x - lead - (RB ? 1 : 0);
     const int K = a.K, D = t.max_depth, nL = n_levels_pw(D), stride = table_stride(D);
     constexpr int LPR = apply_lpr<T>();  // lanes per grid row
     constexpr int RPI = TB / LPR;        // grid rows per work item
@@ -2396,7 +2477,7 @@ __host__ __device__ inline size_t apply_shmem_mf(int dmax, int K) {
            sizeof(int) * 2 * (size_t)K * kApplyLv;
 }
 template <bool RB>
-__device__ __forceinline__ void apply_body_mf(const DevTree<float>& t, const McArgs& a) {
+__device__ __forceinline__ void apply_body_mf(const DevTree<float>& t, const McArgs& a, int lead = 0) {
     extern __shared__ char apply_dyn[];
     __shared__ int mlist[2 * kMaxBatch];     // contributions: move << 8 | endpoint << 7 | image code
     __shared__ int mrow[3 * 2 * kMaxBatch];  // their three table rows (x, y, z), offsets into s.tabs
@@ -2407,12 +2488,13 @@ __device__ __forceinline__ void apply_body_mf(const DevTree<float>& t, const McA
     const int err = *t.err, nD = a.s.plan[0], nacc = a.s.plan[1], nH = a.s.plan[2];
     if (err || !nacc) return;
     if constexpr (RB) {
-        if (blockIdx.x == 0) {
+        if ((int)blockIdx.x == lead) {
             relocate_batch(t, a);
             return;
         }
     }
-    const int bid = blockIdx.x - (RB ? 1 : 0), nblk = gridDim.x - (RB ? 1 : 0);
+    // blocks before lead have another role (the next step's set-up)
+    const int bid = (int)blockIdx.x - lead - (RB ? 1 : 0), nblk = (int)gridDim.x - lead - (RB ? 1 : 0);
     const int K = a.K, D = t.max_depth, nL = n_levels_pw(D), stride = table_stride(D);
     constexpr int RPI = kApplyMfRows;
     const int dmax = t.dw > t.dd ? t.dw : t.dd;
@@ -2726,6 +2808,31 @@ __global__ void __launch_bounds__(TB, (apply_min_waves<T, XMAX>())) k_apply(DevT
         apply_body_mf<RB>(t, a);
     else
         apply_body<T, XMAX, RB>(t, a);
+}
+
+// The update of batched step b and the set-up of step b + 1 in one launch
+// (chains whose scan moved the accepted particles, a.scan_reloc): blocks
+// [0, an.K) set up the next step's trials, the others rewrite the arrays of
+// step b, the two sets running side by side instead of one after the other.
+// The set-up reads the particles and the leaves' slots, which the scan left
+// final, and writes the next step's trial state (an.s.info, an.s.tabs: the
+// other buffers of the pair) and its moves' records; the update reads step
+// b's trial state, its plan and the expansion arrays.  PROP: the proposal's
+// set-up (features, the learned network), else the uniform one.
+template <typename T, int XMAX, bool PROP>
+__global__ void __launch_bounds__(TB, (apply_min_waves<T, XMAX>())) k_apply_next(DevTree<T> t, McArgs a, McArgs an) {
+    extern __shared__ __align__(16) char next_dyn[];  // the update's dynamic LDS, or the proposal's
+    if ((int)blockIdx.x < an.K) {
+        if constexpr (PROP)
+            propose_setup_body(t, an, blockIdx.x, next_dyn);
+        else
+            setup_body(t, an, blockIdx.x);
+        return;
+    }
+    if constexpr (apply_uses_mf<T, XMAX>())
+        apply_body_mf<false>(t, a, an.K);
+    else
+        apply_body<T, XMAX, false>(t, a, an.K);
 }
 
 // ------------------------------------------------------------------ fused single-move update
@@ -3278,6 +3385,34 @@ void launch_apply(const DevTree<T>& t, const McArgs& a, hipStream_t st, hipEvent
 }
 
 template <typename T>
+void launch_apply_next(const DevTree<T>& t, const McArgs& a, const McArgs& an, bool prop, hipStream_t st,
+                       hipEvent_t e0, hipEvent_t e1) {
+    if (apply_relocates(a) || a.ring || !an.scan_reloc)
+        throw std::logic_error("launch_apply_next: the scans of the chain must move the particles");
+    const int dmax = t.dw > t.dd ? t.dw : t.dd;
+    const bool small = apply_xmax<T>(dmax) <= kApplyXSmall<T>;
+    const bool mf = small && apply_uses_mf<T, kApplyXSmall<T>>();
+    size_t sh = mf ? apply_shmem_mf(dmax, a.K) : apply_shmem<T>(dmax, a.K);
+    if (prop && sh < (size_t)kProposeLds) sh = kProposeLds;
+    const int nb = an.K + a.tiles.apply;
+    auto go = [&](auto kernel) {
+        allow_shmem((const void*)kernel, sh);
+        hipExtLaunchKernelGGL(kernel, dim3(nb), dim3(TB), sh, st, e0, e1, 0, t, a, an);
+    };
+    if (small) {
+        if (prop)
+            go(k_apply_next<T, kApplyXSmall<T>, true>);
+        else
+            go(k_apply_next<T, kApplyXSmall<T>, false>);
+    } else {
+        if (prop)
+            go(k_apply_next<T, kApplyXLarge<T>, true>);
+        else
+            go(k_apply_next<T, kApplyXLarge<T>, false>);
+    }
+}
+
+template <typename T>
 void launch_compact(const DevTree<T>& t, const int* leaves, int n_leaves, hipStream_t st) {
     if (n_leaves > 0) hipLaunchKernelGGL(k_compact<T>, dim3(n_leaves), dim3(TB), 0, st, t, leaves);
 }
@@ -3314,6 +3449,8 @@ void launch_replica_compact(const ReplicaLaunch<T>& r, hipStream_t st) {
     template void launch_pair_corr<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t); \
     template void launch_scan<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t);      \
     template void launch_apply<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t); \
+    template void launch_apply_next<T>(const DevTree<T>&, const McArgs&, const McArgs&, bool, hipStream_t, hipEvent_t, \
+                                       hipEvent_t);                                                          \
     template void launch_replica_step<T>(const ReplicaLaunch<T>&, hipStream_t);                            \
     template void launch_replica_compact<T>(const ReplicaLaunch<T>&, hipStream_t);
 INSTANTIATE(double)

@@ -296,12 +296,14 @@ Handle* create_handle(HPDMKParams params, int precision, int n_src, const double
     h->sc.part = dalloc<double>(kPartStride * (size_t)kMaxBatch * max_trial_blocks(D), o);
     h->sc.rec = dalloc<TrialRec>(kMaxBatch, o);
     h->sc.info = dalloc<TrialInfo>(kMaxBatch, o);
+    h->info_alt = dalloc<TrialInfo>(kMaxBatch, o);
     h->sc.accept = dalloc<int>(kMaxBatch, o);
     h->sc.corr = dalloc<double>(kMaxBatch * kMaxBatch, o);
     h->sc.gram = precision == HPDMK_FP64 ? nullptr : dalloc<double>((size_t)kGramMaxK * kGramMaxK * gram_blocks(Mw), o);
     h->sc.ovl = dalloc<unsigned char>(kMaxBatch * kMaxBatch, o);
     const size_t tsz = precision == HPDMK_FP64 ? sizeof(Cplx<double>) : sizeof(Cplx<float>);
     h->sc.tabs = dalloc<char>(tsz * (size_t)table_stride(D) * kMaxBatch, o);
+    h->tabs_alt = dalloc<char>(tsz * (size_t)table_stride(D) * kMaxBatch, o);
     if (2 * (size_t)h->n_pw > 32 * (size_t)kMarkWords)
         throw HpdmkError(HPDMK_EINVAL, "tree has more expansion boxes than the update planner's bitmap holds");
     h->sc.dest = dalloc<int>(2 * (size_t)h->n_pw + 2, o);
@@ -403,6 +405,19 @@ McArgs mc_args(Handle* h, const MoveBuf& b, int move0, int K, int mode, double b
 
 // Runs the loaded move list as a Metropolis chain (mode 0 plain, 1 batched;
 // batch 0 = the tuned batch size); returns the GPU time in ms.
+// a batched step's arguments with its trial state in buffer pair p (0: the
+// scratch's own, 1: the alternate): consecutive steps alternate, so that a
+// step's set-up can run beside the previous step's update, whose scan moves
+// the particles
+static McArgs alt_args(const Handle* h, McArgs a, int p) {
+    if (p) {
+        a.s.info = h->info_alt;
+        a.s.tabs = h->tabs_alt;
+    }
+    a.scan_reloc = 1;
+    return a;
+}
+
 double run_chain(Handle* h, double beta, int mode, int batch) {
     check_beta(beta);
     MoveBuf& mb = h->chain;
@@ -441,6 +456,8 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
     HIPX(hipEventCreate(&e1.e));
     HIPX(hipEventRecord(e0.e, h->stream));
     dispatch(h, [&](auto& dv) {
+        // whether the next call of compact_if_due(moved) compacts
+        auto compact_due = [&](int moved) { return h->since_compact + moved >= kCompactEvery; };
         auto compact_if_due = [&](int moved) {
             h->since_compact += moved;
             if (h->since_compact >= kCompactEvery) {
@@ -569,21 +586,38 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
             // each round of launches
             if (!h->step_ctr) h->step_ctr = dalloc<int>(1, h->owned);
             HIPX(hipMemsetAsync(h->step_ctr, 0, sizeof(int), h->stream));
-            int done = 0;
+            int done = 0, par = 0;
+            bool ready = false;  // the step's set-up ran beside the previous update
             while (done < n) {
                 const int steps = (n - done + batch - 1) / batch;
                 for (int k = 0; k < steps; ++k) {
-                    McArgs a = mc_args(h, mb, 0, batch, kTrialBatch, beta);
-                    a.move_ctr = h->step_ctr;
-                    a.n_moves = n;
-                    timed(h, kKPropose,
-                          [&](hipEvent_t x, hipEvent_t y) { launch_propose_setup(dv, a, h->stream, x, y); });
+                    auto step_args = [&](int p) {
+                        McArgs s = alt_args(h, mc_args(h, mb, 0, batch, kTrialBatch, beta), p);
+                        s.move_ctr = h->step_ctr;
+                        s.n_moves = n;
+                        return s;
+                    };
+                    const McArgs a = step_args(par);
+                    if (!ready)
+                        timed(h, kKPropose,
+                              [&](hipEvent_t x, hipEvent_t y) { launch_propose_setup(dv, a, h->stream, x, y); });
                     timed(h, kKTrial, [&](hipEvent_t x, hipEvent_t y) { launch_trial(dv, a, h->stream, x, y); });
                     // record blocks (dE, threshold with the reverse-move log-ratio), then the pairs
                     timed(h, kKCorr, [&](hipEvent_t x, hipEvent_t y) { launch_pair_corr(dv, a, h->stream, x, y); });
                     timed(h, kKScan, [&](hipEvent_t x, hipEvent_t y) { launch_scan(dv, a, h->stream, x, y); });
-                    timed(h, kKUpdate, [&](hipEvent_t x, hipEvent_t y) { launch_apply(dv, a, h->stream, x, y); });
+                    // the next step's proposals beside this update, unless a
+                    // slot compaction comes between them (the last step of
+                    // the chain sets up no trial: its counter is at the end)
+                    ready = !compact_due(batch);
+                    if (ready) {
+                        const McArgs an = step_args(par ^ 1);
+                        timed(h, kKUpdate,
+                              [&](hipEvent_t x, hipEvent_t y) { launch_apply_next(dv, a, an, true, h->stream, x, y); });
+                    } else {
+                        timed(h, kKUpdate, [&](hipEvent_t x, hipEvent_t y) { launch_apply(dv, a, h->stream, x, y); });
+                    }
                     compact_if_due(batch);
+                    par ^= 1;
                 }
                 HIPX(hipMemcpyAsync(&done, h->step_ctr, sizeof(int), hipMemcpyDeviceToHost, h->stream));
                 HIPX(hipStreamSynchronize(h->stream));
@@ -592,20 +626,37 @@ double run_chain(Handle* h, double beta, int mode, int batch) {
                 if (err) break;  // reported by check_err below
             }
         } else {
-            for (auto& bt : batches) {
-                const int m0 = bt.first, k = bt.second;
-                const McArgs a = mc_args(h, mb, m0, k, kTrialBatch, beta);
-                if (a.prop.rf > 0.0)  // features only (uniform proposals)
-                    timed(h, kKPropose,
-                          [&](hipEvent_t x, hipEvent_t y) { launch_propose_setup(dv, a, h->stream, x, y); });
-                else
-                    timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
+            int par = 0;
+            bool ready = false;  // the batch's set-up ran beside the previous update
+            for (size_t b = 0; b < batches.size(); ++b) {
+                const int m0 = batches[b].first, k = batches[b].second;
+                const McArgs a = alt_args(h, mc_args(h, mb, m0, k, kTrialBatch, beta), par);
+                const bool feat = a.prop.rf > 0.0;  // features only (uniform proposals)
+                if (!ready) {
+                    if (feat)
+                        timed(h, kKPropose,
+                              [&](hipEvent_t x, hipEvent_t y) { launch_propose_setup(dv, a, h->stream, x, y); });
+                    else
+                        timed(h, kKSetup, [&](hipEvent_t x, hipEvent_t y) { launch_setup(dv, a, h->stream, x, y); });
+                }
                 timed(h, kKTrial, [&](hipEvent_t x, hipEvent_t y) { launch_trial(dv, a, h->stream, x, y); });
                 // the record blocks run for a batch of one too (k_scan reads its record)
                 timed(h, kKCorr, [&](hipEvent_t x, hipEvent_t y) { launch_pair_corr(dv, a, h->stream, x, y); });
                 timed(h, kKScan, [&](hipEvent_t x, hipEvent_t y) { launch_scan(dv, a, h->stream, x, y); });
-                timed(h, kKUpdate, [&](hipEvent_t x, hipEvent_t y) { launch_apply(dv, a, h->stream, x, y); });
+                // the next batch's set-up beside this update, unless a slot
+                // compaction comes between them
+                ready = b + 1 < batches.size() && !compact_due(k);
+                if (ready) {
+                    const McArgs an =
+                        alt_args(h, mc_args(h, mb, batches[b + 1].first, batches[b + 1].second, kTrialBatch, beta),
+                                 par ^ 1);
+                    timed(h, kKUpdate,
+                          [&](hipEvent_t x, hipEvent_t y) { launch_apply_next(dv, a, an, feat, h->stream, x, y); });
+                } else {
+                    timed(h, kKUpdate, [&](hipEvent_t x, hipEvent_t y) { launch_apply(dv, a, h->stream, x, y); });
+                }
                 compact_if_due(k);
+                par ^= 1;
             }
         }
     });

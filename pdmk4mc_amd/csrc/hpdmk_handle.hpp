@@ -111,6 +111,10 @@ struct Handle {
     bool mfma_ready = false;
     // Monte Carlo scratch (kMaxBatch trials)
     McScratch sc{};
+    // the other trial-state buffers (TrialInfo, phase rows) of batched
+    // chains, whose consecutive steps alternate between the two
+    TrialInfo* info_alt = nullptr;
+    void* tabs_alt = nullptr;
     unsigned long long* probe = nullptr;
     // move lists in HBM: `single` serves eval/update calls, `chain` the staged Metropolis run
     MoveBuf single, chain;
