@@ -477,8 +477,9 @@ def main():
     if k_batch > 1:
         work["pair_corr"] = ("fp32" if prec == P.HPDMK_MIXED else "fp64",
                              pair_corr_flops(st_p, prof_moves, k_batch, nw, nd, gram))
-    symbol = {"trial": "k_trial", "update": "k_apply", "pair_corr": "k_pair_corr", "scan": "k_scan",
-              "setup": "k_setup", "records": "k_records", "propose": "k_propose"}
+    # the update kernel of a batched step also sets up the next step
+    symbol = {"trial": "k_trial", "update": "k_apply_next", "pair_corr": "k_pair_corr", "scan": "k_scan",
+              "setup": "k_setup", "records": "k_records", "propose": "k_propose_setup"}
     per_kernel = {}
     for k, (bound, w) in work.items():
         ms_k, n_k = kt[k]
@@ -502,7 +503,10 @@ def main():
     if pk is None:  # a kernel without an algorithmic-work figure (decisions): report the update
         dom = "update"
         pk = per_kernel["update"]
-    traffic = measured_traffic(args.traffic_file, dom)
+    # the counter summary holds the learned chain's kernels under their
+    # names and the uniform chain's update and scan under name_uniform
+    traffic = measured_traffic(args.traffic_file, dom if learned_w is not None or dom not in ("update", "scan")
+                               else dom + "_uniform")
     traffic_src = None
     if traffic is not None:
         try:
@@ -517,6 +521,9 @@ def main():
                 "achieved": pk["achieved"], "peak": pk["peak"], "unit": pk["unit"], "frac": pk["frac"],
                 "traffic": traffic, "traffic_source": traffic_src,
                 "chain": "batched + learned (headline)" if learned_w is not None else "batched uniform", "work_per_launch": pk["work_per_launch"], "avg_launch_us": pk["avg_launch_us"],
+                "work_note": "update: distinct expansion-array bytes the batch rewrites (read + write); the launch "
+                             "also sets up the next step's trials, whose bytes are not counted"
+                             if dom == "update" else None,
                 "kernel_ms": {k: [round(v[0], 3), v[1]] for k, v in kt.items()}, "per_kernel": per_kernel,
                 "update_bytes_per_launch_per_move_sum": int(update_bytes_per_move_sum / max(kt["update"][1], 1)),
                 "window_gram_mflop_per_launch": round(gram_mflop(k_batch, Mw), 2) if gram else None,
