@@ -136,6 +136,27 @@ def test_batched_chain_equals_plain(precision, batch):
         assert_same_chain(dE1, acc1, dE2, acc2, thresholds(u), MIXED_DE_BOUND)
 
 
+@pytest.mark.parametrize("batch", [64, 128])
+def test_batched_chain_equals_plain_across_compactions(batch):
+    """5000 moves: the slot compactions (every 2048 moves) fall between
+    batches, so the run alternates between updates that set up the next batch
+    in the same launch and updates followed by a compaction and a set-up of
+    its own; batches over 64 move their particles in the plan."""
+    log("batched vs plain across compactions", batch)
+    r, q, g1 = make(2000, 40, 6, P.HPDMK_FP64, seed=9)
+    _, _, g2 = make(2000, 40, 6, P.HPDMK_FP64, seed=9)
+    idx, d, u = _moves(2000, 5000, 13)
+    dE1, acc1, _ = g1.mc_chain(idx, d, u, beta=1.0, mode=0)
+    dE2, acc2, _ = g2.mc_chain(idx, d, u, beta=1.0, mode=1, batch=batch)
+    assert (acc1 == acc2).all()
+    assert np.abs(dE1 - dE2).max() < 1e-9
+    assert np.abs(g1.positions() - g2.positions()).max() < 1e-12
+    assert np.abs(g1.energy_parts() - g2.energy_parts()).max() < 1e-9
+    assert 0.2 < acc2.mean() < 0.95
+    g1.close()
+    g2.close()
+
+
 @pytest.mark.parametrize("precision", [P.HPDMK_FP64, P.HPDMK_MIXED])
 def test_batched_chain_equals_plain_digits3(precision):
     """Digits 3: smaller window and difference grids (the Gram blocks' mode

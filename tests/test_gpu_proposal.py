@@ -173,6 +173,37 @@ def test_batched_learned_chain_equals_plain(precision, batch, spheres):
         assert np.abs(H1[:k] - H2[:k]).max(initial=0.0) < MIXED_DE_BOUND
 
 
+def test_batched_learned_chain_equals_plain_across_compactions():
+    """5000 learned moves on hard spheres (FP64): slot compactions between
+    batches and several host rounds of steps (the stale-proposal cuts leave
+    moves for later rounds, whose first step was set up by the last update
+    of the round before)."""
+    log("batched learned across compactions")
+    r, q, L = hard_sphere_system(2000, 0.2, 1.0, seed=11)
+    w = P.mlp_init(7, scale=1.0)
+    rng = np.random.default_rng(14)
+    m = 5000
+    idx, z, u = rng.integers(0, 2000, m), rng.normal(size=(m, 3)), rng.uniform(0, 1, m)
+    runs = []
+    for mode in (0, 1):
+        g = P.HPDMKPtTree(P.HPDMKParams(16, 6, L, init=P.DIRECT), r, q, P.HPDMK_FP64)
+        g.form_outgoing_pw()
+        g.form_incoming_pw()
+        g.mc_set_model(2.0, 1.0)
+        g.mc_set_proposal(learned=True, weights=w, rf=2.0, **CFG)
+        dE, acc, _ = g.mc_chain(idx, z, u, beta=1.0, mode=mode, batch=64)
+        dl, _, H = g.mc_proposal_results()
+        runs.append((dE, acc, dl, H, g.positions()))
+        g.close()
+    (dE1, a1, d1, H1, x1), (dE2, a2, d2, H2, x2) = runs
+    assert (a1 == a2).all()
+    assert np.abs(d1 - d2).max() < 1e-12 and np.abs(H1 - H2).max() < 1e-9
+    fin = np.isfinite(dE1)
+    assert (np.isfinite(dE2) == fin).all() and np.abs(dE1[fin] - dE2[fin]).max() < 1e-9
+    assert np.abs(x1 - x2).max() < 1e-12
+    assert 0.05 < a1.mean() < 0.95
+
+
 def test_policy_gradient_training_raises_expected_jump():
     log("policy gradient")
     r, q, L = hard_sphere_system(3000, 0.2, 1.0, seed=13)
