@@ -233,12 +233,13 @@ __host__ __device__ inline int gram_rows(const DevTree<T>& t, const McArgs& a) {
     return (gram_blocks(t.Mw) + tb - 1) / tb;
 }
 
-// One Metropolis step of every replica of a set (one move each): per-replica
+// Metropolis steps of every replica of a set (one move each): per-replica
 // device state and arguments live in device arrays indexed by blockIdx.z.
 template <typename T>
 struct ReplicaLaunch {
     const DevTree<T>* trees;  // n
     const McArgs* args;       // n (move_ctr set)
+    const McArgs* args_alt;   // n: the same with each replica's other trial-state buffers
     int* move_ctr;            // n step counters
     const int* const* leaves; // n leaf lists
     const int* n_leaves;      // n
@@ -247,8 +248,12 @@ struct ReplicaLaunch {
     Tiles tiles;  // trial shape (the same for every replica)
 };
 
+// k consecutive Metropolis steps of every replica (one move each): a set-up
+// first, then per step the trial, the decision and the update, every update
+// but the last setting up the next step in its launch (the steps' trial
+// state alternating between args and args_alt)
 template <typename T>
-void launch_replica_step(const ReplicaLaunch<T>& r, hipStream_t st);
+void launch_replica_steps(const ReplicaLaunch<T>& r, int k, hipStream_t st);
 // move_ctr[0..n) += inc
 void launch_advance(int* move_ctr, int n, hipStream_t st, int inc = 1);
 template <typename T>

@@ -3186,12 +3186,11 @@ __global__ void __launch_bounds__(TB) k_update1(DevTree<T> t, McArgs a) {
 // (conjugated for an incoming array), whose loads were issued first.
 constexpr int kRep1J = 8;                      // elements per thread and item
 constexpr int kRep1Rows = kRep1J * TB / 4 + 2;  // grid rows a chunk spans (grid rows hold >= 5 elements)
+// bx, gx: this block's index and the number of update blocks of the replica
 template <typename T>
-__global__ void __launch_bounds__(TB) k_apply1_rep(const DevTree<T>* __restrict__ ts, const McArgs* __restrict__ as) {
+__device__ __forceinline__ void apply1_rep_body(const DevTree<T>& t, const McArgs& a, int bx, int gx) {
     __shared__ Cplx<T> sx[2][kMaxGridD], syz[2][kRep1Rows];
     __shared__ T ssg[2];
-    const DevTree<T>& t = ts[blockIdx.z];
-    const McArgs& a = as[blockIdx.z];
     if (*t.err) return;
     const int nD = a.s.plan[0], nacc = a.s.plan[1], nH = a.s.plan[2];
     if (!nacc) return;  // uniform: rejected, or no move left
@@ -3201,7 +3200,7 @@ __global__ void __launch_bounds__(TB) k_apply1_rep(const DevTree<T>* __restrict_
     const TrialInfo& ti = a.s.info[0];
     const int D = t.max_depth, nL = n_levels_pw(D), n2 = 2 * t.n_pw_nodes;
     const Cplx<T>* tb = tabs_of<T>(a.s, table_stride(D), 0);
-    for (int it = blockIdx.x; it < total; it += gridDim.x) {
+    for (int it = bx; it < total; it += gx) {
         const bool is_root = it < nR;
         int chunk = it, key = 0;
         if (!is_root) {
@@ -3286,6 +3285,27 @@ __global__ void __launch_bounds__(TB) k_apply1_rep(const DevTree<T>* __restrict_
             if (e < M) dst[e] = {pf[j].x + val.x, pf[j].y + sy * val.y};
         }
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TB) k_apply1_rep(const DevTree<T>* __restrict__ ts, const McArgs* __restrict__ as) {
+    apply1_rep_body(ts[blockIdx.z], as[blockIdx.z], blockIdx.x, gridDim.x);
+}
+
+// The update of every replica's step and the set-up of its next step (asn:
+// the next step's arguments, its trial state in the other buffers of the
+// pair) in one launch: block 0 of each replica's row sets up, the others
+// update.  The decision kernel moved the particle and advanced the move
+// counter, so the set-up reads the next move and final positions.
+template <typename T>
+__global__ void __launch_bounds__(TB) k_apply1_next_rep(const DevTree<T>* __restrict__ ts,
+                                                      const McArgs* __restrict__ as,
+                                                      const McArgs* __restrict__ asn) {
+    if (blockIdx.x == 0) {
+        setup_body(ts[blockIdx.z], asn[blockIdx.z], 0);
+        return;
+    }
+    apply1_rep_body(ts[blockIdx.z], as[blockIdx.z], (int)blockIdx.x - 1, (int)gridDim.x - 1);
 }
 
 // grids up to 26 (float) or 28 (double) points per axis use the narrow variant
@@ -3418,16 +3438,23 @@ void launch_compact(const DevTree<T>& t, const int* leaves, int n_leaves, hipStr
 }
 
 template <typename T>
-void launch_replica_step(const ReplicaLaunch<T>& r, hipStream_t st) {
+void launch_replica_steps(const ReplicaLaunch<T>& r, int k, hipStream_t st) {
     const dim3 rep(1, 1, r.n);
     hipLaunchKernelGGL(k_setup_rep<T>, rep, dim3(TB), 0, st, r.trees, r.args);
-    // one trial row: with 256 replicas the trial kernel fills the chip, so the
-    // decision plans the whole update (a marking row measured slower)
-    hipLaunchKernelGGL(k_trial_rep<T>, dim3(trial_blocks(r.max_depth, r.tiles), 1, r.n), dim3(TB), 0, st, r.trees,
-                       r.args);
-    hipLaunchKernelGGL(k_decide_rep<T>, rep, dim3(TB), 0, st, r.trees, r.args);
-    hipLaunchKernelGGL(k_apply1_rep<T>, dim3(r.apply_blocks, 1, r.n), dim3(TB), 0, st, r.trees, r.args);
-    // the move counters advance in k_decide_rep
+    for (int j = 0; j < k; ++j) {
+        const McArgs* a = (j & 1) ? r.args_alt : r.args;
+        const McArgs* an = (j & 1) ? r.args : r.args_alt;
+        // one trial row: with 256 replicas the trial kernel fills the chip, so the
+        // decision plans the whole update (a marking row measured slower)
+        hipLaunchKernelGGL(k_trial_rep<T>, dim3(trial_blocks(r.max_depth, r.tiles), 1, r.n), dim3(TB), 0, st,
+                           r.trees, a);
+        hipLaunchKernelGGL(k_decide_rep<T>, rep, dim3(TB), 0, st, r.trees, a);  // advances the move counters
+        if (j + 1 < k)
+            hipLaunchKernelGGL(k_apply1_next_rep<T>, dim3(1 + r.apply_blocks, 1, r.n), dim3(TB), 0, st, r.trees,
+                               a, an);
+        else
+            hipLaunchKernelGGL(k_apply1_rep<T>, dim3(r.apply_blocks, 1, r.n), dim3(TB), 0, st, r.trees, a);
+    }
 }
 
 void launch_advance(int* move_ctr, int n, hipStream_t st, int inc) {
@@ -3451,7 +3478,7 @@ void launch_replica_compact(const ReplicaLaunch<T>& r, hipStream_t st) {
     template void launch_apply<T>(const DevTree<T>&, const McArgs&, hipStream_t, hipEvent_t, hipEvent_t); \
     template void launch_apply_next<T>(const DevTree<T>&, const McArgs&, const McArgs&, bool, hipStream_t, hipEvent_t, \
                                        hipEvent_t);                                                          \
-    template void launch_replica_step<T>(const ReplicaLaunch<T>&, hipStream_t);                            \
+    template void launch_replica_steps<T>(const ReplicaLaunch<T>&, int, hipStream_t);                      \
     template void launch_replica_compact<T>(const ReplicaLaunch<T>&, hipStream_t);
 INSTANTIATE(double)
 INSTANTIATE(float)

@@ -30,6 +30,7 @@ struct ReplicaSet {
     std::vector<void*> owned;
     void* trees = nullptr;  // DevTree<T>[n]
     McArgs* args = nullptr;
+    McArgs* args_alt = nullptr;  // the same with each replica's other trial-state buffers
     int* move_ctr = nullptr;
     const int** leaves = nullptr;
     int* n_leaves = nullptr;
@@ -62,6 +63,7 @@ ReplicaLaunch<T> launch_args(ReplicaSet* s) {
     ReplicaLaunch<T> r{};
     r.trees = static_cast<const DevTree<T>*>(s->trees);
     r.args = s->args;
+    r.args_alt = s->args_alt;
     r.move_ctr = s->move_ctr;
     r.leaves = s->leaves;
     r.n_leaves = s->n_leaves;
@@ -125,6 +127,7 @@ int hpdmk_replicas_create(int n_rep, const HPDMKParams* params, int precision, c
         const size_t tsz = precision == HPDMK_FP64 ? sizeof(DevTree<double>) : sizeof(DevTree<float>);
         s->trees = dalloc<char>(tsz * n_rep, s->owned);
         s->args = dalloc<McArgs>(n_rep, s->owned);
+        s->args_alt = dalloc<McArgs>(n_rep, s->owned);
         s->move_ctr = dalloc<int>(n_rep, s->owned);
         s->leaves = (const int**)dalloc<const int*>(n_rep, s->owned);
         s->n_leaves = dalloc<int>(n_rep, s->owned);
@@ -201,6 +204,15 @@ int hpdmk_replicas_mc_run(hpdmk_replicas s, double beta, int graph_steps, double
             args[r].tiles = Tiles{};  // the grid of a replica step uses the default shape
         }
         HIPX(hipMemcpyAsync(set->args, args.data(), sizeof(McArgs) * args.size(), hipMemcpyHostToDevice, set->stream));
+        // consecutive steps alternate between the two trial-state buffers of
+        // each replica (a step's set-up runs in the previous step's update launch)
+        std::vector<McArgs> alt(args);
+        for (int r = 0; r < set->n(); ++r) {
+            alt[r].s.info = set->reps[r]->info_alt;
+            alt[r].s.tabs = set->reps[r]->tabs_alt;
+        }
+        HIPX(hipMemcpyAsync(set->args_alt, alt.data(), sizeof(McArgs) * alt.size(), hipMemcpyHostToDevice,
+                            set->stream));
         HIPX(hipMemsetAsync(set->move_ctr, 0, sizeof(int) * set->n(), set->stream));
         hipEvent_t e0 = nullptr, e1 = nullptr;
         hipGraph_t graph = nullptr;
@@ -214,13 +226,13 @@ int hpdmk_replicas_mc_run(hpdmk_replicas s, double beta, int graph_steps, double
         try {
             HIPX(hipEventCreate(&e0));
             HIPX(hipEventCreate(&e1));
+            // count consecutive steps, self-contained: a set-up first, the
+            // last update without the next step's set-up
             auto step = [&](int count) {
-                for (int k = 0; k < count; ++k) {
-                    if (set->precision == HPDMK_FP64)
-                        launch_replica_step(launch_args<double>(set), set->stream);
-                    else
-                        launch_replica_step(launch_args<float>(set), set->stream);
-                }
+                if (set->precision == HPDMK_FP64)
+                    launch_replica_steps(launch_args<double>(set), count, set->stream);
+                else
+                    launch_replica_steps(launch_args<float>(set), count, set->stream);
             };
             auto compact = [&] {
                 if (set->precision == HPDMK_FP64)
