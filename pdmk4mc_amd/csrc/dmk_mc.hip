@@ -367,7 +367,11 @@ __device__ __forceinline__ bool propose_body(const DevTree<T>& t, const McArgs& 
             sx = (code % 3 - 1) * t.L;
             sy = ((code / 3) % 3 - 1) * t.L;
             sz = (code / 9 - 1) * t.L;
-            const double bs = t.boxsize[t.level[nb]];
+            // the leaf's slots are loaded with its level and cell (used only
+            // when it reaches the ball), and its size is L / 2^level exactly
+            // (the boxsize table's value), so one round of loads decides
+            const int lv = t.level[nb], ss = t.slot_start[nb], sn = t.slot_count[nb];
+            const double bs = ldexp(t.L, -lv);
             const double lo[3] = {t.cell[3 * nb] * bs + sx, t.cell[3 * nb + 1] * bs + sy, t.cell[3 * nb + 2] * bs + sz};
             const double x[3] = {ox, oy, oz};
             double g2 = 0;
@@ -377,8 +381,8 @@ __device__ __forceinline__ bool propose_body(const DevTree<T>& t, const McArgs& 
             }
             reach = g2 < rf2;
             if (reach) {
-                s0 = t.slot_start[nb];
-                n = t.slot_count[nb];
+                s0 = ss;
+                n = sn;
             }
         }
         const unsigned long long bal = __ballot(reach);
