@@ -175,6 +175,25 @@ def test_batched_chain_equals_plain_digits3(precision):
         assert_same_chain(dE1, acc1, dE2, acc2, thresholds(u), MIXED_DE_BOUND)
 
 
+@pytest.mark.parametrize("digits", [9, 12])
+def test_batched_chain_equals_plain_wide_grids(digits):
+    """Digits 9 and 12 (FP64): grids wider than 28 points per axis take the
+    wide instantiation of the update (and of the update that sets up the
+    next batch in its launch)."""
+    log("batched vs plain, wide grids", digits)
+    r, q, g1 = make(2000, 40, digits, P.HPDMK_FP64, seed=21)
+    _, _, g2 = make(2000, 40, digits, P.HPDMK_FP64, seed=21)
+    idx, d, u = _moves(2000, 400, 23)
+    dE1, acc1, _ = g1.mc_chain(idx, d, u, beta=1.0, mode=0)
+    dE2, acc2, _ = g2.mc_chain(idx, d, u, beta=1.0, mode=1, batch=64)
+    assert (acc1 == acc2).all()
+    assert np.abs(dE1 - dE2).max() < 1e-9
+    assert np.abs(g1.positions() - g2.positions()).max() < 1e-12
+    assert np.abs(g1.energy_parts() - g2.energy_parts()).max() < 1e-9
+    g1.close()
+    g2.close()
+
+
 @pytest.mark.parametrize("precision", [P.HPDMK_FP64, P.HPDMK_MIXED])
 def test_batched_chain_single_trial_batches(precision):
     """Moves of one particle in a row cut the batch at every repeat: batches
