@@ -2057,8 +2057,20 @@ __global__ void __launch_bounds__(TB) k_scan(DevTree<T> t, McArgs a) {
             a.s.plan[2] = tf;
             probe_add(a.s, kPrPlanMark, tp);
         }
-        __syncthreads();
-        plan_update<T, true>(t, a, K, !apply_relocates(a) && !rl);
+        if (rl) {
+            // the particles move below: the plan needs only the accepted count
+            if (threadIdx.x < 64) {
+                const unsigned long long acc = __ballot((int)threadIdx.x < nc && accs[threadIdx.x] == 1);
+                if (threadIdx.x == 0) {
+                    const int nacc = __popcll(acc);
+                    a.s.plan[1] = nacc;
+                    if (a.ml.stats && nacc) a.ml.stats[kStatsPerMove * base + 7] = tf + tb;
+                }
+            }
+        } else {
+            __syncthreads();
+            plan_update<T, true>(t, a, K, !apply_relocates(a));
+        }
     } else {
         plan_update(t, a, K, !apply_relocates(a) && !rl);
     }
