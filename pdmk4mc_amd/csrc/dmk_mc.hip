@@ -3202,11 +3202,23 @@ __global__ void __launch_bounds__(TB) k_update1(DevTree<T> t, McArgs a) {
 // (conjugated for an incoming array), whose loads were issued first.
 constexpr int kRep1J = 8;                      // elements per thread and item
 constexpr int kRep1Rows = kRep1J * TB / 4 + 2;  // grid rows a chunk spans (grid rows hold >= 5 elements)
-// bx, gx: this block's index and the number of update blocks of the replica
+// items of a block whose descriptions are formed together (one thread each)
+constexpr int kRep1Pass = 64;
+
+// bx, gx: this block's index and the number of update blocks of the replica.
+// The block's items (bx, bx + gx, ...) are described first, a pass of up to
+// kRep1Pass at a time, one thread per item (the array, the chunk, and per
+// endpoint the table rows at its image shift and the sign), so that the
+// chain of dependent lookups (plan entry, box, cells) is paid once per pass;
+// then every item stages its rows, and the elements of the next item are
+// loaded before the current item's sums are formed and stored.
 template <typename T>
 __device__ __forceinline__ void apply1_rep_body(const DevTree<T>& t, const McArgs& a, int bx, int gx) {
     __shared__ Cplx<T> sx[2][kMaxGridD], syz[2][kRep1Rows];
-    __shared__ T ssg[2];
+    __shared__ int ik[kRep1Pass];       // kind << 28 | slot (kind 2: the root)
+    __shared__ int ie0[kRep1Pass];      // first element of the item's chunk
+    __shared__ int irow[kRep1Pass][6];  // table rows (x, y, z) of the target, then the origin
+    __shared__ int isg[kRep1Pass];      // signs: (sg_t + 1) | (sg_o + 1) << 2
     if (*t.err) return;
     const int nD = a.s.plan[0], nacc = a.s.plan[1], nH = a.s.plan[2];
     if (!nacc) return;  // uniform: rejected, or no move left
@@ -3216,89 +3228,123 @@ __device__ __forceinline__ void apply1_rep_body(const DevTree<T>& t, const McArg
     const TrialInfo& ti = a.s.info[0];
     const int D = t.max_depth, nL = n_levels_pw(D), n2 = 2 * t.n_pw_nodes;
     const Cplx<T>* tb = tabs_of<T>(a.s, table_stride(D), 0);
-    for (int it = bx; it < total; it += gx) {
-        const bool is_root = it < nR;
-        int chunk = it, key = 0;
-        if (!is_root) {
-            const int r = it - nR, arr = r / nA;
-            chunk = r % nA;
-            key = a.s.dest[arr < nH ? arr : n2 - 1 - (arr - nH)];
-        }
-        const int slot = key >> 1, kind = is_root ? 2 : key & 1;
-        const int node = is_root ? 0 : a.pn.node[slot];
-        const int l = is_root ? 0 : a.pn.level[slot];
-        const int d = is_root ? t.dw : t.dd, M = is_root ? t.Mw : t.Md;
-        Cplx<T>* dst = is_root ? t.root : (kind == 0 ? t.out : t.inc) + (size_t)slot * t.Md;
-        const int e0 = chunk * CH;
-        Cplx<T> pf[kRep1J];
-#pragma unroll
-        for (int j = 0; j < kRep1J; ++j) pf[j] = dst[min(e0 + (int)threadIdx.x + TB * j, M - 1)];
-        // rows of endpoint ep (0 target, 1 origin) at the image shift of its
-        // box next to the array's box; sign -1 at the origin, 0 when the
-        // endpoint does not touch the array
-        const Cplx<T>* rx[2];
-        const Cplx<T>* ry[2];
-        const Cplx<T>* rz[2];
-        T sg[2];
-#pragma unroll
-        for (int ep = 0; ep < 2; ++ep) {
-            rx[ep] = tb + root_row(ep, 0);
-            ry[ep] = tb + root_row(ep, 1);
-            rz[ep] = tb + root_row(ep, 2);
-            sg[ep] = is_root ? (ep == 0 ? T(1) : T(-1)) : T(0);
-            if (is_root || l > (ep == 0 ? ti.lam_t : ti.lam_o)) continue;
-            const double* x = ep == 0 ? ti.t : ti.o;
-            const int nl = 1 << l;
-            int sh[3];
-            bool same = true, adj = true;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const int bc = cell_at(x[c], t.L, l);
-                const int cc = t.cell[3 * node + c];
-                same = same && bc == cc;
-                sh[c] = image_shift(bc, cc, nl);
-                adj = adj && sh[c] != 2;
+    const int nk = total > bx ? (total - bx + gx - 1) / gx : 0;  // items of this block
+    auto dst_of = [&](int k) -> Cplx<T>* {
+        const int kind = ik[k] >> 28, slot = ik[k] & 0x0fffffff;
+        return kind == 2 ? t.root : (kind == 0 ? t.out : t.inc) + (size_t)slot * t.Md;
+    };
+    for (int p0 = 0; p0 < nk; p0 += kRep1Pass) {
+        const int np = min(kRep1Pass, nk - p0);
+        __syncthreads();  // the previous pass's descriptions are consumed
+        if ((int)threadIdx.x < np) {
+            const int it = bx + (p0 + (int)threadIdx.x) * gx;
+            const bool is_root = it < nR;
+            int chunk = it, key = 0;
+            if (!is_root) {
+                const int r = it - nR, arr = r / nA;
+                chunk = r % nA;
+                key = a.s.dest[arr < nH ? arr : n2 - 1 - (arr - nH)];
             }
-            if (!(same || (kind == 1 && adj))) continue;
-            const int li = l - 2;
-            rx[ep] = tb + level_row(nL, ep, li, 0, same ? 0 : sh[0]);
-            ry[ep] = tb + level_row(nL, ep, li, 1, same ? 0 : sh[1]);
-            rz[ep] = tb + level_row(nL, ep, li, 2, same ? 0 : sh[2]);
-            sg[ep] = ep == 0 ? T(1) : T(-1);
-        }
-        const int r0 = e0 / d, nr = min((min(e0 + CH, M) - 1) / d - r0 + 1, kRep1Rows);
-        __syncthreads();  // the previous item's staged rows are consumed
-        for (int k = threadIdx.x; k < 2 * (d + nr); k += TB) {
-            const int ep = k & 1, w = k >> 1;
-            if (w < d) {
-                sx[ep][w] = rx[ep][w];
-            } else {
-                const int row = r0 + w - d, y = row % d, z = row / d;
-                Cplx<T> yz = cmul(ry[ep][y], rz[ep][z]);
-                yz.x *= sg[ep];
-                yz.y *= sg[ep];
-                syz[ep][w - d] = yz;
+            const int slot = key >> 1, kind = is_root ? 2 : key & 1;
+            const int node = is_root ? 0 : a.pn.node[slot];
+            const int l = is_root ? 0 : a.pn.level[slot];
+            int rows[6], sgb = 0;
+            // rows of endpoint ep (0 target, 1 origin) at the image shift of
+            // its box next to the array's box; sign -1 at the origin, 0 when
+            // the endpoint does not touch the array
+#pragma unroll
+            for (int ep = 0; ep < 2; ++ep) {
+                int sg = is_root ? (ep == 0 ? 1 : -1) : 0;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) rows[3 * ep + c] = root_row(ep, c);
+                if (!is_root && l <= (ep == 0 ? ti.lam_t : ti.lam_o)) {
+                    const double* x = ep == 0 ? ti.t : ti.o;
+                    const int nl = 1 << l;
+                    int sh[3];
+                    bool same = true, adj = true;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const int bc = cell_at(x[c], t.L, l);
+                        const int cc = t.cell[3 * node + c];
+                        same = same && bc == cc;
+                        sh[c] = image_shift(bc, cc, nl);
+                        adj = adj && sh[c] != 2;
+                    }
+                    if (same || (kind == 1 && adj)) {
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) rows[3 * ep + c] = level_row(nL, ep, l - 2, c, same ? 0 : sh[c]);
+                        sg = ep == 0 ? 1 : -1;
+                    }
+                }
+                sgb |= (sg + 1) << (2 * ep);
             }
+            ik[threadIdx.x] = kind << 28 | slot;
+            ie0[threadIdx.x] = chunk * CH;
+            isg[threadIdx.x] = sgb;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) irow[threadIdx.x][c] = rows[c];
         }
-        if (threadIdx.x < 2) ssg[threadIdx.x] = sg[threadIdx.x];
         __syncthreads();
-        const T sy = kind == 1 ? T(-1) : T(1);  // in[C] += conj(tau s)
+        // the first item's elements
+        Cplx<T> pf[kRep1J];
+        {
+            const Cplx<T>* dst = dst_of(0);
+            const int M = (ik[0] >> 28) == 2 ? t.Mw : t.Md, e0 = ie0[0];
 #pragma unroll
-        for (int j = 0; j < kRep1J; ++j) {
-            const int e = e0 + threadIdx.x + TB * j;
-            const int x = e % d, rr = e / d - r0;
-            Cplx<T> val = {T(0), T(0)};
-#pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                if (ssg[c] == T(0)) continue;
-                const Cplx<T> yz = syz[c][rr < nr ? rr : 0];
-                const Cplx<T> xv = sx[c][x];
-                val.x = fma(xv.x, yz.x, val.x);
-                val.x = fma(-xv.y, yz.y, val.x);
-                val.y = fma(xv.x, yz.y, val.y);
-                val.y = fma(xv.y, yz.x, val.y);
+            for (int j = 0; j < kRep1J; ++j) pf[j] = dst[min(e0 + (int)threadIdx.x + TB * j, M - 1)];
+        }
+        for (int k = 0; k < np; ++k) {
+            const bool is_root = (ik[k] >> 28) == 2;
+            const int kind = ik[k] >> 28;
+            const int d = is_root ? t.dw : t.dd, M = is_root ? t.Mw : t.Md;
+            const int e0 = ie0[k], sgb = isg[k];
+            const T sg[2] = {T((sgb & 3) - 1), T(((sgb >> 2) & 3) - 1)};
+            Cplx<T>* dst = dst_of(k);
+            const int r0 = e0 / d, nr = min((min(e0 + CH, M) - 1) / d - r0 + 1, kRep1Rows);
+            __syncthreads();  // the previous item's staged rows are consumed
+            for (int q = threadIdx.x; q < 2 * (d + nr); q += TB) {
+                const int ep = q & 1, w = q >> 1;
+                if (w < d) {
+                    sx[ep][w] = tb[irow[k][3 * ep] + w];
+                } else {
+                    const int row = r0 + w - d, y = row % d, z = row / d;
+                    Cplx<T> yz = cmul(tb[irow[k][3 * ep + 1] + y], tb[irow[k][3 * ep + 2] + z]);
+                    yz.x *= sg[ep];
+                    yz.y *= sg[ep];
+                    syz[ep][w - d] = yz;
+                }
             }
-            if (e < M) dst[e] = {pf[j].x + val.x, pf[j].y + sy * val.y};
+            __syncthreads();
+            // the next item's elements, in flight while this one is summed and stored
+            Cplx<T> pn[kRep1J];
+            if (k + 1 < np) {
+                const Cplx<T>* dn = dst_of(k + 1);
+                const int Mn = (ik[k + 1] >> 28) == 2 ? t.Mw : t.Md, en = ie0[k + 1];
+#pragma unroll
+                for (int j = 0; j < kRep1J; ++j) pn[j] = dn[min(en + (int)threadIdx.x + TB * j, Mn - 1)];
+            }
+            const T sy = kind == 1 ? T(-1) : T(1);  // in[C] += conj(tau s)
+#pragma unroll
+            for (int j = 0; j < kRep1J; ++j) {
+                const int e = e0 + threadIdx.x + TB * j;
+                const int x = e % d, rr = e / d - r0;
+                Cplx<T> val = {T(0), T(0)};
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    if (sg[c] == T(0)) continue;
+                    const Cplx<T> yz = syz[c][rr < nr ? rr : 0];
+                    const Cplx<T> xv = sx[c][x];
+                    val.x = fma(xv.x, yz.x, val.x);
+                    val.x = fma(-xv.y, yz.y, val.x);
+                    val.y = fma(xv.x, yz.y, val.y);
+                    val.y = fma(xv.y, yz.x, val.y);
+                }
+                if (e < M) dst[e] = {pf[j].x + val.x, pf[j].y + sy * val.y};
+            }
+            if (k + 1 < np) {
+#pragma unroll
+                for (int j = 0; j < kRep1J; ++j) pf[j] = pn[j];
+            }
         }
     }
 }
