@@ -23,6 +23,13 @@ namespace {
 // least 64 arrivals beyond its initial count)
 constexpr int kReplicaCompactEvery = 512;
 
+// blocks per trial role of a replica step (window, difference, near field):
+// with hundreds of replicas the trial kernel has blocks enough, and fewer,
+// longer role blocks per replica run the step faster (256 ragged replicas at
+// digits 6: 168.5 us per step at the single-system default {2, 8, 8},
+// 151-156 us over {1-2, 1-4, 2-4})
+constexpr Tiles kReplicaTiles{1, 2, 4, 1024};
+
 struct ReplicaSet {
     int precision = HPDMK_FP64;
     hipStream_t stream = nullptr;
@@ -76,7 +83,7 @@ ReplicaLaunch<T> launch_args(ReplicaSet* s) {
         const DevTree<T>& d = dev<T>(h.get());
         r.dmax = std::max(r.dmax, std::max(d.dw, d.dd));
     }
-    r.tiles = Tiles{};
+    r.tiles = kReplicaTiles;
     return r;
 }
 
@@ -201,7 +208,7 @@ int hpdmk_replicas_mc_run(hpdmk_replicas s, double beta, int graph_steps, double
         for (int r = 0; r < set->n(); ++r) {
             args[r].move_ctr = set->move_ctr + r;
             args[r].n_moves = n;
-            args[r].tiles = Tiles{};  // the grid of a replica step uses the default shape
+            args[r].tiles = kReplicaTiles;  // the grid of a replica step
         }
         HIPX(hipMemcpyAsync(set->args, args.data(), sizeof(McArgs) * args.size(), hipMemcpyHostToDevice, set->stream));
         // consecutive steps alternate between the two trial-state buffers of
