@@ -794,8 +794,12 @@ def main():
         for s_ in [float(x) for x in args.hs_judge_steps.split(",")]:
             scan[s_] = energy_autocorrelation(gh, args.hs_n, args.hs_sweeps, False, s_, None, hcfg,
                                               args.seed + 34, batch)
+            print(f"# hard spheres: uniform step {s_}: tau_int {scan[s_]['tau_int_sweeps']} sweeps, "
+                  f"{scan[s_]['decorrelated_samples_per_s']} decorrelated samples/s", file=sys.stderr, flush=True)
         best_s = max(scan, key=lambda k: scan[k]["decorrelated_samples_per_s"])
         lac = energy_autocorrelation(gh, args.hs_n, args.hs_sweeps, True, 0.0, w_h, hcfg, args.seed + 35, batch)
+        print(f"# hard spheres: learned: tau_int {lac['tau_int_sweeps']} sweeps, "
+              f"{lac['decorrelated_samples_per_s']} decorrelated samples/s", file=sys.stderr, flush=True)
         out = {}
         for name in ("uniform", "learned"):
             if name == "learned":
