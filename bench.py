@@ -864,9 +864,16 @@ def main():
         gd = rng_g.uniform(-1, 1, (Mg + Ns, 3))
         gu = rng_g.uniform(0, 1, Mg + Ns)
         sdE = gg.eval_shift_energy_batch(gi[:Ns], gd[:Ns])
+        # launch shape tuned on a twin of this system (its own moves, outside the timed region)
+        tiles_g, batch_g = None, batch
+        if args.tune:
+            rng_tg = np.random.default_rng(args.seed + 42)
+            tiles_g, _ = gg.autotune(rng_tg.integers(0, NL, args.tune_moves), rng_tg.uniform(-1, 1, (args.tune_moves, 3)),
+                                     rng_tg.uniform(0, 1, args.tune_moves))
+            batch_g = 0
         gg.mc_load(gi[Ns:], gd[Ns:], gu[Ns:])
         t0 = time.perf_counter()
-        gg.mc_run(1.0, 1, batch)
+        gg.mc_run(1.0, 1, batch_g)
         wg = time.perf_counter() - t0
         _, ag = gg.mc_results()
         depth = gg.max_depth
@@ -880,6 +887,7 @@ def main():
                       "levels": depth, "build_energy_s": round(t_bg, 3), "E": Eg,
                       "octree_ms": {"device": round(ms_gdev, 3), "host": round(ms_ghost, 3)},
                       "moves_per_s": round(Mg / wg, 1), "moves": Mg, "acceptance": round(float(ag.mean()), 4),
+                      "tiles": tiles_g,
                       "E_vs_ewald_rel": float(abs(Eg - E_ewg) / abs(E_ewg)),
                       "dE_vs_ewald_max_rel_to_E": float(np.abs(sdE - dE_ew).max() / abs(E_ewg)),
                       "dE_vs_ewald_mean_rel_to_E": float(np.abs(sdE - dE_ew).mean() / abs(E_ewg)),
