@@ -785,6 +785,14 @@ def main():
         hcfg = dict(rf=args.hs_rf, max_mean=0.5, sigma_min=0.02, sigma_max=2.0)
         rng_h = np.random.default_rng(args.seed + 32)
         PR.run_uniform(gh, args.hs_n, 1.0, args.hs_equil_sweeps * args.hs_n, 1.0, batch, rng_h)  # equilibration
+        # launch shape of this system (on a twin, hard-sphere model included),
+        # used by every chain below: training, the judge's scan, the learned chain
+        tiles_h, batch_h = None, batch
+        if args.tune:
+            rng_th = np.random.default_rng(args.seed + 36)
+            tiles_h, _ = gh.autotune(rng_th.integers(0, args.hs_n, args.tune_moves),
+                                     rng_th.uniform(-1, 1, (args.tune_moves, 3)), rng_th.uniform(0, 1, args.tune_moves))
+            batch_h = 0
         t0 = time.time()
         w_h, hist_h = PR.train_proposal(gh, args.hs_n, hcfg, warm_step=args.hs_train_step, warm_moves=args.hs_train,
                                         rounds=args.hs_pg_rounds, round_moves=100000, warm_epochs=args.train_epochs,
@@ -793,20 +801,20 @@ def main():
         scan = {}
         for s_ in [float(x) for x in args.hs_judge_steps.split(",")]:
             scan[s_] = energy_autocorrelation(gh, args.hs_n, args.hs_sweeps, False, s_, None, hcfg,
-                                              args.seed + 34, batch)
+                                              args.seed + 34, batch_h)
             print(f"# hard spheres: uniform step {s_}: tau_int {scan[s_]['tau_int_sweeps']} sweeps, "
                   f"{scan[s_]['decorrelated_samples_per_s']} decorrelated samples/s", file=sys.stderr, flush=True)
         best_s = max(scan, key=lambda k: scan[k]["decorrelated_samples_per_s"])
-        lac = energy_autocorrelation(gh, args.hs_n, args.hs_sweeps, True, 0.0, w_h, hcfg, args.seed + 35, batch)
+        lac = energy_autocorrelation(gh, args.hs_n, args.hs_sweeps, True, 0.0, w_h, hcfg, args.seed + 35, batch_h)
         print(f"# hard spheres: learned: tau_int {lac['tau_int_sweeps']} sweeps, "
               f"{lac['decorrelated_samples_per_s']} decorrelated samples/s", file=sys.stderr, flush=True)
         out = {}
         for name in ("uniform", "learned"):
             if name == "learned":
-                dEh, ah, dlh, Hh, _, _, sec = PR.run_learned(gh, args.hs_n, w_h, hcfg, args.hs_moves, 1.0, batch,
+                dEh, ah, dlh, Hh, _, _, sec = PR.run_learned(gh, args.hs_n, w_h, hcfg, args.hs_moves, 1.0, batch_h,
                                                              rng_h)
             else:
-                dEh, ah, dlh, _, sec = PR.run_uniform(gh, args.hs_n, best_s, args.hs_moves, 1.0, batch, rng_h)
+                dEh, ah, dlh, _, sec = PR.run_uniform(gh, args.hs_n, best_s, args.hs_moves, 1.0, batch_h, rng_h)
                 Hh = np.zeros(len(dEh))
             e = PR.esjd(dEh, Hh, dlh, 1.0)
             out[name] = {"moves_per_s": round(args.hs_moves / sec, 1), "acceptance": round(float(ah.mean()), 4),
@@ -816,7 +824,7 @@ def main():
         ub = scan[best_s]
         hs_info = {"workload": f"N={args.hs_n} charged hard spheres, sigma 1, packing 0.2 (L={hL:.2f}), "
                                f"Bjerrum length 2, digits {args.digits}, leaf {args.leaf}, batched chain",
-                   "feature_radius": args.hs_rf, "proposal": hcfg,
+                   "feature_radius": args.hs_rf, "proposal": hcfg, "tiles": tiles_h,
                    "training": {"warm_start": {k: (round(v, 4) if isinstance(v, float) else v)
                                                for k, v in hist_h[0].items()},
                                 "warm_uniform_step": args.hs_train_step,
