@@ -136,6 +136,22 @@ def test_batched_chain_equals_plain(precision, batch):
         assert_same_chain(dE1, acc1, dE2, acc2, thresholds(u), MIXED_DE_BOUND)
 
 
+def test_batched_chain_mixed_across_compactions():
+    """Mixed precision over 5000 moves (the matrix-core update, with and
+    without the next batch's set-up in its launch, across compactions):
+    the plain FP64 chain's decisions except at a near tie."""
+    log("batched mixed vs plain FP64 across compactions")
+    r, q, g1 = make(2000, 40, 6, P.HPDMK_FP64, seed=9)
+    _, _, g2 = make(2000, 40, 6, P.HPDMK_MIXED, seed=9)
+    idx, d, u = _moves(2000, 5000, 13)
+    dE1, acc1, _ = g1.mc_chain(idx, d, u, beta=1.0, mode=0)
+    dE2, acc2, _ = g2.mc_chain(idx, d, u, beta=1.0, mode=1, batch=64)
+    k = assert_same_chain(dE1, acc1, dE2, acc2, thresholds(u), MIXED_DE_BOUND)
+    log(f"first differing decision at move {k} of {len(acc1)}")
+    g1.close()
+    g2.close()
+
+
 @pytest.mark.parametrize("batch", [64, 128])
 def test_batched_chain_equals_plain_across_compactions(batch):
     """5000 moves: the slot compactions (every 2048 moves) fall between

@@ -204,6 +204,35 @@ def test_batched_learned_chain_equals_plain_across_compactions():
     assert 0.05 < a1.mean() < 0.95
 
 
+def test_batched_learned_chain_mixed_across_compactions():
+    """The mixed-precision learned chain over 5000 moves on hard spheres
+    (BF16 network, matrix-core update with the next proposals in its launch,
+    compactions between batches): the plain FP64 learned chain's decisions
+    except at a near tie of dE - H / beta."""
+    log("batched learned mixed across compactions")
+    r, q, L = hard_sphere_system(2000, 0.2, 1.0, seed=11)
+    w = P.mlp_init(7, scale=1.0)
+    rng = np.random.default_rng(15)
+    m = 5000
+    idx, z, u = rng.integers(0, 2000, m), rng.normal(size=(m, 3)), rng.uniform(0, 1, m)
+    runs = []
+    for mode, prec in ((0, P.HPDMK_FP64), (1, P.HPDMK_MIXED)):
+        g = P.HPDMKPtTree(P.HPDMKParams(16, 6, L, init=P.DIRECT), r, q, prec)
+        g.form_outgoing_pw()
+        g.form_incoming_pw()
+        g.mc_set_model(2.0, 1.0)
+        g.mc_set_proposal(learned=True, weights=w, rf=2.0, **CFG)
+        dE, acc, _ = g.mc_chain(idx, z, u, beta=1.0, mode=mode, batch=64)
+        dl, _, H = g.mc_proposal_results()
+        runs.append((dE, acc, dl, H))
+        g.close()
+    (dE1, a1, d1, H1), (dE2, a2, d2, H2) = runs
+    k = assert_same_chain(dE1, a1, dE2, a2, thresholds(u), MIXED_DE_BOUND, score_ref=dE1 - H1)
+    log(f"first differing decision at move {k} of {m}")
+    assert np.abs(d1[:k] - d2[:k]).max(initial=0.0) < 1e-5
+    assert np.abs(H1[:k] - H2[:k]).max(initial=0.0) < MIXED_DE_BOUND
+
+
 def test_policy_gradient_training_raises_expected_jump():
     log("policy gradient")
     r, q, L = hard_sphere_system(3000, 0.2, 1.0, seed=13)
